@@ -1,0 +1,126 @@
+/*
+ * alps_oracle.h -- CPU restatement of the ALPS dispersion-tensor path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path:
+ * only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py
+ * may load it.  The product (alps_amd/) never links or calls it.
+ *
+ * Layout of every array is plain C row-major with the index order of the
+ * reference's Fortran arrays (species outermost here; see the PP/F0/DF0
+ * macros in alps_oracle.c).  Complex numbers are C99 double complex.
+ */
+#ifndef ALPS_ORACLE_H
+#define ALPS_ORACLE_H
+
+#include <complex.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_MAXSPEC 8
+#define ORC_MAXFITS 4
+
+typedef struct {
+  /* sizes */
+  int nspec, nperp, npar;
+  /* grid and distribution: pp[nspec][nperp+1][npar+1][2], f0[nspec][nperp+1][npar+1],
+     df0[nspec][nperp+1][npar+1][2] (only interior points are meaningful) */
+  double *pp;
+  double *f0;
+  double *df0;
+  /* species: density, charge, mass */
+  double ns[ORC_MAXSPEC], qs[ORC_MAXSPEC], ms[ORC_MAXSPEC];
+  double vA;
+  /* numerics (ALPS_var.f90 names) */
+  int kperp_norm;             /* logical kperp_norm */
+  int positions_principal;
+  int n_resonance_interval;
+  double Tlim;
+  double Bessel_zero;
+  /* analytic continuation: ACmethod 1 (fits) or 2 (Chebyshev) */
+  int ACmethod[ORC_MAXSPEC];
+  int n_fits[ORC_MAXSPEC];
+  int fit_type[ORC_MAXSPEC][ORC_MAXFITS];
+  double perp_correction[ORC_MAXSPEC][ORC_MAXFITS];
+  /* param_fit[nspec][nperp+1][5][ORC_MAXFITS] */
+  double *param_fit;
+  int poly_kind[ORC_MAXSPEC];
+  int poly_order[ORC_MAXSPEC];
+  int logfit[ORC_MAXSPEC];
+  double poly_log_max[ORC_MAXSPEC];
+  /* poly_fit_coeffs[nspec][nperp+1][poly_stride] */
+  double *poly_fit_coeffs;
+  int poly_stride;
+  /* wavevector state */
+  double kperp, kpar;
+  int nmax[ORC_MAXSPEC];
+  /* bessel[s] : (nmax[s]+3) x (nperp+1), row n+1 holds order n (n = -1 .. nmax+1) */
+  double *bessel[ORC_MAXSPEC];
+  /* root-search controls */
+  int numiter;
+  double D_threshold, D_prec, D_gap, D_tol;
+  int secant_method;
+  int nproc;                  /* rank count whose nmax bump is emulated (1 = none) */
+  /* statistics */
+  long long n_disp_calls;
+  int n_threads;              /* OpenMP threads used inside disp (0 = default) */
+} orc_plasma;
+
+/* Bessel functions (ALPS_fns_rel.f90:1580-1735). */
+double orc_bessj(int n, double x);
+double orc_bessj0(double x);
+double orc_bessj1(double x);
+
+/* Setup (ALPS_fns.f90:34-118, 3971-4075, 4214-4255). */
+void orc_derivative_f0(orc_plasma *P);
+int orc_determine_nmax(orc_plasma *P);
+int orc_set_k(orc_plasma *P, double kperp, double kpar);
+void orc_free_k(orc_plasma *P);
+
+/* Hot path pieces (ALPS_fns.f90:252-1812, ALPS_analyt.f90:32-363). */
+void orc_determine_resonances(const orc_plasma *P, int sproc, double complex om, int nn,
+                              int *found_plus, int *found_minus);
+double complex orc_full_integrate(const orc_plasma *P, int sproc, double complex om, int nn,
+                                  int mode, int found_res);
+double complex orc_integrate(const orc_plasma *P, int sproc, double complex om, int nn,
+                             int mode, int iparmin, int iparmax);
+double complex orc_integrate_res(const orc_plasma *P, int sproc, double complex om, int nn,
+                                 int mode);
+double complex orc_landau_integrate(const orc_plasma *P, int sproc, double complex om, int nn,
+                                    int mode);
+double complex orc_funct_g(const orc_plasma *P, int sproc, double ppar_real, int iperp,
+                           double complex om, int nn, int mode);
+double complex orc_int_ee(const orc_plasma *P, int sproc);
+double complex orc_eval_fit(const orc_plasma *P, int is, int iperp, double complex ppar_valC);
+
+/* disp(om): det D; optional chi0 [nspec*9], chi0_low [nspec*27] (n = -1,0,1), wave [9],
+   all row-major [s][i][j] / [s][i][j][n+1]. */
+double complex orc_disp(orc_plasma *P, double complex om, double complex *chi0,
+                        double complex *chi0_low, double complex *wave);
+
+/* Root finders (ALPS_fns.f90:1815-2195). Return the number of iterations. */
+int orc_secant(orc_plasma *P, double complex *om);
+int orc_secant_osc(orc_plasma *P, double complex *om);
+double complex orc_rtsec(orc_plasma *P, double complex xin, int *iflag);
+
+/* Map search (ALPS_fns.f90:3595-3966).  cal/val are [nr][ni] row-major.
+   iroots is [numroots][2] (ir, ii, 1-based).  Returns the number of minima recorded. */
+int orc_map_search(orc_plasma *P, double omi, double omf, double gami, double gamf,
+                   int nr, int ni, int loggridw, int loggridg,
+                   double complex *om_out, double complex *cal, double *val,
+                   int numroots, int *iroots);
+int orc_find_minima(const double *val, int nr, int ni, int numroots, int *iroots);
+
+/* om_scan (ALPS_fns.f90:2198-2600) without eigenfunction/heating output.
+   roots[nroots] in/out.  Output arrays sized [nt+1]: row 0 is the initial state.
+   scan_type 0..4; diff, diff2 as computed by ALPS_io.f90:482-548.
+   Returns 0, or 9 when every root has been dropped (alps_error(9)). */
+int orc_om_scan(orc_plasma *P, int scan_type, int log_scan, double diff, double diff2,
+                int n_out, int n_res, int nroots, double complex *roots,
+                double *kperp_out, double *kpar_out, double complex *roots_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,257 @@
+"""ctypes front end of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py to check and time the HIP path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+
+MAXSPEC = 8
+MAXFITS = 4
+
+
+class _Plasma(C.Structure):
+    _fields_ = [
+        ("nspec", C.c_int), ("nperp", C.c_int), ("npar", C.c_int),
+        ("pp", C.POINTER(C.c_double)), ("f0", C.POINTER(C.c_double)),
+        ("df0", C.POINTER(C.c_double)),
+        ("ns", C.c_double * MAXSPEC), ("qs", C.c_double * MAXSPEC), ("ms", C.c_double * MAXSPEC),
+        ("vA", C.c_double),
+        ("kperp_norm", C.c_int), ("positions_principal", C.c_int),
+        ("n_resonance_interval", C.c_int), ("Tlim", C.c_double), ("Bessel_zero", C.c_double),
+        ("ACmethod", C.c_int * MAXSPEC), ("n_fits", C.c_int * MAXSPEC),
+        ("fit_type", (C.c_int * MAXFITS) * MAXSPEC),
+        ("perp_correction", (C.c_double * MAXFITS) * MAXSPEC),
+        ("param_fit", C.POINTER(C.c_double)),
+        ("poly_kind", C.c_int * MAXSPEC), ("poly_order", C.c_int * MAXSPEC),
+        ("logfit", C.c_int * MAXSPEC), ("poly_log_max", C.c_double * MAXSPEC),
+        ("poly_fit_coeffs", C.POINTER(C.c_double)), ("poly_stride", C.c_int),
+        ("kperp", C.c_double), ("kpar", C.c_double),
+        ("nmax", C.c_int * MAXSPEC),
+        ("bessel", C.POINTER(C.c_double) * MAXSPEC),
+        ("numiter", C.c_int), ("D_threshold", C.c_double), ("D_prec", C.c_double),
+        ("D_gap", C.c_double), ("D_tol", C.c_double), ("secant_method", C.c_int),
+        ("nproc", C.c_int),
+        ("n_disp_calls", C.c_longlong), ("n_threads", C.c_int),
+    ]
+
+
+class _C128(C.Structure):
+    _fields_ = [("re", C.c_double), ("im", C.c_double)]
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the Makefile beside this file."""
+    if force or not os.path.exists(_LIB):
+        subprocess.run(["make", "-B" if force else "-s", "-C", _HERE, "liboracle.so"],
+                       check=True, capture_output=True)
+    return _LIB
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = C.CDLL(_LIB)
+        P = C.POINTER(_Plasma)
+        dp = C.POINTER(C.c_double)
+        ip = C.POINTER(C.c_int)
+        L.orc_bessj.restype = C.c_double
+        L.orc_bessj.argtypes = [C.c_int, C.c_double]
+        L.orc_bessj0.restype = C.c_double
+        L.orc_bessj0.argtypes = [C.c_double]
+        L.orc_bessj1.restype = C.c_double
+        L.orc_bessj1.argtypes = [C.c_double]
+        L.orc_derivative_f0.argtypes = [P]
+        L.orc_set_k.argtypes = [P, C.c_double, C.c_double]
+        L.orc_set_k.restype = C.c_int
+        L.orc_free_k.argtypes = [P]
+        L.orc_disp.restype = _C128
+        L.orc_disp.argtypes = [P, _C128, dp, dp, dp]
+        L.orc_full_integrate.restype = _C128
+        L.orc_full_integrate.argtypes = [P, C.c_int, _C128, C.c_int, C.c_int, C.c_int]
+        L.orc_landau_integrate.restype = _C128
+        L.orc_landau_integrate.argtypes = [P, C.c_int, _C128, C.c_int, C.c_int]
+        L.orc_integrate_res.restype = _C128
+        L.orc_integrate_res.argtypes = [P, C.c_int, _C128, C.c_int, C.c_int]
+        L.orc_int_ee.restype = _C128
+        L.orc_int_ee.argtypes = [P, C.c_int]
+        L.orc_eval_fit.restype = _C128
+        L.orc_eval_fit.argtypes = [P, C.c_int, C.c_int, _C128]
+        L.orc_determine_resonances.argtypes = [P, C.c_int, _C128, C.c_int, ip, ip]
+        L.orc_secant_osc.restype = C.c_int
+        L.orc_secant_osc.argtypes = [P, dp]
+        L.orc_secant.restype = C.c_int
+        L.orc_secant.argtypes = [P, dp]
+        L.orc_map_search.restype = C.c_int
+        L.orc_map_search.argtypes = [P, C.c_double, C.c_double, C.c_double, C.c_double,
+                                     C.c_int, C.c_int, C.c_int, C.c_int, dp, dp, dp, C.c_int, ip]
+        L.orc_find_minima.restype = C.c_int
+        L.orc_find_minima.argtypes = [dp, C.c_int, C.c_int, C.c_int, ip]
+        L.orc_om_scan.restype = C.c_int
+        L.orc_om_scan.argtypes = [P, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
+                                  C.c_int, C.c_int, dp, dp, dp, dp]
+        _lib = L
+    return _lib
+
+
+def _c(z) -> _C128:
+    z = complex(z)
+    return _C128(z.real, z.imag)
+
+
+def _z(c: _C128) -> complex:
+    return complex(c.re, c.im)
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Oracle:
+    """CPU restatement of ALPS's disp()/secant/map_search/om_scan for one plasma."""
+
+    def __init__(self, plasma, nproc: int = 1, n_threads: int = 0):
+        L = lib()
+        p = plasma
+        if p.nspec > MAXSPEC or p.max_fits > MAXFITS:
+            raise ValueError("too many species or fits for the oracle")
+        self.plasma = p
+        self._pp = np.ascontiguousarray(p.pp, dtype=np.float64)
+        self._f0 = np.ascontiguousarray(p.f0, dtype=np.float64)
+        self._df0 = np.zeros(self._pp.shape)
+        pf = np.zeros((p.nspec, p.nperp + 1, 5, MAXFITS))
+        pf[..., : p.max_fits] = p.param_fit
+        self._pf = pf
+        self._poly = np.ascontiguousarray(p.poly_fit_coeffs, dtype=np.float64)
+        s = _Plasma()
+        s.nspec, s.nperp, s.npar = p.nspec, p.nperp, p.npar
+        s.pp, s.f0, s.df0 = _ptr(self._pp), _ptr(self._f0), _ptr(self._df0)
+        for i in range(p.nspec):
+            s.ns[i], s.qs[i], s.ms[i] = p.ns[i], p.qs[i], p.ms[i]
+            s.ACmethod[i] = int(p.ACmethod[i])
+            s.n_fits[i] = int(p.n_fits[i])
+            for f in range(p.max_fits):
+                s.fit_type[i][f] = int(p.fit_type[i, f])
+                s.perp_correction[i][f] = float(p.perp_correction[i, f])
+            s.poly_kind[i] = int(p.poly_kind[i])
+            s.poly_order[i] = int(p.poly_order[i])
+            s.logfit[i] = int(p.logfit[i])
+            s.poly_log_max[i] = float(p.poly_log_max[i])
+        s.vA = p.vA
+        s.kperp_norm = int(bool(p.kperp_norm))
+        s.positions_principal = int(p.positions_principal)
+        s.n_resonance_interval = int(p.n_resonance_interval)
+        s.Tlim = p.Tlim
+        s.Bessel_zero = p.Bessel_zero
+        s.param_fit = _ptr(self._pf)
+        s.poly_fit_coeffs = _ptr(self._poly)
+        s.poly_stride = self._poly.shape[2]
+        r = p.roots
+        s.numiter, s.D_threshold, s.D_prec = r.numiter, r.D_threshold, r.D_prec
+        s.D_gap, s.D_tol, s.secant_method = r.D_gap, r.D_tol, r.secant_method
+        s.nproc = nproc
+        s.n_threads = n_threads
+        self.s = s
+        L.orc_derivative_f0(C.byref(s))
+
+    def __del__(self):
+        try:
+            lib().orc_free_k(C.byref(self.s))
+        except Exception:
+            pass
+
+    @property
+    def df0(self) -> np.ndarray:
+        return self._df0
+
+    @property
+    def nmax(self):
+        return [self.s.nmax[i] for i in range(self.plasma.nspec)]
+
+    @property
+    def n_disp_calls(self) -> int:
+        return self.s.n_disp_calls
+
+    def set_k(self, kperp: float, kpar: float):
+        rc = lib().orc_set_k(C.byref(self.s), kperp, kpar)
+        if rc != 0:
+            raise RuntimeError(f"orc_set_k failed ({rc})")
+        return self.nmax
+
+    def bessel_table(self, s: int) -> np.ndarray:
+        """bessel_array(-1:nmax+1, 0:nperp) of species s (0-based) as [nmax+3, nperp+1]."""
+        n = (self.s.nmax[s] + 3) * (self.plasma.nperp + 1)
+        a = np.ctypeslib.as_array(self.s.bessel[s], shape=(n,)).copy()
+        return a.reshape(self.s.nmax[s] + 3, self.plasma.nperp + 1)
+
+    def disp(self, om, want_tensors: bool = False):
+        ns = self.plasma.nspec
+        if want_tensors:
+            chi0 = np.zeros(ns * 9, dtype=np.complex128)
+            low = np.zeros(ns * 27, dtype=np.complex128)
+            wave = np.zeros(9, dtype=np.complex128)
+            d = lib().orc_disp(C.byref(self.s), _c(om), _ptr(chi0), _ptr(low), _ptr(wave))
+            return _z(d), chi0.reshape(ns, 3, 3), low.reshape(ns, 3, 3, 3), wave.reshape(3, 3)
+        return _z(lib().orc_disp(C.byref(self.s), _c(om), None, None, None))
+
+    def full_integrate(self, sproc: int, om, nn: int, mode: int, found: bool) -> complex:
+        return _z(lib().orc_full_integrate(C.byref(self.s), sproc, _c(om), nn, mode, int(found)))
+
+    def landau(self, sproc: int, om, nn: int, mode: int) -> complex:
+        return _z(lib().orc_landau_integrate(C.byref(self.s), sproc, _c(om), nn, mode))
+
+    def int_ee(self, sproc: int) -> complex:
+        return _z(lib().orc_int_ee(C.byref(self.s), sproc))
+
+    def eval_fit(self, sproc: int, iperp: int, p) -> complex:
+        return _z(lib().orc_eval_fit(C.byref(self.s), sproc, iperp, _c(p)))
+
+    def resonances(self, sproc: int, om, nn: int):
+        a, b = C.c_int(), C.c_int()
+        lib().orc_determine_resonances(C.byref(self.s), sproc, _c(om), nn, C.byref(a), C.byref(b))
+        return bool(a.value), bool(b.value)
+
+    def secant_osc(self, om):
+        z = np.array([complex(om)], dtype=np.complex128)
+        it = lib().orc_secant_osc(C.byref(self.s), _ptr(z))
+        return complex(z[0]), it
+
+    def secant(self, om):
+        z = np.array([complex(om)], dtype=np.complex128)
+        it = lib().orc_secant(C.byref(self.s), _ptr(z))
+        return complex(z[0]), it
+
+    def map_search(self, omi, omf, gami, gamf, nr, ni, loggridw=False, loggridg=False,
+                   numroots=100):
+        om = np.zeros(nr * ni, dtype=np.complex128)
+        cal = np.zeros(nr * ni, dtype=np.complex128)
+        val = np.zeros(nr * ni)
+        iroots = np.zeros(2 * numroots, dtype=np.int32)
+        n = lib().orc_map_search(C.byref(self.s), omi, omf, gami, gamf, nr, ni, int(loggridw),
+                                 int(loggridg), _ptr(om), _ptr(cal), _ptr(val), numroots,
+                                 iroots.ctypes.data_as(C.POINTER(C.c_int)))
+        return (om.reshape(nr, ni), cal.reshape(nr, ni), val.reshape(nr, ni),
+                iroots.reshape(numroots, 2)[: min(n, numroots)], n)
+
+    def om_scan(self, scan_type, log_scan, diff, diff2, n_out, n_res, roots):
+        roots = np.array(roots, dtype=np.complex128)
+        nr = roots.size
+        kp = np.zeros(n_out + 1)
+        kz = np.zeros(n_out + 1)
+        out = np.zeros((n_out + 1) * nr, dtype=np.complex128)
+        rc = lib().orc_om_scan(C.byref(self.s), scan_type, int(log_scan), diff, diff2, n_out,
+                               n_res, nr, _ptr(roots), _ptr(kp), _ptr(kz), _ptr(out))
+        return rc, kp, kz, out.reshape(n_out + 1, nr)
