@@ -1,0 +1,860 @@
+// Host side of the C-ABI declared in include/alpsg.h.
+//
+// A handle owns one HIP stream, the device copies of the plasma (grids, f0,
+// df0, fit tables), the per-k tables and the staging buffers of the batched
+// entry points.  Nothing here evaluates physics on the CPU: every number that
+// reaches det D is computed by the kernels of alpsg_tables.hip and
+// alpsg_disp.hip; the host only moves data, sequences launches and runs the
+// scalar control flow of the root searches (secant_osc state, map minima,
+// k-path stepping), which the reference also runs on one rank.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <memory>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/alpsg.h"
+#include "alpsg_device.h"
+
+// kernels (alpsg_tables.hip, alpsg_disp.hip)
+hipError_t launch_derivative(hipStream_t st, int nspec, int nperp, int npar, const double *f0,
+                             const double *pperp, const double *ppar, double *df1, double *df2);
+hipError_t launch_int_ee(hipStream_t st, int nspec, int nperp, int npar, const double *pperp,
+                         const double *ppar, const double *df1, const double *df2,
+                         const double *qs, const double *ms, double *out);
+hipError_t launch_bessel_max(hipStream_t st, int nspec, int nperp, int ntry, double kperp,
+                             const double *pperp, const double *qs, const int *mtab, double *out);
+hipError_t launch_species_k(hipStream_t st, int nperp, int npar, int nmax, double kperp,
+                            double kpar, double q, double m, const double *pperp,
+                            const double *ppar, const double *df1, const double *df2,
+                            const int *mtab, double *bes, double *atab, double *nodes);
+hipError_t launch_disp(hipStream_t st, int nch, const ProbDev *prob, int nom, const double2 *om,
+                       const int *active, double2 *det, double2 *chi, double2 *low,
+                       double2 *wave);
+
+namespace {
+
+thread_local std::string g_err;
+const int MTAB_N = 4096;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return fail(ALPSG_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  hipError_t ensure(size_t cnt) {
+    if (cnt <= n && p) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    hipError_t e = hipMalloc((void **)&p, std::max<size_t>(cnt, 1) * sizeof(T));
+    if (e == hipSuccess) n = std::max<size_t>(cnt, 1);
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+}  // namespace
+
+struct alpsg_ctx {
+  hipStream_t st = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int nspec = 0, nperp = 0, npar = 0;
+  int kperp_norm = 1, P = 5, NRI = 100;
+  double vA = 0, Tlim = 0.01, Bessel_zero = 1e-45;
+  bool have_k = false;
+  double kperp = 0, kpar = 0;
+  int nmax[ALPSG_MAXSPEC] = {0};
+  long long evals = 0;
+  double last_ms = 0.0;
+  // host copies
+  std::vector<double> qs, ms, ns, pperp_h, ppar_h;
+  // static device data
+  DevBuf<double> pperp, ppar, f0, df1, df2, qs_d, ms_d, int_ee, fitc, polyc;
+  DevBuf<int> mtab;
+  std::vector<double> int_ee_h;
+  // per-k device data
+  DevBuf<double> bes[ALPSG_MAXSPEC], atab[ALPSG_MAXSPEC], nodes[ALPSG_MAXSPEC];
+  DevBuf<double> bmax;
+  // problem descriptor
+  ProbDev prob{};
+  DevBuf<ProbDev> prob_d;
+  // batch staging
+  DevBuf<double2> om_d, det_d, chi_d, low_d, wave_d;
+  DevBuf<int> act_d;
+  int fixed_nch = 0;  // 0 = heuristic / tuned
+
+  ~alpsg_ctx() {
+    if (st) (void)hipStreamSynchronize(st);
+    DevBuf<double> *all[] = {&pperp, &ppar, &f0, &df1, &df2, &qs_d, &ms_d, &int_ee, &fitc, &polyc,
+                             &bmax};
+    for (auto *b : all) b->release();
+    for (int s = 0; s < ALPSG_MAXSPEC; ++s) {
+      bes[s].release();
+      atab[s].release();
+      nodes[s].release();
+    }
+    mtab.release();
+    prob_d.release();
+    om_d.release();
+    det_d.release();
+    chi_d.release();
+    low_d.release();
+    wave_d.release();
+    act_d.release();
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+
+namespace {
+
+// Fortran column-major index helpers
+inline size_t f_pp(int nspec, int nperp, int npar, int s, int iperp, int ipar, int c) {
+  return (size_t)s + (size_t)nspec * (iperp + (size_t)(nperp + 1) * (ipar + (size_t)(npar + 1) * c));
+}
+
+int pick_nch(int n) {
+  // enough wavefronts to cover 256 CUs several times over, fewest lanes per omega
+  if (n >= 65536) return 1;
+  if (n >= 16384) return 2;
+  if (n >= 4096) return 4;
+  if (n >= 1024) return 8;
+  if (n >= 256) return 16;
+  if (n >= 64) return 32;
+  return 64;
+}
+
+int run_disp(alpsg_ctx *h, int n, const double *om, double *det, double *chi, double *low,
+             double *wave, int nch_override = 0) {
+  if (!h->have_k) return fail(ALPSG_E_STATE, "alpsg_set_k has not been called");
+  if (n < 0) return fail(ALPSG_E_ARG, "negative batch size");
+  if (n == 0) return ALPSG_OK;
+  HIPCHK(h->om_d.ensure(n));
+  HIPCHK(h->det_d.ensure(n));
+  if (chi) HIPCHK(h->chi_d.ensure((size_t)n * h->nspec * 9));
+  if (low) {
+    HIPCHK(h->low_d.ensure((size_t)n * h->nspec * 27));
+    HIPCHK(hipMemsetAsync(h->low_d.p, 0, sizeof(double2) * n * h->nspec * 27, h->st));
+  }
+  if (wave) HIPCHK(h->wave_d.ensure((size_t)n * 9));
+  HIPCHK(hipMemcpyAsync(h->om_d.p, om, sizeof(double2) * n, hipMemcpyHostToDevice, h->st));
+  int nch = nch_override ? nch_override : (h->fixed_nch ? h->fixed_nch : pick_nch(n));
+  HIPCHK(hipEventRecord(h->ev0, h->st));
+  HIPCHK(launch_disp(h->st, nch, h->prob_d.p, n, h->om_d.p, nullptr, h->det_d.p,
+                     chi ? h->chi_d.p : nullptr, low ? h->low_d.p : nullptr,
+                     wave ? h->wave_d.p : nullptr));
+  HIPCHK(hipEventRecord(h->ev1, h->st));
+  HIPCHK(hipMemcpyAsync(det, h->det_d.p, sizeof(double2) * n, hipMemcpyDeviceToHost, h->st));
+  if (chi)
+    HIPCHK(hipMemcpyAsync(chi, h->chi_d.p, sizeof(double2) * n * h->nspec * 9,
+                          hipMemcpyDeviceToHost, h->st));
+  if (low)
+    HIPCHK(hipMemcpyAsync(low, h->low_d.p, sizeof(double2) * n * h->nspec * 27,
+                          hipMemcpyDeviceToHost, h->st));
+  if (wave)
+    HIPCHK(hipMemcpyAsync(wave, h->wave_d.p, sizeof(double2) * n * 9, hipMemcpyDeviceToHost,
+                          h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  h->last_ms = ms;
+  h->evals += n;
+  return ALPSG_OK;
+}
+
+typedef std::complex<double> cplx;
+
+// secant_osc (ALPS_fns.f90:1919-2101) for many independent roots: each root is
+// a small state machine whose disp() requests are served in batches.
+struct OscState {
+  cplx om, prevom, prev2om, prev3om, prev4om, prevD, D, minom, minD, jump, dplus;
+  double damping;
+  int iter, osc, phase, done;
+};
+
+int secant_osc_batch(alpsg_ctx *h, const alpsg_root_params *rp, int nroots, cplx *om, cplx *Dfin,
+                     int *iters) {
+  const cplx delta(1.e-6, 1.e-8);
+  const double lambda = (double)0.1f, osc_threshold = (double)1.e-3f, cap = (double)0.1f;
+  std::vector<OscState> S(nroots);
+  std::vector<cplx> req(nroots), res(nroots);
+  std::vector<int> who;
+  // phases: 0 = need D(om) initial, 1 = need D(prevom) initial, 2 = need D(om) iteration,
+  //         3 = need D(prevom+1e-8), 4 = need D(om*(1+delta)), 5 = need D(om*(1-delta))
+  for (int r = 0; r < nroots; ++r) {
+    OscState &s = S[r];
+    s.om = om[r];
+    s.phase = 0;
+    s.done = 0;
+    s.iter = 0;
+    s.osc = 0;
+    s.damping = 1.0;
+  }
+  auto decide = [&](OscState &s) {
+    // convergence test and the rest of the iteration body after D (and the
+    // optional prevD refresh) are known; may request Newton evaluations.
+    if (std::abs(s.D) < rp->D_threshold) {
+      s.jump = 0.0;
+      s.done = 1;
+      return;
+    }
+    if (s.iter > 4) {
+      auto close = [&](cplx a) {
+        return (std::fabs(s.om.real() - a.real()) < (std::fabs(s.om.real()) * osc_threshold)) &&
+               (std::fabs(s.om.imag() - a.imag()) < (std::fabs(s.om.imag()) * osc_threshold));
+      };
+      if (close(s.prevom) || close(s.prev2om) || close(s.prev3om) || close(s.prev4om)) {
+        s.osc = s.osc + 1;
+        s.damping = std::min(0.5, s.damping * 0.75);
+      }
+    }
+    if (s.osc > 1) {
+      s.phase = 4;
+      return;
+    }
+    s.jump = s.damping * s.D * (s.om - s.prevom) / (s.D - s.prevD);
+    s.phase = 6;  // finish
+  };
+  auto finish = [&](OscState &s) {
+    if (std::abs(s.jump) > cap * std::abs(s.om)) s.jump = (cap * std::abs(s.om)) * (s.jump / std::abs(s.jump));
+    if (std::abs(s.D) > std::abs(s.prevD)) s.jump = 0.5 * s.jump;
+    s.prev4om = s.prev3om;
+    s.prev3om = s.prev2om;
+    s.prev2om = s.prevom;
+    s.prevom = s.om;
+    s.prevD = s.D;
+    if (std::abs(s.D) < std::abs(s.minD)) {
+      s.minom = s.om;
+      s.minD = s.D;
+    }
+    s.om = s.om - s.jump;
+    s.phase = 2;
+  };
+  auto start_iter = [&](OscState &s) {
+    if (s.iter <= rp->numiter - 1) {
+      s.iter = s.iter + 1;
+      s.phase = 2;
+    } else {
+      s.done = 1;
+    }
+  };
+  for (int guard = 0; guard < 100000; ++guard) {
+    who.clear();
+    for (int r = 0; r < nroots; ++r) {
+      OscState &s = S[r];
+      if (s.done) continue;
+      cplx z;
+      switch (s.phase) {
+        case 0: z = s.om; break;
+        case 1: z = s.prevom; break;
+        case 2: z = s.om; break;
+        case 3: z = s.prevom; break;
+        case 4: z = s.om * (1.0 + delta); break;
+        default: z = s.om * (1.0 - delta); break;
+      }
+      req[who.size()] = z;
+      who.push_back(r);
+    }
+    if (who.empty()) break;
+    int rc = run_disp(h, (int)who.size(), reinterpret_cast<const double *>(req.data()),
+                      reinterpret_cast<double *>(res.data()), nullptr, nullptr, nullptr);
+    if (rc) return rc;
+    for (size_t t = 0; t < who.size(); ++t) {
+      OscState &s = S[who[t]];
+      cplx D = res[t];
+      switch (s.phase) {
+        case 0:
+          s.D = D;
+          s.minom = s.om;
+          s.minD = D;
+          s.prevom = s.om * (1.0 - rp->D_prec);
+          s.prev2om = s.prev3om = s.prev4om = s.om;
+          s.phase = 1;
+          break;
+        case 1:
+          s.prevD = D;
+          if (std::abs(s.prevD) < std::abs(s.minD)) {
+            s.minom = s.prevom;
+            s.minD = s.prevD;
+          }
+          s.iter = 0;
+          start_iter(s);
+          break;
+        case 2:
+          s.D = D;
+          if (std::abs(s.D - s.prevD) < 1.e-80) {
+            s.prevom = s.prevom + 1.e-8;
+            s.phase = 3;
+            break;
+          }
+          decide(s);
+          if (s.phase == 6) {
+            finish(s);
+            start_iter(s);
+          }
+          break;
+        case 3:
+          s.prevD = D;
+          decide(s);
+          if (s.phase == 6) {
+            finish(s);
+            start_iter(s);
+          }
+          break;
+        case 4:
+          s.dplus = D;
+          s.phase = 5;
+          break;
+        case 5: {
+          cplx Dprime = (s.dplus - D) / (2.0 * s.om * delta);
+          s.jump = s.D / (Dprime + lambda * s.D);
+          finish(s);
+          start_iter(s);
+        } break;
+      }
+    }
+  }
+  for (int r = 0; r < nroots; ++r) {
+    OscState &s = S[r];
+    if (s.iter >= rp->numiter) s.om = s.minom;
+    om[r] = s.om;
+    if (iters) iters[r] = s.iter;
+  }
+  if (Dfin) {
+    int rc = run_disp(h, nroots, reinterpret_cast<const double *>(om),
+                      reinterpret_cast<double *>(Dfin), nullptr, nullptr, nullptr);
+    if (rc) return rc;
+  }
+  return ALPSG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *alpsg_last_error(void) { return g_err.c_str(); }
+
+int alpsg_create(const alpsg_plasma_desc *d, alpsg_handle *out) {
+  if (!d || !out) return fail(ALPSG_E_ARG, "null argument");
+  *out = nullptr;
+  const int nspec = d->nspec, nperp = d->nperp, npar = d->npar;
+  if (nspec < 1 || nspec > ALPSG_MAXSPEC) return fail(ALPSG_E_ARG, "nspec out of range 1..8");
+  if (!d->pp || (!d->f0 && !d->df0) || !d->ns || !d->qs || !d->ms || !d->ACmethod || !d->n_fits)
+    return fail(ALPSG_E_ARG, "missing plasma array");
+  const int P = d->positions_principal;
+  if (nperp < 4 || npar < 2 * P + 8) return fail(ALPSG_E_ARG, "grid too small for positions_principal");
+  if (d->n_resonance_interval < 2) return fail(ALPSG_E_ARG, "n_resonance_interval < 2");
+  for (int s = 0; s < nspec; ++s) {
+    if (d->usebM && d->usebM[s]) return fail(ALPSG_E_UNSUPPORTED, "use_bM species are not handled by the engine");
+    if (d->relativistic && d->relativistic[s]) return fail(ALPSG_E_UNSUPPORTED, "relativistic species are not handled by the engine");
+    if (d->ACmethod[s] != 1 && d->ACmethod[s] != 2) return fail(ALPSG_E_UNSUPPORTED, "ACmethod must be 1 or 2");
+    if (d->qs[s] == 0.0 || d->ms[s] <= 0.0) return fail(ALPSG_E_ARG, "species charge/mass");
+    if (d->ACmethod[s] == 1) {
+      if (d->n_fits[s] < 1 || d->n_fits[s] > ALPSG_MAXFITS || d->n_fits[s] > d->max_fits)
+        return fail(ALPSG_E_ARG, "n_fits out of range");
+      if (!d->fit_type || !d->perp_correction || !d->param_fit) return fail(ALPSG_E_ARG, "missing fit arrays");
+      for (int f = 0; f < d->n_fits[s]; ++f) {
+        int ft = d->fit_type[s + (size_t)nspec * f];
+        if (ft != 1 && ft != 2 && ft != 6) return fail(ALPSG_E_UNSUPPORTED, "fit_type must be 1, 2 or 6");
+      }
+      if (d->param_fit_nperp < nperp) return fail(ALPSG_E_ARG, "param_fit_nperp < nperp");
+    } else {
+      if (!d->poly_kind || !d->poly_order || !d->logfit || !d->poly_log_max || !d->poly_fit_coeffs)
+        return fail(ALPSG_E_ARG, "missing Chebyshev arrays");
+      if (d->poly_kind[s] != 1) return fail(ALPSG_E_UNSUPPORTED, "poly_kind must be 1");
+      if (d->poly_order[s] < 1 || d->poly_order[s] > d->poly_max_order) return fail(ALPSG_E_ARG, "poly_order");
+    }
+  }
+  std::unique_ptr<alpsg_ctx> hold(new alpsg_ctx());
+  alpsg_ctx *h = hold.get();
+  auto bail = [&](int code) { return code; };
+  h->nspec = nspec;
+  h->nperp = nperp;
+  h->npar = npar;
+  h->kperp_norm = d->kperp_norm ? 1 : 0;
+  h->P = P;
+  h->NRI = d->n_resonance_interval;
+  h->vA = d->vA;
+  h->Tlim = d->Tlim;
+  h->Bessel_zero = d->Bessel_zero;
+  h->qs.assign(d->qs, d->qs + nspec);
+  h->ms.assign(d->ms, d->ms + nspec);
+  h->ns.assign(d->ns, d->ns + nspec);
+  // separable, uniform, increasing grid: pperp(iperp) from pp(s,iperp,1,1), ppar(ipar) from pp(s,2,ipar,2)
+  h->pperp_h.resize((size_t)nspec * (nperp + 1));
+  h->ppar_h.resize((size_t)nspec * (npar + 1));
+  for (int s = 0; s < nspec; ++s) {
+    for (int i = 0; i <= nperp; ++i) h->pperp_h[(size_t)s * (nperp + 1) + i] = d->pp[f_pp(nspec, nperp, npar, s, i, 1, 0)];
+    for (int j = 0; j <= npar; ++j) h->ppar_h[(size_t)s * (npar + 1) + j] = d->pp[f_pp(nspec, nperp, npar, s, 2, j, 1)];
+    for (int i = 0; i <= nperp; ++i)
+      for (int j = 0; j <= npar; ++j) {
+        if (d->pp[f_pp(nspec, nperp, npar, s, i, j, 0)] != h->pperp_h[(size_t)s * (nperp + 1) + i] ||
+            d->pp[f_pp(nspec, nperp, npar, s, i, j, 1)] != h->ppar_h[(size_t)s * (npar + 1) + j])
+          return bail(fail(ALPSG_E_UNSUPPORTED, "momentum grid is not a tensor product of pperp and ppar"));
+      }
+    const double *pa = &h->ppar_h[(size_t)s * (npar + 1)];
+    const double dpa = pa[2] - pa[1];
+    if (!(dpa > 0)) return bail(fail(ALPSG_E_ARG, "ppar grid must increase"));
+    for (int j = 0; j < npar; ++j)
+      if (!(pa[j + 1] > pa[j]) || std::fabs((pa[j + 1] - pa[j]) - dpa) > 1e-6 * dpa)
+        return bail(fail(ALPSG_E_UNSUPPORTED, "ppar grid is not uniform"));
+  }
+  HIPCHK(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
+  HIPCHK(hipEventCreate(&h->ev0));
+  HIPCHK(hipEventCreate(&h->ev1));
+  const size_t ngrid = (size_t)nspec * (nperp + 1) * (npar + 1);
+  HIPCHK(h->pperp.ensure(h->pperp_h.size()));
+  HIPCHK(h->ppar.ensure(h->ppar_h.size()));
+  HIPCHK(h->df1.ensure(ngrid));
+  HIPCHK(h->df2.ensure(ngrid));
+  HIPCHK(h->qs_d.ensure(nspec));
+  HIPCHK(h->ms_d.ensure(nspec));
+  HIPCHK(h->int_ee.ensure(nspec));
+  HIPCHK(hipMemcpyAsync(h->pperp.p, h->pperp_h.data(), sizeof(double) * h->pperp_h.size(), hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(h->ppar.p, h->ppar_h.data(), sizeof(double) * h->ppar_h.size(), hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(h->qs_d.p, d->qs, sizeof(double) * nspec, hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(h->ms_d.p, d->ms, sizeof(double) * nspec, hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipMemsetAsync(h->df1.p, 0, sizeof(double) * ngrid, h->st));
+  HIPCHK(hipMemsetAsync(h->df2.p, 0, sizeof(double) * ngrid, h->st));
+  if (d->f0) {
+    // f0 to the [s][iperp][ipar] layout, derivatives on the device
+    std::vector<double> f0c(ngrid);
+    for (int s = 0; s < nspec; ++s)
+      for (int i = 0; i <= nperp; ++i)
+        for (int j = 0; j <= npar; ++j)
+          f0c[((size_t)s * (nperp + 1) + i) * (npar + 1) + j] =
+              d->f0[(size_t)s + (size_t)nspec * (i + (size_t)(nperp + 1) * j)];
+    HIPCHK(h->f0.ensure(ngrid));
+    HIPCHK(hipMemcpyAsync(h->f0.p, f0c.data(), sizeof(double) * ngrid, hipMemcpyHostToDevice, h->st));
+    HIPCHK(launch_derivative(h->st, nspec, nperp, npar, h->f0.p, h->pperp.p, h->ppar.p, h->df1.p, h->df2.p));
+    HIPCHK(hipStreamSynchronize(h->st));
+  } else {
+    std::vector<double> a(ngrid, 0.0), b(ngrid, 0.0);
+    for (int s = 0; s < nspec; ++s)
+      for (int i = 1; i <= nperp - 1; ++i)
+        for (int j = 1; j <= npar - 1; ++j) {
+          size_t src = (size_t)s + (size_t)nspec * ((i - 1) + (size_t)(nperp - 1) * ((j - 1)));
+          size_t str = (size_t)nspec * (nperp - 1) * (npar - 1);
+          a[((size_t)s * (nperp + 1) + i) * (npar + 1) + j] = d->df0[src];
+          b[((size_t)s * (nperp + 1) + i) * (npar + 1) + j] = d->df0[src + str];
+        }
+    HIPCHK(hipMemcpyAsync(h->df1.p, a.data(), sizeof(double) * ngrid, hipMemcpyHostToDevice, h->st));
+    HIPCHK(hipMemcpyAsync(h->df2.p, b.data(), sizeof(double) * ngrid, hipMemcpyHostToDevice, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+  }
+  HIPCHK(launch_int_ee(h->st, nspec, nperp, npar, h->pperp.p, h->ppar.p, h->df1.p, h->df2.p, h->qs_d.p, h->ms_d.p, h->int_ee.p));
+  h->int_ee_h.resize(nspec);
+  HIPCHK(hipMemcpyAsync(h->int_ee_h.data(), h->int_ee.p, sizeof(double) * nspec, hipMemcpyDeviceToHost, h->st));
+
+  // analytic-continuation tables
+  std::vector<size_t> fit_off(nspec + 1, 0);
+  for (int s = 0; s < nspec; ++s)
+    fit_off[s + 1] = fit_off[s] + (d->ACmethod[s] == 1 ? (size_t)(nperp + 1) * d->n_fits[s] * FIT_STRIDE : 0);
+  std::vector<double> fitc(std::max<size_t>(fit_off[nspec], 1), 0.0);
+  int maxorder = 0;
+  for (int s = 0; s < nspec; ++s)
+    if (d->ACmethod[s] == 2) maxorder = std::max(maxorder, d->poly_order[s]);
+  std::vector<double> polyc((size_t)nspec * (nperp + 1) * (maxorder + 1), 0.0);
+  const int npf = d->param_fit_nperp;
+  for (int s = 0; s < nspec; ++s) {
+    if (d->ACmethod[s] == 1) {
+      for (int i = 0; i <= nperp; ++i) {
+        const double pv = h->pperp_h[(size_t)s * (nperp + 1) + i];
+        for (int f = 0; f < d->n_fits[s]; ++f) {
+          auto prm = [&](int k) {
+            return d->param_fit[(size_t)s + (size_t)nspec * (i + (size_t)(npf + 1) * (k + 5 * (size_t)f))];
+          };
+          const double pc = d->perp_correction[s + (size_t)nspec * f];
+          const int ft = d->fit_type[s + (size_t)nspec * f];
+          double *c = &fitc[fit_off[s] + ((size_t)i * d->n_fits[s] + f) * FIT_STRIDE];
+          c[1] = prm(1);
+          c[2] = prm(2);
+          if (ft == 1) {
+            c[0] = prm(0) * std::exp(-pc * (pv * pv));
+          } else if (ft == 2) {
+            c[0] = prm(0);
+            c[3] = prm(3);
+            c[4] = pc * prm(4) * (pv * pv);
+          } else {
+            c[0] = prm(0);
+            c[3] = prm(3) * pc * (pv * pv);
+          }
+        }
+      }
+    } else {
+      for (int i = 0; i <= nperp; ++i)
+        for (int n = 0; n <= d->poly_order[s]; ++n)
+          polyc[((size_t)s * (nperp + 1) + i) * (maxorder + 1) + n] =
+              d->poly_fit_coeffs[(size_t)s + (size_t)nspec * (i + (size_t)(nperp + 1) * n)];
+    }
+  }
+  HIPCHK(h->fitc.ensure(fitc.size()));
+  HIPCHK(h->polyc.ensure(polyc.size()));
+  HIPCHK(hipMemcpyAsync(h->fitc.p, fitc.data(), sizeof(double) * fitc.size(), hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(h->polyc.p, polyc.data(), sizeof(double) * polyc.size(), hipMemcpyHostToDevice, h->st));
+  // Miller start indices m(n) = 2*((n+int(sqrt(float(40n))))/2) (ALPS_fns_rel.f90:1622)
+  std::vector<int> mt(MTAB_N, 0);
+  for (int n = 1; n < MTAB_N; ++n) mt[n] = 2 * ((n + (int)std::sqrt((float)(40 * n))) / 2);
+  HIPCHK(h->mtab.ensure(MTAB_N));
+  HIPCHK(hipMemcpyAsync(h->mtab.p, mt.data(), sizeof(int) * MTAB_N, hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+
+  // problem descriptor (per-k pointers filled by set_k)
+  ProbDev &pd = h->prob;
+  std::memset(&pd, 0, sizeof(pd));
+  pd.nspec = nspec;
+  pd.kperp_norm = h->kperp_norm;
+  pd.P = P;
+  pd.NRI = h->NRI;
+  pd.vA = d->vA;
+  pd.Tlim = d->Tlim;
+  for (int s = 0; s < nspec; ++s) {
+    SpecDev &S = pd.sp[s];
+    const double *pe = &h->pperp_h[(size_t)s * (nperp + 1)];
+    const double *pa = &h->ppar_h[(size_t)s * (npar + 1)];
+    S.nperp = nperp;
+    S.npar = npar;
+    S.acmethod = d->ACmethod[s];
+    S.nfits = d->ACmethod[s] == 1 ? d->n_fits[s] : 0;
+    for (int f = 0; f < S.nfits; ++f) S.fit_type[f] = d->fit_type[s + (size_t)nspec * f];
+    S.q = d->qs[s];
+    S.m = d->ms[s];
+    S.dens = d->ns[s];
+    S.dpperp = pe[2] - pe[1];
+    S.dppar = pa[2] - pa[1];
+    S.dppar_abs = std::fabs(pa[2] - pa[1]);
+    S.int_ee = h->int_ee_h[s];
+    S.pperp = h->pperp.p + (size_t)s * (nperp + 1);
+    S.ppar = h->ppar.p + (size_t)s * (npar + 1);
+    S.fitc = h->fitc.p + fit_off[s];
+    if (d->ACmethod[s] == 2) {
+      S.poly_kind = d->poly_kind[s];
+      S.poly_order = d->poly_order[s];
+      S.poly_logfit = d->logfit[s] ? 1 : 0;
+      S.poly_log_max = d->poly_log_max[s];
+      S.poly_norm1 = 5.e-1 * (pa[npar] + pa[0]);
+      S.poly_norm2 = 5.e-1 * (pa[npar] - pa[0]);
+      S.polyc = h->polyc.p + (size_t)s * (nperp + 1) * (maxorder + 1);
+    }
+  }
+  HIPCHK(h->prob_d.ensure(1));
+  *out = hold.release();
+  return ALPSG_OK;
+}
+
+int alpsg_destroy(alpsg_handle h) {
+  delete h;
+  return ALPSG_OK;
+}
+
+int alpsg_set_k(alpsg_handle h, double kperp, double kpar, const int *nmax_in, int *nmax_out) {
+  if (!h) return fail(ALPSG_E_ARG, "null handle");
+  if (!(kpar != 0.0) || !std::isfinite(kpar) || !std::isfinite(kperp))
+    return fail(ALPSG_E_ARG, "kpar must be finite and nonzero");
+  const int nspec = h->nspec, nperp = h->nperp, npar = h->npar;
+  int nmax[ALPSG_MAXSPEC];
+  if (nmax_in) {
+    for (int s = 0; s < nspec; ++s) {
+      if (nmax_in[s] < 1 || nmax_in[s] >= MTAB_N - 2) return fail(ALPSG_E_ARG, "nmax_in out of range");
+      nmax[s] = nmax_in[s];
+    }
+  } else {
+    // determine_nmax: first n >= 1 with max_iperp |J_n| <= Bessel_zero.  The
+    // device evaluates n = 1..ntry, where ntry bounds the answer through
+    // |J_n(x)| <= (x/2)^n / n!.
+    double xmax = 0.0;
+    for (int s = 0; s < nspec; ++s)
+      for (int i = 0; i <= nperp; ++i)
+        xmax = std::max(xmax, std::fabs(kperp * h->pperp_h[(size_t)s * (nperp + 1) + i] / h->qs[s]));
+    int ntry = 1;
+    double lb = 0.0, lz = std::log(std::max(h->Bessel_zero, 1e-300)) - std::log(1e6);
+    while (ntry < MTAB_N - 16) {
+      lb = ntry * std::log(std::max(xmax, 1e-300) / 2.0) - std::lgamma(ntry + 1.0);
+      if (lb < lz && ntry > xmax) break;
+      ++ntry;
+    }
+    ntry = std::min(ntry + 8, MTAB_N - 4);
+    HIPCHK(h->bmax.ensure((size_t)nspec * ntry));
+    HIPCHK(launch_bessel_max(h->st, nspec, nperp, ntry, kperp, h->pperp.p, h->qs_d.p, h->mtab.p, h->bmax.p));
+    std::vector<double> bm((size_t)nspec * ntry);
+    HIPCHK(hipMemcpyAsync(bm.data(), h->bmax.p, sizeof(double) * bm.size(), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    for (int s = 0; s < nspec; ++s) {
+      int n = 0;
+      for (int t = 1; t <= ntry; ++t)
+        if (!(bm[(size_t)s * ntry + (t - 1)] > h->Bessel_zero)) {
+          n = t;
+          break;
+        }
+      if (n == 0) return fail(ALPSG_E_ARG, "Bessel cutoff not reached (kperp too large?)");
+      nmax[s] = n;
+    }
+  }
+  for (int s = 0; s < nspec; ++s) {
+    SpecDev &S = h->prob.sp[s];
+    const int nm = nmax[s];
+    HIPCHK(h->bes[s].ensure((size_t)(nm + 3) * (nperp + 1)));
+    HIPCHK(h->atab[s].ensure((size_t)(nm + 1) * (nperp + 1) * ATAB_STRIDE));
+    HIPCHK(h->nodes[s].ensure((size_t)(nm + 1) * (npar - 1) * NODE_STRIDE));
+    HIPCHK(launch_species_k(h->st, nperp, npar, nm, kperp, kpar, h->qs[s], h->ms[s],
+                            h->pperp.p + (size_t)s * (nperp + 1), h->ppar.p + (size_t)s * (npar + 1),
+                            h->df1.p + (size_t)s * (nperp + 1) * (npar + 1),
+                            h->df2.p + (size_t)s * (nperp + 1) * (npar + 1), h->mtab.p, h->bes[s].p,
+                            h->atab[s].p, h->nodes[s].p));
+    S.nmax = nm;
+    S.z = h->kperp_norm ? kperp / h->qs[s] : 1.0 / h->qs[s];
+    S.atab = h->atab[s].p;
+    S.nodes = h->nodes[s].p;
+    h->nmax[s] = nm;
+  }
+  h->prob.kperp = kperp;
+  h->prob.kpar = kpar;
+  HIPCHK(hipMemcpyAsync(h->prob_d.p, &h->prob, sizeof(ProbDev), hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  h->kperp = kperp;
+  h->kpar = kpar;
+  h->have_k = true;
+  if (nmax_out)
+    for (int s = 0; s < nspec; ++s) nmax_out[s] = nmax[s];
+  return ALPSG_OK;
+}
+
+int alpsg_get_nmax(alpsg_handle h, int *nmax_out) {
+  if (!h || !nmax_out) return fail(ALPSG_E_ARG, "null argument");
+  if (!h->have_k) return fail(ALPSG_E_STATE, "alpsg_set_k has not been called");
+  for (int s = 0; s < h->nspec; ++s) nmax_out[s] = h->nmax[s];
+  return ALPSG_OK;
+}
+
+long long alpsg_eval_count(alpsg_handle h) { return h ? h->evals : -1; }
+
+double alpsg_last_kernel_ms(alpsg_handle h) { return h ? h->last_ms : -1.0; }
+
+int alpsg_disp_batch(alpsg_handle h, int n, const double *om, double *det, double *chi0,
+                     double *chi0_low, double *wave) {
+  if (!h || (n > 0 && (!om || !det))) return fail(ALPSG_E_ARG, "null argument");
+  return run_disp(h, n, om, det, chi0, chi0_low, wave);
+}
+
+int alpsg_disp_plain(alpsg_handle h, int n, const double *om, double *det) {
+  if (!h || (n > 0 && (!om || !det))) return fail(ALPSG_E_ARG, "null argument");
+  for (int i = 0; i < n; ++i) {
+    int rc = run_disp(h, 1, om + 2 * i, det + 2 * i, nullptr, nullptr, nullptr, 64);
+    if (rc) return rc;
+  }
+  return ALPSG_OK;
+}
+
+int alpsg_secant_batch(alpsg_handle h, const alpsg_root_params *rp, int nroots, double *om,
+                       double *D_final, int *iters) {
+  if (!h || !rp || (nroots > 0 && !om)) return fail(ALPSG_E_ARG, "null argument");
+  if (!h->have_k) return fail(ALPSG_E_STATE, "alpsg_set_k has not been called");
+  if (rp->secant_method != 2) return fail(ALPSG_E_UNSUPPORTED, "only secant_method = 2 (secant_osc) is batched");
+  if (nroots <= 0) return ALPSG_OK;
+  return secant_osc_batch(h, rp, nroots, reinterpret_cast<cplx *>(om),
+                          reinterpret_cast<cplx *>(D_final), iters);
+}
+
+int alpsg_map_search(alpsg_handle h, double omi, double omf, double gami, double gamf, int nr,
+                     int ni, int loggridw, int loggridg, double *om_out, double *cal_out,
+                     double *val_out, int numroots, int *iroots, int *nfound) {
+  if (!h) return fail(ALPSG_E_ARG, "null handle");
+  if (nr < 1 || ni < 1) return fail(ALPSG_E_ARG, "nr, ni must be >= 1");
+  const size_t N = (size_t)nr * ni;
+  std::vector<cplx> om(N), cal(N);
+  std::vector<double> val(N);
+  double dr = omf - omi, di = gamf - gami;
+  if (nr > 1) dr = (omf - omi) / (1.0 * (nr - 1));
+  if (ni > 1) di = (gamf - gami) / (1.0 * (ni - 1));
+  for (int ir = 1; ir <= nr; ++ir) {
+    double wr;
+    if (loggridw) {
+      wr = omi;
+      if (nr > 1) wr = omi * std::pow(omf / omi, (1.0 * (ir - 1)) / (1.0 * (nr - 1)));
+    } else {
+      wr = omi + dr * (1.0 * (ir - 1));
+    }
+    for (int ii = 1; ii <= ni; ++ii) {
+      double wi;
+      if (loggridg) {
+        wi = gami;
+        if (ni > 1) wi = gami * std::pow(gamf / gami, (1.0 * (ii - 1)) / (1.0 * (ni - 1)));
+      } else {
+        wi = gami + di * (1.0 * (ii - 1));
+      }
+      om[(size_t)(ir - 1) + (size_t)nr * (ii - 1)] = cplx(wr, wi);
+    }
+  }
+  int rc = run_disp(h, (int)N, reinterpret_cast<const double *>(om.data()),
+                    reinterpret_cast<double *>(cal.data()), nullptr, nullptr, nullptr);
+  if (rc) return rc;
+  for (size_t k = 0; k < N; ++k) {
+    // log10|D| and the NaN / Inf sentinels of ALPS_fns.f90:3723-3742
+    val[k] = std::log10(std::abs(cal[k]));
+    const double tmp = cal[k].real();
+    if (cal[k].imag() != 0.0) {
+      if (!(cplx(tmp, 0.0) != cal[k])) {
+        cal[k] = 999999.0;
+        val[k] = 999999.0;
+      }
+    } else if (cplx(tmp, 0.0) != cal[k]) {
+      cal[k] = 999999.0;
+      val[k] = 999999.0;
+    }
+    if (std::fabs(tmp) > 1.e100) {
+      cal[k] = 899999.0;
+      val[k] = 899999.0;
+    }
+  }
+  int n = 0;
+  if (nr > 1 && ni > 1) {
+    auto V = [&](int ir, int ii) { return val[(size_t)(ir - 1) + (size_t)nr * (ii - 1)]; };
+    for (int ii = ni; ii >= 1; --ii)
+      for (int ir = 1; ir <= nr; ++ir) {
+        bool rok, hit = false;
+        if (ir == 1)
+          rok = V(ir, ii) < V(ir + 1, ii);
+        else if (ir == nr)
+          rok = V(ir, ii) < V(ir - 1, ii);
+        else
+          rok = V(ir, ii) < V(ir - 1, ii) && V(ir, ii) < V(ir + 1, ii);
+        if (rok) {
+          if (ii == 1)
+            hit = V(ir, ii) < V(ir, ii + 1);
+          else if (ii == ni)
+            hit = V(ir, ii) < V(ir, ii - 1);
+          else
+            hit = V(ir, ii) < V(ir, ii - 1) && V(ir, ii) < V(ir, ii + 1);
+        }
+        if (hit) {
+          if (iroots && n < numroots) {
+            iroots[2 * n] = ir;
+            iroots[2 * n + 1] = ii;
+          }
+          ++n;
+        }
+      }
+  }
+  if (nfound) *nfound = n;
+  if (om_out) std::memcpy(om_out, om.data(), sizeof(cplx) * N);
+  if (cal_out) std::memcpy(cal_out, cal.data(), sizeof(cplx) * N);
+  if (val_out) std::memcpy(val_out, val.data(), sizeof(double) * N);
+  return ALPSG_OK;
+}
+
+int alpsg_om_scan(alpsg_handle h, const alpsg_root_params *rp, int scan_type, int log_scan,
+                  double diff, double diff2, int n_out, int n_res, int nroots, double *roots,
+                  double *kperp_out, double *kpar_out, double *roots_out) {
+  if (!h || !rp || !roots || !kperp_out || !kpar_out || !roots_out) return fail(ALPSG_E_ARG, "null argument");
+  if (!h->have_k) return fail(ALPSG_E_STATE, "alpsg_set_k has not been called");
+  if (scan_type < 0 || scan_type > 4 || n_out < 1 || n_res < 1 || nroots < 1) return fail(ALPSG_E_ARG, "scan parameters");
+  cplx *R = reinterpret_cast<cplx *>(roots);
+  cplx *RO = reinterpret_cast<cplx *>(roots_out);
+  std::vector<int> jump(nroots, 1);
+  int row = 0;
+  kperp_out[0] = h->kperp;
+  kpar_out[0] = h->kpar;
+  for (int r = 0; r < nroots; ++r) RO[r] = R[r];
+  const double kperp_last = h->kperp, kpar_last = h->kpar;
+  const double theta_0 = std::atan(kperp_last / kpar_last);
+  const double k_0 = std::sqrt(kperp_last * kperp_last + kpar_last * kpar_last);
+  const int nt = n_out * n_res;
+  std::vector<cplx> om;
+  std::vector<int> idx;
+  for (int it = 1; it <= nt; ++it) {
+    double kperp = h->kperp, kpar = h->kpar;
+    switch (scan_type) {
+      case 0:
+        if (log_scan) {
+          kperp = std::pow(10.0, std::log10(kperp_last) + diff * it);
+          kpar = std::pow(10.0, std::log10(kpar_last) + diff2 * it);
+        } else {
+          kperp = kperp_last + diff * it;
+          kpar = kpar_last + diff2 * it;
+        }
+        break;
+      case 1: {
+        double th = log_scan ? std::pow(10.0, std::log10(theta_0) + diff * it) : theta_0 + diff * it;
+        kperp = k_0 * std::sin(th);
+        kpar = k_0 * std::cos(th);
+      } break;
+      case 2: {
+        double kt = log_scan ? std::pow(10.0, std::log10(k_0) + diff * it) : (k_0) + diff * it;
+        kperp = kt * std::sin(theta_0);
+        kpar = kt * std::cos(theta_0);
+      } break;
+      case 3:
+        kperp = log_scan ? std::pow(10.0, std::log10(kperp_last) + diff * it) : kperp_last + diff * it;
+        break;
+      case 4:
+        kpar = log_scan ? std::pow(10.0, std::log10(kpar_last) + diff * it) : kpar_last + diff * it;
+        break;
+    }
+    // scan type 4 keeps kperp, so nmax and the Bessel table are kept as well
+    int rc;
+    if (scan_type != 4) {
+      rc = alpsg_set_k(h, kperp, kpar, nullptr, nullptr);
+    } else {
+      int nm[ALPSG_MAXSPEC];
+      for (int s = 0; s < h->nspec; ++s) nm[s] = h->nmax[s];
+      rc = alpsg_set_k(h, kperp, kpar, nm, nullptr);
+    }
+    if (rc) return rc;
+    bool any = false;
+    for (int r = 0; r < nroots; ++r) any = any || jump[r];
+    if (!any) return fail(ALPSG_E_NOTFOUND, "all roots dropped during the scan (alps_error(9))");
+    om.clear();
+    idx.clear();
+    for (int r = 0; r < nroots; ++r)
+      if (jump[r]) {
+        om.push_back(R[r]);
+        idx.push_back(r);
+      }
+    std::vector<cplx> Df(om.size());
+    rc = secant_osc_batch(h, rp, (int)om.size(), om.data(), Df.data(), nullptr);
+    if (rc) return rc;
+    for (size_t t = 0; t < idx.size(); ++t) R[idx[t]] = om[t];
+    for (size_t t = 0; t < idx.size(); ++t) {
+      int r = idx[t];
+      double re = R[r].real();
+      if (std::isnan(re) || std::fabs(re) >= 1.79769313486231570815e+308) jump[r] = 0;
+      for (int imm = 0; imm < r; ++imm)
+        if (std::abs(R[r] - R[imm]) < rp->D_gap) {
+          R[r] = 0.0;
+          jump[r] = 0;
+        }
+    }
+    if (it % n_res == 0) {
+      ++row;
+      kperp_out[row] = h->kperp;
+      kpar_out[row] = h->kpar;
+      for (int r = 0; r < nroots; ++r) RO[(size_t)row * nroots + r] = R[r];
+    }
+  }
+  return ALPSG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,96 @@
+// Device-side data of the dispersion engine: one ProbDev per handle holds the
+// per-species tables that the kernels read.  All tables live in HBM and are
+// built on the device (alpsg_tables.hip); the structs only carry pointers and
+// scalars, so a ProbDev can be copied to device memory as one block.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#define ALPSG_DMAXSPEC 8
+#define ALPSG_DMAXFITS 4
+#define NODE_STRIDE 8   // doubles per (n, ipar) node record: G1 H1 G2 H2 G3 H3 ppar pad
+#define ATAB_STRIDE 4   // doubles per (n, iperp) entry: J^2, pperp^2 J'^2, pperp J J', J
+#define FIT_STRIDE 8    // doubles per (iperp, fit) entry of the analytic-continuation table
+
+struct SpecDev {
+  int nperp, npar, nmax;
+  int acmethod, nfits;
+  int fit_type[ALPSG_DMAXFITS];
+  int poly_kind, poly_order, poly_logfit;
+  double q, m, dens;
+  double dpperp;      // pp(2,2,1) - pp(1,2,1)
+  double dppar;       // pp(2,2,2) - pp(2,1,2)  (integrate_res, determine_resonances)
+  double dppar_abs;   // abs(...)               (integrate, funct_g, landau_integrate)
+  double z;           // T-tensor Bessel scale: kperp/q (kperp_norm) or 1/q
+  double int_ee;      // omega-independent ee term of this species
+  double poly_log_max, poly_norm1, poly_norm2;
+  const double *pperp;  // [nperp+1]
+  const double *ppar;   // [npar+1]
+  const double *atab;   // [(nmax+1)*(nperp+1)*ATAB_STRIDE]
+  const double *nodes;  // [(nmax+1)*(npar-1)*NODE_STRIDE]
+  const double *fitc;   // [(nperp+1)*nfits*FIT_STRIDE]
+  const double *polyc;  // [(nperp+1)*(poly_order+1)]
+};
+
+struct ProbDev {
+  int nspec, kperp_norm, P, NRI;
+  double kperp, kpar, vA, Tlim;
+  SpecDev sp[ALPSG_DMAXSPEC];
+};
+
+// ---------------------------------------------------------------------------
+// Small complex helpers (double2 = (re, im)).
+// ---------------------------------------------------------------------------
+struct cd {
+  double re, im;
+};
+__device__ __forceinline__ cd mk(double r, double i) { return cd{r, i}; }
+__device__ __forceinline__ cd operator+(cd a, cd b) { return cd{a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ cd operator-(cd a, cd b) { return cd{a.re - b.re, a.im - b.im}; }
+__device__ __forceinline__ cd operator*(cd a, cd b) {
+  return cd{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__device__ __forceinline__ cd operator*(double s, cd a) { return cd{s * a.re, s * a.im}; }
+__device__ __forceinline__ cd operator*(cd a, double s) { return cd{s * a.re, s * a.im}; }
+__device__ __forceinline__ cd conjc(cd a) { return cd{a.re, -a.im}; }
+__device__ __forceinline__ cd timesI(cd a) { return cd{-a.im, a.re}; }
+__device__ __forceinline__ double cabs2(cd a) { return a.re * a.re + a.im * a.im; }
+__device__ __forceinline__ double cabsd(cd a) { return hypot(a.re, a.im); }
+
+// Reciprocal of a double: hardware estimate refined by two Newton steps
+// (full double precision for |x| in the normal range).
+__device__ __forceinline__ double rcp_d(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  return r;
+}
+// s / z for complex z
+__device__ __forceinline__ cd cscale_inv(double s, cd z) {
+  double inv = s * rcp_d(cabs2(z));
+  return cd{z.re * inv, -z.im * inv};
+}
+// a / b, complex (Smith's algorithm)
+__device__ __forceinline__ cd cdiv(cd a, cd b) {
+  if (fabs(b.re) >= fabs(b.im)) {
+    double r = b.im / b.re, d = b.re + b.im * r;
+    return cd{(a.re + a.im * r) / d, (a.im - a.re * r) / d};
+  } else {
+    double r = b.re / b.im, d = b.re * r + b.im;
+    return cd{(a.re * r + a.im) / d, (a.im * r - a.re) / d};
+  }
+}
+__device__ __forceinline__ cd cexpd(cd z) {
+  double e = exp(z.re), s, c;
+  sincos(z.im, &s, &c);
+  return cd{e * c, e * s};
+}
+__device__ __forceinline__ cd clogd(cd z) { return cd{log(hypot(z.re, z.im)), atan2(z.im, z.re)}; }
+// principal z**p for real p
+__device__ __forceinline__ cd cpowr(cd z, double p) {
+  if (z.re == 0.0 && z.im == 0.0) return cd{0.0, 0.0};
+  cd l = clogd(z);
+  return cexpd(cd{p * l.re, p * l.im});
+}

@@ -1,0 +1,319 @@
+// Per-handle and per-k tables of the dispersion engine, built on the device.
+//
+//   k_derivative   df0 from f0 (derivative_f0, ALPS_fns.f90:103-115)
+//   k_int_ee       omega-independent ee term (int_ee, ALPS_fns.f90:1457-1555)
+//   k_bessel_max   max_iperp |J_n| for n = 1..ntry (determine_nmax, ALPS_fns.f90:4017-4035)
+//   k_bessel       J_n table, n = -1..nmax+1 (determine_bessel_array, ALPS_fns.f90:4214-4255)
+//   k_atab         J^2, pperp^2 J'^2, pperp J J' per (n, iperp) (int_T, ALPS_fns.f90:1639-1707)
+//   k_moments      pperp-reduced node tables G_a(n,ipar), H_a(n,ipar)
+//
+// The pperp moments rest on the factorisation of resU*int_T
+// (ALPS_fns.f90:1587-1592, 1659-1703): the numerator of resU is
+// q*(om*df1 + A) with A = (kpar/m)*(pperp*df2 - ppar*df1), the denominator
+// depends on (n, ipar, om) only, and T = C_mode(n) * b_mode(ipar) * a(n, iperp)
+// with a in {J^2, pperp^2 J'^2, pperp J J'} and b in {1, ppar, ppar^2}.  The
+// trapezoid weights of integrate() (ALPS_fns.f90:838-860) are separable,
+// w = wperp(iperp) * wpar(ipar) with wperp = 2,...,2,1, so
+//   G_a(n,ipar) = sum_iperp wperp a(n,iperp) df1(iperp,ipar)
+//   H_a(n,ipar) = sum_iperp wperp a(n,iperp) A(iperp,ipar)
+// are omega-independent and every later omega needs only (n, ipar) data.
+#include "alpsg_device.h"
+
+// ---------------------------------------------------------------------------
+// Bessel functions (ALPS_fns_rel.f90:1580-1735), evaluated with the same
+// operation order as the reference and without FMA contraction, so the table
+// matches the CPU restatement up to the last bit of sin/cos/sqrt.
+// ---------------------------------------------------------------------------
+__device__ static double dev_bessj0(double x) {
+#pragma clang fp contract(off)
+  const double P1 = 1.e0, P2 = -.1098628627e-2, P3 = .2734510407e-4, P4 = -.2073370639e-5,
+               P5 = .2093887211e-6;
+  const double Q1 = -.1562499995e-1, Q2 = .1430488765e-3, Q3 = -.6911147651e-5,
+               Q4 = .7621095161e-6, Q5 = -.9349451520e-7;
+  const double R1 = 57568490574.e0, R2 = -13362590354.e0, R3 = 651619640.7e0,
+               R4 = -11214424.18e0, R5 = 77392.33017e0, R6 = -184.9052456e0;
+  const double S1 = 57568490411.e0, S2 = 1029532985.e0, S3 = 9494680.718e0, S4 = 59272.64853e0,
+               S5 = 267.8532712e0, S6 = 1.e0;
+  if (x == 0.0) return 1.0;
+  double ax = fabs(x);
+  if (ax < 8.0) {
+    double y = x * x;
+    double fr = R1 + y * (R2 + y * (R3 + y * (R4 + y * (R5 + y * R6))));
+    double fs = S1 + y * (S2 + y * (S3 + y * (S4 + y * (S5 + y * S6))));
+    return fr / fs;
+  }
+  double z = 8.0 / ax, y = z * z, xx = ax - .785398164e0;
+  double fp = P1 + y * (P2 + y * (P3 + y * (P4 + y * P5)));
+  double fq = Q1 + y * (Q2 + y * (Q3 + y * (Q4 + y * Q5)));
+  return sqrt(.636619772e0 / ax) * (fp * cos(xx) - z * fq * sin(xx));
+}
+
+__device__ static double dev_bessj1(double x) {
+#pragma clang fp contract(off)
+  const double P1 = 1.e0, P2 = .183105e-2, P3 = -.3516396496e-4, P4 = .2457520174e-5,
+               P5 = -.240337019e-6, P6 = .636619772e0;
+  const double Q1 = .04687499995e0, Q2 = -.2002690873e-3, Q3 = .8449199096e-5,
+               Q4 = -.88228987e-6, Q5 = .105787412e-6;
+  const double R1 = 72362614232.e0, R2 = -7895059235.e0, R3 = 242396853.1e0,
+               R4 = -2972611.439e0, R5 = 15704.48260e0, R6 = -30.16036606e0;
+  const double S1 = 144725228442.e0, S2 = 2300535178.e0, S3 = 18583304.74e0,
+               S4 = 99447.43394e0, S5 = 376.9991397e0, S6 = 1.e0;
+  double ax = fabs(x);
+  if (ax < 8.0) {
+    double y = x * x;
+    double fr = R1 + y * (R2 + y * (R3 + y * (R4 + y * (R5 + y * R6))));
+    double fs = S1 + y * (S2 + y * (S3 + y * (S4 + y * (S5 + y * S6))));
+    return x * (fr / fs);
+  }
+  double z = 8.0 / ax, y = z * z;
+  double xx = ax - (double)2.35619491f;  // default-real literal in the reference
+  double fp = P1 + y * (P2 + y * (P3 + y * (P4 + y * P5)));
+  double fq = Q1 + y * (Q2 + y * (Q3 + y * (Q4 + y * Q5)));
+  double r = sqrt(P6 / ax) * (cos(xx) * fp - z * sin(xx) * fq);
+  return r * copysign(S6, x);
+}
+
+// BESSJ with the Miller start index m = 2*((n+int(sqrt(float(40n))))/2)
+// supplied by the host (mtab[n]) so that it is the same integer as the CPU's
+// single-precision sqrt gives.
+__device__ static double dev_bessj(int n, double x, const int *__restrict__ mtab) {
+#pragma clang fp contract(off)
+  const double BIGNO = 1.e10, BIGNI = 1.e-10;
+  if (n == 0) return dev_bessj0(x);
+  if (n == 1) return dev_bessj1(x);
+  if (x == 0.0) return 0.0;
+  double tox = 2.0 / x;
+  if (x > (double)(float)n) {
+    double bjm = dev_bessj0(x), bj = dev_bessj1(x), bjp;
+    for (int j = 1; j <= n - 1; ++j) {
+      bjp = j * tox * bj - bjm;
+      bjm = bj;
+      bj = bjp;
+    }
+    return bj;
+  }
+  int m = mtab[n];
+  double res = 0.0, sum = 0.0, bjp = 0.0, bj = 1.0, bjm;
+  int jsum = 0;
+  for (int j = m; j >= 1; --j) {
+    bjm = j * tox * bj - bjp;
+    bjp = bj;
+    bj = bjm;
+    if (fabs(bj) > BIGNO) {
+      bj = bj * BIGNI;
+      bjp = bjp * BIGNI;
+      res = res * BIGNI;
+      sum = sum * BIGNI;
+    }
+    if (jsum != 0) sum = sum + bj;
+    jsum = 1 - jsum;
+    if (j == n) res = bjp;
+  }
+  sum = 2.0 * sum - bj;
+  return res / sum;
+}
+
+// df0 on the interior (derivative_f0).  f0, pp in the handle's C layout:
+// f0[s][iperp][ipar], pperp[s][iperp], ppar[s][ipar]; df1/df2 [s][iperp][ipar].
+__global__ void k_derivative(int nspec, int nperp, int npar, const double *__restrict__ f0,
+                             const double *__restrict__ pperp, const double *__restrict__ ppar,
+                             double *__restrict__ df1, double *__restrict__ df2) {
+  int ipar = blockIdx.x * blockDim.x + threadIdx.x;
+  int iperp = blockIdx.y;
+  int s = blockIdx.z;
+  if (ipar < 1 || ipar > npar - 1 || iperp < 1 || iperp > nperp - 1) return;
+  size_t row = (size_t)s * (nperp + 1) * (npar + 1);
+  size_t o = row + (size_t)iperp * (npar + 1) + ipar;
+  const double *pe = pperp + (size_t)s * (nperp + 1);
+  const double *pa = ppar + (size_t)s * (npar + 1);
+  df1[o] = (f0[o + (npar + 1)] - f0[o - (npar + 1)]) / (pe[iperp + 1] - pe[iperp - 1]);
+  df2[o] = (f0[o + 1] - f0[o - 1]) / (pa[ipar + 1] - pa[ipar - 1]);
+}
+
+// Copy a user-supplied df0 (already interior-only) is done on the host side.
+
+// int_ee for one species per block (blockDim = 256).  Weights follow the
+// reference loops: corners 2,2,1,1 and the (1,1) corner's quirk
+// pp(..,1)*df0(..,1) in place of pp(..,2)*df0(..,1).
+__global__ void k_int_ee(int nperp, int npar, const double *__restrict__ pperp,
+                         const double *__restrict__ ppar, const double *__restrict__ df1,
+                         const double *__restrict__ df2, const double *__restrict__ qs,
+                         const double *__restrict__ ms, double *__restrict__ out) {
+  __shared__ double red[256];
+  int s = blockIdx.x;
+  const double *pe = pperp + (size_t)s * (nperp + 1);
+  const double *pa = ppar + (size_t)s * (npar + 1);
+  const double *d1 = df1 + (size_t)s * (nperp + 1) * (npar + 1);
+  const double *d2 = df2 + (size_t)s * (nperp + 1) * (npar + 1);
+  double acc = 0.0;
+  int n_int = (nperp - 1) * (npar - 1);
+  for (int t = threadIdx.x; t < n_int; t += blockDim.x) {
+    int iperp = 1 + t / (npar - 1), ipar = 1 + t % (npar - 1);
+    size_t o = (size_t)iperp * (npar + 1) + ipar;
+    double wperp = (iperp == nperp - 1) ? 1.0 : 2.0;
+    double wpar = (ipar == 1 || ipar == npar - 1) ? 1.0 : 2.0;
+    double term;
+    if (iperp == 1 && ipar == 1)
+      term = pa[ipar] * (d2[o] * pe[iperp] - pe[iperp] * d1[o]);
+    else
+      term = pa[ipar] * (d2[o] * pe[iperp] - pa[ipar] * d1[o]);
+    acc += (wperp * wpar) * term;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double dpperp = pe[2] - pe[1];
+    double dppar = fabs(pa[2] - pa[1]);
+    double v = red[0];
+    v = v * 2.0 * 3.141592653589793238462643383279502884 * qs[s] / ms[s];
+    v = v * dpperp * dppar * 0.25;
+    out[s] = v;
+  }
+}
+
+// max_iperp |J_n(kperp*pperp/q)| for n = 1..ntry: one block per (n, species).
+__global__ void k_bessel_max(int nperp, int ntry, double kperp, const double *__restrict__ pperp,
+                             const double *__restrict__ qs, const int *__restrict__ mtab,
+                             double *__restrict__ out) {
+  __shared__ double red[64];
+  int n = blockIdx.x + 1;
+  int s = blockIdx.y;
+  const double *pe = pperp + (size_t)s * (nperp + 1);
+  double m = 0.0;
+  for (int ip = threadIdx.x; ip <= nperp; ip += blockDim.x) {
+    double z = kperp * pe[ip] / qs[s];
+    m = fmax(m, fabs(dev_bessj(n, z, mtab)));
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[(size_t)s * ntry + (n - 1)] = red[0];
+}
+
+// bessel_array(n, iperp) for n = -1..nmax+1 -> bes[(n+1)*(nperp+1) + iperp]
+__global__ void k_bessel(int nperp, int nmax, double kperp, double q,
+                         const double *__restrict__ pperp, const int *__restrict__ mtab,
+                         double *__restrict__ bes) {
+  int ip = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = (int)blockIdx.y - 1;
+  if (ip > nperp || n > nmax + 1) return;
+  double z = kperp * pperp[ip] / q;
+  double v = (n == -1) ? -dev_bessj(1, z, mtab) : dev_bessj(n, z, mtab);
+  bes[(size_t)(n + 1) * (nperp + 1) + ip] = v;
+}
+
+// a-types per (n >= 0, iperp): J^2, pperp^2 J'^2, pperp J J', J.
+// J'_n = (J_{n-1} - J_{n+1})/2 for n >= 1, -J_1 for n = 0 (int_T, ALPS_fns.f90:1647-1656).
+// The negative harmonics use the same values: J_{-n}^2 = J_n^2 and
+// J_{-n} J'_{-n} = J_n J'_n exactly in floating point.
+__global__ void k_atab(int nperp, int nmax, const double *__restrict__ pperp,
+                       const double *__restrict__ bes, double *__restrict__ atab) {
+  int ip = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = blockIdx.y;
+  if (ip > nperp || n > nmax) return;
+  const int W = nperp + 1;
+  double J = bes[(size_t)(n + 1) * W + ip];
+  double Jp = (n >= 1) ? 0.5 * (bes[(size_t)n * W + ip] - bes[(size_t)(n + 2) * W + ip])
+                       : -bes[(size_t)2 * W + ip];
+  double p = pperp[ip];
+  double *o = atab + ((size_t)n * W + ip) * ATAB_STRIDE;
+  o[0] = J * J;
+  o[1] = (p * p) * Jp * Jp;
+  o[2] = p * J * Jp;
+  o[3] = J;
+}
+
+// Node tables: one thread per (n, ipar), summing over iperp.  The a-values of a
+// given n are shared by the whole block (staged in LDS); df1/df2 rows are read
+// coalesced along ipar.
+__global__ void k_moments(int nperp, int npar, int nmax, double kpar, double m,
+                          const double *__restrict__ pperp, const double *__restrict__ ppar,
+                          const double *__restrict__ df1, const double *__restrict__ df2,
+                          const double *__restrict__ atab, double *__restrict__ nodes) {
+  extern __shared__ double lds_a[];  // (nperp-1) * 3
+  int n = blockIdx.y;
+  int ipar = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int W = nperp + 1;
+  for (int t = threadIdx.x; t < (nperp - 1); t += blockDim.x) {
+    const double *a = atab + ((size_t)n * W + (t + 1)) * ATAB_STRIDE;
+    double wperp = (t + 1 == nperp - 1) ? 1.0 : 2.0;
+    lds_a[3 * t + 0] = wperp * a[0];
+    lds_a[3 * t + 1] = wperp * a[1];
+    lds_a[3 * t + 2] = wperp * a[2];
+  }
+  __syncthreads();
+  if (ipar > npar - 1) return;
+  double pa = ppar[ipar];
+  double kom = kpar / m;
+  double g1 = 0, h1 = 0, g2 = 0, h2 = 0, g3 = 0, h3 = 0;
+  for (int iperp = 1; iperp <= nperp - 1; ++iperp) {
+    size_t o = (size_t)iperp * (npar + 1) + ipar;
+    double d1 = df1[o], d2 = df2[o];
+    double A = kom * (pperp[iperp] * d2 - pa * d1);
+    double a1 = lds_a[3 * (iperp - 1) + 0], a2 = lds_a[3 * (iperp - 1) + 1],
+           a3 = lds_a[3 * (iperp - 1) + 2];
+    g1 += a1 * d1;
+    h1 += a1 * A;
+    g2 += a2 * d1;
+    h2 += a2 * A;
+    g3 += a3 * d1;
+    h3 += a3 * A;
+  }
+  double *o = nodes + ((size_t)n * (npar - 1) + (ipar - 1)) * NODE_STRIDE;
+  o[0] = g1;
+  o[1] = h1;
+  o[2] = g2;
+  o[3] = h2;
+  o[4] = g3;
+  o[5] = h3;
+  o[6] = pa;
+  o[7] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers (called from alpsg_api.cpp).
+// ---------------------------------------------------------------------------
+hipError_t launch_derivative(hipStream_t st, int nspec, int nperp, int npar, const double *f0,
+                             const double *pperp, const double *ppar, double *df1, double *df2) {
+  dim3 grid((npar + 1 + 127) / 128, nperp + 1, nspec);
+  hipLaunchKernelGGL(k_derivative, grid, dim3(128), 0, st, nspec, nperp, npar, f0, pperp, ppar,
+                     df1, df2);
+  return hipGetLastError();
+}
+
+hipError_t launch_int_ee(hipStream_t st, int nspec, int nperp, int npar, const double *pperp,
+                         const double *ppar, const double *df1, const double *df2,
+                         const double *qs, const double *ms, double *out) {
+  hipLaunchKernelGGL(k_int_ee, dim3(nspec), dim3(256), 0, st, nperp, npar, pperp, ppar, df1, df2,
+                     qs, ms, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_bessel_max(hipStream_t st, int nspec, int nperp, int ntry, double kperp,
+                             const double *pperp, const double *qs, const int *mtab,
+                             double *out) {
+  hipLaunchKernelGGL(k_bessel_max, dim3(ntry, nspec), dim3(64), 0, st, nperp, ntry, kperp, pperp,
+                     qs, mtab, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_species_k(hipStream_t st, int nperp, int npar, int nmax, double kperp,
+                            double kpar, double q, double m, const double *pperp,
+                            const double *ppar, const double *df1, const double *df2,
+                            const int *mtab, double *bes, double *atab, double *nodes) {
+  hipLaunchKernelGGL(k_bessel, dim3((nperp + 1 + 63) / 64, nmax + 3), dim3(64), 0, st, nperp,
+                     nmax, kperp, q, pperp, mtab, bes);
+  hipLaunchKernelGGL(k_atab, dim3((nperp + 1 + 63) / 64, nmax + 1), dim3(64), 0, st, nperp, nmax,
+                     pperp, bes, atab);
+  size_t lds = sizeof(double) * 3 * (size_t)(nperp - 1);
+  hipLaunchKernelGGL(k_moments, dim3((npar - 1 + 63) / 64, nmax + 1), dim3(64), lds, st, nperp,
+                     npar, nmax, kpar, m, pperp, ppar, df1, df2, atab, nodes);
+  return hipGetLastError();
+}

@@ -1,0 +1,291 @@
+"""Python host of the gfx950 dispersion engine, mirroring ALPS's call surface.
+
+The reference's public procedures (ALPS_fns.f90:26-29) map onto
+:class:`Engine` methods:
+
+=====================================  =====================================
+reference                              this module
+=====================================  =====================================
+``disp(om)``            :252           ``Engine.disp(om)`` / ``disp_batch``
+``determine_nmax`` +    :3971, :4214   ``Engine.set_k(kperp, kpar)``
+``determine_bessel_array``
+``secant_osc(om, in)``  :1919          ``Engine.secant_osc(om)`` (batched)
+``refine_guess``        :3793          ``Engine.refine_guess(guesses)``
+``map_search``          :3595          ``Engine.map_search(...)``
+``om_scan(ik)``         :2198          ``Engine.om_scan(...)``
+=====================================  =====================================
+
+Every call goes through the C-ABI of ``libalpsg.so`` (include/alpsg.h); if the
+library is missing the import fails: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .plasma import Plasma, RootParams
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libalpsg.so")
+
+ALPSG_OK = 0
+_ERRNAMES = {1: "ALPSG_E_ARG", 2: "ALPSG_E_UNSUPPORTED", 3: "ALPSG_E_DEVICE",
+             4: "ALPSG_E_STATE", 5: "ALPSG_E_NOTFOUND"}
+
+# every symbol declared in include/alpsg.h
+EXPORTS = ["alpsg_last_error", "alpsg_create", "alpsg_destroy", "alpsg_set_k",
+           "alpsg_disp_batch", "alpsg_disp_plain", "alpsg_secant_batch", "alpsg_map_search",
+           "alpsg_om_scan", "alpsg_get_nmax", "alpsg_eval_count", "alpsg_last_kernel_ms"]
+
+
+class AlpsgError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{_ERRNAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class _Desc(C.Structure):
+    _fields_ = [
+        ("nspec", C.c_int), ("nperp", C.c_int), ("npar", C.c_int),
+        ("pp", C.c_void_p), ("f0", C.c_void_p), ("df0", C.c_void_p),
+        ("ns", C.c_void_p), ("qs", C.c_void_p), ("ms", C.c_void_p),
+        ("vA", C.c_double), ("kperp_norm", C.c_int), ("positions_principal", C.c_int),
+        ("n_resonance_interval", C.c_int), ("Tlim", C.c_double), ("Bessel_zero", C.c_double),
+        ("usebM", C.c_void_p), ("relativistic", C.c_void_p),
+        ("ACmethod", C.c_void_p), ("n_fits", C.c_void_p), ("max_fits", C.c_int),
+        ("fit_type", C.c_void_p), ("perp_correction", C.c_void_p), ("param_fit", C.c_void_p),
+        ("param_fit_nperp", C.c_int),
+        ("poly_kind", C.c_void_p), ("poly_order", C.c_void_p), ("logfit", C.c_void_p),
+        ("poly_log_max", C.c_void_p), ("poly_fit_coeffs", C.c_void_p),
+        ("poly_max_order", C.c_int),
+    ]
+
+
+class _RootParams(C.Structure):
+    _fields_ = [("secant_method", C.c_int), ("numiter", C.c_int), ("D_threshold", C.c_double),
+                ("D_prec", C.c_double), ("D_gap", C.c_double), ("D_tol", C.c_double)]
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libalpsg.so and declare the C-ABI prototypes (raises if absent)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is not built: run `python -m alps_amd.build`")
+    L = C.CDLL(path)
+    vp, ip, dp = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)
+    L.alpsg_last_error.restype = C.c_char_p
+    L.alpsg_create.argtypes = [C.POINTER(_Desc), C.POINTER(vp)]
+    L.alpsg_destroy.argtypes = [vp]
+    L.alpsg_set_k.argtypes = [vp, C.c_double, C.c_double, ip, ip]
+    L.alpsg_disp_batch.argtypes = [vp, C.c_int, dp, dp, dp, dp, dp]
+    L.alpsg_disp_plain.argtypes = [vp, C.c_int, dp, dp]
+    L.alpsg_secant_batch.argtypes = [vp, C.POINTER(_RootParams), C.c_int, dp, dp, ip]
+    L.alpsg_map_search.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
+                                   C.c_int, C.c_int, C.c_int, dp, dp, dp, C.c_int, ip, ip]
+    L.alpsg_om_scan.argtypes = [vp, C.POINTER(_RootParams), C.c_int, C.c_int, C.c_double,
+                                C.c_double, C.c_int, C.c_int, C.c_int, dp, dp, dp, dp]
+    L.alpsg_get_nmax.argtypes = [vp, ip]
+    L.alpsg_eval_count.argtypes = [vp]
+    L.alpsg_eval_count.restype = C.c_longlong
+    L.alpsg_last_kernel_ms.argtypes = [vp]
+    L.alpsg_last_kernel_ms.restype = C.c_double
+    for name in ["alpsg_create", "alpsg_destroy", "alpsg_set_k", "alpsg_disp_batch",
+                 "alpsg_disp_plain", "alpsg_secant_batch", "alpsg_map_search", "alpsg_om_scan",
+                 "alpsg_get_nmax"]:
+        getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _check(rc: int):
+    if rc != ALPSG_OK:
+        raise AlpsgError(rc, load_library().alpsg_last_error().decode())
+
+
+def _rp(r: RootParams) -> _RootParams:
+    return _RootParams(r.secant_method, r.numiter, r.D_threshold, r.D_prec, r.D_gap, r.D_tol)
+
+
+class Engine:
+    """One device-resident plasma (a C-ABI handle)."""
+
+    def __init__(self, plasma: Plasma, use_df0: bool = False):
+        L = load_library()
+        p = plasma
+        self.plasma = p
+        fo = p.fortran()
+        keep = dict(fo)
+        keep["ns"], keep["qs"], keep["ms"] = p.ns.copy(), p.qs.copy(), p.ms.copy()
+        keep["ACmethod"] = p.ACmethod.astype(np.int32)
+        keep["n_fits"] = p.n_fits.astype(np.int32)
+        keep["poly_kind"] = p.poly_kind.astype(np.int32)
+        keep["poly_order"] = p.poly_order.astype(np.int32)
+        keep["logfit"] = p.logfit.astype(np.int32)
+        keep["poly_log_max"] = p.poly_log_max.astype(np.float64)
+        keep["fit_type_f"] = np.asfortranarray(p.fit_type.astype(np.int32))
+        if use_df0:
+            df0 = np.zeros((p.nspec, p.nperp - 1, p.npar - 1, 2))
+            pp, f0 = p.pp, p.f0
+            df0[..., 0] = ((f0[:, 2:, 1:-1] - f0[:, :-2, 1:-1]) /
+                           (pp[:, 2:, 1:-1, 0] - pp[:, :-2, 1:-1, 0]))
+            df0[..., 1] = ((f0[:, 1:-1, 2:] - f0[:, 1:-1, :-2]) /
+                           (pp[:, 1:-1, 2:, 1] - pp[:, 1:-1, :-2, 1]))
+            keep["df0"] = np.asfortranarray(df0)
+        self._keep = keep
+        d = _Desc()
+        d.nspec, d.nperp, d.npar = p.nspec, p.nperp, p.npar
+        d.pp = keep["pp"].ctypes.data
+        d.f0 = None if use_df0 else keep["f0"].ctypes.data
+        d.df0 = keep["df0"].ctypes.data if use_df0 else None
+        d.ns, d.qs, d.ms = keep["ns"].ctypes.data, keep["qs"].ctypes.data, keep["ms"].ctypes.data
+        d.vA = p.vA
+        d.kperp_norm = int(bool(p.kperp_norm))
+        d.positions_principal = int(p.positions_principal)
+        d.n_resonance_interval = int(p.n_resonance_interval)
+        d.Tlim, d.Bessel_zero = p.Tlim, p.Bessel_zero
+        d.usebM = d.relativistic = None
+        d.ACmethod, d.n_fits = keep["ACmethod"].ctypes.data, keep["n_fits"].ctypes.data
+        d.max_fits = p.max_fits
+        d.fit_type = keep["fit_type_f"].ctypes.data
+        d.perp_correction = keep["perp_correction"].ctypes.data
+        d.param_fit = keep["param_fit"].ctypes.data
+        d.param_fit_nperp = p.param_fit.shape[1] - 1
+        d.poly_kind, d.poly_order = keep["poly_kind"].ctypes.data, keep["poly_order"].ctypes.data
+        d.logfit, d.poly_log_max = keep["logfit"].ctypes.data, keep["poly_log_max"].ctypes.data
+        d.poly_fit_coeffs = keep["poly_fit_coeffs"].ctypes.data
+        d.poly_max_order = p.poly_fit_coeffs.shape[2] - 1
+        h = C.c_void_p()
+        _check(L.alpsg_create(C.byref(d), C.byref(h)))
+        self._h = h
+        self.kperp = self.kpar = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().alpsg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- wavevector -------------------------------------------------------
+    def set_k(self, kperp: float, kpar: float, nmax=None):
+        ns = self.plasma.nspec
+        out = np.zeros(ns, dtype=np.int32)
+        nin = None if nmax is None else np.asarray(nmax, dtype=np.int32)
+        _check(load_library().alpsg_set_k(self._h, kperp, kpar, _ip(nin), _ip(out)))
+        self.kperp, self.kpar = kperp, kpar
+        return [int(x) for x in out]
+
+    @property
+    def nmax(self):
+        out = np.zeros(self.plasma.nspec, dtype=np.int32)
+        _check(load_library().alpsg_get_nmax(self._h, _ip(out)))
+        return [int(x) for x in out]
+
+    @property
+    def eval_count(self) -> int:
+        return int(load_library().alpsg_eval_count(self._h))
+
+    @property
+    def last_kernel_ms(self) -> float:
+        return float(load_library().alpsg_last_kernel_ms(self._h))
+
+    # -- disp ---------------------------------------------------------------
+    def disp_batch(self, om, want_tensors: bool = False):
+        om = np.ascontiguousarray(np.asarray(om, dtype=np.complex128).ravel())
+        n = om.size
+        det = np.zeros(n, dtype=np.complex128)
+        ns = self.plasma.nspec
+        chi = low = wave = None
+        if want_tensors:
+            chi = np.zeros(n * ns * 9, dtype=np.complex128)
+            low = np.zeros(n * ns * 27, dtype=np.complex128)
+            wave = np.zeros(n * 9, dtype=np.complex128)
+        _check(load_library().alpsg_disp_batch(self._h, n, _dp(om), _dp(det), _dp(chi), _dp(low),
+                                              _dp(wave)))
+        if not want_tensors:
+            return det
+        # Fortran layouts -> [n][s][i][j], [n][s][i][j][q], [n][i][j]
+        chi = chi.reshape(n, 3, 3, ns).transpose(0, 3, 2, 1)
+        low = low.reshape(n, 3, 3, 3, ns).transpose(0, 4, 3, 2, 1)
+        wave = wave.reshape(n, 3, 3).transpose(0, 2, 1)
+        return det, chi, low, wave
+
+    def disp(self, om):
+        """det D at one complex frequency (ALPS_fns.f90:252)."""
+        return complex(self.disp_batch([om])[0])
+
+    def disp_plain(self, om):
+        om = np.ascontiguousarray(np.asarray(om, dtype=np.complex128).ravel())
+        det = np.zeros(om.size, dtype=np.complex128)
+        _check(load_library().alpsg_disp_plain(self._h, om.size, _dp(om), _dp(det)))
+        return det
+
+    # -- root searches --------------------------------------------------------
+    def secant_osc_batch(self, guesses, roots: RootParams | None = None):
+        r = roots or self.plasma.roots
+        om = np.ascontiguousarray(np.asarray(guesses, dtype=np.complex128).ravel()).copy()
+        dfin = np.zeros_like(om)
+        it = np.zeros(om.size, dtype=np.int32)
+        rp = _rp(r)
+        _check(load_library().alpsg_secant_batch(self._h, C.byref(rp), om.size, _dp(om), _dp(dfin),
+                                                _ip(it)))
+        return om, dfin, it
+
+    def secant_osc(self, om, roots: RootParams | None = None):
+        o, d, it = self.secant_osc_batch([om], roots)
+        return complex(o[0]), int(it[0])
+
+    def refine_guess(self, guesses, roots: RootParams | None = None):
+        """refine_guess (ALPS_fns.f90:3793-3856): secant on each guess, then det D there."""
+        om, dfin, _ = self.secant_osc_batch(guesses, roots)
+        return om, dfin
+
+    def map_search(self, omi, omf, gami, gamf, nr, ni, loggridw=False, loggridg=False,
+                   numroots=100):
+        """det D on the nr x ni grid plus the local minima (ALPS_fns.f90:3595-3788).
+        Returns om[nr, ni], cal[nr, ni], val[nr, ni], iroots [(ir, ii)], nfound."""
+        N = nr * ni
+        om = np.zeros(N, dtype=np.complex128)
+        cal = np.zeros(N, dtype=np.complex128)
+        val = np.zeros(N)
+        iroots = np.zeros(2 * numroots, dtype=np.int32)
+        nf = np.zeros(1, dtype=np.int32)
+        _check(load_library().alpsg_map_search(self._h, omi, omf, gami, gamf, nr, ni,
+                                              int(loggridw), int(loggridg), _dp(om), _dp(cal),
+                                              _dp(val), numroots, _ip(iroots), _ip(nf)))
+        n = int(nf[0])
+        f = lambda a: a.reshape(ni, nr).T  # Fortran order (ir fastest)
+        return f(om), f(cal), f(val), iroots.reshape(numroots, 2)[: min(n, numroots)], n
+
+    def om_scan(self, scan_type, log_scan, diff, diff2, n_out, n_res, roots,
+                rparams: RootParams | None = None):
+        r = rparams or self.plasma.roots
+        roots = np.ascontiguousarray(np.asarray(roots, dtype=np.complex128).ravel()).copy()
+        nr = roots.size
+        kp = np.zeros(n_out + 1)
+        kz = np.zeros(n_out + 1)
+        out = np.zeros((n_out + 1) * nr, dtype=np.complex128)
+        rp = _rp(r)
+        _check(load_library().alpsg_om_scan(self._h, C.byref(rp), scan_type, int(log_scan), diff,
+                                           diff2, n_out, n_res, nr, _dp(roots), _dp(kp), _dp(kz),
+                                           _dp(out)))
+        return kp, kz, out.reshape(n_out + 1, nr)

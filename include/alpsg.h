@@ -1,0 +1,141 @@
+/*
+ * alpsg.h -- C-ABI of the MI355X (gfx950) dispersion-tensor engine.
+ *
+ * The engine replaces the harmonic loop of ALPS's disp() and the root
+ * searches that call it.  Every entry point is plain C: pointers, sizes and
+ * an int status (0 = ok, never aborts).  Complex numbers are interleaved
+ * (re, im) doubles, i.e. Fortran complex(c_double_complex) / C double _Complex.
+ * Multi-dimensional arrays use the Fortran (column-major) layout of the
+ * reference's own arrays, so a Fortran caller passes its alps_var arrays
+ * unchanged.  One handle owns its device copies and one HIP stream; calls on
+ * one handle are serialised by the caller.
+ */
+#ifndef ALPSG_H
+#define ALPSG_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ALPSG_OK 0
+#define ALPSG_E_ARG 1         /* invalid argument / shape */
+#define ALPSG_E_UNSUPPORTED 2 /* feature outside the engine (usebM, relativistic, ...) */
+#define ALPSG_E_DEVICE 3      /* HIP runtime error */
+#define ALPSG_E_STATE 4       /* call order (e.g. disp before set_k) */
+#define ALPSG_E_NOTFOUND 5    /* no root survived a scan (reference alps_error(9)) */
+
+#define ALPSG_MAXSPEC 8
+#define ALPSG_MAXFITS 4
+
+typedef struct alpsg_ctx *alpsg_handle;
+
+/* Plasma description: the alps_var globals disp() reads (ALPS_var.f90:19-427).
+   Array layouts are the Fortran ones:
+     pp(1:nspec,0:nperp,0:npar,1:2)        ALPS_com.f90:202
+     f0(1:nspec,0:nperp,0:npar)            ALPS.f90:72
+     df0(1:nspec,1:nperp-1,1:npar-1,1:2)   ALPS_com.f90:200   (used when f0 == NULL)
+     fit_type(1:nspec,1:max_fits), perp_correction(1:nspec,1:max_fits)
+     param_fit(1:nspec,0:param_fit_nperp,1:5,1:max_fits)  ALPS_io.f90:188
+     poly_fit_coeffs(1:nspec,0:nperp,0:poly_max_order)    ALPS_io.f90:270  */
+typedef struct alpsg_plasma_desc {
+  int nspec, nperp, npar;
+  const double *pp;
+  const double *f0;  /* may be NULL if df0 is given */
+  const double *df0; /* may be NULL if f0 is given */
+  const double *ns, *qs, *ms; /* [nspec] */
+  double vA;
+  int kperp_norm;
+  int positions_principal;
+  int n_resonance_interval;
+  double Tlim;
+  double Bessel_zero;
+  const int *usebM;        /* [nspec] or NULL; nonzero -> ALPSG_E_UNSUPPORTED */
+  const int *relativistic; /* [nspec] or NULL; nonzero -> ALPSG_E_UNSUPPORTED */
+  const int *ACmethod;     /* [nspec]: 1 fits, 2 Chebyshev */
+  const int *n_fits;       /* [nspec] */
+  int max_fits;
+  const int *fit_type;
+  const double *perp_correction;
+  const double *param_fit;
+  int param_fit_nperp; /* upper bound of param_fit's 2nd dimension, max(nperp,ngamma) */
+  const int *poly_kind, *poly_order, *logfit; /* [nspec] (may be NULL if no ACmethod 2) */
+  const double *poly_log_max;                 /* [nspec] */
+  const double *poly_fit_coeffs;
+  int poly_max_order;
+} alpsg_plasma_desc;
+
+/* Root-search controls (&system namelist, ALPS_var.f90:147-165). */
+typedef struct alpsg_root_params {
+  int secant_method; /* 0 secant, 1 rtsec, 2 secant_osc (only 2 is batched on the device) */
+  int numiter;
+  double D_threshold, D_prec, D_gap, D_tol;
+} alpsg_root_params;
+
+/* Error text of the last failing call on this thread. */
+const char *alpsg_last_error(void);
+
+/* Create a handle: uploads pp and f0 (or df0), builds df0 on the device with
+   the central differences of derivative_f0 (ALPS_fns.f90:103-115), the
+   omega-independent int_ee term (ALPS_fns.f90:1457-1555) and the fit tables.
+   Replaces the state set up by ALPS.f90:70-89 for the disp() path. */
+int alpsg_create(const alpsg_plasma_desc *desc, alpsg_handle *out);
+
+int alpsg_destroy(alpsg_handle h);
+
+/* Set the wavevector: determine_nmax (ALPS_fns.f90:3971-4075) unless nmax_in
+   is non-NULL, then the Bessel table (determine_bessel_array,
+   ALPS_fns.f90:4214-4255) and the pperp moment tables, all kept in HBM.
+   nmax_out [nspec] may be NULL. */
+int alpsg_set_k(alpsg_handle h, double kperp, double kpar, const int *nmax_in, int *nmax_out);
+
+/* Batched disp(om) (ALPS_fns.f90:252-636): det D for n omegas.
+   om, det: [2n].  Optional outputs (NULL to skip), per omega in the Fortran
+   layout of the reference globals:
+     chi0     (1:nspec,1:3,1:3)        -> [n][nspec*9*2]
+     chi0_low (1:nspec,1:3,1:3,-1:1)   -> [n][nspec*27*2]
+     wave     (1:3,1:3)                -> [n][9*2]                      */
+int alpsg_disp_batch(alpsg_handle h, int n, const double *om, double *det, double *chi0,
+                     double *chi0_low, double *wave);
+
+/* The same evaluation with one launch per omega (the plain path). */
+int alpsg_disp_plain(alpsg_handle h, int n, const double *om, double *det);
+
+/* Batched secant_osc (ALPS_fns.f90:1919-2101) for nroots independent roots at
+   the handle's k: om [2*nroots] in/out.  D_final [2*nroots] (optional) gets
+   disp at the returned root (the "final instance of disp", ALPS_fns.f90:2523,
+   3847); iters [nroots] (optional) the iteration count. */
+int alpsg_secant_batch(alpsg_handle h, const alpsg_root_params *rp, int nroots, double *om,
+                       double *D_final, int *iters);
+
+/* map_search (ALPS_fns.f90:3595-3788): det D on the nr x ni omega grid in one
+   launch, log10|D| with the reference's NaN/Inf sentinels, local minima
+   (find_minima, ALPS_fns.f90:3860-3966, at most numroots recorded).
+   om_out, cal_out: [2*nr*ni] and val_out [nr*ni] in Fortran order (ir fastest)
+   (each may be NULL); iroots [2*numroots] gets (ir, ii) 1-based; *nfound the
+   number of minima (may exceed numroots, as counted by the reference). */
+int alpsg_map_search(alpsg_handle h, double omi, double omf, double gami, double gamf, int nr,
+                     int ni, int loggridw, int loggridg, double *om_out, double *cal_out,
+                     double *val_out, int numroots, int *iroots, int *nfound);
+
+/* om_scan (ALPS_fns.f90:2198-2600) without eigenfunction/heating output:
+   follows nroots roots along the wavevector path; diff/diff2 are the step
+   sizes computed by scan_read (ALPS_io.f90:482-548).  roots [2*nroots] in/out.
+   Outputs have n_out+1 rows (row 0 = the starting point):
+   kperp_out, kpar_out [n_out+1]; roots_out [(n_out+1)*nroots*2]. */
+int alpsg_om_scan(alpsg_handle h, const alpsg_root_params *rp, int scan_type, int log_scan,
+                  double diff, double diff2, int n_out, int n_res, int nroots, double *roots,
+                  double *kperp_out, double *kpar_out, double *roots_out);
+
+/* Diagnostics: nmax per species at the current k and the number of disp
+   evaluations done by this handle. */
+int alpsg_get_nmax(alpsg_handle h, int *nmax_out);
+long long alpsg_eval_count(alpsg_handle h);
+
+/* Device time (ms) of the last alpsg_disp_batch / alpsg_map_search kernel
+   sweep, measured with HIP events on the handle's stream. */
+double alpsg_last_kernel_ms(alpsg_handle h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
