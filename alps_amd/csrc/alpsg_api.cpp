@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <complex>
 #include <memory>
@@ -104,6 +105,7 @@ struct alpsg_ctx {
   DevBuf<double2> om_d, det_d, chi_d, low_d, wave_d;
   DevBuf<int> act_d;
   int fixed_nch = 0;  // 0 = heuristic / tuned
+  std::vector<double> root_ms;  // completion time of each root of the last secant batch
 
   ~alpsg_ctx() {
     if (st) (void)hipStreamSynchronize(st);
@@ -202,6 +204,8 @@ int secant_osc_batch(alpsg_ctx *h, const alpsg_root_params *rp, int nroots, cplx
   std::vector<OscState> S(nroots);
   std::vector<cplx> req(nroots), res(nroots);
   std::vector<int> who;
+  const auto t0 = std::chrono::steady_clock::now();
+  h->root_ms.assign(nroots, 0.0);
   // phases: 0 = need D(om) initial, 1 = need D(prevom) initial, 2 = need D(om) iteration,
   //         3 = need D(prevom+1e-8), 4 = need D(om*(1+delta)), 5 = need D(om*(1-delta))
   for (int r = 0; r < nroots; ++r) {
@@ -282,9 +286,12 @@ int secant_osc_batch(alpsg_ctx *h, const alpsg_root_params *rp, int nroots, cplx
     int rc = run_disp(h, (int)who.size(), reinterpret_cast<const double *>(req.data()),
                       reinterpret_cast<double *>(res.data()), nullptr, nullptr, nullptr);
     if (rc) return rc;
+    const double now_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     for (size_t t = 0; t < who.size(); ++t) {
       OscState &s = S[who[t]];
       cplx D = res[t];
+      h->root_ms[who[t]] = now_ms;
       switch (s.phase) {
         case 0:
           s.D = D;
@@ -650,6 +657,13 @@ int alpsg_get_nmax(alpsg_handle h, int *nmax_out) {
 }
 
 long long alpsg_eval_count(alpsg_handle h) { return h ? h->evals : -1; }
+
+int alpsg_last_root_times(alpsg_handle h, int nroots, double *ms_out) {
+  if (!h || !ms_out) return fail(ALPSG_E_ARG, "null argument");
+  if (nroots != (int)h->root_ms.size()) return fail(ALPSG_E_ARG, "nroots differs from the last secant batch");
+  for (int r = 0; r < nroots; ++r) ms_out[r] = h->root_ms[r];
+  return ALPSG_OK;
+}
 
 double alpsg_last_kernel_ms(alpsg_handle h) { return h ? h->last_ms : -1.0; }
 

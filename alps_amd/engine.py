@@ -37,7 +37,8 @@ _ERRNAMES = {1: "ALPSG_E_ARG", 2: "ALPSG_E_UNSUPPORTED", 3: "ALPSG_E_DEVICE",
 # every symbol declared in include/alpsg.h
 EXPORTS = ["alpsg_last_error", "alpsg_create", "alpsg_destroy", "alpsg_set_k",
            "alpsg_disp_batch", "alpsg_disp_plain", "alpsg_secant_batch", "alpsg_map_search",
-           "alpsg_om_scan", "alpsg_get_nmax", "alpsg_eval_count", "alpsg_last_kernel_ms"]
+           "alpsg_om_scan", "alpsg_get_nmax", "alpsg_eval_count", "alpsg_last_kernel_ms",
+           "alpsg_last_root_times"]
 
 
 class AlpsgError(RuntimeError):
@@ -94,6 +95,8 @@ def load_library(path: str = LIB_PATH):
     L.alpsg_get_nmax.argtypes = [vp, ip]
     L.alpsg_eval_count.argtypes = [vp]
     L.alpsg_eval_count.restype = C.c_longlong
+    L.alpsg_last_root_times.argtypes = [vp, C.c_int, dp]
+    L.alpsg_last_root_times.restype = C.c_int
     L.alpsg_last_kernel_ms.argtypes = [vp]
     L.alpsg_last_kernel_ms.restype = C.c_double
     for name in ["alpsg_create", "alpsg_destroy", "alpsg_set_k", "alpsg_disp_batch",
@@ -253,6 +256,12 @@ class Engine:
     def secant_osc(self, om, roots: RootParams | None = None):
         o, d, it = self.secant_osc_batch([om], roots)
         return complex(o[0]), int(it[0])
+
+    def last_root_times_ms(self, nroots: int):
+        """Completion time of each root of the last secant batch (ms from its start)."""
+        out = np.zeros(nroots)
+        _check(load_library().alpsg_last_root_times(self._h, nroots, _dp(out)))
+        return out
 
     def refine_guess(self, guesses, roots: RootParams | None = None):
         """refine_guess (ALPS_fns.f90:3793-3856): secant on each guess, then det D there."""

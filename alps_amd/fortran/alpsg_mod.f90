@@ -1,0 +1,190 @@
+!! ISO_C_BINDING interface of the gfx950 dispersion engine (include/alpsg.h)
+!! for the ALPS Fortran host.  The module mirrors the reference's call
+!! surface: alpsg_disp(om) stands in for disp(om) (ALPS_fns.f90:252),
+!! alpsg_secant_osc for secant_osc (ALPS_fns.f90:1919), alpsg_map for
+!! map_search (ALPS_fns.f90:3595).  Arrays are passed exactly as alps_var
+!! holds them (column-major, species index fastest).
+module alpsg
+  use iso_c_binding
+  implicit none
+  private
+
+  integer(c_int), parameter, public :: ALPSG_OK = 0, ALPSG_E_ARG = 1, &
+       ALPSG_E_UNSUPPORTED = 2, ALPSG_E_DEVICE = 3, ALPSG_E_STATE = 4, ALPSG_E_NOTFOUND = 5
+
+  type, bind(C), public :: alpsg_plasma_desc
+     integer(c_int) :: nspec, nperp, npar
+     type(c_ptr) :: pp = c_null_ptr, f0 = c_null_ptr, df0 = c_null_ptr
+     type(c_ptr) :: ns = c_null_ptr, qs = c_null_ptr, ms = c_null_ptr
+     real(c_double) :: vA
+     integer(c_int) :: kperp_norm, positions_principal, n_resonance_interval
+     real(c_double) :: Tlim, Bessel_zero
+     type(c_ptr) :: usebM = c_null_ptr, relativistic = c_null_ptr
+     type(c_ptr) :: ACmethod = c_null_ptr, n_fits = c_null_ptr
+     integer(c_int) :: max_fits
+     type(c_ptr) :: fit_type = c_null_ptr, perp_correction = c_null_ptr, param_fit = c_null_ptr
+     integer(c_int) :: param_fit_nperp
+     type(c_ptr) :: poly_kind = c_null_ptr, poly_order = c_null_ptr, logfit = c_null_ptr
+     type(c_ptr) :: poly_log_max = c_null_ptr, poly_fit_coeffs = c_null_ptr
+     integer(c_int) :: poly_max_order
+  end type alpsg_plasma_desc
+
+  type, bind(C), public :: alpsg_root_params
+     integer(c_int) :: secant_method, numiter
+     real(c_double) :: D_threshold, D_prec, D_gap, D_tol
+  end type alpsg_root_params
+
+  public :: alpsg_create, alpsg_destroy, alpsg_set_k, alpsg_disp_batch, alpsg_disp_plain
+  public :: alpsg_secant_batch, alpsg_map_search, alpsg_om_scan, alpsg_get_nmax
+  public :: alpsg_eval_count, alpsg_error_text, alpsg_disp, alpsg_secant_osc
+
+  interface
+     function alpsg_create(desc, h) bind(C, name='alpsg_create') result(rc)
+       import :: c_int, c_ptr, alpsg_plasma_desc
+       type(alpsg_plasma_desc), intent(in) :: desc
+       type(c_ptr), intent(out) :: h
+       integer(c_int) :: rc
+     end function alpsg_create
+
+     function alpsg_destroy(h) bind(C, name='alpsg_destroy') result(rc)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+       integer(c_int) :: rc
+     end function alpsg_destroy
+
+     function alpsg_set_k(h, kperp, kpar, nmax_in, nmax_out) bind(C, name='alpsg_set_k') result(rc)
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), value :: kperp, kpar
+       type(c_ptr), value :: nmax_in
+       integer(c_int), intent(out) :: nmax_out(*)
+       integer(c_int) :: rc
+     end function alpsg_set_k
+
+     function alpsg_disp_batch(h, n, om, det, chi0, chi0_low, wave) &
+          bind(C, name='alpsg_disp_batch') result(rc)
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: h
+       integer(c_int), value :: n
+       complex(c_double_complex), intent(in) :: om(*)
+       complex(c_double_complex), intent(out) :: det(*)
+       type(c_ptr), value :: chi0, chi0_low, wave
+       integer(c_int) :: rc
+     end function alpsg_disp_batch
+
+     function alpsg_disp_plain(h, n, om, det) bind(C, name='alpsg_disp_plain') result(rc)
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: h
+       integer(c_int), value :: n
+       complex(c_double_complex), intent(in) :: om(*)
+       complex(c_double_complex), intent(out) :: det(*)
+       integer(c_int) :: rc
+     end function alpsg_disp_plain
+
+     function alpsg_secant_batch(h, rp, nroots, om, D_final, iters) &
+          bind(C, name='alpsg_secant_batch') result(rc)
+       import :: c_int, c_ptr, c_double_complex, alpsg_root_params
+       type(c_ptr), value :: h
+       type(alpsg_root_params), intent(in) :: rp
+       integer(c_int), value :: nroots
+       complex(c_double_complex), intent(inout) :: om(*)
+       complex(c_double_complex), intent(out) :: D_final(*)
+       integer(c_int), intent(out) :: iters(*)
+       integer(c_int) :: rc
+     end function alpsg_secant_batch
+
+     function alpsg_map_search(h, omi, omf, gami, gamf, nr, ni, loggridw, loggridg, om_out, &
+          cal_out, val_out, numroots, iroots, nfound) bind(C, name='alpsg_map_search') result(rc)
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), value :: omi, omf, gami, gamf
+       integer(c_int), value :: nr, ni, loggridw, loggridg, numroots
+       type(c_ptr), value :: om_out, cal_out, val_out
+       integer(c_int), intent(out) :: iroots(2, *)
+       integer(c_int), intent(out) :: nfound
+       integer(c_int) :: rc
+     end function alpsg_map_search
+
+     function alpsg_om_scan(h, rp, scan_type, log_scan, diff, diff2, n_out, n_res, nroots, roots, &
+          kperp_out, kpar_out, roots_out) bind(C, name='alpsg_om_scan') result(rc)
+       import :: c_int, c_ptr, c_double, c_double_complex, alpsg_root_params
+       type(c_ptr), value :: h
+       type(alpsg_root_params), intent(in) :: rp
+       integer(c_int), value :: scan_type, log_scan, n_out, n_res, nroots
+       real(c_double), value :: diff, diff2
+       complex(c_double_complex), intent(inout) :: roots(*)
+       real(c_double), intent(out) :: kperp_out(*), kpar_out(*)
+       complex(c_double_complex), intent(out) :: roots_out(*)
+       integer(c_int) :: rc
+     end function alpsg_om_scan
+
+     function alpsg_get_nmax(h, nmax_out) bind(C, name='alpsg_get_nmax') result(rc)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+       integer(c_int), intent(out) :: nmax_out(*)
+       integer(c_int) :: rc
+     end function alpsg_get_nmax
+
+     function alpsg_eval_count(h) bind(C, name='alpsg_eval_count') result(n)
+       import :: c_ptr, c_long_long
+       type(c_ptr), value :: h
+       integer(c_long_long) :: n
+     end function alpsg_eval_count
+
+     function alpsg_last_error_c() bind(C, name='alpsg_last_error') result(p)
+       import :: c_ptr
+       type(c_ptr) :: p
+     end function alpsg_last_error_c
+  end interface
+
+contains
+
+  !> Text of the last error (alpsg_last_error) as a Fortran string.
+  function alpsg_error_text() result(msg)
+    character(len=256) :: msg
+    character(kind=c_char), pointer :: s(:)
+    type(c_ptr) :: p
+    integer :: i
+    msg = ' '
+    p = alpsg_last_error_c()
+    if (.not. c_associated(p)) return
+    call c_f_pointer(p, s, [256])
+    do i = 1, 256
+       if (s(i) == c_null_char) exit
+       msg(i:i) = s(i)
+    enddo
+  end function alpsg_error_text
+
+  !> det D at one frequency: the drop-in body of disp(om) (ALPS_fns.f90:252).
+  !! chi0, chi0_low and wave (alps_var side effects) are filled when present.
+  function alpsg_disp(h, om, rc, chi0, chi0_low, wave) result(det)
+    type(c_ptr), intent(in) :: h
+    complex(c_double_complex), intent(in) :: om
+    integer(c_int), intent(out) :: rc
+    complex(c_double_complex), intent(out), target, optional :: chi0(:,:,:), chi0_low(:,:,:,:), wave(:,:)
+    complex(c_double_complex) :: det
+    complex(c_double_complex) :: o(1), d(1)
+    type(c_ptr) :: pc, pl, pw
+    pc = c_null_ptr; pl = c_null_ptr; pw = c_null_ptr
+    if (present(chi0)) pc = c_loc(chi0)
+    if (present(chi0_low)) pl = c_loc(chi0_low)
+    if (present(wave)) pw = c_loc(wave)
+    o(1) = om
+    rc = alpsg_disp_batch(h, 1_c_int, o, d, pc, pl, pw)
+    det = d(1)
+  end function alpsg_disp
+
+  !> secant_osc(om, in) (ALPS_fns.f90:1919) through the batched device solver.
+  subroutine alpsg_secant_osc(h, rp, om, rc)
+    type(c_ptr), intent(in) :: h
+    type(alpsg_root_params), intent(in) :: rp
+    complex(c_double_complex), intent(inout) :: om
+    integer(c_int), intent(out) :: rc
+    complex(c_double_complex) :: o(1), d(1)
+    integer(c_int) :: it(1)
+    o(1) = om
+    rc = alpsg_secant_batch(h, rp, 1_c_int, o, d, it)
+    om = o(1)
+  end subroutine alpsg_secant_osc
+
+end module alpsg

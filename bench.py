@@ -1,0 +1,186 @@
+"""Benchmark of the batched map search + root refinement on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): 2-species
+bi-Maxwellian plasma (protons, electrons with m_p/m_e = 1836, T_e = T_p,
+beta_par = 1, v_A/c = 1e-4), f0 tabulated in FP64 on nperp x npar = 128 x 256,
+kperp d_p = kpar d_p = 0.5.  One step = det D on the 256 x 256 complex-omega
+map (Re w in [0.01, 2], Im w in [-0.5, 0.05]) in one fused launch, local
+minima (at most 100), batched secant_osc refinement of every minimum
+(|det D| < 1e-12) and det D at every refined root.
+
+Prints one JSON line.  value = chi-tensor (det D) evaluations per second over
+the timed steps; roots/s and p50/p99 root latency (from the start of the step
+to the root's convergence) are reported beside it.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("roots/sec and p50/p99 root latency at |Δω/ω|≤1e-10; "
+          "chi-tensor evals/sec")
+FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix) dense peak
+HBM_PEAK_GBS = 8000.0
+
+M_E = 5.44662e-4
+
+
+def make_plasma(nperp=128, npar=256):
+    from alps_amd.plasma import RootParams, bimaxwellian_plasma
+    rp = RootParams(secant_method=2, numiter=50, D_threshold=1e-12, D_prec=1e-5, D_gap=1e-5)
+    return bimaxwellian_plasma(ms=[1.0, M_E], qs=[1.0, -1.0], ns=[1.0, 1.0], taus=[1.0, 1.0],
+                               alphs=[1.0, 1.0], ps=[0.0, 0.0], beta=1.0, vA=1e-4,
+                               nperp=nperp, npar=npar, maxP=8.0, positions_principal=3,
+                               n_resonance_interval=100, Tlim=0.01, Bessel_zero=1e-45, roots=rp)
+
+
+MAP = dict(omi=0.01, omf=2.0, gami=-0.5, gamf=0.05)
+KPERP = KPAR = 0.5
+
+
+def flops_per_eval(nmax, npar):
+    """Algorithmic FP64 flops of one non-relativistic det D evaluation:
+    54 per (species, harmonic, ppar node) of the resonant-denominator sweep."""
+    return 54.0 * sum((2 * n + 1) * (npar - 1) for n in nmax)
+
+
+def run_step(E, nr, ni, numroots=100):
+    t0 = time.perf_counter()
+    om, cal, val, iroots, nfound = E.map_search(MAP["omi"], MAP["omf"], MAP["gami"], MAP["gamf"],
+                                                nr, ni, numroots=numroots)
+    map_kernel_ms = E.last_kernel_ms
+    t_map = (time.perf_counter() - t0) * 1e3
+    guesses = np.array([om[i - 1, j - 1] for i, j in iroots])
+    roots, dfin = E.refine_guess(guesses) if guesses.size else (guesses, guesses)
+    lat = t_map + (E.last_root_times_ms(guesses.size) if guesses.size else np.zeros(0))
+    return dict(roots=roots, dfin=dfin, guesses=guesses, latency_ms=lat, map_kernel_ms=map_kernel_ms,
+                cal=cal, om=om, nfound=nfound)
+
+
+def cpu_baseline(plasma, om_grid, cal_gpu, n_sample, threads):
+    """The CPU restatement (oracle/) on a strided sample of the same map."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle
+    O = Oracle(plasma, n_threads=threads)
+    O.set_k(KPERP, KPAR)
+    flat_om = om_grid.T.ravel()
+    flat_cal = cal_gpu.T.ravel()
+    idx = np.linspace(0, flat_om.size - 1, n_sample).astype(int)
+    t0 = time.perf_counter()
+    vals = [O.disp(flat_om[k], want_tensors=True) for k in idx]
+    dt = time.perf_counter() - t0
+    err = 0.0
+    from_scale = []
+    for (d, chi, low, wave), k in zip(vals, idx):
+        w = np.abs(wave)
+        sc = (w[0, 0] * (w[1, 1] * w[2, 2] + w[1, 2] ** 2) + 2 * w[0, 1] * w[1, 2] * w[0, 2]
+              + w[0, 2] ** 2 * w[1, 1] + w[0, 1] ** 2 * w[2, 2])
+        err = max(err, abs(flat_cal[k] - d) / sc)
+    return dict(value=n_sample / dt, unit="chi-tensor evals/s", cores=threads, kind="port",
+                sample=f"{n_sample} strided points of the 256x256 map, full 128x256 grid, "
+                       f"oracle/liboracle.so with {threads} OpenMP threads",
+                wall_s=dt), err
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nr", type=int, default=256)
+    ap.add_argument("--ni", type=int, default=256)
+    ap.add_argument("--cpu-sample", type=int, default=64)
+    ap.add_argument("--cpu-threads", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--plain", type=int, default=32, help="omegas timed on the plain path")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the omega/request axis shards across GPUs with no communication, "
+              "see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from alps_amd.engine import Engine
+    p = make_plasma()
+    E = Engine(p)
+    nmax = E.set_k(KPERP, KPAR)
+    for _ in range(args.warmup):
+        run_step(E, args.nr, args.ni)
+
+    ev0 = E.eval_count
+    res = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res.append(run_step(E, args.nr, args.ni))
+    t1 = time.perf_counter()
+    evals = E.eval_count - ev0
+    wall = t1 - t0
+    ms_step = wall * 1e3 / args.steps
+    n_roots = sum(r["roots"].size for r in res)
+    lat = np.concatenate([r["latency_ms"] for r in res]) if n_roots else np.zeros(1)
+    kern = np.array([r["map_kernel_ms"] for r in res])
+    last = res[-1]
+    conv = int(np.sum(np.abs(last["dfin"]) < p.roots.D_threshold)) if last["roots"].size else 0
+
+    # roofline of the dominant kernel (the map's fused disp launch)
+    f_eval = flops_per_eval(nmax, p.npar)
+    flops_launch = f_eval * args.nr * args.ni
+    achieved = flops_launch / (kern.mean() * 1e-3) / 1e12
+
+    # plain path: one omega per launch
+    plain_om = last["om"].ravel()[:: max(1, last["om"].size // args.plain)][: args.plain]
+    E.disp_plain(plain_om[:2])
+    tp = time.perf_counter()
+    E.disp_plain(plain_om)
+    plain_rate = plain_om.size / (time.perf_counter() - tp)
+
+    out = {
+        "metric": METRIC,
+        "value": evals / wall,
+        "unit": "chi-tensor evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (bi-Maxwellian f0 tables from the generator formulas)",
+        "config": {"workload": "map_search 256x256 omega map + secant refinement of every "
+                               "minimum, 2 species, nperp x npar = 128 x 256, "
+                               "kperp d_p = kpar d_p = 0.5",
+                   "nmax": nmax, "map": [args.nr, args.ni], "omega_re": [MAP["omi"], MAP["omf"]],
+                   "omega_im": [MAP["gami"], MAP["gamf"]], "D_threshold": p.roots.D_threshold},
+        "roots_per_sec": n_roots / wall,
+        "roots_per_step": n_roots / args.steps,
+        "roots_converged_last_step": conv,
+        "root_latency_p50_ms": float(np.percentile(lat, 50)),
+        "root_latency_p99_ms": float(np.percentile(lat, 99)),
+        "plain_evals_per_sec": plain_rate,
+        "map_kernel_ms": float(kern.mean()),
+        "roofline": {"bound": "mfma", "compute": "fp64 (VALU)", "achieved": achieved,
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "flops_per_launch": flops_launch,
+                     "kernel": "k_disp (map launch, %d omegas)" % (args.nr * args.ni)},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        cb, err = cpu_baseline(p, last["om"], last["cal"], args.cpu_sample,
+                               min(args.cpu_threads, os.cpu_count() or 1))
+        out["cpu_baseline"] = cb
+        out["parity_vs_oracle_on_sample"] = err
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

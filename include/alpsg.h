@@ -131,6 +131,10 @@ int alpsg_om_scan(alpsg_handle h, const alpsg_root_params *rp, int scan_type, in
 int alpsg_get_nmax(alpsg_handle h, int *nmax_out);
 long long alpsg_eval_count(alpsg_handle h);
 
+/* Wall time (ms, from the start of the last alpsg_secant_batch on this handle)
+   at which each of its nroots roots finished its secant iterations. */
+int alpsg_last_root_times(alpsg_handle h, int nroots, double *ms_out);
+
 /* Device time (ms) of the last alpsg_disp_batch / alpsg_map_search kernel
    sweep, measured with HIP events on the handle's stream. */
 double alpsg_last_kernel_ms(alpsg_handle h);

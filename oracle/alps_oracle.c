@@ -47,6 +47,15 @@
 
 static const double ORC_PI = 3.141592653589793238462643383279502884; /* 4.d0*atan(1.d0) */
 
+/* Branch counters (test instrumentation): see ORC_BR_* in alps_oracle.h. */
+long long orc_branch_counts[ORC_NBRANCH];
+void orc_reset_branch_counts(void) { memset(orc_branch_counts, 0, sizeof(orc_branch_counts)); }
+#define COUNT(b)           \
+  do {                     \
+    _Pragma("omp atomic") \
+    orc_branch_counts[b]++; \
+  } while (0)
+
 /* ---- index helpers (Fortran indices, see the reference declarations) ---- */
 /* pp(1:nspec,0:nperp,0:npar,1:2)       ALPS_com.f90:202 */
 #define PP(P, is, iperp, ipar, k) \
@@ -538,10 +547,14 @@ double complex orc_integrate_res(const orc_plasma *P, int sproc, double complex 
       ipar_res = npar - 1 + ipar;
   }
 
-  if ((ipar_res - pos) <= 2)
+  if ((ipar_res - pos) <= 2) {
+    COUNT(ORC_BR_EDGE_LOW);
     return orc_integrate(P, sproc, om, nn, mode, ipar_res + pos, npar - 1);
-  if ((ipar_res + pos) >= (npar - 2))
+  }
+  if ((ipar_res + pos) >= (npar - 2)) {
+    COUNT(ORC_BR_EDGE_HIGH);
     return orc_integrate(P, sproc, om, nn, mode, 1, ipar_res - pos);
+  }
 
   lowerlimit = ipar_res - pos;
   integrate_norm = orc_integrate(P, sproc, om, nn, mode, 1, lowerlimit);
@@ -558,6 +571,7 @@ double complex orc_integrate_res(const orc_plasma *P, int sproc, double complex 
 
 #define FG(x, ip) orc_funct_g(P, sproc, (x), (ip), om, nn, mode)
   if (fabs(denomI) > P->Tlim) {
+    COUNT(ORC_BR_PV_TLIM);
     ppar = creal(p_res);
     res = res + 2.0 * FG(ppar, 1) / (ppar - denomR - ii * denomI);
     res = res - 2.0 * FG(2.0 * denomR - ppar, 1) / (ppar - denomR + ii * denomI);
@@ -590,6 +604,7 @@ double complex orc_integrate_res(const orc_plasma *P, int sproc, double complex 
       res = res - 2.0 * FG(2.0 * denomR - ppar, nperp - 1) / (ppar - denomR + ii * denomI);
     }
   } else {
+    COUNT(ORC_BR_PV_ANALYTIC);
     ppar = creal(p_res) + capDelta;
     gprimetr = (FG(denomR + dppar, 1) - FG(denomR - dppar, 1)) / (2.0 * dppar);
     res = res + 2.0 * 2.0 * gprimetr * (((ppar - denomR) * (ppar - denomR)) /
@@ -638,6 +653,7 @@ double complex orc_integrate_res(const orc_plasma *P, int sproc, double complex 
   /* tiny rest, ALPS_fns.f90:1169-1230 */
   ntiny = (int)((PP(P, sproc, 2, upperlimit, 2) - creal(p_res) - capDelta) / smdelta);
   if (ntiny > 0) {
+    COUNT(ORC_BR_TINY_REST);
     correction = ((PP(P, sproc, 2, upperlimit, 2) - creal(p_res) - capDelta) / (1.0 * ntiny)) / smdelta;
     ppar = creal(p_res) + capDelta;
     res = res + 2.0 * correction * (FG(ppar, 1) / (ppar - denomR - ii * denomI));
@@ -688,8 +704,10 @@ double complex orc_landau_integrate(const orc_plasma *P, int sproc, double compl
     fpar_f = orc_eval_fit(P, sproc, iperp, p_res - dppar);
     fperp_i = orc_eval_fit(P, sproc, iperp + 1, p_res);
     fperp_f = orc_eval_fit(P, sproc, iperp - 1, p_res);
-    if (cabs(fpar_i) == 0.0 || cabs(fpar_f) == 0.0 || cabs(fperp_i) == 0.0 || cabs(fpar_f) == 0.0)
+    if (cabs(fpar_i) == 0.0 || cabs(fpar_f) == 0.0 || cabs(fperp_i) == 0.0 || cabs(fpar_f) == 0.0) {
+      COUNT(ORC_BR_LANDAU_ZERO);
       return 0.0;
+    }
     dfperp_C = (fperp_i - fperp_f) / (2.0 * dpperp);
     dfpar_C = (fpar_i - fpar_f) / (2.0 * dppar);
     res = res - h * int_T_res(P, sproc, nn, iperp, p_res, mode) * (qs / fabs(kpar)) *
@@ -755,10 +773,19 @@ double complex orc_int_ee(const orc_plasma *P, int sproc) {
 /* full_integrate, ALPS_fns.f90:750-792 (non-relativistic branches). */
 double complex orc_full_integrate(const orc_plasma *P, int sproc, double complex om, int nn,
                                   int mode, int found_res) {
-  if (!found_res) return orc_integrate(P, sproc, om, nn, mode, 1, P->npar - 1);
-  if (cimag(om) > 0.0) return orc_integrate_res(P, sproc, om, nn, mode);
-  if (cimag(om) < 0.0)
+  if (!found_res) {
+    COUNT(ORC_BR_NONRES);
+    return orc_integrate(P, sproc, om, nn, mode, 1, P->npar - 1);
+  }
+  if (cimag(om) > 0.0) {
+    COUNT(ORC_BR_RES_POS);
+    return orc_integrate_res(P, sproc, om, nn, mode);
+  }
+  if (cimag(om) < 0.0) {
+    COUNT(ORC_BR_RES_NEG);
     return orc_integrate_res(P, sproc, om, nn, mode) + 2.0 * orc_landau_integrate(P, sproc, om, nn, mode);
+  }
+  COUNT(ORC_BR_RES_REAL);
   return orc_integrate_res(P, sproc, om, nn, mode) + orc_landau_integrate(P, sproc, om, nn, mode);
 }
 

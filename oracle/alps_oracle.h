@@ -67,6 +67,23 @@ typedef struct {
   int n_threads;              /* OpenMP threads used inside disp (0 = default) */
 } orc_plasma;
 
+/* Branch counters incremented by the hot-path restatement (test instrumentation). */
+enum {
+  ORC_BR_NONRES = 0,     /* full_integrate without resonance */
+  ORC_BR_RES_POS,        /* resonant, Im om > 0 */
+  ORC_BR_RES_NEG,        /* resonant, Im om < 0 (2 x Landau) */
+  ORC_BR_RES_REAL,       /* resonant, Im om = 0 (1 x Landau) */
+  ORC_BR_EDGE_LOW,       /* integrate_res edge fall-back, low side */
+  ORC_BR_EDGE_HIGH,      /* integrate_res edge fall-back, high side */
+  ORC_BR_PV_TLIM,        /* |Im p_res| > Tlim: Eq. (3.5) ladder */
+  ORC_BR_PV_ANALYTIC,    /* |Im p_res| <= Tlim: Eq. (3.6) analytic form */
+  ORC_BR_TINY_REST,      /* tiny-rest integration */
+  ORC_BR_LANDAU_ZERO,    /* landau_integrate early return */
+  ORC_NBRANCH
+};
+extern long long orc_branch_counts[ORC_NBRANCH];
+void orc_reset_branch_counts(void);
+
 /* Bessel functions (ALPS_fns_rel.f90:1580-1735). */
 double orc_bessj(int n, double x);
 double orc_bessj0(double x);

@@ -107,6 +107,19 @@ def lib():
     return _lib
 
 
+BRANCHES = ["nonres", "res_pos", "res_neg", "res_real", "edge_low", "edge_high", "pv_tlim",
+            "pv_analytic", "tiny_rest", "landau_zero"]
+
+
+def reset_branch_counts():
+    lib().orc_reset_branch_counts()
+
+
+def branch_counts() -> dict:
+    arr = (C.c_longlong * len(BRANCHES)).in_dll(lib(), "orc_branch_counts")
+    return {name: int(arr[i]) for i, name in enumerate(BRANCHES)}
+
+
 def _c(z) -> _C128:
     z = complex(z)
     return _C128(z.real, z.imag)

@@ -1,0 +1,107 @@
+"""GPU root searches against the CPU restatement and the reference golden:
+secant_osc roots to |dw/w| <= 1e-10, map_search values and minima, the
+test_kpar_fast k-scan, and the Fortran host binding."""
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import cases
+from oracle import Oracle
+from test_oracle_golden import GOLD, kpar_fast_scan, printed_equal
+
+pytestmark = pytest.mark.gpu
+
+ROOT_TOL = 1e-10
+
+
+def _engine(p):
+    from alps_amd.engine import Engine
+    return Engine(p)
+
+
+def test_secant_osc_roots_match_oracle():
+    p = cases.small_bimax()
+    p.roots.D_threshold = 1e-14
+    O, E = Oracle(p), _engine(p)
+    O.set_k(0.3, 0.3)
+    E.set_k(0.3, 0.3)
+    guesses = [0.29 - 0.001j, 0.5 - 0.05j, 1.1 - 0.1j, 0.05 - 0.02j]
+    g_roots, g_dfin, g_it = E.secant_osc_batch(guesses)
+    for k, w0 in enumerate(guesses):
+        o_root, o_it = O.secant_osc(w0)
+        assert abs(g_roots[k] - o_root) <= ROOT_TOL * abs(o_root), (w0, g_roots[k], o_root)
+        # a root in the batch equals the same root solved alone
+        alone, _ = E.secant_osc(w0)
+        assert abs(alone - g_roots[k]) <= 1e-13 * abs(alone)
+
+
+def test_map_search_matches_oracle():
+    p = cases.small_bimax()
+    O, E = Oracle(p), _engine(p)
+    O.set_k(0.3, 0.3)
+    E.set_k(0.3, 0.3)
+    args = (0.05, 1.5, -0.4, 0.05, 18, 12)
+    om_o, cal_o, val_o, ir_o, n_o = O.map_search(*args)
+    om_g, cal_g, val_g, ir_g, n_g = E.map_search(*args)
+    assert np.array_equal(om_o, om_g)
+    assert np.max(np.abs(cal_g - cal_o) / np.abs(cal_o)) <= 1e-10
+    assert np.max(np.abs(val_g - val_o)) <= 1e-10
+    assert n_o == n_g and np.array_equal(ir_o, ir_g)
+
+
+def test_refine_after_map_matches_oracle():
+    p = cases.small_bimax()
+    p.roots.D_threshold = 1e-14
+    O, E = Oracle(p), _engine(p)
+    O.set_k(0.3, 0.3)
+    E.set_k(0.3, 0.3)
+    om, cal, val, iroots, n = E.map_search(0.05, 1.5, -0.4, 0.05, 18, 12)
+    guesses = [om[i - 1, j - 1] for i, j in iroots]
+    assert len(guesses) >= 1
+    roots, dfin = E.refine_guess(guesses)
+    for g, r in zip(guesses, roots):
+        ro, _ = O.secant_osc(g)
+        assert abs(r - ro) <= ROOT_TOL * abs(ro)
+
+
+@pytest.mark.slow
+def test_kpar_fast_scan_on_gpu_matches_golden_and_oracle():
+    gold = np.loadtxt(GOLD)
+    p = cases.kpar_fast()
+    E = _engine(p)
+    assert E.set_k(1e-2, 1e-2) == [21, 13]
+    om, (kp, kz, roots) = kpar_fast_scan(E)
+    for i in range(33):
+        row = [kp[i], kz[i], roots[i, 0].real, roots[i, 0].imag]
+        for v, g in zip(row, gold[i]):
+            assert printed_equal(v, g), (i, row, gold[i])
+    O = Oracle(p)
+    O.set_k(1e-2, 1e-2)
+    _, (rc, okp, okz, oroots) = kpar_fast_scan(O)
+    rel = np.abs(roots[:, 0] - oroots[:, 0]) / np.abs(oroots[:, 0])
+    assert np.max(rel) <= ROOT_TOL, rel
+
+
+def test_fortran_host_matches_python_host():
+    from alps_amd import build
+    exe = os.path.join(os.path.dirname(build.__file__), "fortran", "alpsg_demo")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran demo not built")
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    lines = out.stdout.split("\n")
+    nmax = [int(x) for x in next(l for l in lines if l.startswith("NMAX")).split()[1:]]
+    disp = [[float(x) for x in l.split()[1:]] for l in lines if l.startswith("DISP")]
+    root = [float(x) for x in next(l for l in lines if l.startswith("ROOT")).split()[1:]]
+    p = cases.small_bimax()
+    E = _engine(p)
+    assert E.set_k(0.3, 0.3) == nmax
+    for r in disp:
+        d = E.disp(complex(r[0], r[1]))
+        assert abs(d - complex(r[2], r[3])) <= 1e-13 * abs(d)
+    p.roots.D_threshold = 1e-15
+    w, _ = E.secant_osc(0.5 - 0.05j)
+    assert abs(w - complex(*root)) <= 1e-12 * abs(w)

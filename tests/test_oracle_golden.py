@@ -1,0 +1,44 @@
+"""The CPU restatement reproduces the reference's committed test_kpar_fast scan
+(tests/test_kpar_fast.py of ALPS checks it at rtol 1e-2; here every printed
+digit must agree)."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import cases
+from oracle import Oracle
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "test_kpar_fast.scan_kpara_1.root_1")
+
+
+def printed_equal(v, g):
+    """v rounds to the es14.4 value g (half a unit of the last printed digit)."""
+    if g == 0.0:
+        return abs(v) < 1e-300
+    e = math.floor(math.log10(abs(g)))
+    return abs(v - g) <= 0.5 * 10.0 ** (e - 4) * (1 + 1e-9)
+
+
+def kpar_fast_scan(solver):
+    """refine_guess at the starting k, then om_scan(1): scan_type 4, swf = 0.1,
+    32 log steps (test_kpar_fast.in)."""
+    om, _ = solver.secant_osc(9.9e-3 - 5.5e-6j)
+    diff = (math.log10(1e-1) - math.log10(1e-2)) / (32 * 1.0)
+    return om, solver.om_scan(4, True, diff, 0.0, 32, 1, [om])
+
+
+@pytest.mark.slow
+def test_oracle_reproduces_kpar_fast_golden():
+    gold = np.loadtxt(GOLD)
+    p = cases.kpar_fast()
+    O = Oracle(p)
+    assert O.set_k(1e-2, 1e-2) == [21, 13]
+    om, (rc, kp, kz, roots) = kpar_fast_scan(O)
+    assert rc == 0
+    assert gold.shape == (33, 4)
+    for i in range(33):
+        row = [kp[i], kz[i], roots[i, 0].real, roots[i, 0].imag]
+        for v, g in zip(row, gold[i]):
+            assert printed_equal(v, g), (i, row, gold[i])

@@ -1,0 +1,123 @@
+"""Physics pins of the CPU restatement: the tabulated-f0 integration of disp()
+converges to the closed-form bi-Maxwellian susceptibility, the analytic
+continuations reproduce the tables they continue, derivatives and the ee term
+match their closed forms."""
+import math
+
+import numpy as np
+import pytest
+
+import cases
+from alps_amd.plasma import bimaxwellian_plasma
+from nhds_closed_form import chi_bimax
+from oracle import Oracle
+
+KP, KZ = 0.3, 0.3
+
+
+def _proton(nperp, npar, alph=1.0, drift=0.0, kperp_norm=True):
+    return bimaxwellian_plasma(ms=[1.0], qs=[1.0], ns=[1.0], taus=[1.0], alphs=[alph],
+                               ps=[drift], beta=1.0, vA=1e-4, nperp=nperp, npar=npar, maxP=8.0,
+                               positions_principal=3, n_resonance_interval=100, Tlim=0.01,
+                               Bessel_zero=1e-45, kperp_norm=kperp_norm)
+
+
+def _err(p, om, alph, drift):
+    O = Oracle(p)
+    O.set_k(KP, KZ)
+    d, chi0, low, wave = O.disp(om, want_tensors=True)
+    got = chi0[0] * (om * om * p.vA * p.vA)   # chi0 is chi/(om vA)^2 for both kperp_norm
+    ref = chi_bimax(om, KP, KZ, 1.0, 1.0, 1.0, 1.0, alph, drift)
+    return np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("om", [0.5 + 0.05j, 0.5 - 0.05j, 1.2 + 0.1j, 0.7 - 0.05j])
+@pytest.mark.parametrize("alph,drift", [(1.0, 0.0), (2.0, 0.0), (1.0, 0.6)])
+def test_chi_converges_to_closed_form(om, alph, drift):
+    e1 = _err(_proton(64, 128, alph, drift), om, alph, drift)
+    e2 = _err(_proton(128, 256, alph, drift), om, alph, drift)
+    assert e2 < 5e-3, (e1, e2)
+    assert e2 < 0.7 * e1, (e1, e2)   # the error shrinks with the grid spacing
+
+
+def test_chi_analytic_branch_close_to_closed_form():
+    """|Im p_res| <= Tlim: Eq. (3.6) form; accurate to the method's own ~1e-2."""
+    om = 0.5 - 0.002j
+    assert _err(_proton(128, 256), om, 1.0, 0.0) < 2e-2
+
+
+def test_kperp_norm_false_same_physical_chi():
+    om = 0.6 - 0.03j
+    a = Oracle(_proton(48, 96))
+    b = Oracle(_proton(48, 96, kperp_norm=False))
+    a.set_k(KP, KZ)
+    b.set_k(KP, KZ)
+    ca = a.disp(om, want_tensors=True)[1]
+    cb = b.disp(om, want_tensors=True)[1]
+    np.testing.assert_allclose(cb, ca, rtol=1e-12, atol=1e-12 * np.max(np.abs(ca)))
+
+
+def test_maxwell_and_kappa_fits_reproduce_tables():
+    """eval_fit at real ppar equals the generated f0 (the ideal fit is exact)."""
+    for p in (cases.small_bimax(), cases.kappa_case()):
+        O = Oracle(p)
+        for s in range(p.nspec):
+            for i in (0, 3, p.nperp // 2, p.nperp):
+                for j in (0, 7, p.npar // 2, p.npar):
+                    v = O.eval_fit(s + 1, i, p.pp[s, i, j, 1])
+                    assert abs(v.imag) == 0.0
+                    assert abs(v.real / p.f0[s, i, j] - 1) < 1e-11, (s, i, j)
+
+
+def test_chebyshev_continuation_reproduces_table():
+    p = cases.poly_case(order=8)
+    O = Oracle(p)
+    for i in (0, 5, 12):
+        for j in (1, 10, 24, 40):
+            v = O.eval_fit(1, i, p.pp[0, i, j, 1])
+            if np.log10(p.f0[0, i, j]) < -p.poly_log_max[0]:
+                assert v == 0   # log-fit clipped at -poly_log_max (ALPS_analyt.f90:343-352)
+            else:
+                assert abs(v.real / p.f0[0, i, j] - 1) < 1e-6
+    # outside [-1, 1] of the Chebyshev variable the continuation is zero
+    assert O.eval_fit(1, 3, p.pp[0, 3, -1, 1] + 1.0) == 0
+
+
+def test_derivatives_match_gaussian():
+    """Central differences of derivative_f0 agree with the exact derivative of the
+    Gaussian table within the Taylor remainder (h^2/6) * max |third derivative|."""
+    p = _proton(64, 128, alph=2.0)
+    O = Oracle(p)
+    df = O.df0
+    pe = p.pp[0, :, 0, 0]
+    pa = p.pp[0, 0, :, 1]
+    a, b = 1.0, 2.0   # f ~ exp(-ppar^2/a - pperp^2/b)
+    f = p.f0[0]
+    hp, hq = pe[1] - pe[0], pa[1] - pa[0]
+
+    def bound(x, c, h, fv):
+        # third derivative of exp(-x^2/c) is f*(12x/c^2 - 8x^3/c^3); factor 4 covers [x-h, x+h]
+        return 4 * h * h / 6 * fv * (8 * abs(x) ** 3 / c ** 3 + 12 * abs(x) / c ** 2 + 1)
+
+    for i in (3, 20, 40):
+        for j in (10, 64, 100):
+            exact1 = -2 * pe[i] / b * f[i, j]
+            exact2 = -2 * pa[j] / a * f[i, j]
+            assert abs(df[0, i, j, 0] - exact1) <= bound(pe[i], b, hp, f[i, j])
+            assert abs(df[0, i, j, 1] - exact2) <= bound(pa[j], a, hq, f[i, j])
+    # near the centre the stencil is accurate far beyond a factor-2 slip
+    assert abs(df[0, 20, 64, 1] / (-2 * pa[64] / a * f[20, 64]) - 1) < 0.05 or abs(pa[64]) < 1e-12
+
+
+def test_int_ee_closed_form():
+    """int_ee -> (2 pi q/m) norm sqrt(pi a) (a - b)/2 for f = norm exp(-ppar^2/a - pperp^2/b)."""
+    errs = []
+    for nperp, npar in [(64, 128), (128, 256)]:
+        p = _proton(nperp, npar, alph=2.0)
+        O = Oracle(p)
+        got = O.int_ee(1).real
+        a, b = 1.0, 2.0
+        norm = math.pi ** -1.5 / 2.0
+        ref = 2 * math.pi * norm * math.sqrt(math.pi * a) * (a - b) / 2
+        errs.append(abs(got / ref - 1))
+    assert errs[1] < 0.03 and errs[1] < errs[0], errs
