@@ -35,9 +35,14 @@ hipError_t launch_species_k(hipStream_t st, int nperp, int npar, int nmax, doubl
                             double kpar, double q, double m, const double *pperp,
                             const double *ppar, const double *df1, const double *df2,
                             const int *mtab, double *bes, double *atab, double *nodes);
-hipError_t launch_disp(hipStream_t st, int nch, const ProbDev *prob, int nom, const double2 *om,
-                       const int *active, double2 *det, double2 *chi, double2 *low,
-                       double2 *wave);
+hipError_t launch_sweep(hipStream_t st, int nch, int W, const ProbDev *prob, int nom,
+                        const double2 *om, const int *active, double2 *chiA, double2 *lowA);
+hipError_t launch_resonant(hipStream_t st, int nch, const ProbDev *prob, int nspec, int nom,
+                           const double2 *om, const int *active, double2 *chiB, double2 *lowB);
+hipError_t launch_assemble(hipStream_t st, const ProbDev *prob, int nom, int W, const double2 *om,
+                           const int *active, const double2 *chiA, const double2 *chiB,
+                           const double2 *lowA, const double2 *lowB, double2 *det, double2 *chi,
+                           double2 *low, double2 *wave);
 
 namespace {
 
@@ -102,7 +107,7 @@ struct alpsg_ctx {
   ProbDev prob{};
   DevBuf<ProbDev> prob_d;
   // batch staging
-  DevBuf<double2> om_d, det_d, chi_d, low_d, wave_d;
+  DevBuf<double2> om_d, det_d, chi_d, low_d, wave_d, chiA_d, chiB_d, lowA_d, lowB_d;
   DevBuf<int> act_d;
   int fixed_nch = 0;  // 0 = heuristic / tuned
   std::vector<double> root_ms;  // completion time of each root of the last secant batch
@@ -124,6 +129,10 @@ struct alpsg_ctx {
     chi_d.release();
     low_d.release();
     wave_d.release();
+    chiA_d.release();
+    chiB_d.release();
+    lowA_d.release();
+    lowB_d.release();
     act_d.release();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -138,36 +147,60 @@ inline size_t f_pp(int nspec, int nperp, int npar, int s, int iperp, int ipar, i
   return (size_t)s + (size_t)nspec * (iperp + (size_t)(nperp + 1) * (ipar + (size_t)(npar + 1) * c));
 }
 
-int pick_nch(int n) {
-  // enough wavefronts to cover 256 CUs several times over, fewest lanes per omega
-  if (n >= 65536) return 1;
-  if (n >= 16384) return 2;
-  if (n >= 4096) return 4;
-  if (n >= 1024) return 8;
-  if (n >= 256) return 16;
-  if (n >= 64) return 32;
-  return 64;
+// Launch shape of the three disp kernels for a batch of n omegas: lanes per
+// omega in the node sweep (nch_a) and its wavefronts per workgroup (w_a), lanes
+// per omega in the resonant kernel (nch_b).
+struct LaunchShape {
+  int nch_a, w_a, nch_b;
+};
+
+LaunchShape pick_shape(int n) {
+  if (n >= 32768) return {1, 4, 16};
+  if (n >= 8192) return {2, 4, 16};
+  if (n >= 2048) return {8, 4, 32};
+  if (n >= 256) return {16, 8, 32};
+  if (n >= 32) return {32, 8, 64};
+  return {64, 8, 64};
+}
+
+// Runs the disp kernels on n omegas already resident in om_dev.
+int disp_device(alpsg_ctx *h, int n, const double2 *om_dev, const int *active_dev, bool want_chi,
+                bool want_low, bool want_wave, LaunchShape sh) {
+  const int ns = h->nspec;
+  HIPCHK(h->det_d.ensure(n));
+  HIPCHK(h->chiA_d.ensure((size_t)n * sh.w_a * ns * 6));
+  HIPCHK(h->chiB_d.ensure((size_t)n * ns * 6));
+  if (want_chi) HIPCHK(h->chi_d.ensure((size_t)n * ns * 9));
+  if (want_low) {
+    HIPCHK(h->low_d.ensure((size_t)n * ns * 27));
+    HIPCHK(h->lowA_d.ensure((size_t)n * ns * 18));
+    HIPCHK(h->lowB_d.ensure((size_t)n * ns * 18));
+  }
+  if (want_wave) HIPCHK(h->wave_d.ensure((size_t)n * 9));
+  HIPCHK(launch_sweep(h->st, sh.nch_a, sh.w_a, h->prob_d.p, n, om_dev, active_dev, h->chiA_d.p,
+                      want_low ? h->lowA_d.p : nullptr));
+  HIPCHK(launch_resonant(h->st, sh.nch_b, h->prob_d.p, ns, n, om_dev, active_dev, h->chiB_d.p,
+                         want_low ? h->lowB_d.p : nullptr));
+  HIPCHK(launch_assemble(h->st, h->prob_d.p, n, sh.w_a, om_dev, active_dev, h->chiA_d.p,
+                         h->chiB_d.p, want_low ? h->lowA_d.p : nullptr,
+                         want_low ? h->lowB_d.p : nullptr, h->det_d.p,
+                         want_chi ? h->chi_d.p : nullptr, want_low ? h->low_d.p : nullptr,
+                         want_wave ? h->wave_d.p : nullptr));
+  return ALPSG_OK;
 }
 
 int run_disp(alpsg_ctx *h, int n, const double *om, double *det, double *chi, double *low,
-             double *wave, int nch_override = 0) {
+             double *wave, int plain = 0) {
   if (!h->have_k) return fail(ALPSG_E_STATE, "alpsg_set_k has not been called");
   if (n < 0) return fail(ALPSG_E_ARG, "negative batch size");
   if (n == 0) return ALPSG_OK;
   HIPCHK(h->om_d.ensure(n));
-  HIPCHK(h->det_d.ensure(n));
-  if (chi) HIPCHK(h->chi_d.ensure((size_t)n * h->nspec * 9));
-  if (low) {
-    HIPCHK(h->low_d.ensure((size_t)n * h->nspec * 27));
-    HIPCHK(hipMemsetAsync(h->low_d.p, 0, sizeof(double2) * n * h->nspec * 27, h->st));
-  }
-  if (wave) HIPCHK(h->wave_d.ensure((size_t)n * 9));
   HIPCHK(hipMemcpyAsync(h->om_d.p, om, sizeof(double2) * n, hipMemcpyHostToDevice, h->st));
-  int nch = nch_override ? nch_override : (h->fixed_nch ? h->fixed_nch : pick_nch(n));
+  LaunchShape sh = plain ? LaunchShape{64, 8, 64} : pick_shape(n);
+  if (!plain && h->fixed_nch) sh.nch_a = h->fixed_nch;
   HIPCHK(hipEventRecord(h->ev0, h->st));
-  HIPCHK(launch_disp(h->st, nch, h->prob_d.p, n, h->om_d.p, nullptr, h->det_d.p,
-                     chi ? h->chi_d.p : nullptr, low ? h->low_d.p : nullptr,
-                     wave ? h->wave_d.p : nullptr));
+  int rc = disp_device(h, n, h->om_d.p, nullptr, chi != nullptr, low != nullptr, wave != nullptr, sh);
+  if (rc) return rc;
   HIPCHK(hipEventRecord(h->ev1, h->st));
   HIPCHK(hipMemcpyAsync(det, h->det_d.p, sizeof(double2) * n, hipMemcpyDeviceToHost, h->st));
   if (chi)
@@ -557,6 +590,21 @@ int alpsg_create(const alpsg_plasma_desc *d, alpsg_handle *out) {
     S.pperp = h->pperp.p + (size_t)s * (nperp + 1);
     S.ppar = h->ppar.p + (size_t)s * (npar + 1);
     S.fitc = h->fitc.p + fit_off[s];
+    if (d->ACmethod[s] == 1) {
+      bool uni = true;
+      for (int f = 0; f < d->n_fits[s] && uni; ++f) {
+        if (d->fit_type[s + (size_t)nspec * f] != 1) uni = false;
+        for (int k = 1; k <= 2 && uni; ++k) {
+          const double v0 = d->param_fit[(size_t)s + (size_t)nspec * (0 + (size_t)(npf + 1) * (k + 5 * (size_t)f))];
+          for (int i = 1; i <= nperp; ++i)
+            if (d->param_fit[(size_t)s + (size_t)nspec * (i + (size_t)(npf + 1) * (k + 5 * (size_t)f))] != v0) {
+              uni = false;
+              break;
+            }
+        }
+      }
+      S.fit_uniform = uni ? 1 : 0;
+    }
     if (d->ACmethod[s] == 2) {
       S.poly_kind = d->poly_kind[s];
       S.poly_order = d->poly_order[s];
@@ -676,7 +724,7 @@ int alpsg_disp_batch(alpsg_handle h, int n, const double *om, double *det, doubl
 int alpsg_disp_plain(alpsg_handle h, int n, const double *om, double *det) {
   if (!h || (n > 0 && (!om || !det))) return fail(ALPSG_E_ARG, "null argument");
   for (int i = 0; i < n; ++i) {
-    int rc = run_disp(h, 1, om + 2 * i, det + 2 * i, nullptr, nullptr, nullptr, 64);
+    int rc = run_disp(h, 1, om + 2 * i, det + 2 * i, nullptr, nullptr, nullptr, 1);
     if (rc) return rc;
   }
   return ALPSG_OK;

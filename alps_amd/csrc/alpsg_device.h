@@ -8,7 +8,7 @@
 
 #define ALPSG_DMAXSPEC 8
 #define ALPSG_DMAXFITS 4
-#define NODE_STRIDE 8   // doubles per (n, ipar) node record: G1 H1 G2 H2 G3 H3 ppar pad
+#define NODE_STRIDE 8   // doubles per (n, ipar) node record: G1 H1 G2 H2 G3 H3 ppar ppar^2
 #define ATAB_STRIDE 4   // doubles per (n, iperp) entry: J^2, pperp^2 J'^2, pperp J J', J
 #define FIT_STRIDE 8    // doubles per (iperp, fit) entry of the analytic-continuation table
 
@@ -17,6 +17,7 @@ struct SpecDev {
   int acmethod, nfits;
   int fit_type[ALPSG_DMAXFITS];
   int poly_kind, poly_order, poly_logfit;
+  int fit_uniform;    // every fit Maxwellian with pperp-independent ppar parameters
   double q, m, dens;
   double dpperp;      // pp(2,2,1) - pp(1,2,1)
   double dppar;       // pp(2,2,2) - pp(2,1,2)  (integrate_res, determine_resonances)

@@ -1,32 +1,36 @@
-// Fused batched evaluation of disp(om) (ALPS_fns.f90:252-636).
+// Batched evaluation of disp(om) (ALPS_fns.f90:252-636) in three launches.
 //
-// One workgroup is one 64-lane wavefront.  Its lanes form OMB = 64/NCH groups
-// of NCH lanes; each group owns one omega and walks every harmonic row of
-// every species, so a wavefront evaluates OMB omegas side by side:
-//   * node sweep  (integrate, ALPS_fns.f90:799-864): the NCH lanes of a group
-//     split the ppar nodes of the row; the +n and -n rows of a harmonic share
-//     one pass over the pperp-reduced node table (alpsg_tables.hip);
-//   * principal value (integrate_res, ALPS_fns.f90:870-1238): the ladder of
-//     n_resonance_interval sub-steps is split over the group's lanes, with
-//     funct_g's linear interpolation applied to the pperp-summed node values;
-//   * Landau term (landau_integrate, ALPS_fns.f90:1327-1452): the pperp
-//     points are split over the group's lanes; the analytic continuation
-//     (eval_fit) is evaluated once per (omega, n) and shared by all 6 modes;
-//   * group sums use cross-lane shuffles; the species sum, the dielectric
-//     tensor, the wave tensor and det D are formed in registers.
+//  k_sweep     the trapezoid sums of integrate() (ALPS_fns.f90:799-864) over
+//              the ppar nodes of every harmonic row, on the pperp-reduced node
+//              tables (alpsg_tables.hip).  The +n and -n rows of a harmonic
+//              share one pass over the table; harmonic rows are dealt to the
+//              W wavefronts of a workgroup.  Within a wavefront OMB = 64/NCH
+//              omegas run side by side, NCH lanes per omega.  With NCH = 1 a
+//              lane owns one omega and the node records are wave-uniform
+//              scalar loads, so the sweep is pure FP64 VALU work.
+//  k_resonant  the principal-value part of integrate_res (ALPS_fns.f90:870-1238)
+//              and landau_integrate (ALPS_fns.f90:1327-1452) of the harmonics
+//              whose resonance lies in or near the ppar grid; the NCH lanes of
+//              an omega split the sub-step ladder and the pperp points.
+//  k_assemble  species sums, ee term, n_s q_s normalisation, dielectric tensor,
+//              wave tensor and det D (ALPS_fns.f90:479-624).
 // Every branch decision (resonance detection, ipar_res, ipar_close, ladder
-// positions) is computed without FMA contraction in the reference's
-// operation order, so the GPU takes the same branches as the CPU restatement.
+// positions) is evaluated without FMA contraction in the reference's
+// operation order, so the device takes the same branches as the CPU
+// restatement.
 #include "alpsg_device.h"
 
 namespace {
 
 constexpr double PI_D = 3.141592653589793238462643383279502884;
 
+typedef const __attribute__((address_space(4))) double *cdptr;  // invariant (scalar-loadable)
+
 struct RowClass {
   int a1, b1, a2, b2;  // ppar index ranges of the trapezoid (integrate calls)
   int pv;              // principal-value part present
   int landau;          // Landau multiplier 0, 1, 2 (full_integrate, ALPS_fns.f90:782-790)
+  int found;
   int ipar_res, upper;
   double pr, pim;      // resonant momentum
 };
@@ -47,26 +51,22 @@ __device__ __noinline__ void classify(const SpecDev &S, int P, double kpar, doub
   rc.landau = 0;
   rc.ipar_res = 0;
   rc.upper = 0;
+  rc.a1 = 1;
+  rc.b1 = npar - 1;
   bool found = (pa[1] <= pr && pr < pa[npar]);
   const double pos = (double)P;
   if (pr < pa[1] && pr >= (pa[1] - pos * d)) found = true;
   if (pr >= pa[npar - 1] && pr < (pa[npar - 1] + pos * d)) found = true;
-  if (!found) {
-    rc.a1 = 1;
-    rc.b1 = npar - 1;
-    return;
-  }
+  rc.found = found;
+  if (!found) return;
   int ipr = 0;
   if (pa[1] <= pr && pr < pa[npar - 1]) {
-    int lo = 1, hi = npar - 2;  // largest i in [1, npar-2] with pa[i] <= pr
-    while (lo < hi) {
-      int mid = (lo + hi + 1) >> 1;
-      if (pa[mid] <= pr)
-        lo = mid;
-      else
-        hi = mid - 1;
-    }
-    ipr = lo;
+    // the unique i in [1, npar-2] with pa[i] <= pr < pa[i+1] (strictly increasing grid)
+    int k = 1 + (int)((pr - pa[1]) / d);
+    k = max(1, min(k, npar - 2));
+    while (k > 1 && pa[k] > pr) --k;
+    while (k < npar - 2 && pa[k + 1] <= pr) ++k;
+    ipr = k;
   }
   for (int ip = 0; ip <= P; ++ip) {
     if (pr >= (pa[0] - d * ip) && pr < (pa[0] - d * (ip - 1))) ipr = -ip;
@@ -92,13 +92,6 @@ __device__ __noinline__ void classify(const SpecDev &S, int P, double kpar, doub
   rc.pv = 1;
 }
 
-// trapezoid weight of node i for integrate(a1,b1) + integrate(a2,b2)
-__device__ __forceinline__ double node_weight(const RowClass &rc, int i) {
-  int w = (i == rc.a1) + (i == rc.b1) + 2 * (i > rc.a1 && i < rc.b1) + (i == rc.a2) +
-          (i == rc.b2) + 2 * (i > rc.a2 && i < rc.b2);
-  return (double)w;
-}
-
 // funct_g's ipar_close (ALPS_fns.f90:1285-1294): the last ipar in [1,npar-1]
 // with |ppar(ipar) - x| <= dppar/2, clamped to [2, npar-2].
 __device__ __noinline__ int ipar_close(const SpecDev &S, double x) {
@@ -115,7 +108,7 @@ __device__ __noinline__ int ipar_close(const SpecDev &S, double x) {
   else
     k0 = 1 + (int)floor(t + 0.5);
   int best = 0;
-  int khi = min(k0 + 2, npar - 1), klo = max(k0 - 2, 1);
+  const int khi = min(k0 + 2, npar - 1), klo = max(k0 - 2, 1);
   for (int k = khi; k >= klo; --k)
     if (fabs(pa[k] - x) <= half) {
       best = k;
@@ -126,29 +119,31 @@ __device__ __noinline__ int ipar_close(const SpecDev &S, double x) {
   return best;
 }
 
-// pperp-summed funct_g at real ppar x for the 6 node products
-// [U1, p U1, p^2 U1, U2, U3, p U3] with U_a = om*G_a + H_a (no -q/kpar factor).
-__device__ __forceinline__ void interp6(const SpecDev &S, const double *__restrict__ rows, double x,
-                                        double omr, double omi, cd out[6]) {
+// node products [U1, p U1, p^2 U1, U2, U3, p U3], U_a = om*G_a + H_a
+__device__ __forceinline__ void node6(const double *r, double omr, double omi, cd u[6]) {
+  const cd U1 = mk(omr * r[0] + r[1], omi * r[0]);
+  const cd U2 = mk(omr * r[2] + r[3], omi * r[2]);
+  const cd U3 = mk(omr * r[4] + r[5], omi * r[4]);
+  u[0] = U1;
+  u[1] = r[6] * U1;
+  u[2] = r[7] * U1;
+  u[3] = U2;
+  u[4] = U3;
+  u[5] = r[6] * U3;
+}
+
+// pperp-summed funct_g at real ppar x (without the -q/kpar factor), summed into
+// acc with the complex weight wt: acc += wt * interp(x)
+__device__ __forceinline__ void interp6_acc(const SpecDev &S, const double *__restrict__ rows,
+                                            double x, double omr, double omi, cd wt, cd acc[6]) {
   const int ic = ipar_close(S, x);
-  cd u[3][6];
+  cd um[6], u0[6], up[6];
+  node6(rows + (size_t)(ic - 2) * NODE_STRIDE, omr, omi, um);
+  node6(rows + (size_t)(ic - 1) * NODE_STRIDE, omr, omi, u0);
+  node6(rows + (size_t)(ic)*NODE_STRIDE, omr, omi, up);
+  const double f = 0.5 * (x - S.ppar[ic]) / S.dppar_abs;
 #pragma unroll
-  for (int t = 0; t < 3; ++t) {
-    const double *r = rows + (size_t)(ic - 1 + t - 1) * NODE_STRIDE;
-    double G1 = r[0], H1 = r[1], G2 = r[2], H2 = r[3], G3 = r[4], H3 = r[5], p = r[6];
-    cd U1 = mk(omr * G1 + H1, omi * G1), U2 = mk(omr * G2 + H2, omi * G2),
-       U3 = mk(omr * G3 + H3, omi * G3);
-    u[t][0] = U1;
-    u[t][1] = p * U1;
-    u[t][2] = (p * p) * U1;
-    u[t][3] = U2;
-    u[t][4] = U3;
-    u[t][5] = p * U3;
-  }
-  const double dx = x - S.ppar[ic];
-  const double inv = 1.0 / S.dppar_abs;
-#pragma unroll
-  for (int k = 0; k < 6; ++k) out[k] = u[1][k] + (0.5 * ((u[2][k] - u[0][k]) * inv)) * dx;
+  for (int k = 0; k < 6; ++k) acc[k] = acc[k] + wt * (u0[k] + f * (up[k] - um[k]));
 }
 
 __device__ __forceinline__ bool czero(cd a) { return a.re == 0.0 && a.im == 0.0; }
@@ -169,7 +164,7 @@ __device__ cd eval_fit(const SpecDev &S, int iperp, cd p) {
       bm1 = b;
     }
     if (S.poly_logfit) {
-      double lm = S.poly_log_max;
+      const double lm = S.poly_log_max;
       if (f.re < -lm || f.im < -lm || f.re > lm || f.im > lm) return mk(0.0, 0.0);
       const double ln10 = 2.302585092994045684017991454684364208;
       return cexpd(mk(f.re * ln10, f.im * ln10));
@@ -179,19 +174,19 @@ __device__ cd eval_fit(const SpecDev &S, int iperp, cd p) {
   cd f = mk(0.0, 0.0);
   for (int fi = 0; fi < S.nfits; ++fi) {
     const double *c = S.fitc + ((size_t)iperp * S.nfits + fi) * FIT_STRIDE;
-    cd dq = mk(p.re - c[2], p.im);
-    cd sq = dq * dq;
+    const cd dq = mk(p.re - c[2], p.im);
+    const cd sq = dq * dq;
     switch (S.fit_type[fi]) {
       case 1:  // Maxwell: c0 = u1*exp(-pc*pperp^2), c1 = u2, c2 = u3
         f = f + c[0] * cexpd(mk(-c[1] * sq.re, -c[1] * sq.im));
         break;
       case 2: {  // kappa: c3 = u4, c4 = pc*u5*pperp^2
-        cd kp = mk(1.0 + c[1] * sq.re + c[4], c[1] * sq.im);
+        const cd kp = mk(1.0 + c[1] * sq.re + c[4], c[1] * sq.im);
         f = f + c[0] * cpowr(kp, c[3]);
       } break;
       case 6: {  // bi-Moyal: c3 = u4*pc*pperp^2
-        cd arg = mk(c[3] + c[1] * sq.re, c[1] * sq.im);
-        cd e = cexpd(arg);
+        const cd arg = mk(c[3] + c[1] * sq.re, c[1] * sq.im);
+        const cd e = cexpd(arg);
         f = f + c[0] * cexpd(mk(0.5 * (arg.re - e.re), 0.5 * (arg.im - e.im)));
       } break;
       default:
@@ -220,12 +215,10 @@ __device__ __forceinline__ int gor(int v) {
 
 // mode order k: 0 xx, 1 yy, 2 zz, 3 xy, 4 xz, 5 yz; node-product index per mode
 __device__ constexpr int ACC_OF_MODE[6] = {0, 3, 2, 4, 1, 5};
-// (i,j) of mode k (0-based)
 __device__ constexpr int MI[6] = {0, 1, 2, 0, 0, 1};
 __device__ constexpr int MJ[6] = {0, 1, 2, 1, 2, 2};
 
-// T-tensor constant C_k(n) (int_T, ALPS_fns.f90:1659-1703), real and imaginary
-// parts: modes xy and yz are purely imaginary.
+// T-tensor constant C_k(n) (int_T, ALPS_fns.f90:1659-1703)
 __device__ __forceinline__ cd tconst(const ProbDev &Pd, const SpecDev &S, int n, int k) {
   const double kp = Pd.kperp, z = S.z;
   const bool kn = Pd.kperp_norm != 0;
@@ -245,8 +238,168 @@ __device__ __forceinline__ cd tconst(const ProbDev &Pd, const SpecDev &S, int n,
   }
 }
 
-// Principal-value and tiny-rest contributions of one resonant row
-// (integrate_res after the two integrate() calls), partial sums of this lane.
+// one node of one row: acc += w * U * (num / den) with the b-weights
+__device__ __forceinline__ void node_acc(double w, double dre, double dim, const cd &U1,
+                                         const cd &U2, const cd &U3, double p, double p2,
+                                         cd acc[6]) {
+  const double inv = w * rcp_d(dre * dre + dim * dim);
+  const cd r = mk(dre * inv, -dim * inv);
+  const cd u1 = U1 * r, u2 = U2 * r, u3 = U3 * r;
+  acc[0] = acc[0] + u1;
+  acc[1] = acc[1] + p * u1;
+  acc[2] = acc[2] + p2 * u1;
+  acc[3] = acc[3] + u2;
+  acc[4] = acc[4] + u3;
+  acc[5] = acc[5] + p * u3;
+}
+
+// endpoint contribution (weight 1) of node e for a row with denominator shift nq
+__device__ __forceinline__ void endpoint_acc(const double *__restrict__ rows, int e, int npar,
+                                             double m, double kpar, double omr, double omi,
+                                             double nq, cd acc[6]) {
+  if (e < 1 || e > npar - 1) return;
+  const double *r = rows + (size_t)(e - 1) * NODE_STRIDE;
+  const cd U1 = mk(omr * r[0] + r[1], omi * r[0]);
+  const cd U2 = mk(omr * r[2] + r[3], omi * r[2]);
+  const cd U3 = mk(omr * r[4] + r[5], omi * r[4]);
+  node_acc(1.0, (m * omr - kpar * r[6]) - nq, m * omi, U1, U2, U3, r[6], r[7], acc);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// k_sweep: trapezoid node sums of every harmonic row pair.
+// chiA[iom][W][nspec][6] gets this wavefront's partial chi per species (before
+// the n_s q_s factor); lowA[iom][nspec][3][6] (if non-null) the n = -1, 0, 1
+// rows.
+// ---------------------------------------------------------------------------
+template <int NCH, int W>
+__global__ __launch_bounds__(64 * W) void k_sweep(const ProbDev *__restrict__ probp, int nom,
+                                                  const double2 *__restrict__ om_in,
+                                                  const int *__restrict__ active,
+                                                  double2 *__restrict__ chiA,
+                                                  double2 *__restrict__ lowA) {
+  constexpr int OMB = 64 / NCH;
+  const ProbDev &Pd = *probp;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int g = lane / NCH, c = lane % NCH;
+  const int iom = blockIdx.x * OMB + g;
+  const bool valid = (iom < nom) && (active == nullptr || active[iom] != 0);
+  const double2 ov = valid ? om_in[iom] : make_double2(1.0, 0.5);
+  const double omr = ov.x, omi = ov.y;
+  const int nspec = Pd.nspec;
+  const double kpar = Pd.kpar;
+  int rowbase = 0;
+  for (int s = 0; s < nspec; ++s) {
+    const SpecDev &S = Pd.sp[s];
+    const int npar = S.npar;
+    const double m = S.m;
+    const double scaleN = 2.0 * PI_D * S.dpperp * S.dppar_abs * 0.25 * S.q;
+    cd chi[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) chi[k] = mk(0.0, 0.0);
+    for (int nn = 0; nn <= S.nmax; ++nn) {
+      if (((rowbase + nn) % W) != wv) continue;
+      const double *rows = S.nodes + (size_t)nn * (npar - 1) * NODE_STRIDE;
+      const cdptr srows = (cdptr)rows;
+      RowClass rp, rm;
+      classify(S, Pd.P, kpar, omr, omi, nn, rp);
+      if (nn > 0) classify(S, Pd.P, kpar, omr, omi, -nn, rm);
+      const double nq = (1.0 * nn) * S.q;
+      // interior (weight-2) ranges (a, b) of the two integrate() calls per sign
+      const int pl1 = rp.a1 + 1, pc1 = max(rp.b1 - rp.a1 - 1, 0);
+      const int pl2 = rp.a2 + 1, pc2 = max(rp.b2 - rp.a2 - 1, 0);
+      const int ml1 = nn > 0 ? rm.a1 + 1 : 0, mc1 = nn > 0 ? max(rm.b1 - rm.a1 - 1, 0) : 0;
+      const int ml2 = nn > 0 ? rm.a2 + 1 : 0, mc2 = nn > 0 ? max(rm.b2 - rm.a2 - 1, 0) : 0;
+      cd ap[6], am[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) ap[k] = am[k] = mk(0.0, 0.0);
+      const double dim = m * omi;
+      for (int i = 2 + c; i <= npar - 2; i += NCH) {
+        const double G1 = srows[(i - 1) * NODE_STRIDE + 0], H1 = srows[(i - 1) * NODE_STRIDE + 1];
+        const double G2 = srows[(i - 1) * NODE_STRIDE + 2], H2 = srows[(i - 1) * NODE_STRIDE + 3];
+        const double G3 = srows[(i - 1) * NODE_STRIDE + 4], H3 = srows[(i - 1) * NODE_STRIDE + 5];
+        const double p = srows[(i - 1) * NODE_STRIDE + 6], p2 = srows[(i - 1) * NODE_STRIDE + 7];
+        const cd U1 = mk(omr * G1 + H1, omi * G1);
+        const cd U2 = mk(omr * G2 + H2, omi * G2);
+        const cd U3 = mk(omr * G3 + H3, omi * G3);
+        const double base = m * omr - kpar * p;
+        const bool inP = ((unsigned)(i - pl1) < (unsigned)pc1) | ((unsigned)(i - pl2) < (unsigned)pc2);
+        const bool inM = ((unsigned)(i - ml1) < (unsigned)mc1) | ((unsigned)(i - ml2) < (unsigned)mc2);
+        node_acc(inP ? 2.0 : 0.0, inP ? base - nq : 1.0, dim, U1, U2, U3, p, p2, ap);
+        if (nn > 0) node_acc(inM ? 2.0 : 0.0, inM ? base + nq : 1.0, dim, U1, U2, U3, p, p2, am);
+      }
+      if (c == 0) {
+        endpoint_acc(rows, rp.a1, npar, m, kpar, omr, omi, nq, ap);
+        endpoint_acc(rows, rp.b1, npar, m, kpar, omr, omi, nq, ap);
+        if (rp.a2 > 0) {
+          endpoint_acc(rows, rp.a2, npar, m, kpar, omr, omi, nq, ap);
+          endpoint_acc(rows, rp.b2, npar, m, kpar, omr, omi, nq, ap);
+        }
+        if (nn > 0) {
+          endpoint_acc(rows, rm.a1, npar, m, kpar, omr, omi, -nq, am);
+          endpoint_acc(rows, rm.b1, npar, m, kpar, omr, omi, -nq, am);
+          if (rm.a2 > 0) {
+            endpoint_acc(rows, rm.a2, npar, m, kpar, omr, omi, -nq, am);
+            endpoint_acc(rows, rm.b2, npar, m, kpar, omr, omi, -nq, am);
+          }
+        }
+      }
+      if (NCH > 1) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+          ap[k] = gsumc<NCH>(ap[k]);
+          if (nn > 0) am[k] = gsumc<NCH>(am[k]);
+        }
+      }
+      cd vp[6], vm[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        vp[k] = (scaleN * ap[ACC_OF_MODE[k]]) * tconst(Pd, S, nn, k);
+        vm[k] = (scaleN * am[ACC_OF_MODE[k]]) * tconst(Pd, S, -nn, k);
+      }
+      if (nn == 0) {
+        chi[1] = chi[1] + vp[1];
+        chi[2] = chi[2] + vp[2];
+        chi[5] = chi[5] + vp[5];
+      } else {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) chi[k] = (chi[k] + vp[k]) + vm[k];
+      }
+      if (lowA != nullptr && valid && c == 0 && nn <= 1) {
+        double2 *o = lowA + ((size_t)iom * nspec + s) * 18;
+        if (nn == 0) {
+#pragma unroll
+          for (int k = 0; k < 6; ++k) {
+            const bool use = (k == 1 || k == 2 || k == 5);
+            o[6 * 1 + k] = use ? make_double2(vp[k].re, vp[k].im) : make_double2(0.0, 0.0);
+          }
+        } else {
+#pragma unroll
+          for (int k = 0; k < 6; ++k) {
+            o[6 * 2 + k] = make_double2(vp[k].re, vp[k].im);
+            o[6 * 0 + k] = make_double2(vm[k].re, vm[k].im);
+          }
+        }
+      }
+    }
+    rowbase += S.nmax + 1;
+    if (valid && c == 0) {
+      double2 *o = chiA + (((size_t)iom * W + wv) * nspec + s) * 6;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) o[k] = make_double2(chi[k].re, chi[k].im);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_resonant: principal-value and Landau parts of the resonant harmonics of
+// species blockIdx.y.  chiB[iom][nspec][6]; lowB[iom][nspec][3][6] (if non-null).
+// ---------------------------------------------------------------------------
+namespace {
+
+// Principal-value and tiny-rest sums of one resonant row (integrate_res after
+// its two integrate() calls), this lane's share.
 template <int NCH>
 __device__ void pv_part(const SpecDev &S, const ProbDev &Pd, const double *__restrict__ rows,
                         const RowClass &rc, double omr, double omi, int c, cd pv[6]) {
@@ -258,7 +411,6 @@ __device__ void pv_part(const SpecDev &S, const ProbDev &Pd, const double *__res
     capDelta = pr - S.ppar[rc.ipar_res - P];
     smdelta = capDelta / (1.0 * NRI);
   }
-  cd a[6], b[6];
   if (fabs(pim) > Pd.Tlim) {
     for (int j = c; j <= NRI; j += NCH) {
       double x, xm;
@@ -268,12 +420,9 @@ __device__ void pv_part(const SpecDev &S, const ProbDev &Pd, const double *__res
         xm = 2.0 * pr - x;
       }
       const double w = (j == 0 || j == NRI) ? 1.0 : 2.0;
-      cd r1 = cscale_inv(w, mk(x - pr, -pim));
-      cd r2 = conjc(r1);
-      interp6(S, rows, x, omr, omi, a);
-      interp6(S, rows, xm, omr, omi, b);
-#pragma unroll
-      for (int k = 0; k < 6; ++k) pv[k] = pv[k] + (a[k] * r1 - b[k] * r2);
+      const cd r1 = cscale_inv(w, mk(x - pr, -pim));
+      interp6_acc(S, rows, x, omr, omi, r1, pv);
+      interp6_acc(S, rows, xm, omr, omi, mk(-r1.re, r1.im), pv);  // - conj(r1)
     }
   } else {
     double sphi = 0.0;
@@ -295,17 +444,12 @@ __device__ void pv_part(const SpecDev &S, const ProbDev &Pd, const double *__res
         xp = pr + S.dppar;
         xq = pr - S.dppar;
       }
-      interp6(S, rows, xp, omr, omi, a);
-      interp6(S, rows, xq, omr, omi, b);
-      const double inv2d = 1.0 / (2.0 * S.dppar);
-#pragma unroll
-      for (int k = 0; k < 6; ++k) pv[k] = pv[k] + (2.0 * sphi) * ((a[k] - b[k]) * inv2d);
+      const double f = 2.0 * sphi / (2.0 * S.dppar);
+      interp6_acc(S, rows, xp, omr, omi, mk(f, 0.0), pv);
+      interp6_acc(S, rows, xq, omr, omi, mk(-f, 0.0), pv);
       if (pim != 0.0) {
-        interp6(S, rows, pr, omr, omi, a);
         const double sg = (pim > 0.0) ? 1.0 : -1.0;
-        const double f = sg * 2.0 * PI_D / smdelta;
-#pragma unroll
-        for (int k = 0; k < 6; ++k) pv[k] = pv[k] + timesI(f * a[k]);
+        interp6_acc(S, rows, pr, omr, omi, mk(0.0, sg * 2.0 * PI_D / smdelta), pv);
       }
     }
   }
@@ -322,22 +466,23 @@ __device__ void pv_part(const SpecDev &S, const ProbDev &Pd, const double *__res
       cs = corr * smdelta;
     }
   }
-  for (int kk = c; kk <= ntiny && ntiny > 0; kk += NCH) {
-    double x;
-    {
+  if (ntiny > 0) {
+    for (int kk = c; kk <= ntiny; kk += NCH) {
+      double x;
+      {
 #pragma clang fp contract(off)
-      x = (kk == 0) ? x0 : x0 + cs * kk;
+        x = (kk == 0) ? x0 : x0 + cs * kk;
+      }
+      const double w = (kk == 0 || kk == ntiny) ? 1.0 : 2.0;
+      interp6_acc(S, rows, x, omr, omi, cscale_inv(corr * w, mk(x - pr, -pim)), pv);
     }
-    const double w = (kk == 0 || kk == ntiny) ? 1.0 : 2.0;
-    cd r1 = cscale_inv(corr * w, mk(x - pr, -pim));
-    interp6(S, rows, x, omr, omi, a);
-#pragma unroll
-    for (int k = 0; k < 6; ++k) pv[k] = pv[k] + a[k] * r1;
   }
 }
 
-// Landau sums L_a = sum_iperp h a(n,iperp) K(iperp) of this lane (a = J^2,
-// pperp^2 J'^2, pperp J J') and the early-return flag of landau_integrate.
+// Landau sums L_a = sum_iperp h a(n,iperp) K(iperp) of this lane, and the
+// early-return flag of landau_integrate.  When every fit of the species is a
+// Maxwellian whose ppar parameters do not depend on pperp, exp(-u2 (p-u3)^2) is
+// the same for every iperp and is evaluated once (bit-identical values).
 template <int NCH>
 __device__ void landau_part(const SpecDev &S, const ProbDev &Pd, const double *__restrict__ arow,
                             const RowClass &rc, double omr, double omi, int c, cd L[3],
@@ -347,26 +492,53 @@ __device__ void landau_part(const SpecDev &S, const ProbDev &Pd, const double *_
   const cd p = mk(rc.pr, rc.pim);
   const cd pp_ = mk(rc.pr + d, rc.pim), pm_ = mk(rc.pr - d, rc.pim);
   const double qk = S.q / fabs(Pd.kpar), km = Pd.kpar / S.m;
+  const bool uni = S.fit_uniform != 0;
+  cd E0[ALPSG_DMAXFITS], Ep[ALPSG_DMAXFITS], Em[ALPSG_DMAXFITS];
+  if (uni) {
+    for (int fi = 0; fi < S.nfits; ++fi) {
+      const double *cf = S.fitc + (size_t)fi * FIT_STRIDE;  // u2, u3 of iperp = 0
+      const double u2 = cf[1], u3 = cf[2];
+      cd dq, sq;
+      dq = mk(p.re - u3, p.im);
+      sq = dq * dq;
+      E0[fi] = cexpd(mk(-u2 * sq.re, -u2 * sq.im));
+      dq = mk(pp_.re - u3, pp_.im);
+      sq = dq * dq;
+      Ep[fi] = cexpd(mk(-u2 * sq.re, -u2 * sq.im));
+      dq = mk(pm_.re - u3, pm_.im);
+      sq = dq * dq;
+      Em[fi] = cexpd(mk(-u2 * sq.re, -u2 * sq.im));
+    }
+  }
+  auto F = [&](int ip, int which) -> cd {
+    if (!uni) return eval_fit(S, ip, which == 0 ? p : (which > 0 ? pp_ : pm_));
+    cd f = mk(0.0, 0.0);
+    for (int fi = 0; fi < S.nfits; ++fi) {
+      const double c0 = S.fitc[((size_t)ip * S.nfits + fi) * FIT_STRIDE];
+      f = f + c0 * (which == 0 ? E0[fi] : (which > 0 ? Ep[fi] : Em[fi]));
+    }
+    return f;
+  };
   for (int ip = c; ip <= nperp; ip += NCH) {
     double h;
     cd dfperp, dfpar;
     if (ip >= 1 && ip <= nperp - 1) {
       h = (ip == nperp - 1) ? 0.5 : 1.0;
-      cd fpi = eval_fit(S, ip, pp_), fpf = eval_fit(S, ip, pm_);
-      cd fperpi = eval_fit(S, ip + 1, p), fperpf = eval_fit(S, ip - 1, p);
+      const cd fpi = F(ip, 1), fpf = F(ip, -1);
+      const cd fperpi = F(ip + 1, 0), fperpf = F(ip - 1, 0);
       if (czero(fpi) || czero(fpf) || czero(fperpi)) zero = 1;
       dfperp = (fperpi - fperpf) * (1.0 / (2.0 * dpp));
       dfpar = (fpi - fpf) * (1.0 / (2.0 * d));
     } else if (ip == 0) {
       h = 0.5;
-      dfperp = (eval_fit(S, 1, p) - eval_fit(S, 0, p)) * (1.0 / dpp);
-      dfpar = (eval_fit(S, 0, pp_) - eval_fit(S, 0, pm_)) * (1.0 / (2.0 * d));
+      dfperp = (F(1, 0) - F(0, 0)) * (1.0 / dpp);
+      dfpar = (F(0, 1) - F(0, -1)) * (1.0 / (2.0 * d));
     } else {
       h = 0.5;
-      dfperp = (eval_fit(S, nperp, p) - eval_fit(S, nperp - 1, p)) * (1.0 / dpp);
-      dfpar = (eval_fit(S, nperp, pp_) - eval_fit(S, nperp, pm_)) * (1.0 / (2.0 * d));
+      dfperp = (F(nperp, 0) - F(nperp - 1, 0)) * (1.0 / dpp);
+      dfpar = (F(nperp, 1) - F(nperp, -1)) * (1.0 / (2.0 * d));
     }
-    cd K = qk * ((S.pperp[ip] * dfpar - p * dfperp) * km + mk(omr, omi) * dfperp);
+    const cd K = qk * ((S.pperp[ip] * dfpar - p * dfperp) * km + mk(omr, omi) * dfperp);
     const double *a = arow + (size_t)ip * ATAB_STRIDE;
     L[0] = L[0] + (h * a[0]) * K;
     L[1] = L[1] + (h * a[1]) * K;
@@ -376,185 +548,144 @@ __device__ void landau_part(const SpecDev &S, const ProbDev &Pd, const double *_
 
 }  // namespace
 
-// chi/chi_low/wave are written only when the pointers are non-null.
 template <int NCH>
-__global__ __launch_bounds__(64) void k_disp(const ProbDev *__restrict__ probp, int nom,
-                                             const double2 *__restrict__ om_in,
-                                             const int *__restrict__ active,
-                                             double2 *__restrict__ det_out,
-                                             double2 *__restrict__ chi_out,
-                                             double2 *__restrict__ low_out,
-                                             double2 *__restrict__ wave_out) {
+__global__ __launch_bounds__(64) void k_resonant(const ProbDev *__restrict__ probp, int nom,
+                                                 const double2 *__restrict__ om_in,
+                                                 const int *__restrict__ active,
+                                                 double2 *__restrict__ chiB,
+                                                 double2 *__restrict__ lowB) {
   constexpr int OMB = 64 / NCH;
   const ProbDev &Pd = *probp;
   const int lane = threadIdx.x;
   const int g = lane / NCH, c = lane % NCH;
   const int iom = blockIdx.x * OMB + g;
+  const int s = blockIdx.y;
   const bool valid = (iom < nom) && (active == nullptr || active[iom] != 0);
-  const double2 omv = valid ? om_in[iom] : make_double2(1.0, 0.5);
-  const double omr = omv.x, omi = omv.y;
+  const double2 ov = valid ? om_in[iom] : make_double2(1.0, 0.5);
+  const double omr = ov.x, omi = ov.y;
   const int nspec = Pd.nspec;
-  const bool want_low = (low_out != nullptr) && valid && c == 0;
+  const SpecDev &S = Pd.sp[s];
+  const int npar = S.npar, P = Pd.P;
+  const double kpar = Pd.kpar;
+  cd res[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) res[k] = mk(0.0, 0.0);
+  double2 *lo = (lowB != nullptr && valid && c == 0) ? lowB + ((size_t)iom * nspec + s) * 18 : nullptr;
+  if (lo)
+    for (int t = 0; t < 18; ++t) lo[t] = make_double2(0.0, 0.0);
 
-  cd eps[6];
+  // harmonics whose p_res = (m w - n q)/kpar can satisfy determine_resonances:
+  // Re p_res in [pa[1] - P dppar, max(pa[npar], pa[npar-1] + P dppar)), widened by one
+  int nlo = 1, nhi = 0;
+  if (valid && omr == omr) {
+    const double L = S.ppar[1] - P * S.dppar;
+    const double U = fmax(S.ppar[npar], S.ppar[npar - 1] + P * S.dppar);
+    const double n1 = (S.m * omr - kpar * L) / S.q, n2 = (S.m * omr - kpar * U) / S.q;
+    const double lo_d = fmin(n1, n2) - 1.0, hi_d = fmax(n1, n2) + 1.0;
+    nlo = (int)fmax(floor(lo_d), -(double)S.nmax);
+    nhi = (int)fmin(ceil(hi_d), (double)S.nmax);
+  }
+  const double scaleN = 2.0 * PI_D * S.dpperp * 0.25 * (-S.q / kpar);
+  for (int n = nlo; n <= nhi; ++n) {
+    RowClass rc;
+    classify(S, P, kpar, omr, omi, n, rc);
+    if (!rc.found || (!rc.pv && !rc.landau)) continue;
+    const int an = n < 0 ? -n : n;
+    const double *rows = S.nodes + (size_t)an * (npar - 1) * NODE_STRIDE;
+    cd add[6];
 #pragma unroll
-  for (int k = 0; k < 6; ++k) eps[k] = mk(0.0, 0.0);
-
-  for (int s = 0; s < nspec; ++s) {
-    const SpecDev &S = Pd.sp[s];
-    const int npar = S.npar, nmax = S.nmax;
-    const double scaleN = 2.0 * PI_D * S.dpperp * S.dppar_abs * 0.25 * S.q;
-    cd chi[6];
+    for (int k = 0; k < 6; ++k) add[k] = mk(0.0, 0.0);
+    if (rc.pv) {
+      cd pv[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) chi[k] = mk(0.0, 0.0);
-
-    for (int nn = 0; nn <= nmax; ++nn) {
-      const double *rows = S.nodes + (size_t)nn * (npar - 1) * NODE_STRIDE;
-      RowClass rp, rm;
-      classify(S, Pd.P, Pd.kpar, omr, omi, nn, rp);
-      if (nn > 0) {
-        classify(S, Pd.P, Pd.kpar, omr, omi, -nn, rm);
-      } else {
-        rm.a1 = rm.b1 = rm.a2 = rm.b2 = -1000;
-        rm.pv = 0;
-        rm.landau = 0;
-      }
-      const double nq = (1.0 * nn) * S.q;
-      cd ap[6], am[6];
-#pragma unroll
-      for (int k = 0; k < 6; ++k) ap[k] = am[k] = mk(0.0, 0.0);
-
-      // ---- node sweep: integrate() over the ppar nodes of this row pair ----
-      for (int i = 1 + c; i <= npar - 1; i += NCH) {
-        const double *r = rows + (size_t)(i - 1) * NODE_STRIDE;
-        const double G1 = r[0], H1 = r[1], G2 = r[2], H2 = r[3], G3 = r[4], H3 = r[5],
-                     pa = r[6];
-        const cd n1 = mk(omr * G1 + H1, omi * G1), n2 = mk(omr * G2 + H2, omi * G2),
-                 n3 = mk(omr * G3 + H3, omi * G3);
-        const double base = S.m * omr - Pd.kpar * pa;
-        const double dimag = S.m * omi;
-        const double p2 = pa * pa;
-        const double wp = node_weight(rp, i);
-        if (wp != 0.0) {
-          cd rr = cscale_inv(wp, mk(base - nq, dimag));
-          cd u1 = n1 * rr, u2 = n2 * rr, u3 = n3 * rr;
-          ap[0] = ap[0] + u1;
-          ap[1] = ap[1] + pa * u1;
-          ap[2] = ap[2] + p2 * u1;
-          ap[3] = ap[3] + u2;
-          ap[4] = ap[4] + u3;
-          ap[5] = ap[5] + pa * u3;
-        }
-        const double wm = node_weight(rm, i);
-        if (wm != 0.0) {
-          cd rr = cscale_inv(wm, mk(base + nq, dimag));
-          cd u1 = n1 * rr, u2 = n2 * rr, u3 = n3 * rr;
-          am[0] = am[0] + u1;
-          am[1] = am[1] + pa * u1;
-          am[2] = am[2] + p2 * u1;
-          am[3] = am[3] + u2;
-          am[4] = am[4] + u3;
-          am[5] = am[5] + pa * u3;
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < 6; ++k) {
-        ap[k] = gsumc<NCH>(ap[k]);
-        am[k] = gsumc<NCH>(am[k]);
-      }
-
-      // ---- resonant extras for the + and - rows ----
-      cd rowv[2][6];
-      for (int sg = 0; sg < 2; ++sg) {
-        const RowClass &rc = sg == 0 ? rp : rm;
-        const int n = sg == 0 ? nn : -nn;
-        cd *acc = sg == 0 ? ap : am;
-        cd Ck[6];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-          Ck[k] = tconst(Pd, S, n, k);
-          rowv[sg][k] = (scaleN * acc[ACC_OF_MODE[k]]) * Ck[k];
-        }
-        if (sg == 1 && nn == 0) continue;
-        if (rc.pv) {
-          cd pv[6];
-#pragma unroll
-          for (int k = 0; k < 6; ++k) pv[k] = mk(0.0, 0.0);
-          pv_part<NCH>(S, Pd, rows, rc, omr, omi, c, pv);
-          double capDelta, smdelta;
-          {
+      for (int k = 0; k < 6; ++k) pv[k] = mk(0.0, 0.0);
+      pv_part<NCH>(S, Pd, rows, rc, omr, omi, c, pv);
+      double capDelta, smdelta;
+      {
 #pragma clang fp contract(off)
-            capDelta = rc.pr - S.ppar[rc.ipar_res - Pd.P];
-            smdelta = capDelta / (1.0 * Pd.NRI);
-          }
-          const double scalePV = 2.0 * PI_D * smdelta * S.dpperp * 0.25 * (-S.q / Pd.kpar);
-#pragma unroll
-          for (int k = 0; k < 6; ++k) {
-            cd v = gsumc<NCH>(pv[ACC_OF_MODE[k]]);
-            rowv[sg][k] = rowv[sg][k] + (scalePV * v) * Ck[k];
-          }
-        }
-        if (rc.landau) {
-          cd L[3] = {mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0)};
-          int zero = 0;
-          landau_part<NCH>(S, Pd, S.atab + (size_t)nn * (S.nperp + 1) * ATAB_STRIDE, rc, omr, omi,
-                           c, L, zero);
-          zero = gor<NCH>(zero);
-          if (!zero) {
-            const cd p = mk(rc.pr, rc.pim);
-            cd Ls[3] = {gsumc<NCH>(L[0]), gsumc<NCH>(L[1]), gsumc<NCH>(L[2])};
-            // T_res (int_T_res, ALPS_fns.f90:1768-1808): zz, xz, yz take p_res
-            cd Lm[6] = {Ls[0] * Ck[0], Ls[1] * Ck[1], (Ls[0] * (p * p)) * Ck[2],
-                        Ls[2] * Ck[3], (Ls[0] * p) * Ck[4], (Ls[2] * p) * Ck[5]};
-            const double f = -(double)rc.landau * S.dpperp * PI_D * 2.0 * PI_D;
-#pragma unroll
-            for (int k = 0; k < 6; ++k) rowv[sg][k] = rowv[sg][k] + timesI(f * Lm[k]);
-          }
-        }
+        capDelta = rc.pr - S.ppar[rc.ipar_res - P];
+        smdelta = capDelta / (1.0 * Pd.NRI);
       }
-
-      // ---- species accumulation in the reference's order ----
-      if (nn == 0) {
-        chi[1] = chi[1] + rowv[0][1];
-        chi[2] = chi[2] + rowv[0][2];
-        chi[5] = chi[5] + rowv[0][5];
-      } else {
+      const double sc = scaleN * smdelta;
 #pragma unroll
-        for (int k = 0; k < 6; ++k) chi[k] = (chi[k] + rowv[0][k]) + rowv[1][k];
-      }
-      if (want_low && nn <= 1) {
-        // chi_low(s,i,j,-1:1) in Fortran order: s + nspec*(i + 3*(j + 3*q))
+      for (int k = 0; k < 6; ++k)
+        add[k] = add[k] + (sc * gsumc<NCH>(pv[ACC_OF_MODE[k]])) * tconst(Pd, S, n, k);
+    }
+    if (rc.landau) {
+      cd L[3] = {mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0)};
+      int zero = 0;
+      landau_part<NCH>(S, Pd, S.atab + (size_t)an * (S.nperp + 1) * ATAB_STRIDE, rc, omr, omi, c,
+                       L, zero);
+      zero = gor<NCH>(zero);
+      if (!zero) {
+        const cd p = mk(rc.pr, rc.pim);
+        const cd Ls0 = gsumc<NCH>(L[0]), Ls1 = gsumc<NCH>(L[1]), Ls2 = gsumc<NCH>(L[2]);
+        // T_res (int_T_res, ALPS_fns.f90:1768-1808): zz, xz, yz take p_res
+        const cd Lm[6] = {Ls0, Ls1, Ls0 * (p * p), Ls2, Ls0 * p, Ls2 * p};
+        const double f = -(double)rc.landau * S.dpperp * PI_D * 2.0 * PI_D;
 #pragma unroll
-        for (int k = 0; k < 6; ++k) {
-          if (nn == 0 && (k == 0 || k == 3 || k == 4)) continue;
-          const int i = MI[k], j = MJ[k];
-          double2 *o = low_out + (size_t)iom * nspec * 27;
-          if (nn == 0) {
-            o[s + nspec * (i + 3 * (j + 3 * 1))] = make_double2(rowv[0][k].re, rowv[0][k].im);
-          } else {
-            o[s + nspec * (i + 3 * (j + 3 * 2))] = make_double2(rowv[0][k].re, rowv[0][k].im);
-            o[s + nspec * (i + 3 * (j + 3 * 0))] = make_double2(rowv[1][k].re, rowv[1][k].im);
-          }
-        }
+        for (int k = 0; k < 6; ++k) add[k] = add[k] + timesI(f * (Lm[k] * tconst(Pd, S, n, k)));
       }
     }
+    if (n == 0) {
+      add[0] = add[3] = add[4] = mk(0.0, 0.0);
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) res[k] = res[k] + add[k];
+    if (lo && an <= 1) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) lo[6 * (n + 1) + k] = make_double2(add[k].re, add[k].im);
+    }
+  }
+  if (valid && c == 0) {
+    double2 *o = chiB + ((size_t)iom * nspec + s) * 6;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) o[k] = make_double2(res[k].re, res[k].im);
+  }
+}
 
-    // ---- ee term, density*charge normalisation (ALPS_fns.f90:479-514) ----
+// ---------------------------------------------------------------------------
+// k_assemble: one thread per omega.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(128) void k_assemble(
+    const ProbDev *__restrict__ probp, int nom, int W, const double2 *__restrict__ om_in,
+    const int *__restrict__ active, const double2 *__restrict__ chiA,
+    const double2 *__restrict__ chiB, const double2 *__restrict__ lowA,
+    const double2 *__restrict__ lowB, double2 *__restrict__ det_out, double2 *__restrict__ chi_out,
+    double2 *__restrict__ low_out, double2 *__restrict__ wave_out) {
+  const int iom = blockIdx.x * blockDim.x + threadIdx.x;
+  if (iom >= nom) return;
+  if (active != nullptr && active[iom] == 0) return;
+  const ProbDev &Pd = *probp;
+  const int nspec = Pd.nspec;
+  const double2 ov = om_in[iom];
+  const cd om = mk(ov.x, ov.y);
+  cd eps[6];
+  for (int k = 0; k < 6; ++k) eps[k] = mk(0.0, 0.0);
+  cd dn = (om * om) * (Pd.vA * Pd.vA);
+  if (!Pd.kperp_norm) dn = dn * (Pd.kperp * Pd.kperp);
+  for (int s = 0; s < nspec; ++s) {
+    const SpecDev &S = Pd.sp[s];
+    cd chi[6];
+    for (int k = 0; k < 6; ++k) {
+      cd v = mk(0.0, 0.0);
+      for (int w = 0; w < W; ++w) {
+        const double2 t = chiA[(((size_t)iom * W + w) * nspec + s) * 6 + k];
+        v = v + mk(t.x, t.y);
+      }
+      const double2 t = chiB[((size_t)iom * nspec + s) * 6 + k];
+      chi[k] = v + mk(t.x, t.y);
+    }
     const double ee = Pd.kperp_norm ? S.int_ee : Pd.kperp * Pd.kperp * S.int_ee;
     chi[2] = chi[2] + mk(ee, 0.0);
     const double norm = S.dens * S.q;
-#pragma unroll
     for (int k = 0; k < 6; ++k) {
       chi[k] = chi[k] * norm;
       eps[k] = eps[k] + chi[k];
     }
-    const cd om2 = mk(omr, omi) * mk(omr, omi);
-    cd dn = om2 * (Pd.vA * Pd.vA);
-    if (!Pd.kperp_norm) dn = dn * (Pd.kperp * Pd.kperp);
-    if (chi_out != nullptr && valid && c == 0) {
+    if (chi_out != nullptr) {
       double2 *o = chi_out + (size_t)iom * nspec * 9;
       cd v[3][3];
-#pragma unroll
       for (int k = 0; k < 6; ++k) v[MI[k]][MJ[k]] = cdiv(chi[k], dn);
       v[1][0] = mk(-v[0][1].re, -v[0][1].im);
       v[2][0] = v[0][2];
@@ -562,22 +693,19 @@ __global__ __launch_bounds__(64) void k_disp(const ProbDev *__restrict__ probp, 
       for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) o[s + nspec * (i + 3 * j)] = make_double2(v[i][j].re, v[i][j].im);
     }
-    if (want_low) {
+    if (low_out != nullptr) {
       double2 *o = low_out + (size_t)iom * nspec * 27;
-      // add the ee term to the n = 0 zz entry, normalise, symmetrise
-      {
-        double2 t = o[s + nspec * (2 + 3 * (2 + 3 * 1))];
-        o[s + nspec * (2 + 3 * (2 + 3 * 1))] = make_double2(t.x + ee, t.y);
-      }
       for (int q = 0; q < 3; ++q) {
         cd v[3][3];
         for (int i = 0; i < 3; ++i)
           for (int j = 0; j < 3; ++j) v[i][j] = mk(0.0, 0.0);
-#pragma unroll
         for (int k = 0; k < 6; ++k) {
           if (q == 1 && (k == 0 || k == 3 || k == 4)) continue;
-          double2 t = o[s + nspec * (MI[k] + 3 * (MJ[k] + 3 * q))];
-          v[MI[k]][MJ[k]] = cdiv(mk(t.x, t.y) * norm, dn);
+          const double2 a = lowA[((size_t)iom * nspec + s) * 18 + 6 * q + k];
+          const double2 b = lowB[((size_t)iom * nspec + s) * 18 + 6 * q + k];
+          cd t = mk(a.x, a.y) + mk(b.x, b.y);
+          if (q == 1 && k == 2) t = t + mk(ee, 0.0);
+          v[MI[k]][MJ[k]] = cdiv(t * norm, dn);
         }
         v[1][0] = mk(-v[0][1].re, -v[0][1].im);
         v[2][0] = v[0][2];
@@ -588,12 +716,9 @@ __global__ __launch_bounds__(64) void k_disp(const ProbDev *__restrict__ probp, 
       }
     }
   }
-
-  if (!valid || c != 0) return;
-  // ---- dielectric tensor, wave tensor, determinant (ALPS_fns.f90:566-624) ----
-  const cd om = mk(omr, omi);
-  cd unit = Pd.kperp_norm ? (om * Pd.vA) * (om * Pd.vA)
-                          : ((Pd.kperp * om) * Pd.vA) * ((Pd.kperp * om) * Pd.vA);
+  // dielectric tensor, wave tensor, determinant (ALPS_fns.f90:566-624)
+  const cd unit = Pd.kperp_norm ? (om * Pd.vA) * (om * Pd.vA)
+                                : ((Pd.kperp * om) * Pd.vA) * ((Pd.kperp * om) * Pd.vA);
   double enx2, enz2, enxnz;
   if (Pd.kperp_norm) {
     enx2 = Pd.kperp * Pd.kperp;
@@ -605,46 +730,65 @@ __global__ __launch_bounds__(64) void k_disp(const ProbDev *__restrict__ probp, 
     enz2 = (Pd.kpar * Pd.kpar) * k2;
     enxnz = Pd.kpar * (k2 * Pd.kperp);
   }
-  cd w11 = (eps[0] + unit) - mk(enz2, 0.0);
-  cd w22 = ((eps[1] + unit) - mk(enz2, 0.0)) - mk(enx2, 0.0);
-  cd w33 = (eps[2] + unit) - mk(enx2, 0.0);
-  cd w13 = eps[4] + mk(enxnz, 0.0);
-  cd w12 = eps[3];
-  cd w23 = eps[5];
-  cd det = w11 * (w22 * w33 + w23 * w23) + 2.0 * w12 * w23 * w13 - (w13 * w13) * w22 +
-           (w12 * w12) * w33;
+  const cd w11 = (eps[0] + unit) - mk(enz2, 0.0);
+  const cd w22 = ((eps[1] + unit) - mk(enz2, 0.0)) - mk(enx2, 0.0);
+  const cd w33 = (eps[2] + unit) - mk(enx2, 0.0);
+  const cd w13 = eps[4] + mk(enxnz, 0.0);
+  const cd w12 = eps[3];
+  const cd w23 = eps[5];
+  const cd det = w11 * (w22 * w33 + w23 * w23) + 2.0 * w12 * w23 * w13 - (w13 * w13) * w22 +
+                 (w12 * w12) * w33;
   det_out[iom] = make_double2(det.re, det.im);
   if (wave_out != nullptr) {
     double2 *o = wave_out + (size_t)iom * 9;
-    cd W[3][3] = {{w11, w12, w13},
-                  {mk(-w12.re, -w12.im), w22, w23},
-                  {w13, mk(-w23.re, -w23.im), w33}};
+    const cd Wt[3][3] = {{w11, w12, w13},
+                         {mk(-w12.re, -w12.im), w22, w23},
+                         {w13, mk(-w23.re, -w23.im), w33}};
     for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) o[i + 3 * j] = make_double2(W[i][j].re, W[i][j].im);
+      for (int j = 0; j < 3; ++j) o[i + 3 * j] = make_double2(Wt[i][j].re, Wt[i][j].im);
   }
 }
 
-hipError_t launch_disp(hipStream_t st, int nch, const ProbDev *prob, int nom, const double2 *om,
-                       const int *active, double2 *det, double2 *chi, double2 *low,
-                       double2 *wave) {
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_sweep(hipStream_t st, int nch, int W, const ProbDev *prob, int nom,
+                        const double2 *om, const int *active, double2 *chiA, double2 *lowA) {
   if (nom <= 0) return hipSuccess;
-  switch (nch) {
-#define CASE(N)                                                                              \
-  case N: {                                                                                  \
-    int grid = (nom + (64 / N) - 1) / (64 / N);                                              \
-    hipLaunchKernelGGL(k_disp<N>, dim3(grid), dim3(64), 0, st, prob, nom, om, active, det, chi, \
-                       low, wave);                                                           \
-  } break;
-    CASE(1)
-    CASE(2)
-    CASE(4)
-    CASE(8)
-    CASE(16)
-    CASE(32)
-    CASE(64)
-#undef CASE
-    default:
-      return hipErrorInvalidValue;
+#define SW(N, WV)                                                                              \
+  if (nch == N && W == WV) {                                                                   \
+    const int grid = (nom + (64 / N) - 1) / (64 / N);                                          \
+    hipLaunchKernelGGL((k_sweep<N, WV>), dim3(grid), dim3(64 * WV), 0, st, prob, nom, om, active, \
+                       chiA, lowA);                                                            \
+    return hipGetLastError();                                                                  \
   }
+  SW(1, 1) SW(1, 2) SW(1, 4) SW(2, 4) SW(4, 4) SW(8, 4) SW(16, 4) SW(16, 8) SW(32, 8) SW(64, 8)
+  SW(64, 16)
+#undef SW
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_resonant(hipStream_t st, int nch, const ProbDev *prob, int nspec, int nom,
+                           const double2 *om, const int *active, double2 *chiB, double2 *lowB) {
+  if (nom <= 0) return hipSuccess;
+#define RS(N)                                                                                  \
+  if (nch == N) {                                                                              \
+    const int grid = (nom + (64 / N) - 1) / (64 / N);                                          \
+    hipLaunchKernelGGL((k_resonant<N>), dim3(grid, nspec), dim3(64), 0, st, prob, nom, om, active, \
+                       chiB, lowB);                                                            \
+    return hipGetLastError();                                                                  \
+  }
+  RS(4) RS(8) RS(16) RS(32) RS(64)
+#undef RS
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_assemble(hipStream_t st, const ProbDev *prob, int nom, int W, const double2 *om,
+                           const int *active, const double2 *chiA, const double2 *chiB,
+                           const double2 *lowA, const double2 *lowB, double2 *det, double2 *chi,
+                           double2 *low, double2 *wave) {
+  if (nom <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_assemble, dim3((nom + 127) / 128), dim3(128), 0, st, prob, nom, W, om,
+                     active, chiA, chiB, lowA, lowB, det, chi, low, wave);
   return hipGetLastError();
 }

@@ -274,7 +274,7 @@ __global__ void k_moments(int nperp, int npar, int nmax, double kpar, double m,
   o[4] = g3;
   o[5] = h3;
   o[6] = pa;
-  o[7] = 0.0;
+  o[7] = pa * pa;
 }
 
 // ---------------------------------------------------------------------------
