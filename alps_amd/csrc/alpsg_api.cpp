@@ -35,14 +35,16 @@ hipError_t launch_species_k(hipStream_t st, int nperp, int npar, int nmax, doubl
                             double kpar, double q, double m, const double *pperp,
                             const double *ppar, const double *df1, const double *df2,
                             const int *mtab, double *bes, double *atab, double *nodes);
-hipError_t launch_sweep(hipStream_t st, int nch, int W, const ProbDev *prob, int nom,
+hipError_t launch_sweep(hipStream_t st, int nch, int W, int RC, const ProbDev *prob, int nom,
                         const double2 *om, const int *active, double2 *chiA, double2 *lowA);
-hipError_t launch_resonant(hipStream_t st, int nch, const ProbDev *prob, int nspec, int nom,
-                           const double2 *om, const int *active, double2 *chiB, double2 *lowB);
-hipError_t launch_assemble(hipStream_t st, const ProbDev *prob, int nom, int W, const double2 *om,
-                           const int *active, const double2 *chiA, const double2 *chiB,
-                           const double2 *lowA, const double2 *lowB, double2 *det, double2 *chi,
-                           double2 *low, double2 *wave);
+hipError_t launch_resonant(hipStream_t st, int nch, int RB, const ProbDev *prob, int nspec, int nom,
+                           const double2 *om, const int *active, double2 *chiB, double2 *lowB,
+                           bool general);
+void set_nspec_hint(int n);
+hipError_t launch_assemble(hipStream_t st, const ProbDev *prob, int nom, int W, int RB,
+                           const double2 *om, const int *active, const double2 *chiA,
+                           const double2 *chiB, const double2 *lowA, const double2 *lowB,
+                           double2 *det, double2 *chi, double2 *low, double2 *wave);
 
 namespace {
 
@@ -111,6 +113,7 @@ struct alpsg_ctx {
   DevBuf<int> act_d;
   int fixed_nch = 0;  // 0 = heuristic / tuned
   std::vector<double> root_ms;  // completion time of each root of the last secant batch
+  bool any_general_fit = false;  // a species needs eval_fit in the Landau term
 
   ~alpsg_ctx() {
     if (st) (void)hipStreamSynchronize(st);
@@ -148,40 +151,43 @@ inline size_t f_pp(int nspec, int nperp, int npar, int s, int iperp, int ipar, i
 }
 
 // Launch shape of the three disp kernels for a batch of n omegas: lanes per
-// omega in the node sweep (nch_a) and its wavefronts per workgroup (w_a), lanes
-// per omega in the resonant kernel (nch_b).
+// omega in the node sweep (nch_a), its wavefronts per workgroup (w_a) and
+// workgroups per omega block (rc_a); lanes per omega in the resonant kernel
+// (nch_b) and its harmonic blocks (rb).
 struct LaunchShape {
-  int nch_a, w_a, nch_b;
+  int nch_a, w_a, rc_a, nch_b, rb;
 };
 
 LaunchShape pick_shape(int n) {
-  if (n >= 32768) return {1, 4, 16};
-  if (n >= 8192) return {2, 4, 16};
-  if (n >= 2048) return {8, 4, 32};
-  if (n >= 256) return {16, 8, 32};
-  if (n >= 32) return {32, 8, 64};
-  return {64, 8, 64};
+  if (n >= 32768) return {1, 4, 1, 16, 1};
+  if (n >= 8192) return {2, 4, 1, 16, 1};
+  if (n >= 2048) return {8, 4, 1, 32, 1};
+  if (n >= 256) return {16, 8, 1, 32, 2};
+  if (n >= 32) return {32, 8, 2, 64, 4};
+  return {64, 8, 4, 64, 8};
 }
 
 // Runs the disp kernels on n omegas already resident in om_dev.
 int disp_device(alpsg_ctx *h, int n, const double2 *om_dev, const int *active_dev, bool want_chi,
                 bool want_low, bool want_wave, LaunchShape sh) {
   const int ns = h->nspec;
+  const int TW = sh.w_a * sh.rc_a;
   HIPCHK(h->det_d.ensure(n));
-  HIPCHK(h->chiA_d.ensure((size_t)n * sh.w_a * ns * 6));
-  HIPCHK(h->chiB_d.ensure((size_t)n * ns * 6));
+  HIPCHK(h->chiA_d.ensure((size_t)n * TW * ns * 6));
+  HIPCHK(h->chiB_d.ensure((size_t)n * ns * 2 * sh.rb * 6));
   if (want_chi) HIPCHK(h->chi_d.ensure((size_t)n * ns * 9));
   if (want_low) {
     HIPCHK(h->low_d.ensure((size_t)n * ns * 27));
     HIPCHK(h->lowA_d.ensure((size_t)n * ns * 18));
-    HIPCHK(h->lowB_d.ensure((size_t)n * ns * 18));
+    HIPCHK(h->lowB_d.ensure((size_t)n * ns * 2 * sh.rb * 18));
   }
   if (want_wave) HIPCHK(h->wave_d.ensure((size_t)n * 9));
-  HIPCHK(launch_sweep(h->st, sh.nch_a, sh.w_a, h->prob_d.p, n, om_dev, active_dev, h->chiA_d.p,
-                      want_low ? h->lowA_d.p : nullptr));
-  HIPCHK(launch_resonant(h->st, sh.nch_b, h->prob_d.p, ns, n, om_dev, active_dev, h->chiB_d.p,
-                         want_low ? h->lowB_d.p : nullptr));
-  HIPCHK(launch_assemble(h->st, h->prob_d.p, n, sh.w_a, om_dev, active_dev, h->chiA_d.p,
+  HIPCHK(launch_sweep(h->st, sh.nch_a, sh.w_a, sh.rc_a, h->prob_d.p, n, om_dev, active_dev,
+                      h->chiA_d.p, want_low ? h->lowA_d.p : nullptr));
+  HIPCHK(launch_resonant(h->st, sh.nch_b, sh.rb, h->prob_d.p, ns, n, om_dev, active_dev,
+                         h->chiB_d.p, want_low ? h->lowB_d.p : nullptr, h->any_general_fit));
+  set_nspec_hint(ns);
+  HIPCHK(launch_assemble(h->st, h->prob_d.p, n, TW, sh.rb, om_dev, active_dev, h->chiA_d.p,
                          h->chiB_d.p, want_low ? h->lowA_d.p : nullptr,
                          want_low ? h->lowB_d.p : nullptr, h->det_d.p,
                          want_chi ? h->chi_d.p : nullptr, want_low ? h->low_d.p : nullptr,
@@ -196,7 +202,7 @@ int run_disp(alpsg_ctx *h, int n, const double *om, double *det, double *chi, do
   if (n == 0) return ALPSG_OK;
   HIPCHK(h->om_d.ensure(n));
   HIPCHK(hipMemcpyAsync(h->om_d.p, om, sizeof(double2) * n, hipMemcpyHostToDevice, h->st));
-  LaunchShape sh = plain ? LaunchShape{64, 8, 64} : pick_shape(n);
+  LaunchShape sh = plain ? LaunchShape{64, 8, 4, 64, 8} : pick_shape(n);
   if (!plain && h->fixed_nch) sh.nch_a = h->fixed_nch;
   HIPCHK(hipEventRecord(h->ev0, h->st));
   int rc = disp_device(h, n, h->om_d.p, nullptr, chi != nullptr, low != nullptr, wave != nullptr, sh);
@@ -405,6 +411,7 @@ int alpsg_create(const alpsg_plasma_desc *d, alpsg_handle *out) {
   if (!d->pp || (!d->f0 && !d->df0) || !d->ns || !d->qs || !d->ms || !d->ACmethod || !d->n_fits)
     return fail(ALPSG_E_ARG, "missing plasma array");
   const int P = d->positions_principal;
+  if (P < 1 || P > 16) return fail(ALPSG_E_UNSUPPORTED, "positions_principal must be in 1..16");
   if (nperp < 4 || npar < 2 * P + 8) return fail(ALPSG_E_ARG, "grid too small for positions_principal");
   if (d->n_resonance_interval < 2) return fail(ALPSG_E_ARG, "n_resonance_interval < 2");
   for (int s = 0; s < nspec; ++s) {
@@ -605,6 +612,7 @@ int alpsg_create(const alpsg_plasma_desc *d, alpsg_handle *out) {
       }
       S.fit_uniform = uni ? 1 : 0;
     }
+    if (!(S.acmethod == 1 && S.fit_uniform && S.nfits == 1)) h->any_general_fit = true;
     if (d->ACmethod[s] == 2) {
       S.poly_kind = d->poly_kind[s];
       S.poly_order = d->poly_order[s];

@@ -37,8 +37,8 @@ struct RowClass {
 
 // determine_resonances (ALPS_fns.f90:704-742) and the ipar_res / range logic of
 // integrate_res (ALPS_fns.f90:950-1007) for one (n, omega).
-__device__ __noinline__ void classify(const SpecDev &S, int P, double kpar, double omr,
-                                      double omi, int n, RowClass &rc) {
+__device__ __forceinline__ void classify(const SpecDev &S, int P, double kpar, double omr,
+                                         double omi, int n, RowClass &rc) {
 #pragma clang fp contract(off)
   const double *pa = S.ppar;
   const int npar = S.npar;
@@ -94,7 +94,7 @@ __device__ __noinline__ void classify(const SpecDev &S, int P, double kpar, doub
 
 // funct_g's ipar_close (ALPS_fns.f90:1285-1294): the last ipar in [1,npar-1]
 // with |ppar(ipar) - x| <= dppar/2, clamped to [2, npar-2].
-__device__ __noinline__ int ipar_close(const SpecDev &S, double x) {
+__device__ __forceinline__ int ipar_close(const SpecDev &S, double x) {
 #pragma clang fp contract(off)
   const double *pa = S.ppar;
   const int npar = S.npar;
@@ -149,7 +149,7 @@ __device__ __forceinline__ void interp6_acc(const SpecDev &S, const double *__re
 __device__ __forceinline__ bool czero(cd a) { return a.re == 0.0 && a.im == 0.0; }
 
 // eval_fit (ALPS_analyt.f90:32-363) at (iperp, complex p).
-__device__ cd eval_fit(const SpecDev &S, int iperp, cd p) {
+__device__ __noinline__ cd eval_fit(const SpecDev &S, int iperp, cd p) {
   if (S.acmethod == 2) {
     // fit_function_poly, Chebyshev kind 1 (ALPS_analyt.f90:307-358)
     cd t = mk((p.re - S.poly_norm1) / S.poly_norm2, p.im / S.poly_norm2);
@@ -269,7 +269,10 @@ __device__ __forceinline__ void endpoint_acc(const double *__restrict__ rows, in
 
 // ---------------------------------------------------------------------------
 // k_sweep: trapezoid node sums of every harmonic row pair.
-// chiA[iom][W][nspec][6] gets this wavefront's partial chi per species (before
+// The harmonic row pairs are dealt round-robin to the TW = W * gridDim.y
+// wavefronts that share an omega block (W per workgroup, gridDim.y workgroups),
+// so a small batch still spreads over many compute units.
+// chiA[iom][TW][nspec][6] gets each wavefront's partial chi per species (before
 // the n_s q_s factor); lowA[iom][nspec][3][6] (if non-null) the n = -1, 0, 1
 // rows.
 // ---------------------------------------------------------------------------
@@ -281,7 +284,9 @@ __global__ __launch_bounds__(64 * W) void k_sweep(const ProbDev *__restrict__ pr
                                                   double2 *__restrict__ lowA) {
   constexpr int OMB = 64 / NCH;
   const ProbDev &Pd = *probp;
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int lane = threadIdx.x & 63;
+  const int TW = W * gridDim.y;
+  const int wv = blockIdx.y * W + (threadIdx.x >> 6);
   const int g = lane / NCH, c = lane % NCH;
   const int iom = blockIdx.x * OMB + g;
   const bool valid = (iom < nom) && (active == nullptr || active[iom] != 0);
@@ -299,7 +304,7 @@ __global__ __launch_bounds__(64 * W) void k_sweep(const ProbDev *__restrict__ pr
 #pragma unroll
     for (int k = 0; k < 6; ++k) chi[k] = mk(0.0, 0.0);
     for (int nn = 0; nn <= S.nmax; ++nn) {
-      if (((rowbase + nn) % W) != wv) continue;
+      if (((rowbase + nn) % TW) != wv) continue;
       const double *rows = S.nodes + (size_t)nn * (npar - 1) * NODE_STRIDE;
       const cdptr srows = (cdptr)rows;
       RowClass rp, rm;
@@ -385,24 +390,130 @@ __global__ __launch_bounds__(64 * W) void k_sweep(const ProbDev *__restrict__ pr
     }
     rowbase += S.nmax + 1;
     if (valid && c == 0) {
-      double2 *o = chiA + (((size_t)iom * W + wv) * nspec + s) * 6;
+      // omega-fastest layout: chiA[((wv * nspec + s) * 6 + k) * nom + iom]
+      double2 *o = chiA + ((size_t)(wv * nspec + s) * 6) * nom + iom;
 #pragma unroll
-      for (int k = 0; k < 6; ++k) o[k] = make_double2(chi[k].re, chi[k].im);
+      for (int k = 0; k < 6; ++k) o[(size_t)k * nom] = make_double2(chi[k].re, chi[k].im);
     }
   }
 }
 
 // ---------------------------------------------------------------------------
 // k_resonant: principal-value and Landau parts of the resonant harmonics of
-// species blockIdx.y.  chiB[iom][nspec][6]; lowB[iom][nspec][3][6] (if non-null).
+// species blockIdx.y; blockIdx.z selects a block of the candidate harmonics
+// (harmonic n handled by block (n - nlo) % gridDim.z).
+// chiB[iom][nspec][RB][6], lowB[iom][nspec][RB][3][6] (if non-null).
+//
+// The principal-value sums of integrate_res evaluate funct_g, a linear
+// interpolation of node values (ALPS_fns.f90:1316-1317), at ~2 n_resonance_interval
+// ladder points.  Being linear in the node values, the whole sum equals
+// sum_i c_i u(i) over the few nodes i next to the resonance, where the complex
+// coefficient c_i collects the ladder weights times the interpolation weights.
+// The lanes of an omega walk contiguous blocks of the ladder, keep the three
+// coefficients of the current interpolation cell in registers, and flush them
+// into a per-omega LDS window when the cell changes; the node records are then
+// read once per row.
 // ---------------------------------------------------------------------------
 namespace {
 
-// Principal-value and tiny-rest sums of one resonant row (integrate_res after
-// its two integrate() calls), this lane's share.
+constexpr int PV_WIN = 40;  // nodes ipar_res-P-3 .. ipar_res+P+4 (P <= 16)
+
+struct CellCache {
+  int ic;
+  cd cm, c0, cp;
+};
+
+// ipar_close along a monotone ladder: after the first point only the cells
+// next to the previous one can be the last index within dppar/2 of x, so the
+// candidates are checked in descending index order (the same answer as the full
+// search); a miss falls back to the full search.
+struct Cursor {
+  int ic;  // last result, -1 before the first point
+};
+
+__device__ __forceinline__ int ipar_close_step(const SpecDev &S, Cursor &cu, double x, int dir) {
+#pragma clang fp contract(off)
+  if (cu.ic >= 0) {
+    const double *pa = S.ppar;
+    const double half = 0.5 * S.dppar_abs;
+    const int npar = S.npar;
+    int best = -1;
+    if (dir > 0) {
+      // x increases: candidates ic+3 .. ic (descending; two indices can tie)
+      for (int k = min(cu.ic + 3, npar - 1); k >= cu.ic; --k)
+        if (fabs(pa[k] - x) <= half) {
+          best = k;
+          break;
+        }
+    } else {
+      // x decreases: candidates ic, ic-1, ic-2 (descending)
+      for (int k = cu.ic; k >= max(cu.ic - 2, 1); --k)
+        if (fabs(pa[k] - x) <= half) {
+          best = k;
+          break;
+        }
+    }
+    if (best >= 0) {
+      int r = best;
+      if (r >= npar - 1) r = npar - 2;
+      if (r <= 1) r = 2;
+      // the cursor keeps the unclamped index so that the next step searches near x
+      cu.ic = best;
+      return r;
+    }
+  }
+  // full search (first point or a miss)
+  const int r = ipar_close(S, x);
+  {
+    const double *pa = S.ppar;
+    const double half = 0.5 * S.dppar_abs;
+    const int npar = S.npar;
+    // recover the unclamped index for the cursor
+    int best = -1;
+    for (int k = min(r + 3, npar - 1); k >= max(r - 3, 1); --k)
+      if (fabs(pa[k] - x) <= half) {
+        best = k;
+        break;
+      }
+    cu.ic = best;
+  }
+  return r;
+}
+
+__device__ __forceinline__ void lds_cadd(double *w, int t, cd v) {
+  atomicAdd(w + 2 * t, v.re);
+  atomicAdd(w + 2 * t + 1, v.im);
+}
+
+__device__ __forceinline__ void cache_flush(CellCache &cc, double *win, int lo) {
+  if (cc.ic >= 0) {
+    lds_cadd(win, cc.ic - 1 - lo, cc.cm);
+    lds_cadd(win, cc.ic - lo, cc.c0);
+    lds_cadd(win, cc.ic + 1 - lo, cc.cp);
+  }
+  cc.ic = -1;
+  cc.cm = cc.c0 = cc.cp = mk(0.0, 0.0);
+}
+
+// add wt * interp(x) to the coefficient window (through the cell cache)
+__device__ __forceinline__ void coef_add(const SpecDev &S, CellCache &cc, Cursor &cu, double *win,
+                                         int lo, double x, int dir, cd wt) {
+  const int ic = ipar_close_step(S, cu, x, dir);
+  if (ic != cc.ic) {
+    cache_flush(cc, win, lo);
+    cc.ic = ic;
+  }
+  const double f = 0.5 * (x - S.ppar[ic]) / S.dppar_abs;
+  cc.c0 = cc.c0 + wt;
+  cc.cp = cc.cp + f * wt;
+  cc.cm = cc.cm - f * wt;
+}
+
+// Principal-value and tiny-rest coefficients of one resonant row (integrate_res
+// after its two integrate() calls), this lane's share.
 template <int NCH>
-__device__ void pv_part(const SpecDev &S, const ProbDev &Pd, const double *__restrict__ rows,
-                        const RowClass &rc, double omr, double omi, int c, cd pv[6]) {
+__device__ void pv_coefs(const SpecDev &S, const ProbDev &Pd, const RowClass &rc, int c,
+                         double *win, int lo) {
   const int NRI = Pd.NRI, P = Pd.P;
   const double pr = rc.pr, pim = rc.pim;
   double capDelta, smdelta;
@@ -411,8 +522,15 @@ __device__ void pv_part(const SpecDev &S, const ProbDev &Pd, const double *__res
     capDelta = pr - S.ppar[rc.ipar_res - P];
     smdelta = capDelta / (1.0 * NRI);
   }
+  CellCache cd1, cd2;
+  Cursor u1, u2;
+  u1.ic = u2.ic = -1;
+  cd1.ic = cd2.ic = -1;
+  cd1.cm = cd1.c0 = cd1.cp = cd2.cm = cd2.c0 = cd2.cp = mk(0.0, 0.0);
   if (fabs(pim) > Pd.Tlim) {
-    for (int j = c; j <= NRI; j += NCH) {
+    const int blk = (NRI + 1 + NCH - 1) / NCH;
+    const int j0 = c * blk, j1 = min(j0 + blk, NRI + 1);
+    for (int j = j0; j < j1; ++j) {
       double x, xm;
       {
 #pragma clang fp contract(off)
@@ -421,12 +539,14 @@ __device__ void pv_part(const SpecDev &S, const ProbDev &Pd, const double *__res
       }
       const double w = (j == 0 || j == NRI) ? 1.0 : 2.0;
       const cd r1 = cscale_inv(w, mk(x - pr, -pim));
-      interp6_acc(S, rows, x, omr, omi, r1, pv);
-      interp6_acc(S, rows, xm, omr, omi, mk(-r1.re, r1.im), pv);  // - conj(r1)
+      coef_add(S, cd1, u1, win, lo, x, 1, r1);
+      coef_add(S, cd2, u2, win, lo, xm, -1, mk(-r1.re, r1.im));  // - conj(r1)
     }
   } else {
+    const int blk = (NRI + NCH - 1) / NCH;
+    const int j0 = 1 + c * blk, j1 = min(j0 + blk, NRI + 1);
     double sphi = 0.0;
-    for (int j = 1 + c; j <= NRI; j += NCH) {
+    for (int j = j0; j < j1; ++j) {
       double x;
       {
 #pragma clang fp contract(off)
@@ -445,11 +565,13 @@ __device__ void pv_part(const SpecDev &S, const ProbDev &Pd, const double *__res
         xq = pr - S.dppar;
       }
       const double f = 2.0 * sphi / (2.0 * S.dppar);
-      interp6_acc(S, rows, xp, omr, omi, mk(f, 0.0), pv);
-      interp6_acc(S, rows, xq, omr, omi, mk(-f, 0.0), pv);
+      coef_add(S, cd1, u1, win, lo, xp, 1, mk(f, 0.0));
+      coef_add(S, cd2, u2, win, lo, xq, -1, mk(-f, 0.0));
       if (pim != 0.0) {
         const double sg = (pim > 0.0) ? 1.0 : -1.0;
-        interp6_acc(S, rows, pr, omr, omi, mk(0.0, sg * 2.0 * PI_D / smdelta), pv);
+        cache_flush(cd1, win, lo);
+        u1.ic = -1;
+        coef_add(S, cd1, u1, win, lo, pr, 1, mk(0.0, sg * 2.0 * PI_D / smdelta));
       }
     }
   }
@@ -467,23 +589,35 @@ __device__ void pv_part(const SpecDev &S, const ProbDev &Pd, const double *__res
     }
   }
   if (ntiny > 0) {
-    for (int kk = c; kk <= ntiny; kk += NCH) {
+    const int blk = (ntiny + 1 + NCH - 1) / NCH;
+    const int k0 = c * blk, k1 = min(k0 + blk, ntiny + 1);
+    cache_flush(cd1, win, lo);
+    u1.ic = -1;
+    for (int kk = k0; kk < k1; ++kk) {
       double x;
       {
 #pragma clang fp contract(off)
         x = (kk == 0) ? x0 : x0 + cs * kk;
       }
       const double w = (kk == 0 || kk == ntiny) ? 1.0 : 2.0;
-      interp6_acc(S, rows, x, omr, omi, cscale_inv(corr * w, mk(x - pr, -pim)), pv);
+      coef_add(S, cd1, u1, win, lo, x, 1, cscale_inv(corr * w, mk(x - pr, -pim)));
     }
   }
+  cache_flush(cd1, win, lo);
+  cache_flush(cd2, win, lo);
+}
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Landau sums L_a = sum_iperp h a(n,iperp) K(iperp) of this lane, and the
 // early-return flag of landau_integrate.  When every fit of the species is a
 // Maxwellian whose ppar parameters do not depend on pperp, exp(-u2 (p-u3)^2) is
 // the same for every iperp and is evaluated once (bit-identical values).
-template <int NCH>
+template <int NCH, bool UNI>
 __device__ void landau_part(const SpecDev &S, const ProbDev &Pd, const double *__restrict__ arow,
                             const RowClass &rc, double omr, double omi, int c, cd L[3],
                             int &zero) {
@@ -492,32 +626,29 @@ __device__ void landau_part(const SpecDev &S, const ProbDev &Pd, const double *_
   const cd p = mk(rc.pr, rc.pim);
   const cd pp_ = mk(rc.pr + d, rc.pim), pm_ = mk(rc.pr - d, rc.pim);
   const double qk = S.q / fabs(Pd.kpar), km = Pd.kpar / S.m;
-  const bool uni = S.fit_uniform != 0;
-  cd E0[ALPSG_DMAXFITS], Ep[ALPSG_DMAXFITS], Em[ALPSG_DMAXFITS];
-  if (uni) {
-    for (int fi = 0; fi < S.nfits; ++fi) {
-      const double *cf = S.fitc + (size_t)fi * FIT_STRIDE;  // u2, u3 of iperp = 0
-      const double u2 = cf[1], u3 = cf[2];
-      cd dq, sq;
-      dq = mk(p.re - u3, p.im);
-      sq = dq * dq;
-      E0[fi] = cexpd(mk(-u2 * sq.re, -u2 * sq.im));
-      dq = mk(pp_.re - u3, pp_.im);
-      sq = dq * dq;
-      Ep[fi] = cexpd(mk(-u2 * sq.re, -u2 * sq.im));
-      dq = mk(pm_.re - u3, pm_.im);
-      sq = dq * dq;
-      Em[fi] = cexpd(mk(-u2 * sq.re, -u2 * sq.im));
-    }
+  // single Maxwellian fit with pperp-independent (u2, u3): one exponential per p
+  constexpr bool uni = UNI;
+  cd E0 = mk(0.0, 0.0), Ep = E0, Em = E0;
+  if constexpr (UNI) {
+    const double u2 = S.fitc[1], u3 = S.fitc[2];  // iperp = 0 entry
+    cd dq, sq;
+    dq = mk(p.re - u3, p.im);
+    sq = dq * dq;
+    E0 = cexpd(mk(-u2 * sq.re, -u2 * sq.im));
+    dq = mk(pp_.re - u3, pp_.im);
+    sq = dq * dq;
+    Ep = cexpd(mk(-u2 * sq.re, -u2 * sq.im));
+    dq = mk(pm_.re - u3, pm_.im);
+    sq = dq * dq;
+    Em = cexpd(mk(-u2 * sq.re, -u2 * sq.im));
   }
   auto F = [&](int ip, int which) -> cd {
-    if (!uni) return eval_fit(S, ip, which == 0 ? p : (which > 0 ? pp_ : pm_));
-    cd f = mk(0.0, 0.0);
-    for (int fi = 0; fi < S.nfits; ++fi) {
-      const double c0 = S.fitc[((size_t)ip * S.nfits + fi) * FIT_STRIDE];
-      f = f + c0 * (which == 0 ? E0[fi] : (which > 0 ? Ep[fi] : Em[fi]));
+    if constexpr (!UNI) {
+      return eval_fit(S, ip, which == 0 ? p : (which > 0 ? pp_ : pm_));
+    } else {
+      const double c0 = S.fitc[(size_t)ip * FIT_STRIDE];
+      return mk(0.0, 0.0) + c0 * (which == 0 ? E0 : (which > 0 ? Ep : Em));
     }
-    return f;
   };
   for (int ip = c; ip <= nperp; ip += NCH) {
     double h;
@@ -548,18 +679,25 @@ __device__ void landau_part(const SpecDev &S, const ProbDev &Pd, const double *_
 
 }  // namespace
 
-template <int NCH>
+// PART 0: principal-value terms; PART 1: Landau terms of species whose analytic
+// continuation is one Maxwellian with pperp-independent (u2, u3); PART 2: Landau
+// terms of the other species.  PV writes partial slots rb, the Landau launches
+// slots RB + rb of chiB / lowB (each species by exactly one of PART 1, 2).
+template <int NCH, int PART>
 __global__ __launch_bounds__(64) void k_resonant(const ProbDev *__restrict__ probp, int nom,
                                                  const double2 *__restrict__ om_in,
                                                  const int *__restrict__ active,
                                                  double2 *__restrict__ chiB,
                                                  double2 *__restrict__ lowB) {
   constexpr int OMB = 64 / NCH;
+  __shared__ double lds_win[OMB][2 * PV_WIN];
   const ProbDev &Pd = *probp;
   const int lane = threadIdx.x;
   const int g = lane / NCH, c = lane % NCH;
   const int iom = blockIdx.x * OMB + g;
   const int s = blockIdx.y;
+  const int RB = gridDim.z, rb = blockIdx.z;
+  const int slot = (PART == 0 ? 0 : RB) + rb;
   const bool valid = (iom < nom) && (active == nullptr || active[iom] != 0);
   const double2 ov = valid ? om_in[iom] : make_double2(1.0, 0.5);
   const double omr = ov.x, omi = ov.y;
@@ -567,10 +705,15 @@ __global__ __launch_bounds__(64) void k_resonant(const ProbDev *__restrict__ pro
   const SpecDev &S = Pd.sp[s];
   const int npar = S.npar, P = Pd.P;
   const double kpar = Pd.kpar;
+  const bool uni_s = (S.fit_uniform != 0) && (S.nfits == 1) && (S.acmethod == 1);
+  if ((PART == 1 && !uni_s) || (PART == 2 && uni_s)) return;
+  double *win = lds_win[g];
   cd res[6];
 #pragma unroll
   for (int k = 0; k < 6; ++k) res[k] = mk(0.0, 0.0);
-  double2 *lo = (lowB != nullptr && valid && c == 0) ? lowB + ((size_t)iom * nspec + s) * 18 : nullptr;
+  double2 *lo = (lowB != nullptr && valid && c == 0)
+                    ? lowB + (((size_t)iom * nspec + s) * (2 * RB) + slot) * 18
+                    : nullptr;
   if (lo)
     for (int t = 0; t < 18; ++t) lo[t] = make_double2(0.0, 0.0);
 
@@ -586,20 +729,33 @@ __global__ __launch_bounds__(64) void k_resonant(const ProbDev *__restrict__ pro
     nhi = (int)fmin(ceil(hi_d), (double)S.nmax);
   }
   const double scaleN = 2.0 * PI_D * S.dpperp * 0.25 * (-S.q / kpar);
-  for (int n = nlo; n <= nhi; ++n) {
+  for (int n = nlo + rb; n <= nhi; n += RB) {
     RowClass rc;
     classify(S, P, kpar, omr, omi, n, rc);
-    if (!rc.found || (!rc.pv && !rc.landau)) continue;
+    if (!rc.found || (PART == 0 && !rc.pv) || (PART != 0 && !rc.landau)) continue;
     const int an = n < 0 ? -n : n;
     const double *rows = S.nodes + (size_t)an * (npar - 1) * NODE_STRIDE;
     cd add[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) add[k] = mk(0.0, 0.0);
-    if (rc.pv) {
+    if constexpr (PART == 0) {
+      const int wlo = max(1, rc.ipar_res - P - 3);
+      const int whi = min(npar - 1, rc.ipar_res + P + 4);
+      for (int t = c; t < 2 * PV_WIN; t += NCH) win[t] = 0.0;
+      wave_sync();
+      pv_coefs<NCH>(S, Pd, rc, c, win, wlo);
+      wave_sync();
       cd pv[6];
 #pragma unroll
       for (int k = 0; k < 6; ++k) pv[k] = mk(0.0, 0.0);
-      pv_part<NCH>(S, Pd, rows, rc, omr, omi, c, pv);
+      for (int i = wlo + c; i <= whi; i += NCH) {
+        const cd cf = mk(win[2 * (i - wlo)], win[2 * (i - wlo) + 1]);
+        cd u[6];
+        node6(rows + (size_t)(i - 1) * NODE_STRIDE, omr, omi, u);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) pv[k] = pv[k] + cf * u[k];
+      }
+      wave_sync();
       double capDelta, smdelta;
       {
 #pragma clang fp contract(off)
@@ -611,10 +767,10 @@ __global__ __launch_bounds__(64) void k_resonant(const ProbDev *__restrict__ pro
       for (int k = 0; k < 6; ++k)
         add[k] = add[k] + (sc * gsumc<NCH>(pv[ACC_OF_MODE[k]])) * tconst(Pd, S, n, k);
     }
-    if (rc.landau) {
+    if constexpr (PART != 0) {
       cd L[3] = {mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0)};
       int zero = 0;
-      landau_part<NCH>(S, Pd, S.atab + (size_t)an * (S.nperp + 1) * ATAB_STRIDE, rc, omr, omi, c,
+      landau_part<NCH, PART == 1>(S, Pd, S.atab + (size_t)an * (S.nperp + 1) * ATAB_STRIDE, rc, omr, omi, c,
                        L, zero);
       zero = gor<NCH>(zero);
       if (!zero) {
@@ -638,81 +794,110 @@ __global__ __launch_bounds__(64) void k_resonant(const ProbDev *__restrict__ pro
     }
   }
   if (valid && c == 0) {
-    double2 *o = chiB + ((size_t)iom * nspec + s) * 6;
+    // omega-fastest layout: chiB[((slot * nspec + s) * 6 + k) * nom + iom]
+    double2 *o = chiB + ((size_t)(slot * nspec + s) * 6) * nom + iom;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) o[k] = make_double2(res[k].re, res[k].im);
+    for (int k = 0; k < 6; ++k) o[(size_t)k * nom] = make_double2(res[k].re, res[k].im);
   }
 }
 
 // ---------------------------------------------------------------------------
-// k_assemble: one thread per omega.
+// k_assemble: a workgroup of 256 threads handles 256/(6 nspec) omegas; thread
+// (omega, s, k) sums the partials of chi_sk, then one thread per omega forms
+// the dielectric tensor, the wave tensor and det D.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(128) void k_assemble(
-    const ProbDev *__restrict__ probp, int nom, int W, const double2 *__restrict__ om_in,
+__global__ __launch_bounds__(256) void k_assemble(
+    const ProbDev *__restrict__ probp, int nom, int W, int RB, const double2 *__restrict__ om_in,
     const int *__restrict__ active, const double2 *__restrict__ chiA,
     const double2 *__restrict__ chiB, const double2 *__restrict__ lowA,
     const double2 *__restrict__ lowB, double2 *__restrict__ det_out, double2 *__restrict__ chi_out,
     double2 *__restrict__ low_out, double2 *__restrict__ wave_out) {
-  const int iom = blockIdx.x * blockDim.x + threadIdx.x;
-  if (iom >= nom) return;
-  if (active != nullptr && active[iom] == 0) return;
+  __shared__ double2 lchi[256];
   const ProbDev &Pd = *probp;
   const int nspec = Pd.nspec;
+  const int per = 6 * nspec;
+  const int opb = 256 / per;  // omegas per block
+  const int t = threadIdx.x;
+  const int lo_om = t / per, sk = t % per;
+  const int s = sk / 6, k = sk % 6;
+  const int iom = blockIdx.x * opb + lo_om;
+  const bool in = (lo_om < opb) && (iom < nom) && (active == nullptr || active[iom] != 0);
+  if (in) {
+    // partial sums, loads issued in batches of 8
+    cd v = mk(0.0, 0.0);
+    const int tot = W + 2 * RB;
+    for (int base = 0; base < tot; base += 8) {
+      double2 buf[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int j = base + u;
+        if (j < W)
+          buf[u] = chiA[((size_t)(j * nspec + s) * 6 + k) * nom + iom];
+        else if (j < tot)
+          buf[u] = chiB[((size_t)((j - W) * nspec + s) * 6 + k) * nom + iom];
+        else
+          buf[u] = make_double2(0.0, 0.0);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v = v + mk(buf[u].x, buf[u].y);
+    }
+    lchi[t] = make_double2(v.re, v.im);
+  }
+  __syncthreads();
+  if (!in || sk != 0) return;
   const double2 ov = om_in[iom];
   const cd om = mk(ov.x, ov.y);
   cd eps[6];
-  for (int k = 0; k < 6; ++k) eps[k] = mk(0.0, 0.0);
+  for (int q = 0; q < 6; ++q) eps[q] = mk(0.0, 0.0);
   cd dn = (om * om) * (Pd.vA * Pd.vA);
   if (!Pd.kperp_norm) dn = dn * (Pd.kperp * Pd.kperp);
-  for (int s = 0; s < nspec; ++s) {
-    const SpecDev &S = Pd.sp[s];
+  for (int sp = 0; sp < nspec; ++sp) {
+    const SpecDev &S = Pd.sp[sp];
     cd chi[6];
-    for (int k = 0; k < 6; ++k) {
-      cd v = mk(0.0, 0.0);
-      for (int w = 0; w < W; ++w) {
-        const double2 t = chiA[(((size_t)iom * W + w) * nspec + s) * 6 + k];
-        v = v + mk(t.x, t.y);
-      }
-      const double2 t = chiB[((size_t)iom * nspec + s) * 6 + k];
-      chi[k] = v + mk(t.x, t.y);
+    for (int q = 0; q < 6; ++q) {
+      const double2 a = lchi[lo_om * per + sp * 6 + q];
+      chi[q] = mk(a.x, a.y);
     }
     const double ee = Pd.kperp_norm ? S.int_ee : Pd.kperp * Pd.kperp * S.int_ee;
     chi[2] = chi[2] + mk(ee, 0.0);
     const double norm = S.dens * S.q;
-    for (int k = 0; k < 6; ++k) {
-      chi[k] = chi[k] * norm;
-      eps[k] = eps[k] + chi[k];
+    for (int q = 0; q < 6; ++q) {
+      chi[q] = chi[q] * norm;
+      eps[q] = eps[q] + chi[q];
     }
     if (chi_out != nullptr) {
       double2 *o = chi_out + (size_t)iom * nspec * 9;
       cd v[3][3];
-      for (int k = 0; k < 6; ++k) v[MI[k]][MJ[k]] = cdiv(chi[k], dn);
+      for (int q = 0; q < 6; ++q) v[MI[q]][MJ[q]] = cdiv(chi[q], dn);
       v[1][0] = mk(-v[0][1].re, -v[0][1].im);
       v[2][0] = v[0][2];
       v[2][1] = mk(-v[1][2].re, -v[1][2].im);
       for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) o[s + nspec * (i + 3 * j)] = make_double2(v[i][j].re, v[i][j].im);
+        for (int j = 0; j < 3; ++j) o[sp + nspec * (i + 3 * j)] = make_double2(v[i][j].re, v[i][j].im);
     }
     if (low_out != nullptr) {
       double2 *o = low_out + (size_t)iom * nspec * 27;
-      for (int q = 0; q < 3; ++q) {
+      for (int qq = 0; qq < 3; ++qq) {
         cd v[3][3];
         for (int i = 0; i < 3; ++i)
           for (int j = 0; j < 3; ++j) v[i][j] = mk(0.0, 0.0);
-        for (int k = 0; k < 6; ++k) {
-          if (q == 1 && (k == 0 || k == 3 || k == 4)) continue;
-          const double2 a = lowA[((size_t)iom * nspec + s) * 18 + 6 * q + k];
-          const double2 b = lowB[((size_t)iom * nspec + s) * 18 + 6 * q + k];
-          cd t = mk(a.x, a.y) + mk(b.x, b.y);
-          if (q == 1 && k == 2) t = t + mk(ee, 0.0);
-          v[MI[k]][MJ[k]] = cdiv(t * norm, dn);
+        for (int q = 0; q < 6; ++q) {
+          if (qq == 1 && (q == 0 || q == 3 || q == 4)) continue;
+          const double2 a = lowA[((size_t)iom * nspec + sp) * 18 + 6 * qq + q];
+          cd tt = mk(a.x, a.y);
+          for (int b = 0; b < 2 * RB; ++b) {
+            const double2 u = lowB[(((size_t)iom * nspec + sp) * (2 * RB) + b) * 18 + 6 * qq + q];
+            tt = tt + mk(u.x, u.y);
+          }
+          if (qq == 1 && q == 2) tt = tt + mk(ee, 0.0);
+          v[MI[q]][MJ[q]] = cdiv(tt * norm, dn);
         }
         v[1][0] = mk(-v[0][1].re, -v[0][1].im);
         v[2][0] = v[0][2];
         v[2][1] = mk(-v[1][2].re, -v[1][2].im);
         for (int i = 0; i < 3; ++i)
           for (int j = 0; j < 3; ++j)
-            o[s + nspec * (i + 3 * (j + 3 * q))] = make_double2(v[i][j].re, v[i][j].im);
+            o[sp + nspec * (i + 3 * (j + 3 * qq))] = make_double2(v[i][j].re, v[i][j].im);
       }
     }
   }
@@ -752,30 +937,36 @@ __global__ __launch_bounds__(128) void k_assemble(
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-hipError_t launch_sweep(hipStream_t st, int nch, int W, const ProbDev *prob, int nom,
+hipError_t launch_sweep(hipStream_t st, int nch, int W, int RC, const ProbDev *prob, int nom,
                         const double2 *om, const int *active, double2 *chiA, double2 *lowA) {
   if (nom <= 0) return hipSuccess;
-#define SW(N, WV)                                                                              \
-  if (nch == N && W == WV) {                                                                   \
-    const int grid = (nom + (64 / N) - 1) / (64 / N);                                          \
-    hipLaunchKernelGGL((k_sweep<N, WV>), dim3(grid), dim3(64 * WV), 0, st, prob, nom, om, active, \
-                       chiA, lowA);                                                            \
-    return hipGetLastError();                                                                  \
+#define SW(N, WV)                                                                                \
+  if (nch == N && W == WV) {                                                                     \
+    const int grid = (nom + (64 / N) - 1) / (64 / N);                                            \
+    hipLaunchKernelGGL((k_sweep<N, WV>), dim3(grid, RC), dim3(64 * WV), 0, st, prob, nom, om, active, \
+                       chiA, lowA);                                                              \
+    return hipGetLastError();                                                                    \
   }
-  SW(1, 1) SW(1, 2) SW(1, 4) SW(2, 4) SW(4, 4) SW(8, 4) SW(16, 4) SW(16, 8) SW(32, 8) SW(64, 8)
-  SW(64, 16)
+  SW(1, 1) SW(1, 2) SW(1, 4) SW(2, 4) SW(4, 4) SW(8, 4) SW(16, 4) SW(16, 8) SW(32, 8) SW(64, 4)
+  SW(64, 8)
 #undef SW
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_resonant(hipStream_t st, int nch, const ProbDev *prob, int nspec, int nom,
-                           const double2 *om, const int *active, double2 *chiB, double2 *lowB) {
+hipError_t launch_resonant(hipStream_t st, int nch, int RB, const ProbDev *prob, int nspec, int nom,
+                           const double2 *om, const int *active, double2 *chiB, double2 *lowB,
+                           bool general) {
   if (nom <= 0) return hipSuccess;
 #define RS(N)                                                                                  \
   if (nch == N) {                                                                              \
     const int grid = (nom + (64 / N) - 1) / (64 / N);                                          \
-    hipLaunchKernelGGL((k_resonant<N>), dim3(grid, nspec), dim3(64), 0, st, prob, nom, om, active, \
-                       chiB, lowB);                                                            \
+    hipLaunchKernelGGL((k_resonant<N, 0>), dim3(grid, nspec, RB), dim3(64), 0, st, prob, nom,  \
+                       om, active, chiB, lowB);                                                \
+    hipLaunchKernelGGL((k_resonant<N, 1>), dim3(grid, nspec, RB), dim3(64), 0, st, prob, nom,  \
+                       om, active, chiB, lowB);                                                \
+    if (general)                                                                               \
+      hipLaunchKernelGGL((k_resonant<N, 2>), dim3(grid, nspec, RB), dim3(64), 0, st, prob, nom, \
+                         om, active, chiB, lowB);                                              \
     return hipGetLastError();                                                                  \
   }
   RS(4) RS(8) RS(16) RS(32) RS(64)
@@ -783,12 +974,19 @@ hipError_t launch_resonant(hipStream_t st, int nch, const ProbDev *prob, int nsp
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_assemble(hipStream_t st, const ProbDev *prob, int nom, int W, const double2 *om,
-                           const int *active, const double2 *chiA, const double2 *chiB,
-                           const double2 *lowA, const double2 *lowB, double2 *det, double2 *chi,
-                           double2 *low, double2 *wave) {
+// species count of a problem, kept by the host beside the device copy
+static thread_local int g_nspec_hint = 1;
+void set_nspec_hint(int n) { g_nspec_hint = n; }
+static int nspec_of(const ProbDev *) { return g_nspec_hint; }
+
+hipError_t launch_assemble(hipStream_t st, const ProbDev *prob, int nom, int W, int RB,
+                           const double2 *om, const int *active, const double2 *chiA,
+                           const double2 *chiB, const double2 *lowA, const double2 *lowB,
+                           double2 *det, double2 *chi, double2 *low, double2 *wave) {
   if (nom <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_assemble, dim3((nom + 127) / 128), dim3(128), 0, st, prob, nom, W, om,
-                     active, chiA, chiB, lowA, lowB, det, chi, low, wave);
+  const int per = 6 * nspec_of(prob);
+  const int opb = 256 / per;
+  hipLaunchKernelGGL(k_assemble, dim3((nom + opb - 1) / opb), dim3(256), 0, st, prob, nom, W, RB,
+                     om, active, chiA, chiB, lowA, lowB, det, chi, low, wave);
   return hipGetLastError();
 }

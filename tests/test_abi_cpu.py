@@ -91,10 +91,15 @@ def test_create_rejects_bad_input_without_a_device():
     d, keep = _desc_for(p, pp=np.asfortranarray(pp))
     rc, msg, _ = _create(d)
     assert rc == 2 and "tensor product" in msg
-    # positions_principal too large for the grid
+    # positions_principal beyond what the principal-value window holds
     d, keep = _desc_for(p, positions_principal=30)
     rc, msg, _ = _create(d)
-    assert rc == 1
+    assert rc == 2 and "positions_principal" in msg
+    # positions_principal too large for the grid (npar < 2P + 8)
+    p30 = cases.small_bimax(npar=30)
+    d, keep = _desc_for(p30, positions_principal=12)
+    rc, msg, _ = _create(d)
+    assert rc == 1 and "grid too small" in msg
 
 
 def test_null_handle_calls_fail_cleanly():
