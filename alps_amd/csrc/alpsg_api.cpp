@@ -40,8 +40,7 @@ hipError_t launch_sweep(hipStream_t st, int nch, int W, int RC, const ProbDev *p
 hipError_t launch_resonant(hipStream_t st, int nch, int RB, const ProbDev *prob, int nspec, int nom,
                            const double2 *om, const int *active, double2 *chiB, double2 *lowB,
                            bool general);
-void set_nspec_hint(int n);
-hipError_t launch_assemble(hipStream_t st, const ProbDev *prob, int nom, int W, int RB,
+hipError_t launch_assemble(hipStream_t st, const ProbDev *prob, int nspec, int nom, int W, int RB,
                            const double2 *om, const int *active, const double2 *chiA,
                            const double2 *chiB, const double2 *lowA, const double2 *lowB,
                            double2 *det, double2 *chi, double2 *low, double2 *wave);
@@ -88,6 +87,7 @@ struct DevBuf {
 struct alpsg_ctx {
   hipStream_t st = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev_sweep = nullptr, ev_res = nullptr;  // recorded after k_sweep, after k_resonant
   int nspec = 0, nperp = 0, npar = 0;
   int kperp_norm = 1, P = 5, NRI = 100;
   double vA = 0, Tlim = 0.01, Bessel_zero = 1e-45;
@@ -96,6 +96,7 @@ struct alpsg_ctx {
   int nmax[ALPSG_MAXSPEC] = {0};
   long long evals = 0;
   double last_ms = 0.0;
+  double last_split[3] = {0.0, 0.0, 0.0};  // k_sweep, k_resonant (all parts), k_assemble
   // host copies
   std::vector<double> qs, ms, ns, pperp_h, ppar_h;
   // static device data
@@ -111,9 +112,12 @@ struct alpsg_ctx {
   // batch staging
   DevBuf<double2> om_d, det_d, chi_d, low_d, wave_d, chiA_d, chiB_d, lowA_d, lowB_d;
   DevBuf<int> act_d;
-  int fixed_nch = 0;  // 0 = heuristic / tuned
   std::vector<double> root_ms;  // completion time of each root of the last secant batch
   bool any_general_fit = false;  // a species needs eval_fit in the Landau term
+  // launch shapes chosen by alpsg_autotune (or pinned by alpsg_set_shape),
+  // keyed by the batch-size bucket floor(log2(n))
+  int tuned_valid[32] = {0};
+  int tuned_shape[32][5] = {{0}};
 
   ~alpsg_ctx() {
     if (st) (void)hipStreamSynchronize(st);
@@ -139,6 +143,8 @@ struct alpsg_ctx {
     act_d.release();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    if (ev_sweep) (void)hipEventDestroy(ev_sweep);
+    if (ev_res) (void)hipEventDestroy(ev_res);
     if (st) (void)hipStreamDestroy(st);
   }
 };
@@ -158,6 +164,12 @@ struct LaunchShape {
   int nch_a, w_a, rc_a, nch_b, rb;
 };
 
+int size_bucket(int n) {
+  int b = 0;
+  while ((2 << b) <= n && b < 31) ++b;
+  return b;
+}
+
 LaunchShape pick_shape(int n) {
   if (n >= 32768) return {1, 4, 1, 16, 1};
   if (n >= 8192) return {2, 4, 1, 16, 1};
@@ -165,6 +177,27 @@ LaunchShape pick_shape(int n) {
   if (n >= 256) return {16, 8, 1, 32, 2};
   if (n >= 32) return {32, 8, 2, 64, 4};
   return {64, 8, 4, 64, 8};
+}
+
+LaunchShape shape_for(const alpsg_ctx *h, int n) {
+  const int b = size_bucket(n);
+  if (h->tuned_valid[b]) {
+    const int *t = h->tuned_shape[b];
+    return LaunchShape{t[0], t[1], t[2], t[3], t[4]};
+  }
+  return pick_shape(n);
+}
+
+// (lanes per omega, wavefronts per workgroup) pairs instantiated for k_sweep
+const int kSweepVariants[][2] = {{1, 1}, {1, 2}, {1, 4}, {2, 4}, {4, 4}, {8, 4},
+                                 {16, 4}, {16, 8}, {32, 8}, {64, 4}, {64, 8}};
+const int kResonantLanes[] = {4, 8, 16, 32, 64};
+
+bool shape_is_valid(const int *t) {
+  bool a = false, b = false;
+  for (auto &v : kSweepVariants) a = a || (v[0] == t[0] && v[1] == t[1]);
+  for (int v : kResonantLanes) b = b || (v == t[3]);
+  return a && b && t[2] >= 1 && t[2] <= 64 && t[4] >= 1 && t[4] <= 16 && t[1] * t[2] <= 128;
 }
 
 // Runs the disp kernels on n omegas already resident in om_dev.
@@ -184,10 +217,11 @@ int disp_device(alpsg_ctx *h, int n, const double2 *om_dev, const int *active_de
   if (want_wave) HIPCHK(h->wave_d.ensure((size_t)n * 9));
   HIPCHK(launch_sweep(h->st, sh.nch_a, sh.w_a, sh.rc_a, h->prob_d.p, n, om_dev, active_dev,
                       h->chiA_d.p, want_low ? h->lowA_d.p : nullptr));
+  HIPCHK(hipEventRecord(h->ev_sweep, h->st));
   HIPCHK(launch_resonant(h->st, sh.nch_b, sh.rb, h->prob_d.p, ns, n, om_dev, active_dev,
                          h->chiB_d.p, want_low ? h->lowB_d.p : nullptr, h->any_general_fit));
-  set_nspec_hint(ns);
-  HIPCHK(launch_assemble(h->st, h->prob_d.p, n, TW, sh.rb, om_dev, active_dev, h->chiA_d.p,
+  HIPCHK(hipEventRecord(h->ev_res, h->st));
+  HIPCHK(launch_assemble(h->st, h->prob_d.p, ns, n, TW, sh.rb, om_dev, active_dev, h->chiA_d.p,
                          h->chiB_d.p, want_low ? h->lowA_d.p : nullptr,
                          want_low ? h->lowB_d.p : nullptr, h->det_d.p,
                          want_chi ? h->chi_d.p : nullptr, want_low ? h->low_d.p : nullptr,
@@ -195,15 +229,125 @@ int disp_device(alpsg_ctx *h, int n, const double2 *om_dev, const int *active_de
   return ALPSG_OK;
 }
 
+// Autotuner: times the node-sweep variants (lanes per omega x wavefronts per
+// workgroup x workgroups per omega block) and the resonant-kernel variants
+// (lanes per omega x harmonic blocks) separately on the batch, keeps the
+// fastest of each for the batch-size bucket of n.
+int autotune(alpsg_ctx *h, int n, const double2 *om_host, int *shape_out) {
+  if (!h->have_k) return fail(ALPSG_E_STATE, "alpsg_set_k has not been called");
+  if (n < 1) return fail(ALPSG_E_ARG, "n must be >= 1");
+  std::vector<double2> om(n);
+  if (om_host) {
+    for (int i = 0; i < n; ++i) om[i] = om_host[i];
+  } else {
+    // a spread of frequencies: resonant and non-resonant rows, both signs of Im
+    for (int i = 0; i < n; ++i) {
+      const double t = (i + 0.5) / n;
+      om[i] = make_double2(0.05 + 1.95 * t, -0.3 + 0.32 * std::fmod(7.0 * t, 1.0));
+    }
+  }
+  const int ns = h->nspec;
+  HIPCHK(h->om_d.ensure(n));
+  HIPCHK(hipMemcpyAsync(h->om_d.p, om.data(), sizeof(double2) * n, hipMemcpyHostToDevice, h->st));
+  const size_t budget = (size_t)512 << 20;
+  auto time_launch = [&](auto &&launch, float &best) -> int {
+    HIPCHK(launch());  // warm-up
+    float tot = 0.f;
+    for (int r = 0; r < 2; ++r) {
+      HIPCHK(hipEventRecord(h->ev0, h->st));
+      HIPCHK(launch());
+      HIPCHK(hipEventRecord(h->ev1, h->st));
+      HIPCHK(hipEventSynchronize(h->ev1));
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+      tot += ms;
+    }
+    best = tot / 2;
+    return ALPSG_OK;
+  };
+  std::vector<std::pair<float, LaunchShape>> candA, candB;
+  const int rcs[] = {1, 2, 3, 4, 6, 8, 12, 16};
+  for (auto &v : kSweepVariants) {
+    const int omb = 64 / v[0];
+    const int blocks = (n + omb - 1) / omb;
+    for (int rc : rcs) {
+      const int TW = v[1] * rc;
+      if (TW > 128) continue;
+      if ((size_t)n * TW * ns * 6 * sizeof(double2) > budget) continue;
+      if (rc > 1 && (size_t)blocks * rc > 65536) continue;
+      HIPCHK(h->chiA_d.ensure((size_t)n * TW * ns * 6));
+      float t;
+      int rc_ = time_launch(
+          [&]() {
+            return launch_sweep(h->st, v[0], v[1], rc, h->prob_d.p, n, h->om_d.p, nullptr,
+                                h->chiA_d.p, nullptr);
+          },
+          t);
+      if (rc_) return rc_;
+      candA.push_back({t, LaunchShape{v[0], v[1], rc, 0, 0}});
+    }
+  }
+  const int rbs[] = {1, 2, 4, 8};
+  for (int nb : kResonantLanes)
+    for (int rb : rbs) {
+      if ((size_t)n * ns * 2 * rb * 6 * sizeof(double2) > budget) continue;
+      HIPCHK(h->chiB_d.ensure((size_t)n * ns * 2 * rb * 6));
+      float t;
+      int rc_ = time_launch(
+          [&]() {
+            return launch_resonant(h->st, nb, rb, h->prob_d.p, ns, n, h->om_d.p, nullptr,
+                                   h->chiB_d.p, nullptr, h->any_general_fit);
+          },
+          t);
+      if (rc_) return rc_;
+      candB.push_back({t, LaunchShape{0, 0, 0, nb, rb}});
+    }
+  if (candA.empty() || candB.empty()) return fail(ALPSG_E_ARG, "batch too large to tune");
+  auto by_time = [](const std::pair<float, LaunchShape> &a, const std::pair<float, LaunchShape> &b) {
+    return a.first < b.first;
+  };
+  std::sort(candA.begin(), candA.end(), by_time);
+  std::sort(candB.begin(), candB.end(), by_time);
+  // the assemble pass reads W*RC + 2*RB partials per omega: settle the
+  // choice among the leaders with the three kernels timed together
+  LaunchShape best{};
+  float tbest = 1e30f;
+  for (size_t a = 0; a < std::min<size_t>(3, candA.size()); ++a)
+    for (size_t b = 0; b < std::min<size_t>(2, candB.size()); ++b) {
+      LaunchShape sh = candA[a].second;
+      sh.nch_b = candB[b].second.nch_b;
+      sh.rb = candB[b].second.rb;
+      float t;
+      int rc_ = time_launch(
+          [&]() -> hipError_t {
+            return disp_device(h, n, h->om_d.p, nullptr, false, false, false, sh) == ALPSG_OK
+                       ? hipSuccess
+                       : hipErrorUnknown;
+          },
+          t);
+      if (rc_) return rc_;
+      if (t < tbest) {
+        tbest = t;
+        best = sh;
+      }
+    }
+  const int b = size_bucket(n);
+  const int sh[5] = {best.nch_a, best.w_a, best.rc_a, best.nch_b, best.rb};
+  for (int k = 0; k < 5; ++k) h->tuned_shape[b][k] = sh[k];
+  h->tuned_valid[b] = 1;
+  if (shape_out)
+    for (int k = 0; k < 5; ++k) shape_out[k] = sh[k];
+  return ALPSG_OK;
+}
+
 int run_disp(alpsg_ctx *h, int n, const double *om, double *det, double *chi, double *low,
-             double *wave, int plain = 0) {
+             double *wave) {
   if (!h->have_k) return fail(ALPSG_E_STATE, "alpsg_set_k has not been called");
   if (n < 0) return fail(ALPSG_E_ARG, "negative batch size");
   if (n == 0) return ALPSG_OK;
   HIPCHK(h->om_d.ensure(n));
   HIPCHK(hipMemcpyAsync(h->om_d.p, om, sizeof(double2) * n, hipMemcpyHostToDevice, h->st));
-  LaunchShape sh = plain ? LaunchShape{64, 8, 4, 64, 8} : pick_shape(n);
-  if (!plain && h->fixed_nch) sh.nch_a = h->fixed_nch;
+  LaunchShape sh = shape_for(h, n);
   HIPCHK(hipEventRecord(h->ev0, h->st));
   int rc = disp_device(h, n, h->om_d.p, nullptr, chi != nullptr, low != nullptr, wave != nullptr, sh);
   if (rc) return rc;
@@ -222,6 +366,13 @@ int run_disp(alpsg_ctx *h, int n, const double *om, double *det, double *chi, do
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
   h->last_ms = ms;
+  float a = 0.f, b = 0.f, c = 0.f;
+  HIPCHK(hipEventElapsedTime(&a, h->ev0, h->ev_sweep));
+  HIPCHK(hipEventElapsedTime(&b, h->ev_sweep, h->ev_res));
+  HIPCHK(hipEventElapsedTime(&c, h->ev_res, h->ev1));
+  h->last_split[0] = a;
+  h->last_split[1] = b;
+  h->last_split[2] = c;
   h->evals += n;
   return ALPSG_OK;
 }
@@ -472,6 +623,8 @@ int alpsg_create(const alpsg_plasma_desc *d, alpsg_handle *out) {
   HIPCHK(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
   HIPCHK(hipEventCreate(&h->ev0));
   HIPCHK(hipEventCreate(&h->ev1));
+  HIPCHK(hipEventCreate(&h->ev_sweep));
+  HIPCHK(hipEventCreate(&h->ev_res));
   const size_t ngrid = (size_t)nspec * (nperp + 1) * (npar + 1);
   HIPCHK(h->pperp.ensure(h->pperp_h.size()));
   HIPCHK(h->ppar.ensure(h->ppar_h.size()));
@@ -705,6 +858,34 @@ int alpsg_set_k(alpsg_handle h, double kperp, double kpar, const int *nmax_in, i
   return ALPSG_OK;
 }
 
+int alpsg_autotune(alpsg_handle h, int n, const double *om, int *shape_out) {
+  if (!h) return fail(ALPSG_E_ARG, "null handle");
+  return autotune(h, n, reinterpret_cast<const double2 *>(om), shape_out);
+}
+
+int alpsg_set_shape(alpsg_handle h, int n, const int *shape) {
+  if (!h) return fail(ALPSG_E_ARG, "null handle");
+  if (n < 1) return fail(ALPSG_E_ARG, "n must be >= 1");
+  const int b = size_bucket(n);
+  if (!shape) {
+    h->tuned_valid[b] = 0;
+    return ALPSG_OK;
+  }
+  if (!shape_is_valid(shape)) return fail(ALPSG_E_ARG, "launch shape not instantiated");
+  for (int k = 0; k < 5; ++k) h->tuned_shape[b][k] = shape[k];
+  h->tuned_valid[b] = 1;
+  return ALPSG_OK;
+}
+
+int alpsg_get_shape(alpsg_handle h, int n, int *shape_out) {
+  if (!h || !shape_out) return fail(ALPSG_E_ARG, "null argument");
+  if (n < 1) return fail(ALPSG_E_ARG, "n must be >= 1");
+  LaunchShape sh = shape_for(h, n);
+  const int t[5] = {sh.nch_a, sh.w_a, sh.rc_a, sh.nch_b, sh.rb};
+  for (int k = 0; k < 5; ++k) shape_out[k] = t[k];
+  return ALPSG_OK;
+}
+
 int alpsg_get_nmax(alpsg_handle h, int *nmax_out) {
   if (!h || !nmax_out) return fail(ALPSG_E_ARG, "null argument");
   if (!h->have_k) return fail(ALPSG_E_STATE, "alpsg_set_k has not been called");
@@ -723,6 +904,12 @@ int alpsg_last_root_times(alpsg_handle h, int nroots, double *ms_out) {
 
 double alpsg_last_kernel_ms(alpsg_handle h) { return h ? h->last_ms : -1.0; }
 
+int alpsg_last_kernel_split(alpsg_handle h, double *ms3) {
+  if (!h || !ms3) return fail(ALPSG_E_ARG, "null argument");
+  for (int k = 0; k < 3; ++k) ms3[k] = h->last_split[k];
+  return ALPSG_OK;
+}
+
 int alpsg_disp_batch(alpsg_handle h, int n, const double *om, double *det, double *chi0,
                      double *chi0_low, double *wave) {
   if (!h || (n > 0 && (!om || !det))) return fail(ALPSG_E_ARG, "null argument");
@@ -732,7 +919,7 @@ int alpsg_disp_batch(alpsg_handle h, int n, const double *om, double *det, doubl
 int alpsg_disp_plain(alpsg_handle h, int n, const double *om, double *det) {
   if (!h || (n > 0 && (!om || !det))) return fail(ALPSG_E_ARG, "null argument");
   for (int i = 0; i < n; ++i) {
-    int rc = run_disp(h, 1, om + 2 * i, det + 2 * i, nullptr, nullptr, nullptr, 1);
+    int rc = run_disp(h, 1, om + 2 * i, det + 2 * i, nullptr, nullptr, nullptr);
     if (rc) return rc;
   }
   return ALPSG_OK;

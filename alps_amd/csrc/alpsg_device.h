@@ -68,6 +68,13 @@ __device__ __forceinline__ double rcp_d(double x) {
   r = __builtin_fma(r, e, r);
   return r;
 }
+// Reciprocal with one Newton step (relative error below 1e-13 for |x| in the
+// normal range): used in the node sweep, where it is summed over many nodes.
+__device__ __forceinline__ double rcp1_d(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  const double e = __builtin_fma(-x, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
 // s / z for complex z
 __device__ __forceinline__ cd cscale_inv(double s, cd z) {
   double inv = s * rcp_d(cabs2(z));

@@ -238,11 +238,14 @@ __device__ __forceinline__ cd tconst(const ProbDev &Pd, const SpecDev &S, int n,
   }
 }
 
-// one node of one row: acc += w * U * (num / den) with the b-weights
-__device__ __forceinline__ void node_acc(double w, double dre, double dim, const cd &U1,
-                                         const cd &U2, const cd &U3, double p, double p2,
-                                         cd acc[6]) {
-  const double inv = w * rcp_d(dre * dre + dim * dim);
+// one node of one row: acc += (w/den) * U with the b-weights (w is half the
+// trapezoid weight: the factor 2 is applied to the row sum)
+template <bool UNIT>
+__device__ __forceinline__ void node_acc(double w, double dre, double dim, double dim2,
+                                         const cd &U1, const cd &U2, const cd &U3, double p,
+                                         double p2, cd acc[6]) {
+  double inv = rcp1_d(__builtin_fma(dre, dre, dim2));
+  if constexpr (!UNIT) inv = w * inv;
   const cd r = mk(dre * inv, -dim * inv);
   const cd u1 = U1 * r, u2 = U2 * r, u3 = U3 * r;
   acc[0] = acc[0] + u1;
@@ -251,6 +254,44 @@ __device__ __forceinline__ void node_acc(double w, double dre, double dim, const
   acc[3] = acc[3] + u2;
   acc[4] = acc[4] + u3;
   acc[5] = acc[5] + p * u3;
+}
+
+// interior nodes i = 2..npar-2 (weight 2) of the +n row (and the -n row when
+// PAIR), lanes of an omega group strided by NCH.  STD: every interior node is in
+// range; otherwise node i is in range when it lies strictly inside one of the
+// two integrate() intervals of that row.
+template <int NCH, bool STD, bool PAIR>
+__device__ __forceinline__ void sweep_nodes(cdptr srows, int npar, int c, double omr, double omi,
+                                            double m, double kpar, double nq, double dim, int pl1,
+                                            int pc1, int pl2, int pc2, int ml1, int mc1, int ml2,
+                                            int mc2, cd ap[6], cd am[6]) {
+  const double dim2 = dim * dim;
+  const double mom = m * omr;
+  for (int i = 2 + c; i <= npar - 2; i += NCH) {
+    const double G1 = srows[(i - 1) * NODE_STRIDE + 0], H1 = srows[(i - 1) * NODE_STRIDE + 1];
+    const double G2 = srows[(i - 1) * NODE_STRIDE + 2], H2 = srows[(i - 1) * NODE_STRIDE + 3];
+    const double G3 = srows[(i - 1) * NODE_STRIDE + 4], H3 = srows[(i - 1) * NODE_STRIDE + 5];
+    const double p = srows[(i - 1) * NODE_STRIDE + 6], p2 = srows[(i - 1) * NODE_STRIDE + 7];
+    cd U1, U2, U3;
+    {
+      // mul + add keeps each instruction to one scalar (table) operand
+#pragma clang fp contract(off)
+      U1 = mk(omr * G1 + H1, omi * G1);
+      U2 = mk(omr * G2 + H2, omi * G2);
+      U3 = mk(omr * G3 + H3, omi * G3);
+    }
+    const double base = __builtin_fma(-kpar, p, mom);
+    if constexpr (STD) {
+      node_acc<true>(1.0, base - nq, dim, dim2, U1, U2, U3, p, p2, ap);
+      if (PAIR) node_acc<true>(1.0, base + nq, dim, dim2, U1, U2, U3, p, p2, am);
+    } else {
+      const bool inP = ((unsigned)(i - pl1) < (unsigned)pc1) | ((unsigned)(i - pl2) < (unsigned)pc2);
+      const bool inM = ((unsigned)(i - ml1) < (unsigned)mc1) | ((unsigned)(i - ml2) < (unsigned)mc2);
+      node_acc<false>(inP ? 1.0 : 0.0, inP ? base - nq : 1.0, dim, dim2, U1, U2, U3, p, p2, ap);
+      if (PAIR)
+        node_acc<false>(inM ? 1.0 : 0.0, inM ? base + nq : 1.0, dim, dim2, U1, U2, U3, p, p2, am);
+    }
+  }
 }
 
 // endpoint contribution (weight 1) of node e for a row with denominator shift nq
@@ -262,7 +303,8 @@ __device__ __forceinline__ void endpoint_acc(const double *__restrict__ rows, in
   const cd U1 = mk(omr * r[0] + r[1], omi * r[0]);
   const cd U2 = mk(omr * r[2] + r[3], omi * r[2]);
   const cd U3 = mk(omr * r[4] + r[5], omi * r[4]);
-  node_acc(1.0, (m * omr - kpar * r[6]) - nq, m * omi, U1, U2, U3, r[6], r[7], acc);
+  const double dim = m * omi;
+  node_acc<false>(0.5, (m * omr - kpar * r[6]) - nq, dim, dim * dim, U1, U2, U3, r[6], r[7], acc);
 }
 
 }  // namespace
@@ -283,6 +325,7 @@ __global__ __launch_bounds__(64 * W) void k_sweep(const ProbDev *__restrict__ pr
                                                   double2 *__restrict__ chiA,
                                                   double2 *__restrict__ lowA) {
   constexpr int OMB = 64 / NCH;
+  __shared__ double lchi[W][12][64];
   const ProbDev &Pd = *probp;
   const int lane = threadIdx.x & 63;
   const int TW = W * gridDim.y;
@@ -299,10 +342,12 @@ __global__ __launch_bounds__(64 * W) void k_sweep(const ProbDev *__restrict__ pr
     const SpecDev &S = Pd.sp[s];
     const int npar = S.npar;
     const double m = S.m;
-    const double scaleN = 2.0 * PI_D * S.dpperp * S.dppar_abs * 0.25 * S.q;
-    cd chi[6];
+    // 2 pi dpperp dppar / 4 * q, times 2: node sums carry half the trapezoid weights
+    const double scaleN = 2.0 * (2.0 * PI_D * S.dpperp * S.dppar_abs * 0.25 * S.q);
+    // the species' running chi lives in LDS ([component][lane]) between rows
+    double *lc = &lchi[threadIdx.x >> 6][0][lane];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) chi[k] = mk(0.0, 0.0);
+    for (int t = 0; t < 12; ++t) lc[64 * t] = 0.0;
     for (int nn = 0; nn <= S.nmax; ++nn) {
       if (((rowbase + nn) % TW) != wv) continue;
       const double *rows = S.nodes + (size_t)nn * (npar - 1) * NODE_STRIDE;
@@ -320,19 +365,20 @@ __global__ __launch_bounds__(64 * W) void k_sweep(const ProbDev *__restrict__ pr
 #pragma unroll
       for (int k = 0; k < 6; ++k) ap[k] = am[k] = mk(0.0, 0.0);
       const double dim = m * omi;
-      for (int i = 2 + c; i <= npar - 2; i += NCH) {
-        const double G1 = srows[(i - 1) * NODE_STRIDE + 0], H1 = srows[(i - 1) * NODE_STRIDE + 1];
-        const double G2 = srows[(i - 1) * NODE_STRIDE + 2], H2 = srows[(i - 1) * NODE_STRIDE + 3];
-        const double G3 = srows[(i - 1) * NODE_STRIDE + 4], H3 = srows[(i - 1) * NODE_STRIDE + 5];
-        const double p = srows[(i - 1) * NODE_STRIDE + 6], p2 = srows[(i - 1) * NODE_STRIDE + 7];
-        const cd U1 = mk(omr * G1 + H1, omi * G1);
-        const cd U2 = mk(omr * G2 + H2, omi * G2);
-        const cd U3 = mk(omr * G3 + H3, omi * G3);
-        const double base = m * omr - kpar * p;
-        const bool inP = ((unsigned)(i - pl1) < (unsigned)pc1) | ((unsigned)(i - pl2) < (unsigned)pc2);
-        const bool inM = ((unsigned)(i - ml1) < (unsigned)mc1) | ((unsigned)(i - ml2) < (unsigned)mc2);
-        node_acc(inP ? 2.0 : 0.0, inP ? base - nq : 1.0, dim, U1, U2, U3, p, p2, ap);
-        if (nn > 0) node_acc(inM ? 2.0 : 0.0, inM ? base + nq : 1.0, dim, U1, U2, U3, p, p2, am);
+      // rows whose trapezoid covers the whole grid in every lane take the
+      // uniform-weight loop (no per-node range tests)
+      const bool stdp = (rp.a2 < 0) && rp.a1 == 1 && rp.b1 == npar - 1;
+      const bool stdm = (nn == 0) || ((rm.a2 < 0) && rm.a1 == 1 && rm.b1 == npar - 1);
+      if (__all(stdp && stdm)) {
+        if (nn > 0)
+          sweep_nodes<NCH, true, true>(srows, npar, c, omr, omi, m, kpar, nq, dim, 0, 0, 0, 0, 0, 0,
+                                       0, 0, ap, am);
+        else
+          sweep_nodes<NCH, true, false>(srows, npar, c, omr, omi, m, kpar, nq, dim, 0, 0, 0, 0, 0,
+                                        0, 0, 0, ap, am);
+      } else {
+        sweep_nodes<NCH, false, true>(srows, npar, c, omr, omi, m, kpar, nq, dim, pl1, pc1, pl2, pc2,
+                                      ml1, mc1, ml2, mc2, ap, am);
       }
       if (c == 0) {
         endpoint_acc(rows, rp.a1, npar, m, kpar, omr, omi, nq, ap);
@@ -364,12 +410,19 @@ __global__ __launch_bounds__(64 * W) void k_sweep(const ProbDev *__restrict__ pr
         vm[k] = (scaleN * am[ACC_OF_MODE[k]]) * tconst(Pd, S, -nn, k);
       }
       if (nn == 0) {
-        chi[1] = chi[1] + vp[1];
-        chi[2] = chi[2] + vp[2];
-        chi[5] = chi[5] + vp[5];
+        lc[64 * 2] += vp[1].re;
+        lc[64 * 3] += vp[1].im;
+        lc[64 * 4] += vp[2].re;
+        lc[64 * 5] += vp[2].im;
+        lc[64 * 10] += vp[5].re;
+        lc[64 * 11] += vp[5].im;
       } else {
 #pragma unroll
-        for (int k = 0; k < 6; ++k) chi[k] = (chi[k] + vp[k]) + vm[k];
+        for (int k = 0; k < 6; ++k) {
+          const cd t = (mk(lc[64 * (2 * k)], lc[64 * (2 * k + 1)]) + vp[k]) + vm[k];
+          lc[64 * (2 * k)] = t.re;
+          lc[64 * (2 * k + 1)] = t.im;
+        }
       }
       if (lowA != nullptr && valid && c == 0 && nn <= 1) {
         double2 *o = lowA + ((size_t)iom * nspec + s) * 18;
@@ -393,7 +446,7 @@ __global__ __launch_bounds__(64 * W) void k_sweep(const ProbDev *__restrict__ pr
       // omega-fastest layout: chiA[((wv * nspec + s) * 6 + k) * nom + iom]
       double2 *o = chiA + ((size_t)(wv * nspec + s) * 6) * nom + iom;
 #pragma unroll
-      for (int k = 0; k < 6; ++k) o[(size_t)k * nom] = make_double2(chi[k].re, chi[k].im);
+      for (int k = 0; k < 6; ++k) o[(size_t)k * nom] = make_double2(lc[64 * (2 * k)], lc[64 * (2 * k + 1)]);
     }
   }
 }
@@ -974,17 +1027,12 @@ hipError_t launch_resonant(hipStream_t st, int nch, int RB, const ProbDev *prob,
   return hipErrorInvalidValue;
 }
 
-// species count of a problem, kept by the host beside the device copy
-static thread_local int g_nspec_hint = 1;
-void set_nspec_hint(int n) { g_nspec_hint = n; }
-static int nspec_of(const ProbDev *) { return g_nspec_hint; }
-
-hipError_t launch_assemble(hipStream_t st, const ProbDev *prob, int nom, int W, int RB,
+hipError_t launch_assemble(hipStream_t st, const ProbDev *prob, int nspec, int nom, int W, int RB,
                            const double2 *om, const int *active, const double2 *chiA,
                            const double2 *chiB, const double2 *lowA, const double2 *lowB,
                            double2 *det, double2 *chi, double2 *low, double2 *wave) {
   if (nom <= 0) return hipSuccess;
-  const int per = 6 * nspec_of(prob);
+  const int per = 6 * nspec;
   const int opb = 256 / per;
   hipLaunchKernelGGL(k_assemble, dim3((nom + opb - 1) / opb), dim3(256), 0, st, prob, nom, W, RB,
                      om, active, chiA, chiB, lowA, lowB, det, chi, low, wave);

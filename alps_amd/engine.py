@@ -38,7 +38,8 @@ _ERRNAMES = {1: "ALPSG_E_ARG", 2: "ALPSG_E_UNSUPPORTED", 3: "ALPSG_E_DEVICE",
 EXPORTS = ["alpsg_last_error", "alpsg_create", "alpsg_destroy", "alpsg_set_k",
            "alpsg_disp_batch", "alpsg_disp_plain", "alpsg_secant_batch", "alpsg_map_search",
            "alpsg_om_scan", "alpsg_get_nmax", "alpsg_eval_count", "alpsg_last_kernel_ms",
-           "alpsg_last_root_times"]
+           "alpsg_last_root_times", "alpsg_autotune", "alpsg_set_shape", "alpsg_get_shape",
+           "alpsg_last_kernel_split"]
 
 
 class AlpsgError(RuntimeError):
@@ -97,6 +98,12 @@ def load_library(path: str = LIB_PATH):
     L.alpsg_eval_count.restype = C.c_longlong
     L.alpsg_last_root_times.argtypes = [vp, C.c_int, dp]
     L.alpsg_last_root_times.restype = C.c_int
+    L.alpsg_autotune.argtypes = [vp, C.c_int, dp, ip]
+    L.alpsg_set_shape.argtypes = [vp, C.c_int, ip]
+    L.alpsg_get_shape.argtypes = [vp, C.c_int, ip]
+    for name in ["alpsg_autotune", "alpsg_set_shape", "alpsg_get_shape",
+           "alpsg_last_kernel_split"]:
+        getattr(L, name).restype = C.c_int
     L.alpsg_last_kernel_ms.argtypes = [vp]
     L.alpsg_last_kernel_ms.restype = C.c_double
     for name in ["alpsg_create", "alpsg_destroy", "alpsg_set_k", "alpsg_disp_batch",
@@ -211,6 +218,13 @@ class Engine:
     def last_kernel_ms(self) -> float:
         return float(load_library().alpsg_last_kernel_ms(self._h))
 
+    @property
+    def last_kernel_split_ms(self):
+        """(k_sweep, k_resonant, k_assemble) device times of the last batch, ms."""
+        out = np.zeros(3)
+        _check(load_library().alpsg_last_kernel_split(self._h, _dp(out)))
+        return tuple(float(x) for x in out)
+
     # -- disp ---------------------------------------------------------------
     def disp_batch(self, om, want_tensors: bool = False):
         om = np.ascontiguousarray(np.asarray(om, dtype=np.complex128).ravel())
@@ -256,6 +270,28 @@ class Engine:
     def secant_osc(self, om, roots: RootParams | None = None):
         o, d, it = self.secant_osc_batch([om], roots)
         return complex(o[0]), int(it[0])
+
+    # -- launch shapes ----------------------------------------------------------
+    SWEEP_VARIANTS = [(1, 1), (1, 2), (1, 4), (2, 4), (4, 4), (8, 4), (16, 4), (16, 8), (32, 8),
+                      (64, 4), (64, 8)]
+    RESONANT_LANES = [4, 8, 16, 32, 64]
+
+    def autotune(self, n: int, om=None):
+        """Time the kernel variants on a batch of n omegas; returns the chosen shape
+        (sweep lanes/omega, waves/workgroup, row chunks, resonant lanes/omega, harmonic blocks)."""
+        out = np.zeros(5, dtype=np.int32)
+        o = None if om is None else np.ascontiguousarray(np.asarray(om, dtype=np.complex128).ravel())
+        _check(load_library().alpsg_autotune(self._h, int(n), _dp(o), _ip(out)))
+        return tuple(int(x) for x in out)
+
+    def set_shape(self, n: int, shape=None):
+        sh = None if shape is None else np.asarray(shape, dtype=np.int32)
+        _check(load_library().alpsg_set_shape(self._h, int(n), _ip(sh)))
+
+    def get_shape(self, n: int):
+        out = np.zeros(5, dtype=np.int32)
+        _check(load_library().alpsg_get_shape(self._h, int(n), _ip(out)))
+        return tuple(int(x) for x in out)
 
     def last_root_times_ms(self, nroots: int):
         """Completion time of each root of the last secant batch (ms from its start)."""

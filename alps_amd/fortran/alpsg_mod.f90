@@ -37,6 +37,7 @@ module alpsg
   public :: alpsg_create, alpsg_destroy, alpsg_set_k, alpsg_disp_batch, alpsg_disp_plain
   public :: alpsg_secant_batch, alpsg_map_search, alpsg_om_scan, alpsg_get_nmax
   public :: alpsg_eval_count, alpsg_error_text, alpsg_disp, alpsg_secant_osc
+  public :: alpsg_autotune, alpsg_set_shape, alpsg_get_shape
 
   interface
      function alpsg_create(desc, h) bind(C, name='alpsg_create') result(rc)
@@ -131,6 +132,31 @@ module alpsg
        integer(c_long_long) :: n
      end function alpsg_eval_count
 
+     function alpsg_autotune(h, n, om, shape_out) bind(C, name='alpsg_autotune') result(rc)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+       integer(c_int), value :: n
+       type(c_ptr), value :: om
+       integer(c_int), intent(out) :: shape_out(5)
+       integer(c_int) :: rc
+     end function alpsg_autotune
+
+     function alpsg_set_shape(h, n, shape) bind(C, name='alpsg_set_shape') result(rc)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+       integer(c_int), value :: n
+       integer(c_int), intent(in) :: shape(5)
+       integer(c_int) :: rc
+     end function alpsg_set_shape
+
+     function alpsg_get_shape(h, n, shape_out) bind(C, name='alpsg_get_shape') result(rc)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+       integer(c_int), value :: n
+       integer(c_int), intent(out) :: shape_out(5)
+       integer(c_int) :: rc
+     end function alpsg_get_shape
+
      function alpsg_last_error_c() bind(C, name='alpsg_last_error') result(p)
        import :: c_ptr
        type(c_ptr) :: p
@@ -161,7 +187,8 @@ contains
     type(c_ptr), intent(in) :: h
     complex(c_double_complex), intent(in) :: om
     integer(c_int), intent(out) :: rc
-    complex(c_double_complex), intent(out), target, optional :: chi0(:,:,:), chi0_low(:,:,:,:), wave(:,:)
+    complex(c_double_complex), intent(out), target, optional, contiguous :: chi0(:,:,:), &
+         chi0_low(:,:,:,:), wave(:,:)
     complex(c_double_complex) :: det
     complex(c_double_complex) :: o(1), d(1)
     type(c_ptr) :: pc, pl, pw

@@ -57,11 +57,13 @@ def run_step(E, nr, ni, numroots=100):
     om, cal, val, iroots, nfound = E.map_search(MAP["omi"], MAP["omf"], MAP["gami"], MAP["gamf"],
                                                 nr, ni, numroots=numroots)
     map_kernel_ms = E.last_kernel_ms
+    split = E.last_kernel_split_ms
     t_map = (time.perf_counter() - t0) * 1e3
     guesses = np.array([om[i - 1, j - 1] for i, j in iroots])
     roots, dfin = E.refine_guess(guesses) if guesses.size else (guesses, guesses)
     lat = t_map + (E.last_root_times_ms(guesses.size) if guesses.size else np.zeros(0))
     return dict(roots=roots, dfin=dfin, guesses=guesses, latency_ms=lat, map_kernel_ms=map_kernel_ms,
+                split_ms=split,
                 cal=cal, om=om, nfound=nfound)
 
 
@@ -101,6 +103,12 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain", type=int, default=32, help="omegas timed on the plain path")
+    ap.add_argument("--shapes", default=None,
+                    help="JSON {batch size: [5 ints]} pinning launch shapes instead of autotuning "
+                         "(e.g. the launch_shapes of an earlier run, for a profiler pass)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 3 device-to-device copies of MIB MiB (torch) and exit: the "
+                         "known-size kernel that scales the profiler's FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -108,11 +116,37 @@ def main():
               "see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_probe:
+        import torch
+        nb = args.copy_probe << 20
+        a = torch.ones(nb // 8, dtype=torch.float64, device="cuda")
+        c = torch.empty_like(a)
+        for _ in range(3):
+            c.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": nb, "copies": 3}))
+        return
+
     from alps_amd.engine import Engine
     p = make_plasma()
     E = Engine(p)
     nmax = E.set_k(KPERP, KPAR)
-    for _ in range(args.warmup):
+    # warm-up, then tune the launch shapes on this workload's own batches: the
+    # map grid, and the secant request batches (size classes 1, 2, 4, ... up to
+    # the number of minima)
+    first = run_step(E, args.nr, args.ni)
+    if args.shapes:
+        shapes = {int(k): tuple(v) for k, v in json.loads(args.shapes).items()}
+        for k, v in shapes.items():
+            E.set_shape(k, v)
+    else:
+        shapes = {args.nr * args.ni: E.autotune(args.nr * args.ni, first["om"].T.ravel())}
+        g = first["guesses"]
+        k = 1
+        while g.size and k <= 2 * g.size:
+            shapes[k] = E.autotune(k, np.resize(g, k))
+            k *= 2
+    for _ in range(max(0, args.warmup - 1)):
         run_step(E, args.nr, args.ni)
 
     ev0 = E.eval_count
@@ -127,13 +161,15 @@ def main():
     n_roots = sum(r["roots"].size for r in res)
     lat = np.concatenate([r["latency_ms"] for r in res]) if n_roots else np.zeros(1)
     kern = np.array([r["map_kernel_ms"] for r in res])
+    split = np.array([r["split_ms"] for r in res]).mean(axis=0)
     last = res[-1]
     conv = int(np.sum(np.abs(last["dfin"]) < p.roots.D_threshold)) if last["roots"].size else 0
 
-    # roofline of the dominant kernel (the map's fused disp launch)
+    # roofline of the dominant kernel: k_sweep of the map batch (HIP events
+    # around it on the engine's stream, averaged over the timed steps)
     f_eval = flops_per_eval(nmax, p.npar)
     flops_launch = f_eval * args.nr * args.ni
-    achieved = flops_launch / (kern.mean() * 1e-3) / 1e12
+    achieved = flops_launch / (split[0] * 1e-3) / 1e12
 
     # plain path: one omega per launch
     plain_om = last["om"].ravel()[:: max(1, last["om"].size // args.plain)][: args.plain]
@@ -167,11 +203,16 @@ def main():
         "root_latency_p99_ms": float(np.percentile(lat, 99)),
         "plain_evals_per_sec": plain_rate,
         "map_kernel_ms": float(kern.mean()),
+        "map_kernel_split_ms": {"k_sweep": float(split[0]), "k_resonant": float(split[1]),
+                                "k_assemble": float(split[2])},
+        "map_evals_per_sec_device": args.nr * args.ni / (kern.mean() * 1e-3),
+        "launch_shapes": {str(k): list(v) for k, v in shapes.items()},
         "roofline": {"bound": "mfma", "compute": "fp64 (VALU)", "achieved": achieved,
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
                      "flops_per_launch": flops_launch,
-                     "kernel": "k_disp (map launch, %d omegas)" % (args.nr * args.ni)},
+                     "kernel": "k_sweep (map batch, %d omegas)" % (args.nr * args.ni),
+                     "kernel_ms": float(split[0])},
         "cpu_baseline": None,
     }
     if not args.no_cpu:

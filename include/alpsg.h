@@ -126,6 +126,18 @@ int alpsg_om_scan(alpsg_handle h, const alpsg_root_params *rp, int scan_type, in
                   double diff, double diff2, int n_out, int n_res, int nroots, double *roots,
                   double *kperp_out, double *kpar_out, double *roots_out);
 
+/* Launch-shape autotuner.  A shape is 5 ints: lanes per omega in the node
+   sweep, its wavefronts per workgroup, its workgroups per omega block
+   (harmonic-row chunks), lanes per omega in the resonant kernels, and their
+   harmonic blocks.  alpsg_autotune times the instantiated variants on a batch
+   of n omegas at the current k (om [2n] or NULL for a built-in spread) and
+   keeps the fastest for batches of n's size class (floor(log2 n));
+   alpsg_set_shape pins a shape for that class (NULL restores the default);
+   alpsg_get_shape reports the shape a batch of n would use. */
+int alpsg_autotune(alpsg_handle h, int n, const double *om, int *shape_out);
+int alpsg_set_shape(alpsg_handle h, int n, const int *shape);
+int alpsg_get_shape(alpsg_handle h, int n, int *shape_out);
+
 /* Diagnostics: nmax per species at the current k and the number of disp
    evaluations done by this handle. */
 int alpsg_get_nmax(alpsg_handle h, int *nmax_out);
@@ -138,6 +150,11 @@ int alpsg_last_root_times(alpsg_handle h, int nroots, double *ms_out);
 /* Device time (ms) of the last alpsg_disp_batch / alpsg_map_search kernel
    sweep, measured with HIP events on the handle's stream. */
 double alpsg_last_kernel_ms(alpsg_handle h);
+
+/* The same interval split by kernel: ms3[0] the node sweep (k_sweep),
+   ms3[1] the resonant-harmonic kernels (k_resonant, all parts), ms3[2] the
+   assembly (k_assemble). */
+int alpsg_last_kernel_split(alpsg_handle h, double *ms3);
 
 #ifdef __cplusplus
 }

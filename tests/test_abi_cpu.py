@@ -108,6 +108,8 @@ def test_null_handle_calls_fail_cleanly():
     assert L.alpsg_set_k(None, 0.1, 0.1, None, None) == 1
     assert L.alpsg_disp_batch(None, 1, None, None, None, None, None) == 1
     assert L.alpsg_eval_count(None) == -1
+    assert L.alpsg_autotune(None, 16, None, None) == 1
+    assert L.alpsg_set_shape(None, 16, None) == 1
     assert L.alpsg_destroy(None) == 0
 
 

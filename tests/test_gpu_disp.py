@@ -95,3 +95,32 @@ def test_df0_input_matches_f0_input():
     b.set_k(0.3, 0.3)
     oms = np.array(cases.BRANCH_OMEGAS)
     assert np.array_equal(a.disp_batch(oms), b.disp_batch(oms))
+
+
+def test_every_launch_shape_gives_the_same_results():
+    """Each instantiated kernel variant (lanes per omega, wavefronts, row chunks,
+    harmonic blocks) sums the same terms: results agree to rounding."""
+    from alps_amd.engine import Engine
+    p = cases.three_species()
+    E = Engine(p)
+    E.set_k(0.5, 0.2)
+    oms = np.array(cases.BRANCH_OMEGAS * 3)
+    n = oms.size
+    ref, chi_r, low_r, _ = E.disp_batch(oms, want_tensors=True)
+    scale = np.max(np.abs(chi_r), axis=(2, 3))
+    shapes = []
+    for i, (nch, w) in enumerate(Engine.SWEEP_VARIANTS):
+        nb = Engine.RESONANT_LANES[i % len(Engine.RESONANT_LANES)]
+        shapes.append((nch, w, 1 + i % 3, nb, 1 + (i % 4)))
+    for sh in shapes:
+        E.set_shape(n, sh)
+        assert E.get_shape(n) == sh
+        d, chi, low, _ = E.disp_batch(oms, want_tensors=True)
+        e = np.max(np.abs(chi - chi_r), axis=(2, 3)) / scale
+        assert np.all(e <= 1e-12), (sh, e.max())
+    E.set_shape(n, None)
+    sh = E.autotune(n, oms)
+    assert E.get_shape(n) == sh
+    d, chi, low, _ = E.disp_batch(oms, want_tensors=True)
+    assert np.max(np.abs(chi - chi_r) / scale[:, :, None, None]) <= 1e-12
+    E.close()
