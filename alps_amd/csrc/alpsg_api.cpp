@@ -548,6 +548,227 @@ int secant_osc_batch(alpsg_ctx *h, const alpsg_root_params *rp, int nroots, cplx
   return ALPSG_OK;
 }
 
+// secant (ALPS_fns.f90:1815-1917) for many independent roots.  Phases: 0 =
+// need D(prevom) and 1 = need D(om) before the loop, 2 = need D(om) in an
+// iteration, 3 = need D(prevom + 1e-8) after a stalled difference.
+struct SecState {
+  cplx om, prevom, Dprev, D, minom, minD;
+  int iter, phase, done;
+};
+
+int secant_batch(alpsg_ctx *h, const alpsg_root_params *rp, int nroots, cplx *om, int *iters) {
+  std::vector<SecState> S(nroots);
+  std::vector<cplx> req(nroots), res(nroots);
+  std::vector<int> who;
+  const auto t0 = std::chrono::steady_clock::now();
+  h->root_ms.assign(nroots, 0.0);
+  for (int r = 0; r < nroots; ++r) {
+    SecState &s = S[r];
+    s.om = om[r];
+    s.prevom = s.om * (1.0 - rp->D_prec);
+    s.phase = 0;
+    s.done = 0;
+    s.iter = 0;
+  }
+  auto start_iter = [&](SecState &s) {
+    if (s.iter <= rp->numiter - 1) {
+      s.iter = s.iter + 1;
+      s.phase = 2;
+    } else {
+      s.done = 1;
+    }
+  };
+  auto step = [&](SecState &s) {
+    cplx jump;
+    bool go = true;
+    if (std::abs(s.D) < rp->D_threshold) {
+      jump = 0.0;
+      go = false;
+    } else {
+      jump = s.D * (s.om - s.prevom) / (s.D - s.Dprev);
+    }
+    s.prevom = s.om;
+    s.om = s.om - jump;
+    s.Dprev = s.D;
+    if (std::abs(s.D) < std::abs(s.minD)) {  // the reference records the updated om
+      s.minom = s.om;
+      s.minD = s.D;
+    }
+    if (go)
+      start_iter(s);
+    else
+      s.done = 1;
+  };
+  for (;;) {
+    who.clear();
+    for (int r = 0; r < nroots; ++r) {
+      SecState &s = S[r];
+      if (s.done) continue;
+      req[who.size()] = (s.phase == 0 || s.phase == 3) ? s.prevom : s.om;
+      who.push_back(r);
+    }
+    if (who.empty()) break;
+    int rc = run_disp(h, (int)who.size(), reinterpret_cast<const double *>(req.data()),
+                      reinterpret_cast<double *>(res.data()), nullptr, nullptr, nullptr);
+    if (rc) return rc;
+    const double now_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    for (size_t t = 0; t < who.size(); ++t) {
+      SecState &s = S[who[t]];
+      const cplx D = res[t];
+      h->root_ms[who[t]] = now_ms;
+      switch (s.phase) {
+        case 0:
+          s.Dprev = D;
+          s.phase = 1;
+          break;
+        case 1:
+          s.minD = D;
+          s.minom = s.om;
+          if (std::abs(s.Dprev) < std::abs(s.minD)) {
+            s.minom = s.prevom;
+            s.minD = s.Dprev;
+          }
+          start_iter(s);
+          break;
+        case 2:
+          s.D = D;
+          if (std::abs(s.D - s.Dprev) < 1.e-80) {
+            s.prevom = s.prevom + 1.e-8;
+            s.phase = 3;
+            break;
+          }
+          step(s);
+          break;
+        default:
+          s.Dprev = D;
+          step(s);
+          break;
+      }
+    }
+  }
+  for (int r = 0; r < nroots; ++r) {
+    if (S[r].iter >= rp->numiter) S[r].om = S[r].minom;
+    om[r] = S[r].om;
+    if (iters) iters[r] = S[r].iter;
+  }
+  return ALPSG_OK;
+}
+
+// rtsec (ALPS_fns.f90:2105-2195) for many independent roots.  Phases: 0 =
+// need D(x1), 1 = need D(x2), 2 = need D(rts) in an iteration.
+struct RtsecState {
+  cplx x1, x2, xl, fl, f, rts, dx;
+  int j, phase, done;
+};
+
+int rtsec_batch(alpsg_ctx *h, const alpsg_root_params *rp, int nroots, cplx *om, int *iters) {
+  std::vector<RtsecState> S(nroots);
+  std::vector<cplx> req(nroots), res(nroots);
+  std::vector<int> who;
+  const auto t0 = std::chrono::steady_clock::now();
+  h->root_ms.assign(nroots, 0.0);
+  for (int r = 0; r < nroots; ++r) {
+    RtsecState &s = S[r];
+    s.x1 = om[r] * 1.0;
+    s.x2 = om[r] * (1.0 + rp->D_prec);
+    s.j = 0;
+    s.phase = 0;
+    s.done = 0;
+  }
+  auto next = [&](RtsecState &s) {
+    // one pass of the do-loop body up to the disp call
+    if (s.j + 1 > rp->numiter - 1) {
+      s.done = 1;
+      return;
+    }
+    s.j = s.j + 1;
+    if (std::abs(s.f - s.fl) > 1.e-40)
+      s.dx = (s.xl - s.rts) * s.f / (s.f - s.fl);
+    else
+      s.dx = (s.x2 - s.x1) / 25.0;
+    s.xl = s.rts;
+    s.fl = s.f;
+    s.rts = s.rts + s.dx / 2.0;
+    s.phase = 2;
+  };
+  for (;;) {
+    who.clear();
+    for (int r = 0; r < nroots; ++r) {
+      RtsecState &s = S[r];
+      if (s.done) continue;
+      req[who.size()] = s.phase == 0 ? s.x1 : (s.phase == 1 ? s.x2 : s.rts);
+      who.push_back(r);
+    }
+    if (who.empty()) break;
+    int rc = run_disp(h, (int)who.size(), reinterpret_cast<const double *>(req.data()),
+                      reinterpret_cast<double *>(res.data()), nullptr, nullptr, nullptr);
+    if (rc) return rc;
+    const double now_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    for (size_t t = 0; t < who.size(); ++t) {
+      RtsecState &s = S[who[t]];
+      const cplx D = res[t];
+      h->root_ms[who[t]] = now_ms;
+      switch (s.phase) {
+        case 0:
+          s.fl = D;
+          s.phase = 1;
+          break;
+        case 1:
+          s.f = D;
+          if (std::abs(s.fl) < std::abs(s.f)) {
+            s.rts = s.x1;
+            s.xl = s.x2;
+            std::swap(s.fl, s.f);
+          } else {
+            s.xl = s.x1;
+            s.rts = s.x2;
+          }
+          next(s);
+          break;
+        default:
+          s.f = D;
+          if (std::abs(s.dx) < rp->D_tol || std::abs(s.f) == 0.0)
+            s.done = 1;
+          else
+            next(s);
+          break;
+      }
+    }
+  }
+  for (int r = 0; r < nroots; ++r) {
+    om[r] = S[r].rts;
+    if (iters) iters[r] = S[r].j;
+  }
+  return ALPSG_OK;
+}
+
+// The root solver the &system namelist selects (secant_method, as in
+// om_scan, ALPS_fns.f90:2507-2514, and refine_guess, ALPS_fns.f90:3830-3843),
+// then the "final instance of disp" at each root when Dfin is given.
+int root_batch(alpsg_ctx *h, const alpsg_root_params *rp, int nroots, cplx *om, cplx *Dfin,
+               int *iters) {
+  int rc;
+  switch (rp->secant_method) {
+    case 0:
+      rc = secant_batch(h, rp, nroots, om, iters);
+      break;
+    case 1:
+      rc = rtsec_batch(h, rp, nroots, om, iters);
+      break;
+    case 2:
+      return secant_osc_batch(h, rp, nroots, om, Dfin, iters);
+    default:
+      return fail(ALPSG_E_ARG, "secant_method must be 0, 1 or 2");
+  }
+  if (rc) return rc;
+  if (Dfin)
+    return run_disp(h, nroots, reinterpret_cast<const double *>(om),
+                    reinterpret_cast<double *>(Dfin), nullptr, nullptr, nullptr);
+  return ALPSG_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -929,10 +1150,9 @@ int alpsg_secant_batch(alpsg_handle h, const alpsg_root_params *rp, int nroots, 
                        double *D_final, int *iters) {
   if (!h || !rp || (nroots > 0 && !om)) return fail(ALPSG_E_ARG, "null argument");
   if (!h->have_k) return fail(ALPSG_E_STATE, "alpsg_set_k has not been called");
-  if (rp->secant_method != 2) return fail(ALPSG_E_UNSUPPORTED, "only secant_method = 2 (secant_osc) is batched");
   if (nroots <= 0) return ALPSG_OK;
-  return secant_osc_batch(h, rp, nroots, reinterpret_cast<cplx *>(om),
-                          reinterpret_cast<cplx *>(D_final), iters);
+  return root_batch(h, rp, nroots, reinterpret_cast<cplx *>(om), reinterpret_cast<cplx *>(D_final),
+                    iters);
 }
 
 int alpsg_map_search(alpsg_handle h, double omi, double omf, double gami, double gamf, int nr,
@@ -1091,7 +1311,7 @@ int alpsg_om_scan(alpsg_handle h, const alpsg_root_params *rp, int scan_type, in
         idx.push_back(r);
       }
     std::vector<cplx> Df(om.size());
-    rc = secant_osc_batch(h, rp, (int)om.size(), om.data(), Df.data(), nullptr);
+    rc = root_batch(h, rp, (int)om.size(), om.data(), Df.data(), nullptr);
     if (rc) return rc;
     for (size_t t = 0; t < idx.size(); ++t) R[idx[t]] = om[t];
     for (size_t t = 0; t < idx.size(); ++t) {

@@ -257,7 +257,10 @@ class Engine:
         return det
 
     # -- root searches --------------------------------------------------------
-    def secant_osc_batch(self, guesses, roots: RootParams | None = None):
+    def solve_batch(self, guesses, roots: RootParams | None = None):
+        """The root solver roots.secant_method selects (0 secant, 1 rtsec,
+        2 secant_osc) on every guess, batched; returns (roots, D at the roots,
+        iterations)."""
         r = roots or self.plasma.roots
         om = np.ascontiguousarray(np.asarray(guesses, dtype=np.complex128).ravel()).copy()
         dfin = np.zeros_like(om)
@@ -266,6 +269,12 @@ class Engine:
         _check(load_library().alpsg_secant_batch(self._h, C.byref(rp), om.size, _dp(om), _dp(dfin),
                                                 _ip(it)))
         return om, dfin, it
+
+    def secant_osc_batch(self, guesses, roots: RootParams | None = None):
+        """secant_osc (ALPS_fns.f90:1919-2101) on every guess, batched."""
+        import dataclasses
+        r = roots or self.plasma.roots
+        return self.solve_batch(guesses, dataclasses.replace(r, secant_method=2))
 
     def secant_osc(self, om, roots: RootParams | None = None):
         o, d, it = self.secant_osc_batch([om], roots)
@@ -301,7 +310,7 @@ class Engine:
 
     def refine_guess(self, guesses, roots: RootParams | None = None):
         """refine_guess (ALPS_fns.f90:3793-3856): secant on each guess, then det D there."""
-        om, dfin, _ = self.secant_osc_batch(guesses, roots)
+        om, dfin, _ = self.solve_batch(guesses, roots)
         return om, dfin
 
     def map_search(self, omi, omf, gami, gamf, nr, ni, loggridw=False, loggridg=False,

@@ -66,7 +66,7 @@ typedef struct alpsg_plasma_desc {
 
 /* Root-search controls (&system namelist, ALPS_var.f90:147-165). */
 typedef struct alpsg_root_params {
-  int secant_method; /* 0 secant, 1 rtsec, 2 secant_osc (only 2 is batched on the device) */
+  int secant_method; /* 0 secant, 1 rtsec, 2 secant_osc */
   int numiter;
   double D_threshold, D_prec, D_gap, D_tol;
 } alpsg_root_params;
@@ -100,9 +100,10 @@ int alpsg_disp_batch(alpsg_handle h, int n, const double *om, double *det, doubl
 /* The same evaluation with one launch per omega (the plain path). */
 int alpsg_disp_plain(alpsg_handle h, int n, const double *om, double *det);
 
-/* Batched secant_osc (ALPS_fns.f90:1919-2101) for nroots independent roots at
-   the handle's k: om [2*nroots] in/out.  D_final [2*nroots] (optional) gets
-   disp at the returned root (the "final instance of disp", ALPS_fns.f90:2523,
+/* Batched root refinement for nroots independent roots at the handle's k with
+   the solver rp->secant_method selects: secant (ALPS_fns.f90:1815-1917),
+   rtsec (:2105-2195) or secant_osc (:1919-2101).  om [2*nroots] in/out.
+   D_final [2*nroots] (optional) gets disp at the returned root (the "final instance of disp", ALPS_fns.f90:2523,
    3847); iters [nroots] (optional) the iteration count. */
 int alpsg_secant_batch(alpsg_handle h, const alpsg_root_params *rp, int nroots, double *om,
                        double *D_final, int *iters);

@@ -31,9 +31,12 @@
  *   determine_nmax             brute force max|J_n| over the grid
  *   map_search/find_minima     brute force on small maps; minima of a closed
  *                              form |f(w)|
- * Parity unpinned: secant (method 0) and rtsec (method 1) -- no reference
- * golden uses them; the landau_integrate early-return quirk and the int_ee
- * corner term are restated from the source text only.
+ *   secant/rtsec               the reference's printed test_kpar_fast root;
+ *                              rtsec's half-step contraction
+ * Parity unpinned: the trajectory details of secant and rtsec beyond their
+ * converged roots (e.g. secant's best point being the updated omega), the
+ * landau_integrate early-return quirk and the int_ee corner term are restated
+ * from the source text only.
  */
 #include "alps_oracle.h"
 

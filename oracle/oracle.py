@@ -95,6 +95,8 @@ def lib():
         L.orc_secant_osc.argtypes = [P, dp]
         L.orc_secant.restype = C.c_int
         L.orc_secant.argtypes = [P, dp]
+        L.orc_rtsec.restype = _C128
+        L.orc_rtsec.argtypes = [P, _C128, C.POINTER(C.c_int)]
         L.orc_map_search.restype = C.c_int
         L.orc_map_search.argtypes = [P, C.c_double, C.c_double, C.c_double, C.c_double,
                                      C.c_int, C.c_int, C.c_int, C.c_int, dp, dp, dp, C.c_int, ip]
@@ -246,6 +248,11 @@ class Oracle:
         z = np.array([complex(om)], dtype=np.complex128)
         it = lib().orc_secant(C.byref(self.s), _ptr(z))
         return complex(z[0]), it
+
+    def rtsec(self, om):
+        flag = C.c_int(0)
+        r = lib().orc_rtsec(C.byref(self.s), _c(om), C.byref(flag))
+        return _z(r), flag.value
 
     def map_search(self, omi, omf, gami, gamf, nr, ni, loggridw=False, loggridg=False,
                    numroots=100):

@@ -38,6 +38,25 @@ def test_secant_osc_roots_match_oracle():
         assert abs(alone - g_roots[k]) <= 1e-13 * abs(alone)
 
 
+@pytest.mark.parametrize("method", [0, 1])
+def test_secant_and_rtsec_batches_match_oracle(method):
+    """secant (ALPS_fns.f90:1815) and rtsec (:2105) as batched host state
+    machines on the device disp agree with the restatement root by root."""
+    import dataclasses
+    p = cases.small_bimax()
+    p = dataclasses.replace(p, roots=dataclasses.replace(p.roots, secant_method=method,
+                                                         D_threshold=1e-14, D_tol=1e-13))
+    O, E = Oracle(p), _engine(p)
+    O.set_k(0.3, 0.3)
+    E.set_k(0.3, 0.3)
+    guesses = [0.29 - 0.001j, 0.5 - 0.05j, 1.1 - 0.1j]
+    g_roots, g_dfin, g_it = E.solve_batch(guesses)
+    for k, w0 in enumerate(guesses):
+        o_root, o_it = O.secant(w0) if method == 0 else O.rtsec(w0)
+        assert abs(g_roots[k] - o_root) <= ROOT_TOL * abs(o_root), (w0, g_roots[k], o_root)
+        assert abs(g_dfin[k] - E.disp(g_roots[k])) == 0.0
+
+
 def test_map_search_matches_oracle():
     p = cases.small_bimax()
     O, E = Oracle(p), _engine(p)

@@ -42,3 +42,29 @@ def test_oracle_reproduces_kpar_fast_golden():
         row = [kp[i], kz[i], roots[i, 0].real, roots[i, 0].imag]
         for v, g in zip(row, gold[i]):
             assert printed_equal(v, g), (i, row, gold[i])
+
+
+def test_oracle_secant_and_rtsec_reach_the_golden_root():
+    """secant (method 0) and rtsec (method 1) from the test_kpar_fast guess land on
+    the root the reference printed (first row of the scan golden).  rtsec moves
+    half a secant step per iteration (ALPS_fns.f90:2170), so near the root its
+    error halves per step: tightening D_tol by 1e3 costs about log2(1e3) = 10
+    more iterations."""
+    import dataclasses
+    gold = np.loadtxt(GOLD)
+    p = cases.kpar_fast()
+    O = Oracle(p)
+    O.set_k(1e-2, 1e-2)
+    w0 = 9.9e-3 - 5.5e-6j
+    r, it = O.secant(w0)
+    assert printed_equal(r.real, gold[0][2]) and printed_equal(r.imag, gold[0][3]), r
+    assert 1 <= it < p.roots.numiter
+    its = []
+    for tol in (1e-10, 1e-13):
+        q = dataclasses.replace(p, roots=dataclasses.replace(p.roots, D_tol=tol))
+        Q = Oracle(q)
+        Q.set_k(1e-2, 1e-2)
+        rr, itr = Q.rtsec(w0)
+        assert printed_equal(rr.real, gold[0][2]) and printed_equal(rr.imag, gold[0][3]), rr
+        its.append(itr)
+    assert 8 <= its[1] - its[0] <= 12, its
