@@ -40,6 +40,9 @@ hipError_t launch_sweep(hipStream_t st, int nch, int W, int RC, const ProbDev *p
 hipError_t launch_resonant(hipStream_t st, int nch, int RB, const ProbDev *prob, int nspec, int nom,
                            const double2 *om, const int *active, double2 *chiB, double2 *lowB,
                            bool general);
+hipError_t launch_nhds(hipStream_t st, const ProbDev *prob, int nspec, int nom, const double2 *om,
+                       const int *active, double2 *chiA, double2 *lowA);
+int nhds_tables(double z, int bm_nmax, double bessel_zero, std::vector<double2> &tab);
 hipError_t launch_assemble(hipStream_t st, const ProbDev *prob, int nspec, int nom, int W, int RB,
                            const double2 *om, const int *active, const double2 *chiA,
                            const double2 *chiB, const double2 *lowA, const double2 *lowB,
@@ -114,6 +117,13 @@ struct alpsg_ctx {
   DevBuf<int> act_d;
   std::vector<double> root_ms;  // completion time of each root of the last secant batch
   bool any_general_fit = false;  // a species needs eval_fit in the Landau term
+  // closed-form (use_bM) species and their &bM_spec parameters
+  int usebm[ALPSG_MAXSPEC] = {0};
+  bool any_bm = false;
+  int bm_nmax[ALPSG_MAXSPEC] = {0};
+  double bm_bz[ALPSG_MAXSPEC] = {0}, bm_beta[ALPSG_MAXSPEC] = {0}, bm_alpha[ALPSG_MAXSPEC] = {0},
+         bm_pdrift[ALPSG_MAXSPEC] = {0};
+  DevBuf<double2> bmi[ALPSG_MAXSPEC];
   // launch shapes chosen by alpsg_autotune (or pinned by alpsg_set_shape),
   // keyed by the batch-size bucket floor(log2(n))
   int tuned_valid[32] = {0};
@@ -128,6 +138,7 @@ struct alpsg_ctx {
       bes[s].release();
       atab[s].release();
       nodes[s].release();
+      bmi[s].release();
     }
     mtab.release();
     prob_d.release();
@@ -220,6 +231,9 @@ int disp_device(alpsg_ctx *h, int n, const double2 *om_dev, const int *active_de
   HIPCHK(hipEventRecord(h->ev_sweep, h->st));
   HIPCHK(launch_resonant(h->st, sh.nch_b, sh.rb, h->prob_d.p, ns, n, om_dev, active_dev,
                          h->chiB_d.p, want_low ? h->lowB_d.p : nullptr, h->any_general_fit));
+  if (h->any_bm)
+    HIPCHK(launch_nhds(h->st, h->prob_d.p, ns, n, om_dev, active_dev, h->chiA_d.p,
+                       want_low ? h->lowA_d.p : nullptr));
   HIPCHK(hipEventRecord(h->ev_res, h->st));
   HIPCHK(launch_assemble(h->st, h->prob_d.p, ns, n, TW, sh.rb, om_dev, active_dev, h->chiA_d.p,
                          h->chiB_d.p, want_low ? h->lowA_d.p : nullptr,
@@ -787,10 +801,19 @@ int alpsg_create(const alpsg_plasma_desc *d, alpsg_handle *out) {
   if (nperp < 4 || npar < 2 * P + 8) return fail(ALPSG_E_ARG, "grid too small for positions_principal");
   if (d->n_resonance_interval < 2) return fail(ALPSG_E_ARG, "n_resonance_interval < 2");
   for (int s = 0; s < nspec; ++s) {
-    if (d->usebM && d->usebM[s]) return fail(ALPSG_E_UNSUPPORTED, "use_bM species are not handled by the engine");
     if (d->relativistic && d->relativistic[s]) return fail(ALPSG_E_UNSUPPORTED, "relativistic species are not handled by the engine");
-    if (d->ACmethod[s] != 1 && d->ACmethod[s] != 2) return fail(ALPSG_E_UNSUPPORTED, "ACmethod must be 1 or 2");
     if (d->qs[s] == 0.0 || d->ms[s] <= 0.0) return fail(ALPSG_E_ARG, "species charge/mass");
+    if (d->usebM && d->usebM[s]) {
+      // closed-form species: no table, fits or continuation are read
+      if (!d->bM_nmax || !d->bM_Bessel_zero || !d->bM_beta || !d->bM_alpha || !d->bM_pdrift)
+        return fail(ALPSG_E_ARG, "missing bM_spec arrays for a use_bM species");
+      if (d->bM_beta[s] < 0.0 || (d->bM_beta[s] > 0.0 && !(d->bM_alpha[s] > 0.0)) || d->ns[s] <= 0.0)
+        return fail(ALPSG_E_ARG, "bM_spec parameters");
+      if (d->bM_beta[s] == 0.0 && !d->kperp_norm)
+        return fail(ALPSG_E_UNSUPPORTED, "cold-plasma species need kperp_norm (the reference stops)");
+      continue;
+    }
+    if (d->ACmethod[s] != 1 && d->ACmethod[s] != 2) return fail(ALPSG_E_UNSUPPORTED, "ACmethod must be 1 or 2");
     if (d->ACmethod[s] == 1) {
       if (d->n_fits[s] < 1 || d->n_fits[s] > ALPSG_MAXFITS || d->n_fits[s] > d->max_fits)
         return fail(ALPSG_E_ARG, "n_fits out of range");
@@ -828,6 +851,7 @@ int alpsg_create(const alpsg_plasma_desc *d, alpsg_handle *out) {
   for (int s = 0; s < nspec; ++s) {
     for (int i = 0; i <= nperp; ++i) h->pperp_h[(size_t)s * (nperp + 1) + i] = d->pp[f_pp(nspec, nperp, npar, s, i, 1, 0)];
     for (int j = 0; j <= npar; ++j) h->ppar_h[(size_t)s * (npar + 1) + j] = d->pp[f_pp(nspec, nperp, npar, s, 2, j, 1)];
+    if (d->usebM && d->usebM[s]) continue;  // the closed form reads no grid
     for (int i = 0; i <= nperp; ++i)
       for (int j = 0; j <= npar; ++j) {
         if (d->pp[f_pp(nspec, nperp, npar, s, i, j, 0)] != h->pperp_h[(size_t)s * (nperp + 1) + i] ||
@@ -892,15 +916,17 @@ int alpsg_create(const alpsg_plasma_desc *d, alpsg_handle *out) {
 
   // analytic-continuation tables
   std::vector<size_t> fit_off(nspec + 1, 0);
+  auto bm = [&](int s) { return d->usebM != nullptr && d->usebM[s] != 0; };
   for (int s = 0; s < nspec; ++s)
-    fit_off[s + 1] = fit_off[s] + (d->ACmethod[s] == 1 ? (size_t)(nperp + 1) * d->n_fits[s] * FIT_STRIDE : 0);
+    fit_off[s + 1] = fit_off[s] + ((d->ACmethod[s] == 1 && !bm(s)) ? (size_t)(nperp + 1) * d->n_fits[s] * FIT_STRIDE : 0);
   std::vector<double> fitc(std::max<size_t>(fit_off[nspec], 1), 0.0);
   int maxorder = 0;
   for (int s = 0; s < nspec; ++s)
-    if (d->ACmethod[s] == 2) maxorder = std::max(maxorder, d->poly_order[s]);
+    if (d->ACmethod[s] == 2 && !bm(s)) maxorder = std::max(maxorder, d->poly_order[s]);
   std::vector<double> polyc((size_t)nspec * (nperp + 1) * (maxorder + 1), 0.0);
   const int npf = d->param_fit_nperp;
   for (int s = 0; s < nspec; ++s) {
+    if (bm(s)) continue;
     if (d->ACmethod[s] == 1) {
       for (int i = 0; i <= nperp; ++i) {
         const double pv = h->pperp_h[(size_t)s * (nperp + 1) + i];
@@ -971,6 +997,30 @@ int alpsg_create(const alpsg_plasma_desc *d, alpsg_handle *out) {
     S.pperp = h->pperp.p + (size_t)s * (nperp + 1);
     S.ppar = h->ppar.p + (size_t)s * (npar + 1);
     S.fitc = h->fitc.p + fit_off[s];
+    if (bm(s)) {
+      // closed form (ALPS_NHDS.f90:163-168): no table terms, no ee term
+      h->usebm[s] = 1;
+      h->any_bm = true;
+      h->bm_nmax[s] = d->bM_nmax[s];
+      h->bm_bz[s] = d->bM_Bessel_zero[s];
+      h->bm_beta[s] = d->bM_beta[s];
+      h->bm_alpha[s] = d->bM_alpha[s];
+      h->bm_pdrift[s] = d->bM_pdrift[s];
+      S.usebm = 1;
+      S.bm_cold = d->bM_beta[s] == 0.0 ? 1 : 0;
+      S.bm_Omega = d->qs[s] / d->ms[s];
+      S.bm_ell = std::sqrt(d->ms[s] / (d->ns[s] * d->qs[s] * d->qs[s]));
+      S.bm_vth = std::sqrt(d->bM_beta[s] / (d->ns[s] * d->ms[s]));
+      S.bm_vd = d->bM_pdrift[s] / d->ms[s];
+      S.bm_alpha = d->bM_alpha[s];
+      S.int_ee = 0.0;
+      S.acmethod = 1;
+      S.nfits = 1;
+      S.fit_type[0] = 1;
+      S.fit_uniform = 1;
+      S.nmax = -1;
+      continue;
+    }
     if (d->ACmethod[s] == 1) {
       bool uni = true;
       for (int f = 0; f < d->n_fits[s] && uni; ++f) {
@@ -1013,18 +1063,26 @@ int alpsg_set_k(alpsg_handle h, double kperp, double kpar, const int *nmax_in, i
     return fail(ALPSG_E_ARG, "kpar must be finite and nonzero");
   const int nspec = h->nspec, nperp = h->nperp, npar = h->npar;
   int nmax[ALPSG_MAXSPEC];
+  bool all_bm = true;
+  for (int s = 0; s < nspec; ++s) all_bm = all_bm && h->usebm[s];
   if (nmax_in) {
     for (int s = 0; s < nspec; ++s) {
+      if (h->usebm[s]) {
+        nmax[s] = 1;
+        continue;
+      }
       if (nmax_in[s] < 1 || nmax_in[s] >= MTAB_N - 2) return fail(ALPSG_E_ARG, "nmax_in out of range");
       nmax[s] = nmax_in[s];
     }
+  } else if (all_bm) {
+    for (int s = 0; s < nspec; ++s) nmax[s] = 1;
   } else {
     // determine_nmax: first n >= 1 with max_iperp |J_n| <= Bessel_zero.  The
     // device evaluates n = 1..ntry, where ntry bounds the answer through
     // |J_n(x)| <= (x/2)^n / n!.
     double xmax = 0.0;
     for (int s = 0; s < nspec; ++s)
-      for (int i = 0; i <= nperp; ++i)
+      for (int i = 0; i <= nperp && !h->usebm[s]; ++i)
         xmax = std::max(xmax, std::fabs(kperp * h->pperp_h[(size_t)s * (nperp + 1) + i] / h->qs[s]));
     int ntry = 1;
     double lb = 0.0, lz = std::log(std::max(h->Bessel_zero, 1e-300)) - std::log(1e6);
@@ -1040,6 +1098,10 @@ int alpsg_set_k(alpsg_handle h, double kperp, double kpar, const int *nmax_in, i
     HIPCHK(hipMemcpyAsync(bm.data(), h->bmax.p, sizeof(double) * bm.size(), hipMemcpyDeviceToHost, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
     for (int s = 0; s < nspec; ++s) {
+      if (h->usebm[s]) {  // determine_nmax, ALPS_fns.f90:4012-4013
+        nmax[s] = 1;
+        continue;
+      }
       int n = 0;
       for (int t = 1; t <= ntry; ++t)
         if (!(bm[(size_t)s * ntry + (t - 1)] > h->Bessel_zero)) {
@@ -1053,6 +1115,24 @@ int alpsg_set_k(alpsg_handle h, double kperp, double kpar, const int *nmax_in, i
   for (int s = 0; s < nspec; ++s) {
     SpecDev &S = h->prob.sp[s];
     const int nm = nmax[s];
+    h->nmax[s] = nm;
+    if (h->usebm[s]) {
+      // omega-independent part of calc_chi (ALPS_NHDS.f90:163-185)
+      if (!S.bm_cold) {
+        const double a = kperp * S.bm_vth / S.bm_Omega;
+        S.bm_z = ((0.5 * a) * a) * S.bm_alpha;
+        const double b = S.bm_vth / S.bm_Omega;
+        S.bm_zp = ((0.5 * b) * b) * S.bm_alpha;
+        std::vector<double2> tab;
+        S.bm_nrun = nhds_tables(S.bm_z, h->bm_nmax[s], h->bm_bz[s], tab);
+        HIPCHK(h->bmi[s].ensure(tab.size()));
+        HIPCHK(hipMemcpyAsync(h->bmi[s].p, tab.data(), sizeof(double2) * tab.size(),
+                              hipMemcpyHostToDevice, h->st));
+        S.bmi = h->bmi[s].p;
+      }
+      S.nmax = -1;
+      continue;
+    }
     HIPCHK(h->bes[s].ensure((size_t)(nm + 3) * (nperp + 1)));
     HIPCHK(h->atab[s].ensure((size_t)(nm + 1) * (nperp + 1) * ATAB_STRIDE));
     HIPCHK(h->nodes[s].ensure((size_t)(nm + 1) * (npar - 1) * NODE_STRIDE));
@@ -1065,7 +1145,6 @@ int alpsg_set_k(alpsg_handle h, double kperp, double kpar, const int *nmax_in, i
     S.z = h->kperp_norm ? kperp / h->qs[s] : 1.0 / h->qs[s];
     S.atab = h->atab[s].p;
     S.nodes = h->nodes[s].p;
-    h->nmax[s] = nm;
   }
   h->prob.kperp = kperp;
   h->prob.kpar = kpar;

@@ -31,6 +31,10 @@ struct SpecDev {
   const double *nodes;  // [(nmax+1)*(npar-1)*NODE_STRIDE]
   const double *fitc;   // [(nperp+1)*nfits*FIT_STRIDE]
   const double *polyc;  // [(nperp+1)*(poly_order+1)]
+  // closed-form (use_bM) species: NHDS parameters (ALPS_NHDS.f90:163-168)
+  int usebm, bm_cold, bm_nrun;
+  double bm_Omega, bm_ell, bm_vth, bm_vd, bm_alpha, bm_z, bm_zp;
+  const double2 *bmi;   // [2*bm_nrun+1]: (I_n(z) e^-z, dI_n/dz e^-z), n = -nrun..nrun
 };
 
 struct ProbDev {

@@ -62,6 +62,8 @@ class _Desc(C.Structure):
         ("poly_kind", C.c_void_p), ("poly_order", C.c_void_p), ("logfit", C.c_void_p),
         ("poly_log_max", C.c_void_p), ("poly_fit_coeffs", C.c_void_p),
         ("poly_max_order", C.c_int),
+        ("bM_nmax", C.c_void_p), ("bM_Bessel_zero", C.c_void_p), ("bM_beta", C.c_void_p),
+        ("bM_alpha", C.c_void_p), ("bM_pdrift", C.c_void_p),
     ]
 
 
@@ -148,6 +150,10 @@ class Engine:
         keep["logfit"] = p.logfit.astype(np.int32)
         keep["poly_log_max"] = p.poly_log_max.astype(np.float64)
         keep["fit_type_f"] = np.asfortranarray(p.fit_type.astype(np.int32))
+        keep["usebM"] = p.usebM.astype(np.int32)
+        keep["bM_nmax"] = p.bM_nmax.astype(np.int32)
+        for name in ("bM_Bessel_zero", "bM_beta", "bM_alpha", "bM_pdrift"):
+            keep[name] = getattr(p, name).astype(np.float64)
         if use_df0:
             df0 = np.zeros((p.nspec, p.nperp - 1, p.npar - 1, 2))
             pp, f0 = p.pp, p.f0
@@ -168,7 +174,12 @@ class Engine:
         d.positions_principal = int(p.positions_principal)
         d.n_resonance_interval = int(p.n_resonance_interval)
         d.Tlim, d.Bessel_zero = p.Tlim, p.Bessel_zero
-        d.usebM = d.relativistic = None
+        d.usebM = keep["usebM"].ctypes.data
+        d.relativistic = None
+        d.bM_nmax = keep["bM_nmax"].ctypes.data
+        d.bM_Bessel_zero = keep["bM_Bessel_zero"].ctypes.data
+        d.bM_beta, d.bM_alpha = keep["bM_beta"].ctypes.data, keep["bM_alpha"].ctypes.data
+        d.bM_pdrift = keep["bM_pdrift"].ctypes.data
         d.ACmethod, d.n_fits = keep["ACmethod"].ctypes.data, keep["n_fits"].ctypes.data
         d.max_fits = p.max_fits
         d.fit_type = keep["fit_type_f"].ctypes.data

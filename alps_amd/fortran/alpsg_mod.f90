@@ -27,6 +27,8 @@ module alpsg
      type(c_ptr) :: poly_kind = c_null_ptr, poly_order = c_null_ptr, logfit = c_null_ptr
      type(c_ptr) :: poly_log_max = c_null_ptr, poly_fit_coeffs = c_null_ptr
      integer(c_int) :: poly_max_order
+     type(c_ptr) :: bM_nmax = c_null_ptr, bM_Bessel_zero = c_null_ptr, bM_beta = c_null_ptr
+     type(c_ptr) :: bM_alpha = c_null_ptr, bM_pdrift = c_null_ptr
   end type alpsg_plasma_desc
 
   type, bind(C), public :: alpsg_root_params

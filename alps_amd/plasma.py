@@ -59,6 +59,14 @@ class Plasma:
     logfit: np.ndarray = None            # [nspec]
     poly_log_max: np.ndarray = None      # [nspec]
     poly_fit_coeffs: np.ndarray = None   # [nspec, nperp+1, maxorder+1]
+    # closed-form species (use_bM) and their &bM_spec parameters
+    # (ALPS_var.f90:332-350, defaults of ALPS_io.f90:145-150)
+    usebM: np.ndarray = None             # [nspec] 0/1
+    bM_nmax: np.ndarray = None           # [nspec], 500
+    bM_Bessel_zero: np.ndarray = None    # [nspec], 1e-50
+    bM_beta: np.ndarray = None           # [nspec], 1 (0 = cold plasma)
+    bM_alpha: np.ndarray = None          # [nspec], 1
+    bM_pdrift: np.ndarray = None         # [nspec], 0
     roots: RootParams = field(default_factory=RootParams)
 
     def __post_init__(self):
@@ -88,6 +96,12 @@ class Plasma:
             self.poly_log_max = np.zeros(n)
         if self.poly_fit_coeffs is None:
             self.poly_fit_coeffs = np.zeros((n, self.nperp + 1, 1))
+        for name, val, dt in (("usebM", 0, np.int32), ("bM_nmax", 500, np.int32),
+                              ("bM_Bessel_zero", 1e-50, np.float64), ("bM_beta", 1.0, np.float64),
+                              ("bM_alpha", 1.0, np.float64), ("bM_pdrift", 0.0, np.float64)):
+            v = getattr(self, name)
+            setattr(self, name, np.full(n, val, dtype=dt) if v is None
+                    else np.asarray(v, dtype=dt).ravel().copy())
         self.ACmethod = np.asarray(self.ACmethod, dtype=np.int32)
         self.n_fits = np.asarray(self.n_fits, dtype=np.int32)
         self.fit_type = np.ascontiguousarray(self.fit_type, dtype=np.int32)

@@ -19,7 +19,7 @@ extern "C" {
 
 #define ALPSG_OK 0
 #define ALPSG_E_ARG 1         /* invalid argument / shape */
-#define ALPSG_E_UNSUPPORTED 2 /* feature outside the engine (usebM, relativistic, ...) */
+#define ALPSG_E_UNSUPPORTED 2 /* feature outside the engine (relativistic, ...) */
 #define ALPSG_E_DEVICE 3      /* HIP runtime error */
 #define ALPSG_E_STATE 4       /* call order (e.g. disp before set_k) */
 #define ALPSG_E_NOTFOUND 5    /* no root survived a scan (reference alps_error(9)) */
@@ -49,7 +49,7 @@ typedef struct alpsg_plasma_desc {
   int n_resonance_interval;
   double Tlim;
   double Bessel_zero;
-  const int *usebM;        /* [nspec] or NULL; nonzero -> ALPSG_E_UNSUPPORTED */
+  const int *usebM;        /* [nspec] or NULL; nonzero: closed-form (NHDS) species */
   const int *relativistic; /* [nspec] or NULL; nonzero -> ALPSG_E_UNSUPPORTED */
   const int *ACmethod;     /* [nspec]: 1 fits, 2 Chebyshev */
   const int *n_fits;       /* [nspec] */
@@ -62,6 +62,11 @@ typedef struct alpsg_plasma_desc {
   const double *poly_log_max;                 /* [nspec] */
   const double *poly_fit_coeffs;
   int poly_max_order;
+  /* closed-form species (usebM[s] != 0): the &bM_spec parameters
+     (ALPS_var.f90:332-350), [nspec] each; may be NULL when no species uses it.
+     bM_beta 0 selects the cold-plasma susceptibility. */
+  const int *bM_nmax;
+  const double *bM_Bessel_zero, *bM_beta, *bM_alpha, *bM_pdrift;
 } alpsg_plasma_desc;
 
 /* Root-search controls (&system namelist, ALPS_var.f90:147-165). */

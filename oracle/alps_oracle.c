@@ -189,7 +189,12 @@ double orc_bessj(int n, double x) {
 
 /* derivative_f0, ALPS_fns.f90:96-118 (central differences on the interior). */
 void orc_derivative_f0(orc_plasma *P) {
-  for (int is = 1; is <= P->nspec; ++is)
+  for (int is = 1; is <= P->nspec; ++is) {
+    if (P->usebM[is - 1]) { /* no derivatives for closed-form species (:98-100) */
+      for (int iperp = 0; iperp <= P->nperp; ++iperp)
+        for (int ipar = 0; ipar <= P->npar; ++ipar) DF0(P, is, iperp, ipar, 1) = DF0(P, is, iperp, ipar, 2) = 0.0;
+      continue;
+    }
     for (int iperp = 1; iperp <= P->nperp - 1; ++iperp)
       for (int ipar = 1; ipar <= P->npar - 1; ++ipar) {
         DF0(P, is, iperp, ipar, 1) = (F0(P, is, iperp + 1, ipar) - F0(P, is, iperp - 1, ipar)) /
@@ -197,6 +202,7 @@ void orc_derivative_f0(orc_plasma *P) {
         DF0(P, is, iperp, ipar, 2) = (F0(P, is, iperp, ipar + 1) - F0(P, is, iperp, ipar - 1)) /
                                      (PP(P, is, iperp, ipar + 1, 2) - PP(P, is, iperp, ipar - 1, 2));
       }
+  }
 }
 
 /* determine_nmax, ALPS_fns.f90:3971-4075, including the "more processes than
@@ -205,6 +211,11 @@ int orc_determine_nmax(orc_plasma *P) {
   const int ipar = 1;
   int max_procs = P->nspec;
   for (int is = 1; is <= P->nspec; ++is) {
+    if (P->usebM[is - 1]) { /* :4012-4013 */
+      P->nmax[is - 1] = 1;
+      max_procs = max_procs + 1;
+      continue;
+    }
     int nn = 0;
     double besselmax = 10.0;
     while (besselmax > P->Bessel_zero) {
@@ -222,7 +233,7 @@ int orc_determine_nmax(orc_plasma *P) {
   }
   int is = 1;
   while (max_procs < (P->nproc - 1)) {
-    P->nmax[is - 1] = P->nmax[is - 1] + 1;
+    if (!P->usebM[is - 1]) P->nmax[is - 1] = P->nmax[is - 1] + 1;
     is = is + 1;
     max_procs = max_procs + 1;
     if (is > P->nspec) is = 1;
@@ -839,6 +850,19 @@ double complex orc_disp(orc_plasma *P, double complex om, double complex *chi0_o
   static const int SI[6] = {0, 1, 2, 0, 0, 1}, SJ[6] = {0, 1, 2, 1, 2, 2};
 
   for (int sproc = 1; sproc <= nspec; ++sproc) {
+    double complex schi[3][3] = {{0}};
+    double complex schi_low[3][3][3];
+    memset(schi_low, 0, sizeof(schi_low));
+    if (P->usebM[sproc - 1]) {
+      /* closed form on the rank holding n = 0, ALPS_fns.f90:338-360 */
+      double complex cn[3][3], cl[3][3][3];
+      if (orc_calc_chi(P, sproc, P->kpar, P->kperp, om, cn, cl) != 0) return NAN;
+      const double nq = P->ns[sproc - 1] * P->qs[sproc - 1];
+      for (int k = 0; k < 6; ++k) {
+        schi[SI[k]][SJ[k]] = cn[SI[k]][SJ[k]] / nq;
+        for (int q = 0; q < 3; ++q) schi_low[SI[k]][SJ[k]][q] = cl[SI[k]][SJ[k]][q] / nq;
+      }
+    } else {
     int nmax = P->nmax[sproc - 1];
     double complex (*tp)[6] = malloc(sizeof(double complex[6]) * (nmax + 1));
     double complex (*tm)[6] = malloc(sizeof(double complex[6]) * (nmax + 1));
@@ -849,9 +873,6 @@ double complex orc_disp(orc_plasma *P, double complex om, double complex *chi0_o
     for (int nn = 0; nn <= nmax; ++nn) harmonic_terms(P, sproc, om, nn, tp[nn], tm[nn]);
 
     /* accumulation in increasing n, as the rank with nlim = [0, nmax] does */
-    double complex schi[3][3] = {{0}};
-    double complex schi_low[3][3][3];
-    memset(schi_low, 0, sizeof(schi_low));
     for (int nn = 0; nn <= nmax; ++nn) {
       for (int k = 0; k < 6; ++k) {
         if (nn == 0) {
@@ -877,6 +898,7 @@ double complex orc_disp(orc_plasma *P, double complex om, double complex *chi0_o
     } else {
       schi[2][2] = schi[2][2] + P->kperp * P->kperp * ee;
       schi_low[2][2][1] = schi_low[2][2][1] + P->kperp * P->kperp * ee;
+    }
     }
     double norm = P->ns[sproc - 1] * P->qs[sproc - 1];
     for (int k = 0; k < 6; ++k) {

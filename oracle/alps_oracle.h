@@ -65,6 +65,12 @@ typedef struct {
   /* statistics */
   long long n_disp_calls;
   int n_threads;              /* OpenMP threads used inside disp (0 = default) */
+  /* closed-form species (use_bM = T) and their &bM_spec parameters
+     (ALPS_var.f90:332-350; defaults ALPS_io.f90:145-150) */
+  int usebM[ORC_MAXSPEC];
+  int bM_nmax[ORC_MAXSPEC];
+  double bM_Bessel_zero[ORC_MAXSPEC], bM_beta[ORC_MAXSPEC], bM_alpha[ORC_MAXSPEC];
+  double bM_pdrift[ORC_MAXSPEC];
 } orc_plasma;
 
 /* Branch counters incremented by the hot-path restatement (test instrumentation). */
@@ -115,6 +121,15 @@ double complex orc_eval_fit(const orc_plasma *P, int is, int iperp, double compl
    all row-major [s][i][j] / [s][i][j][n+1]. */
 double complex orc_disp(orc_plasma *P, double complex om, double complex *chi0,
                         double complex *chi0_low, double complex *wave);
+
+/* Closed-form bi-Maxwellian / cold susceptibility (ALPS_NHDS.f90, alps_oracle_nhds.c).
+   chi [3][3], chi_low [3][3][3] (n = -1, 0, 1 last); j is the 1-based species. */
+double orc_bessi(int n, double x);
+double orc_besselI(int n, double x);
+void orc_wofz(double xi, double yi, double *u, double *v, int *flag);
+int orc_nhds_nmaxrun(const orc_plasma *P, int j, double kperp);
+int orc_calc_chi(const orc_plasma *P, int j, double kz, double kperp, double complex x,
+                 double complex chi[3][3], double complex chi_low[3][3][3]);
 
 /* Root finders (ALPS_fns.f90:1815-2195). Return the number of iterations. */
 int orc_secant(orc_plasma *P, double complex *om);

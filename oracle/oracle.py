@@ -41,6 +41,9 @@ class _Plasma(C.Structure):
         ("D_gap", C.c_double), ("D_tol", C.c_double), ("secant_method", C.c_int),
         ("nproc", C.c_int),
         ("n_disp_calls", C.c_longlong), ("n_threads", C.c_int),
+        ("usebM", C.c_int * MAXSPEC), ("bM_nmax", C.c_int * MAXSPEC),
+        ("bM_Bessel_zero", C.c_double * MAXSPEC), ("bM_beta", C.c_double * MAXSPEC),
+        ("bM_alpha", C.c_double * MAXSPEC), ("bM_pdrift", C.c_double * MAXSPEC),
     ]
 
 
@@ -95,6 +98,14 @@ def lib():
         L.orc_secant_osc.argtypes = [P, dp]
         L.orc_secant.restype = C.c_int
         L.orc_secant.argtypes = [P, dp]
+        L.orc_bessi.restype = C.c_double
+        L.orc_bessi.argtypes = [C.c_int, C.c_double]
+        L.orc_wofz.restype = None
+        L.orc_wofz.argtypes = [C.c_double, C.c_double, dp, dp, C.POINTER(C.c_int)]
+        L.orc_nhds_nmaxrun.restype = C.c_int
+        L.orc_nhds_nmaxrun.argtypes = [P, C.c_int, C.c_double]
+        L.orc_calc_chi.restype = C.c_int
+        L.orc_calc_chi.argtypes = [P, C.c_int, C.c_double, C.c_double, _C128, dp, dp]
         L.orc_rtsec.restype = _C128
         L.orc_rtsec.argtypes = [P, _C128, C.POINTER(C.c_int)]
         L.orc_map_search.restype = C.c_int
@@ -111,6 +122,20 @@ def lib():
 
 BRANCHES = ["nonres", "res_pos", "res_neg", "res_real", "edge_low", "edge_high", "pv_tlim",
             "pv_analytic", "tiny_rest", "landau_zero"]
+
+
+def bessi(n: int, x: float) -> float:
+    """BESSI of ALPS_NHDS.f90: I_n(x) exp(-x)."""
+    return lib().orc_bessi(n, x)
+
+
+def wofz(z) -> complex:
+    """WOFZ of ALPS_NHDS.f90: the Faddeeva function w(z)."""
+    u, v, f = C.c_double(), C.c_double(), C.c_int()
+    lib().orc_wofz(complex(z).real, complex(z).imag, C.byref(u), C.byref(v), C.byref(f))
+    if f.value:
+        raise OverflowError("WOFZ overflow flag")
+    return complex(u.value, v.value)
 
 
 def reset_branch_counts():
@@ -165,6 +190,12 @@ class Oracle:
             s.poly_order[i] = int(p.poly_order[i])
             s.logfit[i] = int(p.logfit[i])
             s.poly_log_max[i] = float(p.poly_log_max[i])
+            s.usebM[i] = int(p.usebM[i])
+            s.bM_nmax[i] = int(p.bM_nmax[i])
+            s.bM_Bessel_zero[i] = float(p.bM_Bessel_zero[i])
+            s.bM_beta[i] = float(p.bM_beta[i])
+            s.bM_alpha[i] = float(p.bM_alpha[i])
+            s.bM_pdrift[i] = float(p.bM_pdrift[i])
         s.vA = p.vA
         s.kperp_norm = int(bool(p.kperp_norm))
         s.positions_principal = int(p.positions_principal)
@@ -248,6 +279,20 @@ class Oracle:
         z = np.array([complex(om)], dtype=np.complex128)
         it = lib().orc_secant(C.byref(self.s), _ptr(z))
         return complex(z[0]), it
+
+    def calc_chi(self, sproc: int, om, kpar=None, kperp=None):
+        """NHDS chi [3,3] and chi_low [3,3,3] (n = -1, 0, 1) of species sproc (1-based)."""
+        chi = np.zeros(9, dtype=np.complex128)
+        low = np.zeros(27, dtype=np.complex128)
+        kz = self.s.kpar if kpar is None else kpar
+        kp = self.s.kperp if kperp is None else kperp
+        rc = lib().orc_calc_chi(C.byref(self.s), sproc, kz, kp, _c(om), _ptr(chi), _ptr(low))
+        if rc != 0:
+            raise ValueError("calc_chi stops here (cold species with kperp_norm = F)")
+        return chi.reshape(3, 3), low.reshape(3, 3, 3)
+
+    def nhds_nmaxrun(self, sproc: int, kperp: float) -> int:
+        return lib().orc_nhds_nmaxrun(C.byref(self.s), sproc, kperp)
 
     def rtsec(self, om):
         flag = C.c_int(0)

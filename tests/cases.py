@@ -56,6 +56,20 @@ def poly_case(nperp=24, npar=48, order=8):
                   poly_fit_coeffs=coeffs)
 
 
+def bm_mix(kperp_norm=True, cold=True):
+    """Closed-form (use_bM) species beside a tabulated one: an anisotropic
+    drifting bi-Maxwellian proton population and, with ``cold``, a cold
+    drifting alpha population (test_cold_plasma.in style), both use_bM, plus
+    tabulated electrons."""
+    import dataclasses
+    p = three_species()
+    p = dataclasses.replace(p, kperp_norm=kperp_norm)
+    return dataclasses.replace(
+        p, usebM=[1, 0, 1 if cold else 0], bM_nmax=[500, 500, 500],
+        bM_Bessel_zero=[1e-50, 1e-50, 1e-50], bM_beta=[1.0, 1.0, 0.0 if cold else 1.0],
+        bM_alpha=[1.5, 1.0, 1.0], bM_pdrift=[0.2, 0.0, 1.0])
+
+
 def kpar_fast(nperp=120, npar=240):
     """The reference's tests/test_kpar_fast.in (with test_kpar_fast_dist.in)."""
     rp = RootParams(secant_method=2, numiter=50, D_threshold=1e-15, D_prec=1e-5, D_gap=1e-5)

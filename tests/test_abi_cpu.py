@@ -79,9 +79,17 @@ def test_create_rejects_bad_input_without_a_device():
     d, keep = _desc_for(p, nspec=0)
     rc, msg, h = _create(d)
     assert rc == 1 and "nspec" in msg and not h.value
+    # a use_bM species needs its &bM_spec parameters
     d, keep = _desc_for(p, usebM=np.array([1, 0], dtype=np.int32))
     rc, msg, _ = _create(d)
-    assert rc == 2 and "use_bM" in msg
+    assert rc == 1 and "bM_spec" in msg
+    # cold use_bM species need kperp_norm (calc_chi_cold stops otherwise)
+    bm = dict(usebM=np.array([1, 0], dtype=np.int32), bM_nmax=np.array([500, 500], dtype=np.int32),
+              bM_Bessel_zero=np.array([1e-50, 1e-50]), bM_beta=np.array([0.0, 1.0]),
+              bM_alpha=np.array([1.0, 1.0]), bM_pdrift=np.array([0.0, 0.0]))
+    d, keep = _desc_for(p, kperp_norm=0, **bm)
+    rc, msg, _ = _create(d)
+    assert rc == 2 and "cold" in msg
     d, keep = _desc_for(p, relativistic=np.array([0, 1], dtype=np.int32))
     rc, msg, _ = _create(d)
     assert rc == 2 and "relativistic" in msg

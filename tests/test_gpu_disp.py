@@ -73,6 +73,14 @@ def test_chebyshev_continuation():
     _check(p, 0.3, 0.3, cases.BRANCH_OMEGAS)
 
 
+@pytest.mark.parametrize("kperp_norm,cold", [(True, True), (False, False)])
+def test_closed_form_species(kperp_norm, cold):
+    """use_bM species (NHDS calc_chi, hot and cold) beside a tabulated one."""
+    p = cases.bm_mix(kperp_norm=kperp_norm, cold=cold)
+    _check(p, 0.5, 0.2, cases.BRANCH_OMEGAS)
+    _check(p, 0.5, -0.2, cases.BRANCH_OMEGAS[:4])
+
+
 def test_single_omega_paths_agree():
     """The plain one-omega-per-launch path gives the batched results."""
     from alps_amd.engine import Engine
