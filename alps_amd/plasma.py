@@ -22,7 +22,7 @@ import numpy as np
 
 from . import distribution as _dist
 
-__all__ = ["Plasma", "RootParams", "bimaxwellian_plasma", "kappa_plasma"]
+__all__ = ["Plasma", "RootParams", "bimaxwellian_plasma", "kappa_plasma", "juttner_plasma"]
 
 
 @dataclass
@@ -67,6 +67,16 @@ class Plasma:
     bM_beta: np.ndarray = None           # [nspec], 1 (0 = cold plasma)
     bM_alpha: np.ndarray = None          # [nspec], 1
     bM_pdrift: np.ndarray = None         # [nspec], 0
+    # relativistic species and their (Gamma, pparbar) tables as derivative_f0_rel
+    # leaves them (ALPS_fns_rel.f90:19-408); the *_rel arrays hold the
+    # relativistic species only, in species order: [nspec_rel, ngamma+1, npparbar+1]
+    relativistic: np.ndarray = None      # [nspec] 0/1
+    ngamma: int = 0
+    npparbar: int = 0
+    gamma_rel: np.ndarray = None
+    pparbar_rel: np.ndarray = None
+    f0_rel: np.ndarray = None
+    df0_rel: np.ndarray = None           # [nspec_rel, ngamma+1, npparbar+1, 2]
     roots: RootParams = field(default_factory=RootParams)
 
     def __post_init__(self):
@@ -102,6 +112,19 @@ class Plasma:
             v = getattr(self, name)
             setattr(self, name, np.full(n, val, dtype=dt) if v is None
                     else np.asarray(v, dtype=dt).ravel().copy())
+        self.relativistic = (np.zeros(n, dtype=np.int32) if self.relativistic is None
+                             else np.asarray(self.relativistic, dtype=np.int32).ravel().copy())
+        nrel = int(self.relativistic.sum())
+        if nrel:
+            shp = (nrel, self.ngamma + 1, self.npparbar + 1)
+            for name in ("gamma_rel", "pparbar_rel", "f0_rel"):
+                a = np.ascontiguousarray(getattr(self, name), dtype=np.float64)
+                if a.shape != shp:
+                    raise ValueError(f"{name} must have shape {shp}")
+                setattr(self, name, a)
+            self.df0_rel = np.ascontiguousarray(self.df0_rel, dtype=np.float64)
+            if self.df0_rel.shape != shp + (2,):
+                raise ValueError(f"df0_rel must have shape {shp + (2,)}")
         self.ACmethod = np.asarray(self.ACmethod, dtype=np.int32)
         self.n_fits = np.asarray(self.n_fits, dtype=np.int32)
         self.fit_type = np.ascontiguousarray(self.fit_type, dtype=np.int32)
@@ -129,14 +152,22 @@ class Plasma:
     def max_fits(self) -> int:
         return self.fit_type.shape[1]
 
+    @property
+    def nspec_rel(self) -> int:
+        return int(self.relativistic.sum())
+
     def fortran(self) -> dict:
         """Column-major copies with the Fortran dimension order (species fastest)."""
         f = np.asfortranarray
-        return dict(
+        out = dict(
             pp=f(self.pp), f0=f(self.f0), fit_type=f(self.fit_type),
             perp_correction=f(self.perp_correction), param_fit=f(self.param_fit),
             poly_fit_coeffs=f(self.poly_fit_coeffs),
         )
+        if self.nspec_rel:
+            out.update(gamma_rel=f(self.gamma_rel), pparbar_rel=f(self.pparbar_rel),
+                       f0_rel=f(self.f0_rel), df0_rel=f(self.df0_rel))
+        return out
 
 
 def bimaxwellian_plasma(ms, qs, ns, taus, alphs, ps, beta, vA, nperp, npar, maxP,
@@ -178,3 +209,34 @@ def kappa_plasma(ms, qs, ns, taus, alphs, ps, kappas, beta, vA, nperp, npar, max
     roots = numerics.pop("roots", RootParams())
     return Plasma(pp=pp, f0=f0, ns=ns, qs=qs, ms=ms, vA=vA, fit_type=np.full((nspec, 1), 2),
                   perp_correction=perp_corr, param_fit=param_fit, roots=roots, **numerics)
+
+
+def juttner_plasma(ms, qs, ns, taus, beta, vA, nperp, npar, maxP, ngamma, npparbar,
+                   analytic_rel=False, **numerics) -> Plasma:
+    """A plasma of relativistic Juettner species (distributions=3, alphs = 1, no
+    drift) with fit_type-4 fits (Gamma only) at the generator's ideal values,
+    and the (Gamma, pparbar) tables of derivative_f0_rel.  ``analytic_rel``
+    evaluates f0 on the new grid from the Juettner formula instead of the
+    reference's spline of the table (needed for large tables)."""
+    import math
+    sp = [_dist.SpeciesDist(ms=m, taus=t, alphs=1.0, ps=0.0, distribution=3)
+          for m, t in zip(ms, taus)]
+    pp, f0, ideal = _dist.generate(sp, beta, vA, nperp, npar, maxP)
+    nspec = len(sp)
+    npf = max(nperp, ngamma)
+    param_fit = np.zeros((nspec, npf + 1, 5, 1))
+    perp_corr = np.ones((nspec, 1))
+    rel = [_dist.relativistic_tables(
+        pp[s], f0[s], ms[s], vA, ngamma, npparbar,
+        f_of=(lambda g, q, c=ideal[s]["perpcorr"]: np.exp(-c * g)) if analytic_rel else None)
+        for s in range(nspec)]
+    for s in range(nspec):
+        perp_corr[s, 0] = ideal[s]["perpcorr"]
+        param_fit[s, :, 0, 0] = ideal[s]["fit_1"]
+    roots = numerics.pop("roots", RootParams())
+    return Plasma(pp=pp, f0=f0, ns=ns, qs=qs, ms=ms, vA=vA, fit_type=np.full((nspec, 1), 4),
+                  perp_correction=perp_corr, param_fit=param_fit, roots=roots,
+                  relativistic=np.ones(nspec, dtype=np.int32), ngamma=ngamma, npparbar=npparbar,
+                  gamma_rel=np.stack([r[0] for r in rel]), pparbar_rel=np.stack([r[1] for r in rel]),
+                  f0_rel=np.stack([r[2] for r in rel]), df0_rel=np.stack([r[3] for r in rel]),
+                  **numerics)

@@ -70,8 +70,9 @@ void orc_reset_branch_counts(void) { memset(orc_branch_counts, 0, sizeof(orc_bra
 #define DF0(P, is, iperp, ipar, k) \
   ((P)->df0[((((size_t)((is)-1) * ((P)->nperp + 1) + (iperp)) * ((P)->npar + 1) + (ipar)) * 2) + ((k)-1)])
 /* param_fit(1:nspec,0:nperp,5,nfits)   ALPS_io.f90:188 */
+#define PARAM_NP(P) ((P)->param_np > 0 ? (P)->param_np : (P)->nperp)
 #define PARAM(P, is, iperp, k, ifit) \
-  ((P)->param_fit[((((size_t)((is)-1) * ((P)->nperp + 1) + (iperp)) * 5 + ((k)-1)) * ORC_MAXFITS) + ((ifit)-1)])
+  ((P)->param_fit[((((size_t)((is)-1) * (PARAM_NP(P) + 1) + (iperp)) * 5 + ((k)-1)) * ORC_MAXFITS) + ((ifit)-1)])
 /* poly_fit_coeffs(1:nspec,0:nperp,0:order) */
 #define POLYC(P, is, iperp, n) \
   ((P)->poly_fit_coeffs[(((size_t)((is)-1) * ((P)->nperp + 1) + (iperp)) * (P)->poly_stride) + (n)])
@@ -298,6 +299,10 @@ static double complex fit_function(const orc_plasma *P, int is, int iperp, doubl
       double complex sq =
           csqrt(1.0 + ((pperp_val * pperp_val) + csq(ppar_val - p3)) * vA * vA / (m * m));
       f = f + p1 * cexp(-p2 * sq);
+    } else if (ft == 4) { /* Juettner in Gamma only */
+      f = f + p1 * exp(-pc * pperp_val);
+    } else if (ft == 5) { /* Juettner in Gamma and pparbar */
+      f = f + p1 * exp(-pc * pperp_val) * cexp(-p2 * csq(ppar_val - p3));
     } else if (ft == 6) { /* bi-Moyal */
       double complex arg = p4 * pc * (pperp_val * pperp_val) + p2 * csq(ppar_val - p3);
       f = f + p1 * cexp(0.5 * (arg - cexp(arg)));
@@ -338,7 +343,17 @@ static double complex fit_function_poly(const orc_plasma *P, int is, int iperp,
 /* eval_fit, ALPS_analyt.f90:32-181 (ACmethod 1 and 2). */
 double complex orc_eval_fit(const orc_plasma *P, int is, int iperp, double complex ppar_valC) {
   if (P->ACmethod[is - 1] == 2) return fit_function_poly(P, is, iperp, ppar_valC);
-  return fit_function(P, is, iperp, PP(P, is, iperp, 1, 1), ppar_valC);
+  /* pperp_val is set per fit in order; the last fit's choice is used for all
+     (ALPS_analyt.f90:79-151): pperp for types 1-3 and 6, Gamma for 4 and 5 */
+  double pperp_val = 0.0;
+  for (int ifit = 1; ifit <= P->n_fits[is - 1]; ++ifit) {
+    const int ft = P->fit_type[is - 1][ifit - 1];
+    if (ft == 4 || ft == 5)
+      pperp_val = P->gamma_rel[is - 1][(size_t)iperp * (P->npparbar + 1) + 1];
+    else
+      pperp_val = PP(P, is, iperp, 1, 1);
+  }
+  return fit_function(P, is, iperp, pperp_val, ppar_valC);
 }
 
 /* ========================================================================
@@ -348,6 +363,10 @@ double complex orc_eval_fit(const orc_plasma *P, int is, int iperp, double compl
 /* determine_resonances, ALPS_fns.f90:641-745 (non-relativistic branch). */
 void orc_determine_resonances(const orc_plasma *P, int sproc, double complex om, int nn,
                               int *found_plus, int *found_minus) {
+  if (P->relativistic[sproc - 1]) {
+    orc_determine_resonances_rel(P, sproc, om, nn, found_plus, found_minus);
+    return;
+  }
   int npar = P->npar, ipar;
   double dppar = PP(P, sproc, 2, 2, 2) - PP(P, sproc, 2, 1, 2);
   double ms = P->ms[sproc - 1], qs = P->qs[sproc - 1], kpar = P->kpar;
@@ -463,11 +482,15 @@ static double complex int_T_res(const orc_plasma *P, int sproc, int nn, int iper
   return 0.0;
 }
 
-/* resU, ALPS_fns.f90:1560-1596 (gamma = 1, non-relativistic). */
+/* resU, ALPS_fns.f90:1560-1596 (gamma of the grid point for relativistic species). */
 static inline double complex resU(const orc_plasma *P, int sproc, double complex om, int nn,
                                   int iperp, int ipar) {
-  const double gamma = 1.0;
+  double gamma = 1.0;
   double qs = P->qs[sproc - 1], ms = P->ms[sproc - 1], kpar = P->kpar;
+  if (P->relativistic[sproc - 1]) {
+    const double a = PP(P, sproc, iperp, ipar, 1), b = PP(P, sproc, iperp, ipar, 2);
+    gamma = sqrt((a * a + b * b) * (P->vA * P->vA) / (ms * ms) + 1.0);
+  }
   double df1 = DF0(P, sproc, iperp, ipar, 1), df2 = DF0(P, sproc, iperp, ipar, 2);
   double p1 = PP(P, sproc, iperp, ipar, 1), p2 = PP(P, sproc, iperp, ipar, 2);
   double complex num = qs * (om * df1 + (kpar / (gamma * ms)) * (p1 * df2 - p2 * df1));
@@ -791,6 +814,13 @@ double complex orc_full_integrate(const orc_plasma *P, int sproc, double complex
     COUNT(ORC_BR_NONRES);
     return orc_integrate(P, sproc, om, nn, mode, 1, P->npar - 1);
   }
+  if (P->relativistic[sproc - 1]) { /* ALPS_fns.f90:774-781 */
+    if (cimag(om) > 0.0) return orc_integrate_res_rel(P, sproc, om, nn, mode);
+    if (cimag(om) < 0.0)
+      return orc_integrate_res_rel(P, sproc, om, nn, mode) +
+             2.0 * orc_landau_integrate_rel(P, sproc, om, nn, mode);
+    return orc_integrate_res_rel(P, sproc, om, nn, mode) + orc_landau_integrate_rel(P, sproc, om, nn, mode);
+  }
   if (cimag(om) > 0.0) {
     COUNT(ORC_BR_RES_POS);
     return orc_integrate_res(P, sproc, om, nn, mode);
@@ -890,7 +920,15 @@ double complex orc_disp(orc_plasma *P, double complex om, double complex *chi0_o
     }
     free(tp);
     free(tm);
-    /* ee term (nlim(1) == 0), ALPS_fns.f90:479-496 */
+    /* ee term (nlim(1) == 0), ALPS_fns.f90:479-496; the relativistic one is
+       added to chi only, not to chi_low */
+    if (P->relativistic[sproc - 1]) {
+      double complex eer = orc_int_ee_rel(P, sproc);
+      if (P->kperp_norm)
+        schi[2][2] = schi[2][2] + eer;
+      else
+        schi[2][2] = schi[2][2] + P->kperp * P->kperp * eer;
+    } else {
     double complex ee = orc_int_ee(P, sproc);
     if (P->kperp_norm) {
       schi[2][2] = schi[2][2] + ee;
@@ -898,6 +936,7 @@ double complex orc_disp(orc_plasma *P, double complex om, double complex *chi0_o
     } else {
       schi[2][2] = schi[2][2] + P->kperp * P->kperp * ee;
       schi_low[2][2][1] = schi_low[2][2][1] + P->kperp * P->kperp * ee;
+    }
     }
     }
     double norm = P->ns[sproc - 1] * P->qs[sproc - 1];

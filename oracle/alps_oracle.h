@@ -71,6 +71,15 @@ typedef struct {
   int bM_nmax[ORC_MAXSPEC];
   double bM_Bessel_zero[ORC_MAXSPEC], bM_beta[ORC_MAXSPEC], bM_alpha[ORC_MAXSPEC];
   double bM_pdrift[ORC_MAXSPEC];
+  /* relativistic species (ALPS_var.f90: relativistic, ngamma, npparbar, the
+     *_rel grids of derivative_f0_rel) and the upper bound of param_fit's
+     second dimension, max(nperp, ngamma) (0: nperp) */
+  int relativistic[ORC_MAXSPEC];
+  int ngamma, npparbar;
+  int param_np;
+  /* per species (NULL unless relativistic): [ngamma+1][npparbar+1], df0_rel [..][..][2] */
+  double *gamma_rel[ORC_MAXSPEC], *pparbar_rel[ORC_MAXSPEC], *f0_rel[ORC_MAXSPEC],
+      *df0_rel[ORC_MAXSPEC];
 } orc_plasma;
 
 /* Branch counters incremented by the hot-path restatement (test instrumentation). */
@@ -130,6 +139,17 @@ void orc_wofz(double xi, double yi, double *u, double *v, int *flag);
 int orc_nhds_nmaxrun(const orc_plasma *P, int j, double kperp);
 int orc_calc_chi(const orc_plasma *P, int j, double kz, double kperp, double complex x,
                  double complex chi[3][3], double complex chi_low[3][3][3]);
+
+/* Relativistic path (ALPS_fns_rel.f90, alps_oracle_rel.c).  orc_rel_error is
+   set to 8 where the reference calls alps_error(8). */
+extern int orc_rel_error;
+double orc_gamma_fn(double x);
+double complex orc_cbessj(double complex z, int nu);
+double complex orc_integrate_res_rel(const orc_plasma *P, int s, double complex om, int nn, int mode);
+double complex orc_landau_integrate_rel(const orc_plasma *P, int s, double complex om, int nn, int mode);
+double complex orc_int_ee_rel(const orc_plasma *P, int s);
+void orc_determine_resonances_rel(const orc_plasma *P, int s, double complex om, int nn, int *fp,
+                                  int *fm);
 
 /* Root finders (ALPS_fns.f90:1815-2195). Return the number of iterations. */
 int orc_secant(orc_plasma *P, double complex *om);

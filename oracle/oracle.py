@@ -44,6 +44,10 @@ class _Plasma(C.Structure):
         ("usebM", C.c_int * MAXSPEC), ("bM_nmax", C.c_int * MAXSPEC),
         ("bM_Bessel_zero", C.c_double * MAXSPEC), ("bM_beta", C.c_double * MAXSPEC),
         ("bM_alpha", C.c_double * MAXSPEC), ("bM_pdrift", C.c_double * MAXSPEC),
+        ("relativistic", C.c_int * MAXSPEC), ("ngamma", C.c_int), ("npparbar", C.c_int),
+        ("param_np", C.c_int),
+        ("gamma_rel", C.POINTER(C.c_double) * MAXSPEC), ("pparbar_rel", C.POINTER(C.c_double) * MAXSPEC),
+        ("f0_rel", C.POINTER(C.c_double) * MAXSPEC), ("df0_rel", C.POINTER(C.c_double) * MAXSPEC),
     ]
 
 
@@ -106,6 +110,12 @@ def lib():
         L.orc_nhds_nmaxrun.argtypes = [P, C.c_int, C.c_double]
         L.orc_calc_chi.restype = C.c_int
         L.orc_calc_chi.argtypes = [P, C.c_int, C.c_double, C.c_double, _C128, dp, dp]
+        L.orc_gamma_fn.restype = C.c_double
+        L.orc_gamma_fn.argtypes = [C.c_double]
+        L.orc_cbessj.restype = _C128
+        L.orc_cbessj.argtypes = [_C128, C.c_int]
+        L.orc_int_ee_rel.restype = _C128
+        L.orc_int_ee_rel.argtypes = [P, C.c_int]
         L.orc_rtsec.restype = _C128
         L.orc_rtsec.argtypes = [P, _C128, C.POINTER(C.c_int)]
         L.orc_map_search.restype = C.c_int
@@ -136,6 +146,20 @@ def wofz(z) -> complex:
     if f.value:
         raise OverflowError("WOFZ overflow flag")
     return complex(u.value, v.value)
+
+
+def gamma_fn(x: float) -> float:
+    """Gamma of ALPS_fns_rel.f90 (Lanczos series)."""
+    return lib().orc_gamma_fn(x)
+
+
+def cbessj(z, nu: int) -> complex:
+    """CBESSJ of ALPS_fns_rel.f90: J_nu(z) from 21 terms of its power series."""
+    return _z(lib().orc_cbessj(_c(z), nu))
+
+
+def rel_error() -> int:
+    return C.c_int.in_dll(lib(), "orc_rel_error").value
 
 
 def reset_branch_counts():
@@ -172,7 +196,7 @@ class Oracle:
         self._pp = np.ascontiguousarray(p.pp, dtype=np.float64)
         self._f0 = np.ascontiguousarray(p.f0, dtype=np.float64)
         self._df0 = np.zeros(self._pp.shape)
-        pf = np.zeros((p.nspec, p.nperp + 1, 5, MAXFITS))
+        pf = np.zeros((p.nspec, p.param_fit.shape[1], 5, MAXFITS))
         pf[..., : p.max_fits] = p.param_fit
         self._pf = pf
         self._poly = np.ascontiguousarray(p.poly_fit_coeffs, dtype=np.float64)
@@ -209,6 +233,18 @@ class Oracle:
         s.numiter, s.D_threshold, s.D_prec = r.numiter, r.D_threshold, r.D_prec
         s.D_gap, s.D_tol, s.secant_method = r.D_gap, r.D_tol, r.secant_method
         s.nproc = nproc
+        s.param_np = p.param_fit.shape[1] - 1
+        s.ngamma, s.npparbar = int(p.ngamma), int(p.npparbar)
+        self._rel = []
+        r = 0
+        for i in range(p.nspec):
+            s.relativistic[i] = int(p.relativistic[i])
+            if p.relativistic[i]:
+                arrs = [np.ascontiguousarray(a[r]) for a in
+                        (p.gamma_rel, p.pparbar_rel, p.f0_rel, p.df0_rel)]
+                self._rel.append(arrs)
+                s.gamma_rel[i], s.pparbar_rel[i], s.f0_rel[i], s.df0_rel[i] = [_ptr(a) for a in arrs]
+                r += 1
         s.n_threads = n_threads
         self.s = s
         L.orc_derivative_f0(C.byref(s))
@@ -293,6 +329,9 @@ class Oracle:
 
     def nhds_nmaxrun(self, sproc: int, kperp: float) -> int:
         return lib().orc_nhds_nmaxrun(C.byref(self.s), sproc, kperp)
+
+    def int_ee_rel(self, sproc: int) -> complex:
+        return _z(lib().orc_int_ee_rel(C.byref(self.s), sproc))
 
     def rtsec(self, om):
         flag = C.c_int(0)
