@@ -46,6 +46,13 @@ MAP = dict(omi=0.01, omf=2.0, gami=-0.5, gamf=0.05)
 KPERP = KPAR = 0.5
 
 
+# HBM traffic of the map-batch k_sweep launch by its (lanes per omega, wavefronts,
+# workgroups per omega block) shape, from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE passes on this workload, each scaled by the counter's reading for
+# a 256 MiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).
+SWEEP_TRAFFIC_BYTES = {(1, 4, 3): 13542.0 * 1024 * 2.0 + 147456.0 * 1024}
+
+
 def flops_per_eval(nmax, npar):
     """Algorithmic FP64 flops of one non-relativistic det D evaluation:
     54 per (species, harmonic, ppar node) of the resonant-denominator sweep."""
@@ -209,7 +216,8 @@ def main():
         "launch_shapes": {str(k): list(v) for k, v in shapes.items()},
         "roofline": {"bound": "mfma", "compute": "fp64 (VALU)", "achieved": achieved,
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": SWEEP_TRAFFIC_BYTES.get(tuple(shapes.get(args.nr * args.ni, (0, 0, 0))[:3])),
                      "flops_per_launch": flops_launch,
                      "kernel": "k_sweep (map batch, %d omegas)" % (args.nr * args.ni),
                      "kernel_ms": float(split[0])},
