@@ -43,10 +43,17 @@ hipError_t launch_resonant(hipStream_t st, int nch, int RB, const ProbDev *prob,
 hipError_t launch_nhds(hipStream_t st, const ProbDev *prob, int nspec, int nom, const double2 *om,
                        const int *active, double2 *chiA, double2 *lowA);
 int nhds_tables(double z, int bm_nmax, double bessel_zero, std::vector<double2> &tab);
+hipError_t launch_rel_bessel(hipStream_t st, int npts, int nmax, double zfac, const double *grel,
+                             const double *pbrel, const int *mtab, double *out);
+hipError_t launch_rel(hipStream_t st, const ProbDev *prob, int nspec, int nmaxR, int NB, int nom,
+                      const double2 *om, const int *active, double2 *chiR, double2 *lowR,
+                      int *flags, int *err);
+int rel_slots(int NB, int nmaxR);
 hipError_t launch_assemble(hipStream_t st, const ProbDev *prob, int nspec, int nom, int W, int RB,
                            const double2 *om, const int *active, const double2 *chiA,
                            const double2 *chiB, const double2 *lowA, const double2 *lowB,
-                           double2 *det, double2 *chi, double2 *low, double2 *wave);
+                           const double2 *chiR, int RR, const double2 *lowR, double2 *det,
+                           double2 *chi, double2 *low, double2 *wave);
 
 namespace {
 
@@ -124,6 +131,18 @@ struct alpsg_ctx {
   double bm_bz[ALPSG_MAXSPEC] = {0}, bm_beta[ALPSG_MAXSPEC] = {0}, bm_alpha[ALPSG_MAXSPEC] = {0},
          bm_pdrift[ALPSG_MAXSPEC] = {0};
   DevBuf<double2> bmi[ALPSG_MAXSPEC];
+  // relativistic species: (Gamma, pparbar) tables, per-k J_n tables, Gamma(m)
+  int relsp[ALPSG_MAXSPEC] = {0};
+  bool any_rel = false;
+  int ngamma = 0, npparbar = 0;
+  DevBuf<double> grel[ALPSG_MAXSPEC], pbrel[ALPSG_MAXSPEC], f0rel[ALPSG_MAXSPEC],
+      relbes[ALPSG_MAXSPEC], gampp[ALPSG_MAXSPEC];
+  DevBuf<double2> dfrel[ALPSG_MAXSPEC];
+  DevBuf<int2> rowlim[ALPSG_MAXSPEC];
+  DevBuf<double> gtab;
+  DevBuf<double2> chiR_d, lowR_d;
+  DevBuf<int> flags_d, err_d;
+  int rel_NB = 1;
   // launch shapes chosen by alpsg_autotune (or pinned by alpsg_set_shape),
   // keyed by the batch-size bucket floor(log2(n))
   int tuned_valid[32] = {0};
@@ -139,6 +158,13 @@ struct alpsg_ctx {
       atab[s].release();
       nodes[s].release();
       bmi[s].release();
+      grel[s].release();
+      pbrel[s].release();
+      f0rel[s].release();
+      relbes[s].release();
+      gampp[s].release();
+      dfrel[s].release();
+      rowlim[s].release();
     }
     mtab.release();
     prob_d.release();
@@ -152,6 +178,11 @@ struct alpsg_ctx {
     lowA_d.release();
     lowB_d.release();
     act_d.release();
+    gtab.release();
+    chiR_d.release();
+    lowR_d.release();
+    flags_d.release();
+    err_d.release();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (ev_sweep) (void)hipEventDestroy(ev_sweep);
@@ -234,10 +265,22 @@ int disp_device(alpsg_ctx *h, int n, const double2 *om_dev, const int *active_de
   if (h->any_bm)
     HIPCHK(launch_nhds(h->st, h->prob_d.p, ns, n, om_dev, active_dev, h->chiA_d.p,
                        want_low ? h->lowA_d.p : nullptr));
+  int RR = 0;
+  if (h->any_rel) {
+    const int nmaxR = h->prob.nmaxR;
+    const int NB = std::max(1, std::min(2 * nmaxR + 1, 1024 / std::max(n, 1)));
+    RR = rel_slots(NB, nmaxR);
+    HIPCHK(h->chiR_d.ensure((size_t)RR * ns * 6 * n));
+    HIPCHK(h->flags_d.ensure((size_t)n * ns * (2 * nmaxR + 1)));
+    if (want_low) HIPCHK(h->lowR_d.ensure((size_t)n * ns * 18));
+    HIPCHK(launch_rel(h->st, h->prob_d.p, ns, nmaxR, NB, n, om_dev, active_dev, h->chiR_d.p,
+                      want_low ? h->lowR_d.p : nullptr, h->flags_d.p, h->err_d.p));
+  }
   HIPCHK(hipEventRecord(h->ev_res, h->st));
   HIPCHK(launch_assemble(h->st, h->prob_d.p, ns, n, TW, sh.rb, om_dev, active_dev, h->chiA_d.p,
                          h->chiB_d.p, want_low ? h->lowA_d.p : nullptr,
-                         want_low ? h->lowB_d.p : nullptr, h->det_d.p,
+                         want_low ? h->lowB_d.p : nullptr, RR ? h->chiR_d.p : nullptr, RR,
+                         (RR && want_low) ? h->lowR_d.p : nullptr, h->det_d.p,
                          want_chi ? h->chi_d.p : nullptr, want_low ? h->low_d.p : nullptr,
                          want_wave ? h->wave_d.p : nullptr));
   return ALPSG_OK;
@@ -376,7 +419,18 @@ int run_disp(alpsg_ctx *h, int n, const double *om, double *det, double *chi, do
   if (wave)
     HIPCHK(hipMemcpyAsync(wave, h->wave_d.p, sizeof(double2) * n * 9, hipMemcpyDeviceToHost,
                           h->st));
+  int relerr = 0;
+  if (h->any_rel)
+    HIPCHK(hipMemcpyAsync(&relerr, h->err_d.p, sizeof(int), hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
+  if (relerr) {
+    const int zero = 0;
+    HIPCHK(hipMemcpy(h->err_d.p, &zero, sizeof(int), hipMemcpyHostToDevice));
+    return fail(ALPSG_E_ARG,
+                "a resonance's principal-value window exceeds the subluminal part of a Gamma row "
+                "(the reference stops with alps_error(8)); refine npparbar or lower "
+                "positions_principal");
+  }
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
   h->last_ms = ms;
@@ -800,8 +854,8 @@ int alpsg_create(const alpsg_plasma_desc *d, alpsg_handle *out) {
   if (P < 1 || P > 16) return fail(ALPSG_E_UNSUPPORTED, "positions_principal must be in 1..16");
   if (nperp < 4 || npar < 2 * P + 8) return fail(ALPSG_E_ARG, "grid too small for positions_principal");
   if (d->n_resonance_interval < 2) return fail(ALPSG_E_ARG, "n_resonance_interval < 2");
+  int nrel = 0;
   for (int s = 0; s < nspec; ++s) {
-    if (d->relativistic && d->relativistic[s]) return fail(ALPSG_E_UNSUPPORTED, "relativistic species are not handled by the engine");
     if (d->qs[s] == 0.0 || d->ms[s] <= 0.0) return fail(ALPSG_E_ARG, "species charge/mass");
     if (d->usebM && d->usebM[s]) {
       // closed-form species: no table, fits or continuation are read
@@ -811,6 +865,26 @@ int alpsg_create(const alpsg_plasma_desc *d, alpsg_handle *out) {
         return fail(ALPSG_E_ARG, "bM_spec parameters");
       if (d->bM_beta[s] == 0.0 && !d->kperp_norm)
         return fail(ALPSG_E_UNSUPPORTED, "cold-plasma species need kperp_norm (the reference stops)");
+      continue;
+    }
+    if (d->relativistic && d->relativistic[s]) {
+      // Gamma-grid species: Juettner fits in Gamma (types 4, 5) indexed by igamma
+      ++nrel;
+      if (!d->gamma_rel || !d->pparbar_rel || !d->f0_rel || !d->df0_rel)
+        return fail(ALPSG_E_ARG, "missing relativistic tables");
+      if (d->ngamma < 4 || d->npparbar < 2 * P + 8)
+        return fail(ALPSG_E_ARG, "relativistic grid too small (ngamma >= 4, npparbar >= 2P+8)");
+      if (d->ACmethod[s] != 1)
+        return fail(ALPSG_E_UNSUPPORTED, "relativistic species need fit continuation (ACmethod 1)");
+      if (d->n_fits[s] < 1 || d->n_fits[s] > ALPSG_MAXFITS || d->n_fits[s] > d->max_fits)
+        return fail(ALPSG_E_ARG, "n_fits out of range");
+      if (!d->fit_type || !d->perp_correction || !d->param_fit) return fail(ALPSG_E_ARG, "missing fit arrays");
+      for (int f = 0; f < d->n_fits[s]; ++f) {
+        const int ft = d->fit_type[s + (size_t)nspec * f];
+        if (ft != 4 && ft != 5)
+          return fail(ALPSG_E_UNSUPPORTED, "relativistic species need fit_type 4 or 5");
+      }
+      if (d->param_fit_nperp < std::max(nperp, d->ngamma)) return fail(ALPSG_E_ARG, "param_fit_nperp < max(nperp, ngamma)");
       continue;
     }
     if (d->ACmethod[s] != 1 && d->ACmethod[s] != 2) return fail(ALPSG_E_UNSUPPORTED, "ACmethod must be 1 or 2");
@@ -917,8 +991,20 @@ int alpsg_create(const alpsg_plasma_desc *d, alpsg_handle *out) {
   // analytic-continuation tables
   std::vector<size_t> fit_off(nspec + 1, 0);
   auto bm = [&](int s) { return d->usebM != nullptr && d->usebM[s] != 0; };
-  for (int s = 0; s < nspec; ++s)
-    fit_off[s + 1] = fit_off[s] + ((d->ACmethod[s] == 1 && !bm(s)) ? (size_t)(nperp + 1) * d->n_fits[s] * FIT_STRIDE : 0);
+  auto relf = [&](int s) { return !bm(s) && d->relativistic != nullptr && d->relativistic[s] != 0; };
+  // index of each species among the relativistic ones (determine_sproc_rel, ALPS_fns_rel.f90:437-455)
+  int rel_index[ALPSG_MAXSPEC];
+  int nspec_rel = 0;
+  for (int s = 0; s < nspec; ++s) rel_index[s] = (d->relativistic && d->relativistic[s]) ? nspec_rel++ : -1;
+  const int ngam = d->ngamma, npb = d->npparbar;
+  // Fortran layout (1:nspec_rel, 0:ngamma, 0:npparbar[, 1:2])
+  auto frel = [&](const double *a, int r, int ig, int ip, int c) {
+    return a[(size_t)r + (size_t)nspec_rel * (ig + (size_t)(ngam + 1) * (ip + (size_t)(npb + 1) * c))];
+  };
+  for (int s = 0; s < nspec; ++s) {
+    const int rows = relf(s) ? ngam + 1 : nperp + 1;
+    fit_off[s + 1] = fit_off[s] + ((d->ACmethod[s] == 1 && !bm(s)) ? (size_t)rows * d->n_fits[s] * FIT_STRIDE : 0);
+  }
   std::vector<double> fitc(std::max<size_t>(fit_off[nspec], 1), 0.0);
   int maxorder = 0;
   for (int s = 0; s < nspec; ++s)
@@ -927,6 +1013,26 @@ int alpsg_create(const alpsg_plasma_desc *d, alpsg_handle *out) {
   const int npf = d->param_fit_nperp;
   for (int s = 0; s < nspec; ++s) {
     if (bm(s)) continue;
+    if (relf(s)) {
+      // Juettner fits in Gamma (ALPS_analyt.f90:238-252) on the igamma rows, in
+      // the Maxwell form c0*exp(-c1*(p-c2)^2): type 4 has c1 = c2 = 0
+      const int r = rel_index[s];
+      for (int ig = 0; ig <= ngam; ++ig) {
+        const double gv = frel(d->gamma_rel, r, ig, 1, 0);
+        for (int f = 0; f < d->n_fits[s]; ++f) {
+          auto prm = [&](int k) {
+            return d->param_fit[(size_t)s + (size_t)nspec * (ig + (size_t)(npf + 1) * (k + 5 * (size_t)f))];
+          };
+          const double pc = d->perp_correction[s + (size_t)nspec * f];
+          const int ft = d->fit_type[s + (size_t)nspec * f];
+          double *c = &fitc[fit_off[s] + ((size_t)ig * d->n_fits[s] + f) * FIT_STRIDE];
+          c[0] = prm(0) * std::exp(-pc * gv);
+          c[1] = ft == 5 ? prm(1) : 0.0;
+          c[2] = ft == 5 ? prm(2) : 0.0;
+        }
+      }
+      continue;
+    }
     if (d->ACmethod[s] == 1) {
       for (int i = 0; i <= nperp; ++i) {
         const double pv = h->pperp_h[(size_t)s * (nperp + 1) + i];
@@ -968,6 +1074,33 @@ int alpsg_create(const alpsg_plasma_desc *d, alpsg_handle *out) {
   HIPCHK(h->mtab.ensure(MTAB_N));
   HIPCHK(hipMemcpyAsync(h->mtab.p, mt.data(), sizeof(int) * MTAB_N, hipMemcpyHostToDevice, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
+  if (nrel > 0) {
+    // Gamma(m) by the Lanczos series of ALPS_fns_rel.f90:1738-1791 and Fact(k)
+    // (:1557-1574), as CBESSJ divides by them
+    std::vector<double> gt(GTAB_N + 21, 0.0);
+    const double cof[6] = {76.18009173, -86.50532033, 24.01409822, -1.231739516, 0.120858003e-2,
+                           -0.536382e-5};
+    for (int mm = 1; mm < GTAB_N; ++mm) {
+      double x = (1.0 * mm) - 1.0, tmp = x + 5.5, ser = 1.0;
+      tmp = (x + 0.5) * std::log(tmp) - tmp;
+      for (int j = 0; j < 6; ++j) {
+        x = x + 1.0;
+        ser = ser + cof[j] / x;
+      }
+      gt[mm] = std::exp(tmp + std::log(2.50662827465 * ser));
+    }
+    for (int k = 0; k <= 20; ++k) {
+      double f = 1.0;
+      for (int i = 2; i <= k; ++i) f = f * (1.0 * i);
+      gt[GTAB_N + k] = f;
+    }
+    HIPCHK(h->gtab.ensure(gt.size()));
+    HIPCHK(hipMemcpy(h->gtab.p, gt.data(), sizeof(double) * gt.size(), hipMemcpyHostToDevice));
+    HIPCHK(h->err_d.ensure(1));
+    HIPCHK(hipMemset(h->err_d.p, 0, sizeof(int)));
+    h->ngamma = d->ngamma;
+    h->npparbar = d->npparbar;
+  }
 
   // problem descriptor (per-k pointers filled by set_k)
   ProbDev &pd = h->prob;
@@ -978,6 +1111,7 @@ int alpsg_create(const alpsg_plasma_desc *d, alpsg_handle *out) {
   pd.NRI = h->NRI;
   pd.vA = d->vA;
   pd.Tlim = d->Tlim;
+  pd.gtab = h->gtab.p;
   for (int s = 0; s < nspec; ++s) {
     SpecDev &S = pd.sp[s];
     const double *pe = &h->pperp_h[(size_t)s * (nperp + 1)];
@@ -1017,6 +1151,102 @@ int alpsg_create(const alpsg_plasma_desc *d, alpsg_handle *out) {
       S.acmethod = 1;
       S.nfits = 1;
       S.fit_type[0] = 1;
+      S.fit_uniform = 1;
+      S.nmax = -1;
+      continue;
+    }
+    if (relf(s)) {
+      // (Gamma, pparbar) tables, subluminal row limits, Gamma of the (pperp, ppar)
+      // points and the omega-independent int_ee_rel (ALPS_fns_rel.f90:1097-1215)
+      const int r = rel_index[s];
+      const size_t np = (size_t)(ngam + 1) * (npb + 1);
+      std::vector<double> G(np), PBv(np), F(np);
+      std::vector<double2> DF(np);
+      for (int ig = 0; ig <= ngam; ++ig)
+        for (int ip = 0; ip <= npb; ++ip) {
+          const size_t o = (size_t)ig * (npb + 1) + ip;
+          G[o] = frel(d->gamma_rel, r, ig, ip, 0);
+          PBv[o] = frel(d->pparbar_rel, r, ig, ip, 0);
+          F[o] = frel(d->f0_rel, r, ig, ip, 0);
+          DF[o] = make_double2(frel(d->df0_rel, r, ig, ip, 0), frel(d->df0_rel, r, ig, ip, 1));
+        }
+      for (int ip = 0; ip < npb; ++ip)
+        if (!(PBv[(size_t)2 * (npb + 1) + ip + 1] > PBv[(size_t)2 * (npb + 1) + ip]))
+          return bail(fail(ALPSG_E_ARG, "pparbar_rel rows must increase"));
+      std::vector<int2> lim(ngam + 1);
+      for (int ig = 0; ig <= ngam; ++ig) {
+        int lo = 1, up = npb - 1;
+        bool fl = false, fu = false;
+        const double *f = &F[(size_t)ig * (npb + 1)];
+        for (int ip = 1; ip <= npb - 1; ++ip) {
+          if (!fl && f[ip - 1] <= -1.0 && f[ip] > -1.0) {
+            lo = ip;
+            fl = true;
+          }
+          if (!fu && f[ip] > -1.0 && f[ip + 1] <= -1.0) {
+            up = ip;
+            fu = true;
+          }
+        }
+        lim[ig] = make_int2(lo, up);
+      }
+      const double vA = d->vA, m = d->ms[s], q = d->qs[s];
+      const double dg = G[(size_t)2 * (npb + 1) + 2] - G[(size_t)1 * (npb + 1) + 2];
+      const double dpb = PBv[(size_t)2 * (npb + 1) + 2] - PBv[(size_t)2 * (npb + 1) + 1];
+      double ee = 0.0;
+      {
+        auto term = [&](int ig, int ip) {
+          const size_t o = (size_t)ig * (npb + 1) + ip;
+          return PBv[o] * DF[o].y;
+        };
+        int ig = ngam - 1;
+        ee = ee + term(ig, lim[ig].x);
+        ee = ee + term(ig, lim[ig].y);
+        for (int ip = lim[ig].x + 1; ip <= lim[ig].y - 1; ++ip) ee = ee + 2.0 * term(ig, ip);
+        for (ig = 1; ig <= ngam - 2; ++ig) {
+          for (int ip = lim[ig].x + 1; ip <= lim[ig].y - 1; ++ip) ee = ee + 4.0 * term(ig, ip);
+          ee = ee + 2.0 * term(ig, lim[ig].x);
+          ee = ee + 2.0 * term(ig, lim[ig].y);
+        }
+        const double PI = 3.141592653589793238462643383279502884;
+        ee = ee * 2.0 * PI * q / m;
+        ee = ee * dg * dpb * 0.25 * ((m / vA) * (m / vA) * (m / vA));
+      }
+      std::vector<double> gpp((size_t)(nperp + 1) * (npar + 1));
+      for (int i = 0; i <= nperp; ++i)
+        for (int j = 0; j <= npar; ++j) {
+          const double a = pe[i], b = pa[j];
+          gpp[(size_t)i * (npar + 1) + j] = std::sqrt((a * a + b * b) * (vA * vA) / (m * m) + 1.0);
+        }
+      HIPCHK(h->grel[s].ensure(np));
+      HIPCHK(h->pbrel[s].ensure(np));
+      HIPCHK(h->f0rel[s].ensure(np));
+      HIPCHK(h->dfrel[s].ensure(np));
+      HIPCHK(h->rowlim[s].ensure(ngam + 1));
+      HIPCHK(h->gampp[s].ensure(gpp.size()));
+      HIPCHK(hipMemcpy(h->grel[s].p, G.data(), sizeof(double) * np, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(h->pbrel[s].p, PBv.data(), sizeof(double) * np, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(h->f0rel[s].p, F.data(), sizeof(double) * np, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(h->dfrel[s].p, DF.data(), sizeof(double2) * np, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(h->rowlim[s].p, lim.data(), sizeof(int2) * (ngam + 1), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(h->gampp[s].p, gpp.data(), sizeof(double) * gpp.size(), hipMemcpyHostToDevice));
+      h->relsp[s] = 1;
+      h->any_rel = true;
+      S.rel = 1;
+      S.ng = ngam;
+      S.npb = npb;
+      S.grel = h->grel[s].p;
+      S.pbrel = h->pbrel[s].p;
+      S.f0rel = h->f0rel[s].p;
+      S.dfrel = h->dfrel[s].p;
+      S.rowlim = h->rowlim[s].p;
+      S.gam_pp = h->gampp[s].p;
+      S.df1 = h->df1.p + (size_t)s * (nperp + 1) * (npar + 1);
+      S.df2 = h->df2.p + (size_t)s * (nperp + 1) * (npar + 1);
+      S.rel_dg = dg;
+      S.rel_dpb = dpb;
+      S.int_ee = ee;
+      for (int f = 0; f < S.nfits; ++f) S.fit_type[f] = 1;  // the Maxwell form of the table
       S.fit_uniform = 1;
       S.nmax = -1;
       continue;
@@ -1112,6 +1342,7 @@ int alpsg_set_k(alpsg_handle h, double kperp, double kpar, const int *nmax_in, i
       nmax[s] = n;
     }
   }
+  int nmaxR = 0;
   for (int s = 0; s < nspec; ++s) {
     SpecDev &S = h->prob.sp[s];
     const int nm = nmax[s];
@@ -1145,9 +1376,26 @@ int alpsg_set_k(alpsg_handle h, double kperp, double kpar, const int *nmax_in, i
     S.z = h->kperp_norm ? kperp / h->qs[s] : 1.0 / h->qs[s];
     S.atab = h->atab[s].p;
     S.nodes = h->nodes[s].p;
+    if (h->relsp[s]) {
+      // the harmonics run in k_rel_nonres / k_rel_res, not in the sweep
+      const double m = h->ms[s], q = h->qs[s], vA = h->vA;
+      S.nmax_rel = nm;
+      S.nmax = -1;
+      S.rel_pref = ((2.0 * 3.141592653589793238462643383279502884) * ((m / vA) * (m / vA) * (m / vA))) *
+                   (q * vA / (kpar * m));
+      S.rel_zfac = kperp * m / (vA * q);
+      S.rel_zbar = h->kperp_norm ? kperp * m / (vA * q) : m / (vA * q);
+      const int npts = (h->ngamma + 1) * (h->npparbar + 1);
+      HIPCHK(h->relbes[s].ensure((size_t)(nm + 2) * npts));
+      HIPCHK(launch_rel_bessel(h->st, npts, nm, S.rel_zfac, S.grel, S.pbrel, h->mtab.p,
+                               h->relbes[s].p));
+      S.relbes = h->relbes[s].p;
+      nmaxR = std::max(nmaxR, nm);
+    }
   }
   h->prob.kperp = kperp;
   h->prob.kpar = kpar;
+  h->prob.nmaxR = nmaxR;
   HIPCHK(hipMemcpyAsync(h->prob_d.p, &h->prob, sizeof(ProbDev), hipMemcpyHostToDevice, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
   h->kperp = kperp;

@@ -11,6 +11,7 @@
 #define NODE_STRIDE 8   // doubles per (n, ipar) node record: G1 H1 G2 H2 G3 H3 ppar ppar^2
 #define ATAB_STRIDE 4   // doubles per (n, iperp) entry: J^2, pperp^2 J'^2, pperp J J', J
 #define FIT_STRIDE 8    // doubles per (iperp, fit) entry of the analytic-continuation table
+#define GTAB_N 4200     // Gamma(m) table entries (m < GTAB_N), followed by Fact(0..20)
 
 struct SpecDev {
   int nperp, npar, nmax;
@@ -35,11 +36,25 @@ struct SpecDev {
   int usebm, bm_cold, bm_nrun;
   double bm_Omega, bm_ell, bm_vth, bm_vd, bm_alpha, bm_z, bm_zp;
   const double2 *bmi;   // [2*bm_nrun+1]: (I_n(z) e^-z, dI_n/dz e^-z), n = -nrun..nrun
+  // relativistic species (ALPS_fns_rel.f90): the (Gamma, pparbar) grid as
+  // derivative_f0_rel leaves it, point index ig*(npb+1)+ip
+  int rel, ng, npb, nmax_rel;
+  const double *grel, *pbrel, *f0rel;
+  const double2 *dfrel;   // (df/dGamma, df/dpparbar)
+  const int2 *rowlim;     // [ng+1]: first and last index of each row inside the subluminal cone
+  const double *relbes;   // per k: J_n(z) at every point, n = 0..nmax_rel+1: [n][point]
+  const double *gam_pp;   // Gamma at the (pperp, ppar) points: [iperp*(npar+1)+ipar]
+  const double *df1, *df2;  // df0 on the (pperp, ppar) grid: [iperp*(npar+1)+ipar]
+  double rel_pref;        // 2 pi (m/vA)^3 (q vA/(kpar m))
+  double rel_zfac, rel_zbar;  // z = zfac * pperpbar; zbar of int_T_rel
+  double rel_dg, rel_dpb;
 };
 
 struct ProbDev {
   int nspec, kperp_norm, P, NRI;
   double kperp, kpar, vA, Tlim;
+  const double *gtab;  // Gamma(m) of ALPS_fns_rel.f90:1738 for m = 0..GTAB_N-1, then Fact(k), k = 0..20
+  int nmaxR;           // largest nmax over the relativistic species
   SpecDev sp[ALPSG_DMAXSPEC];
 };
 

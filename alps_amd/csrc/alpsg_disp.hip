@@ -19,6 +19,7 @@
 // operation order, so the device takes the same branches as the CPU
 // restatement.
 #include "alpsg_device.h"
+#include "alpsg_fit.h"
 
 namespace {
 
@@ -147,54 +148,6 @@ __device__ __forceinline__ void interp6_acc(const SpecDev &S, const double *__re
 }
 
 __device__ __forceinline__ bool czero(cd a) { return a.re == 0.0 && a.im == 0.0; }
-
-// eval_fit (ALPS_analyt.f90:32-363) at (iperp, complex p).
-__device__ __noinline__ cd eval_fit(const SpecDev &S, int iperp, cd p) {
-  if (S.acmethod == 2) {
-    // fit_function_poly, Chebyshev kind 1 (ALPS_analyt.f90:307-358)
-    cd t = mk((p.re - S.poly_norm1) / S.poly_norm2, p.im / S.poly_norm2);
-    if (cabsd(t) > 1.0) return mk(0.0, 0.0);
-    const double *c = S.polyc + (size_t)iperp * (S.poly_order + 1);
-    cd bm2 = mk(1.0, 0.0), bm1 = t, f = c[0] * bm2;
-    f = f + c[1] * bm1;
-    for (int n = 2; n <= S.poly_order; ++n) {
-      cd b = (2.0 * t) * bm1 - bm2;
-      f = f + c[n] * b;
-      bm2 = bm1;
-      bm1 = b;
-    }
-    if (S.poly_logfit) {
-      const double lm = S.poly_log_max;
-      if (f.re < -lm || f.im < -lm || f.re > lm || f.im > lm) return mk(0.0, 0.0);
-      const double ln10 = 2.302585092994045684017991454684364208;
-      return cexpd(mk(f.re * ln10, f.im * ln10));
-    }
-    return f;
-  }
-  cd f = mk(0.0, 0.0);
-  for (int fi = 0; fi < S.nfits; ++fi) {
-    const double *c = S.fitc + ((size_t)iperp * S.nfits + fi) * FIT_STRIDE;
-    const cd dq = mk(p.re - c[2], p.im);
-    const cd sq = dq * dq;
-    switch (S.fit_type[fi]) {
-      case 1:  // Maxwell: c0 = u1*exp(-pc*pperp^2), c1 = u2, c2 = u3
-        f = f + c[0] * cexpd(mk(-c[1] * sq.re, -c[1] * sq.im));
-        break;
-      case 2: {  // kappa: c3 = u4, c4 = pc*u5*pperp^2
-        const cd kp = mk(1.0 + c[1] * sq.re + c[4], c[1] * sq.im);
-        f = f + c[0] * cpowr(kp, c[3]);
-      } break;
-      case 6: {  // bi-Moyal: c3 = u4*pc*pperp^2
-        const cd arg = mk(c[3] + c[1] * sq.re, c[1] * sq.im);
-        const cd e = cexpd(arg);
-        f = f + c[0] * cexpd(mk(0.5 * (arg.re - e.re), 0.5 * (arg.im - e.im)));
-      } break;
-      default:
-        break;
-    }
-  }
-  return f;
-}
 
 template <int NCH>
 __device__ __forceinline__ double gsum(double v) {
@@ -863,7 +816,8 @@ __global__ __launch_bounds__(256) void k_assemble(
     const ProbDev *__restrict__ probp, int nom, int W, int RB, const double2 *__restrict__ om_in,
     const int *__restrict__ active, const double2 *__restrict__ chiA,
     const double2 *__restrict__ chiB, const double2 *__restrict__ lowA,
-    const double2 *__restrict__ lowB, double2 *__restrict__ det_out, double2 *__restrict__ chi_out,
+    const double2 *__restrict__ lowB, const double2 *__restrict__ chiR, int RR,
+    const double2 *__restrict__ lowR, double2 *__restrict__ det_out, double2 *__restrict__ chi_out,
     double2 *__restrict__ low_out, double2 *__restrict__ wave_out) {
   __shared__ double2 lchi[256];
   const ProbDev &Pd = *probp;
@@ -894,6 +848,11 @@ __global__ __launch_bounds__(256) void k_assemble(
 #pragma unroll
       for (int u = 0; u < 8; ++u) v = v + mk(buf[u].x, buf[u].y);
     }
+    if (Pd.sp[s].rel)  // relativistic species: the partial slots of k_rel_nonres / k_rel_res
+      for (int j = 0; j < RR; ++j) {
+        const double2 a = chiR[((size_t)(j * nspec + s) * 6 + k) * nom + iom];
+        v = v + mk(a.x, a.y);
+      }
     lchi[t] = make_double2(v.re, v.im);
   }
   __syncthreads();
@@ -936,13 +895,21 @@ __global__ __launch_bounds__(256) void k_assemble(
           for (int j = 0; j < 3; ++j) v[i][j] = mk(0.0, 0.0);
         for (int q = 0; q < 6; ++q) {
           if (qq == 1 && (q == 0 || q == 3 || q == 4)) continue;
-          const double2 a = lowA[((size_t)iom * nspec + sp) * 18 + 6 * qq + q];
-          cd tt = mk(a.x, a.y);
-          for (int b = 0; b < 2 * RB; ++b) {
-            const double2 u = lowB[(((size_t)iom * nspec + sp) * (2 * RB) + b) * 18 + 6 * qq + q];
-            tt = tt + mk(u.x, u.y);
+          cd tt;
+          if (S.rel) {
+            // relativistic: one of k_rel_nonres / k_rel_res wrote the term; the
+            // ee term goes into chi only (ALPS_fns.f90:481-486)
+            const double2 a = lowR[((size_t)iom * nspec + sp) * 18 + 6 * qq + q];
+            tt = mk(a.x, a.y);
+          } else {
+            const double2 a = lowA[((size_t)iom * nspec + sp) * 18 + 6 * qq + q];
+            tt = mk(a.x, a.y);
+            for (int b = 0; b < 2 * RB; ++b) {
+              const double2 u = lowB[(((size_t)iom * nspec + sp) * (2 * RB) + b) * 18 + 6 * qq + q];
+              tt = tt + mk(u.x, u.y);
+            }
+            if (qq == 1 && q == 2) tt = tt + mk(ee, 0.0);
           }
-          if (qq == 1 && q == 2) tt = tt + mk(ee, 0.0);
           v[MI[q]][MJ[q]] = cdiv(tt * norm, dn);
         }
         v[1][0] = mk(-v[0][1].re, -v[0][1].im);
@@ -1030,11 +997,12 @@ hipError_t launch_resonant(hipStream_t st, int nch, int RB, const ProbDev *prob,
 hipError_t launch_assemble(hipStream_t st, const ProbDev *prob, int nspec, int nom, int W, int RB,
                            const double2 *om, const int *active, const double2 *chiA,
                            const double2 *chiB, const double2 *lowA, const double2 *lowB,
-                           double2 *det, double2 *chi, double2 *low, double2 *wave) {
+                           const double2 *chiR, int RR, const double2 *lowR, double2 *det,
+                           double2 *chi, double2 *low, double2 *wave) {
   if (nom <= 0) return hipSuccess;
   const int per = 6 * nspec;
   const int opb = 256 / per;
   hipLaunchKernelGGL(k_assemble, dim3((nom + opb - 1) / opb), dim3(256), 0, st, prob, nom, W, RB,
-                     om, active, chiA, chiB, lowA, lowB, det, chi, low, wave);
+                     om, active, chiA, chiB, lowA, lowB, chiR, RR, lowR, det, chi, low, wave);
   return hipGetLastError();
 }

@@ -1,0 +1,650 @@
+// Relativistic species (relativistic = T) of disp(om), for a batch of omegas.
+//
+// The reference treats a relativistic species in two ways per harmonic n,
+// chosen by determine_resonances (ALPS_fns.f90:680-701), which asks whether
+// the resonance p_res = (Gamma m om - n q)/kpar of some point of the
+// (pperp, ppar) grid falls into that point's own ppar cell:
+//
+//  * no such point: integrate() over the (pperp, ppar) grid with the point's
+//    Gamma in resU (ALPS_fns.f90:799-864, 1560-1596).  Gamma couples pperp and
+//    ppar, so the pperp moments of the non-relativistic sweep do not apply;
+//    k_rel_nonres sums the 2-D trapezoid directly, one workgroup per
+//    (omega, species, harmonic block), after deciding the flag of each n with
+//    a workgroup-wide OR over the grid.
+//  * otherwise: integrate_res_rel and landau_integrate_rel along the
+//    resonance curve on the (Gamma, pparbar) grid (ALPS_fns_rel.f90:460-1092):
+//    k_rel_res, one workgroup per (omega, species, n), lanes over the Gamma
+//    rows.  The row integral is the trapezoid of resU_rel * int_T_rel around
+//    the principal-value window, plus principal_integral_rel; int_T_rel reads
+//    J_n from a per-k table on the same grid (k_rel_bessel) and the Landau
+//    term evaluates int_T_res_rel with the 21-term series CBESSJ.
+//
+// Both kernels write their partial chi into slots of chiR (non-resonant:
+// slot = harmonic block; resonant: slot = NB + n + nmaxR) and the n = -1, 0, 1
+// terms into lowR, read by k_assemble for relativistic species only.
+// Decision-critical arithmetic (resonance flags, window limits, the
+// principal-value ladder positions) is evaluated without FMA contraction in
+// the reference's operation order.
+#include "alpsg_device.h"
+#include "alpsg_fit.h"
+
+namespace {
+
+constexpr double PI_R = 3.141592653589793238462643383279502884;
+
+__device__ __forceinline__ cd cdivr(cd a, double d) { return cd{a.re / d, a.im / d}; }
+__device__ __forceinline__ cd cdivc(cd a, cd b) {
+  if (fabs(b.re) >= fabs(b.im)) {
+    const double r = b.im / b.re, d = b.re + b.im * r;
+    return cd{(a.re + a.im * r) / d, (a.im - a.re * r) / d};
+  }
+  const double r = b.re / b.im, d = b.re * r + b.im;
+  return cd{(a.re * r + a.im) / d, (a.im * r - a.re) / d};
+}
+__device__ __forceinline__ cd csqrtd(cd z) {
+  // principal square root
+  const double r = hypot(z.re, z.im);
+  if (r == 0.0) return mk(0.0, 0.0);
+  if (z.re >= 0.0) {
+    const double t = sqrt(0.5 * (r + z.re));
+    return mk(t, z.im / (2.0 * t));
+  }
+  const double t = sqrt(0.5 * (r - z.re));
+  return mk(fabs(z.im) / (2.0 * t), copysign(t, z.im));
+}
+__device__ __forceinline__ double m1p(int n) { return (n % 2 == 0) ? 1.0 : -1.0; }
+
+// mode order k: 0 xx, 1 yy, 2 zz, 3 xy, 4 xz, 5 yz
+__device__ constexpr int ACC_OF_MODE_R[6] = {0, 3, 2, 4, 1, 5};
+
+// T-tensor constant C_k(n) of int_T (ALPS_fns.f90:1659-1703) for the
+// (pperp, ppar) grid; z = kperp/q or 1/q.
+__device__ __forceinline__ cd tconst_r(const ProbDev &Pd, const SpecDev &S, int n, int k) {
+  const double kp = Pd.kperp, z = S.z;
+  const bool kn = Pd.kperp_norm != 0;
+  switch (k) {
+    case 0: return mk(1.0 * (n * n) / (z * z), 0.0);
+    case 1: return mk(kn ? 1.0 : kp * kp, 0.0);
+    case 2: return mk(kn ? 1.0 : kp * kp, 0.0);
+    case 3: return mk(0.0, (kn ? 1.0 : kp) * (1.0 * n) / z);
+    case 4: return mk((kn ? 1.0 : kp) * (1.0 * n) / z, 0.0);
+    default: return mk(0.0, kn ? -1.0 : -(kp * kp));
+  }
+}
+
+// workgroup sum of a complex value (blockDim.x = 256 or 64), result to all
+template <int NT>
+__device__ __forceinline__ cd block_sum(cd v, double *lds) {
+  for (int o = 32; o > 0; o >>= 1) {
+    v.re += __shfl_xor(v.re, o, 64);
+    v.im += __shfl_xor(v.im, o, 64);
+  }
+  if constexpr (NT == 64) return v;
+  const int w = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+    lds[2 * w] = v.re;
+    lds[2 * w + 1] = v.im;
+  }
+  __syncthreads();
+  cd t = mk(0.0, 0.0);
+  for (int i = 0; i < NT / 64; ++i) t = t + mk(lds[2 * i], lds[2 * i + 1]);
+  return t;
+}
+
+// ---------------------------------------------------------------------------
+// k_rel_nonres: flags of every harmonic of the block and the integrate()
+// terms of the non-resonant ones.  grid (omega, species, NB), 256 threads.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_rel_nonres(const ProbDev *__restrict__ probp, int nom,
+                                                    const double2 *__restrict__ om_in,
+                                                    const int *__restrict__ active,
+                                                    double2 *__restrict__ chiR,
+                                                    double2 *__restrict__ lowR,
+                                                    int *__restrict__ flags) {
+  __shared__ double lds[16];
+  const ProbDev &Pd = *probp;
+  const int iom = blockIdx.x, s = blockIdx.y, NB = gridDim.z, nb = blockIdx.z;
+  const SpecDev &S = Pd.sp[s];
+  if (!S.rel) return;
+  if (!(active == nullptr || active[iom] != 0)) return;
+  const int nspec = Pd.nspec, nperp = S.nperp, npar = S.npar, nmax = S.nmax_rel;
+  const int nstride = 2 * Pd.nmaxR + 1;
+  const double2 ov = om_in[iom];
+  const cd om = mk(ov.x, ov.y);
+  const double m = S.m, q = S.q, kpar = Pd.kpar;
+  const double scale = 2.0 * PI_R * S.dpperp * S.dppar_abs * 0.25;
+  cd res[6];
+  for (int k = 0; k < 6; ++k) res[k] = mk(0.0, 0.0);
+  for (int n = -nmax + nb; n <= nmax; n += NB) {
+    int hit = 0;
+    {
+#pragma clang fp contract(off)
+      const double nq = 1.0 * n * q;
+      for (int pt = threadIdx.x; pt < (nperp + 1) * npar; pt += 256) {
+        const int ip = pt / npar, ia = pt % npar;
+        const double g = S.gam_pp[ip * (npar + 1) + ia];
+        const double pr = (g * m * om.re - nq) / kpar;
+        if (S.ppar[ia] <= pr && S.ppar[ia + 1] > pr) hit = 1;
+      }
+    }
+    const int found = __syncthreads_or(hit);
+    if (threadIdx.x == 0) flags[((size_t)iom * nspec + s) * nstride + n + Pd.nmaxR] = found;
+    if (found) continue;
+    const int an = n < 0 ? -n : n;
+    const double *at = S.atab + (size_t)an * (nperp + 1) * ATAB_STRIDE;
+    cd acc[6];
+    for (int k = 0; k < 6; ++k) acc[k] = mk(0.0, 0.0);
+    for (int pt = threadIdx.x; pt < (nperp - 1) * (npar - 1); pt += 256) {
+      const int ip = 1 + pt / (npar - 1), ia = 1 + pt % (npar - 1);
+      const double w = (ip == nperp - 1 ? 1.0 : 2.0) * ((ia == 1 || ia == npar - 1) ? 1.0 : 2.0);
+      const size_t o = (size_t)ip * (npar + 1) + ia;
+      const double g = S.gam_pp[o], d1 = S.df1[o], d2 = S.df2[o];
+      const double p1 = S.pperp[ip], p2 = S.ppar[ia];
+      // resU (ALPS_fns.f90:1587-1592) with the point's Gamma
+      const double A = (kpar / (g * m)) * (p1 * d2 - p2 * d1);
+      const cd num = mk(q * (om.re * d1 + A), q * (om.im * d1));
+      const cd den = mk(((g * m) * om.re - kpar * p2) - (1.0 * n) * q, (g * m) * om.im);
+      const cd r = w * cdivc(num, den);
+      const double a1 = at[ip * ATAB_STRIDE + 0], a2 = at[ip * ATAB_STRIDE + 1],
+                   a3 = at[ip * ATAB_STRIDE + 2];
+      const cd u1 = a1 * r, u3 = a3 * r;
+      acc[0] = acc[0] + u1;
+      acc[1] = acc[1] + p2 * u1;
+      acc[2] = acc[2] + (p2 * p2) * u1;
+      acc[3] = acc[3] + a2 * r;
+      acc[4] = acc[4] + u3;
+      acc[5] = acc[5] + p2 * u3;
+    }
+    cd v[6];
+    for (int k = 0; k < 6; ++k) {
+      const cd t = block_sum<256>(acc[ACC_OF_MODE_R[k]], lds);
+      v[k] = (scale * t) * tconst_r(Pd, S, n, k);
+    }
+    if (n == 0) v[0] = v[3] = v[4] = mk(0.0, 0.0);
+    for (int k = 0; k < 6; ++k) res[k] = res[k] + v[k];
+    if (threadIdx.x == 0 && lowR != nullptr && an <= 1) {
+      double2 *lo = lowR + ((size_t)iom * nspec + s) * 18 + 6 * (n + 1);
+      for (int k = 0; k < 6; ++k) lo[k] = make_double2(v[k].re, v[k].im);
+    }
+  }
+  if (threadIdx.x == 0) {
+    double2 *o = chiR + ((size_t)(nb * nspec + s) * 6) * nom + iom;
+    for (int k = 0; k < 6; ++k) o[(size_t)k * nom] = make_double2(res[k].re, res[k].im);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// the (Gamma, pparbar) grid: helpers of integrate_res_rel
+// ---------------------------------------------------------------------------
+struct RelCtx {
+  const SpecDev *S;
+  cd om;
+  int n, npb;
+  double vA, kpar, kperp;
+  bool kn;
+};
+
+// bessel and besselP of int_T_rel (ALPS_fns_rel.f90:1324-1338) from the J table
+__device__ __forceinline__ void rel_bessel(const RelCtx &C, int pt, double &b, double &bP) {
+  const SpecDev &S = *C.S;
+  const int n = C.n;
+  const size_t np = (size_t)(S.ng + 1) * (S.npb + 1);
+  auto J = [&](int k) { return S.relbes[(size_t)k * np + pt]; };
+  b = n < 0 ? m1p(n) * J(-n) : J(n);
+  if (n >= 1)
+    bP = 0.5 * (J(n - 1) - J(n + 1));
+  else if (n < -1)
+    bP = 0.5 * ((m1p(n - 1) * J(-(n - 1))) - (m1p(n + 1) * J(-(n + 1))));
+  else if (n == 0)
+    bP = -J(1);
+  else
+    bP = 0.5 * (J(2) - J(0));
+}
+
+// int_T_rel for the six modes at grid point pt (ALPS_fns_rel.f90:1340-1366)
+__device__ __forceinline__ void int_T_rel6(const RelCtx &C, int pt, cd T[6]) {
+  const SpecDev &S = *C.S;
+  const double g = S.grel[pt], pb = S.pbrel[pt];
+  double pperpbar;
+  {
+#pragma clang fp contract(off)
+    pperpbar = sqrt(g * g - 1.0 - pb * pb);
+  }
+  double b, bP;
+  rel_bessel(C, pt, b, bP);
+  const double zbar = S.rel_zbar, kp = C.kperp, nn = 1.0 * C.n;
+  if (C.kn) {
+    T[0] = mk(1.0 * (C.n * C.n) * b * b / (zbar * zbar), 0.0);
+    T[1] = mk(bP * bP * pperpbar * pperpbar, 0.0);
+    T[2] = mk(b * b * (pb * pb), 0.0);
+    T[3] = mk(0.0, nn * b * bP * pperpbar / zbar);
+    T[4] = mk(nn * b * b * pb / zbar, 0.0);
+    T[5] = mk(0.0, -(b * bP * pb * pperpbar));
+  } else {
+    T[0] = mk(1.0 * (C.n * C.n) * b * b / (zbar * zbar), 0.0);
+    T[1] = mk(kp * kp * bP * bP * pperpbar * pperpbar, 0.0);
+    T[2] = mk(kp * kp * b * b * (pb * pb), 0.0);
+    T[3] = mk(0.0, nn * kp * b * bP * pperpbar / zbar);
+    T[4] = mk(nn * kp * b * b * pb / zbar, 0.0);
+    T[5] = mk(0.0, -(kp * kp * b * bP * pb * pperpbar));
+  }
+}
+
+// the numerator of resU_rel and funct_g_rel: pref * (om df/dGamma + (kpar/vA) df/dpparbar)
+__device__ __forceinline__ cd rel_num(const RelCtx &C, int pt) {
+  const double2 d = C.S->dfrel[pt];
+  const cd t = mk(C.om.re * d.x + (C.kpar / C.vA) * d.y, C.om.im * d.x);
+  return C.S->rel_pref * t;
+}
+
+// funct_g_rel (ALPS_fns_rel.f90:918-999) for the six modes at real pparbar x of row ig
+__device__ void funct_g_rel6(const RelCtx &C, int ig, double x, cd g[6]) {
+  const SpecDev &S = *C.S;
+  const int npb = C.npb, row = ig * (npb + 1);
+  const double *pb = S.pbrel + row;
+  int close = -2;
+  {
+#pragma clang fp contract(off)
+    // the last ip in [0, npb-1] with pb[ip] <= x < pb[ip+1] (increasing row)
+    int k = (int)floor((x - pb[0]) / S.rel_dpb);
+    if (k >= -1 && k <= npb) {
+      k = max(0, min(k, npb - 1));
+      while (k > 0 && pb[k] > x) --k;
+      while (k < npb - 1 && pb[k + 1] <= x) ++k;
+      if (pb[k + 1] > x && pb[k] <= x) close = k;
+    }
+  }
+  const double *f0 = S.f0rel + row;
+  if (close + 1 >= 0 && close + 1 <= npb && f0[close + 1] <= -1.0) close = close - 1;
+  if (close - 1 >= 0 && close - 1 <= npb && f0[close - 1] <= -1.0) close = close + 1;
+  if (x == pb[npb]) close = npb - 2;
+  if (close >= npb - 1) close = npb - 2;
+  if (close <= 1) close = 2;
+  cd Tp[6], Tm[6], Tc[6];
+  int_T_rel6(C, row + close + 1, Tp);
+  int_T_rel6(C, row + close, Tc);
+  int_T_rel6(C, row + close - 1, Tm);
+  const cd np_ = rel_num(C, row + close + 1), nc = rel_num(C, row + close),
+           nm = rel_num(C, row + close - 1);
+  const double dx = x - pb[close];
+  for (int k = 0; k < 6; ++k) {
+    const cd ip = -1.0 * (np_ * Tp[k]), ic = -1.0 * (nc * Tc[k]), im = -1.0 * (nm * Tm[k]);
+    g[k] = ic + (0.5 * cdivr(ip - im, S.rel_dpb)) * dx;
+  }
+}
+
+// the resU_rel * int_T_rel terms of point ip of row ig, weight w, into acc
+__device__ __forceinline__ void rel_point(const RelCtx &C, int ig, int ip, double w, cd acc[6]) {
+  const SpecDev &S = *C.S;
+  const int pt = ig * (C.npb + 1) + ip;
+  const double G = S.grel[pt], PB = S.pbrel[pt];
+  const cd gom = mk(G * C.om.re * C.vA / C.kpar, G * C.om.im * C.vA / C.kpar);
+  const double nterm = (1.0 * C.n) * S.q * C.vA / (C.kpar * S.m);
+  const cd den = mk((PB - gom.re) + nterm, -gom.im);
+  const cd U = -1.0 * cdivc(rel_num(C, pt), den);
+  cd T[6];
+  int_T_rel6(C, pt, T);
+  for (int k = 0; k < 6; ++k) acc[k] = acc[k] + w * (U * T[k]);
+}
+
+// principal_integral_rel (ALPS_fns_rel.f90:724-913), six modes, added to acc
+__device__ void principal_rel6(const RelCtx &C, int ig, int ipres, int upperlimit, double Tlim,
+                               int pos, int nri, cd acc[6]) {
+  const SpecDev &S = *C.S;
+  const int row = ig * (C.npb + 1);
+  double denomR, denomI, capDelta, smdelta;
+  {
+#pragma clang fp contract(off)
+    const double G = S.grel[row + ipres];
+    const double gr = G * C.om.re * C.vA / C.kpar, gi = G * C.om.im * C.vA / C.kpar;
+    denomR = gr - (1.0 * C.n) * (S.q / S.m) * C.vA / C.kpar;
+    denomI = gi;
+    capDelta = denomR - S.pbrel[1 * (C.npb + 1) + ipres - pos];
+    smdelta = capDelta / (1.0 * nri);
+  }
+  cd res[6];
+  for (int k = 0; k < 6; ++k) res[k] = mk(0.0, 0.0);
+  cd g[6], g2[6];
+  auto pv_pair = [&](double x, double wgt) {
+    funct_g_rel6(C, ig, x, g);
+    funct_g_rel6(C, ig, 2.0 * denomR - x, g2);
+    const cd dm = mk(x - denomR, -denomI), dp = mk(x - denomR, denomI);
+    for (int k = 0; k < 6; ++k)
+      res[k] = (res[k] + cdivc(wgt * g[k], dm)) - cdivc(wgt * g2[k], dp);
+  };
+  if (fabs(denomI) > Tlim) {
+    pv_pair(denomR, 1.0);
+    pv_pair(denomR + capDelta, 1.0);
+    for (int ip = 1; ip <= nri - 1; ++ip) {
+      double x;
+      {
+#pragma clang fp contract(off)
+        x = denomR + smdelta * ip;
+      }
+      pv_pair(x, 2.0);
+    }
+  } else {
+    funct_g_rel6(C, ig, denomR + S.rel_dpb, g);
+    funct_g_rel6(C, ig, denomR - S.rel_dpb, g2);
+    cd gp[6];
+    for (int k = 0; k < 6; ++k) gp[k] = cdivr(g[k] - g2[k], 2.0 * S.rel_dpb);
+    auto lorentz = [&](double x) {
+      const double d = x - denomR;
+      return (d * d) / (d * d + denomI * denomI);
+    };
+    {
+      const double l = lorentz(denomR + capDelta);
+      for (int k = 0; k < 6; ++k) res[k] = res[k] + (2.0 * gp[k]) * l;
+    }
+    for (int ip = 1; ip <= nri - 1; ++ip) {
+      double x;
+      {
+#pragma clang fp contract(off)
+        x = denomR + smdelta * ip;
+      }
+      const double l = lorentz(x);
+      for (int k = 0; k < 6; ++k) res[k] = res[k] + ((2.0 * 2.0) * gp[k]) * l;
+    }
+    if (denomI != 0.0) {
+      funct_g_rel6(C, ig, denomR, g);
+      const double sg = denomI > 0.0 ? 1.0 : -1.0;
+      for (int k = 0; k < 6; ++k)
+        res[k] = res[k] + cdivr(timesI((sg * 2.0 * PI_R) * g[k]), smdelta);
+    }
+  }
+  int ntiny;
+  double base;
+  {
+#pragma clang fp contract(off)
+    base = S.pbrel[row + upperlimit] - denomR - capDelta;
+    ntiny = (int)(base / smdelta);
+  }
+  if (ntiny > 0) {
+    double correction;
+    {
+#pragma clang fp contract(off)
+      correction = (base / (1.0 * ntiny)) / smdelta;
+    }
+    auto tiny = [&](double x, double wgt) {
+      funct_g_rel6(C, ig, x, g);
+      const cd dm = mk(x - denomR, -denomI);
+      for (int k = 0; k < 6; ++k) res[k] = res[k] + (wgt * correction) * cdivc(g[k], dm);
+    };
+    double x0, x1;
+    {
+#pragma clang fp contract(off)
+      x0 = denomR + capDelta;
+      x1 = denomR + capDelta + correction * smdelta * ntiny;
+    }
+    tiny(x0, 1.0);
+    tiny(x1, 1.0);
+    for (int ip = 1; ip <= ntiny - 1; ++ip) {
+      double x;
+      {
+#pragma clang fp contract(off)
+        x = denomR + capDelta + correction * smdelta * ip;
+      }
+      tiny(x, 2.0);
+    }
+  }
+  for (int k = 0; k < 6; ++k) acc[k] = acc[k] + res[k] * smdelta;
+}
+
+// integrate_resU_rel (ALPS_fns_rel.f90:516-719) of row ig, six modes; returns
+// false where the reference calls alps_error(8)
+__device__ bool resU_row6(const RelCtx &C, int ig, int pos, int nri, double Tlim, cd out[6]) {
+  const SpecDev &S = *C.S;
+  const int npb = C.npb;
+  const double dpb = S.rel_dpb;
+  const double *pb2 = S.pbrel + 2 * (npb + 1);  // row 2 of pparbar_rel
+  int ipres = 0;
+  bool found = false;
+  double pr;
+  {
+#pragma clang fp contract(off)
+    const double g1 = S.grel[ig * (npb + 1) + 1];
+    const double tr = g1 * C.om.re - (1.0 * C.n) * S.q / S.m;
+    pr = tr * C.vA / C.kpar;
+    if ((pr * pr) <= (g1 * g1 - 1.0)) {
+      // the unique ip in [1, npb-2] with pb2[ip] <= pr < pb2[ip+1]
+      if (pb2[1] <= pr && pr < pb2[npb - 1]) {
+        int k = 1 + (int)((pr - pb2[1]) / dpb);
+        k = max(1, min(k, npb - 2));
+        while (k > 1 && pb2[k] > pr) --k;
+        while (k < npb - 2 && pb2[k + 1] <= pr) ++k;
+        ipres = k;
+        found = true;
+      }
+    }
+    for (int ip = 0; ip <= pos; ++ip) {
+      if (pr >= (pb2[0] - dpb * ip) && pr < (pb2[0] - dpb * (ip - 1))) {
+        ipres = -ip;
+        found = true;
+      }
+      if (pr >= (pb2[npb - 1] + dpb * ip) && pr < (pb2[npb - 1] + dpb * (ip + 1))) {
+        ipres = npb - 1 + ip;
+        found = true;
+      }
+    }
+  }
+  const int2 lim = S.rowlim[ig];
+  const int lower = lim.x, upper = lim.y;
+  int int_start, int_end, lowerlimit, upperlimit;
+  if (found) {
+    int_start = lower;
+    int_end = upper;
+    lowerlimit = ipres - pos;
+    upperlimit = ipres + pos + 1;
+    if (ipres >= 0 && ipres <= npb) {
+#pragma clang fp contract(off)
+      if (fabs(pr - pb2[ipres]) > (0.5 * dpb)) upperlimit = upperlimit + 1;
+    }
+    if (lowerlimit < lower && upperlimit > upper) return false;
+    if (lowerlimit <= lower) {
+      int_start = 1;
+      lowerlimit = 0;
+      upperlimit = lower;
+    } else if (upperlimit >= upper) {
+      lowerlimit = upper;
+      upperlimit = npb;
+      int_end = npb - 1;
+    }
+  } else {
+    int_start = lower;
+    lowerlimit = upper;
+    int_end = npb - 1;
+    upperlimit = npb;
+  }
+  cd acc[6];
+  for (int k = 0; k < 6; ++k) acc[k] = mk(0.0, 0.0);
+  if (int_start <= lowerlimit) rel_point(C, ig, int_start, 1.0, acc);
+  for (int ip = int_start + 1; ip <= lowerlimit - 1; ++ip) rel_point(C, ig, ip, 2.0, acc);
+  if (int_start < lowerlimit) rel_point(C, ig, lowerlimit, 1.0, acc);
+  if (upperlimit <= int_end) rel_point(C, ig, upperlimit, 1.0, acc);
+  for (int ip = upperlimit + 1; ip <= int_end - 1; ++ip) rel_point(C, ig, ip, 2.0, acc);
+  if (upperlimit < int_end) rel_point(C, ig, int_end, 1.0, acc);
+  for (int k = 0; k < 6; ++k) acc[k] = acc[k] * dpb;
+  if (found && lowerlimit >= int_start && upperlimit <= int_end)
+    principal_rel6(C, ig, ipres, upperlimit, Tlim, pos, nri, acc);
+  for (int k = 0; k < 6; ++k) out[k] = acc[k];
+  return true;
+}
+
+// complex ** integer by repeated squaring
+__device__ __forceinline__ cd cpowi_d(cd a, int n) {
+  cd pw = mk(1.0, 0.0), x = a;
+  unsigned u = (unsigned)n;
+  if (n == 0) return pw;
+  for (;;) {
+    if (u & 1u) pw = pw * x;
+    u >>= 1;
+    if (u)
+      x = x * x;
+    else
+      break;
+  }
+  return pw;
+}
+
+// CBESSJ (ALPS_fns_rel.f90:1502-1553): 21 terms of the power series of J_nu(z),
+// with Gamma(nu+k+1) and Fact(k) from the host-built table
+__device__ cd cbessj_d(const double *gtab, cd z, int nu) {
+  cd sum = mk(0.0, 0.0);
+  const cd w = cdivr(-1.0 * (z * z), 4.0);
+  for (int k = 0; k <= 20; ++k) {
+    cd t = cpowi_d(w, k);
+    t = cdivr(t, gtab[GTAB_N + k]);
+    t = cdivr(t, gtab[nu + k + 1]);
+    sum = sum + t;
+  }
+  return cpowi_d(cdivr(z, 2.0), nu) * sum;
+}
+
+// int_T_res_rel (ALPS_fns_rel.f90:1374-1495), six modes, at complex pparbar p of row ig
+__device__ void int_T_res_rel6(const RelCtx &C, const double *gtab, int ig, cd p, cd T[6]) {
+  const SpecDev &S = *C.S;
+  const double g = S.grel[ig * (C.npb + 1) + 1];
+  const cd pp2 = p * p;
+  const cd pperpbar = csqrtd(mk((g * g - 1.0) - pp2.re, -pp2.im));
+  const cd z = S.rel_zfac * pperpbar;
+  const int n = C.n;
+  cd b, bP;
+  if (n < 0)
+    b = m1p(n) * cbessj_d(gtab, z, -n);
+  else
+    b = cbessj_d(gtab, z, n);
+  if (n >= 1) {
+    bP = 0.5 * (cbessj_d(gtab, z, n - 1) - cbessj_d(gtab, z, n + 1));
+  } else if (n < -1) {
+    bP = 0.5 * ((m1p(n - 1) * cbessj_d(gtab, z, -(n - 1))) -
+                (m1p(n + 1) * cbessj_d(gtab, z, -(n + 1))));
+  } else if (n == 0) {
+    bP = -1.0 * cbessj_d(gtab, z, 1);
+  } else {
+    bP = 0.5 * (cbessj_d(gtab, z, 2) - cbessj_d(gtab, z, 0));
+  }
+  const double zbar = S.rel_zbar, kp = C.kperp, nn = 1.0 * n;
+  const double f = C.kn ? 1.0 : kp, f2 = C.kn ? 1.0 : kp * kp;
+  T[0] = cdivr(1.0 * (n * n) * (b * b), zbar * zbar);
+  T[1] = f2 * (((bP * bP) * pperpbar) * pperpbar);
+  T[2] = f2 * ((b * b) * pp2);
+  T[3] = cdivr(timesI((nn * f) * ((b * bP) * pperpbar)), zbar);
+  T[4] = cdivr((nn * f) * ((b * b) * p), zbar);
+  T[5] = -1.0 * timesI(f2 * (((b * bP) * p) * pperpbar));
+}
+
+// ---------------------------------------------------------------------------
+// k_rel_res: integrate_res_rel (+ landau_integrate_rel for Im om <= 0) of one
+// resonant harmonic.  grid (omega, species, 2 nmaxR + 1), 64 threads.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_rel_res(const ProbDev *__restrict__ probp, int nom,
+                                                const double2 *__restrict__ om_in,
+                                                const int *__restrict__ active,
+                                                double2 *__restrict__ chiR,
+                                                double2 *__restrict__ lowR,
+                                                const int *__restrict__ flags, int NB,
+                                                int *__restrict__ err) {
+  const ProbDev &Pd = *probp;
+  const int iom = blockIdx.x, s = blockIdx.y;
+  const SpecDev &S = Pd.sp[s];
+  if (!S.rel) return;
+  if (!(active == nullptr || active[iom] != 0)) return;
+  const int nspec = Pd.nspec;
+  const int n = (int)blockIdx.z - Pd.nmaxR;
+  const int nstride = 2 * Pd.nmaxR + 1;
+  double2 *o = chiR + ((size_t)((NB + blockIdx.z) * nspec + s) * 6) * nom + iom;
+  const bool inr = (n >= -S.nmax_rel && n <= S.nmax_rel);
+  const bool found = inr && flags[((size_t)iom * nspec + s) * nstride + blockIdx.z] != 0;
+  if (!found) {
+    if (threadIdx.x == 0)
+      for (int k = 0; k < 6; ++k) o[(size_t)k * nom] = make_double2(0.0, 0.0);
+    return;
+  }
+  const double2 ov = om_in[iom];
+  RelCtx C;
+  C.S = &S;
+  C.om = mk(ov.x, ov.y);
+  C.n = n;
+  C.npb = S.npb;
+  C.vA = Pd.vA;
+  C.kpar = Pd.kpar;
+  C.kperp = Pd.kperp;
+  C.kn = Pd.kperp_norm != 0;
+  const int ng = S.ng;
+  cd acc[6], accL[6];
+  for (int k = 0; k < 6; ++k) acc[k] = accL[k] = mk(0.0, 0.0);
+  int bad = 0;
+  for (int ig = 1 + threadIdx.x; ig <= ng - 1; ig += 64) {
+    cd U[6];
+    if (!resU_row6(C, ig, Pd.P, Pd.NRI, Pd.Tlim, U)) {
+      bad = 1;
+      continue;
+    }
+    const double wr = ig <= ng - 2 ? 2.0 : 1.0;
+    for (int k = 0; k < 6; ++k) acc[k] = acc[k] + wr * U[k];
+    if (C.om.im <= 0.0) {
+      // landau_integrate_rel row (ALPS_fns_rel.f90:1065-1083)
+      const double g1 = S.grel[ig * (S.npb + 1) + 1];
+      cd pres;
+      bool in;
+      {
+#pragma clang fp contract(off)
+        const double a = g1 * C.om.re * C.vA / C.kpar, b = g1 * C.om.im * C.vA / C.kpar;
+        pres = mk(a - (1.0 * n) * S.q * C.vA / (C.kpar * S.m), b);
+        in = (pres.re * pres.re) <= (g1 * g1 - 1.0);
+      }
+      if (in) {
+        const double h = (ig == ng - 1) ? 0.5 : 1.0;
+        cd dfg;
+        if (ig == 1)
+          dfg = cdivr(eval_fit(S, ig + 1, pres) - eval_fit(S, ig, pres), S.rel_dg);
+        else
+          dfg = cdivr(eval_fit(S, ig + 1, pres) - eval_fit(S, ig - 1, pres), 2.0 * S.rel_dg);
+        const cd dfp = cdivr(eval_fit(S, ig, mk(pres.re + S.rel_dpb, pres.im)) -
+                                 eval_fit(S, ig, mk(pres.re - S.rel_dpb, pres.im)),
+                             2.0 * S.rel_dpb);
+        const cd fac = h * (C.om * dfg + (C.kpar / C.vA) * dfp);
+        cd T[6];
+        int_T_res_rel6(C, Pd.gtab, ig, pres, T);
+        for (int k = 0; k < 6; ++k) accL[k] = accL[k] - fac * T[k];
+      }
+    }
+  }
+  if (bad) *err = 8;
+  cd v[6];
+  for (int k = 0; k < 6; ++k) {
+    const cd r = block_sum<64>(acc[k], nullptr);
+    v[k] = (r * S.rel_dg) * 0.25;
+    if (C.om.im <= 0.0) {
+      const cd L = block_sum<64>(accL[k], nullptr);
+      const cd Lf = (((timesI(L) * S.rel_dg) * PI_R) * 2.0) * PI_R *
+                    ((S.q * C.vA / (C.kpar * S.m)) * ((S.m / C.vA) * (S.m / C.vA) * (S.m / C.vA)));
+      v[k] = v[k] + (C.om.im < 0.0 ? 2.0 : 1.0) * Lf;
+    }
+  }
+  if (n == 0) v[0] = v[3] = v[4] = mk(0.0, 0.0);
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < 6; ++k) o[(size_t)k * nom] = make_double2(v[k].re, v[k].im);
+    if (lowR != nullptr && n >= -1 && n <= 1) {
+      double2 *lo = lowR + ((size_t)iom * nspec + s) * 18 + 6 * (n + 1);
+      for (int k = 0; k < 6; ++k) lo[k] = make_double2(v[k].re, v[k].im);
+    }
+  }
+}
+
+}  // namespace
+
+// partial slots of chiR: NB harmonic blocks of k_rel_nonres, then 2 nmaxR + 1 of k_rel_res
+int rel_slots(int NB, int nmaxR) { return NB + 2 * nmaxR + 1; }
+
+hipError_t launch_rel(hipStream_t st, const ProbDev *prob, int nspec, int nmaxR, int NB, int nom,
+                      const double2 *om, const int *active, double2 *chiR, double2 *lowR,
+                      int *flags, int *err) {
+  if (nom <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_rel_nonres, dim3(nom, nspec, NB), dim3(256), 0, st, prob, nom, om, active,
+                     chiR, lowR, flags);
+  hipLaunchKernelGGL(k_rel_res, dim3(nom, nspec, 2 * nmaxR + 1), dim3(64), 0, st, prob, nom, om,
+                     active, chiR, lowR, flags, NB, err);
+  return hipGetLastError();
+}

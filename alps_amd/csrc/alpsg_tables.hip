@@ -304,6 +304,28 @@ hipError_t launch_bessel_max(hipStream_t st, int nspec, int nperp, int ntry, dou
   return hipGetLastError();
 }
 
+// J_n(z) on the (Gamma, pparbar) grid of a relativistic species, n = 0..nmax+1,
+// z = (kperp m/(vA q)) sqrt(Gamma^2 - 1 - pparbar^2) (int_T_rel,
+// ALPS_fns_rel.f90:1298-1322); NaN outside the subluminal cone, as there.
+__global__ void k_rel_bessel(int npts, int nmax, double zfac, const double *__restrict__ grel,
+                             const double *__restrict__ pbrel, const int *__restrict__ mtab,
+                             double *__restrict__ out) {
+#pragma clang fp contract(off)
+  const int pt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = blockIdx.y;
+  if (pt >= npts || n > nmax + 1) return;
+  const double g = grel[pt], pb = pbrel[pt];
+  const double pperpbar = sqrt(g * g - 1.0 - pb * pb);
+  out[(size_t)n * npts + pt] = dev_bessj(n, zfac * pperpbar, mtab);
+}
+
+hipError_t launch_rel_bessel(hipStream_t st, int npts, int nmax, double zfac, const double *grel,
+                             const double *pbrel, const int *mtab, double *out) {
+  hipLaunchKernelGGL(k_rel_bessel, dim3((npts + 255) / 256, nmax + 2), dim3(256), 0, st, npts,
+                     nmax, zfac, grel, pbrel, mtab, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_species_k(hipStream_t st, int nperp, int npar, int nmax, double kperp,
                             double kpar, double q, double m, const double *pperp,
                             const double *ppar, const double *df1, const double *df2,

@@ -64,6 +64,9 @@ class _Desc(C.Structure):
         ("poly_max_order", C.c_int),
         ("bM_nmax", C.c_void_p), ("bM_Bessel_zero", C.c_void_p), ("bM_beta", C.c_void_p),
         ("bM_alpha", C.c_void_p), ("bM_pdrift", C.c_void_p),
+        ("ngamma", C.c_int), ("npparbar", C.c_int),
+        ("gamma_rel", C.c_void_p), ("pparbar_rel", C.c_void_p), ("f0_rel", C.c_void_p),
+        ("df0_rel", C.c_void_p),
     ]
 
 
@@ -175,7 +178,12 @@ class Engine:
         d.n_resonance_interval = int(p.n_resonance_interval)
         d.Tlim, d.Bessel_zero = p.Tlim, p.Bessel_zero
         d.usebM = keep["usebM"].ctypes.data
-        d.relativistic = None
+        keep["relativistic"] = p.relativistic.astype(np.int32)
+        d.relativistic = keep["relativistic"].ctypes.data
+        if p.nspec_rel:
+            d.ngamma, d.npparbar = int(p.ngamma), int(p.npparbar)
+            d.gamma_rel, d.pparbar_rel = fo["gamma_rel"].ctypes.data, fo["pparbar_rel"].ctypes.data
+            d.f0_rel, d.df0_rel = fo["f0_rel"].ctypes.data, fo["df0_rel"].ctypes.data
         d.bM_nmax = keep["bM_nmax"].ctypes.data
         d.bM_Bessel_zero = keep["bM_Bessel_zero"].ctypes.data
         d.bM_beta, d.bM_alpha = keep["bM_beta"].ctypes.data, keep["bM_alpha"].ctypes.data

@@ -29,6 +29,9 @@ module alpsg
      integer(c_int) :: poly_max_order
      type(c_ptr) :: bM_nmax = c_null_ptr, bM_Bessel_zero = c_null_ptr, bM_beta = c_null_ptr
      type(c_ptr) :: bM_alpha = c_null_ptr, bM_pdrift = c_null_ptr
+     integer(c_int) :: ngamma = 0, npparbar = 0
+     type(c_ptr) :: gamma_rel = c_null_ptr, pparbar_rel = c_null_ptr
+     type(c_ptr) :: f0_rel = c_null_ptr, df0_rel = c_null_ptr
   end type alpsg_plasma_desc
 
   type, bind(C), public :: alpsg_root_params

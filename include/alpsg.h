@@ -19,7 +19,7 @@ extern "C" {
 
 #define ALPSG_OK 0
 #define ALPSG_E_ARG 1         /* invalid argument / shape */
-#define ALPSG_E_UNSUPPORTED 2 /* feature outside the engine (relativistic, ...) */
+#define ALPSG_E_UNSUPPORTED 2 /* feature outside the engine (Juettner fit 3, ...) */
 #define ALPSG_E_DEVICE 3      /* HIP runtime error */
 #define ALPSG_E_STATE 4       /* call order (e.g. disp before set_k) */
 #define ALPSG_E_NOTFOUND 5    /* no root survived a scan (reference alps_error(9)) */
@@ -50,7 +50,7 @@ typedef struct alpsg_plasma_desc {
   double Tlim;
   double Bessel_zero;
   const int *usebM;        /* [nspec] or NULL; nonzero: closed-form (NHDS) species */
-  const int *relativistic; /* [nspec] or NULL; nonzero -> ALPSG_E_UNSUPPORTED */
+  const int *relativistic; /* [nspec] or NULL; nonzero: relativistic species (see below) */
   const int *ACmethod;     /* [nspec]: 1 fits, 2 Chebyshev */
   const int *n_fits;       /* [nspec] */
   int max_fits;
@@ -67,6 +67,15 @@ typedef struct alpsg_plasma_desc {
      bM_beta 0 selects the cold-plasma susceptibility. */
   const int *bM_nmax;
   const double *bM_Bessel_zero, *bM_beta, *bM_alpha, *bM_pdrift;
+  /* relativistic species (relativistic[s] != 0): the (Gamma, pparbar) grid as
+     derivative_f0_rel (ALPS_fns_rel.f90:19-408) leaves it, over the
+     relativistic species only, in species order (ALPS_com.f90:247-250):
+       gamma_rel, pparbar_rel, f0_rel (1:nspec_rel,0:ngamma,0:npparbar)
+       df0_rel                         (1:nspec_rel,0:ngamma,0:npparbar,1:2)
+     Their fits must be fit types 4 or 5 (Juettner in Gamma), with
+     param_fit_nperp >= ngamma. */
+  int ngamma, npparbar;
+  const double *gamma_rel, *pparbar_rel, *f0_rel, *df0_rel;
 } alpsg_plasma_desc;
 
 /* Root-search controls (&system namelist, ALPS_var.f90:147-165). */

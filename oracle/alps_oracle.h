@@ -94,6 +94,12 @@ enum {
   ORC_BR_PV_ANALYTIC,    /* |Im p_res| <= Tlim: Eq. (3.6) analytic form */
   ORC_BR_TINY_REST,      /* tiny-rest integration */
   ORC_BR_LANDAU_ZERO,    /* landau_integrate early return */
+  ORC_BR_REL_RES,        /* relativistic: integrate_res_rel */
+  ORC_BR_REL_PV_TLIM,    /* relativistic: principal value, Tlim ladder */
+  ORC_BR_REL_PV_ANALYTIC,/* relativistic: principal value, Eq. (3.7) form */
+  ORC_BR_REL_TINY,       /* relativistic: tiny rest */
+  ORC_BR_REL_EDGE,       /* relativistic: window at the edge of the subluminal row (no PV) */
+  ORC_BR_REL_LANDAU,     /* relativistic: Landau row inside the cone */
   ORC_NBRANCH
 };
 extern long long orc_branch_counts[ORC_NBRANCH];

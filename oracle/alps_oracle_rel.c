@@ -26,6 +26,12 @@
 
 static const double PI_R = 3.141592653589793238462643383279502884;
 
+#define COUNT(b)            \
+  do {                      \
+    _Pragma("omp atomic")  \
+    orc_branch_counts[b]++; \
+  } while (0)
+
 #define GR(P, s, ig, ip) ((P)->gamma_rel[(s)-1][(size_t)(ig) * ((P)->npparbar + 1) + (ip)])
 #define PB(P, s, ig, ip) ((P)->pparbar_rel[(s)-1][(size_t)(ig) * ((P)->npparbar + 1) + (ip)])
 #define F0R(P, s, ig, ip) ((P)->f0_rel[(s)-1][(size_t)(ig) * ((P)->npparbar + 1) + (ip)])
@@ -246,6 +252,7 @@ static double complex principal_integral_rel(const orc_plasma *P, int s, double 
   double x;
 #define G(v) funct_g_rel(P, s, (v), ig, om, nn, mode)
   if (fabs(denomI) > P->Tlim) {
+    COUNT(ORC_BR_REL_PV_TLIM);
     x = creal(pres);
     res = res + 1.0 * G(x) / (x - denomR - ii * denomI);
     res = res - 1.0 * G(2.0 * denomR - x) / (x - denomR + ii * denomI);
@@ -258,6 +265,7 @@ static double complex principal_integral_rel(const orc_plasma *P, int s, double 
       res = res - 2.0 * G(2.0 * denomR - x) / (x - denomR + ii * denomI);
     }
   } else {
+    COUNT(ORC_BR_REL_PV_ANALYTIC);
     const double complex gprimetr = (G(denomR + dpb) - G(denomR - dpb)) / (2.0 * dpb);
     x = creal(pres) + capDelta;
     res = res + 2.0 * gprimetr * ((x - denomR) * (x - denomR)) /
@@ -274,6 +282,7 @@ static double complex principal_integral_rel(const orc_plasma *P, int s, double 
   }
   const int ntiny = (int)((PB(P, s, ig, upperlimit) - creal(pres) - capDelta) / smdelta);
   if (ntiny > 0) {
+    COUNT(ORC_BR_REL_TINY);
     const double correction =
         ((PB(P, s, ig, upperlimit) - creal(pres) - capDelta) / (1.0 * ntiny)) / smdelta;
     x = creal(pres) + capDelta;
@@ -353,10 +362,12 @@ static double complex integrate_resU_rel(const orc_plasma *P, int s, double comp
       *err = 8;
       return 0.0;
     } else if (lowerlimit <= lower) {
+      COUNT(ORC_BR_REL_EDGE);
       int_start = 1;
       lowerlimit = 0;
       upperlimit = lower;
     } else if (upperlimit >= upper) {
+      COUNT(ORC_BR_REL_EDGE);
       lowerlimit = upper;
       upperlimit = npb;
       int_end = npb - 1;
@@ -389,6 +400,7 @@ double complex orc_integrate_res_rel(const orc_plasma *P, int s, double complex 
   const double dg = GR(P, s, 2, 2) - GR(P, s, 1, 2);
   double complex res = 0.0;
   int err = 0;
+  COUNT(ORC_BR_REL_RES);
   for (int ig = 1; ig <= ng - 2; ++ig) res = res + 2.0 * integrate_resU_rel(P, s, om, nn, mode, ig, &err);
   res = res + integrate_resU_rel(P, s, om, nn, mode, ng - 1, &err);
   if (err) orc_rel_error = err;
@@ -407,6 +419,7 @@ double complex orc_landau_integrate_rel(const orc_plasma *P, int s, double compl
     const double g1 = GR(P, s, ig, 1);
     const double complex pres = g1 * om * vA / kpar - (1.0 * nn) * qs * vA / (kpar * ms);
     if ((creal(pres) * creal(pres)) <= (g1 * g1 - 1.0)) {
+      COUNT(ORC_BR_REL_LANDAU);
       double h = 1.0;
       if (ig == ng - 1) h = 0.5;
       double complex dfg, dfp;

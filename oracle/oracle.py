@@ -131,7 +131,8 @@ def lib():
 
 
 BRANCHES = ["nonres", "res_pos", "res_neg", "res_real", "edge_low", "edge_high", "pv_tlim",
-            "pv_analytic", "tiny_rest", "landau_zero"]
+            "pv_analytic", "tiny_rest", "landau_zero", "rel_res", "rel_pv_tlim", "rel_pv_analytic",
+            "rel_tiny", "rel_edge", "rel_landau"]
 
 
 def bessi(n: int, x: float) -> float:

@@ -70,6 +70,43 @@ def bm_mix(kperp_norm=True, cold=True):
         bM_alpha=[1.5, 1.0, 1.0], bM_pdrift=[0.2, 0.0, 1.0])
 
 
+REL_OMEGAS = [0.5 + 0.05j, 0.5 - 0.05j, 0.3 - 0.002j, 0.8 + 0.0j, 1.0 - 1.655e-6j,
+              0.0627 - 4.662e-8j, 1.5 - 0.3j, 0.12 + 0.3j]
+
+
+def rel_pair(kperp_norm=True, nperp=12, npar=24, ngamma=20, npparbar=200):
+    """Relativistic pair plasma in the style of test_relativistic.in (Juettner
+    e+/e-, vA = 1, fit type 4), on a small table."""
+    from alps_amd.plasma import juttner_plasma
+    return juttner_plasma(ms=[1.0, 1.0], qs=[1.0, -1.0], ns=[1.0, 1.0], taus=[1.0, 1.0], beta=1.0,
+                          vA=1.0, nperp=nperp, npar=npar, maxP=5.0, ngamma=ngamma,
+                          npparbar=npparbar, positions_principal=5, n_resonance_interval=100,
+                          Tlim=0.01, Bessel_zero=1e-45, kperp_norm=kperp_norm)
+
+
+def rel_mixed():
+    """A relativistic Juettner species (vA = 0.3) beside a non-relativistic
+    bi-Maxwellian one on the same (pperp, ppar) grid size."""
+    import dataclasses
+    from alps_amd.plasma import juttner_plasma
+    r = juttner_plasma(ms=[1.0], qs=[-1.0], ns=[1.0], taus=[1.0], beta=1.0, vA=0.3, nperp=12,
+                       npar=24, maxP=5.0, ngamma=20, npparbar=200, positions_principal=3,
+                       n_resonance_interval=100, Tlim=0.01, Bessel_zero=1e-45)
+    b = bimaxwellian_plasma(ms=[1.0], qs=[1.0], ns=[1.0], taus=[1.0], alphs=[1.5], ps=[0.0],
+                            beta=1.0, vA=0.3, nperp=12, npar=24, maxP=5.0)
+    npf = r.param_fit.shape[1]
+    pf = np.zeros((2, npf, 5, 1))
+    pf[0, : b.param_fit.shape[1]] = b.param_fit[0]
+    pf[1] = r.param_fit[0]
+    return dataclasses.replace(
+        r, pp=np.concatenate([b.pp, r.pp]), f0=np.concatenate([b.f0, r.f0]), ns=[1.0, 1.0],
+        qs=[1.0, -1.0], ms=[1.0, 1.0], fit_type=np.array([[1], [4]]),
+        perp_correction=np.array([[b.perp_correction[0, 0]], [r.perp_correction[0, 0]]]),
+        param_fit=pf, relativistic=[0, 1], ACmethod=None, n_fits=None, poly_kind=None,
+        poly_order=None, logfit=None, poly_log_max=None, poly_fit_coeffs=None, usebM=None,
+        bM_nmax=None, bM_Bessel_zero=None, bM_beta=None, bM_alpha=None, bM_pdrift=None)
+
+
 def kpar_fast(nperp=120, npar=240):
     """The reference's tests/test_kpar_fast.in (with test_kpar_fast_dist.in)."""
     rp = RootParams(secant_method=2, numiter=50, D_threshold=1e-15, D_prec=1e-5, D_gap=1e-5)

@@ -90,9 +90,16 @@ def test_create_rejects_bad_input_without_a_device():
     d, keep = _desc_for(p, kperp_norm=0, **bm)
     rc, msg, _ = _create(d)
     assert rc == 2 and "cold" in msg
+    # a relativistic species needs its (Gamma, pparbar) tables and Juettner fits
     d, keep = _desc_for(p, relativistic=np.array([0, 1], dtype=np.int32))
     rc, msg, _ = _create(d)
-    assert rc == 2 and "relativistic" in msg
+    assert rc == 1 and "relativistic tables" in msg
+    g = np.zeros((1, 11, 41))
+    d, keep = _desc_for(p, relativistic=np.array([0, 1], dtype=np.int32), ngamma=10, npparbar=40,
+                        gamma_rel=g, pparbar_rel=g.copy(), f0_rel=g.copy(),
+                        df0_rel=np.zeros((1, 11, 41, 2)))
+    rc, msg, _ = _create(d)
+    assert rc == 2 and "fit_type 4 or 5" in msg
     # a grid that is not a tensor product of pperp and ppar
     pp = p.pp.copy()
     pp[0, 3, 5, 1] += 1e-3

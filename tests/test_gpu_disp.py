@@ -132,3 +132,31 @@ def test_every_launch_shape_gives_the_same_results():
     d, chi, low, _ = E.disp_batch(oms, want_tensors=True)
     assert np.max(np.abs(chi - chi_r) / scale[:, :, None, None]) <= 1e-12
     E.close()
+
+
+@pytest.mark.parametrize("kperp_norm", [True, False])
+def test_relativistic_pair(kperp_norm):
+    """Relativistic species: resonant harmonics on the (Gamma, pparbar) grid
+    (integrate_res_rel, landau_integrate_rel), the others on the (pperp, ppar)
+    grid with Gamma in resU (ALPS_fns.f90:1584-1585)."""
+    from oracle import rel_error
+    p = cases.rel_pair(kperp_norm=kperp_norm)
+    _check(p, 0.3, 0.1, cases.REL_OMEGAS)
+    assert rel_error() == 0
+
+
+def test_relativistic_beside_nonrelativistic():
+    _check(cases.rel_mixed(), 0.3, 0.2, cases.REL_OMEGAS)
+
+
+def test_relativistic_window_error_is_reported():
+    """A principal-value window wider than a Gamma row's subluminal part is
+    alps_error(8) in the reference; the engine returns an error for it."""
+    from alps_amd.engine import AlpsgError
+    p = cases.rel_pair(ngamma=30, npparbar=30)
+    E = _engine(p)
+    E.set_k(1e-3, 0.1)
+    with pytest.raises(AlpsgError) as ei:
+        E.disp_batch([0.0627 - 4.662e-8j])
+    assert "alps_error(8)" in str(ei.value)
+    E.close()

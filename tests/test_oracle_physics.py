@@ -121,3 +121,28 @@ def test_int_ee_closed_form():
         ref = 2 * math.pi * norm * math.sqrt(math.pi * a) * (a - b) / 2
         errs.append(abs(got / ref - 1))
     assert errs[1] < 0.03 and errs[1] < errs[0], errs
+
+
+def test_branch_omegas_reach_every_branch():
+    """cases.BRANCH_OMEGAS (used by the GPU parity tests) exercise every branch
+    of full_integrate / integrate_res (ALPS_fns.f90:750-1238) on the small
+    plasmas; REL_OMEGAS every branch of the relativistic path."""
+    from oracle import branch_counts, reset_branch_counts
+    reset_branch_counts()
+    for p in (cases.small_bimax(), cases.small_bimax(drift=0.7, alph=2.5)):
+        O = Oracle(p)
+        O.set_k(KP, KZ)
+        for om in cases.BRANCH_OMEGAS:
+            O.disp(om)
+    c = branch_counts()
+    for name in ("nonres", "res_pos", "res_neg", "res_real", "edge_low", "edge_high", "pv_tlim",
+                 "pv_analytic", "tiny_rest"):
+        assert c[name] > 0, (name, c)
+    reset_branch_counts()
+    O = Oracle(cases.rel_pair())
+    O.set_k(0.3, 0.1)
+    for om in cases.REL_OMEGAS:
+        O.disp(om)
+    c = branch_counts()
+    for name in ("rel_res", "rel_pv_tlim", "rel_pv_analytic", "rel_tiny", "rel_edge", "rel_landau"):
+        assert c[name] > 0, (name, c)
