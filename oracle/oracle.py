@@ -160,7 +160,12 @@ def cbessj(z, nu: int) -> complex:
 
 
 def rel_error() -> int:
+    """8 once a relativistic row hit the reference's alps_error(8) (sticky)."""
     return C.c_int.in_dll(lib(), "orc_rel_error").value
+
+
+def reset_rel_error() -> None:
+    C.c_int.in_dll(lib(), "orc_rel_error").value = 0
 
 
 def reset_branch_counts():

@@ -139,7 +139,8 @@ def test_relativistic_pair(kperp_norm):
     """Relativistic species: resonant harmonics on the (Gamma, pparbar) grid
     (integrate_res_rel, landau_integrate_rel), the others on the (pperp, ppar)
     grid with Gamma in resU (ALPS_fns.f90:1584-1585)."""
-    from oracle import rel_error
+    from oracle import rel_error, reset_rel_error
+    reset_rel_error()
     p = cases.rel_pair(kperp_norm=kperp_norm)
     _check(p, 0.3, 0.1, cases.REL_OMEGAS)
     assert rel_error() == 0
