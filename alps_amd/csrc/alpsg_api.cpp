@@ -143,6 +143,8 @@ struct alpsg_ctx {
   DevBuf<double2> chiR_d, lowR_d;
   DevBuf<int> flags_d, err_d;
   int rel_NB = 1;
+  std::vector<double> current_int;  // parallel current density per species
+  bool have_current = false;
   // launch shapes chosen by alpsg_autotune (or pinned by alpsg_set_shape),
   // keyed by the batch-size bucket floor(log2(n))
   int tuned_valid[32] = {0};
@@ -837,6 +839,154 @@ int root_batch(alpsg_ctx *h, const alpsg_root_params *rp, int nroots, cplx *om, 
   return ALPSG_OK;
 }
 
+// calc_eigen (ALPS_fns.f90:2605-2899) for one root from the chi0, chi0_low and
+// wave that disp returned at om (c0, w0), real(om) (c1) and real(om*1.000001)
+// (c2, l2); chi arrays in the Fortran layouts of alpsg_disp_batch.
+void calc_eigen_one(const alpsg_ctx *h, cplx omega, const cplx *c0, const cplx *w0, const cplx *c1,
+                    const cplx *c2, const cplx *l2, bool eigen, bool heat, cplx *electric,
+                    cplx *magnetic, cplx *vmean, cplx *ds, double *Ps, double *Ps_split,
+                    double *ewave_out) {
+  const int nspec = h->nspec;
+  const double kperp = h->kperp, kpar = h->kpar, vA = h->vA;
+  const cplx ii(0.0, 1.0);
+  auto W = [&](int i, int j) { return w0[i + 3 * j]; };  // wave(i+1, j+1)
+  auto CHI = [&](const cplx *c, int s, int i, int j) { return c[s + nspec * (i + 3 * j)]; };
+  auto LOW = [&](int s, int i, int j, int q) { return l2[s + nspec * (i + 3 * (j + 3 * (q + 1)))]; };
+  electric[0] = 1.0;
+  electric[2] = -electric[0] * (W(1, 0) * W(2, 1) - W(2, 0) * W(1, 1));
+  electric[2] = electric[2] / (W(1, 2) * W(2, 1) - W(2, 2) * W(1, 1));
+  if (std::abs(W(2, 1)) != 0.0) {
+    electric[1] = -electric[2] * W(2, 2) - electric[0] * W(2, 0);
+    electric[1] = electric[1] / W(2, 1);
+  } else {
+    electric[1] = W(1, 0) * W(0, 2) - W(0, 0) * W(1, 2);
+    electric[1] = electric[1] / (W(1, 2) * W(0, 1) - W(1, 1) * W(0, 2));
+  }
+  magnetic[0] = -1.0 * kpar * electric[1] / (omega * vA);
+  magnetic[1] = -1.0 * (kperp * electric[2] - kpar * electric[0]) / (omega * vA);
+  magnetic[2] = kperp * electric[1] / (omega * vA);
+  if (eigen) {
+    for (int s = 0; s < nspec; ++s) {
+      const double pf = h->current_int[s] / (h->ns[s] * h->qs[s]);
+      const double pre = (vA * vA) / (h->qs[s] * h->ns[s]);
+      auto sumec = [&](int j) {
+        return electric[0] * CHI(c0, s, j, 0) + electric[1] * CHI(c0, s, j, 1) +
+               electric[2] * CHI(c0, s, j, 2);
+      };
+      cplx *v = vmean + 3 * s;
+      if (pf == 0.0) {
+        for (int j = 0; j < 3; ++j) v[j] = -(pre * ii * omega * sumec(j));
+      } else {
+        for (int j = 0; j < 2; ++j) v[j] = -(pre * ii * omega * sumec(j));
+        v[2] = -(pre * ii * omega * sumec(2)) - pf * kperp * v[0] / (omega - kpar * pf);
+        v[2] = v[2] / (1.0 + (kpar * pf) / (omega - kpar * pf));
+      }
+      ds[s] = (1.0 / vA) * (v[0] * kperp + v[2] * kpar) / (omega - kpar * pf);
+    }
+  }
+  if (!heat) return;
+  cplx chihold[3][3], chih[3][3], dchih[3][3], term1[3];
+  for (int s = 0; s < nspec; ++s) Ps[s] = 0.0;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      cplx a = 0.0, b = 0.0, a2 = 0.0, b2 = 0.0;
+      for (int s = 0; s < nspec; ++s) {
+        a = a + CHI(c1, s, i, j);
+        b = b + std::conj(CHI(c1, s, j, i));
+        a2 = a2 + CHI(c2, s, i, j);
+        b2 = b2 + std::conj(CHI(c2, s, j, i));
+      }
+      chihold[i][j] = 0.5 * (a + b);
+      chih[i][j] = 0.5 * (a2 + b2);
+      dchih[i][j] = (1.000001 * chih[i][j] - chihold[i][j]) / 0.000001;
+    }
+  for (int s = 0; s < nspec; ++s) {
+    cplx t[3];
+    for (int i = 0; i < 3; ++i) {
+      t[i] = 0.0;
+      for (int k = 0; k < 3; ++k) {
+        const cplx chia = -0.5 * ii * (CHI(c1, s, k, i) - std::conj(CHI(c1, s, i, k)));
+        t[i] = t[i] + std::conj(electric[k]) * chia;
+      }
+    }
+    Ps[s] = (t[0] * electric[0] + t[1] * electric[1] + t[2] * electric[2]).real();
+  }
+  for (int i = 0; i < 3; ++i)
+    term1[i] = std::conj(electric[0]) * dchih[0][i] + std::conj(electric[1]) * dchih[1][i] +
+               std::conj(electric[2]) * dchih[2][i];
+  const double ewave =
+      ((term1[0] * electric[0] + term1[1] * electric[1] + term1[2] * electric[2]) +
+       (magnetic[0] * std::conj(magnetic[0]) + magnetic[1] * std::conj(magnetic[1]) +
+        magnetic[2] * std::conj(magnetic[2])))
+          .real();
+  for (int s = 0; s < nspec; ++s) Ps[s] = Ps[s] / ewave;
+  const cplx hi = -0.5 * ii;
+  const cplx exy[3] = {electric[0], electric[1], 0.0};
+  for (int s = 0; s < nspec; ++s) {
+    double *o = Ps_split + 4 * s;
+    double v = (hi * std::conj(electric[1]) * electric[1] * (LOW(s, 1, 1, 0) - std::conj(LOW(s, 1, 1, 0)))).real();
+    v = (v + hi * (electric[2] * std::conj(electric[1]) * LOW(s, 1, 2, 0) -
+                   std::conj(electric[2]) * electric[1] * std::conj(LOW(s, 1, 2, 0)))).real();
+    o[0] = v;
+    v = (hi * (electric[1] * std::conj(electric[2]) * LOW(s, 2, 1, 0) -
+               std::conj(electric[1]) * electric[2] * std::conj(LOW(s, 2, 1, 0)))).real();
+    v = (v + hi * std::conj(electric[2]) * electric[2] * (LOW(s, 2, 2, 0) - std::conj(LOW(s, 2, 2, 0)))).real();
+    o[1] = v;
+    for (int q = 1; q >= -1; q -= 2) {
+      cplx t[3];
+      for (int i = 0; i < 3; ++i) {
+        t[i] = 0.0;
+        for (int k = 0; k < 3; ++k)
+          t[i] = t[i] + std::conj(exy[k]) * (hi * (LOW(s, k, i, q) - std::conj(LOW(s, i, k, q))));
+      }
+      o[q == 1 ? 2 : 3] = (t[0] * exy[0] + t[1] * exy[1] + t[2] * exy[2]).real();
+    }
+    for (int k = 0; k < 4; ++k) o[k] = o[k] / ewave;
+  }
+  *ewave_out = ewave;
+}
+
+int calc_eigen_batch(alpsg_ctx *h, int n, const cplx *om, bool eigen, bool heat, cplx *ef, cplx *bf,
+                     cplx *Us, cplx *ds, double *Ps, double *Ps_split, double *W_EM) {
+  if (eigen && !h->have_current)
+    return fail(ALPSG_E_STATE, "current_int unknown: pass f0 or current_int in the plasma description");
+  const int ns = h->nspec;
+  const int m = heat ? 3 * n : n;
+  std::vector<cplx> w(m), det(m), chi((size_t)m * ns * 9), low((size_t)m * ns * 27), wave((size_t)m * 9);
+  for (int r = 0; r < n; ++r) {
+    w[r] = om[r];
+    if (heat) {
+      w[n + r] = cplx(om[r].real(), 0.0);
+      w[2 * n + r] = cplx((om[r] * 1.000001).real(), 0.0);
+    }
+  }
+  int rc = run_disp(h, m, reinterpret_cast<const double *>(w.data()), reinterpret_cast<double *>(det.data()),
+                    reinterpret_cast<double *>(chi.data()), reinterpret_cast<double *>(low.data()),
+                    reinterpret_cast<double *>(wave.data()));
+  if (rc) return rc;
+  std::vector<cplx> dummy(3 * ns + 8);
+  std::vector<double> dd(4 * ns + 4);
+  for (int r = 0; r < n; ++r) {
+    const cplx *c0 = &chi[(size_t)r * ns * 9], *w0 = &wave[(size_t)r * 9];
+    const cplx *c1 = heat ? &chi[(size_t)(n + r) * ns * 9] : nullptr;
+    const cplx *c2 = heat ? &chi[(size_t)(2 * n + r) * ns * 9] : nullptr;
+    const cplx *l2 = heat ? &low[(size_t)(2 * n + r) * ns * 27] : nullptr;
+    cplx e3[3], b3[3];
+    double ew = 0.0;
+    calc_eigen_one(h, om[r], c0, w0, c1, c2, l2, eigen, heat, e3, b3,
+                   eigen && Us ? Us + (size_t)r * 3 * ns : dummy.data(),
+                   eigen && ds ? ds + (size_t)r * ns : dummy.data() + 3 * ns,
+                   heat && Ps ? Ps + (size_t)r * ns : dd.data(),
+                   heat && Ps_split ? Ps_split + (size_t)r * 4 * ns : dd.data(), &ew);
+    for (int k = 0; k < 3; ++k) {
+      if (ef) ef[(size_t)r * 3 + k] = e3[k];
+      if (bf) bf[(size_t)r * 3 + k] = b3[k];
+    }
+    if (heat && W_EM) W_EM[r] = ew;
+  }
+  return ALPSG_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1100,6 +1250,34 @@ int alpsg_create(const alpsg_plasma_desc *d, alpsg_handle *out) {
     HIPCHK(hipMemset(h->err_d.p, 0, sizeof(int)));
     h->ngamma = d->ngamma;
     h->npparbar = d->npparbar;
+  }
+
+  // current_int (derivative_f0, ALPS_fns.f90:157-190): rectangle sums over the
+  // table in the reference's order; closed-form species n q pdrift / m
+  h->current_int.assign(nspec, 0.0);
+  if (d->current_int) {
+    h->current_int.assign(d->current_int, d->current_int + nspec);
+    h->have_current = true;
+  } else if (d->f0) {
+    const double PI = 3.141592653589793238462643383279502884;
+    for (int s = 0; s < nspec; ++s) {
+      if (d->usebM && d->usebM[s]) {
+        h->current_int[s] = d->ns[s] * d->qs[s] * d->bM_pdrift[s] / d->ms[s];
+        continue;
+      }
+      const double dpp = d->pp[f_pp(nspec, nperp, npar, s, 2, 2, 0)] - d->pp[f_pp(nspec, nperp, npar, s, 1, 2, 0)];
+      const double dpa = std::fabs(d->pp[f_pp(nspec, nperp, npar, s, 2, 2, 1)] -
+                                   d->pp[f_pp(nspec, nperp, npar, s, 2, 1, 1)]);
+      double c = 0.0;
+      for (int i = 0; i <= nperp; ++i)
+        for (int j = 0; j <= npar; ++j)
+          c = c + (d->ns[s] * d->qs[s] / d->ms[s]) * d->pp[f_pp(nspec, nperp, npar, s, i, j, 0)] *
+                      d->pp[f_pp(nspec, nperp, npar, s, i, j, 1)] *
+                      d->f0[(size_t)s + (size_t)nspec * (i + (size_t)(nperp + 1) * j)] * 2.0 * PI *
+                      dpp * dpa;
+      h->current_int[s] = c;
+    }
+    h->have_current = true;
   }
 
   // problem descriptor (per-k pointers filled by set_k)
@@ -1404,6 +1582,18 @@ int alpsg_set_k(alpsg_handle h, double kperp, double kpar, const int *nmax_in, i
   if (nmax_out)
     for (int s = 0; s < nspec; ++s) nmax_out[s] = nmax[s];
   return ALPSG_OK;
+}
+
+int alpsg_calc_eigen(alpsg_handle h, int n, const double *om, int eigen, int heat, double *ef,
+                     double *bf, double *Us, double *ds, double *Ps, double *Ps_split,
+                     double *W_EM) {
+  if (!h || (n > 0 && !om)) return fail(ALPSG_E_ARG, "null argument");
+  if (!h->have_k) return fail(ALPSG_E_STATE, "alpsg_set_k has not been called");
+  if (n <= 0) return ALPSG_OK;
+  return calc_eigen_batch(h, n, reinterpret_cast<const cplx *>(om), eigen != 0, heat != 0,
+                          reinterpret_cast<cplx *>(ef), reinterpret_cast<cplx *>(bf),
+                          reinterpret_cast<cplx *>(Us), reinterpret_cast<cplx *>(ds), Ps, Ps_split,
+                          W_EM);
 }
 
 int alpsg_autotune(alpsg_handle h, int n, const double *om, int *shape_out) {

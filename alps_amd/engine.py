@@ -39,7 +39,7 @@ EXPORTS = ["alpsg_last_error", "alpsg_create", "alpsg_destroy", "alpsg_set_k",
            "alpsg_disp_batch", "alpsg_disp_plain", "alpsg_secant_batch", "alpsg_map_search",
            "alpsg_om_scan", "alpsg_get_nmax", "alpsg_eval_count", "alpsg_last_kernel_ms",
            "alpsg_last_root_times", "alpsg_autotune", "alpsg_set_shape", "alpsg_get_shape",
-           "alpsg_last_kernel_split"]
+           "alpsg_last_kernel_split", "alpsg_calc_eigen"]
 
 
 class AlpsgError(RuntimeError):
@@ -67,6 +67,7 @@ class _Desc(C.Structure):
         ("ngamma", C.c_int), ("npparbar", C.c_int),
         ("gamma_rel", C.c_void_p), ("pparbar_rel", C.c_void_p), ("f0_rel", C.c_void_p),
         ("df0_rel", C.c_void_p),
+        ("current_int", C.c_void_p),
     ]
 
 
@@ -107,7 +108,7 @@ def load_library(path: str = LIB_PATH):
     L.alpsg_set_shape.argtypes = [vp, C.c_int, ip]
     L.alpsg_get_shape.argtypes = [vp, C.c_int, ip]
     for name in ["alpsg_autotune", "alpsg_set_shape", "alpsg_get_shape",
-           "alpsg_last_kernel_split"]:
+           "alpsg_last_kernel_split", "alpsg_calc_eigen"]:
         getattr(L, name).restype = C.c_int
     L.alpsg_last_kernel_ms.argtypes = [vp]
     L.alpsg_last_kernel_ms.restype = C.c_double
@@ -331,6 +332,23 @@ class Engine:
         """refine_guess (ALPS_fns.f90:3793-3856): secant on each guess, then det D there."""
         om, dfin, _ = self.solve_batch(guesses, roots)
         return om, dfin
+
+    def calc_eigen(self, roots, eigen=True, heat=True):
+        """calc_eigen (ALPS_fns.f90:2605-2899) at every root, batched: ef, bf [n, 3],
+        Us [n, nspec, 3], ds [n, nspec], Ps [n, nspec], Ps_split [n, nspec, 4], W_EM [n]."""
+        om = np.ascontiguousarray(np.asarray(roots, dtype=np.complex128).ravel())
+        n, ns = om.size, self.plasma.nspec
+        ef = np.zeros((n, 3), dtype=np.complex128)
+        bf = np.zeros((n, 3), dtype=np.complex128)
+        us = np.zeros((n, ns, 3), dtype=np.complex128)
+        ds = np.zeros((n, ns), dtype=np.complex128)
+        ps = np.zeros((n, ns))
+        psp = np.zeros((n, ns, 4))
+        ew = np.zeros(n)
+        _check(load_library().alpsg_calc_eigen(self._h, n, _dp(om), int(eigen), int(heat), _dp(ef),
+                                               _dp(bf), _dp(us), _dp(ds), _dp(ps), _dp(psp),
+                                               _dp(ew)))
+        return dict(ef=ef, bf=bf, Us=us, ds=ds, Ps=ps, Ps_split=psp, W_EM=ew)
 
     def map_search(self, omi, omf, gami, gamf, nr, ni, loggridw=False, loggridg=False,
                    numroots=100):

@@ -32,6 +32,7 @@ module alpsg
      integer(c_int) :: ngamma = 0, npparbar = 0
      type(c_ptr) :: gamma_rel = c_null_ptr, pparbar_rel = c_null_ptr
      type(c_ptr) :: f0_rel = c_null_ptr, df0_rel = c_null_ptr
+     type(c_ptr) :: current_int = c_null_ptr
   end type alpsg_plasma_desc
 
   type, bind(C), public :: alpsg_root_params
@@ -42,7 +43,7 @@ module alpsg
   public :: alpsg_create, alpsg_destroy, alpsg_set_k, alpsg_disp_batch, alpsg_disp_plain
   public :: alpsg_secant_batch, alpsg_map_search, alpsg_om_scan, alpsg_get_nmax
   public :: alpsg_eval_count, alpsg_error_text, alpsg_disp, alpsg_secant_osc
-  public :: alpsg_autotune, alpsg_set_shape, alpsg_get_shape
+  public :: alpsg_autotune, alpsg_set_shape, alpsg_get_shape, alpsg_calc_eigen
 
   interface
      function alpsg_create(desc, h) bind(C, name='alpsg_create') result(rc)
@@ -136,6 +137,16 @@ module alpsg
        type(c_ptr), value :: h
        integer(c_long_long) :: n
      end function alpsg_eval_count
+
+     function alpsg_calc_eigen(h, n, om, eigen, heat, ef, bf, Us, ds, Ps, Ps_split, W_EM) &
+          bind(C, name='alpsg_calc_eigen') result(rc)
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: h
+       integer(c_int), value :: n, eigen, heat
+       complex(c_double_complex), intent(in) :: om(*)
+       type(c_ptr), value :: ef, bf, Us, ds, Ps, Ps_split, W_EM
+       integer(c_int) :: rc
+     end function alpsg_calc_eigen
 
      function alpsg_autotune(h, n, om, shape_out) bind(C, name='alpsg_autotune') result(rc)
        import :: c_int, c_ptr

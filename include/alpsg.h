@@ -76,6 +76,9 @@ typedef struct alpsg_plasma_desc {
      param_fit_nperp >= ngamma. */
   int ngamma, npparbar;
   const double *gamma_rel, *pparbar_rel, *f0_rel, *df0_rel;
+  /* parallel current density per species (current_int, ALPS_fns.f90:157-190),
+     [nspec] or NULL to compute it from f0; needed by alpsg_calc_eigen only. */
+  const double *current_int;
 } alpsg_plasma_desc;
 
 /* Root-search controls (&system namelist, ALPS_var.f90:147-165). */
@@ -140,6 +143,18 @@ int alpsg_map_search(alpsg_handle h, double omi, double omf, double gami, double
 int alpsg_om_scan(alpsg_handle h, const alpsg_root_params *rp, int scan_type, int log_scan,
                   double diff, double diff2, int n_out, int n_res, int nroots, double *roots,
                   double *kperp_out, double *kpar_out, double *roots_out);
+
+/* calc_eigen (ALPS_fns.f90:2605-2899) for n roots at the handle's k, batched:
+   disp at every root (the "final instance of disp" the reference runs before
+   it), and with heat != 0 at real(om) and real(om*1.000001).  Outputs per
+   root r, in the Fortran layouts of om_scan's .eigen/.heat/.heat_mech rows:
+     ef, bf [r][3] complex; Us [r][nspec][3] complex (Us(1:3,1:nspec));
+     ds [r][nspec] complex; Ps [r][nspec]; Ps_split [r][nspec][4]
+     (Ps_split(1:4,1:nspec)); W_EM [r].
+   Arrays for the part not asked for (eigen == 0 or heat == 0) may be NULL. */
+int alpsg_calc_eigen(alpsg_handle h, int n, const double *om, int eigen, int heat, double *ef,
+                     double *bf, double *Us, double *ds, double *Ps, double *Ps_split,
+                     double *W_EM);
 
 /* Launch-shape autotuner.  A shape is 5 ints: lanes per omega in the node
    sweep, its wavefronts per workgroup, its workgroups per omega block

@@ -1365,3 +1365,149 @@ int orc_om_scan(orc_plasma *P, int scan_type, int log_scan, double diff, double 
   free(jump);
   return 0;
 }
+
+/* ========================================================================
+ * Eigenfunctions and heating rates: ALPS_fns.f90:2605-2899
+ * ======================================================================== */
+
+/* current_int, ALPS_fns.f90:157-190 (rectangle sums over the whole table;
+   closed-form species: n q pdrift / m). */
+void orc_current_int(orc_plasma *P) {
+  for (int is = 1; is <= P->nspec; ++is) {
+    const int s = is - 1;
+    P->current_int[s] = 0.0;
+    if (P->usebM[s]) {
+      P->current_int[s] = P->ns[s] * P->qs[s] * P->bM_pdrift[s] / P->ms[s];
+      continue;
+    }
+    const double dpperp = PP(P, is, 2, 2, 1) - PP(P, is, 1, 2, 1);
+    const double dppar = fabs(PP(P, is, 2, 2, 2) - PP(P, is, 2, 1, 2));
+    for (int iperp = 0; iperp <= P->nperp; ++iperp)
+      for (int ipar = 0; ipar <= P->npar; ++ipar)
+        P->current_int[s] = P->current_int[s] + (P->ns[s] * P->qs[s] / P->ms[s]) *
+                                                   PP(P, is, iperp, ipar, 1) * PP(P, is, iperp, ipar, 2) *
+                                                   F0(P, is, iperp, ipar) * 2.0 * ORC_PI * dpperp * dppar;
+  }
+}
+
+/* calc_eigen, ALPS_fns.f90:2605-2899 */
+void orc_calc_eigen(orc_plasma *P, double complex omega, const double complex *chi0_in,
+                    const double complex *wave_in, int eigen, int heat, double complex *electric,
+                    double complex *magnetic, double complex *vmean, double complex *ds, double *Ps,
+                    double *Ps_split, double *ewave_out) {
+  const int nspec = P->nspec;
+  const double kperp = P->kperp, kpar = P->kpar, vA = P->vA;
+  const double complex ii = I;
+  double complex wave[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) wave[i][j] = wave_in[i * 3 + j];
+#define CHI(c, jj, i, j) (c)[((jj)*3 + (i)) * 3 + (j)]
+  electric[0] = 1.0;
+  electric[2] = -electric[0] * (wave[1][0] * wave[2][1] - wave[2][0] * wave[1][1]);
+  electric[2] = electric[2] / (wave[1][2] * wave[2][1] - wave[2][2] * wave[1][1]);
+  if (cabs(wave[2][1]) != 0.0) {
+    electric[1] = -electric[2] * wave[2][2] - electric[0] * wave[2][0];
+    electric[1] = electric[1] / wave[2][1];
+  } else {
+    electric[1] = wave[1][0] * wave[0][2] - wave[0][0] * wave[1][2];
+    electric[1] = electric[1] / (wave[1][2] * wave[0][1] - wave[1][1] * wave[0][2]);
+  }
+  magnetic[0] = -1.0 * kpar * electric[1] / (omega * vA);
+  magnetic[1] = -1.0 * (kperp * electric[2] - kpar * electric[0]) / (omega * vA);
+  magnetic[2] = kperp * electric[1] / (omega * vA);
+  if (eigen) {
+    double parallel_flow[ORC_MAXSPEC];
+    for (int jj = 0; jj < nspec; ++jj) {
+      for (int j = 0; j < 3; ++j) vmean[jj * 3 + j] = 0.0;
+      parallel_flow[jj] = P->current_int[jj] / (P->ns[jj] * P->qs[jj]);
+      const double pre = pow(vA, 2.0) / (P->qs[jj] * P->ns[jj]);
+#define SUMEC(j) (electric[0] * CHI(chi0_in, jj, j, 0) + electric[1] * CHI(chi0_in, jj, j, 1) + \
+                  electric[2] * CHI(chi0_in, jj, j, 2))
+      if (parallel_flow[jj] == 0.0) {
+        for (int j = 0; j < 3; ++j) vmean[jj * 3 + j] = -(pre * ii * omega * SUMEC(j));
+      } else {
+        for (int j = 0; j < 2; ++j) vmean[jj * 3 + j] = -(pre * ii * omega * SUMEC(j));
+        vmean[jj * 3 + 2] = -(pre * ii * omega * SUMEC(2)) -
+                            parallel_flow[jj] * kperp * vmean[jj * 3 + 0] / (omega - kpar * parallel_flow[jj]);
+        vmean[jj * 3 + 2] =
+            vmean[jj * 3 + 2] / (1 + (kpar * parallel_flow[jj]) / (omega - kpar * parallel_flow[jj]));
+      }
+#undef SUMEC
+    }
+    for (int jj = 0; jj < nspec; ++jj)
+      ds[jj] = (1.0 / vA) * (vmean[jj * 3 + 0] * kperp + vmean[jj * 3 + 2] * kpar) /
+               (omega - kpar * parallel_flow[jj]);
+  }
+  if (!heat) return;
+  double complex c1[ORC_MAXSPEC * 9], l1[ORC_MAXSPEC * 27], w1[9];
+  double complex chia[ORC_MAXSPEC][3][3], chihold[3][3], chih[3][3], dchih[3][3];
+  double complex term[ORC_MAXSPEC][3], term1[3];
+  (void)orc_disp(P, creal(omega) + 0.0 * I, c1, l1, w1);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double complex a = 0.0, b = 0.0;
+      for (int jj = 0; jj < nspec; ++jj) {
+        chia[jj][i][j] = -0.5 * ii * (CHI(c1, jj, i, j) - conj(CHI(c1, jj, j, i)));
+        a = a + CHI(c1, jj, i, j);
+        b = b + conj(CHI(c1, jj, j, i));
+      }
+      chihold[i][j] = 0.5 * (a + b);
+    }
+  for (int i = 0; i < 3; ++i)
+    for (int jj = 0; jj < nspec; ++jj)
+      term[jj][i] = conj(electric[0]) * chia[jj][0][i] + conj(electric[1]) * chia[jj][1][i] +
+                    conj(electric[2]) * chia[jj][2][i];
+  for (int jj = 0; jj < nspec; ++jj)
+    Ps[jj] = creal(term[jj][0] * electric[0] + term[jj][1] * electric[1] + term[jj][2] * electric[2]);
+  (void)orc_disp(P, creal(omega * 1.000001) + 0.0 * I, c1, l1, w1);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double complex a = 0.0, b = 0.0;
+      for (int jj = 0; jj < nspec; ++jj) {
+        a = a + CHI(c1, jj, i, j);
+        b = b + conj(CHI(c1, jj, j, i));
+      }
+      chih[i][j] = 0.5 * (a + b);
+      dchih[i][j] = (1.000001 * chih[i][j] - chihold[i][j]) / 0.000001;
+    }
+  for (int i = 0; i < 3; ++i)
+    term1[i] = conj(electric[0]) * dchih[0][i] + conj(electric[1]) * dchih[1][i] +
+               conj(electric[2]) * dchih[2][i];
+  const double ewave =
+      creal((term1[0] * electric[0] + term1[1] * electric[1] + term1[2] * electric[2]) +
+            (magnetic[0] * conj(magnetic[0]) + magnetic[1] * conj(magnetic[1]) +
+             magnetic[2] * conj(magnetic[2])));
+  for (int jj = 0; jj < nspec; ++jj) Ps[jj] = Ps[jj] / ewave;
+  /* the n = 0, +1, -1 parts from chi0_low of the last disp (at real(omega*1.000001)) */
+#define LOW(jj, i, j, q) l1[(((jj)*3 + (i)) * 3 + (j)) * 3 + ((q) + 1)]
+  const double complex hi = -0.5 * ii;
+  for (int jj = 0; jj < nspec; ++jj) {
+    double v;
+    v = creal(hi * conj(electric[1]) * electric[1] * (LOW(jj, 1, 1, 0) - conj(LOW(jj, 1, 1, 0))));
+    v = creal(v + hi * (electric[2] * conj(electric[1]) * LOW(jj, 1, 2, 0) -
+                        conj(electric[2]) * electric[1] * conj(LOW(jj, 1, 2, 0))));
+    Ps_split[jj * 4 + 0] = v;
+    v = creal(hi * (electric[1] * conj(electric[2]) * LOW(jj, 2, 1, 0) -
+                    conj(electric[1]) * electric[2] * conj(LOW(jj, 2, 1, 0))));
+    v = creal(v + hi * conj(electric[2]) * electric[2] * (LOW(jj, 2, 2, 0) - conj(LOW(jj, 2, 2, 0))));
+    Ps_split[jj * 4 + 1] = v;
+  }
+  const double complex exy[3] = {electric[0], electric[1], 0.0};
+  for (int q = 1; q >= -1; q -= 2) {
+    for (int jj = 0; jj < nspec; ++jj) {
+      double complex t[3];
+      for (int i = 0; i < 3; ++i) {
+        t[i] = 0.0;
+        for (int k = 0; k < 3; ++k) {
+          const double complex ch = hi * (LOW(jj, k, i, q) - conj(LOW(jj, i, k, q)));
+          t[i] = t[i] + conj(exy[k]) * ch;
+        }
+      }
+      Ps_split[jj * 4 + (q == 1 ? 2 : 3)] = creal(t[0] * exy[0] + t[1] * exy[1] + t[2] * exy[2]);
+    }
+  }
+#undef LOW
+#undef CHI
+  for (int k = 0; k < 4 * nspec; ++k) Ps_split[k] = Ps_split[k] / ewave;
+  *ewave_out = ewave;
+}

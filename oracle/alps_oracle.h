@@ -80,6 +80,8 @@ typedef struct {
   /* per species (NULL unless relativistic): [ngamma+1][npparbar+1], df0_rel [..][..][2] */
   double *gamma_rel[ORC_MAXSPEC], *pparbar_rel[ORC_MAXSPEC], *f0_rel[ORC_MAXSPEC],
       *df0_rel[ORC_MAXSPEC];
+  /* parallel current density per species (derivative_f0, ALPS_fns.f90:157-190) */
+  double current_int[ORC_MAXSPEC];
 } orc_plasma;
 
 /* Branch counters incremented by the hot-path restatement (test instrumentation). */
@@ -156,6 +158,19 @@ double complex orc_landau_integrate_rel(const orc_plasma *P, int s, double compl
 double complex orc_int_ee_rel(const orc_plasma *P, int s);
 void orc_determine_resonances_rel(const orc_plasma *P, int s, double complex om, int nn, int *fp,
                                   int *fm);
+
+/* current_int of derivative_f0 (ALPS_fns.f90:157-190), from f0 */
+void orc_current_int(orc_plasma *P);
+
+/* calc_eigen (ALPS_fns.f90:2605-2899) at omega, which must be the last
+   argument of orc_disp (its chi0 and wave are passed in, row-major as orc_disp
+   writes them).  Outputs in the Fortran layouts: ef[3], bf[3], Us[3*nspec]
+   (Us(1:3,1:nspec)), ds[nspec], Ps[nspec], Ps_split[4*nspec]
+   (Ps_split(1:4,1:nspec)), *ewave.  Calls orc_disp twice when heat != 0. */
+void orc_calc_eigen(orc_plasma *P, double complex omega, const double complex *chi0,
+                    const double complex *wave, int eigen, int heat, double complex *ef,
+                    double complex *bf, double complex *Us, double complex *ds, double *Ps,
+                    double *Ps_split, double *ewave);
 
 /* Root finders (ALPS_fns.f90:1815-2195). Return the number of iterations. */
 int orc_secant(orc_plasma *P, double complex *om);

@@ -48,6 +48,7 @@ class _Plasma(C.Structure):
         ("param_np", C.c_int),
         ("gamma_rel", C.POINTER(C.c_double) * MAXSPEC), ("pparbar_rel", C.POINTER(C.c_double) * MAXSPEC),
         ("f0_rel", C.POINTER(C.c_double) * MAXSPEC), ("df0_rel", C.POINTER(C.c_double) * MAXSPEC),
+        ("current_int", C.c_double * MAXSPEC),
     ]
 
 
@@ -116,6 +117,10 @@ def lib():
         L.orc_cbessj.argtypes = [_C128, C.c_int]
         L.orc_int_ee_rel.restype = _C128
         L.orc_int_ee_rel.argtypes = [P, C.c_int]
+        L.orc_current_int.restype = None
+        L.orc_current_int.argtypes = [P]
+        L.orc_calc_eigen.restype = None
+        L.orc_calc_eigen.argtypes = [P, _C128, dp, dp, C.c_int, C.c_int, dp, dp, dp, dp, dp, dp, dp]
         L.orc_rtsec.restype = _C128
         L.orc_rtsec.argtypes = [P, _C128, C.POINTER(C.c_int)]
         L.orc_map_search.restype = C.c_int
@@ -254,6 +259,7 @@ class Oracle:
         s.n_threads = n_threads
         self.s = s
         L.orc_derivative_f0(C.byref(s))
+        L.orc_current_int(C.byref(s))
 
     def __del__(self):
         try:
@@ -338,6 +344,31 @@ class Oracle:
 
     def int_ee_rel(self, sproc: int) -> complex:
         return _z(lib().orc_int_ee_rel(C.byref(self.s), sproc))
+
+    @property
+    def current_int(self):
+        return [self.s.current_int[i] for i in range(self.plasma.nspec)]
+
+    def calc_eigen(self, om, eigen=True, heat=True):
+        """calc_eigen (ALPS_fns.f90:2605-2899) at om after disp(om): electric and
+        magnetic field ef, bf [3], velocities Us [nspec, 3], densities ds
+        [nspec], heating Ps [nspec], Ps_split [nspec, 4], W_EM."""
+        ns = self.plasma.nspec
+        chi = np.zeros(ns * 9, dtype=np.complex128)
+        low = np.zeros(ns * 27, dtype=np.complex128)
+        wave = np.zeros(9, dtype=np.complex128)
+        lib().orc_disp(C.byref(self.s), _c(om), _ptr(chi), _ptr(low), _ptr(wave))
+        ef = np.zeros(3, dtype=np.complex128)
+        bf = np.zeros(3, dtype=np.complex128)
+        us = np.zeros(3 * ns, dtype=np.complex128)
+        ds = np.zeros(ns, dtype=np.complex128)
+        ps = np.zeros(ns)
+        psp = np.zeros(4 * ns)
+        ew = np.zeros(1)
+        lib().orc_calc_eigen(C.byref(self.s), _c(om), _ptr(chi), _ptr(wave), int(eigen), int(heat),
+                             _ptr(ef), _ptr(bf), _ptr(us), _ptr(ds), _ptr(ps), _ptr(psp), _ptr(ew))
+        return dict(ef=ef, bf=bf, Us=us.reshape(ns, 3), ds=ds, Ps=ps, Ps_split=psp.reshape(ns, 4),
+                    W_EM=float(ew[0]))
 
     def rtsec(self, om):
         flag = C.c_int(0)

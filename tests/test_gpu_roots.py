@@ -10,7 +10,7 @@ import pytest
 
 import cases
 from oracle import Oracle
-from test_oracle_golden import GOLD, kpar_fast_scan, printed_equal
+from test_oracle_golden import GOLD, GOLD_EIGEN, GOLD_HEAT, eigen_rows, kpar_fast_scan, printed_equal
 
 pytestmark = pytest.mark.gpu
 
@@ -102,6 +102,36 @@ def test_kpar_fast_scan_on_gpu_matches_golden_and_oracle():
     _, (rc, okp, okz, oroots) = kpar_fast_scan(O)
     rel = np.abs(roots[:, 0] - oroots[:, 0]) / np.abs(oroots[:, 0])
     assert np.max(rel) <= ROOT_TOL, rel
+    # eigenfunctions and heating rates on the device path, against the goldens
+    ge, gh = np.loadtxt(GOLD_EIGEN), np.loadtxt(GOLD_HEAT)
+    for i in range(33):
+        E.set_k(kp[i], kz[i])
+        r = E.calc_eigen([roots[i, 0]])
+        e, h = eigen_rows(kp[i], kz[i], roots[i, 0], {k: v[0] for k, v in r.items()})
+        for j, (v, g) in enumerate(zip(e, ge[i])):
+            assert printed_equal(v, g), ("eigen", i, j, v, g)
+        for j, (v, g) in enumerate(zip(h, gh[i])):
+            assert printed_equal(v, g), ("heat", i, j, v, g)
+
+
+@pytest.mark.parametrize("case", ["drift", "three"])
+def test_calc_eigen_matches_oracle(case):
+    """Batched calc_eigen against the restatement: eigenfunctions to 1e-9;
+    heating rates to 1e-5 (their wave energy differentiates chi over a 1e-6
+    step in omega, which scales chi's 1e-14 differences by 1e6)."""
+    p = cases.small_bimax(drift=0.7, alph=2.5) if case == "drift" else cases.three_species()
+    O, E = Oracle(p), _engine(p)
+    O.set_k(0.3, 0.3)
+    E.set_k(0.3, 0.3)
+    oms = [0.5 - 0.05j, 0.8 - 0.02j, 1.3 - 0.1j]
+    g = E.calc_eigen(oms)
+    for r, om in enumerate(oms):
+        o = O.calc_eigen(om)
+        for key, tol in (("ef", 1e-9), ("bf", 1e-9), ("Us", 1e-9), ("ds", 1e-9),
+                         ("Ps", 1e-5), ("Ps_split", 1e-5)):
+            a, b = np.asarray(g[key][r]), np.asarray(o[key])
+            assert np.max(np.abs(a - b)) <= tol * max(np.max(np.abs(b)), 1e-300), (key, om, a, b)
+        assert abs(g["W_EM"][r] - o["W_EM"]) <= 1e-5 * abs(o["W_EM"])
 
 
 def test_fortran_host_matches_python_host():

@@ -11,6 +11,18 @@ import cases
 from oracle import Oracle
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden", "test_kpar_fast.scan_kpara_1.root_1")
+GOLD_EIGEN = os.path.join(os.path.dirname(__file__), "golden", "test_kpar_fast.eigen_kpara_1.root_1")
+GOLD_HEAT = os.path.join(os.path.dirname(__file__), "golden", "test_kpar_fast.heat_kpara_1.root_1")
+
+
+def eigen_rows(kperp, kpar, w, r):
+    """The .eigen and .heat rows om_scan writes (ALPS_fns.f90:2567-2583):
+    kperp, kpar, omega, ef, bf, Us, ds / kperp, kpar, omega, Ps, W_EM."""
+    e = [kperp, kpar, w.real, w.imag]
+    for a in list(r["ef"]) + list(r["bf"]) + list(np.ravel(r["Us"])) + list(r["ds"]):
+        e += [a.real, a.imag]
+    h = [kperp, kpar, w.real, w.imag] + list(r["Ps"]) + [float(r["W_EM"])]
+    return e, h
 
 
 def printed_equal(v, g):
@@ -42,6 +54,17 @@ def test_oracle_reproduces_kpar_fast_golden():
         row = [kp[i], kz[i], roots[i, 0].real, roots[i, 0].imag]
         for v, g in zip(row, gold[i]):
             assert printed_equal(v, g), (i, row, gold[i])
+    # calc_eigen at every output row: every printed digit of the .eigen and
+    # .heat goldens (32 and 7 columns)
+    ge, gh = np.loadtxt(GOLD_EIGEN), np.loadtxt(GOLD_HEAT)
+    assert ge.shape == (33, 32) and gh.shape == (33, 7)
+    for i in range(33):
+        O.set_k(kp[i], kz[i])
+        e, h = eigen_rows(kp[i], kz[i], roots[i, 0], O.calc_eigen(roots[i, 0]))
+        for j, (v, g) in enumerate(zip(e, ge[i])):
+            assert printed_equal(v, g), ("eigen", i, j, v, g)
+        for j, (v, g) in enumerate(zip(h, gh[i])):
+            assert printed_equal(v, g), ("heat", i, j, v, g)
 
 
 def test_oracle_secant_and_rtsec_reach_the_golden_root():
