@@ -111,7 +111,12 @@ def test_kpar_fast_scan_on_gpu_matches_golden_and_oracle():
         for j, (v, g) in enumerate(zip(e, ge[i])):
             assert printed_equal(v, g), ("eigen", i, j, v, g)
         for j, (v, g) in enumerate(zip(h, gh[i])):
-            assert printed_equal(v, g), ("heat", i, j, v, g)
+            # Ps and W_EM difference chi over a 1e-6 step in omega, which lifts
+            # chi's 1e-14 device/CPU differences to ~1e-8: a value on a rounding
+            # boundary of the fifth printed digit may land one unit away
+            ok = printed_equal(v, g) if j < 4 else abs(v - g) <= 1.0 * 10.0 ** (
+                math.floor(math.log10(abs(g))) - 4) * (1 + 1e-9)
+            assert ok, ("heat", i, j, v, g)
 
 
 @pytest.mark.parametrize("case", ["drift", "three"])
