@@ -16,7 +16,8 @@ LIB = os.path.join(HERE, "libalpsg.so")
 FDIR = os.path.join(HERE, "fortran")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-SOURCES = ["alpsg_api.cpp", "alpsg_tables.hip", "alpsg_disp.hip", "alpsg_nhds.hip", "alpsg_rel.hip"]
+SOURCES = ["alpsg_api.cpp", "alpsg_server.cpp", "alpsg_tables.hip", "alpsg_disp.hip", "alpsg_nhds.hip",
+           "alpsg_rel.hip"]
 # No packed-FP32 instructions; FP64 code is unaffected by this switch.
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -47,7 +48,7 @@ def build_library(verbose: bool = False) -> str:
         _run(cmd)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", tmp, *objs])
     os.replace(tmp, LIB)
     return LIB
 

@@ -22,6 +22,7 @@
 
 #include "../../include/alpsg.h"
 #include "alpsg_device.h"
+#include "alpsg_osc.h"
 
 // kernels (alpsg_tables.hip, alpsg_disp.hip)
 hipError_t launch_derivative(hipStream_t st, int nspec, int nperp, int npar, const double *f0,
@@ -36,10 +37,11 @@ hipError_t launch_species_k(hipStream_t st, int nperp, int npar, int nmax, doubl
                             const double *ppar, const double *df1, const double *df2,
                             const int *mtab, double *bes, double *atab, double *nodes);
 hipError_t launch_sweep(hipStream_t st, int nch, int W, int RC, const ProbDev *prob, int nom,
-                        const double2 *om, const int *active, double2 *chiA, double2 *lowA);
+                        const double2 *om, const int *active, double2 *chiA, double2 *lowA,
+                        const int *pidx = nullptr);
 hipError_t launch_resonant(hipStream_t st, int nch, int RB, const ProbDev *prob, int nspec, int nom,
                            const double2 *om, const int *active, double2 *chiB, double2 *lowB,
-                           bool general);
+                           bool general, const int *pidx = nullptr);
 hipError_t launch_nhds(hipStream_t st, const ProbDev *prob, int nspec, int nom, const double2 *om,
                        const int *active, double2 *chiA, double2 *lowA);
 int nhds_tables(double z, int bm_nmax, double bessel_zero, std::vector<double2> &tab);
@@ -53,7 +55,7 @@ hipError_t launch_assemble(hipStream_t st, const ProbDev *prob, int nspec, int n
                            const double2 *om, const int *active, const double2 *chiA,
                            const double2 *chiB, const double2 *lowA, const double2 *lowB,
                            const double2 *chiR, int RR, const double2 *lowR, double2 *det,
-                           double2 *chi, double2 *low, double2 *wave);
+                           double2 *chi, double2 *low, double2 *wave, const int *pidx = nullptr);
 
 namespace {
 
@@ -93,6 +95,9 @@ struct DevBuf {
 };
 
 }  // namespace
+
+// the error text of the calling thread, for the other translation units
+int alpsg_internal_fail(int code, const std::string &msg) { return fail(code, msg); }
 
 struct alpsg_ctx {
   hipStream_t st = nullptr;
@@ -450,96 +455,20 @@ int run_disp(alpsg_ctx *h, int n, const double *om, double *det, double *chi, do
 typedef std::complex<double> cplx;
 
 // secant_osc (ALPS_fns.f90:1919-2101) for many independent roots: each root is
-// a small state machine whose disp() requests are served in batches.
-struct OscState {
-  cplx om, prevom, prev2om, prev3om, prev4om, prevD, D, minom, minD, jump, dplus;
-  double damping;
-  int iter, osc, phase, done;
-};
-
+// a state machine (alpsg_osc.h) whose disp() requests are served in batches.
 int secant_osc_batch(alpsg_ctx *h, const alpsg_root_params *rp, int nroots, cplx *om, cplx *Dfin,
                      int *iters) {
-  const cplx delta(1.e-6, 1.e-8);
-  const double lambda = (double)0.1f, osc_threshold = (double)1.e-3f, cap = (double)0.1f;
-  std::vector<OscState> S(nroots);
+  std::vector<alpsg_host::OscMachine> S(nroots);
   std::vector<cplx> req(nroots), res(nroots);
   std::vector<int> who;
   const auto t0 = std::chrono::steady_clock::now();
   h->root_ms.assign(nroots, 0.0);
-  // phases: 0 = need D(om) initial, 1 = need D(prevom) initial, 2 = need D(om) iteration,
-  //         3 = need D(prevom+1e-8), 4 = need D(om*(1+delta)), 5 = need D(om*(1-delta))
-  for (int r = 0; r < nroots; ++r) {
-    OscState &s = S[r];
-    s.om = om[r];
-    s.phase = 0;
-    s.done = 0;
-    s.iter = 0;
-    s.osc = 0;
-    s.damping = 1.0;
-  }
-  auto decide = [&](OscState &s) {
-    // convergence test and the rest of the iteration body after D (and the
-    // optional prevD refresh) are known; may request Newton evaluations.
-    if (std::abs(s.D) < rp->D_threshold) {
-      s.jump = 0.0;
-      s.done = 1;
-      return;
-    }
-    if (s.iter > 4) {
-      auto close = [&](cplx a) {
-        return (std::fabs(s.om.real() - a.real()) < (std::fabs(s.om.real()) * osc_threshold)) &&
-               (std::fabs(s.om.imag() - a.imag()) < (std::fabs(s.om.imag()) * osc_threshold));
-      };
-      if (close(s.prevom) || close(s.prev2om) || close(s.prev3om) || close(s.prev4om)) {
-        s.osc = s.osc + 1;
-        s.damping = std::min(0.5, s.damping * 0.75);
-      }
-    }
-    if (s.osc > 1) {
-      s.phase = 4;
-      return;
-    }
-    s.jump = s.damping * s.D * (s.om - s.prevom) / (s.D - s.prevD);
-    s.phase = 6;  // finish
-  };
-  auto finish = [&](OscState &s) {
-    if (std::abs(s.jump) > cap * std::abs(s.om)) s.jump = (cap * std::abs(s.om)) * (s.jump / std::abs(s.jump));
-    if (std::abs(s.D) > std::abs(s.prevD)) s.jump = 0.5 * s.jump;
-    s.prev4om = s.prev3om;
-    s.prev3om = s.prev2om;
-    s.prev2om = s.prevom;
-    s.prevom = s.om;
-    s.prevD = s.D;
-    if (std::abs(s.D) < std::abs(s.minD)) {
-      s.minom = s.om;
-      s.minD = s.D;
-    }
-    s.om = s.om - s.jump;
-    s.phase = 2;
-  };
-  auto start_iter = [&](OscState &s) {
-    if (s.iter <= rp->numiter - 1) {
-      s.iter = s.iter + 1;
-      s.phase = 2;
-    } else {
-      s.done = 1;
-    }
-  };
-  for (int guard = 0; guard < 100000; ++guard) {
+  for (int r = 0; r < nroots; ++r) S[r].start(om[r], *rp);
+  for (;;) {
     who.clear();
     for (int r = 0; r < nroots; ++r) {
-      OscState &s = S[r];
-      if (s.done) continue;
-      cplx z;
-      switch (s.phase) {
-        case 0: z = s.om; break;
-        case 1: z = s.prevom; break;
-        case 2: z = s.om; break;
-        case 3: z = s.prevom; break;
-        case 4: z = s.om * (1.0 + delta); break;
-        default: z = s.om * (1.0 - delta); break;
-      }
-      req[who.size()] = z;
+      if (S[r].done) continue;
+      req[who.size()] = S[r].want();
       who.push_back(r);
     }
     if (who.empty()) break;
@@ -549,66 +478,13 @@ int secant_osc_batch(alpsg_ctx *h, const alpsg_root_params *rp, int nroots, cplx
     const double now_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     for (size_t t = 0; t < who.size(); ++t) {
-      OscState &s = S[who[t]];
-      cplx D = res[t];
       h->root_ms[who[t]] = now_ms;
-      switch (s.phase) {
-        case 0:
-          s.D = D;
-          s.minom = s.om;
-          s.minD = D;
-          s.prevom = s.om * (1.0 - rp->D_prec);
-          s.prev2om = s.prev3om = s.prev4om = s.om;
-          s.phase = 1;
-          break;
-        case 1:
-          s.prevD = D;
-          if (std::abs(s.prevD) < std::abs(s.minD)) {
-            s.minom = s.prevom;
-            s.minD = s.prevD;
-          }
-          s.iter = 0;
-          start_iter(s);
-          break;
-        case 2:
-          s.D = D;
-          if (std::abs(s.D - s.prevD) < 1.e-80) {
-            s.prevom = s.prevom + 1.e-8;
-            s.phase = 3;
-            break;
-          }
-          decide(s);
-          if (s.phase == 6) {
-            finish(s);
-            start_iter(s);
-          }
-          break;
-        case 3:
-          s.prevD = D;
-          decide(s);
-          if (s.phase == 6) {
-            finish(s);
-            start_iter(s);
-          }
-          break;
-        case 4:
-          s.dplus = D;
-          s.phase = 5;
-          break;
-        case 5: {
-          cplx Dprime = (s.dplus - D) / (2.0 * s.om * delta);
-          s.jump = s.D / (Dprime + lambda * s.D);
-          finish(s);
-          start_iter(s);
-        } break;
-      }
+      S[who[t]].feed(res[t]);
     }
   }
   for (int r = 0; r < nroots; ++r) {
-    OscState &s = S[r];
-    if (s.iter >= rp->numiter) s.om = s.minom;
-    om[r] = s.om;
-    if (iters) iters[r] = s.iter;
+    om[r] = S[r].result();
+    if (iters) iters[r] = S[r].iter;
   }
   if (Dfin) {
     int rc = run_disp(h, nroots, reinterpret_cast<const double *>(om),

@@ -276,10 +276,13 @@ __global__ __launch_bounds__(64 * W) void k_sweep(const ProbDev *__restrict__ pr
                                                   const double2 *__restrict__ om_in,
                                                   const int *__restrict__ active,
                                                   double2 *__restrict__ chiA,
-                                                  double2 *__restrict__ lowA) {
+                                                  double2 *__restrict__ lowA,
+                                                  const int *__restrict__ pidx) {
   constexpr int OMB = 64 / NCH;
   __shared__ double lchi[W][12][64];
-  const ProbDev &Pd = *probp;
+  // one problem per launch, or (pidx, one omega per workgroup: NCH = 64) the
+  // problem of each omega
+  const ProbDev &Pd = (NCH == 64 && pidx != nullptr) ? probp[pidx[blockIdx.x]] : *probp;
   const int lane = threadIdx.x & 63;
   const int TW = W * gridDim.y;
   const int wv = blockIdx.y * W + (threadIdx.x >> 6);
@@ -694,10 +697,11 @@ __global__ __launch_bounds__(64) void k_resonant(const ProbDev *__restrict__ pro
                                                  const double2 *__restrict__ om_in,
                                                  const int *__restrict__ active,
                                                  double2 *__restrict__ chiB,
-                                                 double2 *__restrict__ lowB) {
+                                                 double2 *__restrict__ lowB,
+                                                 const int *__restrict__ pidx) {
   constexpr int OMB = 64 / NCH;
   __shared__ double lds_win[OMB][2 * PV_WIN];
-  const ProbDev &Pd = *probp;
+  const ProbDev &Pd = (NCH == 64 && pidx != nullptr) ? probp[pidx[blockIdx.x]] : *probp;
   const int lane = threadIdx.x;
   const int g = lane / NCH, c = lane % NCH;
   const int iom = blockIdx.x * OMB + g;
@@ -818,10 +822,9 @@ __global__ __launch_bounds__(256) void k_assemble(
     const double2 *__restrict__ chiB, const double2 *__restrict__ lowA,
     const double2 *__restrict__ lowB, const double2 *__restrict__ chiR, int RR,
     const double2 *__restrict__ lowR, double2 *__restrict__ det_out, double2 *__restrict__ chi_out,
-    double2 *__restrict__ low_out, double2 *__restrict__ wave_out) {
+    double2 *__restrict__ low_out, double2 *__restrict__ wave_out, int nspec,
+    const int *__restrict__ pidx) {
   __shared__ double2 lchi[256];
-  const ProbDev &Pd = *probp;
-  const int nspec = Pd.nspec;
   const int per = 6 * nspec;
   const int opb = 256 / per;  // omegas per block
   const int t = threadIdx.x;
@@ -829,6 +832,7 @@ __global__ __launch_bounds__(256) void k_assemble(
   const int s = sk / 6, k = sk % 6;
   const int iom = blockIdx.x * opb + lo_om;
   const bool in = (lo_om < opb) && (iom < nom) && (active == nullptr || active[iom] != 0);
+  const ProbDev &Pd = pidx != nullptr ? probp[pidx[min(iom, nom - 1)]] : *probp;
   if (in) {
     // partial sums, loads issued in batches of 8
     cd v = mk(0.0, 0.0);
@@ -958,13 +962,15 @@ __global__ __launch_bounds__(256) void k_assemble(
 // launchers
 // ---------------------------------------------------------------------------
 hipError_t launch_sweep(hipStream_t st, int nch, int W, int RC, const ProbDev *prob, int nom,
-                        const double2 *om, const int *active, double2 *chiA, double2 *lowA) {
+                        const double2 *om, const int *active, double2 *chiA, double2 *lowA,
+                        const int *pidx) {
   if (nom <= 0) return hipSuccess;
+  if (pidx != nullptr && nch != 64) return hipErrorInvalidValue;
 #define SW(N, WV)                                                                                \
   if (nch == N && W == WV) {                                                                     \
     const int grid = (nom + (64 / N) - 1) / (64 / N);                                            \
     hipLaunchKernelGGL((k_sweep<N, WV>), dim3(grid, RC), dim3(64 * WV), 0, st, prob, nom, om, active, \
-                       chiA, lowA);                                                              \
+                       chiA, lowA, pidx);                                                        \
     return hipGetLastError();                                                                    \
   }
   SW(1, 1) SW(1, 2) SW(1, 4) SW(2, 4) SW(4, 4) SW(8, 4) SW(16, 4) SW(16, 8) SW(32, 8) SW(64, 4)
@@ -975,18 +981,19 @@ hipError_t launch_sweep(hipStream_t st, int nch, int W, int RC, const ProbDev *p
 
 hipError_t launch_resonant(hipStream_t st, int nch, int RB, const ProbDev *prob, int nspec, int nom,
                            const double2 *om, const int *active, double2 *chiB, double2 *lowB,
-                           bool general) {
+                           bool general, const int *pidx) {
   if (nom <= 0) return hipSuccess;
+  if (pidx != nullptr && nch != 64) return hipErrorInvalidValue;
 #define RS(N)                                                                                  \
   if (nch == N) {                                                                              \
     const int grid = (nom + (64 / N) - 1) / (64 / N);                                          \
     hipLaunchKernelGGL((k_resonant<N, 0>), dim3(grid, nspec, RB), dim3(64), 0, st, prob, nom,  \
-                       om, active, chiB, lowB);                                                \
+                       om, active, chiB, lowB, pidx);                                          \
     hipLaunchKernelGGL((k_resonant<N, 1>), dim3(grid, nspec, RB), dim3(64), 0, st, prob, nom,  \
-                       om, active, chiB, lowB);                                                \
+                       om, active, chiB, lowB, pidx);                                          \
     if (general)                                                                               \
       hipLaunchKernelGGL((k_resonant<N, 2>), dim3(grid, nspec, RB), dim3(64), 0, st, prob, nom, \
-                         om, active, chiB, lowB);                                              \
+                         om, active, chiB, lowB, pidx);                                        \
     return hipGetLastError();                                                                  \
   }
   RS(4) RS(8) RS(16) RS(32) RS(64)
@@ -998,11 +1005,12 @@ hipError_t launch_assemble(hipStream_t st, const ProbDev *prob, int nspec, int n
                            const double2 *om, const int *active, const double2 *chiA,
                            const double2 *chiB, const double2 *lowA, const double2 *lowB,
                            const double2 *chiR, int RR, const double2 *lowR, double2 *det,
-                           double2 *chi, double2 *low, double2 *wave) {
+                           double2 *chi, double2 *low, double2 *wave, const int *pidx) {
   if (nom <= 0) return hipSuccess;
   const int per = 6 * nspec;
   const int opb = 256 / per;
   hipLaunchKernelGGL(k_assemble, dim3((nom + opb - 1) / opb), dim3(256), 0, st, prob, nom, W, RB,
-                     om, active, chiA, chiB, lowA, lowB, chiR, RR, lowR, det, chi, low, wave);
+                     om, active, chiA, chiB, lowA, lowB, chiR, RR, lowR, det, chi, low, wave, nspec,
+                     pidx);
   return hipGetLastError();
 }

@@ -39,7 +39,9 @@ EXPORTS = ["alpsg_last_error", "alpsg_create", "alpsg_destroy", "alpsg_set_k",
            "alpsg_disp_batch", "alpsg_disp_plain", "alpsg_secant_batch", "alpsg_map_search",
            "alpsg_om_scan", "alpsg_get_nmax", "alpsg_eval_count", "alpsg_last_kernel_ms",
            "alpsg_last_root_times", "alpsg_autotune", "alpsg_set_shape", "alpsg_get_shape",
-           "alpsg_last_kernel_split", "alpsg_calc_eigen"]
+           "alpsg_last_kernel_split", "alpsg_calc_eigen", "alpsg_server_create",
+           "alpsg_server_submit", "alpsg_server_poll", "alpsg_server_stats",
+           "alpsg_server_destroy"]
 
 
 class AlpsgError(RuntimeError):
@@ -110,6 +112,16 @@ def load_library(path: str = LIB_PATH):
     for name in ["alpsg_autotune", "alpsg_set_shape", "alpsg_get_shape",
            "alpsg_last_kernel_split", "alpsg_calc_eigen"]:
         getattr(L, name).restype = C.c_int
+    L.alpsg_server_create.argtypes = [C.c_double, C.c_int, C.POINTER(vp)]
+    L.alpsg_server_submit.argtypes = [vp, C.POINTER(_Desc), C.c_double, C.c_double, dp,
+                                      C.POINTER(_RootParams), C.c_longlong]
+    L.alpsg_server_poll.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_longlong), dp, ip, ip,
+                                    dp, ip]
+    L.alpsg_server_stats.argtypes = [vp, C.POINTER(C.c_longlong), dp]
+    L.alpsg_server_destroy.argtypes = [vp]
+    for name in ["alpsg_server_create", "alpsg_server_submit", "alpsg_server_poll",
+                 "alpsg_server_stats", "alpsg_server_destroy"]:
+        getattr(L, name).restype = C.c_int
     L.alpsg_last_kernel_ms.argtypes = [vp]
     L.alpsg_last_kernel_ms.restype = C.c_double
     for name in ["alpsg_create", "alpsg_destroy", "alpsg_set_k", "alpsg_disp_batch",
@@ -137,68 +149,73 @@ def _rp(r: RootParams) -> _RootParams:
     return _RootParams(r.secant_method, r.numiter, r.D_threshold, r.D_prec, r.D_gap, r.D_tol)
 
 
+def _make_desc(plasma: Plasma, use_df0: bool = False):
+    """The alpsg_plasma_desc of a plasma and the arrays it points into (keep them alive)."""
+    p = plasma
+    fo = p.fortran()
+    keep = dict(fo)
+    keep["ns"], keep["qs"], keep["ms"] = p.ns.copy(), p.qs.copy(), p.ms.copy()
+    keep["ACmethod"] = p.ACmethod.astype(np.int32)
+    keep["n_fits"] = p.n_fits.astype(np.int32)
+    keep["poly_kind"] = p.poly_kind.astype(np.int32)
+    keep["poly_order"] = p.poly_order.astype(np.int32)
+    keep["logfit"] = p.logfit.astype(np.int32)
+    keep["poly_log_max"] = p.poly_log_max.astype(np.float64)
+    keep["fit_type_f"] = np.asfortranarray(p.fit_type.astype(np.int32))
+    keep["usebM"] = p.usebM.astype(np.int32)
+    keep["bM_nmax"] = p.bM_nmax.astype(np.int32)
+    for name in ("bM_Bessel_zero", "bM_beta", "bM_alpha", "bM_pdrift"):
+        keep[name] = getattr(p, name).astype(np.float64)
+    if use_df0:
+        df0 = np.zeros((p.nspec, p.nperp - 1, p.npar - 1, 2))
+        pp, f0 = p.pp, p.f0
+        df0[..., 0] = ((f0[:, 2:, 1:-1] - f0[:, :-2, 1:-1]) /
+                       (pp[:, 2:, 1:-1, 0] - pp[:, :-2, 1:-1, 0]))
+        df0[..., 1] = ((f0[:, 1:-1, 2:] - f0[:, 1:-1, :-2]) /
+                       (pp[:, 1:-1, 2:, 1] - pp[:, 1:-1, :-2, 1]))
+        keep["df0"] = np.asfortranarray(df0)
+    d = _Desc()
+    d.nspec, d.nperp, d.npar = p.nspec, p.nperp, p.npar
+    d.pp = keep["pp"].ctypes.data
+    d.f0 = None if use_df0 else keep["f0"].ctypes.data
+    d.df0 = keep["df0"].ctypes.data if use_df0 else None
+    d.ns, d.qs, d.ms = keep["ns"].ctypes.data, keep["qs"].ctypes.data, keep["ms"].ctypes.data
+    d.vA = p.vA
+    d.kperp_norm = int(bool(p.kperp_norm))
+    d.positions_principal = int(p.positions_principal)
+    d.n_resonance_interval = int(p.n_resonance_interval)
+    d.Tlim, d.Bessel_zero = p.Tlim, p.Bessel_zero
+    d.usebM = keep["usebM"].ctypes.data
+    keep["relativistic"] = p.relativistic.astype(np.int32)
+    d.relativistic = keep["relativistic"].ctypes.data
+    if p.nspec_rel:
+        d.ngamma, d.npparbar = int(p.ngamma), int(p.npparbar)
+        d.gamma_rel, d.pparbar_rel = fo["gamma_rel"].ctypes.data, fo["pparbar_rel"].ctypes.data
+        d.f0_rel, d.df0_rel = fo["f0_rel"].ctypes.data, fo["df0_rel"].ctypes.data
+    d.bM_nmax = keep["bM_nmax"].ctypes.data
+    d.bM_Bessel_zero = keep["bM_Bessel_zero"].ctypes.data
+    d.bM_beta, d.bM_alpha = keep["bM_beta"].ctypes.data, keep["bM_alpha"].ctypes.data
+    d.bM_pdrift = keep["bM_pdrift"].ctypes.data
+    d.ACmethod, d.n_fits = keep["ACmethod"].ctypes.data, keep["n_fits"].ctypes.data
+    d.max_fits = p.max_fits
+    d.fit_type = keep["fit_type_f"].ctypes.data
+    d.perp_correction = keep["perp_correction"].ctypes.data
+    d.param_fit = keep["param_fit"].ctypes.data
+    d.param_fit_nperp = p.param_fit.shape[1] - 1
+    d.poly_kind, d.poly_order = keep["poly_kind"].ctypes.data, keep["poly_order"].ctypes.data
+    d.logfit, d.poly_log_max = keep["logfit"].ctypes.data, keep["poly_log_max"].ctypes.data
+    d.poly_fit_coeffs = keep["poly_fit_coeffs"].ctypes.data
+    d.poly_max_order = p.poly_fit_coeffs.shape[2] - 1
+    return d, keep
+
+
 class Engine:
     """One device-resident plasma (a C-ABI handle)."""
 
     def __init__(self, plasma: Plasma, use_df0: bool = False):
         L = load_library()
-        p = plasma
-        self.plasma = p
-        fo = p.fortran()
-        keep = dict(fo)
-        keep["ns"], keep["qs"], keep["ms"] = p.ns.copy(), p.qs.copy(), p.ms.copy()
-        keep["ACmethod"] = p.ACmethod.astype(np.int32)
-        keep["n_fits"] = p.n_fits.astype(np.int32)
-        keep["poly_kind"] = p.poly_kind.astype(np.int32)
-        keep["poly_order"] = p.poly_order.astype(np.int32)
-        keep["logfit"] = p.logfit.astype(np.int32)
-        keep["poly_log_max"] = p.poly_log_max.astype(np.float64)
-        keep["fit_type_f"] = np.asfortranarray(p.fit_type.astype(np.int32))
-        keep["usebM"] = p.usebM.astype(np.int32)
-        keep["bM_nmax"] = p.bM_nmax.astype(np.int32)
-        for name in ("bM_Bessel_zero", "bM_beta", "bM_alpha", "bM_pdrift"):
-            keep[name] = getattr(p, name).astype(np.float64)
-        if use_df0:
-            df0 = np.zeros((p.nspec, p.nperp - 1, p.npar - 1, 2))
-            pp, f0 = p.pp, p.f0
-            df0[..., 0] = ((f0[:, 2:, 1:-1] - f0[:, :-2, 1:-1]) /
-                           (pp[:, 2:, 1:-1, 0] - pp[:, :-2, 1:-1, 0]))
-            df0[..., 1] = ((f0[:, 1:-1, 2:] - f0[:, 1:-1, :-2]) /
-                           (pp[:, 1:-1, 2:, 1] - pp[:, 1:-1, :-2, 1]))
-            keep["df0"] = np.asfortranarray(df0)
-        self._keep = keep
-        d = _Desc()
-        d.nspec, d.nperp, d.npar = p.nspec, p.nperp, p.npar
-        d.pp = keep["pp"].ctypes.data
-        d.f0 = None if use_df0 else keep["f0"].ctypes.data
-        d.df0 = keep["df0"].ctypes.data if use_df0 else None
-        d.ns, d.qs, d.ms = keep["ns"].ctypes.data, keep["qs"].ctypes.data, keep["ms"].ctypes.data
-        d.vA = p.vA
-        d.kperp_norm = int(bool(p.kperp_norm))
-        d.positions_principal = int(p.positions_principal)
-        d.n_resonance_interval = int(p.n_resonance_interval)
-        d.Tlim, d.Bessel_zero = p.Tlim, p.Bessel_zero
-        d.usebM = keep["usebM"].ctypes.data
-        keep["relativistic"] = p.relativistic.astype(np.int32)
-        d.relativistic = keep["relativistic"].ctypes.data
-        if p.nspec_rel:
-            d.ngamma, d.npparbar = int(p.ngamma), int(p.npparbar)
-            d.gamma_rel, d.pparbar_rel = fo["gamma_rel"].ctypes.data, fo["pparbar_rel"].ctypes.data
-            d.f0_rel, d.df0_rel = fo["f0_rel"].ctypes.data, fo["df0_rel"].ctypes.data
-        d.bM_nmax = keep["bM_nmax"].ctypes.data
-        d.bM_Bessel_zero = keep["bM_Bessel_zero"].ctypes.data
-        d.bM_beta, d.bM_alpha = keep["bM_beta"].ctypes.data, keep["bM_alpha"].ctypes.data
-        d.bM_pdrift = keep["bM_pdrift"].ctypes.data
-        d.ACmethod, d.n_fits = keep["ACmethod"].ctypes.data, keep["n_fits"].ctypes.data
-        d.max_fits = p.max_fits
-        d.fit_type = keep["fit_type_f"].ctypes.data
-        d.perp_correction = keep["perp_correction"].ctypes.data
-        d.param_fit = keep["param_fit"].ctypes.data
-        d.param_fit_nperp = p.param_fit.shape[1] - 1
-        d.poly_kind, d.poly_order = keep["poly_kind"].ctypes.data, keep["poly_order"].ctypes.data
-        d.logfit, d.poly_log_max = keep["logfit"].ctypes.data, keep["poly_log_max"].ctypes.data
-        d.poly_fit_coeffs = keep["poly_fit_coeffs"].ctypes.data
-        d.poly_max_order = p.poly_fit_coeffs.shape[2] - 1
+        self.plasma = plasma
+        d, self._keep = _make_desc(plasma, use_df0)
         h = C.c_void_p()
         _check(L.alpsg_create(C.byref(d), C.byref(h)))
         self._h = h
@@ -380,3 +397,88 @@ class Engine:
                                            diff2, n_out, n_res, nr, _dp(roots), _dp(kp), _dp(kz),
                                            _dp(out)))
         return kp, kz, out.reshape(n_out + 1, nr)
+
+
+class RootServer:
+    """Continuous-batching root server (alpsg_server_* in include/alpsg.h).
+
+    Each request is one secant_osc solve (ALPS_fns.f90:1919-2101) of its own
+    plasma at its own wavevector.  ``submit`` queues it and returns at once;
+    ``poll`` returns finished solves as ``(tag, root, iters, status,
+    latency_ms)`` tuples.  The service thread inside the library batches every
+    admitted solve into shared device launches.
+    """
+
+    def __init__(self, hbm_bytes: float = 8e9, max_slots: int = 4096):
+        L = load_library()
+        h = C.c_void_p()
+        _check(L.alpsg_server_create(float(hbm_bytes), int(max_slots), C.byref(h)))
+        self._h = h
+        self._descs = {}
+
+    def _desc(self, plasma: Plasma):
+        key = id(plasma)
+        hit = self._descs.get(key)
+        if hit is None or hit[0] is not plasma:
+            d, keep = _make_desc(plasma)
+            hit = (plasma, d, keep)
+            self._descs[key] = hit
+        return hit[1]
+
+    def submit(self, plasma: Plasma, kperp: float, kpar: float, guess: complex,
+               roots: RootParams | None = None, tag: int = 0):
+        rp = roots or RootParams(secant_method=2)
+        om = np.array([complex(guess).real, complex(guess).imag], dtype=np.float64)
+        _check(load_library().alpsg_server_submit(
+            self._h, C.byref(self._desc(plasma)), float(kperp), float(kpar), _dp(om),
+            C.byref(_rp(rp)), int(tag)))
+
+    def poll(self, max_results: int = 1024, wait: bool = True):
+        n = int(max_results)
+        tags = np.zeros(n, dtype=np.int64)
+        roots = np.zeros(2 * n)
+        iters = np.zeros(n, dtype=np.int32)
+        status = np.zeros(n, dtype=np.int32)
+        lat = np.zeros(n)
+        got = C.c_int(0)
+        _check(load_library().alpsg_server_poll(
+            self._h, n, int(bool(wait)), tags.ctypes.data_as(C.POINTER(C.c_longlong)), _dp(roots),
+            _ip(iters), _ip(status), _dp(lat), C.byref(got)))
+        k = got.value
+        return [(int(tags[i]), complex(roots[2 * i], roots[2 * i + 1]), int(iters[i]),
+                 int(status[i]), float(lat[i])) for i in range(k)]
+
+    def drain(self, count: int):
+        """Poll until ``count`` results have arrived; returns them keyed by tag."""
+        out = {}
+        while len(out) < count:
+            got = self.poll(max(count - len(out), 1), wait=True)
+            if not got:
+                st = self.stats()
+                if st["finished"] >= st["submitted"]:
+                    raise RuntimeError(f"drain({count}): only {len(out)} results outstanding")
+            for r in got:
+                out[r[0]] = r
+        return out
+
+    def stats(self):
+        c = np.zeros(8, dtype=np.int64)
+        busy = C.c_double(0.0)
+        _check(load_library().alpsg_server_stats(
+            self._h, c.ctypes.data_as(C.POINTER(C.c_longlong)), C.byref(busy)))
+        keys = ("submitted", "finished", "passes", "evals", "admitted", "waiting",
+                "arena_bytes", "arena_free_bytes")
+        out = {k: int(v) for k, v in zip(keys, c)}
+        out["busy_ms"] = busy.value
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().alpsg_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -99,6 +99,139 @@ def cpu_baseline(plasma, om_grid, cal_gpu, n_sample, threads):
                 wall_s=dt), err
 
 
+# -- served root solves ------------------------------------------------------
+SERVE_GRIDS = [(32, 64), (64, 128), (128, 256), (256, 512)]
+
+
+def serve_variant(alph, nperp, npar, rp):
+    """Protons (T_perp/T_par = alph), electrons, alphas at 5 % density drifting
+    at 1 v_A along B (p = m v = 4), quasi-neutral, beta_par = 1, v_A/c = 1e-4."""
+    from alps_amd.plasma import bimaxwellian_plasma
+    return bimaxwellian_plasma(ms=[1.0, M_E, 4.0], qs=[1.0, -1.0, 2.0], ns=[1.0, 1.1, 0.05],
+                               taus=[1.0, 1.0, 1.0], alphs=[alph, 1.0, 1.0], ps=[0.0, 0.0, 4.0],
+                               beta=1.0, vA=1e-4, nperp=nperp, npar=npar, maxP=6.0,
+                               positions_principal=3, n_resonance_interval=100, Tlim=0.01,
+                               Bessel_zero=1e-45, roots=rp)
+
+
+def serve_requests(n, n_variants, seed=7):
+    """n requests over a pool of plasma variants (anisotropy uniform in [0.3, 3],
+    grids cycling 32x64 .. 256x512): (variant, kperp, kpar, guess)."""
+    from alps_amd.plasma import RootParams
+    rng = np.random.default_rng(seed)
+    rp = RootParams(secant_method=2, numiter=50, D_threshold=1e-12, D_prec=1e-5, D_gap=1e-5)
+    pool = []
+    for v in range(n_variants):
+        nperp, npar = SERVE_GRIDS[v % len(SERVE_GRIDS)]
+        pool.append(serve_variant(float(rng.uniform(0.3, 3.0)), nperp, npar, rp))
+    lu = lambda a, b, size: np.exp(rng.uniform(np.log(a), np.log(b), size))
+    kperp, kpar = lu(0.01, 3.0, n), lu(0.01, 3.0, n)
+    guess = lu(0.01, 3.0, n) + 1j * rng.uniform(-0.1, 0.01, n)
+    var = rng.integers(0, n_variants, n)
+    return pool, [(int(var[i]), float(kperp[i]), float(kpar[i]), complex(guess[i]))
+                  for i in range(n)]
+
+
+def run_serve(args):
+    """Poisson stream of root requests through the continuous-batching server;
+    the same requests (a sample) on the plain one-root-per-launch path and on
+    the CPU restatement."""
+    from alps_amd.engine import Engine, RootServer
+    pool, reqs = serve_requests(args.requests, args.variants)
+    rng = np.random.default_rng(11)
+    S = RootServer(hbm_bytes=args.hbm_gb * 1e9, max_slots=args.slots)
+    # warm-up: one request of each grid size
+    for v in range(min(len(pool), len(SERVE_GRIDS))):
+        S.submit(pool[v], 0.5, 0.5, 0.5 - 0.01j, pool[v].roots, tag=-1 - v)
+    S.drain(min(len(pool), len(SERVE_GRIDS)))
+    st0 = S.stats()
+    gaps = rng.exponential(1.0 / args.rate, len(reqs))
+    t_arr = np.cumsum(gaps)
+    t0 = time.perf_counter()
+    for i, (v, kp, kz, g) in enumerate(reqs):
+        dt = t_arr[i] - (time.perf_counter() - t0)
+        if dt > 0:
+            time.sleep(dt)
+        S.submit(pool[v], kp, kz, g, pool[v].roots, tag=i)
+    t_sub = time.perf_counter() - t0
+    got = S.drain(len(reqs))
+    wall = time.perf_counter() - t0
+    st = S.stats()
+    S.close()
+    lat = np.array([got[i][4] for i in range(len(reqs))])
+    status = np.array([got[i][3] for i in range(len(reqs))])
+    iters = np.array([got[i][2] for i in range(len(reqs))])
+    served = np.array([got[i][1] for i in range(len(reqs))])
+    evals = st["evals"] - st0["evals"]
+    passes = st["passes"] - st0["passes"]
+    # plain path: one Engine per variant, set_k and secant_osc with one root per launch
+    engines = {}
+    idx = np.linspace(0, len(reqs) - 1, args.plain_requests).astype(int)
+    for i in idx[:2]:
+        v = reqs[i][0]
+        engines.setdefault(v, Engine(pool[v]))
+    plain_t = 0.0
+    dev = 0.0
+    conv_plain = 0
+    for i in idx:
+        v, kp, kz, g = reqs[i]
+        E = engines.get(v)
+        if E is None:
+            E = engines[v] = Engine(pool[v])
+        tp = time.perf_counter()
+        E.set_k(kp, kz)
+        r, it = E.secant_osc(g)
+        plain_t += time.perf_counter() - tp
+        if status[i] == 0:
+            dev = max(dev, abs(r - served[i]) / max(abs(r), 1e-300))
+        conv_plain += 1
+    out = {
+        "requests": len(reqs),
+        "plasma_variants": len(pool),
+        "arrival_rate_per_s": args.rate,
+        "submit_s": t_sub,
+        "wall_s": wall,
+        "roots_per_sec": int(np.sum(status == 0)) / wall,
+        "converged": int(np.sum((status == 0) & (iters < pool[0].roots.numiter))),
+        "latency_p50_ms": float(np.percentile(lat, 50)),
+        "latency_p99_ms": float(np.percentile(lat, 99)),
+        "status_ok": int(np.sum(status == 0)),
+        "mean_iterations": float(iters.mean()),
+        "passes": passes,
+        "disp_evals": evals,
+        "evals_per_pass": evals / max(passes, 1),
+        "busy_ms": st["busy_ms"] - st0["busy_ms"],
+        "hbm_arena_gb": args.hbm_gb,
+        "slots": args.slots,
+        "plain_roots_per_sec": len(idx) / plain_t,
+        "plain_sample": int(len(idx)),
+        "max_rel_dev_served_vs_plain": dev,
+    }
+    out["speedup_vs_plain"] = out["roots_per_sec"] / out["plain_roots_per_sec"]
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from oracle import Oracle
+        thr = min(args.cpu_threads, os.cpu_count() or 1)
+        n_cpu, t_cpu = 0, 0.0
+        for i in idx:
+            v, kp, kz, g = reqs[i]
+            tc = time.perf_counter()
+            O = Oracle(pool[v], n_threads=thr)
+            O.set_k(kp, kz)
+            O.secant_osc(g)
+            t_cpu += time.perf_counter() - tc
+            n_cpu += 1
+            if t_cpu > args.cpu_seconds:
+                break
+        out["cpu_baseline"] = {"value": n_cpu / t_cpu, "unit": "roots/s", "cores": thr,
+                               "kind": "port",
+                               "sample": f"{n_cpu} requests of the stream (strided), oracle/"
+                                         f"liboracle.so with {thr} OpenMP threads",
+                               "wall_s": t_cpu}
+        out["speedup_vs_cpu"] = out["roots_per_sec"] / out["cpu_baseline"]["value"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +249,16 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (torch) and exit: the "
                          "known-size kernel that scales the profiler's FETCH_SIZE/WRITE_SIZE")
+    ap.add_argument("--workload", choices=["map", "serve"], default="map",
+                    help="map: the batched map search (default); serve: the Poisson stream "
+                         "of independent root requests through the root server")
+    ap.add_argument("--requests", type=int, default=4096)
+    ap.add_argument("--variants", type=int, default=64, help="plasma variants in the request pool")
+    ap.add_argument("--rate", type=float, default=4000.0, help="request arrivals per second")
+    ap.add_argument("--slots", type=int, default=4096)
+    ap.add_argument("--hbm-gb", type=float, default=32.0)
+    ap.add_argument("--plain-requests", type=int, default=32)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -132,6 +275,11 @@ def main():
             c.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": nb, "copies": 3}))
+        return
+
+    if args.workload == "serve":
+        sv = run_serve(args)
+        print(json.dumps({"workload": "serve", **sv}))
         return
 
     from alps_amd.engine import Engine

@@ -186,6 +186,36 @@ double alpsg_last_kernel_ms(alpsg_handle h);
    assembly (k_assemble). */
 int alpsg_last_kernel_split(alpsg_handle h, double *ms3);
 
+/* Root server: continuous batching of independent secant_osc solves
+   (ALPS_fns.f90:1919-2101), each with its own plasma, wavevector and initial
+   frequency -- the solve the reference runs once per root and per scan step
+   (refine_guess, ALPS_fns.f90:3791-3857; om_scan, :2198-2600) -- served from
+   one device.  A service thread admits waiting requests into a fixed HBM
+   arena of hbm_bytes (first fit by each request's footprint: its momentum
+   tables, then its per-k tables once nmax is known), at most max_slots solves
+   at a time, and advances every admitted solve by one disp() per pass, with
+   the solves of uneven grids and nmax in shared launches.  A request's result
+   does not depend on the other requests in flight.
+
+   Served species: tabulated f0, non-relativistic, not use_bM.  rp must ask for
+   secant_osc (secant_method 2).  The descriptor's arrays are copied before
+   alpsg_server_submit returns.  alpsg_server_poll returns finished solves
+   (waits for one when wait != 0 and any are outstanding): tags [max_results],
+   roots [2*max_results], iters, status (ALPSG_OK, or ALPSG_E_ARG for a request
+   that cannot fit the arena or whose Bessel cutoff is not reached) and
+   latency_ms (submit to result); any may be NULL.  alpsg_server_stats gives
+   counts[8] = {submitted, finished, passes, disp evaluations, admitted,
+   waiting, arena bytes, free arena bytes} and the summed pass time.
+   No counterpart in the reference, which solves one root at a time. */
+typedef struct alpsg_server alpsg_server;
+int alpsg_server_create(double hbm_bytes, int max_slots, alpsg_server **out);
+int alpsg_server_submit(alpsg_server *s, const alpsg_plasma_desc *desc, double kperp, double kpar,
+                        const double *om_guess, const alpsg_root_params *rp, long long tag);
+int alpsg_server_poll(alpsg_server *s, int max_results, int wait, long long *tags, double *roots,
+                      int *iters, int *status, double *latency_ms, int *n_out);
+int alpsg_server_stats(alpsg_server *s, long long *counts, double *busy_ms);
+int alpsg_server_destroy(alpsg_server *s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,114 @@
+"""Root server (alpsg_server_*): a served secant_osc solve matches the oracle's
+to |dw/w| <= 1e-10, and its result is the same, bit for bit, when it is solved
+alone on the server and when it shares passes with other solves of other
+plasmas, grids and wavevectors, under arena pressure or not."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+import cases
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+ROOT_TOL = 1e-10
+
+
+def _tight(p):
+    return dataclasses.replace(p, roots=dataclasses.replace(p.roots, secant_method=2,
+                                                            D_threshold=1e-14))
+
+
+def _requests():
+    """(plasma, kperp, kpar, guess): 1-3 species, Maxwell/kappa fits and
+    Chebyshev continuation, three grid sizes, several wavevectors."""
+    a = _tight(cases.small_bimax())
+    b = _tight(cases.small_bimax(nperp=16, npar=40, drift=0.1))
+    c = _tight(cases.three_species())
+    d = _tight(cases.kappa_case())
+    e = _tight(cases.poly_case())
+    return [
+        (a, 0.3, 0.3, 0.29 - 0.001j),
+        (a, 0.3, 0.3, 0.5 - 0.05j),
+        (a, 0.5, 0.2, 0.25 - 0.01j),
+        (b, 0.3, 0.3, 0.3 - 0.002j),
+        (c, 0.2, 0.4, 0.4 - 0.01j),
+        (d, 0.3, 0.3, 0.29 - 0.001j),
+        (e, 0.3, 0.3, 0.29 - 0.001j),
+        (a, 0.1, 0.5, 0.5 - 0.001j),
+    ]
+
+
+def _server(hbm=2e9, slots=256):
+    from alps_amd.engine import RootServer
+    return RootServer(hbm_bytes=hbm, max_slots=slots)
+
+
+def _alone(reqs):
+    out = []
+    S = _server()
+    for i, (p, kp, kz, g) in enumerate(reqs):
+        S.submit(p, kp, kz, g, p.roots, tag=i)
+        (r,) = S.drain(1).values()
+        assert r[3] == 0, r
+        out.append(r)
+    S.close()
+    return out
+
+
+def test_served_roots_match_oracle_and_are_mix_independent():
+    reqs = _requests()
+    alone = _alone(reqs)
+    for (p, kp, kz, g), r in zip(reqs, alone):
+        O = Oracle(p)
+        O.set_k(kp, kz)
+        o_root, _ = O.secant_osc(g)
+        assert abs(r[1] - o_root) <= ROOT_TOL * abs(o_root), (kp, kz, g, r[1], o_root)
+    # every request three times over, interleaved, all in flight together
+    S = _server()
+    order = [(rep, i) for rep in range(3) for i in range(len(reqs))]
+    for rep, i in order:
+        p, kp, kz, g = reqs[i]
+        S.submit(p, kp, kz, g, p.roots, tag=rep * 100 + i)
+    got = S.drain(len(order))
+    st = S.stats()
+    S.close()
+    for rep, i in order:
+        r = got[rep * 100 + i]
+        assert r[3] == 0
+        assert r[1] == alone[i][1] and r[2] == alone[i][2], (rep, i, r, alone[i])
+    assert st["finished"] == len(order) and st["admitted"] == 0
+    assert st["arena_free_bytes"] == st["arena_bytes"]
+    # passes were shared: fewer passes than disp evaluations
+    assert st["passes"] < st["evals"]
+
+
+def test_small_arena_queues_and_rejects_what_cannot_fit():
+    reqs = _requests()
+    alone = _alone(reqs)
+    # an arena that holds only a few requests at once: the rest wait for space
+    S = _server(hbm=1 << 20, slots=64)
+    for i, (p, kp, kz, g) in enumerate(reqs):
+        S.submit(p, kp, kz, g, p.roots, tag=i)
+    big = _tight(cases.config1())
+    S.submit(big, 0.5, 0.5, 0.5 - 0.01j, big.roots, tag=999)  # about 4 MB of tables
+    got = S.drain(len(reqs) + 1)
+    st = S.stats()
+    S.close()
+    for i in range(len(reqs)):
+        assert got[i][3] == 0 and got[i][1] == alone[i][1], (i, got[i], alone[i])
+    assert got[999][3] == 1  # ALPSG_E_ARG: does not fit the arena
+    assert st["arena_free_bytes"] == st["arena_bytes"]
+
+
+def test_slot_limit_one_serialises_without_changing_results():
+    reqs = _requests()[:4]
+    alone = _alone(reqs)
+    S = _server(slots=1)
+    for i, (p, kp, kz, g) in enumerate(reqs):
+        S.submit(p, kp, kz, g, p.roots, tag=i)
+    got = S.drain(len(reqs))
+    S.close()
+    for i in range(len(reqs)):
+        assert got[i][1] == alone[i][1]
