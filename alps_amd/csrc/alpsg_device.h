@@ -50,6 +50,16 @@ struct SpecDev {
   double rel_dg, rel_dpb;
 };
 
+// One (request, species) job of the server's batched table launches: the
+// Bessel cutoff search (bmax, ntry entries) or the per-k tables (bes, atab,
+// nodes for n = 0..nmax).
+struct TabJob {
+  const double *pperp, *ppar, *df1, *df2;
+  double *bes, *atab, *nodes, *bmax;
+  double kperp, kpar, q, m;
+  int nperp, npar, nmax, ntry;
+};
+
 struct ProbDev {
   int nspec, kperp_norm, P, NRI;
   double kperp, kpar, vA, Tlim;

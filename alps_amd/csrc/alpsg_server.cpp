@@ -1,26 +1,34 @@
 // Continuous-batching root server.
 //
-// Independent root solves -- each with its own plasma tables, grid size,
-// wavevector, initial frequency and root controls -- arrive with
-// alpsg_server_submit.  A service thread admits them into a fixed HBM arena as
-// its space and the problem slots allow (first fit, by each request's
-// footprint), builds their tables on the device, and advances every admitted
-// solve by one secant_osc step per device pass: the pending frequencies of all
-// active solves go into one launch of k_sweep / k_resonant / k_assemble in
-// their one-omega-per-workgroup form, where each workgroup reads its own
-// solve's problem descriptor (pidx).  Solves of uneven nmax and nperp x npar
-// therefore share launches.  A finished solve frees its arena blocks and slot
-// at once, and the next waiting request takes them in the following pass.
+// Independent root solves -- each with its own plasma, grid size, wavevector,
+// initial frequency and root controls -- arrive with alpsg_server_submit.  A
+// service thread admits them into a fixed HBM arena as its space and the
+// problem slots allow (first come first served, by each request's footprint),
+// builds their tables on the device, and advances every admitted solve by one
+// secant_osc step per device pass: the pending frequencies of all active solves
+// go into one launch of k_sweep / k_resonant / k_assemble in their
+// one-omega-per-workgroup form, where each workgroup reads its own solve's
+// problem descriptor (pidx).  Solves of uneven nmax and nperp x npar therefore
+// share launches.  A finished solve frees its arena blocks and slot at once,
+// and waiting requests take them in the following pass.
 //
-// Admission is pipelined with the passes: pass i uploads a new request's
-// tables and evaluates max |J_n| over its pperp grid (determine_nmax,
-// ALPS_fns.f90:3971-4075); after pass i's synchronisation the host picks nmax,
-// enqueues the per-k tables, and the solve joins pass i+1.  The launch shape of
-// a pass is fixed, so a solve's arithmetic does not depend on the other solves
-// in the pass: its result is the same alone and in any mix.
+// A plasma (the omega-independent block: f0, df0, fit tables) is uploaded once
+// per residency and shared by every request that names it; registered plasmas
+// stay resident between requests and are evicted, least recently used first,
+// when the arena needs their space.
+//
+// Admission is pipelined with the passes and batched: in pass i every newly
+// admitted request adds its max|J_n| jobs (determine_nmax,
+// ALPS_fns.f90:3971-4075) to one job-list launch; after pass i's
+// synchronisation the host picks nmax, and one job-list launch per table kind
+// builds the per-k tables of all of them for pass i+1.  The launch shape of a
+// pass is fixed and job-list launches run the single-problem device code per
+// job, so a solve's arithmetic does not depend on the other solves: its result
+// is the same alone and in any mix.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -38,19 +46,25 @@
 #include "alpsg_device.h"
 #include "alpsg_osc.h"
 
-// shared with alpsg_api.cpp
+// shared with alpsg_api.cpp / alpsg_tables.hip / alpsg_disp.hip
 int alpsg_internal_fail(int code, const std::string &msg);
 hipError_t launch_derivative(hipStream_t st, int nspec, int nperp, int npar, const double *f0,
                              const double *pperp, const double *ppar, double *df1, double *df2);
 hipError_t launch_int_ee(hipStream_t st, int nspec, int nperp, int npar, const double *pperp,
                          const double *ppar, const double *df1, const double *df2,
                          const double *qs, const double *ms, double *out);
-hipError_t launch_bessel_max(hipStream_t st, int nspec, int nperp, int ntry, double kperp,
-                             const double *pperp, const double *qs, const int *mtab, double *out);
-hipError_t launch_species_k(hipStream_t st, int nperp, int npar, int nmax, double kperp,
-                            double kpar, double q, double m, const double *pperp,
-                            const double *ppar, const double *df1, const double *df2,
-                            const int *mtab, double *bes, double *atab, double *nodes);
+int jobs_blocks_bessel_max(const TabJob &J);
+int jobs_blocks_bessel(const TabJob &J);
+int jobs_blocks_atab(const TabJob &J);
+int jobs_blocks_moments(const TabJob &J);
+hipError_t launch_bessel_max_jobs(hipStream_t st, const TabJob *jobs, const int *start, int njobs,
+                                  int total, const int *mtab);
+hipError_t launch_species_k_jobs(hipStream_t st, const TabJob *jobs, int njobs,
+                                 const int *start_bes, int total_bes, const int *start_atab,
+                                 int total_atab, const int *start_mom, int total_mom,
+                                 int max_nperp, const int *mtab);
+hipError_t launch_scatter_probs(hipStream_t st, const ProbDev *src, const int *slots, int n,
+                                ProbDev *dst);
 hipError_t launch_sweep(hipStream_t st, int nch, int W, int RC, const ProbDev *prob, int nom,
                         const double2 *om, const int *active, double2 *chiA, double2 *lowA,
                         const int *pidx);
@@ -80,37 +94,43 @@ constexpr int MTAB_S = 4096;
 // k_resonant<64, *> over 8 harmonic blocks
 constexpr int SW_W = 8, SW_RC = 4, RS_RB = 8;
 constexpr size_t ALIGN = 256;
+constexpr int MAX_JOBS = 8192;               // (request, species) jobs per table launch
+constexpr size_t STAGE_CAP = (size_t)1 << 20;  // doubles read back per pass (int_ee, max|J_n|)
 
 double now_ms() {
   using namespace std::chrono;
   return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
 
+size_t align_up(size_t b) { return (b + ALIGN - 1) / ALIGN * ALIGN; }
+
 // first-fit allocator over one device allocation
 struct Arena {
   char *base = nullptr;
-  size_t cap = 0;
+  size_t cap = 0, used = 0;
   std::map<size_t, size_t> free_;  // offset -> size
   void init(char *b, size_t c) {
     base = b;
     cap = c;
+    used = 0;
     free_.clear();
     free_[0] = c;
   }
-  // returns offset or SIZE_MAX
-  size_t alloc(size_t sz) {
-    sz = (sz + ALIGN - 1) / ALIGN * ALIGN;
+  size_t alloc(size_t sz) {  // offset, or SIZE_MAX when no free block is large enough
+    sz = align_up(sz);
     for (auto it = free_.begin(); it != free_.end(); ++it)
       if (it->second >= sz) {
         const size_t off = it->first, rest = it->second - sz;
         free_.erase(it);
         if (rest) free_[off + sz] = rest;
+        used += sz;
         return off;
       }
     return SIZE_MAX;
   }
   void release(size_t off, size_t sz) {
-    sz = (sz + ALIGN - 1) / ALIGN * ALIGN;
+    sz = align_up(sz);
+    used -= sz;
     auto it = free_.emplace(off, sz).first;
     auto nx = std::next(it);
     if (nx != free_.end() && it->first + it->second == nx->first) {
@@ -125,11 +145,6 @@ struct Arena {
       }
     }
   }
-  size_t free_bytes() const {
-    size_t t = 0;
-    for (auto &f : free_) t += f.second;
-    return t;
-  }
 };
 
 // bump layout of several arrays inside one arena block
@@ -137,18 +152,14 @@ struct Layout {
   size_t size = 0;
   size_t add(size_t bytes) {
     const size_t o = size;
-    size += (bytes + ALIGN - 1) / ALIGN * ALIGN;
+    size += align_up(bytes);
     return o;
   }
 };
 
-struct Request {
-  long long tag = 0;
-  double kperp = 0, kpar = 0;
-  cplx guess;
-  alpsg_root_params rp{};
-  double t_submit = 0;
-  // host copy of the plasma (species-major, C layout)
+// A plasma: host copy (immutable once built) and its device block.
+struct Plasma {
+  int id = 0;
   int nspec = 0, nperp = 0, npar = 0, P = 3, NRI = 100, kperp_norm = 1;
   double vA = 0, Tlim = 0.01, Bessel_zero = 1e-45;
   std::vector<double> qs, ms, ns, pperp, ppar;
@@ -157,16 +168,33 @@ struct Request {
   std::vector<size_t> fit_off;
   int maxorder = 0;
   bool general = false;  // a species needs the general Landau part
-  // omega-independent block: host image of the uploaded prefix, then device-only arrays
+  // device block: the uploaded prefix (host image img), then df1, df2
   std::vector<char> img;
   size_t o_pperp = 0, o_ppar = 0, o_f0 = 0, o_qs = 0, o_ms = 0, o_fit = 0, o_poly = 0, n_up = 0;
-  size_t o_df1 = 0, o_df2 = 0, o_ee = 0, o_bmax = 0, sz_s = 0;
+  size_t o_df1 = 0, o_df2 = 0, sz = 0;
+  std::atomic<bool> registered{false};
+  // service-thread state
+  size_t off = SIZE_MAX;  // SIZE_MAX: not resident
+  int ee_state = 0;       // 0 absent, 1 int_ee in this pass's readback, 2 int_ee on the host
+  size_t ee_stage = 0;
+  std::vector<double> int_ee;
+  int users = 0;  // admitted requests reading the block
+  long long last_use = 0;
+  bool listed = false;  // in the server's resident list
+};
+
+struct Request {
+  long long tag = 0;
+  double kperp = 0, kpar = 0;
+  cplx guess;
+  alpsg_root_params rp{};
+  double t_submit = 0;
+  std::shared_ptr<Plasma> pl;
   int ntry = 0;
-  // device state
   int slot = -1, state = 0;  // 0 waiting, 1 sizing, 2 active
-  size_t off_s = SIZE_MAX, off_k = SIZE_MAX, sz_k = 0;
-  int stage = -1;
-  std::vector<double> sized;  // int_ee[nspec] then max|J_n| [nspec][ntry], read back
+  size_t stage = 0;          // offset of its max|J_n| rows in the pass readback
+  std::vector<double> bmax;  // [nspec][ntry]
+  size_t off_k = SIZE_MAX, sz_k = 0;
   int nmax[ALPSG_MAXSPEC] = {0};
   OscMachine m;
 };
@@ -178,16 +206,34 @@ struct Result {
   double latency_ms;
 };
 
+// one job-list launch's host and device buffers
+struct JobBuf {
+  TabJob *jobs_h = nullptr, *jobs_d = nullptr;
+  int *start_h = nullptr, *start_d = nullptr;  // 3 arrays of MAX_JOBS + 1
+  hipError_t init() {
+    hipError_t e = hipHostMalloc((void **)&jobs_h, sizeof(TabJob) * MAX_JOBS, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc((void **)&jobs_d, sizeof(TabJob) * MAX_JOBS);
+    if (e == hipSuccess)
+      e = hipHostMalloc((void **)&start_h, sizeof(int) * 3 * (MAX_JOBS + 1), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc((void **)&start_d, sizeof(int) * 3 * (MAX_JOBS + 1));
+    return e;
+  }
+  void free_all() {
+    if (jobs_h) (void)hipHostFree(jobs_h);
+    if (start_h) (void)hipHostFree(start_h);
+    if (jobs_d) (void)hipFree(jobs_d);
+    if (start_d) (void)hipFree(start_d);
+  }
+};
+
 }  // namespace
 
 struct alpsg_server {
   hipStream_t st = nullptr;
   char *arena_mem = nullptr;
   Arena arena;
-  int max_slots = 0, max_batch = 0;
-  static constexpr int max_admit = 64;  // admissions started per pass
+  int max_slots = 0;
   ProbDev *probs_d = nullptr;
-  std::vector<ProbDev> probs_h;
   std::vector<int> free_slots;
   int *mtab_d = nullptr;
   // pass buffers
@@ -195,39 +241,44 @@ struct alpsg_server {
   int *pidx_d = nullptr;
   double2 *om_h = nullptr, *det_h = nullptr;  // pinned
   int *pidx_h = nullptr;                      // pinned
-  double *stage_h = nullptr;                  // pinned: one row per admission of a pass
-  size_t stage_row = 0;                       // doubles per row
-  // requests
+  double *stage_h = nullptr, *stage_d = nullptr;  // per-pass readback of int_ee and max|J_n|
+  JobBuf jb1, jb2;                                // stage-1 and stage-2 job lists
+  ProbDev *newp_h = nullptr, *newp_d = nullptr;   // descriptors activated by a pass
+  int *newslot_h = nullptr, *newslot_d = nullptr;
+  // plasmas and requests
   std::mutex mu;
   std::condition_variable cv, cv_done;
-  std::deque<std::unique_ptr<Request>> queue;  // submitted, not admitted
-  std::vector<std::unique_ptr<Request>> live;  // admitted (sizing or active)
+  std::map<int, std::shared_ptr<Plasma>> registry;  // registered plasmas by id
+  int next_id = 1;
+  std::vector<std::shared_ptr<Plasma>> resident;     // plasmas with a device block
+  std::deque<std::unique_ptr<Request>> queue;        // submitted, not admitted
+  std::vector<std::unique_ptr<Request>> live;        // admitted (sizing or active)
   std::deque<Result> results;
   long long submitted = 0, finished = 0;
-  bool stop = false, failed = false;
+  bool stop = false, failed = false, gc = false;
   std::string fail_msg;
   std::thread th;
   // statistics
-  long long passes = 0, evals = 0;
-  double busy_ms = 0;
+  long long passes = 0, evals = 0, uploads = 0, evictions = 0;
+  // host time (ms): whole passes, admission work, disp launch enqueue, waits on the device
+  double t_pass = 0, t_admit = 0, t_launch = 0, t_wait = 0;
 };
 
 namespace {
 
-constexpr int MAXSPEC_S = ALPSG_MAXSPEC;
-constexpr size_t STAGE_ROW = (size_t)MAXSPEC_S * (MTAB_S + 1);
-
-// Host copy of a served request's plasma (tabulated, non-relativistic species
-// with fits 1/2/6 or Chebyshev continuation), the checks of alpsg_create, and
-// the host image of its omega-independent arena block.
-int prepare_request(const alpsg_plasma_desc *d, Request &r) {
+// Host copy of a plasma (tabulated, non-relativistic species with fits 1/2/6 or
+// Chebyshev continuation), the checks of alpsg_create, and the host image of
+// its device block.
+int prepare_plasma(const alpsg_plasma_desc *d, Plasma &r) {
+  if (!d) return alpsg_internal_fail(ALPSG_E_ARG, "null plasma description");
   const int nspec = d->nspec, nperp = d->nperp, npar = d->npar;
   if (nspec < 1 || nspec > ALPSG_MAXSPEC) return alpsg_internal_fail(ALPSG_E_ARG, "nspec out of range 1..8");
   if (!d->pp || !d->f0 || !d->ns || !d->qs || !d->ms || !d->ACmethod || !d->n_fits)
-    return alpsg_internal_fail(ALPSG_E_ARG, "served requests need pp, f0, ns, qs, ms, ACmethod, n_fits");
+    return alpsg_internal_fail(ALPSG_E_ARG, "served plasmas need pp, f0, ns, qs, ms, ACmethod, n_fits");
   const int P = d->positions_principal;
   if (P < 1 || P > 16) return alpsg_internal_fail(ALPSG_E_UNSUPPORTED, "positions_principal must be in 1..16");
   if (nperp < 4 || npar < 2 * P + 8) return alpsg_internal_fail(ALPSG_E_ARG, "grid too small for positions_principal");
+  if (nperp > 2048) return alpsg_internal_fail(ALPSG_E_UNSUPPORTED, "served plasmas need nperp <= 2048");
   if (d->n_resonance_interval < 2) return alpsg_internal_fail(ALPSG_E_ARG, "n_resonance_interval < 2");
   r.nspec = nspec;
   r.nperp = nperp;
@@ -248,12 +299,12 @@ int prepare_request(const alpsg_plasma_desc *d, Request &r) {
   r.ppar.resize((size_t)nspec * (npar + 1));
   for (int s = 0; s < nspec; ++s) {
     if ((d->usebM && d->usebM[s]) || (d->relativistic && d->relativistic[s]))
-      return alpsg_internal_fail(ALPSG_E_UNSUPPORTED, "served requests take tabulated non-relativistic species");
+      return alpsg_internal_fail(ALPSG_E_UNSUPPORTED, "served plasmas take tabulated non-relativistic species");
     if (d->qs[s] == 0.0 || d->ms[s] <= 0.0) return alpsg_internal_fail(ALPSG_E_ARG, "species charge/mass");
     for (int i = 0; i <= nperp; ++i) r.pperp[(size_t)s * (nperp + 1) + i] = PP(s, i, 1, 0);
     for (int j = 0; j <= npar; ++j) r.ppar[(size_t)s * (npar + 1) + j] = PP(s, 2, j, 1);
-    for (int i = 0; i <= nperp; ++i)
-      for (int j = 0; j <= npar; ++j)
+    for (int j = 0; j <= npar; ++j)
+      for (int i = 0; i <= nperp; ++i)
         if (PP(s, i, j, 0) != r.pperp[(size_t)s * (nperp + 1) + i] ||
             PP(s, i, j, 1) != r.ppar[(size_t)s * (npar + 1) + j])
           return alpsg_internal_fail(ALPSG_E_UNSUPPORTED, "momentum grid is not a tensor product of pperp and ppar");
@@ -316,7 +367,7 @@ int prepare_request(const alpsg_plasma_desc *d, Request &r) {
     }
     r.fit_off[s + 1] = r.fit_off[s] + (r.acm[s] == 1 ? (size_t)(nperp + 1) * r.nfits[s] * FIT_STRIDE : 0);
   }
-  // block layout: the uploaded prefix, then df0, int_ee and max|J_n| (read back together)
+  // block layout: the uploaded prefix, then df0
   const size_t ng = (size_t)nspec * (nperp + 1) * (npar + 1);
   const size_t nfit = std::max<size_t>(r.fit_off[nspec], 1);
   const size_t npoly = (size_t)nspec * (nperp + 1) * (r.maxorder + 1);
@@ -331,9 +382,7 @@ int prepare_request(const alpsg_plasma_desc *d, Request &r) {
   r.n_up = L.size;
   r.o_df1 = L.add(sizeof(double) * ng);
   r.o_df2 = L.add(sizeof(double) * ng);
-  r.o_ee = L.size;  // int_ee and max|J_n| contiguous
-  L.add(sizeof(double) * nspec);
-  r.o_bmax = r.o_ee + sizeof(double) * nspec;
+  r.sz = L.size;
   r.img.assign(r.n_up, 0);
   auto H = [&](size_t o) { return reinterpret_cast<double *>(r.img.data() + o); };
   std::memcpy(H(r.o_pperp), r.pperp.data(), sizeof(double) * r.pperp.size());
@@ -342,8 +391,8 @@ int prepare_request(const alpsg_plasma_desc *d, Request &r) {
   std::memcpy(H(r.o_ms), r.ms.data(), sizeof(double) * nspec);
   double *f0 = H(r.o_f0);
   for (int s = 0; s < nspec; ++s)
-    for (int i = 0; i <= nperp; ++i)
-      for (int j = 0; j <= npar; ++j)
+    for (int j = 0; j <= npar; ++j)
+      for (int i = 0; i <= nperp; ++i)
         f0[((size_t)s * (nperp + 1) + i) * (npar + 1) + j] =
             d->f0[(size_t)s + (size_t)nspec * (i + (size_t)(nperp + 1) * j)];
   // fit_function coefficients (ALPS_analyt.f90:185-266) with the pperp factor
@@ -384,54 +433,104 @@ int prepare_request(const alpsg_plasma_desc *d, Request &r) {
 
 // determine_nmax's search bound (ALPS_fns.f90:3971-4075), as alpsg_set_k
 // bounds it: |J_n(x)| <= (x/2)^n / n!
-void size_request(Request &r) {
+int ntry_bound(const Plasma &P, double kperp) {
   double xmax = 0.0;
-  for (int s = 0; s < r.nspec; ++s)
-    for (int i = 0; i <= r.nperp; ++i)
-      xmax = std::max(xmax, std::fabs(r.kperp * r.pperp[(size_t)s * (r.nperp + 1) + i] / r.qs[s]));
+  for (int s = 0; s < P.nspec; ++s)
+    for (int i = 0; i <= P.nperp; ++i)
+      xmax = std::max(xmax, std::fabs(kperp * P.pperp[(size_t)s * (P.nperp + 1) + i] / P.qs[s]));
   int ntry = 1;
-  const double lz = std::log(std::max(r.Bessel_zero, 1e-300)) - std::log(1e6);
+  const double lz = std::log(std::max(P.Bessel_zero, 1e-300)) - std::log(1e6);
   while (ntry < MTAB_S - 16) {
     const double lb = ntry * std::log(std::max(xmax, 1e-300) / 2.0) - std::lgamma(ntry + 1.0);
     if (lb < lz && ntry > xmax) break;
     ++ntry;
   }
-  r.ntry = std::min(ntry + 8, MTAB_S - 4);
-  r.sz_s = r.o_bmax + (sizeof(double) * (size_t)r.nspec * r.ntry + ALIGN - 1) / ALIGN * ALIGN;
+  return std::min(ntry + 8, MTAB_S - 4);
 }
 
-// stage 1 of admission: the omega-independent block, df0, int_ee and max|J_n|
-// (read back into pinned row 'stage'); returns 1 when the arena has no room
-int admit_static(alpsg_server *S, Request &r, int stage) {
-  const int ns = r.nspec, nperp = r.nperp, npar = r.npar;
-  const size_t off = S->arena.alloc(r.sz_s);
+void evict(alpsg_server *S, Plasma &P) {
+  S->arena.release(P.off, P.sz);
+  P.off = SIZE_MAX;
+  P.ee_state = 0;
+  S->evictions += 1;
+}
+
+// arena block, evicting resident plasmas no admitted request reads (least
+// recently used first) until it fits; SIZE_MAX if it cannot
+size_t alloc_evict(alpsg_server *S, size_t sz) {
+  size_t off = S->arena.alloc(sz);
+  while (off == SIZE_MAX) {
+    Plasma *lru = nullptr;
+    for (auto &p : S->resident)
+      if (p->off != SIZE_MAX && p->users == 0 && p->ee_state != 1 &&
+          (lru == nullptr || p->last_use < lru->last_use))
+        lru = p.get();
+    if (!lru) return SIZE_MAX;
+    evict(S, *lru);
+    off = S->arena.alloc(sz);
+  }
+  return off;
+}
+
+void drop_nonresident(alpsg_server *S) {
+  S->resident.erase(std::remove_if(S->resident.begin(), S->resident.end(),
+                                   [](const std::shared_ptr<Plasma> &p) {
+                                     if (p->off != SIZE_MAX) return false;
+                                     p->listed = false;
+                                     return true;
+                                   }),
+                    S->resident.end());
+}
+
+// a plasma's device block: upload, df0 and int_ee (read back with the pass);
+// returns 1 when the arena has no room
+int upload_plasma(alpsg_server *S, const std::shared_ptr<Plasma> &pp, size_t &stage_used) {
+  Plasma &P = *pp;
+  if (stage_used + P.nspec > STAGE_CAP) return 1;
+  const size_t off = alloc_evict(S, P.sz);
   if (off == SIZE_MAX) return 1;
-  r.off_s = off;
+  P.off = off;
   char *b = S->arena.base + off;
-  auto P = [&](size_t o) { return reinterpret_cast<double *>(b + o); };
-  SCHK(hipMemcpyAsync(b, r.img.data(), r.n_up, hipMemcpyHostToDevice, S->st));
-  SCHK(hipMemsetAsync(P(r.o_df1), 0, r.o_ee - r.o_df1, S->st));
-  SCHK(launch_derivative(S->st, ns, nperp, npar, P(r.o_f0), P(r.o_pperp), P(r.o_ppar), P(r.o_df1), P(r.o_df2)));
-  SCHK(launch_int_ee(S->st, ns, nperp, npar, P(r.o_pperp), P(r.o_ppar), P(r.o_df1), P(r.o_df2), P(r.o_qs),
-                     P(r.o_ms), P(r.o_ee)));
-  SCHK(launch_bessel_max(S->st, ns, nperp, r.ntry, r.kperp, P(r.o_pperp), P(r.o_qs), S->mtab_d, P(r.o_bmax)));
-  r.stage = stage;
-  SCHK(hipMemcpyAsync(S->stage_h + (size_t)stage * S->stage_row, P(r.o_ee),
-                      sizeof(double) * (size_t)ns * (1 + r.ntry), hipMemcpyDeviceToHost, S->st));
-  r.state = 1;
+  auto D = [&](size_t o) { return reinterpret_cast<double *>(b + o); };
+  SCHK(hipMemcpyAsync(b, P.img.data(), P.n_up, hipMemcpyHostToDevice, S->st));
+  SCHK(hipMemsetAsync(D(P.o_df1), 0, P.sz - P.o_df1, S->st));
+  SCHK(launch_derivative(S->st, P.nspec, P.nperp, P.npar, D(P.o_f0), D(P.o_pperp), D(P.o_ppar),
+                         D(P.o_df1), D(P.o_df2)));
+  SCHK(launch_int_ee(S->st, P.nspec, P.nperp, P.npar, D(P.o_pperp), D(P.o_ppar), D(P.o_df1),
+                     D(P.o_df2), D(P.o_qs), D(P.o_ms), S->stage_d + stage_used));
+  P.ee_stage = stage_used;
+  stage_used += P.nspec;
+  P.ee_state = 1;
+  S->uploads += 1;
+  if (!P.listed) {
+    P.listed = true;
+    S->resident.push_back(pp);
+  }
   return 0;
 }
 
-// stage 2: nmax from max|J_n| (determine_nmax), the per-k tables, the problem
-// descriptor; returns 1 when the arena has no room
-int admit_k(alpsg_server *S, Request &r) {
-  const int ns = r.nspec, nperp = r.nperp, npar = r.npar;
-  const double *ee = r.sized.data();
-  const double *bm = ee + ns;
+// fill a prefix-sum start array of one job kind; returns the block total
+int fill_start(const TabJob *jobs, int n, int *start, int (*blocks)(const TabJob &)) {
+  long long t = 0;
+  for (int j = 0; j < n; ++j) {
+    start[j] = (int)t;
+    t += blocks(jobs[j]);
+  }
+  start[n] = (int)t;
+  return (int)t;
+}
+
+// per-k block of an admitted request: nmax from max|J_n| (determine_nmax),
+// bes/atab/nodes jobs and its problem descriptor; returns 1 when the arena
+// has no room, ALPSG_E_ARG when the Bessel cutoff is not reached
+int admit_k(alpsg_server *S, Request &r, int &njobs, int &max_nperp, int &nnew) {
+  const Plasma &P = *r.pl;
+  const int ns = P.nspec, nperp = P.nperp, npar = P.npar;
+  if (njobs + ns > MAX_JOBS) return 1;
   for (int s = 0; s < ns; ++s) {
     int n = 0;
     for (int t = 1; t <= r.ntry; ++t)
-      if (!(bm[(size_t)s * r.ntry + (t - 1)] > r.Bessel_zero)) {
+      if (!(r.bmax[(size_t)s * r.ntry + (t - 1)] > P.Bessel_zero)) {
         n = t;
         break;
       }
@@ -446,74 +545,88 @@ int admit_k(alpsg_server *S, Request &r) {
     oa[s] = L.add(sizeof(double) * (size_t)(nm + 1) * (nperp + 1) * ATAB_STRIDE);
     on[s] = L.add(sizeof(double) * (size_t)(nm + 1) * (npar - 1) * NODE_STRIDE);
   }
-  const size_t off = S->arena.alloc(L.size);
+  const size_t off = alloc_evict(S, L.size);
   if (off == SIZE_MAX) return 1;
   r.off_k = off;
   r.sz_k = L.size;
-  char *bs = S->arena.base + r.off_s, *bk = S->arena.base + off;
+  char *bs = S->arena.base + P.off, *bk = S->arena.base + off;
   auto PS = [&](size_t o) { return reinterpret_cast<double *>(bs + o); };
   auto PK = [&](size_t o) { return reinterpret_cast<double *>(bk + o); };
-  ProbDev &pd = S->probs_h[r.slot];
+  ProbDev &pd = S->newp_h[nnew];
   std::memset(&pd, 0, sizeof(pd));
   pd.nspec = ns;
-  pd.kperp_norm = r.kperp_norm;
-  pd.P = r.P;
-  pd.NRI = r.NRI;
-  pd.vA = r.vA;
-  pd.Tlim = r.Tlim;
+  pd.kperp_norm = P.kperp_norm;
+  pd.P = P.P;
+  pd.NRI = P.NRI;
+  pd.vA = P.vA;
+  pd.Tlim = P.Tlim;
   pd.kperp = r.kperp;
   pd.kpar = r.kpar;
   for (int s = 0; s < ns; ++s) {
     SpecDev &sd = pd.sp[s];
-    const double *pe = &r.pperp[(size_t)s * (nperp + 1)];
-    const double *pa = &r.ppar[(size_t)s * (npar + 1)];
+    const double *pe = &P.pperp[(size_t)s * (nperp + 1)];
+    const double *pa = &P.ppar[(size_t)s * (npar + 1)];
     const int nm = r.nmax[s];
-    double *pperp_d = PS(r.o_pperp) + (size_t)s * (nperp + 1);
-    double *ppar_d = PS(r.o_ppar) + (size_t)s * (npar + 1);
-    SCHK(launch_species_k(S->st, nperp, npar, nm, r.kperp, r.kpar, r.qs[s], r.ms[s], pperp_d, ppar_d,
-                          PS(r.o_df1) + (size_t)s * (nperp + 1) * (npar + 1),
-                          PS(r.o_df2) + (size_t)s * (nperp + 1) * (npar + 1), S->mtab_d, PK(ob[s]),
-                          PK(oa[s]), PK(on[s])));
+    double *pperp_d = PS(P.o_pperp) + (size_t)s * (nperp + 1);
+    double *ppar_d = PS(P.o_ppar) + (size_t)s * (npar + 1);
+    TabJob &J = S->jb2.jobs_h[njobs++];
+    J.pperp = pperp_d;
+    J.ppar = ppar_d;
+    J.df1 = PS(P.o_df1) + (size_t)s * (nperp + 1) * (npar + 1);
+    J.df2 = PS(P.o_df2) + (size_t)s * (nperp + 1) * (npar + 1);
+    J.bes = PK(ob[s]);
+    J.atab = PK(oa[s]);
+    J.nodes = PK(on[s]);
+    J.bmax = nullptr;
+    J.kperp = r.kperp;
+    J.kpar = r.kpar;
+    J.q = P.qs[s];
+    J.m = P.ms[s];
+    J.nperp = nperp;
+    J.npar = npar;
+    J.nmax = nm;
+    J.ntry = 0;
+    max_nperp = std::max(max_nperp, nperp);
     sd.nperp = nperp;
     sd.npar = npar;
     sd.nmax = nm;
-    sd.acmethod = r.acm[s];
-    sd.nfits = r.acm[s] == 1 ? r.nfits[s] : 0;
-    for (int f = 0; f < sd.nfits; ++f) sd.fit_type[f] = r.ftype[(size_t)s * ALPSG_MAXFITS + f];
-    sd.fit_uniform = r.funi[s];
-    sd.q = r.qs[s];
-    sd.m = r.ms[s];
-    sd.dens = r.ns[s];
+    sd.acmethod = P.acm[s];
+    sd.nfits = P.acm[s] == 1 ? P.nfits[s] : 0;
+    for (int f = 0; f < sd.nfits; ++f) sd.fit_type[f] = P.ftype[(size_t)s * ALPSG_MAXFITS + f];
+    sd.fit_uniform = P.funi[s];
+    sd.q = P.qs[s];
+    sd.m = P.ms[s];
+    sd.dens = P.ns[s];
     sd.dpperp = pe[2] - pe[1];
     sd.dppar = pa[2] - pa[1];
     sd.dppar_abs = std::fabs(pa[2] - pa[1]);
-    sd.z = r.kperp_norm ? r.kperp / r.qs[s] : 1.0 / r.qs[s];
-    sd.int_ee = ee[s];
+    sd.z = P.kperp_norm ? r.kperp / P.qs[s] : 1.0 / P.qs[s];
+    sd.int_ee = P.int_ee[s];
     sd.pperp = pperp_d;
     sd.ppar = ppar_d;
     sd.atab = PK(oa[s]);
     sd.nodes = PK(on[s]);
-    sd.fitc = PS(r.o_fit) + r.fit_off[s];
-    if (r.acm[s] == 2) {
+    sd.fitc = PS(P.o_fit) + P.fit_off[s];
+    if (P.acm[s] == 2) {
       sd.poly_kind = 1;
-      sd.poly_order = r.porder[s];
-      sd.poly_logfit = r.plog[s];
-      sd.poly_log_max = r.plogmax[s];
+      sd.poly_order = P.porder[s];
+      sd.poly_logfit = P.plog[s];
+      sd.poly_log_max = P.plogmax[s];
       sd.poly_norm1 = 5.e-1 * (pa[npar] + pa[0]);
       sd.poly_norm2 = 5.e-1 * (pa[npar] - pa[0]);
-      sd.polyc = PS(r.o_poly) + (size_t)s * (nperp + 1) * (r.maxorder + 1);
+      sd.polyc = PS(P.o_poly) + (size_t)s * (nperp + 1) * (P.maxorder + 1);
     }
   }
-  SCHK(hipMemcpyAsync(S->probs_d + r.slot, &pd, sizeof(ProbDev), hipMemcpyHostToDevice, S->st));
+  S->newslot_h[nnew++] = r.slot;
   r.m.start(r.guess, r.rp);
   r.state = 2;
   return 0;
 }
 
 void release(alpsg_server *S, Request &r) {
-  if (r.off_s != SIZE_MAX) S->arena.release(r.off_s, r.sz_s);
   if (r.off_k != SIZE_MAX) S->arena.release(r.off_k, r.sz_k);
-  r.off_s = r.off_k = SIZE_MAX;
+  r.off_k = SIZE_MAX;
+  if (r.state >= 1 && r.pl) r.pl->users -= 1;
   if (r.slot >= 0) S->free_slots.push_back(r.slot);
   r.slot = -1;
 }
@@ -529,9 +642,14 @@ void post_results(alpsg_server *S, std::vector<Result> &done) {
 
 // one pass: admissions, the batched disp of every active solve, results
 int server_pass(alpsg_server *S) {
+  const double tp0 = now_ms();
   std::vector<Result> done;
-  // stage 1 for waiting requests, first come first served, while slots and space last
+  // stage 1 for waiting requests, first come first served, while slots, job
+  // room, readback room and arena space last
   std::vector<Request *> sizing_now;
+  std::vector<Plasma *> uploaded_now;
+  size_t stage_used = 0;
+  int nj1 = 0;
   for (;;) {
     Request *r = nullptr;
     {
@@ -539,12 +657,17 @@ int server_pass(alpsg_server *S) {
       if (S->queue.empty()) break;
       r = S->queue.front().get();
     }
-    if (S->free_slots.empty() || (int)sizing_now.size() >= S->max_admit) break;
-    r->slot = S->free_slots.back();
-    const int rc = admit_static(S, *r, (int)sizing_now.size());
-    if (rc > 1) return rc;
+    Plasma &P = *r->pl;
+    const size_t need = (size_t)P.nspec * r->ntry;
+    if (S->free_slots.empty() || nj1 + P.nspec > MAX_JOBS || stage_used + P.nspec + need > STAGE_CAP)
+      break;
+    int rc = 0;
+    if (P.off == SIZE_MAX) {
+      rc = upload_plasma(S, r->pl, stage_used);
+      if (rc > 1) return rc;
+      if (rc == 0) uploaded_now.push_back(&P);
+    }
     if (rc == 1) {
-      r->slot = -1;
       if (!S->live.empty() || !sizing_now.empty()) break;  // space frees as solves finish
       // nothing admitted can free space: the request does not fit the arena
       std::unique_ptr<Request> own;
@@ -556,20 +679,45 @@ int server_pass(alpsg_server *S) {
       done.push_back(Result{own->tag, cplx(0.0, 0.0), 0, ALPSG_E_ARG, now_ms() - own->t_submit});
       continue;
     }
+    P.users += 1;
+    P.last_use = S->passes;
+    r->slot = S->free_slots.back();
     S->free_slots.pop_back();
+    r->stage = stage_used;
+    stage_used += need;
+    const double *pe = reinterpret_cast<const double *>(S->arena.base + P.off + P.o_pperp);
+    for (int s = 0; s < P.nspec; ++s) {
+      TabJob &J = S->jb1.jobs_h[nj1++];
+      std::memset(&J, 0, sizeof(J));
+      J.pperp = pe + (size_t)s * (P.nperp + 1);
+      J.bmax = S->stage_d + r->stage + (size_t)s * r->ntry;
+      J.kperp = r->kperp;
+      J.q = P.qs[s];
+      J.nperp = P.nperp;
+      J.ntry = r->ntry;
+    }
+    r->state = 1;
     sizing_now.push_back(r);
     std::lock_guard<std::mutex> lk(S->mu);
     S->live.push_back(std::move(S->queue.front()));
     S->queue.pop_front();
   }
+  if (nj1 > 0) {
+    const int total = fill_start(S->jb1.jobs_h, nj1, S->jb1.start_h, jobs_blocks_bessel_max);
+    SCHK(hipMemcpyAsync(S->jb1.jobs_d, S->jb1.jobs_h, sizeof(TabJob) * nj1, hipMemcpyHostToDevice, S->st));
+    SCHK(hipMemcpyAsync(S->jb1.start_d, S->jb1.start_h, sizeof(int) * (nj1 + 1), hipMemcpyHostToDevice, S->st));
+    SCHK(launch_bessel_max_jobs(S->st, S->jb1.jobs_d, S->jb1.start_d, nj1, total, S->mtab_d));
+  }
+  if (stage_used > 0)
+    SCHK(hipMemcpyAsync(S->stage_h, S->stage_d, sizeof(double) * stage_used, hipMemcpyDeviceToHost, S->st));
   // the batch: one pending omega per active solve, grouped by species count
   std::vector<Request *> act;
   for (auto &p : S->live)
     if (p->state == 2 && !p->m.done) act.push_back(p.get());
-  std::stable_sort(act.begin(), act.end(), [](Request *a, Request *b) { return a->nspec < b->nspec; });
-  if ((int)act.size() > S->max_batch) act.resize(S->max_batch);
+  std::stable_sort(act.begin(), act.end(), [](Request *a, Request *b) { return a->pl->nspec < b->pl->nspec; });
   const int nb = (int)act.size();
   const double t0 = now_ms();
+  S->t_admit += t0 - tp0;
   for (int i = 0; i < nb; ++i) {
     const cplx w = act[i]->m.want();
     S->om_h[i] = make_double2(w.real(), w.imag());
@@ -581,10 +729,10 @@ int server_pass(alpsg_server *S) {
     int i0 = 0;
     while (i0 < nb) {
       int i1 = i0;
-      const int ns = act[i0]->nspec;
+      const int ns = act[i0]->pl->nspec;
       bool gen = false;
-      for (; i1 < nb && act[i1]->nspec == ns; ++i1)
-        if (act[i1]->general) gen = true;
+      for (; i1 < nb && act[i1]->pl->nspec == ns; ++i1)
+        if (act[i1]->pl->general) gen = true;
       const int n = i1 - i0;
       SCHK(launch_sweep(S->st, 64, SW_W, SW_RC, S->probs_d, n, S->om_d + i0, nullptr, S->chiA,
                         nullptr, S->pidx_d + i0));
@@ -597,29 +745,36 @@ int server_pass(alpsg_server *S) {
     }
     SCHK(hipMemcpyAsync(S->det_h, S->det_d, sizeof(double2) * nb, hipMemcpyDeviceToHost, S->st));
   }
+  const double t1 = now_ms();
   SCHK(hipStreamSynchronize(S->st));
+  const double t2 = now_ms();
+  S->t_launch += t1 - t0;
+  S->t_wait += t2 - t1;
   S->passes += 1;
   S->evals += nb;
-  S->busy_ms += now_ms() - t0;
-  // int_ee and max|J_n| of this pass's admissions are on the host now
-  for (Request *r : sizing_now) {
-    const double *row = S->stage_h + (size_t)r->stage * S->stage_row;
-    r->sized.assign(row, row + (size_t)r->nspec * (1 + r->ntry));
+  // the readback of this pass: int_ee of the uploaded plasmas, max|J_n| of the admissions
+  for (Plasma *P : uploaded_now) {
+    P->int_ee.assign(S->stage_h + P->ee_stage, S->stage_h + P->ee_stage + P->nspec);
+    P->ee_state = 2;
   }
-  // stage 2 for every sized request, oldest first, while space lasts
+  for (Request *r : sizing_now)
+    r->bmax.assign(S->stage_h + r->stage, S->stage_h + r->stage + (size_t)r->pl->nspec * r->ntry);
+  // stage 2 for every sized request, oldest first, while job room and space last
+  const double t3 = now_ms();
+  int nj2 = 0, max_nperp = 4, nnew = 0;
   for (auto it = S->live.begin(); it != S->live.end();) {
     Request &r = **it;
     if (r.state != 1) {
       ++it;
       continue;
     }
-    const int rc = admit_k(S, r);
+    const int rc = admit_k(S, r, nj2, max_nperp, nnew);
     bool drop = false;
     if (rc == 1) {
-      bool any_active = false;
+      bool any_active = nnew > 0;
       for (auto &p : S->live) any_active = any_active || p->state == 2;
-      if (any_active) break;  // space frees as solves finish
-      drop = true;            // nothing can free space: the request does not fit the arena
+      if (any_active || nj2 + r.pl->nspec > MAX_JOBS) break;  // room frees up later
+      drop = true;  // nothing can free space: the request does not fit the arena
     } else if (rc == ALPSG_E_ARG) {
       drop = true;  // this request's own data (Bessel cutoff not reached)
     } else if (rc) {
@@ -633,6 +788,23 @@ int server_pass(alpsg_server *S) {
     }
     ++it;
   }
+  if (nj2 > 0) {
+    int *sb = S->jb2.start_h, *sa = sb + (MAX_JOBS + 1), *sm = sa + (MAX_JOBS + 1);
+    const int tb = fill_start(S->jb2.jobs_h, nj2, sb, jobs_blocks_bessel);
+    const int ta = fill_start(S->jb2.jobs_h, nj2, sa, jobs_blocks_atab);
+    const int tm = fill_start(S->jb2.jobs_h, nj2, sm, jobs_blocks_moments);
+    SCHK(hipMemcpyAsync(S->jb2.jobs_d, S->jb2.jobs_h, sizeof(TabJob) * nj2, hipMemcpyHostToDevice, S->st));
+    SCHK(hipMemcpyAsync(S->jb2.start_d, S->jb2.start_h, sizeof(int) * 3 * (MAX_JOBS + 1),
+                        hipMemcpyHostToDevice, S->st));
+    int *db = S->jb2.start_d, *da = db + (MAX_JOBS + 1), *dm = da + (MAX_JOBS + 1);
+    SCHK(launch_species_k_jobs(S->st, S->jb2.jobs_d, nj2, db, tb, da, ta, dm, tm, max_nperp, S->mtab_d));
+  }
+  if (nnew > 0) {
+    SCHK(hipMemcpyAsync(S->newp_d, S->newp_h, sizeof(ProbDev) * nnew, hipMemcpyHostToDevice, S->st));
+    SCHK(hipMemcpyAsync(S->newslot_d, S->newslot_h, sizeof(int) * nnew, hipMemcpyHostToDevice, S->st));
+    SCHK(launch_scatter_probs(S->st, S->newp_d, S->newslot_d, nnew, S->probs_d));
+  }
+  S->t_admit += now_ms() - t3;
   // feed the results; retire finished solves
   for (int i = 0; i < nb; ++i) act[i]->m.feed(cplx(S->det_h[i].x, S->det_h[i].y));
   for (auto it = S->live.begin(); it != S->live.end();) {
@@ -645,7 +817,12 @@ int server_pass(alpsg_server *S) {
       ++it;
     }
   }
+  // device blocks of plasmas that are no longer registered and no request reads
+  for (auto &p : S->resident)
+    if (p->off != SIZE_MAX && p->users == 0 && p->ee_state != 1 && !p->registered.load()) evict(S, *p);
+  drop_nonresident(S);
   post_results(S, done);
+  S->t_pass += now_ms() - tp0;
   return ALPSG_OK;
 }
 
@@ -654,8 +831,9 @@ void service_loop(alpsg_server *S) {
   for (;;) {
     {
       std::unique_lock<std::mutex> lk(S->mu);
-      S->cv.wait(lk, [&] { return S->stop || !S->queue.empty() || !S->live.empty(); });
+      S->cv.wait(lk, [&] { return S->stop || S->gc || !S->queue.empty() || !S->live.empty(); });
       if (S->stop && S->queue.empty() && S->live.empty()) return;
+      S->gc = false;
     }
     const int rc = server_pass(S);
     if (rc) {
@@ -666,6 +844,34 @@ void service_loop(alpsg_server *S) {
       return;
     }
   }
+}
+
+int enqueue(alpsg_server *S, std::shared_ptr<Plasma> pl, double kperp, double kpar,
+            const double *om_guess, const alpsg_root_params *rp, long long tag) {
+  if (!om_guess || !rp) return alpsg_internal_fail(ALPSG_E_ARG, "null argument");
+  if (!(kpar != 0.0) || !std::isfinite(kpar) || !std::isfinite(kperp))
+    return alpsg_internal_fail(ALPSG_E_ARG, "kpar must be finite and nonzero");
+  if (!std::isfinite(om_guess[0]) || !std::isfinite(om_guess[1]))
+    return alpsg_internal_fail(ALPSG_E_ARG, "initial frequency must be finite");
+  if (rp->secant_method != 2)
+    return alpsg_internal_fail(ALPSG_E_UNSUPPORTED, "the server runs secant_osc (secant_method 2)");
+  if (rp->numiter < 1 || rp->numiter > 100000) return alpsg_internal_fail(ALPSG_E_ARG, "numiter out of range");
+  std::unique_ptr<Request> r(new Request());
+  r->tag = tag;
+  r->kperp = kperp;
+  r->kpar = kpar;
+  r->guess = cplx(om_guess[0], om_guess[1]);
+  r->rp = *rp;
+  r->ntry = ntry_bound(*pl, kperp);
+  r->pl = std::move(pl);
+  r->t_submit = now_ms();
+  std::lock_guard<std::mutex> lk(S->mu);
+  if (S->failed) return alpsg_internal_fail(ALPSG_E_DEVICE, "server stopped: " + S->fail_msg);
+  if (S->stop) return alpsg_internal_fail(ALPSG_E_STATE, "server is shutting down");
+  S->queue.push_back(std::move(r));
+  S->submitted += 1;
+  S->cv.notify_all();
+  return ALPSG_OK;
 }
 
 }  // namespace
@@ -680,8 +886,7 @@ int alpsg_server_create(double hbm_bytes, int max_slots, alpsg_server **out) {
     return alpsg_internal_fail(ALPSG_E_ARG, "HBM budget out of range 1 MiB..256 GB");
   std::unique_ptr<alpsg_server> S(new alpsg_server());
   S->max_slots = max_slots;
-  S->max_batch = max_slots;
-  // on failure below, the destructor path of alpsg_server_destroy frees what exists
+  // on failure below, alpsg_server_destroy frees what exists
   struct Guard {
     std::unique_ptr<alpsg_server> &p;
     ~Guard() {
@@ -693,14 +898,13 @@ int alpsg_server_create(double hbm_bytes, int max_slots, alpsg_server **out) {
   SCHK(hipMalloc((void **)&S->arena_mem, cap));
   S->arena.init(S->arena_mem, cap);
   SCHK(hipMalloc((void **)&S->probs_d, sizeof(ProbDev) * max_slots));
-  S->probs_h.assign(max_slots, ProbDev{});
   for (int i = max_slots - 1; i >= 0; --i) S->free_slots.push_back(i);
   // Miller start indices m(n) = 2*((n+int(sqrt(float(40n))))/2) (ALPS_fns_rel.f90:1622)
   std::vector<int> mt(MTAB_S, 0);
   for (int n = 1; n < MTAB_S; ++n) mt[n] = 2 * ((n + (int)std::sqrt((float)(40 * n))) / 2);
   SCHK(hipMalloc((void **)&S->mtab_d, sizeof(int) * MTAB_S));
   SCHK(hipMemcpy(S->mtab_d, mt.data(), sizeof(int) * MTAB_S, hipMemcpyHostToDevice));
-  const size_t mb = (size_t)S->max_batch;
+  const size_t mb = (size_t)max_slots;
   SCHK(hipMalloc((void **)&S->om_d, sizeof(double2) * mb));
   SCHK(hipMalloc((void **)&S->det_d, sizeof(double2) * mb));
   SCHK(hipMalloc((void **)&S->pidx_d, sizeof(int) * mb));
@@ -709,39 +913,64 @@ int alpsg_server_create(double hbm_bytes, int max_slots, alpsg_server **out) {
   SCHK(hipHostMalloc((void **)&S->om_h, sizeof(double2) * mb, hipHostMallocDefault));
   SCHK(hipHostMalloc((void **)&S->det_h, sizeof(double2) * mb, hipHostMallocDefault));
   SCHK(hipHostMalloc((void **)&S->pidx_h, sizeof(int) * mb, hipHostMallocDefault));
-  S->stage_row = STAGE_ROW;
-  SCHK(hipHostMalloc((void **)&S->stage_h, sizeof(double) * S->stage_row * alpsg_server::max_admit,
-                     hipHostMallocDefault));
+  SCHK(hipHostMalloc((void **)&S->stage_h, sizeof(double) * STAGE_CAP, hipHostMallocDefault));
+  SCHK(hipMalloc((void **)&S->stage_d, sizeof(double) * STAGE_CAP));
+  SCHK(S->jb1.init());
+  SCHK(S->jb2.init());
+  SCHK(hipHostMalloc((void **)&S->newp_h, sizeof(ProbDev) * mb, hipHostMallocDefault));
+  SCHK(hipMalloc((void **)&S->newp_d, sizeof(ProbDev) * mb));
+  SCHK(hipHostMalloc((void **)&S->newslot_h, sizeof(int) * mb, hipHostMallocDefault));
+  SCHK(hipMalloc((void **)&S->newslot_d, sizeof(int) * mb));
   S->th = std::thread(service_loop, S.get());
   *out = S.release();
   return ALPSG_OK;
 }
 
-int alpsg_server_submit(alpsg_server *S, const alpsg_plasma_desc *d, double kperp, double kpar,
-                        const double *om_guess, const alpsg_root_params *rp, long long tag) {
-  if (!S || !d || !om_guess || !rp) return alpsg_internal_fail(ALPSG_E_ARG, "null argument");
-  if (!(kpar != 0.0) || !std::isfinite(kpar) || !std::isfinite(kperp))
-    return alpsg_internal_fail(ALPSG_E_ARG, "kpar must be finite and nonzero");
-  if (rp->secant_method != 2)
-    return alpsg_internal_fail(ALPSG_E_UNSUPPORTED, "the server runs secant_osc (secant_method 2)");
-  if (rp->numiter < 1 || rp->numiter > 100000) return alpsg_internal_fail(ALPSG_E_ARG, "numiter out of range");
-  std::unique_ptr<Request> r(new Request());
-  const int rc = prepare_request(d, *r);
+int alpsg_server_add_plasma(alpsg_server *S, const alpsg_plasma_desc *d, int *plasma_id) {
+  if (!S || !d || !plasma_id) return alpsg_internal_fail(ALPSG_E_ARG, "null argument");
+  auto pl = std::make_shared<Plasma>();
+  const int rc = prepare_plasma(d, *pl);
   if (rc) return rc;
-  r->tag = tag;
-  r->kperp = kperp;
-  r->kpar = kpar;
-  r->guess = cplx(om_guess[0], om_guess[1]);
-  r->rp = *rp;
-  size_request(*r);
-  r->t_submit = now_ms();
+  pl->registered = true;
   std::lock_guard<std::mutex> lk(S->mu);
-  if (S->failed) return alpsg_internal_fail(ALPSG_E_DEVICE, "server stopped: " + S->fail_msg);
-  if (S->stop) return alpsg_internal_fail(ALPSG_E_STATE, "server is shutting down");
-  S->queue.push_back(std::move(r));
-  S->submitted += 1;
+  pl->id = S->next_id++;
+  S->registry[pl->id] = pl;
+  *plasma_id = pl->id;
+  return ALPSG_OK;
+}
+
+int alpsg_server_remove_plasma(alpsg_server *S, int plasma_id) {
+  if (!S) return alpsg_internal_fail(ALPSG_E_ARG, "null handle");
+  std::lock_guard<std::mutex> lk(S->mu);
+  auto it = S->registry.find(plasma_id);
+  if (it == S->registry.end()) return alpsg_internal_fail(ALPSG_E_ARG, "unknown plasma id");
+  it->second->registered = false;
+  S->registry.erase(it);
+  S->gc = true;  // its device block goes once no request reads it
   S->cv.notify_all();
   return ALPSG_OK;
+}
+
+int alpsg_server_submit_k(alpsg_server *S, int plasma_id, double kperp, double kpar,
+                          const double *om_guess, const alpsg_root_params *rp, long long tag) {
+  if (!S) return alpsg_internal_fail(ALPSG_E_ARG, "null handle");
+  std::shared_ptr<Plasma> pl;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->registry.find(plasma_id);
+    if (it == S->registry.end()) return alpsg_internal_fail(ALPSG_E_ARG, "unknown plasma id");
+    pl = it->second;
+  }
+  return enqueue(S, std::move(pl), kperp, kpar, om_guess, rp, tag);
+}
+
+int alpsg_server_submit(alpsg_server *S, const alpsg_plasma_desc *d, double kperp, double kpar,
+                        const double *om_guess, const alpsg_root_params *rp, long long tag) {
+  if (!S || !d) return alpsg_internal_fail(ALPSG_E_ARG, "null argument");
+  auto pl = std::make_shared<Plasma>();  // unregistered: freed with its request
+  const int rc = prepare_plasma(d, *pl);
+  if (rc) return rc;
+  return enqueue(S, std::move(pl), kperp, kpar, om_guess, rp, tag);
 }
 
 int alpsg_server_poll(alpsg_server *S, int max_results, int wait, long long *tags, double *roots,
@@ -769,7 +998,7 @@ int alpsg_server_poll(alpsg_server *S, int max_results, int wait, long long *tag
   return ALPSG_OK;
 }
 
-int alpsg_server_stats(alpsg_server *S, long long *counts, double *busy_ms) {
+int alpsg_server_stats(alpsg_server *S, long long *counts, double *times_ms) {
   if (!S) return alpsg_internal_fail(ALPSG_E_ARG, "null handle");
   std::lock_guard<std::mutex> lk(S->mu);
   if (counts) {
@@ -780,9 +1009,16 @@ int alpsg_server_stats(alpsg_server *S, long long *counts, double *busy_ms) {
     counts[4] = (long long)S->live.size();
     counts[5] = (long long)S->queue.size();
     counts[6] = (long long)S->arena.cap;
-    counts[7] = (long long)S->arena.free_bytes();
+    counts[7] = (long long)S->arena.used;
+    counts[8] = S->uploads;
+    counts[9] = S->evictions;
   }
-  if (busy_ms) *busy_ms = S->busy_ms;
+  if (times_ms) {
+    times_ms[0] = S->t_pass;
+    times_ms[1] = S->t_admit;
+    times_ms[2] = S->t_launch;
+    times_ms[3] = S->t_wait;
+  }
   return ALPSG_OK;
 }
 
@@ -807,6 +1043,13 @@ int alpsg_server_destroy(alpsg_server *S) {
   if (S->det_h) (void)hipHostFree(S->det_h);
   if (S->pidx_h) (void)hipHostFree(S->pidx_h);
   if (S->stage_h) (void)hipHostFree(S->stage_h);
+  if (S->stage_d) (void)hipFree(S->stage_d);
+  S->jb1.free_all();
+  S->jb2.free_all();
+  if (S->newp_h) (void)hipHostFree(S->newp_h);
+  if (S->newp_d) (void)hipFree(S->newp_d);
+  if (S->newslot_h) (void)hipHostFree(S->newslot_h);
+  if (S->newslot_d) (void)hipFree(S->newslot_d);
   if (S->st) (void)hipStreamDestroy(S->st);
   delete S;
   return ALPSG_OK;

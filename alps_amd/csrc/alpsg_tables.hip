@@ -175,17 +175,14 @@ __global__ void k_int_ee(int nperp, int npar, const double *__restrict__ pperp,
   }
 }
 
-// max_iperp |J_n(kperp*pperp/q)| for n = 1..ntry: one block per (n, species).
-__global__ void k_bessel_max(int nperp, int ntry, double kperp, const double *__restrict__ pperp,
-                             const double *__restrict__ qs, const int *__restrict__ mtab,
-                             double *__restrict__ out) {
-  __shared__ double red[64];
-  int n = blockIdx.x + 1;
-  int s = blockIdx.y;
-  const double *pe = pperp + (size_t)s * (nperp + 1);
+// max_iperp |J_n(kperp*pperp/q)| for one n (one 64-lane block)
+__device__ __forceinline__ void bessel_max_block(int n, int nperp, double kperp,
+                                                 const double *__restrict__ pe, double q,
+                                                 const int *__restrict__ mtab, double *out,
+                                                 double *red) {
   double m = 0.0;
   for (int ip = threadIdx.x; ip <= nperp; ip += blockDim.x) {
-    double z = kperp * pe[ip] / qs[s];
+    double z = kperp * pe[ip] / q;
     m = fmax(m, fabs(dev_bessj(n, z, mtab)));
   }
   red[threadIdx.x] = m;
@@ -194,29 +191,46 @@ __global__ void k_bessel_max(int nperp, int ntry, double kperp, const double *__
     if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[(size_t)s * ntry + (n - 1)] = red[0];
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+// max_iperp |J_n(kperp*pperp/q)| for n = 1..ntry: one block per (n, species).
+__global__ void k_bessel_max(int nperp, int ntry, double kperp, const double *__restrict__ pperp,
+                             const double *__restrict__ qs, const int *__restrict__ mtab,
+                             double *__restrict__ out) {
+  __shared__ double red[64];
+  int n = blockIdx.x + 1;
+  int s = blockIdx.y;
+  bessel_max_block(n, nperp, kperp, pperp + (size_t)s * (nperp + 1), qs[s], mtab,
+                   out + (size_t)s * ntry + (n - 1), red);
 }
 
 // bessel_array(n, iperp) for n = -1..nmax+1 -> bes[(n+1)*(nperp+1) + iperp]
-__global__ void k_bessel(int nperp, int nmax, double kperp, double q,
-                         const double *__restrict__ pperp, const int *__restrict__ mtab,
-                         double *__restrict__ bes) {
-  int ip = blockIdx.x * blockDim.x + threadIdx.x;
-  int n = (int)blockIdx.y - 1;
+__device__ __forceinline__ void bessel_entry(int ip, int n, int nperp, int nmax, double kperp,
+                                             double q, const double *__restrict__ pperp,
+                                             const int *__restrict__ mtab,
+                                             double *__restrict__ bes) {
   if (ip > nperp || n > nmax + 1) return;
   double z = kperp * pperp[ip] / q;
   double v = (n == -1) ? -dev_bessj(1, z, mtab) : dev_bessj(n, z, mtab);
   bes[(size_t)(n + 1) * (nperp + 1) + ip] = v;
 }
 
+__global__ void k_bessel(int nperp, int nmax, double kperp, double q,
+                         const double *__restrict__ pperp, const int *__restrict__ mtab,
+                         double *__restrict__ bes) {
+  bessel_entry(blockIdx.x * blockDim.x + threadIdx.x, (int)blockIdx.y - 1, nperp, nmax, kperp, q,
+               pperp, mtab, bes);
+}
+
 // a-types per (n >= 0, iperp): J^2, pperp^2 J'^2, pperp J J', J.
 // J'_n = (J_{n-1} - J_{n+1})/2 for n >= 1, -J_1 for n = 0 (int_T, ALPS_fns.f90:1647-1656).
 // The negative harmonics use the same values: J_{-n}^2 = J_n^2 and
 // J_{-n} J'_{-n} = J_n J'_n exactly in floating point.
-__global__ void k_atab(int nperp, int nmax, const double *__restrict__ pperp,
-                       const double *__restrict__ bes, double *__restrict__ atab) {
-  int ip = blockIdx.x * blockDim.x + threadIdx.x;
-  int n = blockIdx.y;
+__device__ __forceinline__ void atab_entry(int ip, int n, int nperp, int nmax,
+                                           const double *__restrict__ pperp,
+                                           const double *__restrict__ bes,
+                                           double *__restrict__ atab) {
   if (ip > nperp || n > nmax) return;
   const int W = nperp + 1;
   double J = bes[(size_t)(n + 1) * W + ip];
@@ -230,16 +244,22 @@ __global__ void k_atab(int nperp, int nmax, const double *__restrict__ pperp,
   o[3] = J;
 }
 
+__global__ void k_atab(int nperp, int nmax, const double *__restrict__ pperp,
+                       const double *__restrict__ bes, double *__restrict__ atab) {
+  atab_entry(blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y, nperp, nmax, pperp, bes, atab);
+}
+
 // Node tables: one thread per (n, ipar), summing over iperp.  The a-values of a
 // given n are shared by the whole block (staged in LDS); df1/df2 rows are read
 // coalesced along ipar.
-__global__ void k_moments(int nperp, int npar, int nmax, double kpar, double m,
-                          const double *__restrict__ pperp, const double *__restrict__ ppar,
-                          const double *__restrict__ df1, const double *__restrict__ df2,
-                          const double *__restrict__ atab, double *__restrict__ nodes) {
-  extern __shared__ double lds_a[];  // (nperp-1) * 3
-  int n = blockIdx.y;
-  int ipar = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void moments_block(int bx, int n, int nperp, int npar, double kpar,
+                                              double m, const double *__restrict__ pperp,
+                                              const double *__restrict__ ppar,
+                                              const double *__restrict__ df1,
+                                              const double *__restrict__ df2,
+                                              const double *__restrict__ atab,
+                                              double *__restrict__ nodes, double *lds_a) {
+  int ipar = 1 + bx * blockDim.x + threadIdx.x;
   const int W = nperp + 1;
   for (int t = threadIdx.x; t < (nperp - 1); t += blockDim.x) {
     const double *a = atab + ((size_t)n * W + (t + 1)) * ATAB_STRIDE;
@@ -275,6 +295,75 @@ __global__ void k_moments(int nperp, int npar, int nmax, double kpar, double m,
   o[5] = h3;
   o[6] = pa;
   o[7] = pa * pa;
+}
+
+__global__ void k_moments(int nperp, int npar, int nmax, double kpar, double m,
+                          const double *__restrict__ pperp, const double *__restrict__ ppar,
+                          const double *__restrict__ df1, const double *__restrict__ df2,
+                          const double *__restrict__ atab, double *__restrict__ nodes) {
+  extern __shared__ double lds_a[];  // (nperp-1) * 3
+  moments_block(blockIdx.x, blockIdx.y, nperp, npar, kpar, m, pperp, ppar, df1, df2, atab, nodes,
+                lds_a);
+}
+
+// ---------------------------------------------------------------------------
+// Job-list forms for the root server: one launch covers the tables of many
+// (request, species) jobs.  Job j owns blocks [start[j], start[j+1]) of the
+// flattened grid, in the same (x, y) order as the one-job launch; every block
+// runs the same device code as there, so a job's tables do not depend on the
+// other jobs of its launch.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int job_of(const int *__restrict__ start, int njobs, int b) {
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void k_bessel_max_jobs(const TabJob *__restrict__ jobs, const int *__restrict__ start,
+                                  int njobs, const int *__restrict__ mtab) {
+  __shared__ double red[64];
+  const int j = job_of(start, njobs, blockIdx.x);
+  const TabJob &J = jobs[j];
+  const int n = 1 + (blockIdx.x - start[j]);
+  bessel_max_block(n, J.nperp, J.kperp, J.pperp, J.q, mtab, J.bmax + (n - 1), red);
+}
+
+__global__ void k_bessel_jobs(const TabJob *__restrict__ jobs, const int *__restrict__ start,
+                              int njobs, const int *__restrict__ mtab) {
+  const int j = job_of(start, njobs, blockIdx.x);
+  const TabJob &J = jobs[j];
+  const int b = blockIdx.x - start[j];
+  const int tiles = (J.nperp + 1 + 63) / 64;
+  const int bx = b % tiles, by = b / tiles;
+  bessel_entry(bx * 64 + threadIdx.x, by - 1, J.nperp, J.nmax, J.kperp, J.q, J.pperp, mtab, J.bes);
+}
+
+__global__ void k_atab_jobs(const TabJob *__restrict__ jobs, const int *__restrict__ start,
+                            int njobs) {
+  const int j = job_of(start, njobs, blockIdx.x);
+  const TabJob &J = jobs[j];
+  const int b = blockIdx.x - start[j];
+  const int tiles = (J.nperp + 1 + 63) / 64;
+  const int bx = b % tiles, by = b / tiles;
+  atab_entry(bx * 64 + threadIdx.x, by, J.nperp, J.nmax, J.pperp, J.bes, J.atab);
+}
+
+__global__ void k_moments_jobs(const TabJob *__restrict__ jobs, const int *__restrict__ start,
+                               int njobs) {
+  extern __shared__ double lds_a[];  // (max nperp - 1) * 3
+  const int j = job_of(start, njobs, blockIdx.x);
+  const TabJob &J = jobs[j];
+  const int b = blockIdx.x - start[j];
+  const int tiles = (J.npar - 1 + 63) / 64;
+  const int bx = b % tiles, by = b / tiles;
+  moments_block(bx, by, J.nperp, J.npar, J.kpar, J.m, J.pperp, J.ppar, J.df1, J.df2, J.atab,
+                J.nodes, lds_a);
 }
 
 // ---------------------------------------------------------------------------
@@ -337,5 +426,51 @@ hipError_t launch_species_k(hipStream_t st, int nperp, int npar, int nmax, doubl
   size_t lds = sizeof(double) * 3 * (size_t)(nperp - 1);
   hipLaunchKernelGGL(k_moments, dim3((npar - 1 + 63) / 64, nmax + 1), dim3(64), lds, st, nperp,
                      npar, nmax, kpar, m, pperp, ppar, df1, df2, atab, nodes);
+  return hipGetLastError();
+}
+
+// Block counts per job of the four job-list launches (host side, shared with
+// the caller that builds the start arrays).
+int jobs_blocks_bessel_max(const TabJob &J) { return J.ntry; }
+int jobs_blocks_bessel(const TabJob &J) { return ((J.nperp + 1 + 63) / 64) * (J.nmax + 3); }
+int jobs_blocks_atab(const TabJob &J) { return ((J.nperp + 1 + 63) / 64) * (J.nmax + 1); }
+int jobs_blocks_moments(const TabJob &J) { return ((J.npar - 1 + 63) / 64) * (J.nmax + 1); }
+
+// start arrays: [njobs + 1] each, device copies; total = start[njobs]
+hipError_t launch_bessel_max_jobs(hipStream_t st, const TabJob *jobs, const int *start, int njobs,
+                                  int total, const int *mtab) {
+  if (njobs <= 0 || total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_bessel_max_jobs, dim3(total), dim3(64), 0, st, jobs, start, njobs, mtab);
+  return hipGetLastError();
+}
+
+hipError_t launch_species_k_jobs(hipStream_t st, const TabJob *jobs, int njobs,
+                                 const int *start_bes, int total_bes, const int *start_atab,
+                                 int total_atab, const int *start_mom, int total_mom,
+                                 int max_nperp, const int *mtab) {
+  if (njobs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_bessel_jobs, dim3(total_bes), dim3(64), 0, st, jobs, start_bes, njobs, mtab);
+  hipLaunchKernelGGL(k_atab_jobs, dim3(total_atab), dim3(64), 0, st, jobs, start_atab, njobs);
+  const size_t lds = sizeof(double) * 3 * (size_t)(max_nperp - 1);
+  hipLaunchKernelGGL(k_moments_jobs, dim3(total_mom), dim3(64), lds, st, jobs, start_mom, njobs);
+  return hipGetLastError();
+}
+
+// Problem descriptors of the requests a server pass activates, packed on the
+// host, to their slots of the device descriptor array (one block each).
+__global__ void k_scatter_probs(const ProbDev *__restrict__ src, const int *__restrict__ slots,
+                                int n, ProbDev *__restrict__ dst) {
+  const int i = blockIdx.x;
+  if (i >= n) return;
+  const unsigned long long *a = reinterpret_cast<const unsigned long long *>(src + i);
+  unsigned long long *b = reinterpret_cast<unsigned long long *>(dst + slots[i]);
+  for (int w = threadIdx.x; w < (int)(sizeof(ProbDev) / 8); w += blockDim.x) b[w] = a[w];
+}
+
+hipError_t launch_scatter_probs(hipStream_t st, const ProbDev *src, const int *slots, int n,
+                                ProbDev *dst) {
+  if (n <= 0) return hipSuccess;
+  static_assert(sizeof(ProbDev) % 8 == 0, "ProbDev is copied in 8-byte words");
+  hipLaunchKernelGGL(k_scatter_probs, dim3(n), dim3(256), 0, st, src, slots, n, dst);
   return hipGetLastError();
 }

@@ -40,6 +40,7 @@ EXPORTS = ["alpsg_last_error", "alpsg_create", "alpsg_destroy", "alpsg_set_k",
            "alpsg_om_scan", "alpsg_get_nmax", "alpsg_eval_count", "alpsg_last_kernel_ms",
            "alpsg_last_root_times", "alpsg_autotune", "alpsg_set_shape", "alpsg_get_shape",
            "alpsg_last_kernel_split", "alpsg_calc_eigen", "alpsg_server_create",
+           "alpsg_server_add_plasma", "alpsg_server_remove_plasma", "alpsg_server_submit_k",
            "alpsg_server_submit", "alpsg_server_poll", "alpsg_server_stats",
            "alpsg_server_destroy"]
 
@@ -115,12 +116,17 @@ def load_library(path: str = LIB_PATH):
     L.alpsg_server_create.argtypes = [C.c_double, C.c_int, C.POINTER(vp)]
     L.alpsg_server_submit.argtypes = [vp, C.POINTER(_Desc), C.c_double, C.c_double, dp,
                                       C.POINTER(_RootParams), C.c_longlong]
+    L.alpsg_server_add_plasma.argtypes = [vp, C.POINTER(_Desc), ip]
+    L.alpsg_server_remove_plasma.argtypes = [vp, C.c_int]
+    L.alpsg_server_submit_k.argtypes = [vp, C.c_int, C.c_double, C.c_double, dp,
+                                        C.POINTER(_RootParams), C.c_longlong]
     L.alpsg_server_poll.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_longlong), dp, ip, ip,
                                     dp, ip]
     L.alpsg_server_stats.argtypes = [vp, C.POINTER(C.c_longlong), dp]
     L.alpsg_server_destroy.argtypes = [vp]
     for name in ["alpsg_server_create", "alpsg_server_submit", "alpsg_server_poll",
-                 "alpsg_server_stats", "alpsg_server_destroy"]:
+                 "alpsg_server_stats", "alpsg_server_destroy", "alpsg_server_add_plasma",
+                 "alpsg_server_remove_plasma", "alpsg_server_submit_k"]:
         getattr(L, name).restype = C.c_int
     L.alpsg_last_kernel_ms.argtypes = [vp]
     L.alpsg_last_kernel_ms.restype = C.c_double
@@ -402,11 +408,13 @@ class Engine:
 class RootServer:
     """Continuous-batching root server (alpsg_server_* in include/alpsg.h).
 
-    Each request is one secant_osc solve (ALPS_fns.f90:1919-2101) of its own
-    plasma at its own wavevector.  ``submit`` queues it and returns at once;
-    ``poll`` returns finished solves as ``(tag, root, iters, status,
-    latency_ms)`` tuples.  The service thread inside the library batches every
-    admitted solve into shared device launches.
+    Each request is one secant_osc solve (ALPS_fns.f90:1919-2101) of a plasma
+    at its own wavevector.  ``add_plasma`` registers a plasma once (its device
+    tables are then shared by every request naming it); ``submit`` queues a
+    solve on a registered plasma id, or on a :class:`Plasma` given inline, and
+    returns at once; ``poll`` returns finished solves as ``(tag, root, iters,
+    status, latency_ms)`` tuples.  The service thread inside the library
+    batches every admitted solve into shared device launches.
     """
 
     def __init__(self, hbm_bytes: float = 8e9, max_slots: int = 4096):
@@ -425,13 +433,27 @@ class RootServer:
             self._descs[key] = hit
         return hit[1]
 
-    def submit(self, plasma: Plasma, kperp: float, kpar: float, guess: complex,
+    def add_plasma(self, plasma: Plasma) -> int:
+        d, keep = _make_desc(plasma)
+        pid = C.c_int(0)
+        _check(load_library().alpsg_server_add_plasma(self._h, C.byref(d), C.byref(pid)))
+        return pid.value
+
+    def remove_plasma(self, plasma_id: int):
+        _check(load_library().alpsg_server_remove_plasma(self._h, int(plasma_id)))
+
+    def submit(self, plasma, kperp: float, kpar: float, guess: complex,
                roots: RootParams | None = None, tag: int = 0):
-        rp = roots or RootParams(secant_method=2)
+        """Queue one solve; ``plasma`` is a registered id or a :class:`Plasma`."""
+        rp = roots or (plasma.roots if isinstance(plasma, Plasma) else RootParams(secant_method=2))
         om = np.array([complex(guess).real, complex(guess).imag], dtype=np.float64)
-        _check(load_library().alpsg_server_submit(
-            self._h, C.byref(self._desc(plasma)), float(kperp), float(kpar), _dp(om),
-            C.byref(_rp(rp)), int(tag)))
+        L = load_library()
+        if isinstance(plasma, (int, np.integer)):
+            _check(L.alpsg_server_submit_k(self._h, int(plasma), float(kperp), float(kpar),
+                                           _dp(om), C.byref(_rp(rp)), int(tag)))
+        else:
+            _check(L.alpsg_server_submit(self._h, C.byref(self._desc(plasma)), float(kperp),
+                                         float(kpar), _dp(om), C.byref(_rp(rp)), int(tag)))
 
     def poll(self, max_results: int = 1024, wait: bool = True):
         n = int(max_results)
@@ -462,14 +484,14 @@ class RootServer:
         return out
 
     def stats(self):
-        c = np.zeros(8, dtype=np.int64)
-        busy = C.c_double(0.0)
+        c = np.zeros(10, dtype=np.int64)
+        t = np.zeros(4)
         _check(load_library().alpsg_server_stats(
-            self._h, c.ctypes.data_as(C.POINTER(C.c_longlong)), C.byref(busy)))
+            self._h, c.ctypes.data_as(C.POINTER(C.c_longlong)), _dp(t)))
         keys = ("submitted", "finished", "passes", "evals", "admitted", "waiting",
-                "arena_bytes", "arena_free_bytes")
+                "arena_bytes", "arena_used_bytes", "plasma_uploads", "plasma_evictions")
         out = {k: int(v) for k, v in zip(keys, c)}
-        out["busy_ms"] = busy.value
+        out.update(pass_ms=t[0], admit_ms=t[1], launch_ms=t[2], wait_ms=t[3])
         return out
 
     def close(self):

@@ -140,9 +140,12 @@ def run_serve(args):
     pool, reqs = serve_requests(args.requests, args.variants)
     rng = np.random.default_rng(11)
     S = RootServer(hbm_bytes=args.hbm_gb * 1e9, max_slots=args.slots)
+    # the plasma variants are registered once; requests name them (inline
+    # plasmas with --inline: each request carries and uploads its own tables)
+    ref = list(pool) if args.inline else [S.add_plasma(p) for p in pool]
     # warm-up: one request of each grid size
     for v in range(min(len(pool), len(SERVE_GRIDS))):
-        S.submit(pool[v], 0.5, 0.5, 0.5 - 0.01j, pool[v].roots, tag=-1 - v)
+        S.submit(ref[v], 0.5, 0.5, 0.5 - 0.01j, pool[v].roots, tag=-1 - v)
     S.drain(min(len(pool), len(SERVE_GRIDS)))
     st0 = S.stats()
     gaps = rng.exponential(1.0 / args.rate, len(reqs))
@@ -152,7 +155,7 @@ def run_serve(args):
         dt = t_arr[i] - (time.perf_counter() - t0)
         if dt > 0:
             time.sleep(dt)
-        S.submit(pool[v], kp, kz, g, pool[v].roots, tag=i)
+        S.submit(ref[v], kp, kz, g, pool[v].roots, tag=i)
     t_sub = time.perf_counter() - t0
     got = S.drain(len(reqs))
     wall = time.perf_counter() - t0
@@ -200,8 +203,11 @@ def run_serve(args):
         "passes": passes,
         "disp_evals": evals,
         "evals_per_pass": evals / max(passes, 1),
-        "busy_ms": st["busy_ms"] - st0["busy_ms"],
+        "service_ms": {k: st[k] - st0[k] for k in ("pass_ms", "admit_ms", "launch_ms", "wait_ms")},
         "hbm_arena_gb": args.hbm_gb,
+        "plasmas": "inline per request" if args.inline else "registered once",
+        "plasma_uploads": st["plasma_uploads"] - st0["plasma_uploads"],
+        "plasma_evictions": st["plasma_evictions"] - st0["plasma_evictions"],
         "slots": args.slots,
         "plain_roots_per_sec": len(idx) / plain_t,
         "plain_sample": int(len(idx)),
@@ -259,6 +265,8 @@ def main():
     ap.add_argument("--hbm-gb", type=float, default=32.0)
     ap.add_argument("--plain-requests", type=int, default=32)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--inline", action="store_true",
+                    help="serve: every request carries its plasma (no registry)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "

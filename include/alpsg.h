@@ -191,29 +191,44 @@ int alpsg_last_kernel_split(alpsg_handle h, double *ms3);
    frequency -- the solve the reference runs once per root and per scan step
    (refine_guess, ALPS_fns.f90:3791-3857; om_scan, :2198-2600) -- served from
    one device.  A service thread admits waiting requests into a fixed HBM
-   arena of hbm_bytes (first fit by each request's footprint: its momentum
-   tables, then its per-k tables once nmax is known), at most max_slots solves
-   at a time, and advances every admitted solve by one disp() per pass, with
-   the solves of uneven grids and nmax in shared launches.  A request's result
-   does not depend on the other requests in flight.
+   arena of hbm_bytes (first come first served, by footprint: the plasma's
+   momentum tables, then the request's per-k tables once nmax is known), at
+   most max_slots solves at a time, and advances every admitted solve by one
+   disp() per pass, with solves of uneven grids and nmax in shared launches.
+   A request's result does not depend on the other requests in flight.
 
-   Served species: tabulated f0, non-relativistic, not use_bM.  rp must ask for
-   secant_osc (secant_method 2).  The descriptor's arrays are copied before
-   alpsg_server_submit returns.  alpsg_server_poll returns finished solves
-   (waits for one when wait != 0 and any are outstanding): tags [max_results],
-   roots [2*max_results], iters, status (ALPSG_OK, or ALPSG_E_ARG for a request
-   that cannot fit the arena or whose Bessel cutoff is not reached) and
-   latency_ms (submit to result); any may be NULL.  alpsg_server_stats gives
-   counts[8] = {submitted, finished, passes, disp evaluations, admitted,
-   waiting, arena bytes, free arena bytes} and the summed pass time.
+   Served species: tabulated f0, non-relativistic, not use_bM.  rp must ask
+   for secant_osc (secant_method 2).
+
+   alpsg_server_add_plasma copies a plasma description and returns its id;
+   its device tables are built on first use and kept while the arena has room
+   (least recently used unused plasmas are evicted when it has not).
+   alpsg_server_remove_plasma forgets the id; requests already submitted with
+   it still run.  alpsg_server_submit_k queues one solve on a registered
+   plasma; alpsg_server_submit does the same for a plasma given inline (copied
+   before it returns, freed with the request).
+
+   alpsg_server_poll returns finished solves (waits for one when wait != 0 and
+   any are outstanding): tags [max_results], roots [2*max_results], iters,
+   status (ALPSG_OK, or ALPSG_E_ARG for a request that cannot fit the arena
+   or whose Bessel cutoff is not reached) and latency_ms (submit to result);
+   any may be NULL.  alpsg_server_stats gives counts[10] = {submitted,
+   finished, passes, disp evaluations, admitted, waiting, arena bytes, arena
+   bytes in use, plasma uploads, plasma evictions} and times_ms[4] = the
+   service thread's time in passes, in admission work, in enqueueing the disp
+   launches and in waiting for the device.
    No counterpart in the reference, which solves one root at a time. */
 typedef struct alpsg_server alpsg_server;
 int alpsg_server_create(double hbm_bytes, int max_slots, alpsg_server **out);
+int alpsg_server_add_plasma(alpsg_server *s, const alpsg_plasma_desc *desc, int *plasma_id);
+int alpsg_server_remove_plasma(alpsg_server *s, int plasma_id);
+int alpsg_server_submit_k(alpsg_server *s, int plasma_id, double kperp, double kpar,
+                          const double *om_guess, const alpsg_root_params *rp, long long tag);
 int alpsg_server_submit(alpsg_server *s, const alpsg_plasma_desc *desc, double kperp, double kpar,
                         const double *om_guess, const alpsg_root_params *rp, long long tag);
 int alpsg_server_poll(alpsg_server *s, int max_results, int wait, long long *tags, double *roots,
                       int *iters, int *status, double *latency_ms, int *n_out);
-int alpsg_server_stats(alpsg_server *s, long long *counts, double *busy_ms);
+int alpsg_server_stats(alpsg_server *s, long long *counts, double *times_ms);
 int alpsg_server_destroy(alpsg_server *s);
 
 #ifdef __cplusplus

@@ -79,7 +79,7 @@ def test_served_roots_match_oracle_and_are_mix_independent():
         assert r[3] == 0
         assert r[1] == alone[i][1] and r[2] == alone[i][2], (rep, i, r, alone[i])
     assert st["finished"] == len(order) and st["admitted"] == 0
-    assert st["arena_free_bytes"] == st["arena_bytes"]
+    assert st["arena_used_bytes"] == 0
     # passes were shared: fewer passes than disp evaluations
     assert st["passes"] < st["evals"]
 
@@ -99,7 +99,7 @@ def test_small_arena_queues_and_rejects_what_cannot_fit():
     for i in range(len(reqs)):
         assert got[i][3] == 0 and got[i][1] == alone[i][1], (i, got[i], alone[i])
     assert got[999][3] == 1  # ALPSG_E_ARG: does not fit the arena
-    assert st["arena_free_bytes"] == st["arena_bytes"]
+    assert st["arena_used_bytes"] == 0
 
 
 def test_slot_limit_one_serialises_without_changing_results():
@@ -112,3 +112,39 @@ def test_slot_limit_one_serialises_without_changing_results():
     S.close()
     for i in range(len(reqs)):
         assert got[i][1] == alone[i][1]
+
+
+def test_registered_plasmas_give_the_same_roots_and_survive_eviction():
+    """Requests on registered plasmas (device tables shared between requests,
+    evicted under arena pressure and rebuilt) equal the inline ones bit for bit."""
+    import time
+    reqs = _requests()
+    alone = _alone(reqs)
+    for hbm in (2e9, 1 << 20):
+        S = _server(hbm=hbm, slots=64)
+        ids = {}
+        for p, *_ in reqs:
+            if id(p) not in ids:
+                ids[id(p)] = S.add_plasma(p)
+        for rep in range(2):
+            for i, (p, kp, kz, g) in enumerate(reqs):
+                S.submit(ids[id(p)], kp, kz, g, p.roots, tag=rep * 100 + i)
+        got = S.drain(2 * len(reqs))
+        for rep in range(2):
+            for i in range(len(reqs)):
+                r = got[rep * 100 + i]
+                assert r[3] == 0 and r[1] == alone[i][1], (hbm, rep, i, r, alone[i])
+        st = S.stats()
+        assert st["plasma_uploads"] >= len(ids)
+        if hbm > 1e9:
+            assert st["plasma_uploads"] == len(ids) and st["plasma_evictions"] == 0
+            assert st["arena_used_bytes"] > 0  # registered plasmas stay resident
+        else:  # every re-upload follows an eviction
+            assert st["plasma_uploads"] - len(ids) <= st["plasma_evictions"]
+        for pid in ids.values():
+            S.remove_plasma(pid)
+        t0 = time.time()
+        while S.stats()["arena_used_bytes"] and time.time() - t0 < 10:
+            time.sleep(0.01)
+        assert S.stats()["arena_used_bytes"] == 0
+        S.close()
