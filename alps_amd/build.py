@@ -48,7 +48,8 @@ def build_library(verbose: bool = False) -> str:
         _run(cmd)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", tmp, *objs])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-Wl,-z,defs", "-o", tmp,
+          *objs])
     os.replace(tmp, LIB)
     return LIB
 

@@ -247,6 +247,69 @@ __device__ __forceinline__ void sweep_nodes(cdptr srows, int npar, int c, double
   }
 }
 
+// sweep_nodes with the node records of K consecutive strides loaded before any
+// of them is used, so each lane keeps K records in flight; the nodes are
+// accumulated in the same order as sweep_nodes.
+template <bool STD, bool PAIR>
+__device__ __forceinline__ void node_pair(const double *rec, int i, double omr, double omi,
+                                          double mom, double kpar, double nq, double dim,
+                                          double dim2, int pl1, int pc1, int pl2, int pc2, int ml1,
+                                          int mc1, int ml2, int mc2, cd ap[6], cd am[6]) {
+  const double G1 = rec[0], H1 = rec[1], G2 = rec[2], H2 = rec[3];
+  const double G3 = rec[4], H3 = rec[5], p = rec[6], p2 = rec[7];
+  cd U1, U2, U3;
+  {
+#pragma clang fp contract(off)
+    U1 = mk(omr * G1 + H1, omi * G1);
+    U2 = mk(omr * G2 + H2, omi * G2);
+    U3 = mk(omr * G3 + H3, omi * G3);
+  }
+  const double base = __builtin_fma(-kpar, p, mom);
+  if constexpr (STD) {
+    node_acc<true>(1.0, base - nq, dim, dim2, U1, U2, U3, p, p2, ap);
+    if (PAIR) node_acc<true>(1.0, base + nq, dim, dim2, U1, U2, U3, p, p2, am);
+  } else {
+    const bool inP = ((unsigned)(i - pl1) < (unsigned)pc1) | ((unsigned)(i - pl2) < (unsigned)pc2);
+    const bool inM = ((unsigned)(i - ml1) < (unsigned)mc1) | ((unsigned)(i - ml2) < (unsigned)mc2);
+    node_acc<false>(inP ? 1.0 : 0.0, inP ? base - nq : 1.0, dim, dim2, U1, U2, U3, p, p2, ap);
+    if (PAIR)
+      node_acc<false>(inM ? 1.0 : 0.0, inM ? base + nq : 1.0, dim, dim2, U1, U2, U3, p, p2, am);
+  }
+}
+
+// sweep_nodes in chunks of K strides whose node records are all loaded before
+// the first is used, so each lane keeps K records in flight; a remainder loop
+// takes the last partial chunk.  The nodes are accumulated in the same order
+// as sweep_nodes.
+template <int NCH, bool STD, bool PAIR, int K>
+__device__ __forceinline__ void sweep_nodes_pf(cdptr srows, int npar, int c, double omr,
+                                               double omi, double m, double kpar, double nq,
+                                               double dim, int pl1, int pc1, int pl2, int pc2,
+                                               int ml1, int mc1, int ml2, int mc2, cd ap[6],
+                                               cd am[6]) {
+  const double dim2 = dim * dim;
+  const double mom = m * omr;
+  int i0 = 2 + c;
+  for (; i0 + (K - 1) * NCH <= npar - 2; i0 += NCH * K) {
+    double rec[K][NODE_STRIDE];
+#pragma unroll
+    for (int u = 0; u < K; ++u)
+#pragma unroll
+      for (int t = 0; t < NODE_STRIDE; ++t) rec[u][t] = srows[(i0 + u * NCH - 1) * NODE_STRIDE + t];
+#pragma unroll
+    for (int u = 0; u < K; ++u)
+      node_pair<STD, PAIR>(rec[u], i0 + u * NCH, omr, omi, mom, kpar, nq, dim, dim2, pl1, pc1, pl2,
+                           pc2, ml1, mc1, ml2, mc2, ap, am);
+  }
+  for (; i0 <= npar - 2; i0 += NCH) {
+    double rec[NODE_STRIDE];
+#pragma unroll
+    for (int t = 0; t < NODE_STRIDE; ++t) rec[t] = srows[(i0 - 1) * NODE_STRIDE + t];
+    node_pair<STD, PAIR>(rec, i0, omr, omi, mom, kpar, nq, dim, dim2, pl1, pc1, pl2, pc2, ml1, mc1,
+                         ml2, mc2, ap, am);
+  }
+}
+
 // endpoint contribution (weight 1) of node e for a row with denominator shift nq
 __device__ __forceinline__ void endpoint_acc(const double *__restrict__ rows, int e, int npar,
                                              double m, double kpar, double omr, double omi,
@@ -403,6 +466,131 @@ __global__ __launch_bounds__(64 * W) void k_sweep(const ProbDev *__restrict__ pr
       double2 *o = chiA + ((size_t)(wv * nspec + s) * 6) * nom + iom;
 #pragma unroll
       for (int k = 0; k < 6; ++k) o[(size_t)k * nom] = make_double2(lc[64 * (2 * k)], lc[64 * (2 * k + 1)]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_sweep_req: the node sums of k_sweep for batches in which every omega has
+// its own problem (the root server: one workgroup per omega, descriptor
+// probp[pidx[omega]]).  Such a batch reads each problem's node table once, so
+// the sweep streams the tables from memory; here a wavefront works on
+// RPW = 64 / NCH harmonic row pairs at once, NCH lanes per row, each lane
+// keeping K node records in flight.  Row pair nn of species s goes to row slot
+// (rowbase + nn) % (RPW * TW).  A lane multiplies its own node sums of a row by
+// the row's T constants and keeps the running chi of its rows in registers
+// (sum_rows T_row sum_nodes = sum_lanes sum_rows T_row sum_lane's nodes), so
+// the lanes are summed once per species, in a fixed order, before the
+// wavefront writes its partial chiA slot.  Every omega's result depends only on
+// its own problem and the launch shape.
+// ---------------------------------------------------------------------------
+template <int NCH, int W, int K>
+__global__ __launch_bounds__(64 * W) void k_sweep_req(const ProbDev *__restrict__ probp, int nom,
+                                                      const double2 *__restrict__ om_in,
+                                                      double2 *__restrict__ chiA,
+                                                      const int *__restrict__ pidx) {
+  constexpr int RPW = 64 / NCH;
+  const int iom = blockIdx.x;
+  if (iom >= nom) return;
+  const ProbDev &Pd = probp[pidx[iom]];
+  const int lane = threadIdx.x & 63;
+  const int TW = W * gridDim.y;
+  const int wv = blockIdx.y * W + (threadIdx.x >> 6);
+  const int g = lane / NCH, c = lane % NCH;
+  const int slot = wv * RPW + g, TS = TW * RPW;
+  const double2 ov = om_in[iom];
+  const double omr = ov.x, omi = ov.y;
+  const int nspec = Pd.nspec;
+  const double kpar = Pd.kpar;
+  int rowbase = 0;
+  for (int s = 0; s < nspec; ++s) {
+    const SpecDev &S = Pd.sp[s];
+    const int npar = S.npar;
+    const double m = S.m;
+    const double scaleN = 2.0 * (2.0 * PI_D * S.dpperp * S.dppar_abs * 0.25 * S.q);
+    double run[12];
+#pragma unroll
+    for (int t = 0; t < 12; ++t) run[t] = 0.0;
+    // first row pair of this slot, then every TS-th
+    int nn = ((slot - rowbase) % TS + TS) % TS;
+    for (; nn <= S.nmax; nn += TS) {
+      const double *rows = S.nodes + (size_t)nn * (npar - 1) * NODE_STRIDE;
+      const cdptr srows = (cdptr)rows;
+      RowClass rp, rm;
+      classify(S, Pd.P, kpar, omr, omi, nn, rp);
+      if (nn > 0) classify(S, Pd.P, kpar, omr, omi, -nn, rm);
+      const double nq = (1.0 * nn) * S.q;
+      const int pl1 = rp.a1 + 1, pc1 = max(rp.b1 - rp.a1 - 1, 0);
+      const int pl2 = rp.a2 + 1, pc2 = max(rp.b2 - rp.a2 - 1, 0);
+      const int ml1 = nn > 0 ? rm.a1 + 1 : 0, mc1 = nn > 0 ? max(rm.b1 - rm.a1 - 1, 0) : 0;
+      const int ml2 = nn > 0 ? rm.a2 + 1 : 0, mc2 = nn > 0 ? max(rm.b2 - rm.a2 - 1, 0) : 0;
+      cd ap[6], am[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) ap[k] = am[k] = mk(0.0, 0.0);
+      const double dim = m * omi;
+      const bool stdp = (rp.a2 < 0) && rp.a1 == 1 && rp.b1 == npar - 1;
+      const bool stdm = (nn == 0) || ((rm.a2 < 0) && rm.a1 == 1 && rm.b1 == npar - 1);
+      if (stdp && stdm) {
+        if (nn > 0)
+          sweep_nodes_pf<NCH, true, true, K>(srows, npar, c, omr, omi, m, kpar, nq, dim, 0, 0, 0, 0, 0, 0,
+                                       0, 0, ap, am);
+        else
+          sweep_nodes_pf<NCH, true, false, K>(srows, npar, c, omr, omi, m, kpar, nq, dim, 0, 0, 0, 0, 0,
+                                        0, 0, 0, ap, am);
+      } else {
+        sweep_nodes_pf<NCH, false, true, K>(srows, npar, c, omr, omi, m, kpar, nq, dim, pl1, pc1, pl2, pc2,
+                                      ml1, mc1, ml2, mc2, ap, am);
+      }
+      if (c == 0) {
+        endpoint_acc(rows, rp.a1, npar, m, kpar, omr, omi, nq, ap);
+        endpoint_acc(rows, rp.b1, npar, m, kpar, omr, omi, nq, ap);
+        if (rp.a2 > 0) {
+          endpoint_acc(rows, rp.a2, npar, m, kpar, omr, omi, nq, ap);
+          endpoint_acc(rows, rp.b2, npar, m, kpar, omr, omi, nq, ap);
+        }
+        if (nn > 0) {
+          endpoint_acc(rows, rm.a1, npar, m, kpar, omr, omi, -nq, am);
+          endpoint_acc(rows, rm.b1, npar, m, kpar, omr, omi, -nq, am);
+          if (rm.a2 > 0) {
+            endpoint_acc(rows, rm.a2, npar, m, kpar, omr, omi, -nq, am);
+            endpoint_acc(rows, rm.b2, npar, m, kpar, omr, omi, -nq, am);
+          }
+        }
+      }
+      // this lane's share of the row pair, times the row's T constants; the
+      // lanes are summed once per species below
+      cd vp[6], vm[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        vp[k] = (scaleN * ap[ACC_OF_MODE[k]]) * tconst(Pd, S, nn, k);
+        vm[k] = (scaleN * am[ACC_OF_MODE[k]]) * tconst(Pd, S, -nn, k);
+      }
+      if (nn == 0) {
+        run[2] += vp[1].re;
+        run[3] += vp[1].im;
+        run[4] += vp[2].re;
+        run[5] += vp[2].im;
+        run[10] += vp[5].re;
+        run[11] += vp[5].im;
+      } else {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+          const cd t = (mk(run[2 * k], run[2 * k + 1]) + vp[k]) + vm[k];
+          run[2 * k] = t.re;
+          run[2 * k + 1] = t.im;
+        }
+      }
+    }
+    rowbase += S.nmax + 1;
+    // the lanes of the wavefront, combined pairwise in a fixed order
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1)
+#pragma unroll
+      for (int t = 0; t < 12; ++t) run[t] += __shfl_xor(run[t], o, 64);
+    if (lane == 0) {
+      double2 *o = chiA + ((size_t)(wv * nspec + s) * 6) * nom + iom;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) o[(size_t)k * nom] = make_double2(run[2 * k], run[2 * k + 1]);
     }
   }
 }
@@ -976,6 +1164,24 @@ hipError_t launch_sweep(hipStream_t st, int nch, int W, int RC, const ProbDev *p
   SW(1, 1) SW(1, 2) SW(1, 4) SW(2, 4) SW(4, 4) SW(8, 4) SW(16, 4) SW(16, 8) SW(32, 8) SW(64, 4)
   SW(64, 8)
 #undef SW
+  return hipErrorInvalidValue;
+}
+
+// the per-omega-problem sweep (k_sweep_req): nch lanes per row pair, W
+// wavefronts per workgroup, RC workgroups per omega
+hipError_t launch_sweep_req(hipStream_t st, int nch, int W, int RC, int K, const ProbDev *prob,
+                            int nom, const double2 *om, double2 *chiA, const int *pidx) {
+  if (nom <= 0) return hipSuccess;
+  if (pidx == nullptr) return hipErrorInvalidValue;
+#define SQ(N, WV, KK)                                                                             \
+  if (nch == N && W == WV && K == KK) {                                                           \
+    hipLaunchKernelGGL((k_sweep_req<N, WV, KK>), dim3(nom, RC), dim3(64 * WV), 0, st, prob, nom,  \
+                       om, chiA, pidx);                                                           \
+    return hipGetLastError();                                                                     \
+  }
+  SQ(16, 8, 1) SQ(32, 8, 1) SQ(16, 8, 2) SQ(32, 8, 2) SQ(16, 8, 4) SQ(32, 8, 4) SQ(32, 4, 4)
+  SQ(64, 4, 2) SQ(64, 4, 4)
+#undef SQ
   return hipErrorInvalidValue;
 }
 

@@ -65,9 +65,8 @@ hipError_t launch_species_k_jobs(hipStream_t st, const TabJob *jobs, int njobs,
                                  int max_nperp, const int *mtab);
 hipError_t launch_scatter_probs(hipStream_t st, const ProbDev *src, const int *slots, int n,
                                 ProbDev *dst);
-hipError_t launch_sweep(hipStream_t st, int nch, int W, int RC, const ProbDev *prob, int nom,
-                        const double2 *om, const int *active, double2 *chiA, double2 *lowA,
-                        const int *pidx);
+hipError_t launch_sweep_req(hipStream_t st, int nch, int W, int RC, int K, const ProbDev *prob,
+                            int nom, const double2 *om, double2 *chiA, const int *pidx);
 hipError_t launch_resonant(hipStream_t st, int nch, int RB, const ProbDev *prob, int nspec, int nom,
                            const double2 *om, const int *active, double2 *chiB, double2 *lowB,
                            bool general, const int *pidx);
@@ -90,9 +89,15 @@ using alpsg_host::OscMachine;
   } while (0)
 
 constexpr int MTAB_S = 4096;
-// the fixed launch shape of a pass: k_sweep<64, 8> over 4 workgroups per omega,
-// k_resonant<64, *> over 8 harmonic blocks
-constexpr int SW_W = 8, SW_RC = 4, RS_RB = 8;
+// the fixed launch shape of a pass: k_sweep_req with SW_NCH lanes per row pair,
+// SW_W wavefronts per workgroup, SW_RC workgroups per omega and SW_K node records
+// in flight per lane; k_resonant<64, *> over RS_RB harmonic blocks
+#ifndef ALPSG_SERVE_SWEEP
+#define ALPSG_SERVE_SWEEP 32, 4, 4, 4
+#endif
+constexpr int SW_SHAPE[4] = {ALPSG_SERVE_SWEEP};
+constexpr int SW_NCH = SW_SHAPE[0], SW_W = SW_SHAPE[1], SW_RC = SW_SHAPE[2], SW_K = SW_SHAPE[3];
+constexpr int RS_RB = 8;
 constexpr size_t ALIGN = 256;
 constexpr int MAX_JOBS = 8192;               // (request, species) jobs per table launch
 constexpr size_t STAGE_CAP = (size_t)1 << 20;  // doubles read back per pass (int_ee, max|J_n|)
@@ -734,8 +739,8 @@ int server_pass(alpsg_server *S) {
       for (; i1 < nb && act[i1]->pl->nspec == ns; ++i1)
         if (act[i1]->pl->general) gen = true;
       const int n = i1 - i0;
-      SCHK(launch_sweep(S->st, 64, SW_W, SW_RC, S->probs_d, n, S->om_d + i0, nullptr, S->chiA,
-                        nullptr, S->pidx_d + i0));
+      SCHK(launch_sweep_req(S->st, SW_NCH, SW_W, SW_RC, SW_K, S->probs_d, n, S->om_d + i0,
+                            S->chiA, S->pidx_d + i0));
       SCHK(launch_resonant(S->st, 64, RS_RB, S->probs_d, ns, n, S->om_d + i0, nullptr, S->chiB,
                            nullptr, gen, S->pidx_d + i0));
       SCHK(launch_assemble(S->st, S->probs_d, ns, n, SW_W * SW_RC, RS_RB, S->om_d + i0, nullptr,
@@ -846,8 +851,8 @@ void service_loop(alpsg_server *S) {
   }
 }
 
-int enqueue(alpsg_server *S, std::shared_ptr<Plasma> pl, double kperp, double kpar,
-            const double *om_guess, const alpsg_root_params *rp, long long tag) {
+int make_request(std::shared_ptr<Plasma> pl, double kperp, double kpar, const double *om_guess,
+                 const alpsg_root_params *rp, long long tag, std::unique_ptr<Request> &out) {
   if (!om_guess || !rp) return alpsg_internal_fail(ALPSG_E_ARG, "null argument");
   if (!(kpar != 0.0) || !std::isfinite(kpar) || !std::isfinite(kperp))
     return alpsg_internal_fail(ALPSG_E_ARG, "kpar must be finite and nonzero");
@@ -864,12 +869,21 @@ int enqueue(alpsg_server *S, std::shared_ptr<Plasma> pl, double kperp, double kp
   r->rp = *rp;
   r->ntry = ntry_bound(*pl, kperp);
   r->pl = std::move(pl);
-  r->t_submit = now_ms();
+  out = std::move(r);
+  return ALPSG_OK;
+}
+
+// queue prepared requests (all or none)
+int enqueue(alpsg_server *S, std::vector<std::unique_ptr<Request>> &rs) {
+  const double t = now_ms();
   std::lock_guard<std::mutex> lk(S->mu);
   if (S->failed) return alpsg_internal_fail(ALPSG_E_DEVICE, "server stopped: " + S->fail_msg);
   if (S->stop) return alpsg_internal_fail(ALPSG_E_STATE, "server is shutting down");
-  S->queue.push_back(std::move(r));
-  S->submitted += 1;
+  for (auto &r : rs) {
+    r->t_submit = t;
+    S->queue.push_back(std::move(r));
+  }
+  S->submitted += (long long)rs.size();
   S->cv.notify_all();
   return ALPSG_OK;
 }
@@ -951,17 +965,35 @@ int alpsg_server_remove_plasma(alpsg_server *S, int plasma_id) {
   return ALPSG_OK;
 }
 
-int alpsg_server_submit_k(alpsg_server *S, int plasma_id, double kperp, double kpar,
-                          const double *om_guess, const alpsg_root_params *rp, long long tag) {
+int alpsg_server_submit_batch(alpsg_server *S, int n, const int *plasma_ids, const double *kperp,
+                              const double *kpar, const double *om_guess,
+                              const alpsg_root_params *rp, const long long *tags) {
   if (!S) return alpsg_internal_fail(ALPSG_E_ARG, "null handle");
-  std::shared_ptr<Plasma> pl;
+  if (n < 0) return alpsg_internal_fail(ALPSG_E_ARG, "negative request count");
+  if (n == 0) return ALPSG_OK;
+  if (!plasma_ids || !kperp || !kpar || !om_guess || !rp || !tags)
+    return alpsg_internal_fail(ALPSG_E_ARG, "null argument");
+  std::vector<std::shared_ptr<Plasma>> pls((size_t)n);
   {
     std::lock_guard<std::mutex> lk(S->mu);
-    auto it = S->registry.find(plasma_id);
-    if (it == S->registry.end()) return alpsg_internal_fail(ALPSG_E_ARG, "unknown plasma id");
-    pl = it->second;
+    for (int i = 0; i < n; ++i) {
+      auto it = S->registry.find(plasma_ids[i]);
+      if (it == S->registry.end()) return alpsg_internal_fail(ALPSG_E_ARG, "unknown plasma id");
+      pls[i] = it->second;
+    }
   }
-  return enqueue(S, std::move(pl), kperp, kpar, om_guess, rp, tag);
+  std::vector<std::unique_ptr<Request>> rs((size_t)n);
+  for (int i = 0; i < n; ++i) {
+    const int rc = make_request(std::move(pls[i]), kperp[i], kpar[i], om_guess + 2 * (size_t)i, rp,
+                                tags[i], rs[i]);
+    if (rc) return rc;
+  }
+  return enqueue(S, rs);
+}
+
+int alpsg_server_submit_k(alpsg_server *S, int plasma_id, double kperp, double kpar,
+                          const double *om_guess, const alpsg_root_params *rp, long long tag) {
+  return alpsg_server_submit_batch(S, 1, &plasma_id, &kperp, &kpar, om_guess, rp, &tag);
 }
 
 int alpsg_server_submit(alpsg_server *S, const alpsg_plasma_desc *d, double kperp, double kpar,
@@ -970,7 +1002,10 @@ int alpsg_server_submit(alpsg_server *S, const alpsg_plasma_desc *d, double kper
   auto pl = std::make_shared<Plasma>();  // unregistered: freed with its request
   const int rc = prepare_plasma(d, *pl);
   if (rc) return rc;
-  return enqueue(S, std::move(pl), kperp, kpar, om_guess, rp, tag);
+  std::vector<std::unique_ptr<Request>> rs(1);
+  const int rc2 = make_request(std::move(pl), kperp, kpar, om_guess, rp, tag, rs[0]);
+  if (rc2) return rc2;
+  return enqueue(S, rs);
 }
 
 int alpsg_server_poll(alpsg_server *S, int max_results, int wait, long long *tags, double *roots,

@@ -354,16 +354,76 @@ __global__ void k_atab_jobs(const TabJob *__restrict__ jobs, const int *__restri
   atab_entry(bx * 64 + threadIdx.x, by, J.nperp, J.nmax, J.pperp, J.bes, J.atab);
 }
 
-__global__ void k_moments_jobs(const TabJob *__restrict__ jobs, const int *__restrict__ start,
-                               int njobs) {
-  extern __shared__ double lds_a[];  // (max nperp - 1) * 3
+// moments_block for MB consecutive harmonics n0 .. n0+MB-1 (those above nmax
+// are computed on n = nmax and not stored): every df0 load serves MB
+// harmonics, and the a-values, uniform over the block, are scalar loads.
+constexpr int MB = 8;
+typedef const __attribute__((address_space(4))) double *cdptr_t;
+
+__device__ __forceinline__ void moments_block_mb(int bx, int n0, int nperp, int npar, int nmax,
+                                                 double kpar, double m,
+                                                 const double *__restrict__ pperp,
+                                                 const double *__restrict__ ppar,
+                                                 const double *__restrict__ df1,
+                                                 const double *__restrict__ df2,
+                                                 const double *__restrict__ atab,
+                                                 double *__restrict__ nodes) {
+  const int ipar = 1 + bx * 64 + threadIdx.x;
+  const int ip = ipar <= npar - 1 ? ipar : npar - 1;
+  const int W = nperp + 1;
+  const double pa = ppar[ip];
+  const double kom = kpar / m;
+  const cdptr_t at = (cdptr_t)atab;
+  const cdptr_t pe = (cdptr_t)pperp;
+  double g[MB][3], h[MB][3];
+#pragma unroll
+  for (int u = 0; u < MB; ++u)
+#pragma unroll
+    for (int t = 0; t < 3; ++t) g[u][t] = h[u][t] = 0.0;
+  for (int iperp = 1; iperp <= nperp - 1; ++iperp) {
+    const size_t o = (size_t)iperp * (npar + 1) + ip;
+    const double d1 = df1[o], d2 = df2[o];
+    const double A = kom * (pe[iperp] * d2 - pa * d1);
+    const double wperp = (iperp == nperp - 1) ? 1.0 : 2.0;
+#pragma unroll
+    for (int u = 0; u < MB; ++u) {
+      const int n = min(n0 + u, nmax);
+      const cdptr_t a = at + ((size_t)n * W + iperp) * ATAB_STRIDE;
+      const double a1 = wperp * a[0], a2 = wperp * a[1], a3 = wperp * a[2];
+      g[u][0] += a1 * d1;
+      h[u][0] += a1 * A;
+      g[u][1] += a2 * d1;
+      h[u][1] += a2 * A;
+      g[u][2] += a3 * d1;
+      h[u][2] += a3 * A;
+    }
+  }
+  if (ipar > npar - 1) return;
+#pragma unroll
+  for (int u = 0; u < MB; ++u) {
+    const int n = n0 + u;
+    if (n > nmax) break;
+    double *o = nodes + ((size_t)n * (npar - 1) + (ipar - 1)) * NODE_STRIDE;
+    o[0] = g[u][0];
+    o[1] = h[u][0];
+    o[2] = g[u][1];
+    o[3] = h[u][1];
+    o[4] = g[u][2];
+    o[5] = h[u][2];
+    o[6] = pa;
+    o[7] = pa * pa;
+  }
+}
+
+__global__ __launch_bounds__(64) void k_moments_jobs(const TabJob *__restrict__ jobs,
+                                                     const int *__restrict__ start, int njobs) {
   const int j = job_of(start, njobs, blockIdx.x);
   const TabJob &J = jobs[j];
   const int b = blockIdx.x - start[j];
   const int tiles = (J.npar - 1 + 63) / 64;
   const int bx = b % tiles, by = b / tiles;
-  moments_block(bx, by, J.nperp, J.npar, J.kpar, J.m, J.pperp, J.ppar, J.df1, J.df2, J.atab,
-                J.nodes, lds_a);
+  moments_block_mb(bx, by * MB, J.nperp, J.npar, J.nmax, J.kpar, J.m, J.pperp, J.ppar, J.df1,
+                   J.df2, J.atab, J.nodes);
 }
 
 // ---------------------------------------------------------------------------
@@ -434,7 +494,9 @@ hipError_t launch_species_k(hipStream_t st, int nperp, int npar, int nmax, doubl
 int jobs_blocks_bessel_max(const TabJob &J) { return J.ntry; }
 int jobs_blocks_bessel(const TabJob &J) { return ((J.nperp + 1 + 63) / 64) * (J.nmax + 3); }
 int jobs_blocks_atab(const TabJob &J) { return ((J.nperp + 1 + 63) / 64) * (J.nmax + 1); }
-int jobs_blocks_moments(const TabJob &J) { return ((J.npar - 1 + 63) / 64) * (J.nmax + 1); }
+int jobs_blocks_moments(const TabJob &J) {
+  return ((J.npar - 1 + 63) / 64) * ((J.nmax + 1 + MB - 1) / MB);
+}
 
 // start arrays: [njobs + 1] each, device copies; total = start[njobs]
 hipError_t launch_bessel_max_jobs(hipStream_t st, const TabJob *jobs, const int *start, int njobs,
@@ -451,8 +513,8 @@ hipError_t launch_species_k_jobs(hipStream_t st, const TabJob *jobs, int njobs,
   if (njobs <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_bessel_jobs, dim3(total_bes), dim3(64), 0, st, jobs, start_bes, njobs, mtab);
   hipLaunchKernelGGL(k_atab_jobs, dim3(total_atab), dim3(64), 0, st, jobs, start_atab, njobs);
-  const size_t lds = sizeof(double) * 3 * (size_t)(max_nperp - 1);
-  hipLaunchKernelGGL(k_moments_jobs, dim3(total_mom), dim3(64), lds, st, jobs, start_mom, njobs);
+  (void)max_nperp;
+  hipLaunchKernelGGL(k_moments_jobs, dim3(total_mom), dim3(64), 0, st, jobs, start_mom, njobs);
   return hipGetLastError();
 }
 

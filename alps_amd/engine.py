@@ -41,6 +41,7 @@ EXPORTS = ["alpsg_last_error", "alpsg_create", "alpsg_destroy", "alpsg_set_k",
            "alpsg_last_root_times", "alpsg_autotune", "alpsg_set_shape", "alpsg_get_shape",
            "alpsg_last_kernel_split", "alpsg_calc_eigen", "alpsg_server_create",
            "alpsg_server_add_plasma", "alpsg_server_remove_plasma", "alpsg_server_submit_k",
+           "alpsg_server_submit_batch",
            "alpsg_server_submit", "alpsg_server_poll", "alpsg_server_stats",
            "alpsg_server_destroy"]
 
@@ -120,13 +121,16 @@ def load_library(path: str = LIB_PATH):
     L.alpsg_server_remove_plasma.argtypes = [vp, C.c_int]
     L.alpsg_server_submit_k.argtypes = [vp, C.c_int, C.c_double, C.c_double, dp,
                                         C.POINTER(_RootParams), C.c_longlong]
+    L.alpsg_server_submit_batch.argtypes = [vp, C.c_int, ip, dp, dp, dp, C.POINTER(_RootParams),
+                                            C.POINTER(C.c_longlong)]
     L.alpsg_server_poll.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_longlong), dp, ip, ip,
                                     dp, ip]
     L.alpsg_server_stats.argtypes = [vp, C.POINTER(C.c_longlong), dp]
     L.alpsg_server_destroy.argtypes = [vp]
     for name in ["alpsg_server_create", "alpsg_server_submit", "alpsg_server_poll",
                  "alpsg_server_stats", "alpsg_server_destroy", "alpsg_server_add_plasma",
-                 "alpsg_server_remove_plasma", "alpsg_server_submit_k"]:
+                 "alpsg_server_remove_plasma", "alpsg_server_submit_k",
+                 "alpsg_server_submit_batch"]:
         getattr(L, name).restype = C.c_int
     L.alpsg_last_kernel_ms.argtypes = [vp]
     L.alpsg_last_kernel_ms.restype = C.c_double
@@ -454,6 +458,21 @@ class RootServer:
         else:
             _check(L.alpsg_server_submit(self._h, C.byref(self._desc(plasma)), float(kperp),
                                          float(kpar), _dp(om), C.byref(_rp(rp)), int(tag)))
+
+    def submit_batch(self, plasma_ids, kperp, kpar, guesses, roots: RootParams, tags):
+        """Queue len(tags) solves on registered plasmas in one call."""
+        ids = np.ascontiguousarray(plasma_ids, dtype=np.int32)
+        kp = np.ascontiguousarray(kperp, dtype=np.float64)
+        kz = np.ascontiguousarray(kpar, dtype=np.float64)
+        g = np.asarray(guesses, dtype=np.complex128)
+        om = np.ascontiguousarray(np.stack([g.real, g.imag], axis=-1).ravel())
+        tg = np.ascontiguousarray(tags, dtype=np.int64)
+        n = tg.size
+        if not (ids.size == kp.size == kz.size == g.size == n):
+            raise ValueError("submit_batch: arrays of different lengths")
+        _check(load_library().alpsg_server_submit_batch(
+            self._h, int(n), _ip(ids), _dp(kp), _dp(kz), _dp(om), C.byref(_rp(roots)),
+            tg.ctypes.data_as(C.POINTER(C.c_longlong))))
 
     def poll(self, max_results: int = 1024, wait: bool = True):
         n = int(max_results)

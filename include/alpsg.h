@@ -205,8 +205,10 @@ int alpsg_last_kernel_split(alpsg_handle h, double *ms3);
    (least recently used unused plasmas are evicted when it has not).
    alpsg_server_remove_plasma forgets the id; requests already submitted with
    it still run.  alpsg_server_submit_k queues one solve on a registered
-   plasma; alpsg_server_submit does the same for a plasma given inline (copied
-   before it returns, freed with the request).
+   plasma; alpsg_server_submit_batch queues n of them in one call (all or none;
+   plasma_ids, kperp, kpar, tags [n], om_guess [2n], one rp for all);
+   alpsg_server_submit does the same for a plasma given inline (copied before
+   it returns, freed with the request).
 
    alpsg_server_poll returns finished solves (waits for one when wait != 0 and
    any are outstanding): tags [max_results], roots [2*max_results], iters,
@@ -224,6 +226,9 @@ int alpsg_server_add_plasma(alpsg_server *s, const alpsg_plasma_desc *desc, int 
 int alpsg_server_remove_plasma(alpsg_server *s, int plasma_id);
 int alpsg_server_submit_k(alpsg_server *s, int plasma_id, double kperp, double kpar,
                           const double *om_guess, const alpsg_root_params *rp, long long tag);
+int alpsg_server_submit_batch(alpsg_server *s, int n, const int *plasma_ids, const double *kperp,
+                              const double *kpar, const double *om_guess,
+                              const alpsg_root_params *rp, const long long *tags);
 int alpsg_server_submit(alpsg_server *s, const alpsg_plasma_desc *desc, double kperp, double kpar,
                         const double *om_guess, const alpsg_root_params *rp, long long tag);
 int alpsg_server_poll(alpsg_server *s, int max_results, int wait, long long *tags, double *roots,

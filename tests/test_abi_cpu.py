@@ -34,6 +34,16 @@ def test_library_exports_every_declared_symbol():
         assert re.search(rf"\bT {s}\b", out), s
 
 
+def test_library_has_no_unresolved_internal_symbols():
+    """Every symbol the library needs resolves at load time (RTLD_NOW), so a
+    launcher missing from one translation unit fails here, not on a GPU."""
+    from alps_amd import engine
+    C.CDLL(engine.LIB_PATH, mode=os.RTLD_NOW)
+    out = subprocess.run(["nm", "-D", "--undefined-only", engine.LIB_PATH], capture_output=True,
+                         text=True).stdout
+    assert not re.search(r"\blaunch_\w+|_Z\d+launch_", out), out
+
+
 def test_library_holds_gfx950_code():
     from alps_amd import engine
     data = open(engine.LIB_PATH, "rb").read()

@@ -126,9 +126,11 @@ def test_registered_plasmas_give_the_same_roots_and_survive_eviction():
         for p, *_ in reqs:
             if id(p) not in ids:
                 ids[id(p)] = S.add_plasma(p)
-        for rep in range(2):
-            for i, (p, kp, kz, g) in enumerate(reqs):
-                S.submit(ids[id(p)], kp, kz, g, p.roots, tag=rep * 100 + i)
+        for i, (p, kp, kz, g) in enumerate(reqs):
+            S.submit(ids[id(p)], kp, kz, g, p.roots, tag=i)
+        # the same requests again, queued in one call
+        S.submit_batch([ids[id(p)] for p, *_ in reqs], [r[1] for r in reqs], [r[2] for r in reqs],
+                       [r[3] for r in reqs], reqs[0][0].roots, [100 + i for i in range(len(reqs))])
         got = S.drain(2 * len(reqs))
         for rep in range(2):
             for i in range(len(reqs)):
