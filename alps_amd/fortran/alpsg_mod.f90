@@ -44,6 +44,9 @@ module alpsg
   public :: alpsg_secant_batch, alpsg_map_search, alpsg_om_scan, alpsg_get_nmax
   public :: alpsg_eval_count, alpsg_error_text, alpsg_disp, alpsg_secant_osc
   public :: alpsg_autotune, alpsg_set_shape, alpsg_get_shape, alpsg_calc_eigen
+  public :: alpsg_server_create, alpsg_server_add_plasma, alpsg_server_remove_plasma
+  public :: alpsg_server_submit_k, alpsg_server_submit_batch, alpsg_server_poll
+  public :: alpsg_server_destroy
 
   interface
      function alpsg_create(desc, h) bind(C, name='alpsg_create') result(rc)
@@ -172,6 +175,77 @@ module alpsg
        integer(c_int), intent(out) :: shape_out(5)
        integer(c_int) :: rc
      end function alpsg_get_shape
+
+     !! root server: queue secant_osc solves of any plasma and k, collect roots
+     function alpsg_server_create(hbm_bytes, max_slots, s) &
+          bind(C, name='alpsg_server_create') result(rc)
+       import :: c_int, c_ptr, c_double
+       real(c_double), value :: hbm_bytes
+       integer(c_int), value :: max_slots
+       type(c_ptr), intent(out) :: s
+       integer(c_int) :: rc
+     end function alpsg_server_create
+
+     function alpsg_server_add_plasma(s, desc, plasma_id) &
+          bind(C, name='alpsg_server_add_plasma') result(rc)
+       import :: c_int, c_ptr, alpsg_plasma_desc
+       type(c_ptr), value :: s
+       type(alpsg_plasma_desc), intent(in) :: desc
+       integer(c_int), intent(out) :: plasma_id
+       integer(c_int) :: rc
+     end function alpsg_server_add_plasma
+
+     function alpsg_server_remove_plasma(s, plasma_id) &
+          bind(C, name='alpsg_server_remove_plasma') result(rc)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: s
+       integer(c_int), value :: plasma_id
+       integer(c_int) :: rc
+     end function alpsg_server_remove_plasma
+
+     function alpsg_server_submit_k(s, plasma_id, kperp, kpar, om_guess, rp, tag) &
+          bind(C, name='alpsg_server_submit_k') result(rc)
+       import :: c_int, c_ptr, c_double, c_double_complex, c_long_long, alpsg_root_params
+       type(c_ptr), value :: s
+       integer(c_int), value :: plasma_id
+       real(c_double), value :: kperp, kpar
+       complex(c_double_complex), intent(in) :: om_guess
+       type(alpsg_root_params), intent(in) :: rp
+       integer(c_long_long), value :: tag
+       integer(c_int) :: rc
+     end function alpsg_server_submit_k
+
+     function alpsg_server_submit_batch(s, n, plasma_ids, kperp, kpar, om_guess, rp, tags) &
+          bind(C, name='alpsg_server_submit_batch') result(rc)
+       import :: c_int, c_ptr, c_double, c_double_complex, c_long_long, alpsg_root_params
+       type(c_ptr), value :: s
+       integer(c_int), value :: n
+       integer(c_int), intent(in) :: plasma_ids(*)
+       real(c_double), intent(in) :: kperp(*), kpar(*)
+       complex(c_double_complex), intent(in) :: om_guess(*)
+       type(alpsg_root_params), intent(in) :: rp
+       integer(c_long_long), intent(in) :: tags(*)
+       integer(c_int) :: rc
+     end function alpsg_server_submit_batch
+
+     function alpsg_server_poll(s, max_results, wait, tags, roots, iters, status, latency_ms, &
+          n_out) bind(C, name='alpsg_server_poll') result(rc)
+       import :: c_int, c_ptr, c_double, c_double_complex, c_long_long
+       type(c_ptr), value :: s
+       integer(c_int), value :: max_results, wait
+       integer(c_long_long), intent(out) :: tags(*)
+       complex(c_double_complex), intent(out) :: roots(*)
+       integer(c_int), intent(out) :: iters(*), status(*)
+       real(c_double), intent(out) :: latency_ms(*)
+       integer(c_int), intent(out) :: n_out
+       integer(c_int) :: rc
+     end function alpsg_server_poll
+
+     function alpsg_server_destroy(s) bind(C, name='alpsg_server_destroy') result(rc)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: s
+       integer(c_int) :: rc
+     end function alpsg_server_destroy
 
      function alpsg_last_error_c() bind(C, name='alpsg_last_error') result(p)
        import :: c_ptr
