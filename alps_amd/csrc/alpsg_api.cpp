@@ -1728,3 +1728,196 @@ int alpsg_om_scan(alpsg_handle h, const alpsg_root_params *rp, int scan_type, in
 }
 
 }  // extern "C"
+
+// om_double_scan (ALPS_fns.f90:2900-3591): scan 1 outer; from every output
+// point of scan 1, scan 2 inner.  Each k step solves the roots still followed
+// in one batch (the reference solves them one after another; a root's solve
+// reads only its own previous root, and the root-too-close checks run in root
+// order after the batch, so the results are the same).
+namespace {
+
+void scan_k(const alpsg_scan_params *sc, int it, double kperp_last, double kpar_last, double &kperp,
+            double &kpar) {
+  const double theta_0 = std::atan(kperp_last / kpar_last);
+  const double k_0 = std::sqrt(kperp_last * kperp_last + kpar_last * kpar_last);
+  switch (sc->scan_type) {
+    case 0:
+      if (sc->log_scan) {
+        kperp = std::pow(10.0, std::log10(kperp_last) + sc->diff * it);
+        kpar = std::pow(10.0, std::log10(kpar_last) + sc->diff2 * it);
+      } else {
+        kperp = kperp_last + sc->diff * it;
+        kpar = kpar_last + sc->diff2 * it;
+      }
+      break;
+    case 1: {
+      const double th = sc->log_scan ? std::pow(10.0, std::log10(theta_0) + sc->diff * it)
+                                     : theta_0 + sc->diff * it;
+      kperp = k_0 * std::sin(th);
+      kpar = k_0 * std::cos(th);
+    } break;
+    case 2: {
+      const double kt = sc->log_scan ? std::pow(10.0, std::log10(k_0) + sc->diff * it) : (k_0) + sc->diff * it;
+      kperp = kt * std::sin(theta_0);
+      kpar = kt * std::cos(theta_0);
+    } break;
+    case 3:
+      kperp = sc->log_scan ? std::pow(10.0, std::log10(kperp_last) + sc->diff * it) : kperp_last + sc->diff * it;
+      break;
+    case 4:
+      kpar = sc->log_scan ? std::pow(10.0, std::log10(kpar_last) + sc->diff * it) : kpar_last + sc->diff * it;
+      break;
+  }
+}
+
+// a kpar-only scan keeps nmax and the Bessel table (ALPS_fns.f90:3224-3231)
+int move_k(alpsg_ctx *h, int type, double kperp, double kpar) {
+  if (type != 4) return alpsg_set_k(h, kperp, kpar, nullptr, nullptr);
+  int nm[ALPSG_MAXSPEC];
+  for (int s = 0; s < h->nspec; ++s) nm[s] = h->usebm[s] ? 1 : h->nmax[s];
+  return alpsg_set_k(h, kperp, kpar, nm, nullptr);
+}
+
+}  // namespace
+
+extern "C" int alpsg_om_double_scan(alpsg_handle h, const alpsg_root_params *rp,
+                                    const alpsg_scan_params *s1, const alpsg_scan_params *s2,
+                                    int nroots, double *roots, int eigen, int heat,
+                                    double *kperp_out, double *kpar_out, double *roots_out,
+                                    int *written, double *ef, double *bf, double *Us, double *ds,
+                                    double *Ps, double *Ps_split, double *W_EM) {
+  if (!h || !rp || !s1 || !s2 || !roots || !kperp_out || !kpar_out || !roots_out || !written)
+    return fail(ALPSG_E_ARG, "null argument");
+  if (!h->have_k) return fail(ALPSG_E_STATE, "alpsg_set_k has not been called");
+  if (nroots < 1) return fail(ALPSG_E_ARG, "nroots < 1");
+  for (const alpsg_scan_params *sc : {s1, s2})
+    if (sc->scan_type < 0 || sc->scan_type > 4 || sc->n_out < 1 || sc->n_res < 1)
+      return fail(ALPSG_E_ARG, "scan parameters");
+  if (s1->scan_type == s2->scan_type) return fail(ALPSG_E_ARG, "both scans of the same type (alps_error(5))");
+  if (s1->scan_type == 0 || s2->scan_type == 0)
+    return fail(ALPSG_E_ARG, "scan type 0 in a double scan (alps_error(6))");
+  if ((eigen || heat) && (!ef || !bf))
+    return fail(ALPSG_E_ARG, "ef and bf are needed with eigen or heat");
+  if (eigen && (!Us || !ds)) return fail(ALPSG_E_ARG, "Us and ds are needed with eigen");
+  if (heat && (!Ps || !Ps_split || !W_EM)) return fail(ALPSG_E_ARG, "Ps, Ps_split and W_EM are needed with heat");
+  const int ns = h->nspec;
+  const int nt = s1->n_out * s1->n_res, nt2 = s2->n_out * s2->n_res;
+  const int nrow2 = s2->n_out + 1;
+  const size_t cells = (size_t)(s1->n_out + 1) * nrow2 * nroots;
+  for (size_t c = 0; c < cells; ++c) written[c] = 0;
+  cplx *R = reinterpret_cast<cplx *>(roots);
+  cplx *RO = reinterpret_cast<cplx *>(roots_out);
+  std::vector<int> jump(nroots, 1);
+  std::vector<cplx> omlast(R, R + nroots), wroots(R, R + nroots);
+  const double kperp_last = h->kperp, kpar_last = h->kpar;
+  std::vector<cplx> om, Df;
+  std::vector<int> idx;
+  // solve the followed roots at the handle's k from `from`; results into wroots
+  auto solve = [&](const std::vector<cplx> &from) -> int {
+    om.clear();
+    idx.clear();
+    for (int r = 0; r < nroots; ++r)
+      if (jump[r]) {
+        om.push_back(from[r]);
+        idx.push_back(r);
+      }
+    Df.assign(om.size(), cplx(0.0, 0.0));
+    const int rc = root_batch(h, rp, (int)om.size(), om.data(), Df.data(), nullptr);
+    if (rc) return rc;
+    for (size_t t = 0; t < idx.size(); ++t) wroots[idx[t]] = om[t];
+    return ALPSG_OK;
+  };
+  auto bad = [](cplx w) { return std::isnan(w.real()) || std::fabs(w.real()) >= 1.79769313486231570815e+308; };
+  int rc = ALPSG_OK;
+  for (int it = 0; it <= nt; ++it) {
+    double kperp = h->kperp, kpar = h->kpar;
+    scan_k(s1, it, kperp_last, kpar_last, kperp, kpar);
+    if ((rc = move_k(h, s1->scan_type, kperp, kpar))) return rc;
+    bool any = false;
+    for (int r = 0; r < nroots; ++r) any = any || jump[r];
+    if (!any) return fail(ALPSG_E_NOTFOUND, "all roots dropped during the scan (alps_error(9))");
+    if ((rc = solve(omlast))) return rc;
+    for (size_t t = 0; t < idx.size(); ++t) {
+      const int r = idx[t];
+      omlast[r] = wroots[r];
+      if (bad(wroots[r])) jump[r] = 0;
+      for (int imm = 0; imm < r; ++imm)
+        if (std::abs(wroots[r] - wroots[imm]) < rp->D_gap) {
+          wroots[r] = 0.0;
+          jump[r] = 0;
+        }
+    }
+    if (it % s1->n_res == 0) {
+      const double kperpi = h->kperp, kpari = h->kpar;
+      wroots = omlast;
+      for (int it2 = 0; it2 <= nt2; ++it2) {
+        kperp = h->kperp;
+        kpar = h->kpar;
+        scan_k(s2, it2, kperpi, kpari, kperp, kpar);
+        if ((rc = move_k(h, s2->scan_type, kperp, kpar))) return rc;
+        const bool out = (it2 % s2->n_res == 0);
+        const int row = (it / s1->n_res) * nrow2 + it2 / s2->n_res;
+        std::vector<cplx> from = wroots;
+        if ((rc = solve(from))) return rc;
+        if (out && (eigen || heat) && !idx.empty()) {
+          const size_t m = idx.size();
+          std::vector<cplx> e3(3 * m), b3(3 * m), us(3 * ns * m), dd(ns * m);
+          std::vector<double> ps(ns * m), psp(4 * ns * m), ew(m);
+          std::vector<cplx> w(m);
+          for (size_t t = 0; t < m; ++t) w[t] = wroots[idx[t]];
+          if ((rc = calc_eigen_batch(h, (int)m, w.data(), eigen != 0, heat != 0, e3.data(), b3.data(),
+                                     eigen ? us.data() : nullptr, eigen ? dd.data() : nullptr,
+                                     heat ? ps.data() : nullptr, heat ? psp.data() : nullptr,
+                                     heat ? ew.data() : nullptr)))
+            return rc;
+          for (size_t t = 0; t < m; ++t) {
+            const size_t o = (size_t)row * nroots + idx[t];
+            for (int k = 0; k < 3; ++k) {
+              reinterpret_cast<cplx *>(ef)[3 * o + k] = e3[3 * t + k];
+              reinterpret_cast<cplx *>(bf)[3 * o + k] = b3[3 * t + k];
+            }
+            if (eigen) {
+              for (int k = 0; k < 3 * ns; ++k) reinterpret_cast<cplx *>(Us)[3 * ns * o + k] = us[3 * ns * t + k];
+              for (int k = 0; k < ns; ++k) reinterpret_cast<cplx *>(ds)[ns * o + k] = dd[ns * t + k];
+            }
+            if (heat) {
+              for (int k = 0; k < ns; ++k) Ps[ns * o + k] = ps[ns * t + k];
+              for (int k = 0; k < 4 * ns; ++k) Ps_split[4 * ns * o + k] = psp[4 * ns * t + k];
+              W_EM[o] = ew[t];
+            }
+          }
+        }
+        for (size_t t = 0; t < idx.size(); ++t) {
+          const int r = idx[t];
+          if (bad(wroots[r])) jump[r] = 0;
+          if (std::abs(Df[t]) > 1.e100) jump[r] = 0;  // infty check
+          for (int imm = 0; imm < r; ++imm)
+            if (std::abs(wroots[r] - wroots[imm]) < rp->D_gap) {
+              wroots[r] = 0.0;
+              jump[r] = 0;
+            }
+          if (out) {
+            kperp_out[row] = h->kperp;
+            kpar_out[row] = h->kpar;
+            RO[(size_t)row * nroots + r] = wroots[r];
+            written[(size_t)row * nroots + r] = 1;
+          }
+        }
+      }
+    }
+    // the second scan's coordinates back to their start (itt = 0,
+    // ALPS_fns.f90:3537-3584; a kcstq second scan resets with the k1_k2 form)
+    kperp = h->kperp;
+    kpar = h->kpar;
+    if (s2->scan_type == 2) {
+      alpsg_scan_params k12 = *s2;
+      k12.scan_type = 0;
+      scan_k(&k12, 0, kperp_last, kpar_last, kperp, kpar);
+    } else {
+      scan_k(s2, 0, kperp_last, kpar_last, kperp, kpar);
+    }
+    if ((rc = move_k(h, s2->scan_type, kperp, kpar))) return rc;
+  }
+  for (int r = 0; r < nroots; ++r) R[r] = omlast[r];
+  return ALPSG_OK;
+}

@@ -41,7 +41,7 @@ EXPORTS = ["alpsg_last_error", "alpsg_create", "alpsg_destroy", "alpsg_set_k",
            "alpsg_last_root_times", "alpsg_autotune", "alpsg_set_shape", "alpsg_get_shape",
            "alpsg_last_kernel_split", "alpsg_calc_eigen", "alpsg_server_create",
            "alpsg_server_add_plasma", "alpsg_server_remove_plasma", "alpsg_server_submit_k",
-           "alpsg_server_submit_batch",
+           "alpsg_server_submit_batch", "alpsg_om_double_scan",
            "alpsg_server_submit", "alpsg_server_poll", "alpsg_server_stats",
            "alpsg_server_destroy"]
 
@@ -80,6 +80,11 @@ class _RootParams(C.Structure):
                 ("D_prec", C.c_double), ("D_gap", C.c_double), ("D_tol", C.c_double)]
 
 
+class _ScanParams(C.Structure):
+    _fields_ = [("scan_type", C.c_int), ("log_scan", C.c_int), ("diff", C.c_double),
+                ("diff2", C.c_double), ("n_out", C.c_int), ("n_res", C.c_int)]
+
+
 _lib = None
 
 
@@ -103,6 +108,10 @@ def load_library(path: str = LIB_PATH):
                                    C.c_int, C.c_int, C.c_int, dp, dp, dp, C.c_int, ip, ip]
     L.alpsg_om_scan.argtypes = [vp, C.POINTER(_RootParams), C.c_int, C.c_int, C.c_double,
                                 C.c_double, C.c_int, C.c_int, C.c_int, dp, dp, dp, dp]
+    L.alpsg_om_double_scan.argtypes = [vp, C.POINTER(_RootParams), C.POINTER(_ScanParams),
+                                       C.POINTER(_ScanParams), C.c_int, dp, C.c_int, C.c_int, dp,
+                                       dp, dp, ip, dp, dp, dp, dp, dp, dp, dp]
+    L.alpsg_om_double_scan.restype = C.c_int
     L.alpsg_get_nmax.argtypes = [vp, ip]
     L.alpsg_eval_count.argtypes = [vp]
     L.alpsg_eval_count.restype = C.c_longlong
@@ -393,6 +402,46 @@ class Engine:
         n = int(nf[0])
         f = lambda a: a.reshape(ni, nr).T  # Fortran order (ir fastest)
         return f(om), f(cal), f(val), iroots.reshape(numroots, 2)[: min(n, numroots)], n
+
+    def om_double_scan(self, s1, s2, roots, eigen=False, heat=False,
+                       rparams: RootParams | None = None):
+        """om_double_scan (ALPS_fns.f90:2900-3591) from the current k.  s1, s2:
+        (scan_type, log_scan, diff, diff2, n_out, n_res) of the outer and inner
+        scans.  Returns a dict: kperp, kpar [n1+1, n2+1]; roots and written
+        [n1+1, n2+1, nroots]; last_roots (along scan 1 at its end); with
+        eigen/heat the calc_eigen outputs per (point, root)."""
+        r = rparams or self.plasma.roots
+        roots = np.ascontiguousarray(np.asarray(roots, dtype=np.complex128).ravel()).copy()
+        nr, ns = roots.size, self.plasma.nspec
+        n1, n2 = int(s1[4]) + 1, int(s2[4]) + 1
+        rows = n1 * n2
+        kp, kz = np.zeros(rows), np.zeros(rows)
+        out = np.zeros(rows * nr, dtype=np.complex128)
+        wr = np.zeros(rows * nr, dtype=np.int32)
+        want = eigen or heat
+        ef = np.zeros(rows * nr * 3, dtype=np.complex128) if want else None
+        bf = np.zeros(rows * nr * 3, dtype=np.complex128) if want else None
+        us = np.zeros(rows * nr * 3 * ns, dtype=np.complex128) if eigen else None
+        ds = np.zeros(rows * nr * ns, dtype=np.complex128) if eigen else None
+        ps = np.zeros(rows * nr * ns) if heat else None
+        psp = np.zeros(rows * nr * 4 * ns) if heat else None
+        ew = np.zeros(rows * nr) if heat else None
+        a, b = _ScanParams(*s1), _ScanParams(*s2)
+        _check(load_library().alpsg_om_double_scan(
+            self._h, C.byref(_rp(r)), C.byref(a), C.byref(b), nr, _dp(roots), int(bool(eigen)),
+            int(bool(heat)), _dp(kp), _dp(kz), _dp(out), _ip(wr), _dp(ef), _dp(bf), _dp(us),
+            _dp(ds), _dp(ps), _dp(psp), _dp(ew)))
+        sh = (n1, n2, nr)
+        res = dict(kperp=kp.reshape(n1, n2), kpar=kz.reshape(n1, n2), roots=out.reshape(sh),
+                   written=wr.reshape(sh).astype(bool), last_roots=roots)
+        if want:
+            res.update(ef=ef.reshape(sh + (3,)), bf=bf.reshape(sh + (3,)))
+        if eigen:
+            res.update(Us=us.reshape(sh + (ns, 3)), ds=ds.reshape(sh + (ns,)))
+        if heat:
+            res.update(Ps=ps.reshape(sh + (ns,)), Ps_split=psp.reshape(sh + (ns, 4)),
+                       W_EM=ew.reshape(sh))
+        return res
 
     def om_scan(self, scan_type, log_scan, diff, diff2, n_out, n_res, roots,
                 rparams: RootParams | None = None):

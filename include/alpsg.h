@@ -144,6 +144,34 @@ int alpsg_om_scan(alpsg_handle h, const alpsg_root_params *rp, int scan_type, in
                   double diff, double diff2, int n_out, int n_res, int nroots, double *roots,
                   double *kperp_out, double *kpar_out, double *roots_out);
 
+/* One wavevector scan of a &scan_input namelist: scan_type 0..4 (k1_k2,
+   theta, kcstq, kperp, kpara), log or linear steps, diff/diff2 as scan_read
+   computes them (ALPS_io.f90:482-548), n_out output steps of n_res sub-steps. */
+typedef struct alpsg_scan_params {
+  int scan_type, log_scan;
+  double diff, diff2;
+  int n_out, n_res;
+} alpsg_scan_params;
+
+/* om_double_scan (ALPS_fns.f90:2900-3591): scan s1 outer, and from each of
+   its output points the scan s2 inner; roots are followed with the
+   rp->secant_method solver, all roots at a k in one batch.  roots [2*nroots]
+   in/out (the roots along scan 1 at its last point).  Output point (i1, i2),
+   i1 = 0..s1->n_out, i2 = 0..s2->n_out, is row i1*(s2->n_out+1)+i2:
+   kperp_out, kpar_out [rows]; roots_out [rows*nroots*2]; written
+   [rows*nroots] = 1 where the reference writes the .scan row of that root
+   (the root was still followed).  With eigen/heat, calc_eigen's outputs for
+   each (row, root): ef, bf [..][3] complex, Us [..][nspec][3] complex, ds
+   [..][nspec] complex, Ps [..][nspec], Ps_split [..][nspec][4], W_EM [..];
+   the arrays of the part not asked for may be NULL.  ALPSG_E_ARG for the
+   reference's alps_error(5)/(6) (equal scan types, a type-0 scan),
+   ALPSG_E_NOTFOUND for alps_error(9). */
+int alpsg_om_double_scan(alpsg_handle h, const alpsg_root_params *rp, const alpsg_scan_params *s1,
+                         const alpsg_scan_params *s2, int nroots, double *roots, int eigen,
+                         int heat, double *kperp_out, double *kpar_out, double *roots_out,
+                         int *written, double *ef, double *bf, double *Us, double *ds, double *Ps,
+                         double *Ps_split, double *W_EM);
+
 /* calc_eigen (ALPS_fns.f90:2605-2899) for n roots at the handle's k, batched:
    disp at every root (the "final instance of disp" the reference runs before
    it), and with heat != 0 at real(om) and real(om*1.000001).  Outputs per

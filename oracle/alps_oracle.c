@@ -1367,6 +1367,188 @@ int orc_om_scan(orc_plasma *P, int scan_type, int log_scan, double diff, double 
 }
 
 /* ========================================================================
+ * om_double_scan: ALPS_fns.f90:2900-3591
+ * ======================================================================== */
+
+/* the wavevector of step `it` of a scan from (kperp_last, kpar_last), as the
+   select case blocks of ALPS_fns.f90:3167-3222 / 3312-3365 set it; the
+   components a scan type does not set keep their values */
+static void orc_scan_k(const orc_scan *sc, int it, double kperp_last, double kpar_last,
+                       double *kperp, double *kpar) {
+  const double theta_0 = atan(kperp_last / kpar_last);
+  const double k_0 = sqrt(kperp_last * kperp_last + kpar_last * kpar_last);
+  double theta_1, k_tmp;
+  switch (sc->type) {
+    case 0:
+      if (sc->log_scan) {
+        *kperp = pow(10.0, log10(kperp_last) + sc->diff * it);
+        *kpar = pow(10.0, log10(kpar_last) + sc->diff2 * it);
+      } else {
+        *kperp = kperp_last + sc->diff * it;
+        *kpar = kpar_last + sc->diff2 * it;
+      }
+      break;
+    case 1:
+      theta_1 = sc->log_scan ? pow(10.0, log10(theta_0) + sc->diff * it) : theta_0 + sc->diff * it;
+      *kperp = k_0 * sin(theta_1);
+      *kpar = k_0 * cos(theta_1);
+      break;
+    case 2:
+      k_tmp = sc->log_scan ? pow(10.0, log10(k_0) + sc->diff * it) : (k_0) + sc->diff * it;
+      *kperp = k_tmp * sin(theta_0);
+      *kpar = k_tmp * cos(theta_0);
+      break;
+    case 3:
+      *kperp = sc->log_scan ? pow(10.0, log10(kperp_last) + sc->diff * it) : kperp_last + sc->diff * it;
+      break;
+    case 4:
+      *kpar = sc->log_scan ? pow(10.0, log10(kpar_last) + sc->diff * it) : kpar_last + sc->diff * it;
+      break;
+  }
+}
+
+/* move to (kperp, kpar): a kpar-only scan keeps nmax and the Bessel table
+   (ALPS_fns.f90:3224-3231) */
+static int orc_move_k(orc_plasma *P, int type, double kperp, double kpar) {
+  if (type != 4) return orc_set_k(P, kperp, kpar);
+  P->kperp = kperp;
+  P->kpar = kpar;
+  return 0;
+}
+
+int orc_om_double_scan(orc_plasma *P, const orc_scan *s1, const orc_scan *s2, int nroots,
+                       double complex *roots, int eigen, int heat, double *kperp_out,
+                       double *kpar_out, double complex *roots_out, int *written,
+                       double complex *ef, double complex *bf, double complex *Us,
+                       double complex *ds, double *Ps, double *Ps_split, double *W_EM) {
+  const int nspec = P->nspec;
+  if (s1->type == s2->type) return 5; /* alps_error(5) */
+  if (s1->type == 0 || s2->type == 0) return 6; /* alps_error(6) */
+  const int nt = s1->n_out * s1->n_res, nt2 = s2->n_out * s2->n_res;
+  const int nrow2 = s2->n_out + 1;
+  for (int r = 0; r < (s1->n_out + 1) * nrow2 * nroots; ++r) written[r] = 0;
+  int *jump = malloc(sizeof(int) * nroots);
+  double complex *omlast = malloc(sizeof(double complex) * nroots);
+  double complex *wroots = malloc(sizeof(double complex) * nroots);
+  double complex *chi0 = malloc(sizeof(double complex) * 9 * nspec);
+  double complex wave[9];
+  for (int in = 0; in < nroots; ++in) {
+    jump[in] = 1;
+    omlast[in] = roots[in];
+    wroots[in] = roots[in];
+  }
+  const double kperp_last = P->kperp, kpar_last = P->kpar;
+  int rc = 0;
+  for (int it = 0; it <= nt && rc == 0; ++it) {
+    double kperp = P->kperp, kpar = P->kpar;
+    orc_scan_k(s1, it, kperp_last, kpar_last, &kperp, &kpar);
+    if (orc_move_k(P, s1->type, kperp, kpar) != 0) {
+      rc = -1;
+      break;
+    }
+    int alljump = 0;
+    for (int in = 0; in < nroots; ++in) alljump = alljump || jump[in];
+    if (!alljump) {
+      rc = 9;
+      break;
+    }
+    for (int in = 0; in < nroots; ++in) {
+      if (!jump[in]) continue;
+      double complex omega = omlast[in];
+      wroots[in] = omlast[in];
+      orc_solve(P, &omega);
+      wroots[in] = omega;
+      omlast[in] = omega;
+      (void)orc_disp(P, omega, NULL, NULL, NULL); /* final instance of disp */
+      if (isnan(creal(omega)) || fabs(creal(omega)) >= DBL_MAX) {
+        omega = 0.0;
+        jump[in] = 0;
+      }
+      for (int imm = 0; imm < in; ++imm)
+        if (cabs(wroots[in] - wroots[imm]) < P->D_gap) {
+          wroots[in] = 0.0;
+          jump[in] = 0;
+        }
+    }
+    if (it % s1->n_res == 0) {
+      double kperpi = P->kperp, kpari = P->kpar;
+      for (int it2 = 0; it2 <= nt2; ++it2) {
+        if (it2 == 0) {
+          kperpi = P->kperp;
+          kpari = P->kpar;
+          for (int in = 0; in < nroots; ++in) wroots[in] = omlast[in];
+        }
+        kperp = P->kperp;
+        kpar = P->kpar;
+        orc_scan_k(s2, it2, kperpi, kpari, &kperp, &kpar);
+        if (orc_move_k(P, s2->type, kperp, kpar) != 0) {
+          rc = -1;
+          break;
+        }
+        const int out = (it2 % s2->n_res == 0);
+        const int row = (it / s1->n_res) * nrow2 + it2 / s2->n_res;
+        for (int in = 0; in < nroots; ++in) {
+          if (!jump[in]) continue;
+          double complex omega = wroots[in];
+          orc_solve(P, &omega);
+          wroots[in] = omega;
+          const double complex tmp = orc_disp(P, omega, chi0, NULL, wave);
+          const size_t o = (size_t)row * nroots + in;
+          if (out && (eigen || heat)) {
+            orc_calc_eigen(P, omega, chi0, wave, eigen, heat, ef + 3 * o, bf + 3 * o,
+                           Us + 3 * nspec * o, ds + nspec * o, Ps + nspec * o,
+                           Ps_split + 4 * nspec * o, W_EM + o);
+            omega = wroots[in];
+          }
+          if (isnan(creal(omega)) || fabs(creal(omega)) >= DBL_MAX) {
+            omega = 0.0;
+            jump[in] = 0;
+          }
+          if (cabs(tmp) > 1.e100) { /* infty check */
+            omega = 0.0;
+            jump[in] = 0;
+          }
+          for (int imm = 0; imm < in; ++imm)
+            if (cabs(wroots[in] - wroots[imm]) < P->D_gap) {
+              wroots[in] = 0.0;
+              jump[in] = 0;
+            }
+          if (out) {
+            kperp_out[row] = P->kperp;
+            kpar_out[row] = P->kpar;
+            roots_out[o] = wroots[in];
+            written[o] = 1;
+          }
+        }
+      }
+      if (rc) break;
+    }
+    /* the second scan's coordinates back to their start (itt = 0) */
+    kperp = P->kperp;
+    kpar = P->kpar;
+    if (s2->type == 2) {
+      /* as the reference does for a kcstq second scan: the k1_k2 form */
+      if (s2->log_scan) {
+        kperp = pow(10.0, log10(kperp_last) + s2->diff * 0);
+        kpar = pow(10.0, log10(kpar_last) + s2->diff2 * 0);
+      } else {
+        kperp = kperp_last + s2->diff * 0;
+        kpar = kpar_last + s2->diff2 * 0;
+      }
+    } else {
+      orc_scan_k(s2, 0, kperp_last, kpar_last, &kperp, &kpar);
+    }
+    if (orc_move_k(P, s2->type, kperp, kpar) != 0) rc = -1;
+  }
+  for (int in = 0; in < nroots; ++in) roots[in] = omlast[in];
+  free(jump);
+  free(omlast);
+  free(wroots);
+  free(chi0);
+  return rc;
+}
+
+/* ========================================================================
  * Eigenfunctions and heating rates: ALPS_fns.f90:2605-2899
  * ======================================================================== */
 

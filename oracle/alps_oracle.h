@@ -193,6 +193,30 @@ int orc_om_scan(orc_plasma *P, int scan_type, int log_scan, double diff, double 
                 int n_out, int n_res, int nroots, double complex *roots,
                 double *kperp_out, double *kpar_out, double complex *roots_out);
 
+/* One wavevector scan of &scan_input (ALPS_var.f90 scanner type): scan_type
+   0..4, log or linear steps, diff/diff2 as scan_read computes them
+   (ALPS_io.f90:482-548), n_out output steps of n_res sub-steps each. */
+typedef struct {
+  int type, log_scan;
+  double diff, diff2;
+  int n_out, n_res;
+} orc_scan;
+
+/* om_double_scan (ALPS_fns.f90:2900-3591): scan 1 outer, scan 2 inner from
+   every output point of scan 1.  roots[nroots] in/out (omlast at the end).
+   Output point (i1, i2), i1 = 0..n_out1, i2 = 0..n_out2, is row
+   i1*(n_out2+1)+i2 of kperp_out/kpar_out, and row*nroots+in of roots_out
+   (wroots(in) as written to the .scan file) and written (1 when the reference
+   writes that row for root in, i.e. the root was still followed).  With eigen
+   or heat, calc_eigen's outputs for each written (row, root) go to ef[3],
+   bf[3], Us[3*nspec], ds[nspec], Ps[nspec], Ps_split[4*nspec], W_EM[1] blocks
+   (may be NULL otherwise).  Returns 0, or 9 for alps_error(9). */
+int orc_om_double_scan(orc_plasma *P, const orc_scan *s1, const orc_scan *s2, int nroots,
+                       double complex *roots, int eigen, int heat, double *kperp_out,
+                       double *kpar_out, double complex *roots_out, int *written,
+                       double complex *ef, double complex *bf, double complex *Us,
+                       double complex *ds, double *Ps, double *Ps_split, double *W_EM);
+
 #ifdef __cplusplus
 }
 #endif

@@ -67,6 +67,12 @@ def build(force: bool = False) -> str:
 _lib = None
 
 
+class OrcScan(C.Structure):
+    """orc_scan: one &scan_input scan (type, log, diff, diff2, n_out, n_res)."""
+    _fields_ = [("type", C.c_int), ("log_scan", C.c_int), ("diff", C.c_double),
+                ("diff2", C.c_double), ("n_out", C.c_int), ("n_res", C.c_int)]
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -131,6 +137,10 @@ def lib():
         L.orc_om_scan.restype = C.c_int
         L.orc_om_scan.argtypes = [P, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
                                   C.c_int, C.c_int, dp, dp, dp, dp]
+        L.orc_om_double_scan.restype = C.c_int
+        L.orc_om_double_scan.argtypes = [P, C.POINTER(OrcScan), C.POINTER(OrcScan), C.c_int, dp,
+                                         C.c_int, C.c_int, dp, dp, dp, ip, dp, dp, dp, dp, dp,
+                                         dp, dp]
         _lib = L
     return _lib
 
@@ -386,6 +396,38 @@ class Oracle:
                                  iroots.ctypes.data_as(C.POINTER(C.c_int)))
         return (om.reshape(nr, ni), cal.reshape(nr, ni), val.reshape(nr, ni),
                 iroots.reshape(numroots, 2)[: min(n, numroots)], n)
+
+    def om_double_scan(self, s1, s2, roots, eigen=False, heat=False):
+        """om_double_scan (ALPS_fns.f90:2900-3591).  s1, s2: (type, log_scan,
+        diff, diff2, n_out, n_res).  Returns rc, kperp and kpar [n1+1, n2+1],
+        roots [n1+1, n2+1, nroots], written [same] and, with eigen or heat,
+        the calc_eigen outputs per written (point, root)."""
+        roots = np.array(roots, dtype=np.complex128)
+        nr, ns = roots.size, self.plasma.nspec
+        a, b = OrcScan(*s1), OrcScan(*s2)
+        n1, n2 = s1[4] + 1, s2[4] + 1
+        rows = n1 * n2
+        kp, kz = np.zeros(rows), np.zeros(rows)
+        out = np.zeros(rows * nr, dtype=np.complex128)
+        wr = np.zeros(rows * nr, dtype=np.int32)
+        ef = np.zeros(rows * nr * 3, dtype=np.complex128)
+        bf = np.zeros(rows * nr * 3, dtype=np.complex128)
+        us = np.zeros(rows * nr * 3 * ns, dtype=np.complex128)
+        ds = np.zeros(rows * nr * ns, dtype=np.complex128)
+        ps = np.zeros(rows * nr * ns)
+        psp = np.zeros(rows * nr * 4 * ns)
+        ew = np.zeros(rows * nr)
+        rc = lib().orc_om_double_scan(C.byref(self.s), C.byref(a), C.byref(b), nr, _ptr(roots),
+                                      int(eigen), int(heat), _ptr(kp), _ptr(kz), _ptr(out),
+                                      wr.ctypes.data_as(C.POINTER(C.c_int)), _ptr(ef), _ptr(bf),
+                                      _ptr(us), _ptr(ds), _ptr(ps), _ptr(psp), _ptr(ew))
+        sh = (n1, n2, nr)
+        return dict(rc=rc, kperp=kp.reshape(n1, n2), kpar=kz.reshape(n1, n2),
+                    roots=out.reshape(sh), written=wr.reshape(sh).astype(bool),
+                    last_roots=roots, ef=ef.reshape(sh + (3,)), bf=bf.reshape(sh + (3,)),
+                    Us=us.reshape(sh + (ns, 3)), ds=ds.reshape(sh + (ns,)),
+                    Ps=ps.reshape(sh + (ns,)), Ps_split=psp.reshape(sh + (ns, 4)),
+                    W_EM=ew.reshape(sh))
 
     def om_scan(self, scan_type, log_scan, diff, diff2, n_out, n_res, roots):
         roots = np.array(roots, dtype=np.complex128)
