@@ -45,6 +45,8 @@ hipError_t launch_resonant(hipStream_t st, int nch, int RB, const ProbDev *prob,
 hipError_t launch_nhds(hipStream_t st, const ProbDev *prob, int nspec, int nom, const double2 *om,
                        const int *active, double2 *chiA, double2 *lowA);
 int nhds_tables(double z, int bm_nmax, double bessel_zero, std::vector<double2> &tab);
+hipError_t launch_split_tables(hipStream_t st, int npar, int nmax, double kpar, double q, double m,
+                               const double *nodes, double *crec, double *rsum);
 hipError_t launch_rel_bessel(hipStream_t st, int npts, int nmax, double zfac, const double *grel,
                              const double *pbrel, const int *mtab, double *out);
 hipError_t launch_rel(hipStream_t st, const ProbDev *prob, int nspec, int nmaxR, int NB, int nom,
@@ -120,6 +122,7 @@ struct alpsg_ctx {
   std::vector<double> int_ee_h;
   // per-k device data
   DevBuf<double> bes[ALPSG_MAXSPEC], atab[ALPSG_MAXSPEC], nodes[ALPSG_MAXSPEC];
+  DevBuf<double> crec[ALPSG_MAXSPEC], rsum[ALPSG_MAXSPEC];  // split-form sweep tables
   DevBuf<double> bmax;
   // problem descriptor
   ProbDev prob{};
@@ -1430,6 +1433,15 @@ int alpsg_set_k(alpsg_handle h, double kperp, double kpar, const int *nmax_in, i
     S.z = h->kperp_norm ? kperp / h->qs[s] : 1.0 / h->qs[s];
     S.atab = h->atab[s].p;
     S.nodes = h->nodes[s].p;
+    S.crec = S.rsum = nullptr;
+    if (!h->relsp[s]) {
+      HIPCHK(h->crec[s].ensure((size_t)(nm + 1) * (npar - 1) * CREC_STRIDE));
+      HIPCHK(h->rsum[s].ensure((size_t)(nm + 1) * RSUM_STRIDE));
+      HIPCHK(launch_split_tables(h->st, npar, nm, kpar, h->qs[s], h->ms[s], h->nodes[s].p,
+                                 h->crec[s].p, h->rsum[s].p));
+      S.crec = h->crec[s].p;
+      S.rsum = h->rsum[s].p;
+    }
     if (h->relsp[s]) {
       // the harmonics run in k_rel_nonres / k_rel_res, not in the sweep
       const double m = h->ms[s], q = h->qs[s], vA = h->vA;

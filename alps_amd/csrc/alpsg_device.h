@@ -11,6 +11,8 @@
 #define NODE_STRIDE 8   // doubles per (n, ipar) node record: G1 H1 G2 H2 G3 H3 ppar ppar^2
 #define ATAB_STRIDE 4   // doubles per (n, iperp) entry: J^2, pperp^2 J'^2, pperp J J', J
 #define FIT_STRIDE 8    // doubles per (iperp, fit) entry of the analytic-continuation table
+#define CREC_STRIDE 13  // doubles per (n, ipar) record of the split-form sweep table
+#define RSUM_STRIDE 8   // doubles per harmonic of the split-form row sums
 #define GTAB_N 4200     // Gamma(m) table entries (m < GTAB_N), followed by Fact(0..20)
 
 struct SpecDev {
@@ -30,6 +32,12 @@ struct SpecDev {
   const double *ppar;   // [npar+1]
   const double *atab;   // [(nmax+1)*(nperp+1)*ATAB_STRIDE]
   const double *nodes;  // [(nmax+1)*(npar-1)*NODE_STRIDE]
+  // split form of the node sums for rows every lane integrates over the whole
+  // grid (may be NULL): U/den = G/m + C/den with C = G (kpar p +- n q)/m + H;
+  // crec[n][ipar] = {kpar p, the six coefficients of +n, of -n}, rsum[n] = the
+  // omega-independent sums of the G/m terms over the interior nodes
+  const double *crec;   // [(nmax+1)*(npar-1)*CREC_STRIDE]
+  const double *rsum;   // [(nmax+1)*RSUM_STRIDE]
   const double *fitc;   // [(nperp+1)*nfits*FIT_STRIDE]
   const double *polyc;  // [(nperp+1)*(poly_order+1)]
   // closed-form (use_bM) species: NHDS parameters (ALPS_NHDS.f90:163-168)

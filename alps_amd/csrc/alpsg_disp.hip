@@ -209,6 +209,33 @@ __device__ __forceinline__ void node_acc(double w, double dre, double dim, doubl
   acc[5] = acc[5] + p * u3;
 }
 
+template <bool STD, bool PAIR>
+__device__ __forceinline__ void node_pair(const double *rec, int i, double omr, double omi,
+                                          double mom, double kpar, double nq, double dim,
+                                          double dim2, int pl1, int pc1, int pl2, int pc2, int ml1,
+                                          int mc1, int ml2, int mc2, cd ap[6], cd am[6]) {
+  const double G1 = rec[0], H1 = rec[1], G2 = rec[2], H2 = rec[3];
+  const double G3 = rec[4], H3 = rec[5], p = rec[6], p2 = rec[7];
+  cd U1, U2, U3;
+  {
+#pragma clang fp contract(off)
+    U1 = mk(omr * G1 + H1, omi * G1);
+    U2 = mk(omr * G2 + H2, omi * G2);
+    U3 = mk(omr * G3 + H3, omi * G3);
+  }
+  const double base = __builtin_fma(-kpar, p, mom);
+  if constexpr (STD) {
+    node_acc<true>(1.0, base - nq, dim, dim2, U1, U2, U3, p, p2, ap);
+    if (PAIR) node_acc<true>(1.0, base + nq, dim, dim2, U1, U2, U3, p, p2, am);
+  } else {
+    const bool inP = ((unsigned)(i - pl1) < (unsigned)pc1) | ((unsigned)(i - pl2) < (unsigned)pc2);
+    const bool inM = ((unsigned)(i - ml1) < (unsigned)mc1) | ((unsigned)(i - ml2) < (unsigned)mc2);
+    node_acc<false>(inP ? 1.0 : 0.0, inP ? base - nq : 1.0, dim, dim2, U1, U2, U3, p, p2, ap);
+    if (PAIR)
+      node_acc<false>(inM ? 1.0 : 0.0, inM ? base + nq : 1.0, dim, dim2, U1, U2, U3, p, p2, am);
+  }
+}
+
 // interior nodes i = 2..npar-2 (weight 2) of the +n row (and the -n row when
 // PAIR), lanes of an omega group strided by NCH.  STD: every interior node is in
 // range; otherwise node i is in range when it lies strictly inside one of the
@@ -250,33 +277,6 @@ __device__ __forceinline__ void sweep_nodes(cdptr srows, int npar, int c, double
 // sweep_nodes with the node records of K consecutive strides loaded before any
 // of them is used, so each lane keeps K records in flight; the nodes are
 // accumulated in the same order as sweep_nodes.
-template <bool STD, bool PAIR>
-__device__ __forceinline__ void node_pair(const double *rec, int i, double omr, double omi,
-                                          double mom, double kpar, double nq, double dim,
-                                          double dim2, int pl1, int pc1, int pl2, int pc2, int ml1,
-                                          int mc1, int ml2, int mc2, cd ap[6], cd am[6]) {
-  const double G1 = rec[0], H1 = rec[1], G2 = rec[2], H2 = rec[3];
-  const double G3 = rec[4], H3 = rec[5], p = rec[6], p2 = rec[7];
-  cd U1, U2, U3;
-  {
-#pragma clang fp contract(off)
-    U1 = mk(omr * G1 + H1, omi * G1);
-    U2 = mk(omr * G2 + H2, omi * G2);
-    U3 = mk(omr * G3 + H3, omi * G3);
-  }
-  const double base = __builtin_fma(-kpar, p, mom);
-  if constexpr (STD) {
-    node_acc<true>(1.0, base - nq, dim, dim2, U1, U2, U3, p, p2, ap);
-    if (PAIR) node_acc<true>(1.0, base + nq, dim, dim2, U1, U2, U3, p, p2, am);
-  } else {
-    const bool inP = ((unsigned)(i - pl1) < (unsigned)pc1) | ((unsigned)(i - pl2) < (unsigned)pc2);
-    const bool inM = ((unsigned)(i - ml1) < (unsigned)mc1) | ((unsigned)(i - ml2) < (unsigned)mc2);
-    node_acc<false>(inP ? 1.0 : 0.0, inP ? base - nq : 1.0, dim, dim2, U1, U2, U3, p, p2, ap);
-    if (PAIR)
-      node_acc<false>(inM ? 1.0 : 0.0, inM ? base + nq : 1.0, dim, dim2, U1, U2, U3, p, p2, am);
-  }
-}
-
 // sweep_nodes in chunks of K strides whose node records are all loaded before
 // the first is used, so each lane keeps K records in flight; a remainder loop
 // takes the last partial chunk.  The nodes are accumulated in the same order
@@ -307,6 +307,65 @@ __device__ __forceinline__ void sweep_nodes_pf(cdptr srows, int npar, int c, dou
     for (int t = 0; t < NODE_STRIDE; ++t) rec[t] = srows[(i0 - 1) * NODE_STRIDE + t];
     node_pair<STD, PAIR>(rec, i0, omr, omi, mom, kpar, nq, dim, dim2, pl1, pc1, pl2, pc2, ml1, mc1,
                          ml2, mc2, ap, am);
+  }
+}
+
+constexpr double SPLIT_MAX = 256.0;
+
+// The interior nodes of a full-range row pair in the split form (one lane per
+// omega, records read as wave-uniform scalars): with r = 1/den,
+// acc_k += C_k r for each sign; the G/m terms are the row's rsum, added last.
+template <bool PAIR>
+__device__ __forceinline__ void split_node(const double *r, double base, double nq, double dim2,
+                                           double ndim, cd ap[6], cd am[6]) {
+  {
+    const double dre = base - nq;
+    const double inv = rcp1_d(__builtin_fma(dre, dre, dim2));
+    const double rr = dre * inv, ri = ndim * inv;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      ap[k].re = __builtin_fma(r[1 + k], rr, ap[k].re);
+      ap[k].im = __builtin_fma(r[1 + k], ri, ap[k].im);
+    }
+  }
+  if (PAIR) {
+    const double dre = base + nq;
+    const double inv = rcp1_d(__builtin_fma(dre, dre, dim2));
+    const double rr = dre * inv, ri = ndim * inv;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      am[k].re = __builtin_fma(r[7 + k], rr, am[k].re);
+      am[k].im = __builtin_fma(r[7 + k], ri, am[k].im);
+    }
+  }
+}
+
+// The interior nodes of a full-range row pair in the split form (one lane per
+// omega, records read as wave-uniform scalars, the next record loaded while
+// the current one is used): with r = 1/den, acc_k += C_k r for each sign; the
+// G/m terms are the row's rsum, added last.
+template <bool PAIR>
+__device__ __forceinline__ void sweep_split(cdptr crow, cdptr rsum, int npar, double omr,
+                                            double omi, double m, double nq, double dim,
+                                            cd ap[6], cd am[6]) {
+  const double dim2 = dim * dim, ndim = -dim;
+  const double mom = m * omr;
+  double cur[CREC_STRIDE];
+#pragma unroll
+  for (int t = 0; t < CREC_STRIDE; ++t) cur[t] = crow[1 * CREC_STRIDE + t];  // node 2
+  for (int i = 2; i <= npar - 2; ++i) {
+    const int in = (i < npar - 2) ? i + 1 : i;
+    double nxt[CREC_STRIDE];
+#pragma unroll
+    for (int t = 0; t < CREC_STRIDE; ++t) nxt[t] = crow[(in - 1) * CREC_STRIDE + t];
+    split_node<PAIR>(cur, mom - cur[0], nq, dim2, ndim, ap, am);
+#pragma unroll
+    for (int t = 0; t < CREC_STRIDE; ++t) cur[t] = nxt[t];
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    ap[k].re += rsum[k];
+    if (PAIR) am[k].re += rsum[k];
   }
 }
 
@@ -388,13 +447,28 @@ __global__ __launch_bounds__(64 * W) void k_sweep(const ProbDev *__restrict__ pr
       // uniform-weight loop (no per-node range tests)
       const bool stdp = (rp.a2 < 0) && rp.a1 == 1 && rp.b1 == npar - 1;
       const bool stdm = (nn == 0) || ((rm.a2 < 0) && rm.a1 == 1 && rm.b1 == npar - 1);
+      // the split form sums G/m and C/den, which cancel to U/den when |den| is
+      // large against m|w|: it is used where the cancellation factor
+      // (kpar max|p| + |n q|) / (m |w|) stays below SPLIT_MAX for every lane
+      const bool split_ok =
+          NCH == 1 && S.crec != nullptr &&
+          fabs(kpar) * fmax(fabs(S.ppar[0]), fabs(S.ppar[npar])) + fabs(nq) <=
+              SPLIT_MAX * m * sqrt(omr * omr + omi * omi);
       if (__all(stdp && stdm)) {
-        if (nn > 0)
+        if (__all(split_ok)) {
+          const cdptr crow = (cdptr)(S.crec + (size_t)nn * (npar - 1) * CREC_STRIDE);
+          const cdptr rs = (cdptr)(S.rsum + (size_t)nn * RSUM_STRIDE);
+          if (nn > 0)
+            sweep_split<true>(crow, rs, npar, omr, omi, m, nq, dim, ap, am);
+          else
+            sweep_split<false>(crow, rs, npar, omr, omi, m, nq, dim, ap, am);
+        } else if (nn > 0) {
           sweep_nodes<NCH, true, true>(srows, npar, c, omr, omi, m, kpar, nq, dim, 0, 0, 0, 0, 0, 0,
                                        0, 0, ap, am);
-        else
+        } else {
           sweep_nodes<NCH, true, false>(srows, npar, c, omr, omi, m, kpar, nq, dim, 0, 0, 0, 0, 0,
                                         0, 0, 0, ap, am);
+        }
       } else {
         sweep_nodes<NCH, false, true>(srows, npar, c, omr, omi, m, kpar, nq, dim, pl1, pc1, pl2, pc2,
                                       ml1, mc1, ml2, mc2, ap, am);

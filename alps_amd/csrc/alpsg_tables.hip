@@ -306,6 +306,62 @@ __global__ void k_moments(int nperp, int npar, int nmax, double kpar, double m,
                 lds_a);
 }
 
+// Split-form records of the one-lane-per-omega sweep.  With den = m w - kpar p
+// -+ n q and U_a = w G_a + H_a,
+//   U_a / den = G_a / m + C_a / den,  C_a = G_a (kpar p +- n q) / m + H_a,
+// where C_a is real and omega-independent.  Record (n, ipar):
+//   [0] kpar p, [1..6] C1, p C1, p^2 C1, C2, C3, p C3 for +n, [7..12] for -n.
+__global__ void k_crec(int npar, int nmax, double kpar, double q, double m,
+                       const double *__restrict__ nodes, double *__restrict__ crec) {
+  const int ipar = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = blockIdx.y;
+  if (ipar > npar - 1 || n > nmax) return;
+  const double *r = nodes + ((size_t)n * (npar - 1) + (ipar - 1)) * NODE_STRIDE;
+  const double p = r[6], p2 = r[7];
+  const double kp = kpar * p, nq = (1.0 * n) * q;
+  double *o = crec + ((size_t)n * (npar - 1) + (ipar - 1)) * CREC_STRIDE;
+  o[0] = kp;
+  for (int sg = 0; sg < 2; ++sg) {
+    const double t = (sg == 0 ? kp + nq : kp - nq) / m;
+    const double c1 = r[0] * t + r[1], c2 = r[2] * t + r[3], c3 = r[4] * t + r[5];
+    double *c = o + 1 + 6 * sg;
+    c[0] = c1;
+    c[1] = p * c1;
+    c[2] = p2 * c1;
+    c[3] = c2;
+    c[4] = c3;
+    c[5] = p * c3;
+  }
+}
+
+// rsum[n] = sum over the interior nodes ipar = 2..npar-2 of
+// {G1, p G1, p^2 G1, G2, G3, p G3} / m, in ipar order (one thread per (n, term))
+__global__ void k_rsum(int npar, int nmax, double m, const double *__restrict__ nodes,
+                       double *__restrict__ rsum) {
+  const int n = blockIdx.x;
+  const int k = threadIdx.x;
+  if (n > nmax || k >= 6) {
+    if (n <= nmax && k < RSUM_STRIDE) rsum[(size_t)n * RSUM_STRIDE + k] = 0.0;
+    return;
+  }
+  double s = 0.0;
+  for (int ipar = 2; ipar <= npar - 2; ++ipar) {
+    const double *r = nodes + ((size_t)n * (npar - 1) + (ipar - 1)) * NODE_STRIDE;
+    const double g = (k <= 2) ? r[0] : (k == 3 ? r[2] : r[4]);
+    const double pw = (k == 1 || k == 5) ? r[6] : (k == 2 ? r[7] : 1.0);
+    s += (pw * g) / m;
+  }
+  rsum[(size_t)n * RSUM_STRIDE + k] = s;
+}
+
+hipError_t launch_split_tables(hipStream_t st, int npar, int nmax, double kpar, double q, double m,
+                               const double *nodes, double *crec, double *rsum) {
+  hipLaunchKernelGGL(k_crec, dim3((npar - 1 + 63) / 64, nmax + 1), dim3(64), 0, st, npar, nmax,
+                     kpar, q, m, nodes, crec);
+  hipLaunchKernelGGL(k_rsum, dim3(nmax + 1), dim3(RSUM_STRIDE), 0, st, npar, nmax, m, nodes, rsum);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // Job-list forms for the root server: one launch covers the tables of many
 // (request, species) jobs.  Job j owns blocks [start[j], start[j+1]) of the
