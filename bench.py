@@ -60,7 +60,10 @@ KPERP = KPAR = 0.5
 # workgroups per omega block) shape, from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE passes on this workload, each scaled by the counter's reading for
 # a 256 MiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).
-SWEEP_TRAFFIC_BYTES = {(1, 4, 3): 13542.0 * 1024 * 2.0 + 147456.0 * 1024}
+SWEEP_TRAFFIC_BYTES = {(1, 2, 16): 16054.8125 * 1024 * 2.0 + 393216.0 * 1024,
+                       (1, 4, 3): 18057.3125 * 1024 * 2.0 + 147456.0 * 1024,
+                       (1, 4, 6): 18507.875 * 1024 * 2.0 + 294912.0 * 1024,
+                       (1, 4, 4): 17959.1875 * 1024 * 2.0 + 196608.0 * 1024}
 
 
 def flops_per_eval(nmax, npar):
