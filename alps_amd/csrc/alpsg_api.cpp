@@ -39,6 +39,8 @@ hipError_t launch_species_k(hipStream_t st, int nperp, int npar, int nmax, doubl
 hipError_t launch_sweep(hipStream_t st, int nch, int W, int RC, const ProbDev *prob, int nom,
                         const double2 *om, const int *active, double2 *chiA, double2 *lowA,
                         const int *pidx = nullptr);
+hipError_t launch_sweep_req(hipStream_t st, int nch, int W, int RC, int K, const ProbDev *prob,
+                            int nom, const double2 *om, double2 *chiA, const int *pidx);
 hipError_t launch_resonant(hipStream_t st, int nch, int RB, const ProbDev *prob, int nspec, int nom,
                            const double2 *om, const int *active, double2 *chiB, double2 *lowB,
                            bool general, const int *pidx = nullptr);
@@ -96,6 +98,24 @@ struct DevBuf {
   }
 };
 
+// pinned host staging for the small transfers of every disp call
+struct PinBuf {
+  double2 *p = nullptr;
+  size_t n = 0;
+  hipError_t ensure(size_t cnt) {
+    if (cnt <= n && p) return hipSuccess;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+    hipError_t e = hipHostMalloc((void **)&p, std::max<size_t>(cnt, 1) * sizeof(double2), hipHostMallocDefault);
+    if (e == hipSuccess) n = std::max<size_t>(cnt, 1);
+    return e;
+  }
+  ~PinBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+};
+
 }  // namespace
 
 // the error text of the calling thread, for the other translation units
@@ -123,6 +143,8 @@ struct alpsg_ctx {
   // per-k device data
   DevBuf<double> bes[ALPSG_MAXSPEC], atab[ALPSG_MAXSPEC], nodes[ALPSG_MAXSPEC];
   DevBuf<double> crec[ALPSG_MAXSPEC], rsum[ALPSG_MAXSPEC];  // split-form sweep tables
+  DevBuf<int> zidx;  // zeros: the problem index of every omega of a small batch
+  PinBuf om_pin, det_pin;  // staging of om and det for batches of at most PIN_MAX omegas
   DevBuf<double> bmax;
   // problem descriptor
   ProbDev prob{};
@@ -240,6 +262,11 @@ LaunchShape shape_for(const alpsg_ctx *h, int n) {
   return pick_shape(n);
 }
 
+// the small-batch sweep (k_sweep_req<REQ_NCH, REQ_W, REQ_K> over REQ_TW / REQ_W
+// workgroups per omega) for batches of at most REQ_MAX_N omegas
+constexpr int REQ_MAX_N = 16, REQ_NCH = 32, REQ_W = 4, REQ_TW = 64, REQ_K = 4;
+constexpr int PIN_MAX = 4096;  // batches up to this size copy om and det through pinned memory
+
 // (lanes per omega, wavefronts per workgroup) pairs instantiated for k_sweep
 const int kSweepVariants[][2] = {{1, 1}, {1, 2}, {1, 4}, {2, 4}, {4, 4}, {8, 4},
                                  {16, 4}, {16, 8}, {32, 8}, {64, 4}, {64, 8}};
@@ -267,8 +294,24 @@ int disp_device(alpsg_ctx *h, int n, const double2 *om_dev, const int *active_de
     HIPCHK(h->lowB_d.ensure((size_t)n * ns * 2 * sh.rb * 18));
   }
   if (want_wave) HIPCHK(h->wave_d.ensure((size_t)n * 9));
-  HIPCHK(launch_sweep(h->st, sh.nch_a, sh.w_a, sh.rc_a, h->prob_d.p, n, om_dev, active_dev,
-                      h->chiA_d.p, want_low ? h->lowA_d.p : nullptr));
+  // Small batches (a few roots' secant steps, the one-omega plain path): the
+  // sweep that keeps several node records in flight per lane and spreads each
+  // omega's row pairs over REQ_TW wavefronts (problem index 0 for every omega).
+  // The n_s-row partial layout is the same, so the rest of the chain is unchanged.
+  const bool req = !want_low && active_dev == nullptr && n <= REQ_MAX_N;
+  if (req) {
+    HIPCHK(h->chiA_d.ensure((size_t)n * REQ_TW * ns * 6));
+    if (h->zidx.n < (size_t)REQ_MAX_N) {
+      HIPCHK(h->zidx.ensure(REQ_MAX_N));
+      HIPCHK(hipMemsetAsync(h->zidx.p, 0, sizeof(int) * REQ_MAX_N, h->st));
+    }
+    HIPCHK(launch_sweep_req(h->st, REQ_NCH, REQ_W, REQ_TW / REQ_W, REQ_K, h->prob_d.p, n, om_dev,
+                            h->chiA_d.p, h->zidx.p));
+  } else {
+    HIPCHK(launch_sweep(h->st, sh.nch_a, sh.w_a, sh.rc_a, h->prob_d.p, n, om_dev, active_dev,
+                        h->chiA_d.p, want_low ? h->lowA_d.p : nullptr));
+  }
+  const int TWa = req ? REQ_TW : TW;
   HIPCHK(hipEventRecord(h->ev_sweep, h->st));
   HIPCHK(launch_resonant(h->st, sh.nch_b, sh.rb, h->prob_d.p, ns, n, om_dev, active_dev,
                          h->chiB_d.p, want_low ? h->lowB_d.p : nullptr, h->any_general_fit));
@@ -287,7 +330,7 @@ int disp_device(alpsg_ctx *h, int n, const double2 *om_dev, const int *active_de
                       want_low ? h->lowR_d.p : nullptr, h->flags_d.p, h->err_d.p));
   }
   HIPCHK(hipEventRecord(h->ev_res, h->st));
-  HIPCHK(launch_assemble(h->st, h->prob_d.p, ns, n, TW, sh.rb, om_dev, active_dev, h->chiA_d.p,
+  HIPCHK(launch_assemble(h->st, h->prob_d.p, ns, n, TWa, sh.rb, om_dev, active_dev, h->chiA_d.p,
                          h->chiB_d.p, want_low ? h->lowA_d.p : nullptr,
                          want_low ? h->lowB_d.p : nullptr, RR ? h->chiR_d.p : nullptr, RR,
                          (RR && want_low) ? h->lowR_d.p : nullptr, h->det_d.p,
@@ -413,13 +456,22 @@ int run_disp(alpsg_ctx *h, int n, const double *om, double *det, double *chi, do
   if (n < 0) return fail(ALPSG_E_ARG, "negative batch size");
   if (n == 0) return ALPSG_OK;
   HIPCHK(h->om_d.ensure(n));
-  HIPCHK(hipMemcpyAsync(h->om_d.p, om, sizeof(double2) * n, hipMemcpyHostToDevice, h->st));
+  const bool pin = n <= PIN_MAX;
+  if (pin) {
+    HIPCHK(h->om_pin.ensure(PIN_MAX));
+    HIPCHK(h->det_pin.ensure(PIN_MAX));
+    std::memcpy(h->om_pin.p, om, sizeof(double2) * n);
+    HIPCHK(hipMemcpyAsync(h->om_d.p, h->om_pin.p, sizeof(double2) * n, hipMemcpyHostToDevice, h->st));
+  } else {
+    HIPCHK(hipMemcpyAsync(h->om_d.p, om, sizeof(double2) * n, hipMemcpyHostToDevice, h->st));
+  }
   LaunchShape sh = shape_for(h, n);
   HIPCHK(hipEventRecord(h->ev0, h->st));
   int rc = disp_device(h, n, h->om_d.p, nullptr, chi != nullptr, low != nullptr, wave != nullptr, sh);
   if (rc) return rc;
   HIPCHK(hipEventRecord(h->ev1, h->st));
-  HIPCHK(hipMemcpyAsync(det, h->det_d.p, sizeof(double2) * n, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipMemcpyAsync(pin ? (void *)h->det_pin.p : (void *)det, h->det_d.p, sizeof(double2) * n,
+                        hipMemcpyDeviceToHost, h->st));
   if (chi)
     HIPCHK(hipMemcpyAsync(chi, h->chi_d.p, sizeof(double2) * n * h->nspec * 9,
                           hipMemcpyDeviceToHost, h->st));
@@ -433,6 +485,7 @@ int run_disp(alpsg_ctx *h, int n, const double *om, double *det, double *chi, do
   if (h->any_rel)
     HIPCHK(hipMemcpyAsync(&relerr, h->err_d.p, sizeof(int), hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
+  if (pin) std::memcpy(det, h->det_pin.p, sizeof(double2) * n);
   if (relerr) {
     const int zero = 0;
     HIPCHK(hipMemcpy(h->err_d.p, &zero, sizeof(int), hipMemcpyHostToDevice));

@@ -39,9 +39,10 @@ struct RowClass {
 // determine_resonances (ALPS_fns.f90:704-742) and the ipar_res / range logic of
 // integrate_res (ALPS_fns.f90:950-1007) for one (n, omega).
 __device__ __forceinline__ void classify(const SpecDev &S, int P, double kpar, double omr,
-                                         double omi, int n, RowClass &rc) {
+                                         double omi, int n, RowClass &rc,
+                                         const double *pa_copy = nullptr) {
 #pragma clang fp contract(off)
-  const double *pa = S.ppar;
+  const double *pa = pa_copy ? pa_copy : S.ppar;
   const int npar = S.npar;
   const double d = S.dppar;
   const double pr = (S.m * omr - (1.0 * n) * S.q) / kpar;
@@ -209,11 +210,12 @@ __device__ __forceinline__ void node_acc(double w, double dre, double dim, doubl
   acc[5] = acc[5] + p * u3;
 }
 
-template <bool STD, bool PAIR>
+template <bool STD, bool PAIR, bool FOLD = false>
 __device__ __forceinline__ void node_pair(const double *rec, int i, double omr, double omi,
                                           double mom, double kpar, double nq, double dim,
                                           double dim2, int pl1, int pc1, int pl2, int pc2, int ml1,
-                                          int mc1, int ml2, int mc2, cd ap[6], cd am[6]) {
+                                          int mc1, int ml2, int mc2, cd ap[6], cd am[6],
+                                          int npar = 0) {
   const double G1 = rec[0], H1 = rec[1], G2 = rec[2], H2 = rec[3];
   const double G3 = rec[4], H3 = rec[5], p = rec[6], p2 = rec[7];
   cd U1, U2, U3;
@@ -224,7 +226,12 @@ __device__ __forceinline__ void node_pair(const double *rec, int i, double omr, 
     U3 = mk(omr * G3 + H3, omi * G3);
   }
   const double base = __builtin_fma(-kpar, p, mom);
-  if constexpr (STD) {
+  if constexpr (STD && FOLD) {
+    // a full-range row with its two end nodes (weight 1/2) in the same loop
+    const double w = (i == 1 || i == npar - 1) ? 0.5 : 1.0;
+    node_acc<false>(w, base - nq, dim, dim2, U1, U2, U3, p, p2, ap);
+    if (PAIR) node_acc<false>(w, base + nq, dim, dim2, U1, U2, U3, p, p2, am);
+  } else if constexpr (STD) {
     node_acc<true>(1.0, base - nq, dim, dim2, U1, U2, U3, p, p2, ap);
     if (PAIR) node_acc<true>(1.0, base + nq, dim, dim2, U1, U2, U3, p, p2, am);
   } else {
@@ -247,6 +254,17 @@ __device__ __forceinline__ void sweep_nodes(cdptr srows, int npar, int c, double
                                             int mc2, cd ap[6], cd am[6]) {
   const double dim2 = dim * dim;
   const double mom = m * omr;
+  if constexpr (STD && NCH > 1) {
+    // several lanes per omega: the end nodes join the lanes' loop (weight 1/2)
+    for (int i = 1 + c; i <= npar - 1; i += NCH) {
+      double rec[NODE_STRIDE];
+#pragma unroll
+      for (int t = 0; t < NODE_STRIDE; ++t) rec[t] = srows[(i - 1) * NODE_STRIDE + t];
+      node_pair<true, PAIR, true>(rec, i, omr, omi, mom, kpar, nq, dim, dim2, 0, 0, 0, 0, 0, 0, 0,
+                                  0, ap, am, npar);
+    }
+    return;
+  }
   for (int i = 2 + c; i <= npar - 2; i += NCH) {
     const double G1 = srows[(i - 1) * NODE_STRIDE + 0], H1 = srows[(i - 1) * NODE_STRIDE + 1];
     const double G2 = srows[(i - 1) * NODE_STRIDE + 2], H2 = srows[(i - 1) * NODE_STRIDE + 3];
@@ -289,8 +307,10 @@ __device__ __forceinline__ void sweep_nodes_pf(cdptr srows, int npar, int c, dou
                                                cd am[6]) {
   const double dim2 = dim * dim;
   const double mom = m * omr;
-  int i0 = 2 + c;
-  for (; i0 + (K - 1) * NCH <= npar - 2; i0 += NCH * K) {
+  // full-range rows take their end nodes (weight 1/2) in the same loop
+  const int lo = STD ? 1 : 2, hi = STD ? npar - 1 : npar - 2;
+  int i0 = lo + c;
+  for (; i0 + (K - 1) * NCH <= hi; i0 += NCH * K) {
     double rec[K][NODE_STRIDE];
 #pragma unroll
     for (int u = 0; u < K; ++u)
@@ -298,19 +318,20 @@ __device__ __forceinline__ void sweep_nodes_pf(cdptr srows, int npar, int c, dou
       for (int t = 0; t < NODE_STRIDE; ++t) rec[u][t] = srows[(i0 + u * NCH - 1) * NODE_STRIDE + t];
 #pragma unroll
     for (int u = 0; u < K; ++u)
-      node_pair<STD, PAIR>(rec[u], i0 + u * NCH, omr, omi, mom, kpar, nq, dim, dim2, pl1, pc1, pl2,
-                           pc2, ml1, mc1, ml2, mc2, ap, am);
+      node_pair<STD, PAIR, STD>(rec[u], i0 + u * NCH, omr, omi, mom, kpar, nq, dim, dim2, pl1, pc1,
+                                pl2, pc2, ml1, mc1, ml2, mc2, ap, am, npar);
   }
-  for (; i0 <= npar - 2; i0 += NCH) {
+  for (; i0 <= hi; i0 += NCH) {
     double rec[NODE_STRIDE];
 #pragma unroll
     for (int t = 0; t < NODE_STRIDE; ++t) rec[t] = srows[(i0 - 1) * NODE_STRIDE + t];
-    node_pair<STD, PAIR>(rec, i0, omr, omi, mom, kpar, nq, dim, dim2, pl1, pc1, pl2, pc2, ml1, mc1,
-                         ml2, mc2, ap, am);
+    node_pair<STD, PAIR, STD>(rec, i0, omr, omi, mom, kpar, nq, dim, dim2, pl1, pc1, pl2, pc2, ml1,
+                              mc1, ml2, mc2, ap, am, npar);
   }
 }
 
 constexpr double SPLIT_MAX = 256.0;
+constexpr int LPA = 1025;  // ppar points a sweep workgroup keeps in LDS (npar <= 1024)
 
 // The interior nodes of a full-range row pair in the split form (one lane per
 // omega, records read as wave-uniform scalars): with r = 1/den,
@@ -402,6 +423,7 @@ __global__ __launch_bounds__(64 * W) void k_sweep(const ProbDev *__restrict__ pr
                                                   const int *__restrict__ pidx) {
   constexpr int OMB = 64 / NCH;
   __shared__ double lchi[W][12][64];
+  __shared__ double lpa[LPA];
   // one problem per launch, or (pidx, one omega per workgroup: NCH = 64) the
   // problem of each omega
   const ProbDev &Pd = (NCH == 64 && pidx != nullptr) ? probp[pidx[blockIdx.x]] : *probp;
@@ -420,6 +442,14 @@ __global__ __launch_bounds__(64 * W) void k_sweep(const ProbDev *__restrict__ pr
     const SpecDev &S = Pd.sp[s];
     const int npar = S.npar;
     const double m = S.m;
+    // the species' ppar grid in LDS for the resonance classification
+    const double *pa = S.ppar;
+    if (S.nmax >= 0 && npar + 1 <= LPA) {
+      __syncthreads();
+      for (int t = threadIdx.x; t <= npar; t += blockDim.x) lpa[t] = S.ppar[t];
+      __syncthreads();
+      pa = lpa;
+    }
     // 2 pi dpperp dppar / 4 * q, times 2: node sums carry half the trapezoid weights
     const double scaleN = 2.0 * (2.0 * PI_D * S.dpperp * S.dppar_abs * 0.25 * S.q);
     // the species' running chi lives in LDS ([component][lane]) between rows
@@ -431,8 +461,8 @@ __global__ __launch_bounds__(64 * W) void k_sweep(const ProbDev *__restrict__ pr
       const double *rows = S.nodes + (size_t)nn * (npar - 1) * NODE_STRIDE;
       const cdptr srows = (cdptr)rows;
       RowClass rp, rm;
-      classify(S, Pd.P, kpar, omr, omi, nn, rp);
-      if (nn > 0) classify(S, Pd.P, kpar, omr, omi, -nn, rm);
+      classify(S, Pd.P, kpar, omr, omi, nn, rp, pa);
+      if (nn > 0) classify(S, Pd.P, kpar, omr, omi, -nn, rm, pa);
       const double nq = (1.0 * nn) * S.q;
       // interior (weight-2) ranges (a, b) of the two integrate() calls per sign
       const int pl1 = rp.a1 + 1, pc1 = max(rp.b1 - rp.a1 - 1, 0);
@@ -452,9 +482,10 @@ __global__ __launch_bounds__(64 * W) void k_sweep(const ProbDev *__restrict__ pr
       // (kpar max|p| + |n q|) / (m |w|) stays below SPLIT_MAX for every lane
       const bool split_ok =
           NCH == 1 && S.crec != nullptr &&
-          fabs(kpar) * fmax(fabs(S.ppar[0]), fabs(S.ppar[npar])) + fabs(nq) <=
+          fabs(kpar) * fmax(fabs(pa[0]), fabs(pa[npar])) + fabs(nq) <=
               SPLIT_MAX * m * sqrt(omr * omr + omi * omi);
-      if (__all(stdp && stdm)) {
+      const bool std_row = __all(stdp && stdm);
+      if (std_row) {
         if (__all(split_ok)) {
           const cdptr crow = (cdptr)(S.crec + (size_t)nn * (npar - 1) * CREC_STRIDE);
           const cdptr rs = (cdptr)(S.rsum + (size_t)nn * RSUM_STRIDE);
@@ -473,7 +504,8 @@ __global__ __launch_bounds__(64 * W) void k_sweep(const ProbDev *__restrict__ pr
         sweep_nodes<NCH, false, true>(srows, npar, c, omr, omi, m, kpar, nq, dim, pl1, pc1, pl2, pc2,
                                       ml1, mc1, ml2, mc2, ap, am);
       }
-      if (c == 0) {
+      // (with several lanes per omega a full-range row took its end nodes in the sweep)
+      if (c == 0 && !(NCH > 1 && std_row)) {
         endpoint_acc(rows, rp.a1, npar, m, kpar, omr, omi, nq, ap);
         endpoint_acc(rows, rp.b1, npar, m, kpar, omr, omi, nq, ap);
         if (rp.a2 > 0) {
@@ -576,11 +608,20 @@ __global__ __launch_bounds__(64 * W) void k_sweep_req(const ProbDev *__restrict_
   const double omr = ov.x, omi = ov.y;
   const int nspec = Pd.nspec;
   const double kpar = Pd.kpar;
+  __shared__ double lpa[LPA];
   int rowbase = 0;
   for (int s = 0; s < nspec; ++s) {
     const SpecDev &S = Pd.sp[s];
     const int npar = S.npar;
     const double m = S.m;
+    // the species' ppar grid in LDS for the resonance classification
+    const double *pa = S.ppar;
+    if (S.nmax >= 0 && npar + 1 <= LPA) {
+      __syncthreads();
+      for (int t = threadIdx.x; t <= npar; t += blockDim.x) lpa[t] = S.ppar[t];
+      __syncthreads();
+      pa = lpa;
+    }
     const double scaleN = 2.0 * (2.0 * PI_D * S.dpperp * S.dppar_abs * 0.25 * S.q);
     double run[12];
 #pragma unroll
@@ -591,8 +632,8 @@ __global__ __launch_bounds__(64 * W) void k_sweep_req(const ProbDev *__restrict_
       const double *rows = S.nodes + (size_t)nn * (npar - 1) * NODE_STRIDE;
       const cdptr srows = (cdptr)rows;
       RowClass rp, rm;
-      classify(S, Pd.P, kpar, omr, omi, nn, rp);
-      if (nn > 0) classify(S, Pd.P, kpar, omr, omi, -nn, rm);
+      classify(S, Pd.P, kpar, omr, omi, nn, rp, pa);
+      if (nn > 0) classify(S, Pd.P, kpar, omr, omi, -nn, rm, pa);
       const double nq = (1.0 * nn) * S.q;
       const int pl1 = rp.a1 + 1, pc1 = max(rp.b1 - rp.a1 - 1, 0);
       const int pl2 = rp.a2 + 1, pc2 = max(rp.b2 - rp.a2 - 1, 0);
@@ -615,7 +656,8 @@ __global__ __launch_bounds__(64 * W) void k_sweep_req(const ProbDev *__restrict_
         sweep_nodes_pf<NCH, false, true, K>(srows, npar, c, omr, omi, m, kpar, nq, dim, pl1, pc1, pl2, pc2,
                                       ml1, mc1, ml2, mc2, ap, am);
       }
-      if (c == 0) {
+      // (a full-range row took its end nodes in the sweep)
+      if (c == 0 && !(stdp && stdm)) {
         endpoint_acc(rows, rp.a1, npar, m, kpar, omr, omi, nq, ap);
         endpoint_acc(rows, rp.b1, npar, m, kpar, omr, omi, nq, ap);
         if (rp.a2 > 0) {
