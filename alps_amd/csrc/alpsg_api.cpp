@@ -125,6 +125,7 @@ struct alpsg_ctx {
   hipStream_t st = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t ev_sweep = nullptr, ev_res = nullptr;  // recorded after k_sweep, after k_resonant
+  hipEvent_t ev_done = nullptr;                     // end of a small disp call's work
   int nspec = 0, nperp = 0, npar = 0;
   int kperp_norm = 1, P = 5, NRI = 100;
   double vA = 0, Tlim = 0.01, Bessel_zero = 1e-45;
@@ -145,6 +146,15 @@ struct alpsg_ctx {
   DevBuf<double> crec[ALPSG_MAXSPEC], rsum[ALPSG_MAXSPEC];  // split-form sweep tables
   DevBuf<int> zidx;  // zeros: the problem index of every omega of a small batch
   PinBuf om_pin, det_pin;  // staging of om and det for batches of at most PIN_MAX omegas
+  // captured disp chains of small batches (index n = 1..REQ_MAX_N), with the
+  // shape and buffers they were captured with; graphs_ok turns off after a
+  // failed capture
+  struct Graph {
+    hipGraphExec_t exec = nullptr;
+    int shape[5] = {0, 0, 0, 0, 0};
+    const void *ptrs[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  } graphs[17];
+  bool graphs_ok = true;
   DevBuf<double> bmax;
   // problem descriptor
   ProbDev prob{};
@@ -185,10 +195,15 @@ struct alpsg_ctx {
     DevBuf<double> *all[] = {&pperp, &ppar, &f0, &df1, &df2, &qs_d, &ms_d, &int_ee, &fitc, &polyc,
                              &bmax};
     for (auto *b : all) b->release();
+    for (auto &g : graphs)
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    zidx.release();
     for (int s = 0; s < ALPSG_MAXSPEC; ++s) {
       bes[s].release();
       atab[s].release();
       nodes[s].release();
+      crec[s].release();
+      rsum[s].release();
       bmi[s].release();
       grel[s].release();
       pbrel[s].release();
@@ -219,6 +234,7 @@ struct alpsg_ctx {
     if (ev1) (void)hipEventDestroy(ev1);
     if (ev_sweep) (void)hipEventDestroy(ev_sweep);
     if (ev_res) (void)hipEventDestroy(ev_res);
+    if (ev_done) (void)hipEventDestroy(ev_done);
     if (st) (void)hipStreamDestroy(st);
   }
 };
@@ -450,11 +466,88 @@ int autotune(alpsg_ctx *h, int n, const double2 *om_host, int *shape_out) {
   return ALPSG_OK;
 }
 
+// det D of a small batch (n <= REQ_MAX_N, det only, no closed-form or
+// relativistic species) by replaying the captured chain pinned copy in ->
+// k_sweep_req -> k_resonant parts -> k_assemble -> pinned copy out.  Returns
+// 1 when the chain cannot be captured (the caller then runs it eagerly).
+int graph_disp(alpsg_ctx *h, int n, const double *om, double *det) {
+  const int ns = h->nspec;
+  LaunchShape sh = shape_for(h, n);
+  const int shp[5] = {sh.nch_a, sh.w_a, sh.rc_a, sh.nch_b, sh.rb};
+  // every buffer at its small-batch size before a capture (no allocation inside)
+  HIPCHK(h->om_d.ensure(REQ_MAX_N));
+  HIPCHK(h->det_d.ensure(REQ_MAX_N));
+  HIPCHK(h->chiA_d.ensure((size_t)REQ_MAX_N * REQ_TW * ns * 6));
+  HIPCHK(h->chiB_d.ensure((size_t)REQ_MAX_N * ns * 2 * 16 * 6));
+  HIPCHK(h->om_pin.ensure(PIN_MAX));
+  HIPCHK(h->det_pin.ensure(PIN_MAX));
+  if (h->zidx.n < (size_t)REQ_MAX_N) {
+    HIPCHK(h->zidx.ensure(REQ_MAX_N));
+    HIPCHK(hipMemsetAsync(h->zidx.p, 0, sizeof(int) * REQ_MAX_N, h->st));
+  }
+  const void *ptrs[6] = {h->om_d.p, h->det_d.p, h->chiA_d.p, h->chiB_d.p, h->prob_d.p, h->zidx.p};
+  alpsg_ctx::Graph &g = h->graphs[n];
+  bool same = g.exec != nullptr;
+  for (int k = 0; k < 5 && same; ++k) same = g.shape[k] == shp[k];
+  for (int k = 0; k < 6 && same; ++k) same = g.ptrs[k] == ptrs[k];
+  if (!same) {
+    if (g.exec) {
+      HIPCHK(hipStreamSynchronize(h->st));
+      (void)hipGraphExecDestroy(g.exec);
+      g.exec = nullptr;
+    }
+    HIPCHK(hipStreamSynchronize(h->st));
+    if (hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal) != hipSuccess) return 1;
+    hipError_t e = hipMemcpyAsync(h->om_d.p, h->om_pin.p, sizeof(double2) * n,
+                                  hipMemcpyHostToDevice, h->st);
+    if (e == hipSuccess)
+      e = launch_sweep_req(h->st, REQ_NCH, REQ_W, REQ_TW / REQ_W, REQ_K, h->prob_d.p, n, h->om_d.p,
+                           h->chiA_d.p, h->zidx.p);
+    if (e == hipSuccess)
+      e = launch_resonant(h->st, sh.nch_b, sh.rb, h->prob_d.p, ns, n, h->om_d.p, nullptr,
+                          h->chiB_d.p, nullptr, h->any_general_fit);
+    if (e == hipSuccess)
+      e = launch_assemble(h->st, h->prob_d.p, ns, n, REQ_TW, sh.rb, h->om_d.p, nullptr, h->chiA_d.p,
+                          h->chiB_d.p, nullptr, nullptr, nullptr, 0, nullptr, h->det_d.p, nullptr,
+                          nullptr, nullptr);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(h->det_pin.p, h->det_d.p, sizeof(double2) * n, hipMemcpyDeviceToHost,
+                         h->st);
+    hipGraph_t graph = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(h->st, &graph);
+    hipGraphExec_t exec = nullptr;
+    if (e == hipSuccess && e2 == hipSuccess && graph != nullptr &&
+        hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess) {
+      g.exec = exec;
+      for (int k = 0; k < 5; ++k) g.shape[k] = shp[k];
+      for (int k = 0; k < 6; ++k) g.ptrs[k] = ptrs[k];
+    }
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipGetLastError();
+    if (!g.exec) return 1;
+  }
+  std::memcpy(h->om_pin.p, om, sizeof(double2) * n);
+  HIPCHK(hipGraphLaunch(g.exec, h->st));
+  HIPCHK(hipEventRecord(h->ev_done, h->st));
+  hipError_t q;
+  while ((q = hipEventQuery(h->ev_done)) == hipErrorNotReady) {
+  }
+  HIPCHK(q);
+  std::memcpy(det, h->det_pin.p, sizeof(double2) * n);
+  h->evals += n;
+  return ALPSG_OK;
+}
+
 int run_disp(alpsg_ctx *h, int n, const double *om, double *det, double *chi, double *low,
              double *wave) {
   if (!h->have_k) return fail(ALPSG_E_STATE, "alpsg_set_k has not been called");
   if (n < 0) return fail(ALPSG_E_ARG, "negative batch size");
   if (n == 0) return ALPSG_OK;
+  if (h->graphs_ok && n <= REQ_MAX_N && !chi && !low && !wave && !h->any_bm && !h->any_rel) {
+    const int rc = graph_disp(h, n, om, det);
+    if (rc != 1) return rc;
+    h->graphs_ok = false;  // capture is not available: run eagerly from now on
+  }
   HIPCHK(h->om_d.ensure(n));
   const bool pin = n <= PIN_MAX;
   if (pin) {
@@ -484,7 +577,16 @@ int run_disp(alpsg_ctx *h, int n, const double *om, double *det, double *chi, do
   int relerr = 0;
   if (h->any_rel)
     HIPCHK(hipMemcpyAsync(&relerr, h->err_d.p, sizeof(int), hipMemcpyDeviceToHost, h->st));
-  HIPCHK(hipStreamSynchronize(h->st));
+  if (pin && !h->any_rel && !chi && !low && !wave) {
+    // a root-search step waits on this result: poll for it rather than sleep
+    HIPCHK(hipEventRecord(h->ev_done, h->st));
+    hipError_t q;
+    while ((q = hipEventQuery(h->ev_done)) == hipErrorNotReady) {
+    }
+    HIPCHK(q);
+  } else {
+    HIPCHK(hipStreamSynchronize(h->st));
+  }
   if (pin) std::memcpy(det, h->det_pin.p, sizeof(double2) * n);
   if (relerr) {
     const int zero = 0;
@@ -1026,6 +1128,7 @@ int alpsg_create(const alpsg_plasma_desc *d, alpsg_handle *out) {
   HIPCHK(hipEventCreate(&h->ev1));
   HIPCHK(hipEventCreate(&h->ev_sweep));
   HIPCHK(hipEventCreate(&h->ev_res));
+  HIPCHK(hipEventCreateWithFlags(&h->ev_done, hipEventDisableTiming));
   const size_t ngrid = (size_t)nspec * (nperp + 1) * (npar + 1);
   HIPCHK(h->pperp.ensure(h->pperp_h.size()));
   HIPCHK(h->ppar.ensure(h->ppar_h.size()));

@@ -93,6 +93,29 @@ def test_single_omega_paths_agree():
     assert np.all(np.abs(a - b) <= 1e-12 * np.abs(a) + 1e-300)
 
 
+def test_replayed_small_batches_match_the_large_batch():
+    """Batches of up to 16 omegas (det only) replay a captured graph of their
+    disp chain: every replay, interleaved with large eager batches and with a
+    change of k, equals the large-batch evaluation of the same omegas."""
+    from alps_amd.engine import Engine
+    p = cases.three_species()
+    E = Engine(p)
+    oms = np.array(cases.BRANCH_OMEGAS * 3)
+    assert oms.size > 16
+    for k in ((0.5, 0.2), (0.3, 0.3)):
+        E.set_k(*k)
+        ref, chi, low, wave = E.disp_batch(oms, want_tensors=True)
+        scale = np.array([cases.det_scale(w) for w in wave])
+        for rep in range(3):
+            for n in (1, 3, 16):
+                for i0 in range(0, 16, n):
+                    d = E.disp_batch(oms[i0:i0 + n])
+                    e = np.abs(d - ref[i0:i0 + n]) / scale[i0:i0 + n]
+                    assert np.all(e <= 1e-12), (k, rep, n, i0, e.max())
+            again = E.disp_batch(oms)  # eager work between the replays
+            assert np.all(np.abs(again - ref) / scale <= 1e-13)
+
+
 def test_df0_input_matches_f0_input():
     """Passing df0 (as the Fortran host holds it) equals deriving it on the device."""
     from alps_amd.engine import Engine
