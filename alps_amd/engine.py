@@ -21,6 +21,7 @@ library is missing the import fails: there is no CPU fallback.
 from __future__ import annotations
 
 import ctypes as C
+import dataclasses
 import os
 
 import numpy as np
@@ -43,7 +44,7 @@ EXPORTS = ["alpsg_last_error", "alpsg_create", "alpsg_destroy", "alpsg_set_k",
            "alpsg_server_add_plasma", "alpsg_server_remove_plasma", "alpsg_server_submit_k",
            "alpsg_server_submit_batch", "alpsg_om_double_scan",
            "alpsg_server_submit", "alpsg_server_poll", "alpsg_server_stats",
-           "alpsg_server_destroy"]
+           "alpsg_server_destroy", "alpsg_determine_param_fit"]
 
 
 class AlpsgError(RuntimeError):
@@ -83,6 +84,11 @@ class _RootParams(C.Structure):
 class _ScanParams(C.Structure):
     _fields_ = [("scan_type", C.c_int), ("log_scan", C.c_int), ("diff", C.c_double),
                 ("diff2", C.c_double), ("n_out", C.c_int), ("n_res", C.c_int)]
+
+
+class _FitParams(C.Structure):
+    _fields_ = [("maxsteps_fit", C.c_int), ("lambda_initial_fit", C.c_double),
+                ("lambdafac_fit", C.c_double), ("epsilon_fit", C.c_double)]
 
 
 _lib = None
@@ -141,6 +147,9 @@ def load_library(path: str = LIB_PATH):
                  "alpsg_server_remove_plasma", "alpsg_server_submit_k",
                  "alpsg_server_submit_batch"]:
         getattr(L, name).restype = C.c_int
+    L.alpsg_determine_param_fit.argtypes = [C.POINTER(_Desc), C.POINTER(_FitParams), dp, ip, dp,
+                                            dp, dp]
+    L.alpsg_determine_param_fit.restype = C.c_int
     L.alpsg_last_kernel_ms.argtypes = [vp]
     L.alpsg_last_kernel_ms.restype = C.c_double
     for name in ["alpsg_create", "alpsg_destroy", "alpsg_set_k", "alpsg_disp_batch",
@@ -226,6 +235,32 @@ def _make_desc(plasma: Plasma, use_df0: bool = False):
     d.poly_fit_coeffs = keep["poly_fit_coeffs"].ctypes.data
     d.poly_max_order = p.poly_fit_coeffs.shape[2] - 1
     return d, keep
+
+
+def determine_param_fit(plasma: Plasma, maxsteps_fit: int = 500, lambda_initial_fit: float = 1.0,
+                        lambdafac_fit: float = 10.0, epsilon_fit: float = 1e-8):
+    """determine_param_fit (ALPS_analyt.f90:365-728) on the device: the
+    Levenberg-Marquardt fits from the guesses in ``param_fit[:, 0]`` and the
+    Chebyshev coefficients of ACmethod-2 species.  Returns the plasma with the
+    fitted ``param_fit``, ``fit_type``, ``perp_correction`` and
+    ``poly_fit_coeffs``, and the summed fit quality."""
+    L = load_library()
+    d, keep = _make_desc(plasma)
+    p = plasma
+    pf = np.array(p.param_fit, dtype=np.float64, order="F")
+    ft = np.array(p.fit_type, dtype=np.int32, order="F")
+    pc = np.array(p.perp_correction, dtype=np.float64, order="F")
+    poly = np.array(p.poly_fit_coeffs, dtype=np.float64, order="F")
+    q = C.c_double(0.0)
+    fpar = _FitParams(int(maxsteps_fit), float(lambda_initial_fit), float(lambdafac_fit),
+                      float(epsilon_fit))
+    _check(L.alpsg_determine_param_fit(C.byref(d), C.byref(fpar), _dp(pf), _ip(ft), _dp(pc),
+                                       _dp(poly), C.byref(q)))
+    out = dataclasses.replace(p, param_fit=np.ascontiguousarray(pf),
+                              fit_type=np.ascontiguousarray(ft),
+                              perp_correction=np.ascontiguousarray(pc),
+                              poly_fit_coeffs=np.ascontiguousarray(poly))
+    return out, q.value
 
 
 class Engine:

@@ -91,6 +91,34 @@ typedef struct alpsg_root_params {
 /* Error text of the last failing call on this thread. */
 const char *alpsg_last_error(void);
 
+/* Fit controls (&system namelist, ALPS_var.f90:288-297). */
+typedef struct alpsg_fit_params {
+  int maxsteps_fit;          /* 500 */
+  double lambda_initial_fit; /* 1.0 */
+  double lambdafac_fit;      /* 10.0 */
+  double epsilon_fit;        /* 1e-8 */
+} alpsg_fit_params;
+
+/* determine_param_fit (ALPS_analyt.f90:365-728) on the device, called before
+   alpsg_create as ALPS.f90:80 calls it before the disp() path: the
+   Levenberg-Marquardt fit of every ACmethod-1 species row by row (pperp rows,
+   or Gamma rows of a relativistic species), starting from the guesses in
+   desc->param_fit(s,0,:,:); the Chebyshev coefficients of ACmethod-2 species;
+   and the reset of use_bM, ACmethod-0 and ACmethod-2 species (param_fit 0,
+   fit_type 1, perp_correction 1).  Needs desc->f0 (and the *_rel grids for
+   relativistic species).  Outputs in the layouts of the description:
+   param_fit (1:nspec,0:param_fit_nperp,1:5,1:max_fits), fit_type and
+   perp_correction (1:nspec,1:max_fits), poly_fit_coeffs
+   (1:nspec,0:nperp,0:poly_max_order; may be NULL without ACmethod 2); they may
+   alias the description's arrays.  Rows of param_fit past a species' last
+   fitted row are left as they are.  quality (may be NULL) gets the summed
+   least-squares quality.  fp NULL selects the defaults above.  ALPSG_E_ARG
+   for alps_error(2) (fewer points than parameters), a singular fit matrix
+   (where the reference stops) or alps_error(10) (poly_kind other than 1). */
+int alpsg_determine_param_fit(const alpsg_plasma_desc *desc, const alpsg_fit_params *fp,
+                              double *param_fit, int *fit_type, double *perp_correction,
+                              double *poly_fit_coeffs, double *quality);
+
 /* Create a handle: uploads pp and f0 (or df0), builds df0 on the device with
    the central differences of derivative_f0 (ALPS_fns.f90:103-115), the
    omega-independent int_ee term (ALPS_fns.f90:1457-1555) and the fit tables.

@@ -217,6 +217,18 @@ int orc_om_double_scan(orc_plasma *P, const orc_scan *s1, const orc_scan *s2, in
                        double complex *ef, double complex *bf, double complex *Us,
                        double complex *ds, double *Ps, double *Ps_split, double *W_EM);
 
+/* determine_param_fit (ALPS_analyt.f90:365-728): the Levenberg-Marquardt fits
+   of every ACmethod-1 species, row by row from the initial guesses in
+   param_fit[s][0] (each row starts from the previous row's result), the
+   Chebyshev coefficients of ACmethod-2 species, and the reset of use_bM,
+   ACmethod-0 and ACmethod-2 species (zero param_fit, fit_type 1,
+   perp_correction 1).  Updates P in place; qualitytotal and steps_total
+   (both may be NULL) get the summed least-squares quality and LM steps.
+   Returns 0, 2 for alps_error(2) (fewer points than parameters), 3 for a
+   singular fit matrix (the reference stops), 10 for alps_error(10). */
+int orc_determine_param_fit(orc_plasma *P, int maxsteps, double lambda0, double lambdafac, double eps,
+                            double *qualitytotal, int *steps_total);
+
 #ifdef __cplusplus
 }
 #endif

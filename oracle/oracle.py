@@ -89,6 +89,8 @@ def lib():
         L.orc_bessj1.restype = C.c_double
         L.orc_bessj1.argtypes = [C.c_double]
         L.orc_derivative_f0.argtypes = [P]
+        L.orc_determine_param_fit.restype = C.c_int
+        L.orc_determine_param_fit.argtypes = [P, C.c_int, C.c_double, C.c_double, C.c_double, dp, ip]
         L.orc_set_k.argtypes = [P, C.c_double, C.c_double]
         L.orc_set_k.restype = C.c_int
         L.orc_free_k.argtypes = [P]
@@ -428,6 +430,22 @@ class Oracle:
                     Us=us.reshape(sh + (ns, 3)), ds=ds.reshape(sh + (ns,)),
                     Ps=ps.reshape(sh + (ns,)), Ps_split=psp.reshape(sh + (ns, 4)),
                     W_EM=ew.reshape(sh))
+
+    def determine_param_fit(self, maxsteps=500, lambda0=1.0, lambdafac=10.0, eps=1e-8):
+        """determine_param_fit (ALPS_analyt.f90:365-728) from the initial
+        guesses in param_fit[:, 0]; updates this oracle's fit state (as the
+        reference updates its globals) and returns param_fit [nspec, np+1, 5,
+        max_fits], fit_type, perp_correction [nspec, max_fits],
+        poly_fit_coeffs, the summed quality and LM step count."""
+        q, st = C.c_double(0.0), C.c_int(0)
+        rc = lib().orc_determine_param_fit(C.byref(self.s), int(maxsteps), float(lambda0),
+                                           float(lambdafac), float(eps), C.byref(q), C.byref(st))
+        p = self.plasma
+        mf = p.max_fits
+        ft = np.array([[self.s.fit_type[i][f] for f in range(mf)] for i in range(p.nspec)])
+        pc = np.array([[self.s.perp_correction[i][f] for f in range(mf)] for i in range(p.nspec)])
+        return dict(rc=rc, param_fit=self._pf[..., :mf].copy(), fit_type=ft, perp_correction=pc,
+                    poly_fit_coeffs=self._poly.copy(), quality=q.value, steps=st.value)
 
     def om_scan(self, scan_type, log_scan, diff, diff2, n_out, n_res, roots):
         roots = np.array(roots, dtype=np.complex128)

@@ -152,3 +152,118 @@ def det_scale(wave):
     w = np.abs(wave)
     return (w[0, 0] * (w[1, 1] * w[2, 2] + w[1, 2] ** 2) + 2 * w[0, 1] * w[1, 2] * w[0, 2]
             + w[0, 2] ** 2 * w[1, 1] + w[0, 1] ** 2 * w[2, 2])
+
+
+# ---------------------------------------------------------------------------
+# Fit set-ups (determine_param_fit): tables whose least-squares optimum is
+# known in closed form, and initial guesses away from it.
+# ---------------------------------------------------------------------------
+def with_guess(p, mult, add=None):
+    """p with param_fit reduced to the &ffit guesses: row 0 of the closed-form
+    parameters times ``mult`` plus ``add`` ([5, nfits] each), other rows 0."""
+    import dataclasses
+    pf = np.zeros_like(p.param_fit)
+    g = p.param_fit[:, 0] * np.asarray(mult)
+    if add is not None:
+        g = g + np.asarray(add)
+    pf[:, 0] = g
+    return dataclasses.replace(p, param_fit=pf)
+
+
+def fit_maxwell(nperp=24, npar=48):
+    """Drifting anisotropic protons (ideal perp_correction: u1 constant) and
+    electrons whose perp_correction differs from the table's (u1 varies with
+    pperp): maxwell_fit_params is the exact optimum of the logarithmic fit."""
+    return bimaxwellian_plasma(ms=[1.0, M_E], qs=[1.0, -1.0], ns=[1.0, 1.0], taus=[1.0, 1.0],
+                               alphs=[2.0, 1.0], ps=[0.4, 0.0], beta=1.0, vA=1e-4, nperp=nperp,
+                               npar=npar, maxP=6.0, perpcorr=[None, 1.7e3],
+                               positions_principal=3, n_resonance_interval=100, Tlim=0.01,
+                               Bessel_zero=1e-45)
+
+
+def fit_kappa(nperp=24, npar=48):
+    """Bi-kappa protons (kappa 4, anisotropic) and electrons (kappa 5)."""
+    return kappa_plasma(ms=[1.0, M_E], qs=[1.0, -1.0], ns=[1.0, 1.0], taus=[1.0, 1.0],
+                        alphs=[2.0, 1.0], ps=[0.0, 0.0], kappas=[4.0, 5.0], beta=1.0, vA=1e-4,
+                        nperp=nperp, npar=npar, maxP=6.0, positions_principal=3,
+                        n_resonance_interval=100, Tlim=0.01, Bessel_zero=1e-45)
+
+
+def _replace_species0(p, f0_0, fit_types, pcs, pars, logfit=1):
+    """Species 0 of p with table f0_0, n_fits = len(fit_types) fits of the given
+    types, perp_corrections and (constant over rows) parameters [5] each."""
+    import dataclasses
+    nf = len(fit_types)
+    mf = max(nf, p.max_fits)
+    f0 = p.f0.copy()
+    f0[0] = f0_0
+    ft = np.ones((p.nspec, mf), dtype=np.int64)
+    pc = np.ones((p.nspec, mf))
+    pf = np.zeros((p.nspec, p.param_fit.shape[1], 5, mf))
+    ft[:, : p.max_fits] = p.fit_type
+    pc[:, : p.max_fits] = p.perp_correction
+    pf[..., : p.max_fits] = p.param_fit
+    ft[0, :nf] = fit_types
+    pc[0, :nf] = pcs
+    pf[0, :, :, :] = 0.0
+    for f in range(nf):
+        pf[0, :, :, f] = pars[f]
+    nfits = p.n_fits.copy()
+    nfits[0] = nf
+    lf = p.logfit.copy()
+    lf[0] = logfit
+    return dataclasses.replace(p, f0=f0, fit_type=ft, perp_correction=pc, param_fit=pf,
+                               n_fits=nfits, logfit=lf)
+
+
+def fit_core_beam(nperp=24, npar=48):
+    """Protons as a Maxwellian core plus a 10 % beam drifting at 1.5: two
+    fit_type-1 fits whose optimum is the two components' parameters."""
+    p = small_bimax(nperp=nperp, npar=npar)
+    pperp = p.pp[0, :, 0, 0][:, None]
+    ppar = p.pp[0, 0, :, 1][None, :]
+    core = (0.17, 1.0, 0.0, 0.0, 0.0)
+    beam = (0.025, 2.0, 1.5, 0.0, 0.0)
+    pcs = (1.0, 1.6)
+    f = (core[0] * np.exp(-pcs[0] * pperp ** 2) * np.exp(-core[1] * (ppar - core[2]) ** 2)
+         + beam[0] * np.exp(-pcs[1] * pperp ** 2) * np.exp(-beam[1] * (ppar - beam[2]) ** 2))
+    return _replace_species0(p, f, [1, 1], pcs, [core, beam])
+
+
+def fit_moyal(nperp=24, npar=48):
+    """Protons with a bi-Moyal table (fit_type 6) of known parameters."""
+    p = small_bimax(nperp=nperp, npar=npar)
+    pperp = p.pp[0, :, 0, 0][:, None]
+    ppar = p.pp[0, 0, :, 1][None, :]
+    u = (0.3, 0.04, 0.2, 0.6, 0.0)
+    pc = 0.08
+    a = u[3] * pc * pperp ** 2 + u[1] * (ppar - u[2]) ** 2
+    f = u[0] * np.exp(0.5 * (a - np.exp(a)))
+    return _replace_species0(p, f, [6], [pc], [u])
+
+
+def fit_rel5(ngamma=16, npparbar=60):
+    """A relativistic species with synthetic (Gamma, pparbar) rows: f0_rel of
+    the fit_type-5 form inside a window that widens with Gamma and -1 outside
+    it; the first rows hold too few points for a fit.  The other species is a
+    non-relativistic bi-Maxwellian."""
+    import dataclasses
+    b = small_bimax(nperp=12, npar=24)
+    g = 1.0 + 0.25 * np.arange(ngamma + 1)
+    pb = np.linspace(-6.0, 6.0, npparbar + 1)
+    G = np.repeat(g[:, None], npparbar + 1, axis=1)
+    PB = np.repeat(pb[None, :], ngamma + 1, axis=0)
+    u = (2.0, 0.05, 0.0)
+    pc = 1.3
+    f = u[0] * np.exp(-pc * G) * np.exp(-u[1] * (PB - u[2]) ** 2)
+    half = 6.0 * np.sqrt(g ** 2 - 1.0) / np.sqrt(g[-1] ** 2 - 1.0)
+    f = np.where(np.abs(PB) <= half[:, None], f, -1.0)
+    npf = max(12, ngamma)
+    pf = np.zeros((2, npf + 1, 5, 1))
+    pf[0, :13] = b.param_fit[0]
+    pf[1, :, :3, 0] = u
+    return dataclasses.replace(
+        b, fit_type=np.array([[1], [5]]), perp_correction=np.array([[b.perp_correction[0, 0]], [pc]]),
+        param_fit=pf, relativistic=[0, 1], ngamma=ngamma, npparbar=npparbar,
+        gamma_rel=G[None], pparbar_rel=PB[None], f0_rel=f[None],
+        df0_rel=np.zeros((1, ngamma + 1, npparbar + 1, 2)))

@@ -136,6 +136,27 @@ def test_null_handle_calls_fail_cleanly():
     assert L.alpsg_autotune(None, 16, None, None) == 1
     assert L.alpsg_set_shape(None, 16, None) == 1
     assert L.alpsg_destroy(None) == 0
+    assert L.alpsg_determine_param_fit(None, None, None, None, None, None, None) == 1
+
+
+def test_fit_setup_rejects_bad_input_without_a_device():
+    import ctypes as C
+    from alps_amd.engine import load_library
+    L = load_library()
+    p = cases.small_bimax()
+    pf = np.zeros(p.param_fit.size)
+    ft = np.ones(p.nspec, dtype=np.int32)
+    pc = np.ones(p.nspec)
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+    # f0 is needed (a description with df0 only cannot be fitted)
+    d, keep = _desc_for(p, f0=None)
+    assert L.alpsg_determine_param_fit(C.byref(d), None, dp(pf), ip(ft), dp(pc), None, None) == 1
+    assert "f0" in L.alpsg_last_error().decode()
+    # a fit type outside 1..6
+    d, keep = _desc_for(p, fit_type=np.asfortranarray(np.array([[7], [1]], dtype=np.int32)))
+    assert L.alpsg_determine_param_fit(C.byref(d), None, dp(pf), ip(ft), dp(pc), None, None) == 1
+    assert "fit_type" in L.alpsg_last_error().decode()
 
 
 def test_fortran_binding_compiles():
