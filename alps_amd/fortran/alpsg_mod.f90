@@ -35,6 +35,12 @@ module alpsg
      type(c_ptr) :: current_int = c_null_ptr
   end type alpsg_plasma_desc
 
+  !! fit controls (&system: maxsteps_fit, lambda_initial_fit, lambdafac_fit, epsilon_fit)
+  type, bind(C), public :: alpsg_fit_params
+     integer(c_int) :: maxsteps_fit = 500
+     real(c_double) :: lambda_initial_fit = 1.d0, lambdafac_fit = 10.d0, epsilon_fit = 1.d-8
+  end type alpsg_fit_params
+
   type, bind(C), public :: alpsg_root_params
      integer(c_int) :: secant_method, numiter
      real(c_double) :: D_threshold, D_prec, D_gap, D_tol
@@ -46,7 +52,7 @@ module alpsg
   public :: alpsg_autotune, alpsg_set_shape, alpsg_get_shape, alpsg_calc_eigen
   public :: alpsg_server_create, alpsg_server_add_plasma, alpsg_server_remove_plasma
   public :: alpsg_server_submit_k, alpsg_server_submit_batch, alpsg_server_poll
-  public :: alpsg_server_destroy
+  public :: alpsg_server_destroy, alpsg_determine_param_fit
 
   interface
      function alpsg_create(desc, h) bind(C, name='alpsg_create') result(rc)
@@ -175,6 +181,18 @@ module alpsg
        integer(c_int), intent(out) :: shape_out(5)
        integer(c_int) :: rc
      end function alpsg_get_shape
+
+     !! determine_param_fit (ALPS_analyt.f90:365) on the device; the outputs
+     !! may be the description's own arrays (param_fit, fit_type, ...)
+     function alpsg_determine_param_fit(desc, fp, param_fit, fit_type, perp_correction, &
+          poly_fit_coeffs, quality) bind(C, name='alpsg_determine_param_fit') result(rc)
+       import :: c_int, c_ptr, c_double, alpsg_plasma_desc, alpsg_fit_params
+       type(alpsg_plasma_desc), intent(in) :: desc
+       type(alpsg_fit_params), intent(in) :: fp
+       type(c_ptr), value :: param_fit, fit_type, perp_correction, poly_fit_coeffs
+       real(c_double), intent(out) :: quality
+       integer(c_int) :: rc
+     end function alpsg_determine_param_fit
 
      !! root server: queue secant_osc solves of any plasma and k, collect roots
      function alpsg_server_create(hbm_bytes, max_slots, s) &
