@@ -51,6 +51,10 @@ hipError_t launch_split_tables(hipStream_t st, int npar, int nmax, double kpar, 
                                const double *nodes, double *crec, double *rsum);
 hipError_t launch_rel_bessel(hipStream_t st, int npts, int nmax, double zfac, const double *grel,
                              const double *pbrel, const int *mtab, double *out);
+hipError_t launch_map_post(hipStream_t st, int nr, int ni, const double2 *cal, double2 *calo,
+                           double *val, unsigned char *hit);
+hipError_t launch_osc_round(hipStream_t st, int n, alpsg_osc::OscMachine *S, const double2 *det,
+                            double2 *om, int *round_ctr, int *remaining);
 hipError_t launch_rel(hipStream_t st, const ProbDev *prob, int nspec, int nmaxR, int NB, int nom,
                       const double2 *om, const int *active, double2 *chiR, double2 *lowR,
                       int *flags, int *err);
@@ -162,6 +166,25 @@ struct alpsg_ctx {
   // batch staging
   DevBuf<double2> om_d, det_d, chi_d, low_d, wave_d, chiA_d, chiB_d, lowA_d, lowB_d;
   DevBuf<int> act_d;
+  // map_search outputs on the device: D with sentinels, log10|D|, minimum flags
+  DevBuf<double2> mcal_d;
+  DevBuf<double> mval_d;
+  DevBuf<unsigned char> mhit_d;
+  std::vector<unsigned char> mhit_h;
+  // secant_osc rounds on the device: the solves' state machines, the round
+  // counter and unfinished count ([0], [1]) and its pinned copy, and graphs of
+  // OSC_K rounds keyed by batch size, launch shape and buffers
+  DevBuf<alpsg_osc::OscMachine> osc_d;
+  DevBuf<int> oscctl_d;
+  int *oscctl_pin = nullptr;
+  struct OscGraph {
+    hipGraphExec_t exec = nullptr;
+    int n = 0;
+    int shape[5] = {0, 0, 0, 0, 0};
+    const void *ptrs[9] = {nullptr};
+  } osc_graphs[8];
+  int osc_graph_next = 0;
+  bool osc_graphs_ok = true;
   std::vector<double> root_ms;  // completion time of each root of the last secant batch
   bool any_general_fit = false;  // a species needs eval_fit in the Landau term
   // closed-form (use_bM) species and their &bM_spec parameters
@@ -197,7 +220,15 @@ struct alpsg_ctx {
     for (auto *b : all) b->release();
     for (auto &g : graphs)
       if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    for (auto &g : osc_graphs)
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    osc_d.release();
+    oscctl_d.release();
+    if (oscctl_pin) (void)hipHostFree(oscctl_pin);
     zidx.release();
+    mcal_d.release();
+    mval_d.release();
+    mhit_d.release();
     for (int s = 0; s < ALPSG_MAXSPEC; ++s) {
       bes[s].release();
       atab[s].release();
@@ -612,21 +643,170 @@ int run_disp(alpsg_ctx *h, int n, const double *om, double *det, double *chi, do
 
 typedef std::complex<double> cplx;
 
+const int OSC_K = 8;         // secant rounds per graph replay
+const int OSC_MAX_N = 1024;  // solves of one device-round batch (one workgroup steps them)
+
+// secant_osc for many roots with the state machines on the device: a graph
+// holds OSC_K rounds of [disp of every solve's requested frequency, one step of
+// every unfinished solve], and the host only replays it until no solve is left.
+// Every solve's frequency is evaluated in every round, also after it finished
+// (its value is then ignored), so the batch and the launch shape stay fixed.
+// Returns -1 when the batch cannot run this way (the caller then steps the
+// solves on the host).
+int osc_graph_batch(alpsg_ctx *h, const alpsg_root_params *rp, int n, cplx *om, cplx *Dfin,
+                    int *iters) {
+  if (!h->graphs_ok || !h->osc_graphs_ok || h->any_bm || h->any_rel || n > OSC_MAX_N) return -1;
+  const int ns = h->nspec;
+  LaunchShape sh = shape_for(h, n);
+  const bool req = n <= REQ_MAX_N;
+  const int TW = req ? REQ_TW : sh.w_a * sh.rc_a;
+  const int shp[5] = {sh.nch_a, sh.w_a, sh.rc_a, sh.nch_b, sh.rb};
+  HIPCHK(h->om_d.ensure(std::max(n, REQ_MAX_N)));
+  HIPCHK(h->det_d.ensure(std::max(n, REQ_MAX_N)));
+  HIPCHK(h->chiA_d.ensure((size_t)n * TW * ns * 6));
+  HIPCHK(h->chiB_d.ensure((size_t)n * ns * 2 * sh.rb * 6));
+  if (req && h->zidx.n < (size_t)REQ_MAX_N) {
+    HIPCHK(h->zidx.ensure(REQ_MAX_N));
+    HIPCHK(hipMemsetAsync(h->zidx.p, 0, sizeof(int) * REQ_MAX_N, h->st));
+  }
+  HIPCHK(h->osc_d.ensure(n));
+  HIPCHK(h->oscctl_d.ensure(2));
+  if (!h->oscctl_pin)
+    HIPCHK(hipHostMalloc((void **)&h->oscctl_pin, 2 * sizeof(int), hipHostMallocDefault));
+  std::vector<alpsg_osc::OscMachine> S(n);
+  std::vector<double2> w(n);
+  for (int r = 0; r < n; ++r) {
+    S[r].start(alpsg_osc::zmk(om[r].real(), om[r].imag()), rp->numiter, rp->D_threshold, rp->D_prec);
+    const alpsg_osc::zc x = S[r].want();
+    w[r] = make_double2(x.re, x.im);
+  }
+  const int ctl0[2] = {0, n};
+  HIPCHK(hipMemcpyAsync(h->osc_d.p, S.data(), sizeof(alpsg_osc::OscMachine) * n,
+                        hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(h->om_d.p, w.data(), sizeof(double2) * n, hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(h->oscctl_d.p, ctl0, sizeof(ctl0), hipMemcpyHostToDevice, h->st));
+  const void *ptrs[9] = {h->om_d.p,  h->det_d.p,  h->chiA_d.p,    h->chiB_d.p,   h->prob_d.p,
+                         req ? h->zidx.p : nullptr, h->osc_d.p, h->oscctl_d.p, h->oscctl_pin};
+  alpsg_ctx::OscGraph *g = nullptr;
+  for (auto &c : h->osc_graphs) {
+    if (!c.exec || c.n != n) continue;
+    bool same = true;
+    for (int k = 0; k < 5 && same; ++k) same = c.shape[k] == shp[k];
+    for (int k = 0; k < 9 && same; ++k) same = c.ptrs[k] == ptrs[k];
+    if (same) {
+      g = &c;
+      break;
+    }
+  }
+  if (!g) {
+    g = &h->osc_graphs[h->osc_graph_next];
+    h->osc_graph_next = (h->osc_graph_next + 1) % 8;
+    HIPCHK(hipStreamSynchronize(h->st));
+    if (g->exec) {
+      (void)hipGraphExecDestroy(g->exec);
+      g->exec = nullptr;
+    }
+    if (hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      (void)hipGetLastError();
+      h->osc_graphs_ok = false;
+      return -1;
+    }
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < OSC_K && e == hipSuccess; ++k) {
+      if (req)
+        e = launch_sweep_req(h->st, REQ_NCH, REQ_W, REQ_TW / REQ_W, REQ_K, h->prob_d.p, n, h->om_d.p,
+                             h->chiA_d.p, h->zidx.p);
+      else
+        e = launch_sweep(h->st, sh.nch_a, sh.w_a, sh.rc_a, h->prob_d.p, n, h->om_d.p, nullptr,
+                         h->chiA_d.p, nullptr);
+      if (e == hipSuccess)
+        e = launch_resonant(h->st, sh.nch_b, sh.rb, h->prob_d.p, ns, n, h->om_d.p, nullptr,
+                            h->chiB_d.p, nullptr, h->any_general_fit);
+      if (e == hipSuccess)
+        e = launch_assemble(h->st, h->prob_d.p, ns, n, TW, sh.rb, h->om_d.p, nullptr, h->chiA_d.p,
+                            h->chiB_d.p, nullptr, nullptr, nullptr, 0, nullptr, h->det_d.p, nullptr,
+                            nullptr, nullptr);
+      if (e == hipSuccess)
+        e = launch_osc_round(h->st, n, h->osc_d.p, h->det_d.p, h->om_d.p, h->oscctl_d.p,
+                             h->oscctl_d.p + 1);
+    }
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(h->oscctl_pin, h->oscctl_d.p, 2 * sizeof(int), hipMemcpyDeviceToHost, h->st);
+    hipGraph_t graph = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(h->st, &graph);
+    hipGraphExec_t exec = nullptr;
+    if (e == hipSuccess && e2 == hipSuccess && graph != nullptr &&
+        hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess) {
+      g->exec = exec;
+      g->n = n;
+      for (int k = 0; k < 5; ++k) g->shape[k] = shp[k];
+      for (int k = 0; k < 9; ++k) g->ptrs[k] = ptrs[k];
+    }
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipGetLastError();
+    if (!g->exec) {
+      h->osc_graphs_ok = false;
+      return -1;
+    }
+  }
+  // the bound on rounds: two initial evaluations and at most four per iteration
+  const int maxrep = (2 + 4 * std::max(rp->numiter, 1)) / OSC_K + 2;
+  const auto t0 = std::chrono::steady_clock::now();
+  std::vector<double> trep;
+  bool finished = false;
+  for (int rep = 0; rep < maxrep; ++rep) {
+    HIPCHK(hipGraphLaunch(g->exec, h->st));
+    HIPCHK(hipEventRecord(h->ev_done, h->st));
+    hipError_t q;
+    while ((q = hipEventQuery(h->ev_done)) == hipErrorNotReady) {
+    }
+    HIPCHK(q);
+    trep.push_back(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    if (h->oscctl_pin[1] == 0) {
+      finished = true;
+      break;
+    }
+  }
+  h->evals += (long long)n * OSC_K * (long long)trep.size();
+  if (!finished) return fail(ALPSG_E_DEVICE, "secant rounds on the device did not finish");
+  HIPCHK(hipMemcpy(S.data(), h->osc_d.p, sizeof(alpsg_osc::OscMachine) * n, hipMemcpyDeviceToHost));
+  h->root_ms.assign(n, 0.0);
+  for (int r = 0; r < n; ++r) {
+    const alpsg_osc::zc x = S[r].result();
+    om[r] = cplx(x.re, x.im);
+    if (iters) iters[r] = S[r].iter;
+    const int rep = std::min((int)trep.size() - 1, std::max(0, S[r].done_round) / OSC_K);
+    h->root_ms[r] = trep[rep];
+  }
+  if (Dfin) {
+    int rc = run_disp(h, n, reinterpret_cast<const double *>(om), reinterpret_cast<double *>(Dfin),
+                      nullptr, nullptr, nullptr);
+    if (rc) return rc;
+  }
+  return ALPSG_OK;
+}
+
 // secant_osc (ALPS_fns.f90:1919-2101) for many independent roots: each root is
 // a state machine (alpsg_osc.h) whose disp() requests are served in batches.
 int secant_osc_batch(alpsg_ctx *h, const alpsg_root_params *rp, int nroots, cplx *om, cplx *Dfin,
                      int *iters) {
-  std::vector<alpsg_host::OscMachine> S(nroots);
+  {
+    const int rc = osc_graph_batch(h, rp, nroots, om, Dfin, iters);
+    if (rc != -1) return rc;
+  }
+  std::vector<alpsg_osc::OscMachine> S(nroots);
   std::vector<cplx> req(nroots), res(nroots);
   std::vector<int> who;
   const auto t0 = std::chrono::steady_clock::now();
   h->root_ms.assign(nroots, 0.0);
-  for (int r = 0; r < nroots; ++r) S[r].start(om[r], *rp);
+  for (int r = 0; r < nroots; ++r)
+    S[r].start(alpsg_osc::zmk(om[r].real(), om[r].imag()), rp->numiter, rp->D_threshold, rp->D_prec);
   for (;;) {
     who.clear();
     for (int r = 0; r < nroots; ++r) {
       if (S[r].done) continue;
-      req[who.size()] = S[r].want();
+      const alpsg_osc::zc w = S[r].want();
+      req[who.size()] = cplx(w.re, w.im);
       who.push_back(r);
     }
     if (who.empty()) break;
@@ -637,11 +817,12 @@ int secant_osc_batch(alpsg_ctx *h, const alpsg_root_params *rp, int nroots, cplx
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     for (size_t t = 0; t < who.size(); ++t) {
       h->root_ms[who[t]] = now_ms;
-      S[who[t]].feed(res[t]);
+      S[who[t]].feed(alpsg_osc::zmk(res[t].real(), res[t].imag()));
     }
   }
   for (int r = 0; r < nroots; ++r) {
-    om[r] = S[r].result();
+    const alpsg_osc::zc w = S[r].result();
+    om[r] = cplx(w.re, w.im);
     if (iters) iters[r] = S[r].iter;
   }
   if (Dfin) {
@@ -1722,8 +1903,7 @@ int alpsg_map_search(alpsg_handle h, double omi, double omf, double gami, double
   if (!h) return fail(ALPSG_E_ARG, "null handle");
   if (nr < 1 || ni < 1) return fail(ALPSG_E_ARG, "nr, ni must be >= 1");
   const size_t N = (size_t)nr * ni;
-  std::vector<cplx> om(N), cal(N);
-  std::vector<double> val(N);
+  std::vector<cplx> om(N);
   double dr = omf - omi, di = gamf - gami;
   if (nr > 1) dr = (omf - omi) / (1.0 * (nr - 1));
   if (ni > 1) di = (gamf - gami) / (1.0 * (ni - 1));
@@ -1746,60 +1926,65 @@ int alpsg_map_search(alpsg_handle h, double omi, double omf, double gami, double
       om[(size_t)(ir - 1) + (size_t)nr * (ii - 1)] = cplx(wr, wi);
     }
   }
-  int rc = run_disp(h, (int)N, reinterpret_cast<const double *>(om.data()),
-                    reinterpret_cast<double *>(cal.data()), nullptr, nullptr, nullptr);
+  if (!h->have_k) return fail(ALPSG_E_STATE, "alpsg_set_k has not been called");
+  if (N > (size_t)INT32_MAX) return fail(ALPSG_E_ARG, "map too large");
+  // det D of every grid point, then on the device log10|D| with the NaN / Inf
+  // sentinels of ALPS_fns.f90:3723-3742 and the minimum flags of find_minima
+  HIPCHK(h->om_d.ensure(N));
+  HIPCHK(hipMemcpyAsync(h->om_d.p, om.data(), sizeof(cplx) * N, hipMemcpyHostToDevice, h->st));
+  LaunchShape sh = shape_for(h, (int)N);
+  HIPCHK(hipEventRecord(h->ev0, h->st));
+  int rc = disp_device(h, (int)N, h->om_d.p, nullptr, false, false, false, sh);
   if (rc) return rc;
-  for (size_t k = 0; k < N; ++k) {
-    // log10|D| and the NaN / Inf sentinels of ALPS_fns.f90:3723-3742
-    val[k] = std::log10(std::abs(cal[k]));
-    const double tmp = cal[k].real();
-    if (cal[k].imag() != 0.0) {
-      if (!(cplx(tmp, 0.0) != cal[k])) {
-        cal[k] = 999999.0;
-        val[k] = 999999.0;
-      }
-    } else if (cplx(tmp, 0.0) != cal[k]) {
-      cal[k] = 999999.0;
-      val[k] = 999999.0;
-    }
-    if (std::fabs(tmp) > 1.e100) {
-      cal[k] = 899999.0;
-      val[k] = 899999.0;
-    }
+  HIPCHK(hipEventRecord(h->ev1, h->st));
+  HIPCHK(h->mcal_d.ensure(N));
+  HIPCHK(h->mval_d.ensure(N));
+  HIPCHK(h->mhit_d.ensure(N));
+  HIPCHK(launch_map_post(h->st, nr, ni, h->det_d.p, h->mcal_d.p, h->mval_d.p, h->mhit_d.p));
+  h->mhit_h.resize(N);
+  HIPCHK(hipMemcpyAsync(h->mhit_h.data(), h->mhit_d.p, N, hipMemcpyDeviceToHost, h->st));
+  if (cal_out)
+    HIPCHK(hipMemcpyAsync(cal_out, h->mcal_d.p, sizeof(cplx) * N, hipMemcpyDeviceToHost, h->st));
+  if (val_out)
+    HIPCHK(hipMemcpyAsync(val_out, h->mval_d.p, sizeof(double) * N, hipMemcpyDeviceToHost, h->st));
+  int relerr = 0;
+  if (h->any_rel)
+    HIPCHK(hipMemcpyAsync(&relerr, h->err_d.p, sizeof(int), hipMemcpyDeviceToHost, h->st));
+  if (om_out) std::memcpy(om_out, om.data(), sizeof(cplx) * N);
+  HIPCHK(hipStreamSynchronize(h->st));
+  if (relerr) {
+    const int zero = 0;
+    HIPCHK(hipMemcpy(h->err_d.p, &zero, sizeof(int), hipMemcpyHostToDevice));
+    return fail(ALPSG_E_ARG,
+                "a resonance's principal-value window exceeds the subluminal part of a Gamma row "
+                "(the reference stops with alps_error(8)); refine npparbar or lower "
+                "positions_principal");
   }
+  float ms = 0.f, a = 0.f, b = 0.f, c = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  HIPCHK(hipEventElapsedTime(&a, h->ev0, h->ev_sweep));
+  HIPCHK(hipEventElapsedTime(&b, h->ev_sweep, h->ev_res));
+  HIPCHK(hipEventElapsedTime(&c, h->ev_res, h->ev1));
+  h->last_ms = ms;
+  h->last_split[0] = a;
+  h->last_split[1] = b;
+  h->last_split[2] = c;
+  h->evals += (long long)N;
+  // the minima in find_minima's order: ii from ni down to 1, ir from 1 to nr
   int n = 0;
-  if (nr > 1 && ni > 1) {
-    auto V = [&](int ir, int ii) { return val[(size_t)(ir - 1) + (size_t)nr * (ii - 1)]; };
-    for (int ii = ni; ii >= 1; --ii)
-      for (int ir = 1; ir <= nr; ++ir) {
-        bool rok, hit = false;
-        if (ir == 1)
-          rok = V(ir, ii) < V(ir + 1, ii);
-        else if (ir == nr)
-          rok = V(ir, ii) < V(ir - 1, ii);
-        else
-          rok = V(ir, ii) < V(ir - 1, ii) && V(ir, ii) < V(ir + 1, ii);
-        if (rok) {
-          if (ii == 1)
-            hit = V(ir, ii) < V(ir, ii + 1);
-          else if (ii == ni)
-            hit = V(ir, ii) < V(ir, ii - 1);
-          else
-            hit = V(ir, ii) < V(ir, ii - 1) && V(ir, ii) < V(ir, ii + 1);
-        }
-        if (hit) {
-          if (iroots && n < numroots) {
-            iroots[2 * n] = ir;
-            iroots[2 * n + 1] = ii;
-          }
-          ++n;
-        }
+  const unsigned char *hit = h->mhit_h.data();
+  for (int ii = ni; ii >= 1; --ii) {
+    const unsigned char *row = hit + (size_t)nr * (ii - 1);
+    for (int ir = 1; ir <= nr; ++ir) {
+      if (!row[ir - 1]) continue;
+      if (iroots && n < numroots) {
+        iroots[2 * n] = ir;
+        iroots[2 * n + 1] = ii;
       }
+      ++n;
+    }
   }
   if (nfound) *nfound = n;
-  if (om_out) std::memcpy(om_out, om.data(), sizeof(cplx) * N);
-  if (cal_out) std::memcpy(cal_out, cal.data(), sizeof(cplx) * N);
-  if (val_out) std::memcpy(val_out, val.data(), sizeof(double) * N);
   return ALPSG_OK;
 }
 

@@ -18,8 +18,11 @@
 // positions) is evaluated without FMA contraction in the reference's
 // operation order, so the device takes the same branches as the CPU
 // restatement.
+#include <algorithm>
+
 #include "alpsg_device.h"
 #include "alpsg_fit.h"
+#include "alpsg_osc.h"
 
 namespace {
 
@@ -1334,5 +1337,130 @@ hipError_t launch_assemble(hipStream_t st, const ProbDev *prob, int nspec, int n
   hipLaunchKernelGGL(k_assemble, dim3((nom + opb - 1) / opb), dim3(256), 0, st, prob, nom, W, RB,
                      om, active, chiA, chiB, lowA, lowB, chiR, RR, lowR, det, chi, low, wave, nspec,
                      pidx);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// map_search post-processing (ALPS_fns.f90:3723-3742, find_minima :3860-3966)
+// on the device: log10|D| with the reference's NaN / Inf sentinels, and the
+// strict four-neighbour minima of log10|D| (one-sided at the edges).  Point
+// (ir, ii), 1-based, is element (ir-1) + nr (ii-1).
+// ---------------------------------------------------------------------------
+namespace {
+
+__device__ __forceinline__ void map_value(double2 &c, double &v) {
+  v = log10(hypot(c.x, c.y));
+  const double tmp = c.x;
+  const bool same = (tmp == c.x) && (0.0 == c.y);  // cmplx(tmp, 0) == cal
+  bool sentinel;
+  if (c.y != 0.0)
+    sentinel = same;
+  else
+    sentinel = !same;  // a NaN real part
+  if (sentinel) {
+    c = make_double2(999999.0, 0.0);
+    v = 999999.0;
+  }
+  if (fabs(tmp) > 1.e100) {
+    c = make_double2(899999.0, 0.0);
+    v = 899999.0;
+  }
+}
+
+__device__ __forceinline__ double map_val_at(const double2 *__restrict__ cal, size_t k) {
+  double2 c = cal[k];
+  double v;
+  map_value(c, v);
+  return v;
+}
+
+__global__ __launch_bounds__(256) void k_map_post(int nr, int ni, const double2 *__restrict__ cal,
+                                                  double2 *__restrict__ calo, double *__restrict__ val,
+                                                  unsigned char *__restrict__ hit) {
+  const size_t N = (size_t)nr * ni;
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= N) return;
+  double2 c = cal[k];
+  double v;
+  map_value(c, v);
+  calo[k] = c;
+  val[k] = v;
+  const int ir = (int)(k % nr) + 1, ii = (int)(k / nr) + 1;
+  bool h = false;
+  if (nr > 1 && ni > 1) {
+    bool rok;
+    if (ir == 1)
+      rok = v < map_val_at(cal, k + 1);
+    else if (ir == nr)
+      rok = v < map_val_at(cal, k - 1);
+    else
+      rok = v < map_val_at(cal, k - 1) && v < map_val_at(cal, k + 1);
+    if (rok) {
+      if (ii == 1)
+        h = v < map_val_at(cal, k + nr);
+      else if (ii == ni)
+        h = v < map_val_at(cal, k - nr);
+      else
+        h = v < map_val_at(cal, k - nr) && v < map_val_at(cal, k + nr);
+    }
+  }
+  hit[k] = h ? 1 : 0;
+}
+
+}  // namespace
+
+hipError_t launch_map_post(hipStream_t st, int nr, int ni, const double2 *cal, double2 *calo,
+                           double *val, unsigned char *hit) {
+  const size_t N = (size_t)nr * ni;
+  if (N == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_map_post, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, nr, ni, cal,
+                     calo, val, hit);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// One round of many secant_osc solves on the device: every unfinished solve
+// takes det D at the frequency it asked for, steps its state machine
+// (alpsg_osc.h, the same code the host runs) and writes the next frequency it
+// needs.  One workgroup; round_ctr counts the rounds (read by all threads,
+// advanced by thread 0 after a barrier), remaining gets the number of threads
+// that still hold an unfinished solve.
+// ---------------------------------------------------------------------------
+namespace {
+
+__global__ __launch_bounds__(1024) void k_osc_round(int n, alpsg_osc::OscMachine *__restrict__ S,
+                                                    const double2 *__restrict__ det,
+                                                    double2 *__restrict__ om, int *__restrict__ round_ctr,
+                                                    int *__restrict__ remaining) {
+  const int round = *round_ctr;
+  int left = 0;
+  for (int r = threadIdx.x; r < n; r += blockDim.x) {
+    alpsg_osc::OscMachine m = S[r];
+    if (!m.done) {
+      m.feed(alpsg_osc::zmk(det[r].x, det[r].y));
+      if (m.done) {
+        m.done_round = round;
+      } else {
+        const alpsg_osc::zc w = m.want();
+        om[r] = make_double2(w.re, w.im);
+        left = 1;
+      }
+      S[r] = m;
+    }
+  }
+  const int tot = __syncthreads_count(left);
+  if (threadIdx.x == 0) {
+    *remaining = tot;
+    *round_ctr = round + 1;
+  }
+}
+
+}  // namespace
+
+hipError_t launch_osc_round(hipStream_t st, int n, alpsg_osc::OscMachine *S, const double2 *det,
+                            double2 *om, int *round_ctr, int *remaining) {
+  if (n <= 0) return hipSuccess;
+  const int threads = std::min(1024, (n + 63) / 64 * 64);
+  hipLaunchKernelGGL(k_osc_round, dim3(1), dim3(threads), 0, st, n, S, det, om, round_ctr, remaining);
   return hipGetLastError();
 }

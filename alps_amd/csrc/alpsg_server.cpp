@@ -31,6 +31,7 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <complex>
 #include <condition_variable>
 #include <cstddef>
 #include <cstring>
@@ -78,8 +79,8 @@ hipError_t launch_assemble(hipStream_t st, const ProbDev *prob, int nspec, int n
 
 namespace {
 
-using alpsg_host::cplx;
-using alpsg_host::OscMachine;
+typedef std::complex<double> cplx;
+using alpsg_osc::OscMachine;
 
 #define SCHK(expr)                                                                           \
   do {                                                                                       \
@@ -623,7 +624,7 @@ int admit_k(alpsg_server *S, Request &r, int &njobs, int &max_nperp, int &nnew) 
     }
   }
   S->newslot_h[nnew++] = r.slot;
-  r.m.start(r.guess, r.rp);
+  r.m.start(alpsg_osc::zmk(r.guess.real(), r.guess.imag()), r.rp.numiter, r.rp.D_threshold, r.rp.D_prec);
   r.state = 2;
   return 0;
 }
@@ -724,7 +725,8 @@ int server_pass(alpsg_server *S) {
   const double t0 = now_ms();
   S->t_admit += t0 - tp0;
   for (int i = 0; i < nb; ++i) {
-    const cplx w = act[i]->m.want();
+    const alpsg_osc::zc wz = act[i]->m.want();
+    const cplx w(wz.re, wz.im);
     S->om_h[i] = make_double2(w.real(), w.imag());
     S->pidx_h[i] = act[i]->slot;
   }
@@ -811,11 +813,12 @@ int server_pass(alpsg_server *S) {
   }
   S->t_admit += now_ms() - t3;
   // feed the results; retire finished solves
-  for (int i = 0; i < nb; ++i) act[i]->m.feed(cplx(S->det_h[i].x, S->det_h[i].y));
+  for (int i = 0; i < nb; ++i) act[i]->m.feed(alpsg_osc::zmk(S->det_h[i].x, S->det_h[i].y));
   for (auto it = S->live.begin(); it != S->live.end();) {
     Request &r = **it;
     if (r.state == 2 && r.m.done) {
-      done.push_back(Result{r.tag, r.m.result(), r.m.iter, ALPSG_OK, now_ms() - r.t_submit});
+      done.push_back(Result{r.tag, cplx(r.m.result().re, r.m.result().im), r.m.iter, ALPSG_OK,
+                            now_ms() - r.t_submit});
       release(S, r);
       it = S->live.erase(it);
     } else {
