@@ -55,6 +55,11 @@ hipError_t launch_map_post(hipStream_t st, int nr, int ni, const double2 *cal, d
                            double *val, unsigned char *hit);
 hipError_t launch_osc_round(hipStream_t st, int n, alpsg_osc::OscMachine *S, const double2 *det,
                             double2 *om, int *round_ctr, int *remaining);
+hipError_t launch_resonant_all(hipStream_t st, int nch, int RB, const ProbDev *prob, int nspec,
+                               int nom, const double2 *om, double2 *chiB, bool general);
+hipError_t launch_assemble_osc(hipStream_t st, const ProbDev *prob, int nspec, int nom, int W, int RB,
+                               double2 *om, const double2 *chiA, const double2 *chiB, double2 *det,
+                               alpsg_osc::OscMachine *S, int *round_ctr, int *remaining);
 hipError_t launch_rel(hipStream_t st, const ProbDev *prob, int nspec, int nmaxR, int NB, int nom,
                       const double2 *om, const int *active, double2 *chiR, double2 *lowR,
                       int *flags, int *err);
@@ -360,8 +365,12 @@ int disp_device(alpsg_ctx *h, int n, const double2 *om_dev, const int *active_de
   }
   const int TWa = req ? REQ_TW : TW;
   HIPCHK(hipEventRecord(h->ev_sweep, h->st));
-  HIPCHK(launch_resonant(h->st, sh.nch_b, sh.rb, h->prob_d.p, ns, n, om_dev, active_dev,
-                         h->chiB_d.p, want_low ? h->lowB_d.p : nullptr, h->any_general_fit));
+  if (req)
+    HIPCHK(launch_resonant_all(h->st, sh.nch_b, sh.rb, h->prob_d.p, ns, n, om_dev, h->chiB_d.p,
+                               h->any_general_fit));
+  else
+    HIPCHK(launch_resonant(h->st, sh.nch_b, sh.rb, h->prob_d.p, ns, n, om_dev, active_dev,
+                           h->chiB_d.p, want_low ? h->lowB_d.p : nullptr, h->any_general_fit));
   if (h->any_bm)
     HIPCHK(launch_nhds(h->st, h->prob_d.p, ns, n, om_dev, active_dev, h->chiA_d.p,
                        want_low ? h->lowA_d.p : nullptr));
@@ -535,8 +544,8 @@ int graph_disp(alpsg_ctx *h, int n, const double *om, double *det) {
       e = launch_sweep_req(h->st, REQ_NCH, REQ_W, REQ_TW / REQ_W, REQ_K, h->prob_d.p, n, h->om_d.p,
                            h->chiA_d.p, h->zidx.p);
     if (e == hipSuccess)
-      e = launch_resonant(h->st, sh.nch_b, sh.rb, h->prob_d.p, ns, n, h->om_d.p, nullptr,
-                          h->chiB_d.p, nullptr, h->any_general_fit);
+      e = launch_resonant_all(h->st, sh.nch_b, sh.rb, h->prob_d.p, ns, n, h->om_d.p, h->chiB_d.p,
+                              h->any_general_fit);
     if (e == hipSuccess)
       e = launch_assemble(h->st, h->prob_d.p, ns, n, REQ_TW, sh.rb, h->om_d.p, nullptr, h->chiA_d.p,
                           h->chiB_d.p, nullptr, nullptr, nullptr, 0, nullptr, h->det_d.p, nullptr,
@@ -719,6 +728,18 @@ int osc_graph_batch(alpsg_ctx *h, const alpsg_root_params *rp, int n, cplx *om, 
       else
         e = launch_sweep(h->st, sh.nch_a, sh.w_a, sh.rc_a, h->prob_d.p, n, h->om_d.p, nullptr,
                          h->chiA_d.p, nullptr);
+      if (n <= 256) {
+        // two launches per round: both resonant parts, then the assembly and
+        // the secant step in one workgroup
+        if (e == hipSuccess)
+          e = launch_resonant_all(h->st, sh.nch_b, sh.rb, h->prob_d.p, ns, n, h->om_d.p,
+                                  h->chiB_d.p, h->any_general_fit);
+        if (e == hipSuccess)
+          e = launch_assemble_osc(h->st, h->prob_d.p, ns, n, TW, sh.rb, h->om_d.p, h->chiA_d.p,
+                                  h->chiB_d.p, h->det_d.p, h->osc_d.p, h->oscctl_d.p,
+                                  h->oscctl_d.p + 1);
+        continue;
+      }
       if (e == hipSuccess)
         e = launch_resonant(h->st, sh.nch_b, sh.rb, h->prob_d.p, ns, n, h->om_d.p, nullptr,
                             h->chiB_d.p, nullptr, h->any_general_fit);

@@ -1000,12 +1000,12 @@ __device__ void landau_part(const SpecDev &S, const ProbDev &Pd, const double *_
 // terms of the other species.  PV writes partial slots rb, the Landau launches
 // slots RB + rb of chiB / lowB (each species by exactly one of PART 1, 2).
 template <int NCH, int PART>
-__global__ __launch_bounds__(64) void k_resonant(const ProbDev *__restrict__ probp, int nom,
-                                                 const double2 *__restrict__ om_in,
-                                                 const int *__restrict__ active,
-                                                 double2 *__restrict__ chiB,
-                                                 double2 *__restrict__ lowB,
-                                                 const int *__restrict__ pidx) {
+__device__ __forceinline__ void resonant_body(const ProbDev *__restrict__ probp, int nom,
+                                              const double2 *__restrict__ om_in,
+                                              const int *__restrict__ active,
+                                              double2 *__restrict__ chiB,
+                                              double2 *__restrict__ lowB,
+                                              const int *__restrict__ pidx, int RB, int rb) {
   constexpr int OMB = 64 / NCH;
   __shared__ double lds_win[OMB][2 * PV_WIN];
   const ProbDev &Pd = (NCH == 64 && pidx != nullptr) ? probp[pidx[blockIdx.x]] : *probp;
@@ -1013,7 +1013,6 @@ __global__ __launch_bounds__(64) void k_resonant(const ProbDev *__restrict__ pro
   const int g = lane / NCH, c = lane % NCH;
   const int iom = blockIdx.x * OMB + g;
   const int s = blockIdx.y;
-  const int RB = gridDim.z, rb = blockIdx.z;
   const int slot = (PART == 0 ? 0 : RB) + rb;
   const bool valid = (iom < nom) && (active == nullptr || active[iom] != 0);
   const double2 ov = valid ? om_in[iom] : make_double2(1.0, 0.5);
@@ -1118,26 +1117,53 @@ __global__ __launch_bounds__(64) void k_resonant(const ProbDev *__restrict__ pro
   }
 }
 
+template <int NCH, int PART>
+__global__ __launch_bounds__(64) void k_resonant(const ProbDev *__restrict__ probp, int nom,
+                                                 const double2 *__restrict__ om_in,
+                                                 const int *__restrict__ active,
+                                                 double2 *__restrict__ chiB,
+                                                 double2 *__restrict__ lowB,
+                                                 const int *__restrict__ pidx) {
+  resonant_body<NCH, PART>(probp, nom, om_in, active, chiB, lowB, pidx, gridDim.z, blockIdx.z);
+}
+
+// The principal-value and Landau parts in one launch (small batches, where a
+// launch costs more than the work): z < RB runs the principal-value part of
+// harmonic block z, z >= RB the Landau part of block z - RB (the part-1 or
+// part-2 form, whichever the species takes).
+template <int NCH, bool GEN>
+__global__ __launch_bounds__(64) void k_resonant_all(const ProbDev *__restrict__ probp, int nom,
+                                                     const double2 *__restrict__ om_in,
+                                                     double2 *__restrict__ chiB, int RB) {
+  const int z = blockIdx.z;
+  if (z < RB) {
+    resonant_body<NCH, 0>(probp, nom, om_in, nullptr, chiB, nullptr, nullptr, RB, z);
+    return;
+  }
+  resonant_body<NCH, 1>(probp, nom, om_in, nullptr, chiB, nullptr, nullptr, RB, z - RB);
+  if constexpr (GEN) resonant_body<NCH, 2>(probp, nom, om_in, nullptr, chiB, nullptr, nullptr, RB, z - RB);
+}
+
 // ---------------------------------------------------------------------------
 // k_assemble: a workgroup of 256 threads handles 256/(6 nspec) omegas; thread
 // (omega, s, k) sums the partials of chi_sk, then one thread per omega forms
 // the dielectric tensor, the wave tensor and det D.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_assemble(
-    const ProbDev *__restrict__ probp, int nom, int W, int RB, const double2 *__restrict__ om_in,
-    const int *__restrict__ active, const double2 *__restrict__ chiA,
-    const double2 *__restrict__ chiB, const double2 *__restrict__ lowA,
-    const double2 *__restrict__ lowB, const double2 *__restrict__ chiR, int RR,
-    const double2 *__restrict__ lowR, double2 *__restrict__ det_out, double2 *__restrict__ chi_out,
-    double2 *__restrict__ low_out, double2 *__restrict__ wave_out, int nspec,
-    const int *__restrict__ pidx) {
+__device__ __forceinline__ void assemble_block(
+    int blk, const ProbDev *__restrict__ probp, int nom, int W, int RB,
+    const double2 *__restrict__ om_in, const int *__restrict__ active,
+    const double2 *__restrict__ chiA, const double2 *__restrict__ chiB,
+    const double2 *__restrict__ lowA, const double2 *__restrict__ lowB,
+    const double2 *__restrict__ chiR, int RR, const double2 *__restrict__ lowR,
+    double2 *__restrict__ det_out, double2 *__restrict__ chi_out, double2 *__restrict__ low_out,
+    double2 *__restrict__ wave_out, int nspec, const int *__restrict__ pidx) {
   __shared__ double2 lchi[256];
   const int per = 6 * nspec;
   const int opb = 256 / per;  // omegas per block
   const int t = threadIdx.x;
   const int lo_om = t / per, sk = t % per;
   const int s = sk / 6, k = sk % 6;
-  const int iom = blockIdx.x * opb + lo_om;
+  const int iom = blk * opb + lo_om;
   const bool in = (lo_om < opb) && (iom < nom) && (active == nullptr || active[iom] != 0);
   const ProbDev &Pd = pidx != nullptr ? probp[pidx[min(iom, nom - 1)]] : *probp;
   if (in) {
@@ -1263,6 +1289,18 @@ __global__ __launch_bounds__(256) void k_assemble(
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j) o[i + 3 * j] = make_double2(Wt[i][j].re, Wt[i][j].im);
   }
+}
+
+__global__ __launch_bounds__(256) void k_assemble(
+    const ProbDev *__restrict__ probp, int nom, int W, int RB, const double2 *__restrict__ om_in,
+    const int *__restrict__ active, const double2 *__restrict__ chiA,
+    const double2 *__restrict__ chiB, const double2 *__restrict__ lowA,
+    const double2 *__restrict__ lowB, const double2 *__restrict__ chiR, int RR,
+    const double2 *__restrict__ lowR, double2 *__restrict__ det_out, double2 *__restrict__ chi_out,
+    double2 *__restrict__ low_out, double2 *__restrict__ wave_out, int nspec,
+    const int *__restrict__ pidx) {
+  assemble_block(blockIdx.x, probp, nom, W, RB, om_in, active, chiA, chiB, lowA, lowB, chiR, RR,
+                 lowR, det_out, chi_out, low_out, wave_out, nspec, pidx);
 }
 
 // ---------------------------------------------------------------------------
@@ -1428,10 +1466,9 @@ hipError_t launch_map_post(hipStream_t st, int nr, int ni, const double2 *cal, d
 // ---------------------------------------------------------------------------
 namespace {
 
-__global__ __launch_bounds__(1024) void k_osc_round(int n, alpsg_osc::OscMachine *__restrict__ S,
-                                                    const double2 *__restrict__ det,
-                                                    double2 *__restrict__ om, int *__restrict__ round_ctr,
-                                                    int *__restrict__ remaining) {
+__device__ __forceinline__ void osc_step(int n, alpsg_osc::OscMachine *__restrict__ S,
+                                         const double2 *__restrict__ det, double2 *__restrict__ om,
+                                         int *__restrict__ round_ctr, int *__restrict__ remaining) {
   const int round = *round_ctr;
   int left = 0;
   for (int r = threadIdx.x; r < n; r += blockDim.x) {
@@ -1455,7 +1492,64 @@ __global__ __launch_bounds__(1024) void k_osc_round(int n, alpsg_osc::OscMachine
   }
 }
 
+__global__ __launch_bounds__(1024) void k_osc_round(int n, alpsg_osc::OscMachine *__restrict__ S,
+                                                    const double2 *__restrict__ det,
+                                                    double2 *__restrict__ om, int *__restrict__ round_ctr,
+                                                    int *__restrict__ remaining) {
+  osc_step(n, S, det, om, round_ctr, remaining);
+}
+
+// k_assemble for a small batch in one workgroup (its omegas in chunks of
+// 256 / (6 nspec)), followed by the secant step of every solve.
+__global__ __launch_bounds__(256) void k_assemble_osc(
+    const ProbDev *__restrict__ probp, int nom, int W, int RB, const double2 *__restrict__ om_in,
+    const double2 *__restrict__ chiA, const double2 *__restrict__ chiB, double2 *__restrict__ det,
+    int nspec, alpsg_osc::OscMachine *__restrict__ S, double2 *__restrict__ om_next,
+    int *__restrict__ round_ctr, int *__restrict__ remaining) {
+  const int opb = 256 / (6 * nspec);
+  for (int blk = 0; blk * opb < nom; ++blk) {
+    assemble_block(blk, probp, nom, W, RB, om_in, nullptr, chiA, chiB, nullptr, nullptr, nullptr, 0,
+                   nullptr, det, nullptr, nullptr, nullptr, nspec, nullptr);
+    __syncthreads();
+  }
+  osc_step(nom, S, det, om_next, round_ctr, remaining);
+}
+
 }  // namespace
+
+// both resonant parts in one launch (small batches without active mask or
+// low-order outputs): nch lanes per omega, RB harmonic blocks
+hipError_t launch_resonant_all(hipStream_t st, int nch, int RB, const ProbDev *prob, int nspec,
+                               int nom, const double2 *om, double2 *chiB, bool general) {
+  if (nom <= 0) return hipSuccess;
+#define RA(N)                                                                                  \
+  if (nch == N) {                                                                              \
+    const int grid = (nom + (64 / N) - 1) / (64 / N);                                          \
+    if (general)                                                                               \
+      hipLaunchKernelGGL((k_resonant_all<N, true>), dim3(grid, nspec, 2 * RB), dim3(64), 0, st,  \
+                         prob, nom, om, chiB, RB);                                             \
+    else                                                                                       \
+      hipLaunchKernelGGL((k_resonant_all<N, false>), dim3(grid, nspec, 2 * RB), dim3(64), 0, st, \
+                         prob, nom, om, chiB, RB);                                             \
+    return hipGetLastError();                                                                  \
+  }
+  RA(4) RA(8) RA(16) RA(32) RA(64)
+#undef RA
+  return hipErrorInvalidValue;
+}
+
+// the assembly of at most 256 omegas in one workgroup, then one secant step
+// of each of their solves (om is read as this round's frequencies and
+// overwritten with the next ones)
+hipError_t launch_assemble_osc(hipStream_t st, const ProbDev *prob, int nspec, int nom, int W, int RB,
+                               double2 *om, const double2 *chiA, const double2 *chiB, double2 *det,
+                               alpsg_osc::OscMachine *S, int *round_ctr, int *remaining) {
+  if (nom <= 0) return hipSuccess;
+  if (nom > 256 || 6 * nspec > 256) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_assemble_osc, dim3(1), dim3(256), 0, st, prob, nom, W, RB, om, chiA, chiB, det,
+                     nspec, S, om, round_ctr, remaining);
+  return hipGetLastError();
+}
 
 hipError_t launch_osc_round(hipStream_t st, int n, alpsg_osc::OscMachine *S, const double2 *det,
                             double2 *om, int *round_ctr, int *remaining) {
