@@ -125,6 +125,24 @@ struct PinBuf {
   }
 };
 
+// pinned host bytes, grown on demand (the map's grid and outputs)
+struct PinBytes {
+  unsigned char *p = nullptr;
+  size_t n = 0;
+  hipError_t ensure(size_t cnt) {
+    if (cnt <= n && p) return hipSuccess;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+    hipError_t e = hipHostMalloc((void **)&p, std::max<size_t>(cnt, 1), hipHostMallocDefault);
+    if (e == hipSuccess) n = std::max<size_t>(cnt, 1);
+    return e;
+  }
+  ~PinBytes() {
+    if (p) (void)hipHostFree(p);
+  }
+};
+
 }  // namespace
 
 // the error text of the calling thread, for the other translation units
@@ -175,7 +193,7 @@ struct alpsg_ctx {
   DevBuf<double2> mcal_d;
   DevBuf<double> mval_d;
   DevBuf<unsigned char> mhit_d;
-  std::vector<unsigned char> mhit_h;
+  PinBytes map_pin;  // the grid, then D, log10|D| and the minimum flags
   // secant_osc rounds on the device: the solves' state machines, the round
   // counter and unfinished count ([0], [1]) and its pinned copy, and graphs of
   // OSC_K rounds keyed by batch size, launch shape and buffers
@@ -1924,7 +1942,14 @@ int alpsg_map_search(alpsg_handle h, double omi, double omf, double gami, double
   if (!h) return fail(ALPSG_E_ARG, "null handle");
   if (nr < 1 || ni < 1) return fail(ALPSG_E_ARG, "nr, ni must be >= 1");
   const size_t N = (size_t)nr * ni;
-  std::vector<cplx> om(N);
+  if (!h->have_k) return fail(ALPSG_E_STATE, "alpsg_set_k has not been called");
+  if (N > (size_t)INT32_MAX) return fail(ALPSG_E_ARG, "map too large");
+  // pinned staging: grid [N] complex, D [N] complex, log10|D| [N], flags [N]
+  HIPCHK(h->map_pin.ensure(N * (sizeof(cplx) * 2 + sizeof(double) + 1)));
+  cplx *om = reinterpret_cast<cplx *>(h->map_pin.p);
+  cplx *cal_h = om + N;
+  double *val_h = reinterpret_cast<double *>(cal_h + N);
+  unsigned char *hit = reinterpret_cast<unsigned char *>(val_h + N);
   double dr = omf - omi, di = gamf - gami;
   if (nr > 1) dr = (omf - omi) / (1.0 * (nr - 1));
   if (ni > 1) di = (gamf - gami) / (1.0 * (ni - 1));
@@ -1947,12 +1972,10 @@ int alpsg_map_search(alpsg_handle h, double omi, double omf, double gami, double
       om[(size_t)(ir - 1) + (size_t)nr * (ii - 1)] = cplx(wr, wi);
     }
   }
-  if (!h->have_k) return fail(ALPSG_E_STATE, "alpsg_set_k has not been called");
-  if (N > (size_t)INT32_MAX) return fail(ALPSG_E_ARG, "map too large");
   // det D of every grid point, then on the device log10|D| with the NaN / Inf
   // sentinels of ALPS_fns.f90:3723-3742 and the minimum flags of find_minima
   HIPCHK(h->om_d.ensure(N));
-  HIPCHK(hipMemcpyAsync(h->om_d.p, om.data(), sizeof(cplx) * N, hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(h->om_d.p, om, sizeof(cplx) * N, hipMemcpyHostToDevice, h->st));
   LaunchShape sh = shape_for(h, (int)N);
   HIPCHK(hipEventRecord(h->ev0, h->st));
   int rc = disp_device(h, (int)N, h->om_d.p, nullptr, false, false, false, sh);
@@ -1962,17 +1985,18 @@ int alpsg_map_search(alpsg_handle h, double omi, double omf, double gami, double
   HIPCHK(h->mval_d.ensure(N));
   HIPCHK(h->mhit_d.ensure(N));
   HIPCHK(launch_map_post(h->st, nr, ni, h->det_d.p, h->mcal_d.p, h->mval_d.p, h->mhit_d.p));
-  h->mhit_h.resize(N);
-  HIPCHK(hipMemcpyAsync(h->mhit_h.data(), h->mhit_d.p, N, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipMemcpyAsync(hit, h->mhit_d.p, N, hipMemcpyDeviceToHost, h->st));
   if (cal_out)
-    HIPCHK(hipMemcpyAsync(cal_out, h->mcal_d.p, sizeof(cplx) * N, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipMemcpyAsync(cal_h, h->mcal_d.p, sizeof(cplx) * N, hipMemcpyDeviceToHost, h->st));
   if (val_out)
-    HIPCHK(hipMemcpyAsync(val_out, h->mval_d.p, sizeof(double) * N, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipMemcpyAsync(val_h, h->mval_d.p, sizeof(double) * N, hipMemcpyDeviceToHost, h->st));
   int relerr = 0;
   if (h->any_rel)
     HIPCHK(hipMemcpyAsync(&relerr, h->err_d.p, sizeof(int), hipMemcpyDeviceToHost, h->st));
-  if (om_out) std::memcpy(om_out, om.data(), sizeof(cplx) * N);
+  if (om_out) std::memcpy(om_out, om, sizeof(cplx) * N);
   HIPCHK(hipStreamSynchronize(h->st));
+  if (cal_out) std::memcpy(cal_out, cal_h, sizeof(cplx) * N);
+  if (val_out) std::memcpy(val_out, val_h, sizeof(double) * N);
   if (relerr) {
     const int zero = 0;
     HIPCHK(hipMemcpy(h->err_d.p, &zero, sizeof(int), hipMemcpyHostToDevice));
@@ -1993,7 +2017,6 @@ int alpsg_map_search(alpsg_handle h, double omi, double omf, double gami, double
   h->evals += (long long)N;
   // the minima in find_minima's order: ii from ni down to 1, ir from 1 to nr
   int n = 0;
-  const unsigned char *hit = h->mhit_h.data();
   for (int ii = ni; ii >= 1; --ii) {
     const unsigned char *row = hit + (size_t)nr * (ii - 1);
     for (int ir = 1; ir <= nr; ++ir) {
