@@ -593,18 +593,19 @@ __global__ __launch_bounds__(64 * W) void k_sweep(const ProbDev *__restrict__ pr
 // wavefront writes its partial chiA slot.  Every omega's result depends only on
 // its own problem and the launch shape.
 // ---------------------------------------------------------------------------
+// One workgroup of k_sweep_req: omega bx, workgroup by of the omega's gy.
 template <int NCH, int W, int K>
-__global__ __launch_bounds__(64 * W) void k_sweep_req(const ProbDev *__restrict__ probp, int nom,
-                                                      const double2 *__restrict__ om_in,
-                                                      double2 *__restrict__ chiA,
-                                                      const int *__restrict__ pidx) {
+__device__ __forceinline__ void sweep_req_body(const ProbDev *__restrict__ probp, int nom,
+                                               const double2 *__restrict__ om_in,
+                                               double2 *__restrict__ chiA,
+                                               const int *__restrict__ pidx, int bx, int by, int gy) {
   constexpr int RPW = 64 / NCH;
-  const int iom = blockIdx.x;
+  const int iom = bx;
   if (iom >= nom) return;
   const ProbDev &Pd = probp[pidx[iom]];
   const int lane = threadIdx.x & 63;
-  const int TW = W * gridDim.y;
-  const int wv = blockIdx.y * W + (threadIdx.x >> 6);
+  const int TW = W * gy;
+  const int wv = by * W + (threadIdx.x >> 6);
   const int g = lane / NCH, c = lane % NCH;
   const int slot = wv * RPW + g, TS = TW * RPW;
   const double2 ov = om_in[iom];
@@ -712,6 +713,14 @@ __global__ __launch_bounds__(64 * W) void k_sweep_req(const ProbDev *__restrict_
       for (int k = 0; k < 6; ++k) o[(size_t)k * nom] = make_double2(run[2 * k], run[2 * k + 1]);
     }
   }
+}
+
+template <int NCH, int W, int K>
+__global__ __launch_bounds__(64 * W) void k_sweep_req(const ProbDev *__restrict__ probp, int nom,
+                                                      const double2 *__restrict__ om_in,
+                                                      double2 *__restrict__ chiA,
+                                                      const int *__restrict__ pidx) {
+  sweep_req_body<NCH, W, K>(probp, nom, om_in, chiA, pidx, blockIdx.x, blockIdx.y, gridDim.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -999,20 +1008,21 @@ __device__ void landau_part(const SpecDev &S, const ProbDev &Pd, const double *_
 // continuation is one Maxwellian with pperp-independent (u2, u3); PART 2: Landau
 // terms of the other species.  PV writes partial slots rb, the Landau launches
 // slots RB + rb of chiB / lowB (each species by exactly one of PART 1, 2).
+// One wavefront's share of k_resonant: omega block bx, species by, harmonic
+// block rb of RB; win_lds holds (64 / NCH) windows of 2 PV_WIN doubles.
 template <int NCH, int PART>
 __device__ __forceinline__ void resonant_body(const ProbDev *__restrict__ probp, int nom,
                                               const double2 *__restrict__ om_in,
                                               const int *__restrict__ active,
                                               double2 *__restrict__ chiB,
                                               double2 *__restrict__ lowB,
-                                              const int *__restrict__ pidx, int RB, int rb) {
+                                              const int *__restrict__ pidx, int RB, int rb, int bx,
+                                              int by, int lane, double *win_lds) {
   constexpr int OMB = 64 / NCH;
-  __shared__ double lds_win[OMB][2 * PV_WIN];
-  const ProbDev &Pd = (NCH == 64 && pidx != nullptr) ? probp[pidx[blockIdx.x]] : *probp;
-  const int lane = threadIdx.x;
+  const ProbDev &Pd = (NCH == 64 && pidx != nullptr) ? probp[pidx[bx]] : *probp;
   const int g = lane / NCH, c = lane % NCH;
-  const int iom = blockIdx.x * OMB + g;
-  const int s = blockIdx.y;
+  const int iom = bx * OMB + g;
+  const int s = by;
   const int slot = (PART == 0 ? 0 : RB) + rb;
   const bool valid = (iom < nom) && (active == nullptr || active[iom] != 0);
   const double2 ov = valid ? om_in[iom] : make_double2(1.0, 0.5);
@@ -1023,7 +1033,7 @@ __device__ __forceinline__ void resonant_body(const ProbDev *__restrict__ probp,
   const double kpar = Pd.kpar;
   const bool uni_s = (S.fit_uniform != 0) && (S.nfits == 1) && (S.acmethod == 1);
   if ((PART == 1 && !uni_s) || (PART == 2 && uni_s)) return;
-  double *win = lds_win[g];
+  double *win = win_lds + g * (2 * PV_WIN);
   cd res[6];
 #pragma unroll
   for (int k = 0; k < 6; ++k) res[k] = mk(0.0, 0.0);
@@ -1124,7 +1134,9 @@ __global__ __launch_bounds__(64) void k_resonant(const ProbDev *__restrict__ pro
                                                  double2 *__restrict__ chiB,
                                                  double2 *__restrict__ lowB,
                                                  const int *__restrict__ pidx) {
-  resonant_body<NCH, PART>(probp, nom, om_in, active, chiB, lowB, pidx, gridDim.z, blockIdx.z);
+  __shared__ double win[(64 / NCH) * 2 * PV_WIN];
+  resonant_body<NCH, PART>(probp, nom, om_in, active, chiB, lowB, pidx, gridDim.z, blockIdx.z,
+                           blockIdx.x, blockIdx.y, threadIdx.x, win);
 }
 
 // The principal-value and Landau parts in one launch (small batches, where a
@@ -1135,13 +1147,17 @@ template <int NCH, bool GEN>
 __global__ __launch_bounds__(64) void k_resonant_all(const ProbDev *__restrict__ probp, int nom,
                                                      const double2 *__restrict__ om_in,
                                                      double2 *__restrict__ chiB, int RB) {
-  const int z = blockIdx.z;
+  __shared__ double win[(64 / NCH) * 2 * PV_WIN];
+  const int z = blockIdx.z, bx = blockIdx.x, by = blockIdx.y, lane = threadIdx.x;
   if (z < RB) {
-    resonant_body<NCH, 0>(probp, nom, om_in, nullptr, chiB, nullptr, nullptr, RB, z);
+    resonant_body<NCH, 0>(probp, nom, om_in, nullptr, chiB, nullptr, nullptr, RB, z, bx, by, lane, win);
     return;
   }
-  resonant_body<NCH, 1>(probp, nom, om_in, nullptr, chiB, nullptr, nullptr, RB, z - RB);
-  if constexpr (GEN) resonant_body<NCH, 2>(probp, nom, om_in, nullptr, chiB, nullptr, nullptr, RB, z - RB);
+  resonant_body<NCH, 1>(probp, nom, om_in, nullptr, chiB, nullptr, nullptr, RB, z - RB, bx, by, lane,
+                        win);
+  if constexpr (GEN)
+    resonant_body<NCH, 2>(probp, nom, om_in, nullptr, chiB, nullptr, nullptr, RB, z - RB, bx, by,
+                          lane, win);
 }
 
 // ---------------------------------------------------------------------------
@@ -1515,7 +1531,89 @@ __global__ __launch_bounds__(256) void k_assemble_osc(
   osc_step(nom, S, det, om_next, round_ctr, remaining);
 }
 
+// One whole secant round of a small batch in one launch.  Workgroups
+// [0, nom RC) run the request sweep (omega b % nom, workgroup b / nom of RC);
+// the others run four resonant wavefronts each (both parts, every harmonic
+// block).  Each workgroup publishes its partial sums with a device-scope
+// fence and counts itself in ctl[2]; the workgroup that arrives last
+// assembles det D of every omega and steps every solve (ctl[0] round, ctl[1]
+// unfinished count), then clears ctl[2] for the next round.  No workgroup
+// waits for another.
+template <int NCHS, int K, int NCHR, bool GEN>
+__global__ __launch_bounds__(256) void k_round(const ProbDev *__restrict__ probp, int nom,
+                                               double2 *__restrict__ om, double2 *__restrict__ chiA,
+                                               double2 *__restrict__ chiB, double2 *__restrict__ det,
+                                               const int *__restrict__ zidx, int RC, int RB,
+                                               int nspec, alpsg_osc::OscMachine *__restrict__ S,
+                                               int *__restrict__ ctl) {
+  constexpr int OMB = 64 / NCHR;
+  __shared__ double win[4][OMB * 2 * PV_WIN];
+  __shared__ int s_last;
+  const int A = nom * RC;
+  const int b = blockIdx.x;
+  if (b < A) {
+    sweep_req_body<NCHS, 4, K>(probp, nom, om, chiA, zidx, b % nom, b / nom, RC);
+  } else {
+    const int gx = (nom + OMB - 1) / OMB;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int vb = (b - A) * 4 + wave;
+    if (vb < gx * nspec * 2 * RB) {
+      const int x = vb % gx, y = (vb / gx) % nspec, z = vb / (gx * nspec);
+      if (z < RB) {
+        resonant_body<NCHR, 0>(probp, nom, om, nullptr, chiB, nullptr, nullptr, RB, z, x, y, lane,
+                               win[wave]);
+      } else {
+        resonant_body<NCHR, 1>(probp, nom, om, nullptr, chiB, nullptr, nullptr, RB, z - RB, x, y,
+                               lane, win[wave]);
+        if constexpr (GEN)
+          resonant_body<NCHR, 2>(probp, nom, om, nullptr, chiB, nullptr, nullptr, RB, z - RB, x, y,
+                                 lane, win[wave]);
+      }
+    }
+  }
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) s_last = (atomicAdd(&ctl[2], 1) == (int)gridDim.x - 1) ? 1 : 0;
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  const int opb = 256 / (6 * nspec);
+  for (int blk = 0; blk * opb < nom; ++blk) {
+    assemble_block(blk, probp, nom, 4 * RC, RB, om, nullptr, chiA, chiB, nullptr, nullptr, nullptr, 0,
+                   nullptr, det, nullptr, nullptr, nullptr, nspec, nullptr);
+    __syncthreads();
+  }
+  osc_step(nom, S, det, om, ctl, ctl + 1);
+  if (threadIdx.x == 0) ctl[2] = 0;
+}
+
 }  // namespace
+
+// one launch per secant round of a small batch (request sweep with nchs = 32
+// lanes per row pair and K = 4 records in flight, 4 wavefronts per workgroup,
+// RC workgroups per omega; resonant parts with nchr lanes per omega and RB
+// harmonic blocks); ctl = {round, unfinished, arrivals}
+hipError_t launch_round(hipStream_t st, int nchr, int RB, int RC, const ProbDev *prob, int nspec,
+                        int nom, double2 *om, double2 *chiA, double2 *chiB, double2 *det,
+                        const int *zidx, bool general, alpsg_osc::OscMachine *S, int *ctl) {
+  if (nom <= 0) return hipSuccess;
+  if (nom > 256 || 6 * nspec > 256) return hipErrorInvalidValue;
+#define RD(N)                                                                                      \
+  if (nchr == N) {                                                                                 \
+    const int gx = (nom + (64 / N) - 1) / (64 / N);                                                \
+    const int grid = nom * RC + (gx * nspec * 2 * RB + 3) / 4;                                     \
+    if (general)                                                                                   \
+      hipLaunchKernelGGL((k_round<32, 4, N, true>), dim3(grid), dim3(256), 0, st, prob, nom, om,   \
+                         chiA, chiB, det, zidx, RC, RB, nspec, S, ctl);                            \
+    else                                                                                           \
+      hipLaunchKernelGGL((k_round<32, 4, N, false>), dim3(grid), dim3(256), 0, st, prob, nom, om,  \
+                         chiA, chiB, det, zidx, RC, RB, nspec, S, ctl);                            \
+    return hipGetLastError();                                                                      \
+  }
+  RD(4) RD(8) RD(16) RD(32) RD(64)
+#undef RD
+  return hipErrorInvalidValue;
+}
 
 // both resonant parts in one launch (small batches without active mask or
 // low-order outputs): nch lanes per omega, RB harmonic blocks
