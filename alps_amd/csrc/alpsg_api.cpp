@@ -198,9 +198,8 @@ struct alpsg_ctx {
   DevBuf<unsigned char> mhit_d;
   PinBytes map_pin;  // the grid, then D, log10|D| and the minimum flags
   // secant_osc rounds on the device: the solves' state machines, the round
-  // counter, unfinished count and arrival count of a fused round ([0], [1],
-  // [2]) with a pinned copy of the first two, and graphs of OSC_K rounds keyed
-  // by batch size, launch shape and buffers
+  // counter and unfinished count ([0], [1]) and its pinned copy, and graphs of
+  // OSC_K rounds keyed by batch size, launch shape and buffers
   DevBuf<alpsg_osc::OscMachine> osc_d;
   DevBuf<int> oscctl_d;
   int *oscctl_pin = nullptr;
@@ -701,7 +700,7 @@ int osc_graph_batch(alpsg_ctx *h, const alpsg_root_params *rp, int n, cplx *om, 
     HIPCHK(hipMemsetAsync(h->zidx.p, 0, sizeof(int) * REQ_MAX_N, h->st));
   }
   HIPCHK(h->osc_d.ensure(n));
-  HIPCHK(h->oscctl_d.ensure(3));
+  HIPCHK(h->oscctl_d.ensure(2));
   if (!h->oscctl_pin)
     HIPCHK(hipHostMalloc((void **)&h->oscctl_pin, 2 * sizeof(int), hipHostMallocDefault));
   std::vector<alpsg_osc::OscMachine> S(n);
@@ -711,7 +710,7 @@ int osc_graph_batch(alpsg_ctx *h, const alpsg_root_params *rp, int n, cplx *om, 
     const alpsg_osc::zc x = S[r].want();
     w[r] = make_double2(x.re, x.im);
   }
-  const int ctl0[3] = {0, n, 0};  // round, unfinished solves, workgroups arrived in a round
+  const int ctl0[2] = {0, n};  // round, unfinished solves
   HIPCHK(hipMemcpyAsync(h->osc_d.p, S.data(), sizeof(alpsg_osc::OscMachine) * n,
                         hipMemcpyHostToDevice, h->st));
   HIPCHK(hipMemcpyAsync(h->om_d.p, w.data(), sizeof(double2) * n, hipMemcpyHostToDevice, h->st));
@@ -745,7 +744,7 @@ int osc_graph_batch(alpsg_ctx *h, const alpsg_root_params *rp, int n, cplx *om, 
     hipError_t e = hipSuccess;
     for (int k = 0; k < OSC_K && e == hipSuccess; ++k) {
       if (req && REQ_W == 4 && REQ_NCH == 32 && REQ_K == 4) {
-        // the whole round in one launch
+        // the whole round in two launches
         e = launch_round(h->st, sh.nch_b, sh.rb, REQ_TW / REQ_W, h->prob_d.p, ns, n, h->om_d.p,
                          h->chiA_d.p, h->chiB_d.p, h->det_d.p, h->zidx.p, h->any_general_fit,
                          h->osc_d.p, h->oscctl_d.p);

@@ -1531,92 +1531,75 @@ __global__ __launch_bounds__(256) void k_assemble_osc(
   osc_step(nom, S, det, om_next, round_ctr, remaining);
 }
 
-// One whole secant round of a small batch in one launch.  Workgroups
-// [0, nom RC) run the request sweep (omega b % nom, workgroup b / nom of RC);
-// the others run four resonant wavefronts each (both parts, every harmonic
-// block).  Each workgroup publishes its partial sums with a device-scope
-// fence and counts itself in ctl[2]; the workgroup that arrives last
-// assembles det D of every omega and steps every solve (ctl[0] round, ctl[1]
-// unfinished count), then clears ctl[2] for the next round.  No workgroup
-// waits for another.
+// The request sweep and both resonant parts of a small batch in one launch:
+// workgroups [0, nom RC) run the sweep (omega b % nom, workgroup b / nom of
+// RC), the others four resonant wavefronts each (every part and harmonic
+// block).
 template <int NCHS, int K, int NCHR, bool GEN>
-__global__ __launch_bounds__(256) void k_round(const ProbDev *__restrict__ probp, int nom,
-                                               double2 *__restrict__ om, double2 *__restrict__ chiA,
-                                               double2 *__restrict__ chiB, double2 *__restrict__ det,
-                                               const int *__restrict__ zidx, int RC, int RB,
-                                               int nspec, alpsg_osc::OscMachine *__restrict__ S,
-                                               int *__restrict__ ctl) {
+__global__ __launch_bounds__(256) void k_sweep_res(const ProbDev *__restrict__ probp, int nom,
+                                                   const double2 *__restrict__ om,
+                                                   double2 *__restrict__ chiA,
+                                                   double2 *__restrict__ chiB,
+                                                   const int *__restrict__ zidx, int RC, int RB,
+                                                   int nspec) {
   constexpr int OMB = 64 / NCHR;
   __shared__ double win[4][OMB * 2 * PV_WIN];
-  __shared__ int s_last;
   const int A = nom * RC;
   const int b = blockIdx.x;
   if (b < A) {
     sweep_req_body<NCHS, 4, K>(probp, nom, om, chiA, zidx, b % nom, b / nom, RC);
-  } else {
-    const int gx = (nom + OMB - 1) / OMB;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int vb = (b - A) * 4 + wave;
-    if (vb < gx * nspec * 2 * RB) {
-      const int x = vb % gx, y = (vb / gx) % nspec, z = vb / (gx * nspec);
-      if (z < RB) {
-        resonant_body<NCHR, 0>(probp, nom, om, nullptr, chiB, nullptr, nullptr, RB, z, x, y, lane,
-                               win[wave]);
-      } else {
-        resonant_body<NCHR, 1>(probp, nom, om, nullptr, chiB, nullptr, nullptr, RB, z - RB, x, y,
-                               lane, win[wave]);
-        if constexpr (GEN)
-          resonant_body<NCHR, 2>(probp, nom, om, nullptr, chiB, nullptr, nullptr, RB, z - RB, x, y,
-                                 lane, win[wave]);
-      }
-    }
+    return;
   }
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) s_last = (atomicAdd(&ctl[2], 1) == (int)gridDim.x - 1) ? 1 : 0;
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  const int opb = 256 / (6 * nspec);
-  for (int blk = 0; blk * opb < nom; ++blk) {
-    assemble_block(blk, probp, nom, 4 * RC, RB, om, nullptr, chiA, chiB, nullptr, nullptr, nullptr, 0,
-                   nullptr, det, nullptr, nullptr, nullptr, nspec, nullptr);
-    __syncthreads();
+  const int gx = (nom + OMB - 1) / OMB;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int vb = (b - A) * 4 + wave;
+  if (vb >= gx * nspec * 2 * RB) return;
+  const int x = vb % gx, y = (vb / gx) % nspec, z = vb / (gx * nspec);
+  if (z < RB) {
+    resonant_body<NCHR, 0>(probp, nom, om, nullptr, chiB, nullptr, nullptr, RB, z, x, y, lane,
+                           win[wave]);
+    return;
   }
-  osc_step(nom, S, det, om, ctl, ctl + 1);
-  if (threadIdx.x == 0) ctl[2] = 0;
+  resonant_body<NCHR, 1>(probp, nom, om, nullptr, chiB, nullptr, nullptr, RB, z - RB, x, y, lane,
+                         win[wave]);
+  if constexpr (GEN)
+    resonant_body<NCHR, 2>(probp, nom, om, nullptr, chiB, nullptr, nullptr, RB, z - RB, x, y, lane,
+                           win[wave]);
 }
 
 }  // namespace
 
-// one launch per secant round of a small batch (request sweep with nchs = 32
-// lanes per row pair and K = 4 records in flight, 4 wavefronts per workgroup,
-// RC workgroups per omega; resonant parts with nchr lanes per omega and RB
-// harmonic blocks); ctl = {round, unfinished, arrivals}
+// a secant round of a small batch in two launches: the request sweep (32
+// lanes per row pair, 4 records in flight, 4 wavefronts per workgroup, RC
+// workgroups per omega) with both resonant parts (nchr lanes per omega, RB
+// harmonic blocks), then the assembly and the secant step in one workgroup;
+// ctl = {round, unfinished}
 hipError_t launch_round(hipStream_t st, int nchr, int RB, int RC, const ProbDev *prob, int nspec,
                         int nom, double2 *om, double2 *chiA, double2 *chiB, double2 *det,
                         const int *zidx, bool general, alpsg_osc::OscMachine *S, int *ctl) {
   if (nom <= 0) return hipSuccess;
   if (nom > 256 || 6 * nspec > 256) return hipErrorInvalidValue;
+  bool ok = false;
 #define RD(N)                                                                                      \
   if (nchr == N) {                                                                                 \
     const int gx = (nom + (64 / N) - 1) / (64 / N);                                                \
     const int grid = nom * RC + (gx * nspec * 2 * RB + 3) / 4;                                     \
     if (general)                                                                                   \
-      hipLaunchKernelGGL((k_round<32, 4, N, true>), dim3(grid), dim3(256), 0, st, prob, nom, om,   \
-                         chiA, chiB, det, zidx, RC, RB, nspec, S, ctl);                            \
+      hipLaunchKernelGGL((k_sweep_res<32, 4, N, true>), dim3(grid), dim3(256), 0, st, prob, nom,   \
+                         om, chiA, chiB, zidx, RC, RB, nspec);                                     \
     else                                                                                           \
-      hipLaunchKernelGGL((k_round<32, 4, N, false>), dim3(grid), dim3(256), 0, st, prob, nom, om,  \
-                         chiA, chiB, det, zidx, RC, RB, nspec, S, ctl);                            \
-    return hipGetLastError();                                                                      \
+      hipLaunchKernelGGL((k_sweep_res<32, 4, N, false>), dim3(grid), dim3(256), 0, st, prob, nom,  \
+                         om, chiA, chiB, zidx, RC, RB, nspec);                                     \
+    ok = true;                                                                                     \
   }
   RD(4) RD(8) RD(16) RD(32) RD(64)
 #undef RD
-  return hipErrorInvalidValue;
+  if (!ok) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_assemble_osc, dim3(1), dim3(256), 0, st, prob, nom, 4 * RC, RB, om, chiA, chiB,
+                     det, nspec, S, om, ctl, ctl + 1);
+  return hipGetLastError();
 }
 
-// both resonant parts in one launch (small batches without active mask or
-// low-order outputs): nch lanes per omega, RB harmonic blocks
 hipError_t launch_resonant_all(hipStream_t st, int nch, int RB, const ProbDev *prob, int nspec,
                                int nom, const double2 *om, double2 *chiB, bool general) {
   if (nom <= 0) return hipSuccess;
