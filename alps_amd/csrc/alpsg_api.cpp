@@ -40,7 +40,8 @@ hipError_t launch_sweep(hipStream_t st, int nch, int W, int RC, const ProbDev *p
                         const double2 *om, const int *active, double2 *chiA, double2 *lowA,
                         const int *pidx = nullptr);
 hipError_t launch_sweep_req(hipStream_t st, int nch, int W, int RC, int K, const ProbDev *prob,
-                            int nom, const double2 *om, double2 *chiA, const int *pidx);
+                            int nom, const double2 *om, double2 *chiA, const int *pidx,
+                            bool reduce);
 hipError_t launch_resonant(hipStream_t st, int nch, int RB, const ProbDev *prob, int nspec, int nom,
                            const double2 *om, const int *active, double2 *chiB, double2 *lowB,
                            bool general, const int *pidx = nullptr);
@@ -379,12 +380,12 @@ int disp_device(alpsg_ctx *h, int n, const double2 *om_dev, const int *active_de
       HIPCHK(hipMemsetAsync(h->zidx.p, 0, sizeof(int) * REQ_MAX_N, h->st));
     }
     HIPCHK(launch_sweep_req(h->st, REQ_NCH, REQ_W, REQ_TW / REQ_W, REQ_K, h->prob_d.p, n, om_dev,
-                            h->chiA_d.p, h->zidx.p));
+                            h->chiA_d.p, h->zidx.p, true));
   } else {
     HIPCHK(launch_sweep(h->st, sh.nch_a, sh.w_a, sh.rc_a, h->prob_d.p, n, om_dev, active_dev,
                         h->chiA_d.p, want_low ? h->lowA_d.p : nullptr));
   }
-  const int TWa = req ? REQ_TW : TW;
+  const int TWa = req ? REQ_TW / REQ_W : TW;  // the request sweep writes one partial per workgroup
   HIPCHK(hipEventRecord(h->ev_sweep, h->st));
   if (req)
     HIPCHK(launch_resonant_all(h->st, sh.nch_b, sh.rb, h->prob_d.p, ns, n, om_dev, h->chiB_d.p,
@@ -563,12 +564,12 @@ int graph_disp(alpsg_ctx *h, int n, const double *om, double *det) {
                                   hipMemcpyHostToDevice, h->st);
     if (e == hipSuccess)
       e = launch_sweep_req(h->st, REQ_NCH, REQ_W, REQ_TW / REQ_W, REQ_K, h->prob_d.p, n, h->om_d.p,
-                           h->chiA_d.p, h->zidx.p);
+                           h->chiA_d.p, h->zidx.p, true);
     if (e == hipSuccess)
       e = launch_resonant_all(h->st, sh.nch_b, sh.rb, h->prob_d.p, ns, n, h->om_d.p, h->chiB_d.p,
                               h->any_general_fit);
     if (e == hipSuccess)
-      e = launch_assemble(h->st, h->prob_d.p, ns, n, REQ_TW, sh.rb, h->om_d.p, nullptr, h->chiA_d.p,
+      e = launch_assemble(h->st, h->prob_d.p, ns, n, REQ_TW / REQ_W, sh.rb, h->om_d.p, nullptr, h->chiA_d.p,
                           h->chiB_d.p, nullptr, nullptr, nullptr, 0, nullptr, h->det_d.p, nullptr,
                           nullptr, nullptr);
     if (e == hipSuccess)
@@ -689,7 +690,7 @@ int osc_graph_batch(alpsg_ctx *h, const alpsg_root_params *rp, int n, cplx *om, 
   const int ns = h->nspec;
   LaunchShape sh = shape_for(h, n);
   const bool req = n <= REQ_MAX_N;
-  const int TW = req ? REQ_TW : sh.w_a * sh.rc_a;
+  const int TW = req ? REQ_TW / REQ_W : sh.w_a * sh.rc_a;  // partial slots of the sweep
   const int shp[5] = {sh.nch_a, sh.w_a, sh.rc_a, sh.nch_b, sh.rb};
   HIPCHK(h->om_d.ensure(std::max(n, REQ_MAX_N)));
   HIPCHK(h->det_d.ensure(std::max(n, REQ_MAX_N)));
@@ -752,7 +753,7 @@ int osc_graph_batch(alpsg_ctx *h, const alpsg_root_params *rp, int n, cplx *om, 
       }
       if (req)
         e = launch_sweep_req(h->st, REQ_NCH, REQ_W, REQ_TW / REQ_W, REQ_K, h->prob_d.p, n, h->om_d.p,
-                             h->chiA_d.p, h->zidx.p);
+                             h->chiA_d.p, h->zidx.p, true);
       else
         e = launch_sweep(h->st, sh.nch_a, sh.w_a, sh.rc_a, h->prob_d.p, n, h->om_d.p, nullptr,
                          h->chiA_d.p, nullptr);

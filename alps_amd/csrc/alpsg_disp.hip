@@ -594,7 +594,10 @@ __global__ __launch_bounds__(64 * W) void k_sweep(const ProbDev *__restrict__ pr
 // its own problem and the launch shape.
 // ---------------------------------------------------------------------------
 // One workgroup of k_sweep_req: omega bx, workgroup by of the omega's gy.
-template <int NCH, int W, int K>
+// Each wavefront wv writes partial slot wv of chiA (W gy slots per omega), or
+// with RED the workgroup sums its wavefronts in order and writes slot by (gy
+// slots per omega).
+template <int NCH, int W, int K, bool RED>
 __device__ __forceinline__ void sweep_req_body(const ProbDev *__restrict__ probp, int nom,
                                                const double2 *__restrict__ om_in,
                                                double2 *__restrict__ chiA,
@@ -613,6 +616,7 @@ __device__ __forceinline__ void sweep_req_body(const ProbDev *__restrict__ probp
   const int nspec = Pd.nspec;
   const double kpar = Pd.kpar;
   __shared__ double lpa[LPA];
+  __shared__ double red[W][12];
   int rowbase = 0;
   for (int s = 0; s < nspec; ++s) {
     const SpecDev &S = Pd.sp[s];
@@ -702,12 +706,27 @@ __device__ __forceinline__ void sweep_req_body(const ProbDev *__restrict__ probp
       }
     }
     rowbase += S.nmax + 1;
-    // the lanes of the wavefront, combined pairwise in a fixed order
+    // the lanes of the wavefront, combined pairwise in a fixed order, then
+    // the wavefronts of the workgroup in order: one partial per workgroup
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1)
 #pragma unroll
       for (int t = 0; t < 12; ++t) run[t] += __shfl_xor(run[t], o, 64);
-    if (lane == 0) {
+    if constexpr (RED) {
+      __syncthreads();
+      if (lane == 0)
+#pragma unroll
+        for (int t = 0; t < 12; ++t) red[threadIdx.x >> 6][t] = run[t];
+      __syncthreads();
+      if (threadIdx.x < 12) {
+        double v = red[0][threadIdx.x];
+#pragma unroll
+        for (int w = 1; w < W; ++w) v += red[w][threadIdx.x];
+        double *o = reinterpret_cast<double *>(
+            chiA + ((size_t)(by * nspec + s) * 6 + (threadIdx.x >> 1)) * nom + iom);
+        o[threadIdx.x & 1] = v;
+      }
+    } else if (lane == 0) {
       double2 *o = chiA + ((size_t)(wv * nspec + s) * 6) * nom + iom;
 #pragma unroll
       for (int k = 0; k < 6; ++k) o[(size_t)k * nom] = make_double2(run[2 * k], run[2 * k + 1]);
@@ -715,12 +734,12 @@ __device__ __forceinline__ void sweep_req_body(const ProbDev *__restrict__ probp
   }
 }
 
-template <int NCH, int W, int K>
+template <int NCH, int W, int K, bool RED>
 __global__ __launch_bounds__(64 * W) void k_sweep_req(const ProbDev *__restrict__ probp, int nom,
                                                       const double2 *__restrict__ om_in,
                                                       double2 *__restrict__ chiA,
                                                       const int *__restrict__ pidx) {
-  sweep_req_body<NCH, W, K>(probp, nom, om_in, chiA, pidx, blockIdx.x, blockIdx.y, gridDim.y);
+  sweep_req_body<NCH, W, K, RED>(probp, nom, om_in, chiA, pidx, blockIdx.x, blockIdx.y, gridDim.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -1183,13 +1202,13 @@ __device__ __forceinline__ void assemble_block(
   const bool in = (lo_om < opb) && (iom < nom) && (active == nullptr || active[iom] != 0);
   const ProbDev &Pd = pidx != nullptr ? probp[pidx[min(iom, nom - 1)]] : *probp;
   if (in) {
-    // partial sums, loads issued in batches of 8
+    // partial sums, loads issued in batches of 16
     cd v = mk(0.0, 0.0);
     const int tot = W + 2 * RB;
-    for (int base = 0; base < tot; base += 8) {
-      double2 buf[8];
+    for (int base = 0; base < tot; base += 16) {
+      double2 buf[16];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
+      for (int u = 0; u < 16; ++u) {
         const int j = base + u;
         if (j < W)
           buf[u] = chiA[((size_t)(j * nspec + s) * 6 + k) * nom + iom];
@@ -1199,7 +1218,7 @@ __device__ __forceinline__ void assemble_block(
           buf[u] = make_double2(0.0, 0.0);
       }
 #pragma unroll
-      for (int u = 0; u < 8; ++u) v = v + mk(buf[u].x, buf[u].y);
+      for (int u = 0; u < 16; ++u) v = v + mk(buf[u].x, buf[u].y);
     }
     if (Pd.sp[s].rel)  // relativistic species: the partial slots of k_rel_nonres / k_rel_res
       for (int j = 0; j < RR; ++j) {
@@ -1341,15 +1360,23 @@ hipError_t launch_sweep(hipStream_t st, int nch, int W, int RC, const ProbDev *p
 }
 
 // the per-omega-problem sweep (k_sweep_req): nch lanes per row pair, W
-// wavefronts per workgroup, RC workgroups per omega
+// wavefronts per workgroup, RC workgroups per omega; W RC partial slots per
+// omega, or RC with reduce (the 32/4/4 shape only)
 hipError_t launch_sweep_req(hipStream_t st, int nch, int W, int RC, int K, const ProbDev *prob,
-                            int nom, const double2 *om, double2 *chiA, const int *pidx) {
+                            int nom, const double2 *om, double2 *chiA, const int *pidx,
+                            bool reduce) {
   if (nom <= 0) return hipSuccess;
   if (pidx == nullptr) return hipErrorInvalidValue;
+  if (reduce) {
+    if (!(nch == 32 && W == 4 && K == 4)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_sweep_req<32, 4, 4, true>), dim3(nom, RC), dim3(256), 0, st, prob, nom,
+                       om, chiA, pidx);
+    return hipGetLastError();
+  }
 #define SQ(N, WV, KK)                                                                             \
   if (nch == N && W == WV && K == KK) {                                                           \
-    hipLaunchKernelGGL((k_sweep_req<N, WV, KK>), dim3(nom, RC), dim3(64 * WV), 0, st, prob, nom,  \
-                       om, chiA, pidx);                                                           \
+    hipLaunchKernelGGL((k_sweep_req<N, WV, KK, false>), dim3(nom, RC), dim3(64 * WV), 0, st, prob, \
+                       nom, om, chiA, pidx);                                                      \
     return hipGetLastError();                                                                     \
   }
   SQ(16, 8, 1) SQ(32, 8, 1) SQ(16, 8, 2) SQ(32, 8, 2) SQ(16, 8, 4) SQ(32, 8, 4) SQ(32, 4, 4)
@@ -1547,7 +1574,7 @@ __global__ __launch_bounds__(256) void k_sweep_res(const ProbDev *__restrict__ p
   const int A = nom * RC;
   const int b = blockIdx.x;
   if (b < A) {
-    sweep_req_body<NCHS, 4, K>(probp, nom, om, chiA, zidx, b % nom, b / nom, RC);
+    sweep_req_body<NCHS, 4, K, true>(probp, nom, om, chiA, zidx, b % nom, b / nom, RC);
     return;
   }
   const int gx = (nom + OMB - 1) / OMB;
@@ -1595,7 +1622,7 @@ hipError_t launch_round(hipStream_t st, int nchr, int RB, int RC, const ProbDev 
   RD(4) RD(8) RD(16) RD(32) RD(64)
 #undef RD
   if (!ok) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_assemble_osc, dim3(1), dim3(256), 0, st, prob, nom, 4 * RC, RB, om, chiA, chiB,
+  hipLaunchKernelGGL(k_assemble_osc, dim3(1), dim3(256), 0, st, prob, nom, RC, RB, om, chiA, chiB,
                      det, nspec, S, om, ctl, ctl + 1);
   return hipGetLastError();
 }

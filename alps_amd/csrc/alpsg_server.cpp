@@ -67,7 +67,8 @@ hipError_t launch_species_k_jobs(hipStream_t st, const TabJob *jobs, int njobs,
 hipError_t launch_scatter_probs(hipStream_t st, const ProbDev *src, const int *slots, int n,
                                 ProbDev *dst);
 hipError_t launch_sweep_req(hipStream_t st, int nch, int W, int RC, int K, const ProbDev *prob,
-                            int nom, const double2 *om, double2 *chiA, const int *pidx);
+                            int nom, const double2 *om, double2 *chiA, const int *pidx,
+                            bool reduce);
 hipError_t launch_resonant(hipStream_t st, int nch, int RB, const ProbDev *prob, int nspec, int nom,
                            const double2 *om, const int *active, double2 *chiB, double2 *lowB,
                            bool general, const int *pidx);
@@ -742,7 +743,7 @@ int server_pass(alpsg_server *S) {
         if (act[i1]->pl->general) gen = true;
       const int n = i1 - i0;
       SCHK(launch_sweep_req(S->st, SW_NCH, SW_W, SW_RC, SW_K, S->probs_d, n, S->om_d + i0,
-                            S->chiA, S->pidx_d + i0));
+                            S->chiA, S->pidx_d + i0, false));
       SCHK(launch_resonant(S->st, 64, RS_RB, S->probs_d, ns, n, S->om_d + i0, nullptr, S->chiB,
                            nullptr, gen, S->pidx_d + i0));
       SCHK(launch_assemble(S->st, S->probs_d, ns, n, SW_W * SW_RC, RS_RB, S->om_d + i0, nullptr,
