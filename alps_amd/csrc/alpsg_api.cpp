@@ -23,6 +23,7 @@
 #include "../../include/alpsg.h"
 #include "alpsg_device.h"
 #include "alpsg_osc.h"
+#include "alpsg_relhost.h"
 
 // kernels (alpsg_tables.hip, alpsg_disp.hip)
 hipError_t launch_derivative(hipStream_t st, int nspec, int nperp, int npar, const double *f0,
@@ -46,7 +47,7 @@ hipError_t launch_resonant(hipStream_t st, int nch, int RB, const ProbDev *prob,
                            const double2 *om, const int *active, double2 *chiB, double2 *lowB,
                            bool general, const int *pidx = nullptr);
 hipError_t launch_nhds(hipStream_t st, const ProbDev *prob, int nspec, int nom, const double2 *om,
-                       const int *active, double2 *chiA, double2 *lowA);
+                       const int *active, double2 *chiA, double2 *lowA, const int *pidx = nullptr);
 int nhds_tables(double z, int bm_nmax, double bessel_zero, std::vector<double2> &tab);
 hipError_t launch_split_tables(hipStream_t st, int npar, int nmax, double kpar, double q, double m,
                                const double *nodes, double *crec, double *rsum);
@@ -66,7 +67,7 @@ hipError_t launch_assemble_osc(hipStream_t st, const ProbDev *prob, int nspec, i
                                alpsg_osc::OscMachine *S, int *round_ctr, int *remaining);
 hipError_t launch_rel(hipStream_t st, const ProbDev *prob, int nspec, int nmaxR, int NB, int nom,
                       const double2 *om, const int *active, double2 *chiR, double2 *lowR,
-                      int *flags, int *err);
+                      int *flags, int *err, const int *pidx = nullptr);
 int rel_slots(int NB, int nmaxR);
 hipError_t launch_assemble(hipStream_t st, const ProbDev *prob, int nspec, int nom, int W, int RB,
                            const double2 *om, const int *active, const double2 *chiA,
@@ -1254,6 +1255,122 @@ int calc_eigen_batch(alpsg_ctx *h, int n, const cplx *om, bool eigen, bool heat,
 
 }  // namespace
 
+int rel_host_tables(const alpsg_plasma_desc *d, int s, int r, int nspec_rel, const double *pe,
+                    const double *pa, RelHost &o) {
+  const int ngam = d->ngamma, npb = d->npparbar, nperp = d->nperp, npar = d->npar;
+  // Fortran layout (1:nspec_rel, 0:ngamma, 0:npparbar[, 1:2])
+  auto frel = [&](const double *a, int ig, int ip, int c) {
+    return a[(size_t)r + (size_t)nspec_rel * (ig + (size_t)(ngam + 1) * (ip + (size_t)(npb + 1) * c))];
+  };
+  const size_t np = (size_t)(ngam + 1) * (npb + 1);
+  o.G.resize(np);
+  o.PB.resize(np);
+  o.F.resize(np);
+  o.DF.resize(np);
+  for (int ig = 0; ig <= ngam; ++ig)
+    for (int ip = 0; ip <= npb; ++ip) {
+      const size_t k = (size_t)ig * (npb + 1) + ip;
+      o.G[k] = frel(d->gamma_rel, ig, ip, 0);
+      o.PB[k] = frel(d->pparbar_rel, ig, ip, 0);
+      o.F[k] = frel(d->f0_rel, ig, ip, 0);
+      o.DF[k] = make_double2(frel(d->df0_rel, ig, ip, 0), frel(d->df0_rel, ig, ip, 1));
+    }
+  for (int ip = 0; ip < npb; ++ip)
+    if (!(o.PB[(size_t)2 * (npb + 1) + ip + 1] > o.PB[(size_t)2 * (npb + 1) + ip]))
+      return fail(ALPSG_E_ARG, "pparbar_rel rows must increase");
+  o.lim.resize(ngam + 1);
+  for (int ig = 0; ig <= ngam; ++ig) {
+    int lo = 1, up = npb - 1;
+    bool fl = false, fu = false;
+    const double *f = &o.F[(size_t)ig * (npb + 1)];
+    for (int ip = 1; ip <= npb - 1; ++ip) {
+      if (!fl && f[ip - 1] <= -1.0 && f[ip] > -1.0) {
+        lo = ip;
+        fl = true;
+      }
+      if (!fu && f[ip] > -1.0 && f[ip + 1] <= -1.0) {
+        up = ip;
+        fu = true;
+      }
+    }
+    o.lim[ig] = make_int2(lo, up);
+  }
+  const double vA = d->vA, m = d->ms[s], q = d->qs[s];
+  o.dg = o.G[(size_t)2 * (npb + 1) + 2] - o.G[(size_t)1 * (npb + 1) + 2];
+  o.dpb = o.PB[(size_t)2 * (npb + 1) + 2] - o.PB[(size_t)2 * (npb + 1) + 1];
+  double ee = 0.0;
+  {
+    auto term = [&](int ig, int ip) {
+      const size_t k = (size_t)ig * (npb + 1) + ip;
+      return o.PB[k] * o.DF[k].y;
+    };
+    int ig = ngam - 1;
+    ee = ee + term(ig, o.lim[ig].x);
+    ee = ee + term(ig, o.lim[ig].y);
+    for (int ip = o.lim[ig].x + 1; ip <= o.lim[ig].y - 1; ++ip) ee = ee + 2.0 * term(ig, ip);
+    for (ig = 1; ig <= ngam - 2; ++ig) {
+      for (int ip = o.lim[ig].x + 1; ip <= o.lim[ig].y - 1; ++ip) ee = ee + 4.0 * term(ig, ip);
+      ee = ee + 2.0 * term(ig, o.lim[ig].x);
+      ee = ee + 2.0 * term(ig, o.lim[ig].y);
+    }
+    const double PI = 3.141592653589793238462643383279502884;
+    ee = ee * 2.0 * PI * q / m;
+    ee = ee * o.dg * o.dpb * 0.25 * ((m / vA) * (m / vA) * (m / vA));
+  }
+  o.ee = ee;
+  o.gpp.resize((size_t)(nperp + 1) * (npar + 1));
+  for (int i = 0; i <= nperp; ++i)
+    for (int j = 0; j <= npar; ++j) {
+      const double a = pe[i], b = pa[j];
+      o.gpp[(size_t)i * (npar + 1) + j] = std::sqrt((a * a + b * b) * (vA * vA) / (m * m) + 1.0);
+    }
+  return ALPSG_OK;
+}
+
+void rel_fit_rows(const alpsg_plasma_desc *d, int s, int r, int nspec_rel, double *fitc) {
+  // Juettner fits in Gamma (ALPS_analyt.f90:238-252) on the igamma rows, in
+  // the Maxwell form c0*exp(-c1*(p-c2)^2): type 4 has c1 = c2 = 0
+  const int nspec = d->nspec, ngam = d->ngamma, npb = d->npparbar, npf = d->param_fit_nperp;
+  for (int ig = 0; ig <= ngam; ++ig) {
+    const double gv =
+        d->gamma_rel[(size_t)r + (size_t)nspec_rel * (ig + (size_t)(ngam + 1) * 1)];
+    (void)npb;
+    for (int f = 0; f < d->n_fits[s]; ++f) {
+      auto prm = [&](int k) {
+        return d->param_fit[(size_t)s + (size_t)nspec * (ig + (size_t)(npf + 1) * (k + 5 * (size_t)f))];
+      };
+      const double pc = d->perp_correction[s + (size_t)nspec * f];
+      const int ft = d->fit_type[s + (size_t)nspec * f];
+      double *c = &fitc[((size_t)ig * d->n_fits[s] + f) * FIT_STRIDE];
+      c[0] = prm(0) * std::exp(-pc * gv);
+      c[1] = ft == 5 ? prm(1) : 0.0;
+      c[2] = ft == 5 ? prm(2) : 0.0;
+    }
+  }
+}
+
+void rel_gamma_table(std::vector<double> &gt) {
+  // Gamma(m) by the Lanczos series of ALPS_fns_rel.f90:1738-1791 and Fact(k)
+  // (:1557-1574), as CBESSJ divides by them
+  gt.assign(GTAB_N + 21, 0.0);
+  const double cof[6] = {76.18009173, -86.50532033, 24.01409822, -1.231739516, 0.120858003e-2,
+                         -0.536382e-5};
+  for (int mm = 1; mm < GTAB_N; ++mm) {
+    double x = (1.0 * mm) - 1.0, tmp = x + 5.5, ser = 1.0;
+    tmp = (x + 0.5) * std::log(tmp) - tmp;
+    for (int j = 0; j < 6; ++j) {
+      x = x + 1.0;
+      ser = ser + cof[j] / x;
+    }
+    gt[mm] = std::exp(tmp + std::log(2.50662827465 * ser));
+  }
+  for (int k = 0; k <= 20; ++k) {
+    double f = 1.0;
+    for (int i = 2; i <= k; ++i) f = f * (1.0 * i);
+    gt[GTAB_N + k] = f;
+  }
+}
+
 extern "C" {
 
 const char *alpsg_last_error(void) { return g_err.c_str(); }
@@ -1430,23 +1547,7 @@ int alpsg_create(const alpsg_plasma_desc *d, alpsg_handle *out) {
   for (int s = 0; s < nspec; ++s) {
     if (bm(s)) continue;
     if (relf(s)) {
-      // Juettner fits in Gamma (ALPS_analyt.f90:238-252) on the igamma rows, in
-      // the Maxwell form c0*exp(-c1*(p-c2)^2): type 4 has c1 = c2 = 0
-      const int r = rel_index[s];
-      for (int ig = 0; ig <= ngam; ++ig) {
-        const double gv = frel(d->gamma_rel, r, ig, 1, 0);
-        for (int f = 0; f < d->n_fits[s]; ++f) {
-          auto prm = [&](int k) {
-            return d->param_fit[(size_t)s + (size_t)nspec * (ig + (size_t)(npf + 1) * (k + 5 * (size_t)f))];
-          };
-          const double pc = d->perp_correction[s + (size_t)nspec * f];
-          const int ft = d->fit_type[s + (size_t)nspec * f];
-          double *c = &fitc[fit_off[s] + ((size_t)ig * d->n_fits[s] + f) * FIT_STRIDE];
-          c[0] = prm(0) * std::exp(-pc * gv);
-          c[1] = ft == 5 ? prm(1) : 0.0;
-          c[2] = ft == 5 ? prm(2) : 0.0;
-        }
-      }
+      rel_fit_rows(d, s, rel_index[s], nspec_rel, &fitc[fit_off[s]]);
       continue;
     }
     if (d->ACmethod[s] == 1) {
@@ -1491,25 +1592,8 @@ int alpsg_create(const alpsg_plasma_desc *d, alpsg_handle *out) {
   HIPCHK(hipMemcpyAsync(h->mtab.p, mt.data(), sizeof(int) * MTAB_N, hipMemcpyHostToDevice, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
   if (nrel > 0) {
-    // Gamma(m) by the Lanczos series of ALPS_fns_rel.f90:1738-1791 and Fact(k)
-    // (:1557-1574), as CBESSJ divides by them
-    std::vector<double> gt(GTAB_N + 21, 0.0);
-    const double cof[6] = {76.18009173, -86.50532033, 24.01409822, -1.231739516, 0.120858003e-2,
-                           -0.536382e-5};
-    for (int mm = 1; mm < GTAB_N; ++mm) {
-      double x = (1.0 * mm) - 1.0, tmp = x + 5.5, ser = 1.0;
-      tmp = (x + 0.5) * std::log(tmp) - tmp;
-      for (int j = 0; j < 6; ++j) {
-        x = x + 1.0;
-        ser = ser + cof[j] / x;
-      }
-      gt[mm] = std::exp(tmp + std::log(2.50662827465 * ser));
-    }
-    for (int k = 0; k <= 20; ++k) {
-      double f = 1.0;
-      for (int i = 2; i <= k; ++i) f = f * (1.0 * i);
-      gt[GTAB_N + k] = f;
-    }
+    std::vector<double> gt;
+    rel_gamma_table(gt);
     HIPCHK(h->gtab.ensure(gt.size()));
     HIPCHK(hipMemcpy(h->gtab.p, gt.data(), sizeof(double) * gt.size(), hipMemcpyHostToDevice));
     HIPCHK(h->err_d.ensure(1));
@@ -1602,66 +1686,16 @@ int alpsg_create(const alpsg_plasma_desc *d, alpsg_handle *out) {
     if (relf(s)) {
       // (Gamma, pparbar) tables, subluminal row limits, Gamma of the (pperp, ppar)
       // points and the omega-independent int_ee_rel (ALPS_fns_rel.f90:1097-1215)
-      const int r = rel_index[s];
-      const size_t np = (size_t)(ngam + 1) * (npb + 1);
-      std::vector<double> G(np), PBv(np), F(np);
-      std::vector<double2> DF(np);
-      for (int ig = 0; ig <= ngam; ++ig)
-        for (int ip = 0; ip <= npb; ++ip) {
-          const size_t o = (size_t)ig * (npb + 1) + ip;
-          G[o] = frel(d->gamma_rel, r, ig, ip, 0);
-          PBv[o] = frel(d->pparbar_rel, r, ig, ip, 0);
-          F[o] = frel(d->f0_rel, r, ig, ip, 0);
-          DF[o] = make_double2(frel(d->df0_rel, r, ig, ip, 0), frel(d->df0_rel, r, ig, ip, 1));
-        }
-      for (int ip = 0; ip < npb; ++ip)
-        if (!(PBv[(size_t)2 * (npb + 1) + ip + 1] > PBv[(size_t)2 * (npb + 1) + ip]))
-          return bail(fail(ALPSG_E_ARG, "pparbar_rel rows must increase"));
-      std::vector<int2> lim(ngam + 1);
-      for (int ig = 0; ig <= ngam; ++ig) {
-        int lo = 1, up = npb - 1;
-        bool fl = false, fu = false;
-        const double *f = &F[(size_t)ig * (npb + 1)];
-        for (int ip = 1; ip <= npb - 1; ++ip) {
-          if (!fl && f[ip - 1] <= -1.0 && f[ip] > -1.0) {
-            lo = ip;
-            fl = true;
-          }
-          if (!fu && f[ip] > -1.0 && f[ip + 1] <= -1.0) {
-            up = ip;
-            fu = true;
-          }
-        }
-        lim[ig] = make_int2(lo, up);
-      }
-      const double vA = d->vA, m = d->ms[s], q = d->qs[s];
-      const double dg = G[(size_t)2 * (npb + 1) + 2] - G[(size_t)1 * (npb + 1) + 2];
-      const double dpb = PBv[(size_t)2 * (npb + 1) + 2] - PBv[(size_t)2 * (npb + 1) + 1];
-      double ee = 0.0;
+      RelHost R;
       {
-        auto term = [&](int ig, int ip) {
-          const size_t o = (size_t)ig * (npb + 1) + ip;
-          return PBv[o] * DF[o].y;
-        };
-        int ig = ngam - 1;
-        ee = ee + term(ig, lim[ig].x);
-        ee = ee + term(ig, lim[ig].y);
-        for (int ip = lim[ig].x + 1; ip <= lim[ig].y - 1; ++ip) ee = ee + 2.0 * term(ig, ip);
-        for (ig = 1; ig <= ngam - 2; ++ig) {
-          for (int ip = lim[ig].x + 1; ip <= lim[ig].y - 1; ++ip) ee = ee + 4.0 * term(ig, ip);
-          ee = ee + 2.0 * term(ig, lim[ig].x);
-          ee = ee + 2.0 * term(ig, lim[ig].y);
-        }
-        const double PI = 3.141592653589793238462643383279502884;
-        ee = ee * 2.0 * PI * q / m;
-        ee = ee * dg * dpb * 0.25 * ((m / vA) * (m / vA) * (m / vA));
+        const int rc = rel_host_tables(d, s, rel_index[s], nspec_rel, pe, pa, R);
+        if (rc) return bail(rc);
       }
-      std::vector<double> gpp((size_t)(nperp + 1) * (npar + 1));
-      for (int i = 0; i <= nperp; ++i)
-        for (int j = 0; j <= npar; ++j) {
-          const double a = pe[i], b = pa[j];
-          gpp[(size_t)i * (npar + 1) + j] = std::sqrt((a * a + b * b) * (vA * vA) / (m * m) + 1.0);
-        }
+      const size_t np = R.G.size();
+      const std::vector<double> &G = R.G, &PBv = R.PB, &F = R.F, &gpp = R.gpp;
+      const std::vector<double2> &DF = R.DF;
+      const std::vector<int2> &lim = R.lim;
+      const double dg = R.dg, dpb = R.dpb, ee = R.ee;
       HIPCHK(h->grel[s].ensure(np));
       HIPCHK(h->pbrel[s].ensure(np));
       HIPCHK(h->f0rel[s].ensure(np));

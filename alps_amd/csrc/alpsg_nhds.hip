@@ -277,11 +277,14 @@ __global__ __launch_bounds__(64) void k_nhds(const ProbDev *__restrict__ probp, 
                                              const double2 *__restrict__ om_in,
                                              const int *__restrict__ active,
                                              double2 *__restrict__ chiA,
-                                             double2 *__restrict__ lowA) {
+                                             double2 *__restrict__ lowA,
+                                             const int *__restrict__ pidx) {
   constexpr int OMB = 64 / NCH;
   __shared__ double2 lbuf[OMB][18];
-  const ProbDev &Pd = *probp;
+  // with problem indices one omega per workgroup (NCH = 64)
+  const ProbDev &Pd = (NCH == 64 && pidx != nullptr) ? probp[pidx[min((int)blockIdx.x, nom - 1)]] : *probp;
   const int s = blockIdx.y;
+  if (s >= Pd.nspec) return;
   const SpecDev &S = Pd.sp[s];
   if (!S.usebm) return;
   const int lane = threadIdx.x;
@@ -384,15 +387,16 @@ int nhds_tables(double z, int bm_nmax, double bessel_zero, std::vector<double2> 
 }
 
 hipError_t launch_nhds(hipStream_t st, const ProbDev *prob, int nspec, int nom, const double2 *om,
-                       const int *active, double2 *chiA, double2 *lowA) {
+                       const int *active, double2 *chiA, double2 *lowA, const int *pidx) {
   if (nom <= 0) return hipSuccess;
 #define NH(N)                                                                             \
   {                                                                                       \
     const int grid = (nom + (64 / N) - 1) / (64 / N);                                     \
     hipLaunchKernelGGL((k_nhds<N>), dim3(grid, nspec), dim3(64), 0, st, prob, nom, om, active, \
-                       chiA, lowA);                                                       \
+                       chiA, lowA, pidx);                                                 \
     return hipGetLastError();                                                             \
   }
+  if (pidx != nullptr) NH(64)
   if (nom >= 1024) NH(8)
   if (nom >= 64) NH(32)
   NH(64)

@@ -101,15 +101,16 @@ __global__ __launch_bounds__(256) void k_rel_nonres(const ProbDev *__restrict__ 
                                                     const int *__restrict__ active,
                                                     double2 *__restrict__ chiR,
                                                     double2 *__restrict__ lowR,
-                                                    int *__restrict__ flags) {
+                                                    int *__restrict__ flags,
+                                                    const int *__restrict__ pidx, int nstride) {
   __shared__ double lds[16];
-  const ProbDev &Pd = *probp;
   const int iom = blockIdx.x, s = blockIdx.y, NB = gridDim.z, nb = blockIdx.z;
+  const ProbDev &Pd = pidx != nullptr ? probp[pidx[iom]] : *probp;
+  if (s >= Pd.nspec) return;
   const SpecDev &S = Pd.sp[s];
   if (!S.rel) return;
   if (!(active == nullptr || active[iom] != 0)) return;
   const int nspec = Pd.nspec, nperp = S.nperp, npar = S.npar, nmax = S.nmax_rel;
-  const int nstride = 2 * Pd.nmaxR + 1;
   const double2 ov = om_in[iom];
   const cd om = mk(ov.x, ov.y);
   const double m = S.m, q = S.q, kpar = Pd.kpar;
@@ -544,15 +545,16 @@ __global__ __launch_bounds__(64) void k_rel_res(const ProbDev *__restrict__ prob
                                                 double2 *__restrict__ chiR,
                                                 double2 *__restrict__ lowR,
                                                 const int *__restrict__ flags, int NB,
-                                                int *__restrict__ err) {
-  const ProbDev &Pd = *probp;
+                                                int *__restrict__ err,
+                                                const int *__restrict__ pidx, int nstride) {
   const int iom = blockIdx.x, s = blockIdx.y;
+  const ProbDev &Pd = pidx != nullptr ? probp[pidx[iom]] : *probp;
+  if (s >= Pd.nspec) return;
   const SpecDev &S = Pd.sp[s];
   if (!S.rel) return;
   if (!(active == nullptr || active[iom] != 0)) return;
   const int nspec = Pd.nspec;
   const int n = (int)blockIdx.z - Pd.nmaxR;
-  const int nstride = 2 * Pd.nmaxR + 1;
   double2 *o = chiR + ((size_t)((NB + blockIdx.z) * nspec + s) * 6) * nom + iom;
   const bool inr = (n >= -S.nmax_rel && n <= S.nmax_rel);
   const bool found = inr && flags[((size_t)iom * nspec + s) * nstride + blockIdx.z] != 0;
@@ -611,7 +613,7 @@ __global__ __launch_bounds__(64) void k_rel_res(const ProbDev *__restrict__ prob
       }
     }
   }
-  if (bad) *err = 8;
+  if (bad) err[pidx != nullptr ? iom : 0] = 8;  // with problem indices: one flag per omega
   cd v[6];
   for (int k = 0; k < 6; ++k) {
     const cd r = block_sum<64>(acc[k], nullptr);
@@ -638,13 +640,16 @@ __global__ __launch_bounds__(64) void k_rel_res(const ProbDev *__restrict__ prob
 // partial slots of chiR: NB harmonic blocks of k_rel_nonres, then 2 nmaxR + 1 of k_rel_res
 int rel_slots(int NB, int nmaxR) { return NB + 2 * nmaxR + 1; }
 
+// nmaxR: the largest relativistic nmax of the batch (of any omega's problem
+// with pidx); flags [nom][nspec][2 nmaxR + 1]; err [1], or [nom] with pidx
 hipError_t launch_rel(hipStream_t st, const ProbDev *prob, int nspec, int nmaxR, int NB, int nom,
                       const double2 *om, const int *active, double2 *chiR, double2 *lowR,
-                      int *flags, int *err) {
+                      int *flags, int *err, const int *pidx) {
   if (nom <= 0) return hipSuccess;
+  const int nstride = 2 * nmaxR + 1;
   hipLaunchKernelGGL(k_rel_nonres, dim3(nom, nspec, NB), dim3(256), 0, st, prob, nom, om, active,
-                     chiR, lowR, flags);
+                     chiR, lowR, flags, pidx, nstride);
   hipLaunchKernelGGL(k_rel_res, dim3(nom, nspec, 2 * nmaxR + 1), dim3(64), 0, st, prob, nom, om,
-                     active, chiR, lowR, flags, NB, err);
+                     active, chiR, lowR, flags, NB, err, pidx, nstride);
   return hipGetLastError();
 }

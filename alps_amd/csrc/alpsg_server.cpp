@@ -46,6 +46,7 @@
 #include "../../include/alpsg.h"
 #include "alpsg_device.h"
 #include "alpsg_osc.h"
+#include "alpsg_relhost.h"
 
 // shared with alpsg_api.cpp / alpsg_tables.hip / alpsg_disp.hip
 int alpsg_internal_fail(int code, const std::string &msg);
@@ -77,6 +78,15 @@ hipError_t launch_assemble(hipStream_t st, const ProbDev *prob, int nspec, int n
                            const double2 *chiB, const double2 *lowA, const double2 *lowB,
                            const double2 *chiR, int RR, const double2 *lowR, double2 *det,
                            double2 *chi, double2 *low, double2 *wave, const int *pidx);
+hipError_t launch_nhds(hipStream_t st, const ProbDev *prob, int nspec, int nom, const double2 *om,
+                       const int *active, double2 *chiA, double2 *lowA, const int *pidx);
+int nhds_tables(double z, int bm_nmax, double bessel_zero, std::vector<double2> &tab);
+hipError_t launch_rel_bessel(hipStream_t st, int npts, int nmax, double zfac, const double *grel,
+                             const double *pbrel, const int *mtab, double *out);
+hipError_t launch_rel(hipStream_t st, const ProbDev *prob, int nspec, int nmaxR, int NB, int nom,
+                      const double2 *om, const int *active, double2 *chiR, double2 *lowR,
+                      int *flags, int *err, const int *pidx);
+int rel_slots(int NB, int nmaxR);
 
 namespace {
 
@@ -100,6 +110,7 @@ constexpr int MTAB_S = 4096;
 constexpr int SW_SHAPE[4] = {ALPSG_SERVE_SWEEP};
 constexpr int SW_NCH = SW_SHAPE[0], SW_W = SW_SHAPE[1], SW_RC = SW_SHAPE[2], SW_K = SW_SHAPE[3];
 constexpr int RS_RB = 8;
+constexpr int REL_NB = 4;  // harmonic blocks of k_rel_nonres in a pass (fixed, as every shape here)
 constexpr size_t ALIGN = 256;
 constexpr int MAX_JOBS = 8192;               // (request, species) jobs per table launch
 constexpr size_t STAGE_CAP = (size_t)1 << 20;  // doubles read back per pass (int_ee, max|J_n|)
@@ -175,6 +186,15 @@ struct Plasma {
   std::vector<size_t> fit_off;
   int maxorder = 0;
   bool general = false;  // a species needs the general Landau part
+  // closed-form (use_bM) species: the &bM_spec parameters (ALPS_NHDS.f90:163-168)
+  std::vector<int> isbm, bm_cold, bm_nmax;
+  std::vector<double> bm_Omega, bm_ell, bm_vth, bm_vd, bm_alpha, bm_bz;
+  // relativistic species: (Gamma, pparbar) tables in the device block
+  std::vector<int> isrel;
+  int ngamma = 0, npparbar = 0;
+  std::vector<double> rel_dg, rel_dpb, rel_ee;
+  std::vector<size_t> o_grel, o_pbrel, o_f0rel, o_dfrel, o_rowlim, o_gampp;
+  bool any_bm = false, any_rel = false;
   // device block: the uploaded prefix (host image img), then df1, df2
   std::vector<char> img;
   size_t o_pperp = 0, o_ppar = 0, o_f0 = 0, o_qs = 0, o_ms = 0, o_fit = 0, o_poly = 0, n_up = 0;
@@ -203,6 +223,9 @@ struct Request {
   std::vector<double> bmax;  // [nspec][ntry]
   size_t off_k = SIZE_MAX, sz_k = 0;
   int nmax[ALPSG_MAXSPEC] = {0};
+  std::vector<double2> bmi_h;  // closed-form species' Bessel-I tables (uploaded at admission)
+  int nmaxR = -1;              // largest nmax of its relativistic species, -1 without one
+  bool failed = false;         // its relativistic evaluation hit alps_error(8)
   OscMachine m;
 };
 
@@ -251,6 +274,12 @@ struct alpsg_server {
   double *stage_h = nullptr, *stage_d = nullptr;  // per-pass readback of int_ee and max|J_n|
   JobBuf jb1, jb2;                                // stage-1 and stage-2 job lists
   ProbDev *newp_h = nullptr, *newp_d = nullptr;   // descriptors activated by a pass
+  double *gtab_d = nullptr;                        // Gamma(m) and Fact(k) of the relativistic Landau term
+  double2 *chiR = nullptr;                         // relativistic partial slots of a pass (grown on demand)
+  size_t chiR_n = 0;
+  int *flags_d = nullptr;                          // relativistic resonance flags of a pass
+  size_t flags_n = 0;
+  int *err_d = nullptr, *err_h = nullptr;          // alps_error(8) flags, one per omega of a pass
   int *newslot_h = nullptr, *newslot_d = nullptr;
   // plasmas and requests
   std::mutex mu;
@@ -304,12 +333,74 @@ int prepare_plasma(const alpsg_plasma_desc *d, Plasma &r) {
   };
   r.pperp.resize((size_t)nspec * (nperp + 1));
   r.ppar.resize((size_t)nspec * (npar + 1));
+  r.isbm.assign(nspec, 0);
+  r.bm_cold.assign(nspec, 0);
+  r.bm_nmax.assign(nspec, 0);
+  r.bm_Omega.assign(nspec, 0.0);
+  r.bm_ell.assign(nspec, 0.0);
+  r.bm_vth.assign(nspec, 0.0);
+  r.bm_vd.assign(nspec, 0.0);
+  r.bm_alpha.assign(nspec, 0.0);
+  r.bm_bz.assign(nspec, 0.0);
+  r.isrel.assign(nspec, 0);
+  r.rel_dg.assign(nspec, 0.0);
+  r.rel_dpb.assign(nspec, 0.0);
+  r.rel_ee.assign(nspec, 0.0);
+  r.o_grel.assign(nspec, 0);
+  r.o_pbrel.assign(nspec, 0);
+  r.o_f0rel.assign(nspec, 0);
+  r.o_dfrel.assign(nspec, 0);
+  r.o_rowlim.assign(nspec, 0);
+  r.o_gampp.assign(nspec, 0);
+  r.any_bm = r.any_rel = false;
   for (int s = 0; s < nspec; ++s) {
-    if ((d->usebM && d->usebM[s]) || (d->relativistic && d->relativistic[s]))
-      return alpsg_internal_fail(ALPSG_E_UNSUPPORTED, "served plasmas take tabulated non-relativistic species");
     if (d->qs[s] == 0.0 || d->ms[s] <= 0.0) return alpsg_internal_fail(ALPSG_E_ARG, "species charge/mass");
     for (int i = 0; i <= nperp; ++i) r.pperp[(size_t)s * (nperp + 1) + i] = PP(s, i, 1, 0);
     for (int j = 0; j <= npar; ++j) r.ppar[(size_t)s * (npar + 1) + j] = PP(s, 2, j, 1);
+    if (d->usebM && d->usebM[s]) {
+      // closed-form species: no table, fits or continuation are read (as alpsg_create)
+      if (!d->bM_nmax || !d->bM_Bessel_zero || !d->bM_beta || !d->bM_alpha || !d->bM_pdrift)
+        return alpsg_internal_fail(ALPSG_E_ARG, "missing bM_spec arrays for a use_bM species");
+      if (d->bM_beta[s] < 0.0 || (d->bM_beta[s] > 0.0 && !(d->bM_alpha[s] > 0.0)) || d->ns[s] <= 0.0)
+        return alpsg_internal_fail(ALPSG_E_ARG, "bM_spec parameters");
+      if (d->bM_beta[s] == 0.0 && !d->kperp_norm)
+        return alpsg_internal_fail(ALPSG_E_UNSUPPORTED, "cold-plasma species need kperp_norm (the reference stops)");
+      r.isbm[s] = 1;
+      r.any_bm = true;
+      r.bm_cold[s] = d->bM_beta[s] == 0.0 ? 1 : 0;
+      r.bm_nmax[s] = d->bM_nmax[s];
+      r.bm_bz[s] = d->bM_Bessel_zero[s];
+      r.bm_Omega[s] = d->qs[s] / d->ms[s];
+      r.bm_ell[s] = std::sqrt(d->ms[s] / (d->ns[s] * d->qs[s] * d->qs[s]));
+      r.bm_vth[s] = std::sqrt(d->bM_beta[s] / (d->ns[s] * d->ms[s]));
+      r.bm_vd[s] = d->bM_pdrift[s] / d->ms[s];
+      r.bm_alpha[s] = d->bM_alpha[s];
+      continue;
+    }
+    if (d->relativistic && d->relativistic[s]) {
+      // Gamma-grid species: Juettner fits in Gamma (types 4, 5) indexed by igamma
+      if (!d->gamma_rel || !d->pparbar_rel || !d->f0_rel || !d->df0_rel)
+        return alpsg_internal_fail(ALPSG_E_ARG, "missing relativistic tables");
+      if (d->ngamma < 4 || d->npparbar < 2 * P + 8)
+        return alpsg_internal_fail(ALPSG_E_ARG, "relativistic grid too small (ngamma >= 4, npparbar >= 2P+8)");
+      if (d->ACmethod[s] != 1)
+        return alpsg_internal_fail(ALPSG_E_UNSUPPORTED, "relativistic species need fit continuation (ACmethod 1)");
+      if (d->n_fits[s] < 1 || d->n_fits[s] > ALPSG_MAXFITS || d->n_fits[s] > d->max_fits)
+        return alpsg_internal_fail(ALPSG_E_ARG, "n_fits out of range");
+      if (!d->fit_type || !d->perp_correction || !d->param_fit)
+        return alpsg_internal_fail(ALPSG_E_ARG, "missing fit arrays");
+      for (int f = 0; f < d->n_fits[s]; ++f) {
+        const int ft = d->fit_type[s + (size_t)nspec * f];
+        if (ft != 4 && ft != 5)
+          return alpsg_internal_fail(ALPSG_E_UNSUPPORTED, "relativistic species need fit_type 4 or 5");
+      }
+      if (d->param_fit_nperp < std::max(nperp, d->ngamma))
+        return alpsg_internal_fail(ALPSG_E_ARG, "param_fit_nperp < max(nperp, ngamma)");
+      r.isrel[s] = 1;
+      r.any_rel = true;
+      r.ngamma = d->ngamma;
+      r.npparbar = d->npparbar;
+    }
     for (int j = 0; j <= npar; ++j)
       for (int i = 0; i <= nperp; ++i)
         if (PP(s, i, j, 0) != r.pperp[(size_t)s * (nperp + 1) + i] ||
@@ -337,6 +428,17 @@ int prepare_plasma(const alpsg_plasma_desc *d, Plasma &r) {
   r.maxorder = 0;
   r.general = false;
   for (int s = 0; s < nspec; ++s) {
+    if (r.isbm[s]) {  // no fit table (the closed form has its own continuation)
+      r.fit_off[s + 1] = r.fit_off[s];
+      continue;
+    }
+    if (r.isrel[s]) {  // fit rows over Gamma, in the Maxwell form of the table
+      r.nfits[s] = d->n_fits[s];
+      for (int f = 0; f < r.nfits[s]; ++f) r.ftype[(size_t)s * ALPSG_MAXFITS + f] = 1;
+      r.funi[s] = 1;
+      r.fit_off[s + 1] = r.fit_off[s] + (size_t)(r.ngamma + 1) * r.nfits[s] * FIT_STRIDE;
+      continue;
+    }
     if (r.acm[s] == 1) {
       const int nf = d->n_fits[s];
       if (nf < 1 || nf > ALPSG_MAXFITS || nf > d->max_fits) return alpsg_internal_fail(ALPSG_E_ARG, "n_fits out of range");
@@ -386,6 +488,28 @@ int prepare_plasma(const alpsg_plasma_desc *d, Plasma &r) {
   r.o_ms = L.add(sizeof(double) * nspec);
   r.o_fit = L.add(sizeof(double) * nfit);
   r.o_poly = L.add(sizeof(double) * npoly);
+  // relativistic species: (Gamma, pparbar) grid, f0, df0, row limits, Gamma at
+  // the (pperp, ppar) points
+  int rel_index[ALPSG_MAXSPEC];
+  int nspec_rel = 0;
+  for (int s = 0; s < nspec; ++s) rel_index[s] = (d->relativistic && d->relativistic[s]) ? nspec_rel++ : -1;
+  std::vector<RelHost> rh(nspec);
+  for (int s = 0; s < nspec; ++s) {
+    if (!r.isrel[s]) continue;
+    const int rc = rel_host_tables(d, s, rel_index[s], nspec_rel, &r.pperp[(size_t)s * (nperp + 1)],
+                                   &r.ppar[(size_t)s * (npar + 1)], rh[s]);
+    if (rc) return rc;
+    const size_t np = rh[s].G.size();
+    r.o_grel[s] = L.add(sizeof(double) * np);
+    r.o_pbrel[s] = L.add(sizeof(double) * np);
+    r.o_f0rel[s] = L.add(sizeof(double) * np);
+    r.o_dfrel[s] = L.add(sizeof(double2) * np);
+    r.o_rowlim[s] = L.add(sizeof(int2) * rh[s].lim.size());
+    r.o_gampp[s] = L.add(sizeof(double) * rh[s].gpp.size());
+    r.rel_dg[s] = rh[s].dg;
+    r.rel_dpb[s] = rh[s].dpb;
+    r.rel_ee[s] = rh[s].ee;
+  }
   r.n_up = L.size;
   r.o_df1 = L.add(sizeof(double) * ng);
   r.o_df2 = L.add(sizeof(double) * ng);
@@ -407,6 +531,18 @@ int prepare_plasma(const alpsg_plasma_desc *d, Plasma &r) {
   double *fitc = H(r.o_fit);
   double *polyc = H(r.o_poly);
   for (int s = 0; s < nspec; ++s) {
+    if (!r.isrel[s]) continue;
+    const RelHost &R = rh[s];
+    std::memcpy(r.img.data() + r.o_grel[s], R.G.data(), sizeof(double) * R.G.size());
+    std::memcpy(r.img.data() + r.o_pbrel[s], R.PB.data(), sizeof(double) * R.PB.size());
+    std::memcpy(r.img.data() + r.o_f0rel[s], R.F.data(), sizeof(double) * R.F.size());
+    std::memcpy(r.img.data() + r.o_dfrel[s], R.DF.data(), sizeof(double2) * R.DF.size());
+    std::memcpy(r.img.data() + r.o_rowlim[s], R.lim.data(), sizeof(int2) * R.lim.size());
+    std::memcpy(r.img.data() + r.o_gampp[s], R.gpp.data(), sizeof(double) * R.gpp.size());
+    rel_fit_rows(d, s, rel_index[s], nspec_rel, &fitc[r.fit_off[s]]);
+  }
+  for (int s = 0; s < nspec; ++s) {
+    if (r.isbm[s] || r.isrel[s]) continue;
     if (r.acm[s] == 1) {
       for (int i = 0; i <= nperp; ++i) {
         const double pv = r.pperp[(size_t)s * (nperp + 1) + i];
@@ -535,6 +671,10 @@ int admit_k(alpsg_server *S, Request &r, int &njobs, int &max_nperp, int &nnew) 
   const int ns = P.nspec, nperp = P.nperp, npar = P.npar;
   if (njobs + ns > MAX_JOBS) return 1;
   for (int s = 0; s < ns; ++s) {
+    if (P.isbm[s]) {  // determine_nmax, ALPS_fns.f90:4012-4013
+      r.nmax[s] = 1;
+      continue;
+    }
     int n = 0;
     for (int t = 1; t <= r.ntry; ++t)
       if (!(r.bmax[(size_t)s * r.ntry + (t - 1)] > P.Bessel_zero)) {
@@ -544,13 +684,37 @@ int admit_k(alpsg_server *S, Request &r, int &njobs, int &max_nperp, int &nnew) 
     if (n == 0) return alpsg_internal_fail(ALPSG_E_ARG, "Bessel cutoff not reached (kperp too large?)");
     r.nmax[s] = n;
   }
+  // closed-form species: the omega-independent part of calc_chi
+  // (ALPS_NHDS.f90:163-185) and its Bessel-I table at this kperp
+  double bm_z[ALPSG_MAXSPEC] = {0}, bm_zp[ALPSG_MAXSPEC] = {0};
+  int bm_nrun[ALPSG_MAXSPEC] = {0};
+  size_t bm_off[ALPSG_MAXSPEC] = {0};
+  r.bmi_h.clear();
+  for (int s = 0; s < ns; ++s) {
+    if (!P.isbm[s] || P.bm_cold[s]) continue;
+    const double a = r.kperp * P.bm_vth[s] / P.bm_Omega[s];
+    bm_z[s] = ((0.5 * a) * a) * P.bm_alpha[s];
+    const double b = P.bm_vth[s] / P.bm_Omega[s];
+    bm_zp[s] = ((0.5 * b) * b) * P.bm_alpha[s];
+    std::vector<double2> tab;
+    bm_nrun[s] = nhds_tables(bm_z[s], P.bm_nmax[s], P.bm_bz[s], tab);
+    bm_off[s] = r.bmi_h.size();
+    r.bmi_h.insert(r.bmi_h.end(), tab.begin(), tab.end());
+  }
+  const int npts_rel = (P.ngamma + 1) * (P.npparbar + 1);
   Layout L;
-  size_t ob[ALPSG_MAXSPEC], oa[ALPSG_MAXSPEC], on[ALPSG_MAXSPEC];
+  size_t ob[ALPSG_MAXSPEC], oa[ALPSG_MAXSPEC], on[ALPSG_MAXSPEC], orb[ALPSG_MAXSPEC], obm[ALPSG_MAXSPEC];
   for (int s = 0; s < ns; ++s) {
     const int nm = r.nmax[s];
+    ob[s] = oa[s] = on[s] = orb[s] = obm[s] = 0;
+    if (P.isbm[s]) {
+      if (!P.bm_cold[s]) obm[s] = L.add(sizeof(double2) * (size_t)(2 * bm_nrun[s] + 1));
+      continue;
+    }
     ob[s] = L.add(sizeof(double) * (size_t)(nm + 3) * (nperp + 1));
     oa[s] = L.add(sizeof(double) * (size_t)(nm + 1) * (nperp + 1) * ATAB_STRIDE);
     on[s] = L.add(sizeof(double) * (size_t)(nm + 1) * (npar - 1) * NODE_STRIDE);
+    if (P.isrel[s]) orb[s] = L.add(sizeof(double) * (size_t)(nm + 2) * npts_rel);
   }
   const size_t off = alloc_evict(S, L.size);
   if (off == SIZE_MAX) return 1;
@@ -569,6 +733,8 @@ int admit_k(alpsg_server *S, Request &r, int &njobs, int &max_nperp, int &nnew) 
   pd.Tlim = P.Tlim;
   pd.kperp = r.kperp;
   pd.kpar = r.kpar;
+  pd.gtab = S->gtab_d;
+  pd.nmaxR = 0;
   for (int s = 0; s < ns; ++s) {
     SpecDev &sd = pd.sp[s];
     const double *pe = &P.pperp[(size_t)s * (nperp + 1)];
@@ -576,6 +742,39 @@ int admit_k(alpsg_server *S, Request &r, int &njobs, int &max_nperp, int &nnew) 
     const int nm = r.nmax[s];
     double *pperp_d = PS(P.o_pperp) + (size_t)s * (nperp + 1);
     double *ppar_d = PS(P.o_ppar) + (size_t)s * (npar + 1);
+    if (P.isbm[s]) {
+      // closed form (ALPS_NHDS.f90:163-168): no table terms, no ee term
+      sd.nperp = nperp;
+      sd.npar = npar;
+      sd.q = P.qs[s];
+      sd.m = P.ms[s];
+      sd.dens = P.ns[s];
+      sd.pperp = pperp_d;
+      sd.ppar = ppar_d;
+      sd.usebm = 1;
+      sd.bm_cold = P.bm_cold[s];
+      sd.bm_Omega = P.bm_Omega[s];
+      sd.bm_ell = P.bm_ell[s];
+      sd.bm_vth = P.bm_vth[s];
+      sd.bm_vd = P.bm_vd[s];
+      sd.bm_alpha = P.bm_alpha[s];
+      sd.int_ee = 0.0;
+      sd.acmethod = 1;
+      sd.nfits = 1;
+      sd.fit_type[0] = 1;
+      sd.fit_uniform = 1;
+      sd.nmax = -1;
+      if (!P.bm_cold[s]) {
+        sd.bm_z = bm_z[s];
+        sd.bm_zp = bm_zp[s];
+        sd.bm_nrun = bm_nrun[s];
+        double2 *t = reinterpret_cast<double2 *>(bk + obm[s]);
+        SCHK(hipMemcpyAsync(t, r.bmi_h.data() + bm_off[s], sizeof(double2) * (size_t)(2 * bm_nrun[s] + 1),
+                            hipMemcpyHostToDevice, S->st));
+        sd.bmi = t;
+      }
+      continue;
+    }
     TabJob &J = S->jb2.jobs_h[njobs++];
     J.pperp = pperp_d;
     J.ppar = ppar_d;
@@ -614,7 +813,7 @@ int admit_k(alpsg_server *S, Request &r, int &njobs, int &max_nperp, int &nnew) 
     sd.atab = PK(oa[s]);
     sd.nodes = PK(on[s]);
     sd.fitc = PS(P.o_fit) + P.fit_off[s];
-    if (P.acm[s] == 2) {
+    if (P.acm[s] == 2 && !P.isrel[s]) {
       sd.poly_kind = 1;
       sd.poly_order = P.porder[s];
       sd.poly_logfit = P.plog[s];
@@ -623,7 +822,36 @@ int admit_k(alpsg_server *S, Request &r, int &njobs, int &max_nperp, int &nnew) 
       sd.poly_norm2 = 5.e-1 * (pa[npar] - pa[0]);
       sd.polyc = PS(P.o_poly) + (size_t)s * (nperp + 1) * (P.maxorder + 1);
     }
+    if (P.isrel[s]) {
+      // the harmonics run in k_rel_nonres / k_rel_res, not in the sweep
+      const double m = P.ms[s], q = P.qs[s], vA = P.vA;
+      sd.rel = 1;
+      sd.ng = P.ngamma;
+      sd.npb = P.npparbar;
+      sd.grel = PS(P.o_grel[s]);
+      sd.pbrel = PS(P.o_pbrel[s]);
+      sd.f0rel = PS(P.o_f0rel[s]);
+      sd.dfrel = reinterpret_cast<const double2 *>(bs + P.o_dfrel[s]);
+      sd.rowlim = reinterpret_cast<const int2 *>(bs + P.o_rowlim[s]);
+      sd.gam_pp = PS(P.o_gampp[s]);
+      sd.df1 = PS(P.o_df1) + (size_t)s * (nperp + 1) * (npar + 1);
+      sd.df2 = PS(P.o_df2) + (size_t)s * (nperp + 1) * (npar + 1);
+      sd.rel_dg = P.rel_dg[s];
+      sd.rel_dpb = P.rel_dpb[s];
+      sd.fit_uniform = 1;
+      sd.nmax_rel = nm;
+      sd.nmax = -1;
+      sd.rel_pref = ((2.0 * 3.141592653589793238462643383279502884) * ((m / vA) * (m / vA) * (m / vA))) *
+                    (q * vA / (r.kpar * m));
+      sd.rel_zfac = r.kperp * m / (vA * q);
+      sd.rel_zbar = P.kperp_norm ? r.kperp * m / (vA * q) : m / (vA * q);
+      double *rb = PK(orb[s]);
+      SCHK(launch_rel_bessel(S->st, npts_rel, nm, sd.rel_zfac, sd.grel, sd.pbrel, S->mtab_d, rb));
+      sd.relbes = rb;
+      pd.nmaxR = std::max(pd.nmaxR, nm);
+    }
   }
+  r.nmaxR = P.any_rel ? pd.nmaxR : -1;
   S->newslot_h[nnew++] = r.slot;
   r.m.start(alpsg_osc::zmk(r.guess.real(), r.guess.imag()), r.rp.numiter, r.rp.D_threshold, r.rp.D_prec);
   r.state = 2;
@@ -720,7 +948,7 @@ int server_pass(alpsg_server *S) {
   // the batch: one pending omega per active solve, grouped by species count
   std::vector<Request *> act;
   for (auto &p : S->live)
-    if (p->state == 2 && !p->m.done) act.push_back(p.get());
+    if (p->state == 2 && !p->m.done && !p->failed) act.push_back(p.get());
   std::stable_sort(act.begin(), act.end(), [](Request *a, Request *b) { return a->pl->nspec < b->pl->nspec; });
   const int nb = (int)act.size();
   const double t0 = now_ms();
@@ -731,27 +959,75 @@ int server_pass(alpsg_server *S) {
     S->om_h[i] = make_double2(w.real(), w.imag());
     S->pidx_h[i] = act[i]->slot;
   }
+  bool pass_rel = false;
   if (nb > 0) {
+    // relativistic partial slots and flags: sized for the largest group first
+    size_t need_r = 0, need_f = 0;
+    for (int i0 = 0, i1 = 0; i0 < nb; i0 = i1) {
+      const int ns = act[i0]->pl->nspec;
+      int relmax = -1;
+      for (i1 = i0; i1 < nb && act[i1]->pl->nspec == ns; ++i1) relmax = std::max(relmax, act[i1]->nmaxR);
+      if (relmax < 0) continue;
+      const size_t n = (size_t)(i1 - i0);
+      need_r = std::max(need_r, n * rel_slots(REL_NB, relmax) * ns * 6);
+      need_f = std::max(need_f, n * ns * (size_t)(2 * relmax + 1));
+      pass_rel = true;
+    }
+    if (need_r > S->chiR_n || need_f > S->flags_n) {
+      SCHK(hipStreamSynchronize(S->st));
+      if (need_r > S->chiR_n) {
+        if (S->chiR) (void)hipFree(S->chiR);
+        S->chiR = nullptr;
+        S->chiR_n = 0;
+        SCHK(hipMalloc((void **)&S->chiR, sizeof(double2) * need_r));
+        S->chiR_n = need_r;
+      }
+      if (need_f > S->flags_n) {
+        if (S->flags_d) (void)hipFree(S->flags_d);
+        S->flags_d = nullptr;
+        S->flags_n = 0;
+        SCHK(hipMalloc((void **)&S->flags_d, sizeof(int) * need_f));
+        S->flags_n = need_f;
+      }
+    }
     SCHK(hipMemcpyAsync(S->om_d, S->om_h, sizeof(double2) * nb, hipMemcpyHostToDevice, S->st));
     SCHK(hipMemcpyAsync(S->pidx_d, S->pidx_h, sizeof(int) * nb, hipMemcpyHostToDevice, S->st));
+    if (pass_rel) SCHK(hipMemsetAsync(S->err_d, 0, sizeof(int) * nb, S->st));
     int i0 = 0;
     while (i0 < nb) {
       int i1 = i0;
       const int ns = act[i0]->pl->nspec;
-      bool gen = false;
-      for (; i1 < nb && act[i1]->pl->nspec == ns; ++i1)
+      bool gen = false, bm = false;
+      int relmax = -1;
+      for (; i1 < nb && act[i1]->pl->nspec == ns; ++i1) {
         if (act[i1]->pl->general) gen = true;
+        if (act[i1]->pl->any_bm) bm = true;
+        relmax = std::max(relmax, act[i1]->nmaxR);
+      }
       const int n = i1 - i0;
       SCHK(launch_sweep_req(S->st, SW_NCH, SW_W, SW_RC, SW_K, S->probs_d, n, S->om_d + i0,
                             S->chiA, S->pidx_d + i0, false));
       SCHK(launch_resonant(S->st, 64, RS_RB, S->probs_d, ns, n, S->om_d + i0, nullptr, S->chiB,
                            nullptr, gen, S->pidx_d + i0));
+      // closed-form species: partial slot 0 of their chi (the sweep left it zero)
+      if (bm)
+        SCHK(launch_nhds(S->st, S->probs_d, ns, n, S->om_d + i0, nullptr, S->chiA, nullptr,
+                         S->pidx_d + i0));
+      // relativistic species: REL_NB + 2 relmax + 1 partial slots; an omega whose
+      // problem has fewer harmonics gets zeros in the slots past its own
+      int RR = 0;
+      if (relmax >= 0) {
+        RR = rel_slots(REL_NB, relmax);
+        SCHK(launch_rel(S->st, S->probs_d, ns, relmax, REL_NB, n, S->om_d + i0, nullptr, S->chiR,
+                        nullptr, S->flags_d, S->err_d + i0, S->pidx_d + i0));
+      }
       SCHK(launch_assemble(S->st, S->probs_d, ns, n, SW_W * SW_RC, RS_RB, S->om_d + i0, nullptr,
-                           S->chiA, S->chiB, nullptr, nullptr, nullptr, 0, nullptr, S->det_d + i0,
-                           nullptr, nullptr, nullptr, S->pidx_d + i0));
+                           S->chiA, S->chiB, nullptr, nullptr, RR ? S->chiR : nullptr, RR, nullptr,
+                           S->det_d + i0, nullptr, nullptr, nullptr, S->pidx_d + i0));
       i0 = i1;
     }
     SCHK(hipMemcpyAsync(S->det_h, S->det_d, sizeof(double2) * nb, hipMemcpyDeviceToHost, S->st));
+    if (pass_rel) SCHK(hipMemcpyAsync(S->err_h, S->err_d, sizeof(int) * nb, hipMemcpyDeviceToHost, S->st));
   }
   const double t1 = now_ms();
   SCHK(hipStreamSynchronize(S->st));
@@ -763,6 +1039,10 @@ int server_pass(alpsg_server *S) {
   // the readback of this pass: int_ee of the uploaded plasmas, max|J_n| of the admissions
   for (Plasma *P : uploaded_now) {
     P->int_ee.assign(S->stage_h + P->ee_stage, S->stage_h + P->ee_stage + P->nspec);
+    for (int s = 0; s < P->nspec; ++s) {
+      if (P->isbm[s]) P->int_ee[s] = 0.0;            // the closed form has no ee term
+      if (P->isrel[s]) P->int_ee[s] = P->rel_ee[s];  // int_ee_rel on the (Gamma, pparbar) grid
+    }
     P->ee_state = 2;
   }
   for (Request *r : sizing_now)
@@ -814,9 +1094,21 @@ int server_pass(alpsg_server *S) {
   }
   S->t_admit += now_ms() - t3;
   // feed the results; retire finished solves
-  for (int i = 0; i < nb; ++i) act[i]->m.feed(alpsg_osc::zmk(S->det_h[i].x, S->det_h[i].y));
+  for (int i = 0; i < nb; ++i) {
+    if (pass_rel && S->err_h[i] != 0) {  // alps_error(8): the solve stops with an error
+      act[i]->failed = true;
+      continue;
+    }
+    act[i]->m.feed(alpsg_osc::zmk(S->det_h[i].x, S->det_h[i].y));
+  }
   for (auto it = S->live.begin(); it != S->live.end();) {
     Request &r = **it;
+    if (r.state == 2 && r.failed) {
+      done.push_back(Result{r.tag, cplx(0.0, 0.0), r.m.iter, ALPSG_E_ARG, now_ms() - r.t_submit});
+      release(S, r);
+      it = S->live.erase(it);
+      continue;
+    }
     if (r.state == 2 && r.m.done) {
       done.push_back(Result{r.tag, cplx(r.m.result().re, r.m.result().im), r.m.iter, ALPSG_OK,
                             now_ms() - r.t_submit});
@@ -939,6 +1231,14 @@ int alpsg_server_create(double hbm_bytes, int max_slots, alpsg_server **out) {
   SCHK(hipMalloc((void **)&S->newp_d, sizeof(ProbDev) * mb));
   SCHK(hipHostMalloc((void **)&S->newslot_h, sizeof(int) * mb, hipHostMallocDefault));
   SCHK(hipMalloc((void **)&S->newslot_d, sizeof(int) * mb));
+  SCHK(hipMalloc((void **)&S->err_d, sizeof(int) * mb));
+  SCHK(hipHostMalloc((void **)&S->err_h, sizeof(int) * mb, hipHostMallocDefault));
+  {
+    std::vector<double> gt;
+    rel_gamma_table(gt);
+    SCHK(hipMalloc((void **)&S->gtab_d, sizeof(double) * gt.size()));
+    SCHK(hipMemcpy(S->gtab_d, gt.data(), sizeof(double) * gt.size(), hipMemcpyHostToDevice));
+  }
   S->th = std::thread(service_loop, S.get());
   *out = S.release();
   return ALPSG_OK;
@@ -1089,6 +1389,11 @@ int alpsg_server_destroy(alpsg_server *S) {
   if (S->newp_d) (void)hipFree(S->newp_d);
   if (S->newslot_h) (void)hipHostFree(S->newslot_h);
   if (S->newslot_d) (void)hipFree(S->newslot_d);
+  if (S->err_d) (void)hipFree(S->err_d);
+  if (S->err_h) (void)hipHostFree(S->err_h);
+  if (S->gtab_d) (void)hipFree(S->gtab_d);
+  if (S->chiR) (void)hipFree(S->chiR);
+  if (S->flags_d) (void)hipFree(S->flags_d);
   if (S->st) (void)hipStreamDestroy(S->st);
   delete S;
   return ALPSG_OK;

@@ -150,3 +150,36 @@ def test_registered_plasmas_give_the_same_roots_and_survive_eviction():
             time.sleep(0.01)
         assert S.stats()["arena_used_bytes"] == 0
         S.close()
+
+
+def test_relativistic_and_closed_form_plasmas_are_served():
+    """Plasmas with a relativistic species (the Gamma-grid kernels with a
+    problem index per omega) and with closed-form use_bM species (hot and
+    cold NHDS) are served: roots match the oracle's, are bit-identical alone
+    and in a mix with tabulated plasmas, and a request whose relativistic
+    evaluation hits alps_error(8) fails alone without disturbing the others."""
+    rel = _tight(cases.rel_mixed())
+    bmh = _tight(cases.bm_mix(cold=False))
+    bmc = _tight(cases.bm_mix(cold=True))
+    extra = [(rel, 0.3, 0.2, 0.2 - 0.01j), (bmh, 0.5, 0.2, 0.3 - 0.01j),
+             (bmc, 0.5, 0.2, 0.5 - 0.05j), (bmh, 0.5, 0.2, 0.5 - 0.05j)]
+    alone = _alone(extra)
+    for (p, kp, kz, g), r in zip(extra, alone):
+        O = Oracle(p)
+        O.set_k(kp, kz)
+        w, _ = O.secant_osc(g)
+        assert abs(r[1] - w) <= ROOT_TOL * abs(w), (g, r[1], w)
+    base = _requests()
+    bad = _tight(cases.rel_pair(ngamma=30, npparbar=30))
+    S = _server()
+    for i, (p, kp, kz, g) in enumerate(base + extra):
+        S.submit(p, kp, kz, g, p.roots, tag=i)
+    S.submit(bad, 1e-3, 0.1, 0.0627 - 4.662e-8j, bad.roots, tag=999)
+    got = S.drain(len(base) + len(extra) + 1)
+    S.close()
+    assert got[999][3] == 1  # ALPSG_E_ARG: the reference stops with alps_error(8)
+    for i in range(len(extra)):
+        r = got[len(base) + i]
+        assert r[3] == 0 and r[1] == alone[i][1], (i, r, alone[i])
+    for i in range(len(base)):
+        assert got[i][3] == 0
