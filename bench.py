@@ -145,6 +145,98 @@ def serve_requests(n, n_variants, seed=7):
                   for i in range(n)]
 
 
+def serve_rel_variant(tke, rp, nperp=200, npar=400, ngamma=100, npparbar=200):
+    """Relativistic electrons (Juettner, kT_e / m_e c^2 = tke, the (Gamma,
+    pparbar) tables of derivative_f0_rel) and cold-ish protons (T_p = 0.01
+    T_e), on a 200 x 400 (pperp, ppar) grid, electron-normalised: masses m_e = 1,
+    m_p = 1836, v_A / c = 0.3."""
+    import dataclasses
+    from alps_amd.plasma import bimaxwellian_plasma, juttner_plasma
+    vA = 0.3
+    tau = 2.0 * tke / (vA * vA)  # kT/mc^2 = vA^2 beta tau / (2 m) with beta = 1, m = 1
+    r = juttner_plasma(ms=[1.0], qs=[-1.0], ns=[1.0], taus=[tau], beta=1.0, vA=vA, nperp=nperp,
+                       npar=npar, maxP=5.0, ngamma=ngamma, npparbar=npparbar, analytic_rel=True,
+                       positions_principal=3, n_resonance_interval=100, Tlim=0.01,
+                       Bessel_zero=1e-45)
+    b = bimaxwellian_plasma(ms=[1836.0], qs=[1.0], ns=[1.0], taus=[0.01 * tau], alphs=[1.0],
+                            ps=[0.0], beta=1.0, vA=vA, nperp=nperp, npar=npar, maxP=5.0)
+    npf = r.param_fit.shape[1]
+    pf = np.zeros((2, npf, 5, 1))
+    pf[0] = r.param_fit[0]
+    pf[1, : b.param_fit.shape[1]] = b.param_fit[0]
+    return dataclasses.replace(
+        r, pp=np.concatenate([r.pp, b.pp]), f0=np.concatenate([r.f0, b.f0]), ns=[1.0, 1.0],
+        qs=[-1.0, 1.0], ms=[1.0, 1836.0], fit_type=np.array([[4], [1]]),
+        perp_correction=np.array([[r.perp_correction[0, 0]], [b.perp_correction[0, 0]]]),
+        param_fit=pf, relativistic=[1, 0], ACmethod=None, n_fits=None, poly_kind=None,
+        poly_order=None, logfit=None, poly_log_max=None, poly_fit_coeffs=None, usebM=None,
+        bM_nmax=None, bM_Bessel_zero=None, bM_beta=None, bM_alpha=None, bM_pdrift=None, roots=rp)
+
+
+def run_serve_rel(args, n_req=256, seed=5):
+    """Served kpar-scan root requests on relativistic plasmas (kT_e / m_e c^2 in
+    {0.1, 0.3, 1.0}): kpar log-uniform in [0.05, 2], kperp = 0.1 kpar, guesses
+    near the whistler branch; roots/s and p50/p99 latency, and the plain path on
+    a sample."""
+    from alps_amd.engine import Engine, RootServer
+    from alps_amd.plasma import RootParams
+    rp = RootParams(secant_method=2, numiter=50, D_threshold=1e-12, D_prec=1e-5, D_gap=1e-5)
+    pool = [serve_rel_variant(t, rp) for t in (0.1, 0.3, 1.0)]
+    rng = np.random.default_rng(seed)
+    kz = np.exp(rng.uniform(np.log(0.05), np.log(2.0), n_req))
+    var = rng.integers(0, len(pool), n_req)
+    guess = (0.5 * kz * kz) * (1.0 + rng.uniform(-0.05, 0.05, n_req)) - 1e-3j * kz
+    S = RootServer(hbm_bytes=args.hbm_gb * 1e9, max_slots=min(args.slots, 1024))
+    ids = [S.add_plasma(p) for p in pool]
+    for v in range(len(pool)):  # warm-up: tables and code paths
+        S.submit(ids[v], 0.05, 0.5, 0.125 - 5e-4j, rp, tag=-1 - v)
+    S.drain(len(pool))
+    t0 = time.perf_counter()
+    S.submit_batch(np.array([ids[v] for v in var], dtype=np.int32), 0.1 * kz, kz, guess, rp,
+                   np.arange(n_req))
+    got = S.drain(n_req)
+    wall = time.perf_counter() - t0
+    S.close()
+    lat = np.array([got[i][4] for i in range(n_req)])
+    status = np.array([got[i][3] for i in range(n_req)])
+    iters = np.array([got[i][2] for i in range(n_req)])
+    served = np.array([got[i][1] for i in range(n_req)])
+    idx = np.linspace(0, n_req - 1, 8).astype(int)
+    engines, plain_t, dev, n_cmp = {}, 0.0, 0.0, 0
+    for i in idx:
+        v = int(var[i])
+        E = engines.setdefault(v, Engine(pool[v]))
+        tp = time.perf_counter()
+        try:
+            E.set_k(0.1 * kz[i], kz[i])
+            r, it = E.secant_osc(guess[i])
+        except Exception:
+            r, it = None, rp.numiter
+        plain_t += time.perf_counter() - tp
+        # roots that converged on both paths (|D| < D_threshold fixes them only
+        # to |D| / |dD/dw|, so the two paths' rounding shows at that level)
+        if r is not None and status[i] == 0 and it < rp.numiter and iters[i] < rp.numiter:
+            dev = max(dev, abs(r - served[i]) / max(abs(r), 1e-300))
+            n_cmp += 1
+    return {
+        "workload": "relativistic kpar-scan requests: Juettner electrons (kT/mc^2 0.1, 0.3, 1.0) "
+                    "and cold protons, 200 x 400 (pperp, ppar), 100 x 200 (Gamma, pparbar), "
+                    "kpar in [0.05, 2], all submitted at once",
+        "requests": n_req,
+        "wall_s": wall,
+        "roots_per_sec": int(np.sum(status == 0)) / wall,
+        "status_ok": int(np.sum(status == 0)),
+        "converged": int(np.sum((status == 0) & (iters < rp.numiter))),
+        "latency_p50_ms": float(np.percentile(lat, 50)),
+        "latency_p99_ms": float(np.percentile(lat, 99)),
+        "mean_iterations": float(iters.mean()),
+        "plain_roots_per_sec": len(idx) / plain_t,
+        "plain_sample": int(len(idx)),
+        "max_rel_dev_served_vs_plain_converged": dev,
+        "compared": n_cmp,
+    }
+
+
 def run_serve(args):
     """Poisson stream of root requests through the continuous-batching server;
     the same requests (a sample) on the plain one-root-per-launch path and on
@@ -420,6 +512,8 @@ def main():
     if not args.no_serve:
         # served root solves beside the map line: 4096 independent requests
         out["serve"] = run_serve(args)
+        # and relativistic kpar-scan requests through the same server
+        out["serve_relativistic"] = run_serve_rel(args)
     print(json.dumps(out))
 
 
