@@ -56,14 +56,25 @@ MAP = dict(omi=0.01, omf=2.0, gami=-0.5, gamf=0.05)
 KPERP = KPAR = 0.5
 
 
-# HBM traffic of the map-batch k_sweep launch by its (lanes per omega, wavefronts,
-# workgroups per omega block) shape, from rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE passes on this workload, each scaled by the counter's reading for
-# a 256 MiB device copy (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).
-SWEEP_TRAFFIC_BYTES = {(1, 2, 16): 16054.8125 * 1024 * 2.0 + 393216.0 * 1024,
-                       (1, 4, 3): 18057.3125 * 1024 * 2.0 + 147456.0 * 1024,
-                       (1, 4, 6): 18507.875 * 1024 * 2.0 + 294912.0 * 1024,
-                       (1, 4, 4): 17959.1875 * 1024 * 2.0 + 196608.0 * 1024}
+# HBM traffic of the map-batch k_sweep launch by its shape.
+# FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of k_sweep on the 256 x 256 map
+# batch for every one-lane launch shape (lanes, wavefronts, workgroups per omega
+# block), one counter pass each over `bench.py --pmc-shapes`; a 262144 KiB copy
+# reads FETCH_SIZE 131084-131089 and WRITE_SIZE 262144, so FETCH_SIZE counts
+# half the bytes fetched and WRITE_SIZE all bytes written.
+_SWEEP_PMC_KIB = {
+    (1, 1, 1): (14132.9, 12288.0), (1, 1, 2): (14178.4, 24576.0), (1, 1, 3): (15959.4, 36864.0),
+    (1, 1, 4): (15359.6, 49152.0), (1, 1, 6): (14827.9, 73728.0), (1, 1, 8): (14454.3, 98304.0),
+    (1, 1, 12): (14292.4, 147456.0), (1, 1, 16): (14362.5, 196608.0), (1, 2, 1): (14133.6, 24576.0),
+    (1, 2, 2): (17444.4, 49152.0), (1, 2, 3): (16333.8, 73728.0), (1, 2, 4): (15886.1, 98304.0),
+    (1, 2, 6): (15101.8, 147456.0), (1, 2, 8): (15375.6, 196608.0), (1, 2, 12): (15906.6, 294912.0),
+    (1, 2, 16): (16496.4, 393216.0), (1, 4, 1): (21851.3, 49152.0), (1, 4, 2): (20066.4, 98304.0),
+    (1, 4, 3): (18165.4, 147456.0), (1, 4, 4): (18444.4, 196608.0), (1, 4, 6): (18961.8, 294912.0),
+    (1, 4, 8): (19600.4, 393216.0), (1, 4, 12): (21761.7, 589824.0), (1, 4, 16): (23851.8, 786432.0),
+}
+FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
+SWEEP_TRAFFIC_BYTES = {k: (f * FETCH_SCALE + w * WRITE_SCALE) * 1024.0
+                       for k, (f, w) in _SWEEP_PMC_KIB.items()}
 
 
 def flops_per_eval(nmax, npar):
@@ -375,6 +386,8 @@ def main():
     ap.add_argument("--shapes", default=None,
                     help="JSON {batch size: [5 ints]} pinning launch shapes instead of autotuning "
                          "(e.g. the launch_shapes of an earlier run, for a profiler pass)")
+    ap.add_argument("--pmc-shapes", action="store_true",
+                    help="run one map batch per one-lane sweep shape (for counter passes)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (torch) and exit: the "
                          "known-size kernel that scales the profiler's FETCH_SIZE/WRITE_SIZE")
@@ -413,6 +426,22 @@ def main():
     if args.workload == "serve":
         sv = run_serve(args)
         print(json.dumps({"workload": "serve", **sv}))
+        return
+
+    if args.pmc_shapes:
+        # one map batch per one-lane sweep shape the autotuner can pick, for the
+        # FETCH_SIZE / WRITE_SIZE counter passes (dispatches tell the shape by
+        # their workgroup size and grid y = workgroups per omega block)
+        from alps_amd.engine import Engine
+        E = Engine(make_plasma())
+        E.set_k(KPERP, KPAR)
+        shapes = []
+        for w in (1, 2, 4):
+            for rc in (1, 2, 3, 4, 6, 8, 12, 16):
+                E.set_shape(args.nr * args.ni, (1, w, rc, 4, 4))
+                E.map_search(MAP["omi"], MAP["omf"], MAP["gami"], MAP["gamf"], args.nr, args.ni)
+                shapes.append([1, w, rc])
+        print(json.dumps({"pmc_shapes": shapes}))
         return
 
     from alps_amd.engine import Engine
@@ -498,7 +527,8 @@ def main():
         "roofline": {"bound": "mfma", "compute": "fp64 (VALU)", "achieved": achieved,
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS,
-                     "traffic": SWEEP_TRAFFIC_BYTES.get(tuple(shapes.get(args.nr * args.ni, (0, 0, 0))[:3])),
+                     "traffic": (SWEEP_TRAFFIC_BYTES.get(tuple(shapes.get(args.nr * args.ni, (0,) * 5)[:3]))
+                                 if (args.nr, args.ni) == (256, 256) else None),
                      "flops_per_launch": flops_launch,
                      "kernel": "k_sweep (map batch, %d omegas)" % (args.nr * args.ni),
                      "kernel_ms": float(split[0])},
