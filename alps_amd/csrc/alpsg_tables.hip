@@ -412,9 +412,31 @@ __global__ void k_atab_jobs(const TabJob *__restrict__ jobs, const int *__restri
 
 // moments_block for MB consecutive harmonics n0 .. n0+MB-1 (those above nmax
 // are computed on n = nmax and not stored): every df0 load serves MB
-// harmonics, and the a-values, uniform over the block, are scalar loads.
+// harmonics.  The weighted a-values of the block's harmonics and the pperp
+// values are staged in LDS MCH rows at a time (one row per lane, its loads all
+// in flight together), so the row loop reads them as LDS broadcasts instead of
+// a chain of dependent scalar loads per row.  Sums run over iperp in order, as
+// in moments_block.
 constexpr int MB = 8;
-typedef const __attribute__((address_space(4))) double *cdptr_t;
+constexpr int MCH = 64;
+constexpr int MR = 4;
+typedef const __attribute__((address_space(1))) double *gdptr_t;
+
+__device__ __forceinline__ void moments_row(double (&g)[MB][3], double (&h)[MB][3],
+                                            const double *la_row, double pe, double d1, double d2,
+                                            double kom, double pa) {
+  const double A = kom * (pe * d2 - pa * d1);
+#pragma unroll
+  for (int u = 0; u < MB; ++u) {
+    const double a1 = la_row[3 * u + 0], a2 = la_row[3 * u + 1], a3 = la_row[3 * u + 2];
+    g[u][0] += a1 * d1;
+    h[u][0] += a1 * A;
+    g[u][1] += a2 * d1;
+    h[u][1] += a2 * A;
+    g[u][2] += a3 * d1;
+    h[u][2] += a3 * A;
+  }
+}
 
 __device__ __forceinline__ void moments_block_mb(int bx, int n0, int nperp, int npar, int nmax,
                                                  double kpar, double m,
@@ -424,34 +446,56 @@ __device__ __forceinline__ void moments_block_mb(int bx, int n0, int nperp, int 
                                                  const double *__restrict__ df2,
                                                  const double *__restrict__ atab,
                                                  double *__restrict__ nodes) {
+  __shared__ double la[MCH][MB * 3];
+  __shared__ double lpe[MCH];
   const int ipar = 1 + bx * 64 + threadIdx.x;
   const int ip = ipar <= npar - 1 ? ipar : npar - 1;
   const int W = nperp + 1;
   const double pa = ppar[ip];
   const double kom = kpar / m;
-  const cdptr_t at = (cdptr_t)atab;
-  const cdptr_t pe = (cdptr_t)pperp;
+  // global (not flat) loads: their waits do not hold the LDS reads
+  const gdptr_t f1 = (gdptr_t)df1, f2 = (gdptr_t)df2;
   double g[MB][3], h[MB][3];
 #pragma unroll
   for (int u = 0; u < MB; ++u)
 #pragma unroll
     for (int t = 0; t < 3; ++t) g[u][t] = h[u][t] = 0.0;
-  for (int iperp = 1; iperp <= nperp - 1; ++iperp) {
-    const size_t o = (size_t)iperp * (npar + 1) + ip;
-    const double d1 = df1[o], d2 = df2[o];
-    const double A = kom * (pe[iperp] * d2 - pa * d1);
-    const double wperp = (iperp == nperp - 1) ? 1.0 : 2.0;
+  for (int c0 = 1; c0 <= nperp - 1; c0 += MCH) {
+    const int cn = min(MCH, nperp - c0);  // rows c0 .. c0 + cn - 1 <= nperp - 1
+    __syncthreads();                       // the previous chunk is consumed
+    if ((int)threadIdx.x < cn) {
+      const int iperp = c0 + threadIdx.x;
+      const double wperp = (iperp == nperp - 1) ? 1.0 : 2.0;
+      double v[MB * 3];
 #pragma unroll
-    for (int u = 0; u < MB; ++u) {
-      const int n = min(n0 + u, nmax);
-      const cdptr_t a = at + ((size_t)n * W + iperp) * ATAB_STRIDE;
-      const double a1 = wperp * a[0], a2 = wperp * a[1], a3 = wperp * a[2];
-      g[u][0] += a1 * d1;
-      h[u][0] += a1 * A;
-      g[u][1] += a2 * d1;
-      h[u][1] += a2 * A;
-      g[u][2] += a3 * d1;
-      h[u][2] += a3 * A;
+      for (int u = 0; u < MB; ++u) {
+        const double *a = atab + ((size_t)min(n0 + u, nmax) * W + iperp) * ATAB_STRIDE;
+        v[3 * u + 0] = a[0];
+        v[3 * u + 1] = a[1];
+        v[3 * u + 2] = a[2];
+      }
+#pragma unroll
+      for (int k = 0; k < MB * 3; ++k) la[threadIdx.x][k] = wperp * v[k];
+      lpe[threadIdx.x] = pperp[iperp];
+    }
+    __syncthreads();
+    // rows in groups of MR, the group's df loads issued before its sums, then
+    // the remaining rows one by one
+    int t0 = 0;
+    for (; t0 + MR <= cn; t0 += MR) {
+      double d1[MR], d2[MR];
+#pragma unroll
+      for (int r = 0; r < MR; ++r) {
+        const size_t o = (size_t)(c0 + t0 + r) * (npar + 1) + ip;
+        d1[r] = f1[o];
+        d2[r] = f2[o];
+      }
+#pragma unroll
+      for (int r = 0; r < MR; ++r) moments_row(g, h, la[t0 + r], lpe[t0 + r], d1[r], d2[r], kom, pa);
+    }
+    for (; t0 < cn; ++t0) {
+      const size_t o = (size_t)(c0 + t0) * (npar + 1) + ip;
+      moments_row(g, h, la[t0], lpe[t0], f1[o], f2[o], kom, pa);
     }
   }
   if (ipar > npar - 1) return;
