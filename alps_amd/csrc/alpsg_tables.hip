@@ -176,6 +176,10 @@ __global__ void k_int_ee(int nperp, int npar, const double *__restrict__ pperp,
 }
 
 // max_iperp |J_n(kperp*pperp/q)| for one n (one 64-lane block)
+// One block of BMAX_T threads per n: every thread takes at most a few pperp
+// points, so a block's time is that of one or two Miller recurrences.
+constexpr int BMAX_T = 256;
+
 __device__ __forceinline__ void bessel_max_block(int n, int nperp, double kperp,
                                                  const double *__restrict__ pe, double q,
                                                  const int *__restrict__ mtab, double *out,
@@ -198,7 +202,7 @@ __device__ __forceinline__ void bessel_max_block(int n, int nperp, double kperp,
 __global__ void k_bessel_max(int nperp, int ntry, double kperp, const double *__restrict__ pperp,
                              const double *__restrict__ qs, const int *__restrict__ mtab,
                              double *__restrict__ out) {
-  __shared__ double red[64];
+  __shared__ double red[BMAX_T];
   int n = blockIdx.x + 1;
   int s = blockIdx.y;
   bessel_max_block(n, nperp, kperp, pperp + (size_t)s * (nperp + 1), qs[s], mtab,
@@ -383,7 +387,7 @@ __device__ __forceinline__ int job_of(const int *__restrict__ start, int njobs, 
 
 __global__ void k_bessel_max_jobs(const TabJob *__restrict__ jobs, const int *__restrict__ start,
                                   int njobs, const int *__restrict__ mtab) {
-  __shared__ double red[64];
+  __shared__ double red[BMAX_T];
   const int j = job_of(start, njobs, blockIdx.x);
   const TabJob &J = jobs[j];
   const int n = 1 + (blockIdx.x - start[j]);
@@ -548,7 +552,7 @@ hipError_t launch_int_ee(hipStream_t st, int nspec, int nperp, int npar, const d
 hipError_t launch_bessel_max(hipStream_t st, int nspec, int nperp, int ntry, double kperp,
                              const double *pperp, const double *qs, const int *mtab,
                              double *out) {
-  hipLaunchKernelGGL(k_bessel_max, dim3(ntry, nspec), dim3(64), 0, st, nperp, ntry, kperp, pperp,
+  hipLaunchKernelGGL(k_bessel_max, dim3(ntry, nspec), dim3(BMAX_T), 0, st, nperp, ntry, kperp, pperp,
                      qs, mtab, out);
   return hipGetLastError();
 }
@@ -602,7 +606,7 @@ int jobs_blocks_moments(const TabJob &J) {
 hipError_t launch_bessel_max_jobs(hipStream_t st, const TabJob *jobs, const int *start, int njobs,
                                   int total, const int *mtab) {
   if (njobs <= 0 || total <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_bessel_max_jobs, dim3(total), dim3(64), 0, st, jobs, start, njobs, mtab);
+  hipLaunchKernelGGL(k_bessel_max_jobs, dim3(total), dim3(BMAX_T), 0, st, jobs, start, njobs, mtab);
   return hipGetLastError();
 }
 
