@@ -253,8 +253,9 @@ int alpsg_last_kernel_split(alpsg_handle h, double *ms3);
    disp() per pass, with solves of uneven grids and nmax in shared launches.
    A request's result does not depend on the other requests in flight.
 
-   Served species: tabulated f0, non-relativistic, not use_bM.  rp must ask
-   for secant_osc (secant_method 2).
+   Served species: tabulated f0 (non-relativistic or relativistic, as for
+   alpsg_create) and closed-form use_bM species.  rp must ask for secant_osc
+   (secant_method 2).
 
    alpsg_server_add_plasma copies a plasma description and returns its id;
    its device tables are built on first use and kept while the arena has room
@@ -268,8 +269,10 @@ int alpsg_last_kernel_split(alpsg_handle h, double *ms3);
 
    alpsg_server_poll returns finished solves (waits for one when wait != 0 and
    any are outstanding): tags [max_results], roots [2*max_results], iters,
-   status (ALPSG_OK, or ALPSG_E_ARG for a request that cannot fit the arena
-   or whose Bessel cutoff is not reached) and latency_ms (submit to result);
+   status (ALPSG_OK, or ALPSG_E_ARG for a request that cannot fit the arena,
+   whose Bessel cutoff is not reached, or whose relativistic principal-value
+   window leaves the subluminal cone, the reference's alps_error(8)) and
+   latency_ms (submit to result);
    any may be NULL.  alpsg_server_stats gives counts[10] = {submitted,
    finished, passes, disp evaluations, admitted, waiting, arena bytes, arena
    bytes in use, plasma uploads, plasma evictions} and times_ms[4] = the
