@@ -50,7 +50,8 @@ hipError_t launch_nhds(hipStream_t st, const ProbDev *prob, int nspec, int nom, 
                        const int *active, double2 *chiA, double2 *lowA, const int *pidx = nullptr);
 int nhds_tables(double z, int bm_nmax, double bessel_zero, std::vector<double2> &tab);
 hipError_t launch_split_tables(hipStream_t st, int npar, int nmax, double kpar, double q, double m,
-                               const double *nodes, double *crec, double *rsum);
+                               const double *ppar, const double *nodes, double *crec,
+                               double *rsum);
 hipError_t launch_rel_bessel(hipStream_t st, int npts, int nmax, double zfac, const double *grel,
                              const double *pbrel, const int *mtab, double *out);
 hipError_t launch_map_post(hipStream_t st, int nr, int ni, const double2 *cal, double2 *calo,
@@ -1858,8 +1859,9 @@ int alpsg_set_k(alpsg_handle h, double kperp, double kpar, const int *nmax_in, i
     if (!h->relsp[s]) {
       HIPCHK(h->crec[s].ensure((size_t)(nm + 1) * (npar - 1) * CREC_STRIDE));
       HIPCHK(h->rsum[s].ensure((size_t)(nm + 1) * RSUM_STRIDE));
-      HIPCHK(launch_split_tables(h->st, npar, nm, kpar, h->qs[s], h->ms[s], h->nodes[s].p,
-                                 h->crec[s].p, h->rsum[s].p));
+      HIPCHK(launch_split_tables(h->st, npar, nm, kpar, h->qs[s], h->ms[s],
+                                 h->ppar.p + (size_t)s * (npar + 1), h->nodes[s].p, h->crec[s].p,
+                                 h->rsum[s].p));
       S.crec = h->crec[s].p;
       S.rsum = h->rsum[s].p;
     }

@@ -8,7 +8,7 @@
 
 #define ALPSG_DMAXSPEC 8
 #define ALPSG_DMAXFITS 4
-#define NODE_STRIDE 8   // doubles per (n, ipar) node record: G1 H1 G2 H2 G3 H3 ppar ppar^2
+#define NODE_STRIDE 6   // doubles per (n, ipar) node record: G1 H1 G2 H2 G3 H3 (ppar from the grid)
 #define ATAB_STRIDE 4   // doubles per (n, iperp) entry: J^2, pperp^2 J'^2, pperp J J', J
 #define FIT_STRIDE 8    // doubles per (iperp, fit) entry of the analytic-continuation table
 #define CREC_STRIDE 13  // doubles per (n, ipar) record of the split-form sweep table

@@ -124,17 +124,20 @@ __device__ __forceinline__ int ipar_close(const SpecDev &S, double x) {
   return best;
 }
 
-// node products [U1, p U1, p^2 U1, U2, U3, p U3], U_a = om*G_a + H_a
-__device__ __forceinline__ void node6(const double *r, double omr, double omi, cd u[6]) {
+// node products [U1, p U1, p^2 U1, U2, U3, p U3], U_a = om*G_a + H_a, at the
+// node with ppar p
+__device__ __forceinline__ void node6(const double *r, double p, double omr, double omi,
+                                      cd u[6]) {
   const cd U1 = mk(omr * r[0] + r[1], omi * r[0]);
   const cd U2 = mk(omr * r[2] + r[3], omi * r[2]);
   const cd U3 = mk(omr * r[4] + r[5], omi * r[4]);
+  const double p2 = p * p;
   u[0] = U1;
-  u[1] = r[6] * U1;
-  u[2] = r[7] * U1;
+  u[1] = p * U1;
+  u[2] = p2 * U1;
   u[3] = U2;
   u[4] = U3;
-  u[5] = r[6] * U3;
+  u[5] = p * U3;
 }
 
 // pperp-summed funct_g at real ppar x (without the -q/kpar factor), summed into
@@ -143,9 +146,9 @@ __device__ __forceinline__ void interp6_acc(const SpecDev &S, const double *__re
                                             double x, double omr, double omi, cd wt, cd acc[6]) {
   const int ic = ipar_close(S, x);
   cd um[6], u0[6], up[6];
-  node6(rows + (size_t)(ic - 2) * NODE_STRIDE, omr, omi, um);
-  node6(rows + (size_t)(ic - 1) * NODE_STRIDE, omr, omi, u0);
-  node6(rows + (size_t)(ic)*NODE_STRIDE, omr, omi, up);
+  node6(rows + (size_t)(ic - 2) * NODE_STRIDE, S.ppar[ic - 1], omr, omi, um);
+  node6(rows + (size_t)(ic - 1) * NODE_STRIDE, S.ppar[ic], omr, omi, u0);
+  node6(rows + (size_t)(ic)*NODE_STRIDE, S.ppar[ic + 1], omr, omi, up);
   const double f = 0.5 * (x - S.ppar[ic]) / S.dppar_abs;
 #pragma unroll
   for (int k = 0; k < 6; ++k) acc[k] = acc[k] + wt * (u0[k] + f * (up[k] - um[k]));
@@ -214,13 +217,14 @@ __device__ __forceinline__ void node_acc(double w, double dre, double dim, doubl
 }
 
 template <bool STD, bool PAIR, bool FOLD = false>
-__device__ __forceinline__ void node_pair(const double *rec, int i, double omr, double omi,
+__device__ __forceinline__ void node_pair(const double *rec, double p, int i, double omr,
+                                          double omi,
                                           double mom, double kpar, double nq, double dim,
                                           double dim2, int pl1, int pc1, int pl2, int pc2, int ml1,
                                           int mc1, int ml2, int mc2, cd ap[6], cd am[6],
                                           int npar = 0) {
   const double G1 = rec[0], H1 = rec[1], G2 = rec[2], H2 = rec[3];
-  const double G3 = rec[4], H3 = rec[5], p = rec[6], p2 = rec[7];
+  const double G3 = rec[4], H3 = rec[5], p2 = p * p;
   cd U1, U2, U3;
   {
 #pragma clang fp contract(off)
@@ -251,7 +255,8 @@ __device__ __forceinline__ void node_pair(const double *rec, int i, double omr, 
 // range; otherwise node i is in range when it lies strictly inside one of the
 // two integrate() intervals of that row.
 template <int NCH, bool STD, bool PAIR>
-__device__ __forceinline__ void sweep_nodes(cdptr srows, int npar, int c, double omr, double omi,
+__device__ __forceinline__ void sweep_nodes(cdptr srows, cdptr pa, int npar, int c,
+                                            double omr, double omi,
                                             double m, double kpar, double nq, double dim, int pl1,
                                             int pc1, int pl2, int pc2, int ml1, int mc1, int ml2,
                                             int mc2, cd ap[6], cd am[6]) {
@@ -263,7 +268,7 @@ __device__ __forceinline__ void sweep_nodes(cdptr srows, int npar, int c, double
       double rec[NODE_STRIDE];
 #pragma unroll
       for (int t = 0; t < NODE_STRIDE; ++t) rec[t] = srows[(i - 1) * NODE_STRIDE + t];
-      node_pair<true, PAIR, true>(rec, i, omr, omi, mom, kpar, nq, dim, dim2, 0, 0, 0, 0, 0, 0, 0,
+      node_pair<true, PAIR, true>(rec, pa[i], i, omr, omi, mom, kpar, nq, dim, dim2, 0, 0, 0, 0, 0, 0, 0,
                                   0, ap, am, npar);
     }
     return;
@@ -272,7 +277,7 @@ __device__ __forceinline__ void sweep_nodes(cdptr srows, int npar, int c, double
     const double G1 = srows[(i - 1) * NODE_STRIDE + 0], H1 = srows[(i - 1) * NODE_STRIDE + 1];
     const double G2 = srows[(i - 1) * NODE_STRIDE + 2], H2 = srows[(i - 1) * NODE_STRIDE + 3];
     const double G3 = srows[(i - 1) * NODE_STRIDE + 4], H3 = srows[(i - 1) * NODE_STRIDE + 5];
-    const double p = srows[(i - 1) * NODE_STRIDE + 6], p2 = srows[(i - 1) * NODE_STRIDE + 7];
+    const double p = pa[i], p2 = p * p;
     cd U1, U2, U3;
     {
       // mul + add keeps each instruction to one scalar (table) operand
@@ -303,7 +308,8 @@ __device__ __forceinline__ void sweep_nodes(cdptr srows, int npar, int c, double
 // takes the last partial chunk.  The nodes are accumulated in the same order
 // as sweep_nodes.
 template <int NCH, bool STD, bool PAIR, int K>
-__device__ __forceinline__ void sweep_nodes_pf(cdptr srows, int npar, int c, double omr,
+__device__ __forceinline__ void sweep_nodes_pf(cdptr srows, cdptr pa, int npar, int c,
+                                               double omr,
                                                double omi, double m, double kpar, double nq,
                                                double dim, int pl1, int pc1, int pl2, int pc2,
                                                int ml1, int mc1, int ml2, int mc2, cd ap[6],
@@ -321,14 +327,16 @@ __device__ __forceinline__ void sweep_nodes_pf(cdptr srows, int npar, int c, dou
       for (int t = 0; t < NODE_STRIDE; ++t) rec[u][t] = srows[(i0 + u * NCH - 1) * NODE_STRIDE + t];
 #pragma unroll
     for (int u = 0; u < K; ++u)
-      node_pair<STD, PAIR, STD>(rec[u], i0 + u * NCH, omr, omi, mom, kpar, nq, dim, dim2, pl1, pc1,
+      node_pair<STD, PAIR, STD>(rec[u], pa[i0 + u * NCH], i0 + u * NCH, omr, omi, mom, kpar, nq,
+                                dim, dim2, pl1, pc1,
                                 pl2, pc2, ml1, mc1, ml2, mc2, ap, am, npar);
   }
   for (; i0 <= hi; i0 += NCH) {
     double rec[NODE_STRIDE];
 #pragma unroll
     for (int t = 0; t < NODE_STRIDE; ++t) rec[t] = srows[(i0 - 1) * NODE_STRIDE + t];
-    node_pair<STD, PAIR, STD>(rec, i0, omr, omi, mom, kpar, nq, dim, dim2, pl1, pc1, pl2, pc2, ml1,
+    node_pair<STD, PAIR, STD>(rec, pa[i0], i0, omr, omi, mom, kpar, nq, dim, dim2, pl1, pc1, pl2,
+                              pc2, ml1,
                               mc1, ml2, mc2, ap, am, npar);
   }
 }
@@ -394,7 +402,8 @@ __device__ __forceinline__ void sweep_split(cdptr crow, cdptr rsum, int npar, do
 }
 
 // endpoint contribution (weight 1) of node e for a row with denominator shift nq
-__device__ __forceinline__ void endpoint_acc(const double *__restrict__ rows, int e, int npar,
+__device__ __forceinline__ void endpoint_acc(const double *__restrict__ rows, cdptr pa,
+                                             int e, int npar,
                                              double m, double kpar, double omr, double omi,
                                              double nq, cd acc[6]) {
   if (e < 1 || e > npar - 1) return;
@@ -403,7 +412,8 @@ __device__ __forceinline__ void endpoint_acc(const double *__restrict__ rows, in
   const cd U2 = mk(omr * r[2] + r[3], omi * r[2]);
   const cd U3 = mk(omr * r[4] + r[5], omi * r[4]);
   const double dim = m * omi;
-  node_acc<false>(0.5, (m * omr - kpar * r[6]) - nq, dim, dim * dim, U1, U2, U3, r[6], r[7], acc);
+  const double p = pa[e];
+  node_acc<false>(0.5, (m * omr - kpar * p) - nq, dim, dim * dim, U1, U2, U3, p, p * p, acc);
 }
 
 }  // namespace
@@ -463,6 +473,7 @@ __global__ __launch_bounds__(64 * W) void k_sweep(const ProbDev *__restrict__ pr
       if (((rowbase + nn) % TW) != wv) continue;
       const double *rows = S.nodes + (size_t)nn * (npar - 1) * NODE_STRIDE;
       const cdptr srows = (cdptr)rows;
+      const cdptr spa = (cdptr)S.ppar;  // the nodes' ppar values
       RowClass rp, rm;
       classify(S, Pd.P, kpar, omr, omi, nn, rp, pa);
       if (nn > 0) classify(S, Pd.P, kpar, omr, omi, -nn, rm, pa);
@@ -497,30 +508,30 @@ __global__ __launch_bounds__(64 * W) void k_sweep(const ProbDev *__restrict__ pr
           else
             sweep_split<false>(crow, rs, npar, omr, omi, m, nq, dim, ap, am);
         } else if (nn > 0) {
-          sweep_nodes<NCH, true, true>(srows, npar, c, omr, omi, m, kpar, nq, dim, 0, 0, 0, 0, 0, 0,
+          sweep_nodes<NCH, true, true>(srows, spa, npar, c, omr, omi, m, kpar, nq, dim, 0, 0, 0, 0, 0, 0,
                                        0, 0, ap, am);
         } else {
-          sweep_nodes<NCH, true, false>(srows, npar, c, omr, omi, m, kpar, nq, dim, 0, 0, 0, 0, 0,
+          sweep_nodes<NCH, true, false>(srows, spa, npar, c, omr, omi, m, kpar, nq, dim, 0, 0, 0, 0, 0,
                                         0, 0, 0, ap, am);
         }
       } else {
-        sweep_nodes<NCH, false, true>(srows, npar, c, omr, omi, m, kpar, nq, dim, pl1, pc1, pl2, pc2,
+        sweep_nodes<NCH, false, true>(srows, spa, npar, c, omr, omi, m, kpar, nq, dim, pl1, pc1, pl2, pc2,
                                       ml1, mc1, ml2, mc2, ap, am);
       }
       // (with several lanes per omega a full-range row took its end nodes in the sweep)
       if (c == 0 && !(NCH > 1 && std_row)) {
-        endpoint_acc(rows, rp.a1, npar, m, kpar, omr, omi, nq, ap);
-        endpoint_acc(rows, rp.b1, npar, m, kpar, omr, omi, nq, ap);
+        endpoint_acc(rows, spa, rp.a1, npar, m, kpar, omr, omi, nq, ap);
+        endpoint_acc(rows, spa, rp.b1, npar, m, kpar, omr, omi, nq, ap);
         if (rp.a2 > 0) {
-          endpoint_acc(rows, rp.a2, npar, m, kpar, omr, omi, nq, ap);
-          endpoint_acc(rows, rp.b2, npar, m, kpar, omr, omi, nq, ap);
+          endpoint_acc(rows, spa, rp.a2, npar, m, kpar, omr, omi, nq, ap);
+          endpoint_acc(rows, spa, rp.b2, npar, m, kpar, omr, omi, nq, ap);
         }
         if (nn > 0) {
-          endpoint_acc(rows, rm.a1, npar, m, kpar, omr, omi, -nq, am);
-          endpoint_acc(rows, rm.b1, npar, m, kpar, omr, omi, -nq, am);
+          endpoint_acc(rows, spa, rm.a1, npar, m, kpar, omr, omi, -nq, am);
+          endpoint_acc(rows, spa, rm.b1, npar, m, kpar, omr, omi, -nq, am);
           if (rm.a2 > 0) {
-            endpoint_acc(rows, rm.a2, npar, m, kpar, omr, omi, -nq, am);
-            endpoint_acc(rows, rm.b2, npar, m, kpar, omr, omi, -nq, am);
+            endpoint_acc(rows, spa, rm.a2, npar, m, kpar, omr, omi, -nq, am);
+            endpoint_acc(rows, spa, rm.b2, npar, m, kpar, omr, omi, -nq, am);
           }
         }
       }
@@ -639,6 +650,7 @@ __device__ __forceinline__ void sweep_req_body(const ProbDev *__restrict__ probp
     for (; nn <= S.nmax; nn += TS) {
       const double *rows = S.nodes + (size_t)nn * (npar - 1) * NODE_STRIDE;
       const cdptr srows = (cdptr)rows;
+      const cdptr spa = (cdptr)S.ppar;  // the nodes' ppar values
       RowClass rp, rm;
       classify(S, Pd.P, kpar, omr, omi, nn, rp, pa);
       if (nn > 0) classify(S, Pd.P, kpar, omr, omi, -nn, rm, pa);
@@ -655,29 +667,29 @@ __device__ __forceinline__ void sweep_req_body(const ProbDev *__restrict__ probp
       const bool stdm = (nn == 0) || ((rm.a2 < 0) && rm.a1 == 1 && rm.b1 == npar - 1);
       if (stdp && stdm) {
         if (nn > 0)
-          sweep_nodes_pf<NCH, true, true, K>(srows, npar, c, omr, omi, m, kpar, nq, dim, 0, 0, 0, 0, 0, 0,
+          sweep_nodes_pf<NCH, true, true, K>(srows, spa, npar, c, omr, omi, m, kpar, nq, dim, 0, 0, 0, 0, 0, 0,
                                        0, 0, ap, am);
         else
-          sweep_nodes_pf<NCH, true, false, K>(srows, npar, c, omr, omi, m, kpar, nq, dim, 0, 0, 0, 0, 0,
+          sweep_nodes_pf<NCH, true, false, K>(srows, spa, npar, c, omr, omi, m, kpar, nq, dim, 0, 0, 0, 0, 0,
                                         0, 0, 0, ap, am);
       } else {
-        sweep_nodes_pf<NCH, false, true, K>(srows, npar, c, omr, omi, m, kpar, nq, dim, pl1, pc1, pl2, pc2,
+        sweep_nodes_pf<NCH, false, true, K>(srows, spa, npar, c, omr, omi, m, kpar, nq, dim, pl1, pc1, pl2, pc2,
                                       ml1, mc1, ml2, mc2, ap, am);
       }
       // (a full-range row took its end nodes in the sweep)
       if (c == 0 && !(stdp && stdm)) {
-        endpoint_acc(rows, rp.a1, npar, m, kpar, omr, omi, nq, ap);
-        endpoint_acc(rows, rp.b1, npar, m, kpar, omr, omi, nq, ap);
+        endpoint_acc(rows, spa, rp.a1, npar, m, kpar, omr, omi, nq, ap);
+        endpoint_acc(rows, spa, rp.b1, npar, m, kpar, omr, omi, nq, ap);
         if (rp.a2 > 0) {
-          endpoint_acc(rows, rp.a2, npar, m, kpar, omr, omi, nq, ap);
-          endpoint_acc(rows, rp.b2, npar, m, kpar, omr, omi, nq, ap);
+          endpoint_acc(rows, spa, rp.a2, npar, m, kpar, omr, omi, nq, ap);
+          endpoint_acc(rows, spa, rp.b2, npar, m, kpar, omr, omi, nq, ap);
         }
         if (nn > 0) {
-          endpoint_acc(rows, rm.a1, npar, m, kpar, omr, omi, -nq, am);
-          endpoint_acc(rows, rm.b1, npar, m, kpar, omr, omi, -nq, am);
+          endpoint_acc(rows, spa, rm.a1, npar, m, kpar, omr, omi, -nq, am);
+          endpoint_acc(rows, spa, rm.b1, npar, m, kpar, omr, omi, -nq, am);
           if (rm.a2 > 0) {
-            endpoint_acc(rows, rm.a2, npar, m, kpar, omr, omi, -nq, am);
-            endpoint_acc(rows, rm.b2, npar, m, kpar, omr, omi, -nq, am);
+            endpoint_acc(rows, spa, rm.a2, npar, m, kpar, omr, omi, -nq, am);
+            endpoint_acc(rows, spa, rm.b2, npar, m, kpar, omr, omi, -nq, am);
           }
         }
       }
@@ -735,7 +747,7 @@ __device__ __forceinline__ void sweep_req_body(const ProbDev *__restrict__ probp
 }
 
 template <int NCH, int W, int K, bool RED>
-__global__ __launch_bounds__(64 * W) void k_sweep_req(const ProbDev *__restrict__ probp, int nom,
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2))) void k_sweep_req(const ProbDev *__restrict__ probp, int nom,
                                                       const double2 *__restrict__ om_in,
                                                       double2 *__restrict__ chiA,
                                                       const int *__restrict__ pidx) {
@@ -1096,7 +1108,7 @@ __device__ __forceinline__ void resonant_body(const ProbDev *__restrict__ probp,
       for (int i = wlo + c; i <= whi; i += NCH) {
         const cd cf = mk(win[2 * (i - wlo)], win[2 * (i - wlo) + 1]);
         cd u[6];
-        node6(rows + (size_t)(i - 1) * NODE_STRIDE, omr, omi, u);
+        node6(rows + (size_t)(i - 1) * NODE_STRIDE, S.ppar[i], omr, omi, u);
 #pragma unroll
         for (int k = 0; k < 6; ++k) pv[k] = pv[k] + cf * u[k];
       }

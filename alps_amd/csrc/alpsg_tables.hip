@@ -297,8 +297,6 @@ __device__ __forceinline__ void moments_block(int bx, int n, int nperp, int npar
   o[3] = h2;
   o[4] = g3;
   o[5] = h3;
-  o[6] = pa;
-  o[7] = pa * pa;
 }
 
 __global__ void k_moments(int nperp, int npar, int nmax, double kpar, double m,
@@ -316,12 +314,13 @@ __global__ void k_moments(int nperp, int npar, int nmax, double kpar, double m,
 // where C_a is real and omega-independent.  Record (n, ipar):
 //   [0] kpar p, [1..6] C1, p C1, p^2 C1, C2, C3, p C3 for +n, [7..12] for -n.
 __global__ void k_crec(int npar, int nmax, double kpar, double q, double m,
-                       const double *__restrict__ nodes, double *__restrict__ crec) {
+                       const double *__restrict__ ppar, const double *__restrict__ nodes,
+                       double *__restrict__ crec) {
   const int ipar = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   const int n = blockIdx.y;
   if (ipar > npar - 1 || n > nmax) return;
   const double *r = nodes + ((size_t)n * (npar - 1) + (ipar - 1)) * NODE_STRIDE;
-  const double p = r[6], p2 = r[7];
+  const double p = ppar[ipar], p2 = p * p;
   const double kp = kpar * p, nq = (1.0 * n) * q;
   double *o = crec + ((size_t)n * (npar - 1) + (ipar - 1)) * CREC_STRIDE;
   o[0] = kp;
@@ -340,8 +339,8 @@ __global__ void k_crec(int npar, int nmax, double kpar, double q, double m,
 
 // rsum[n] = sum over the interior nodes ipar = 2..npar-2 of
 // {G1, p G1, p^2 G1, G2, G3, p G3} / m, in ipar order (one thread per (n, term))
-__global__ void k_rsum(int npar, int nmax, double m, const double *__restrict__ nodes,
-                       double *__restrict__ rsum) {
+__global__ void k_rsum(int npar, int nmax, double m, const double *__restrict__ ppar,
+                       const double *__restrict__ nodes, double *__restrict__ rsum) {
   const int n = blockIdx.x;
   const int k = threadIdx.x;
   if (n > nmax || k >= 6) {
@@ -352,17 +351,20 @@ __global__ void k_rsum(int npar, int nmax, double m, const double *__restrict__ 
   for (int ipar = 2; ipar <= npar - 2; ++ipar) {
     const double *r = nodes + ((size_t)n * (npar - 1) + (ipar - 1)) * NODE_STRIDE;
     const double g = (k <= 2) ? r[0] : (k == 3 ? r[2] : r[4]);
-    const double pw = (k == 1 || k == 5) ? r[6] : (k == 2 ? r[7] : 1.0);
+    const double p = ppar[ipar];
+    const double pw = (k == 1 || k == 5) ? p : (k == 2 ? p * p : 1.0);
     s += (pw * g) / m;
   }
   rsum[(size_t)n * RSUM_STRIDE + k] = s;
 }
 
 hipError_t launch_split_tables(hipStream_t st, int npar, int nmax, double kpar, double q, double m,
-                               const double *nodes, double *crec, double *rsum) {
+                               const double *ppar, const double *nodes, double *crec,
+                               double *rsum) {
   hipLaunchKernelGGL(k_crec, dim3((npar - 1 + 63) / 64, nmax + 1), dim3(64), 0, st, npar, nmax,
-                     kpar, q, m, nodes, crec);
-  hipLaunchKernelGGL(k_rsum, dim3(nmax + 1), dim3(RSUM_STRIDE), 0, st, npar, nmax, m, nodes, rsum);
+                     kpar, q, m, ppar, nodes, crec);
+  hipLaunchKernelGGL(k_rsum, dim3(nmax + 1), dim3(RSUM_STRIDE), 0, st, npar, nmax, m, ppar, nodes,
+                     rsum);
   return hipGetLastError();
 }
 
@@ -514,8 +516,6 @@ __device__ __forceinline__ void moments_block_mb(int bx, int n0, int nperp, int 
     o[3] = h[u][1];
     o[4] = g[u][2];
     o[5] = h[u][2];
-    o[6] = pa;
-    o[7] = pa * pa;
   }
 }
 
