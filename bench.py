@@ -408,8 +408,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the omega/request axis shards across GPUs with no communication, "
-              "see DESIGN.md)", file=sys.stderr)
+              "(the omega/request axis would shard across GPUs with no communication: "
+              "one process and one handle or server per GPU)", file=sys.stderr)
         sys.exit(2)
 
     if args.copy_probe:
