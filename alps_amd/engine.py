@@ -422,13 +422,23 @@ class Engine:
         return dict(ef=ef, bf=bf, Us=us, ds=ds, Ps=ps, Ps_split=psp, W_EM=ew)
 
     def map_search(self, omi, omf, gami, gamf, nr, ni, loggridw=False, loggridg=False,
-                   numroots=100):
+                   numroots=100, out=None):
         """det D on the nr x ni grid plus the local minima (ALPS_fns.f90:3595-3788).
-        Returns om[nr, ni], cal[nr, ni], val[nr, ni], iroots [(ir, ii)], nfound."""
+        Returns om[nr, ni], cal[nr, ni], val[nr, ni], iroots [(ir, ii)], nfound.
+        ``out`` = (om, cal, val), flat contiguous arrays of nr*ni complex128,
+        complex128 and float64, receives the grids instead of new arrays (a
+        caller that maps repeatedly reuses them; the returned grids are views)."""
         N = nr * ni
-        om = np.zeros(N, dtype=np.complex128)
-        cal = np.zeros(N, dtype=np.complex128)
-        val = np.zeros(N)
+        if out is None:
+            om = np.zeros(N, dtype=np.complex128)
+            cal = np.zeros(N, dtype=np.complex128)
+            val = np.zeros(N)
+        else:
+            om, cal, val = out
+            for a, dt in ((om, np.complex128), (cal, np.complex128), (val, np.float64)):
+                if a.dtype != dt or a.size != N or not a.flags.c_contiguous:
+                    raise ValueError("map_search: out arrays must be contiguous with nr*ni "
+                                     "elements of complex128, complex128, float64")
         iroots = np.zeros(2 * numroots, dtype=np.int32)
         nf = np.zeros(1, dtype=np.int32)
         _check(load_library().alpsg_map_search(self._h, omi, omf, gami, gamf, nr, ni,
