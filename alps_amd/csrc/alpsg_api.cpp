@@ -56,6 +56,8 @@ hipError_t launch_rel_bessel(hipStream_t st, int npts, int nmax, double zfac, co
                              const double *pbrel, const int *mtab, double *out);
 hipError_t launch_map_post(hipStream_t st, int nr, int ni, const double2 *cal, double2 *calo,
                            double *val, unsigned char *hit);
+hipError_t launch_map_grid(hipStream_t st, int nr, int ni, double omi, double dr, double gami,
+                           double di, double2 *om);
 hipError_t launch_osc_round(hipStream_t st, int n, alpsg_osc::OscMachine *S, const double2 *det,
                             double2 *om, int *round_ctr, int *remaining);
 hipError_t launch_resonant_all(hipStream_t st, int nch, int RB, const ProbDev *prob, int nspec,
@@ -2000,29 +2002,42 @@ int alpsg_map_search(alpsg_handle h, double omi, double omf, double gami, double
   double dr = omf - omi, di = gamf - gami;
   if (nr > 1) dr = (omf - omi) / (1.0 * (nr - 1));
   if (ni > 1) di = (gamf - gami) / (1.0 * (ni - 1));
-  for (int ir = 1; ir <= nr; ++ir) {
-    double wr;
-    if (loggridw) {
-      wr = omi;
-      if (nr > 1) wr = omi * std::pow(omf / omi, (1.0 * (ir - 1)) / (1.0 * (nr - 1)));
-    } else {
-      wr = omi + dr * (1.0 * (ir - 1));
-    }
-    for (int ii = 1; ii <= ni; ++ii) {
-      double wi;
-      if (loggridg) {
-        wi = gami;
-        if (ni > 1) wi = gami * std::pow(gamf / gami, (1.0 * (ii - 1)) / (1.0 * (ni - 1)));
+  // the grid (ALPS_fns.f90:3650-3700) into g[(ir-1) + nr (ii-1)]
+  auto fill_grid = [&](cplx *g) {
+#pragma clang fp contract(off)
+    for (int ir = 1; ir <= nr; ++ir) {
+      double wr;
+      if (loggridw) {
+        wr = omi;
+        if (nr > 1) wr = omi * std::pow(omf / omi, (1.0 * (ir - 1)) / (1.0 * (nr - 1)));
       } else {
-        wi = gami + di * (1.0 * (ii - 1));
+        wr = omi + dr * (1.0 * (ir - 1));
       }
-      om[(size_t)(ir - 1) + (size_t)nr * (ii - 1)] = cplx(wr, wi);
+      for (int ii = 1; ii <= ni; ++ii) {
+        double wi;
+        if (loggridg) {
+          wi = gami;
+          if (ni > 1) wi = gami * std::pow(gamf / gami, (1.0 * (ii - 1)) / (1.0 * (ni - 1)));
+        } else {
+          wi = gami + di * (1.0 * (ii - 1));
+        }
+        g[(size_t)(ir - 1) + (size_t)nr * (ii - 1)] = cplx(wr, wi);
+      }
     }
-  }
+  };
   // det D of every grid point, then on the device log10|D| with the NaN / Inf
-  // sentinels of ALPS_fns.f90:3723-3742 and the minimum flags of find_minima
+  // sentinels of ALPS_fns.f90:3723-3742 and the minimum flags of find_minima.
+  // A linear grid is built on the device (the same products and sums), and the
+  // caller's copy on the host while the device works; a log grid (std::pow) on
+  // the host.
+  const bool dev_grid = !loggridw && !loggridg;
   HIPCHK(h->om_d.ensure(N));
-  HIPCHK(hipMemcpyAsync(h->om_d.p, om, sizeof(cplx) * N, hipMemcpyHostToDevice, h->st));
+  if (dev_grid) {
+    HIPCHK(launch_map_grid(h->st, nr, ni, omi, dr, gami, di, h->om_d.p));
+  } else {
+    fill_grid(om);
+    HIPCHK(hipMemcpyAsync(h->om_d.p, om, sizeof(cplx) * N, hipMemcpyHostToDevice, h->st));
+  }
   LaunchShape sh = shape_for(h, (int)N);
   HIPCHK(hipEventRecord(h->ev0, h->st));
   int rc = disp_device(h, (int)N, h->om_d.p, nullptr, false, false, false, sh);
@@ -2040,7 +2055,12 @@ int alpsg_map_search(alpsg_handle h, double omi, double omf, double gami, double
   int relerr = 0;
   if (h->any_rel)
     HIPCHK(hipMemcpyAsync(&relerr, h->err_d.p, sizeof(int), hipMemcpyDeviceToHost, h->st));
-  if (om_out) std::memcpy(om_out, om, sizeof(cplx) * N);
+  if (om_out) {
+    if (dev_grid)
+      fill_grid(reinterpret_cast<cplx *>(om_out));
+    else
+      std::memcpy(om_out, om, sizeof(cplx) * N);
+  }
   HIPCHK(hipStreamSynchronize(h->st));
   if (cal_out) std::memcpy(cal_out, cal_h, sizeof(cplx) * N);
   if (val_out) std::memcpy(val_out, val_h, sizeof(double) * N);

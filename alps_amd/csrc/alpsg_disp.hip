@@ -1500,7 +1500,30 @@ __global__ __launch_bounds__(256) void k_map_post(int nr, int ni, const double2 
   hit[k] = h ? 1 : 0;
 }
 
+// The linear map grid (ALPS_fns.f90:3650-3700 with loggridw = loggridg = F):
+// point (ir, ii) at k = (ir-1) + nr (ii-1) is (omi + dr (ir-1), gami + di (ii-1)),
+// one product and one sum each, rounded as the host computes them.
+__global__ __launch_bounds__(256) void k_map_grid(int nr, int ni, double omi, double dr,
+                                                  double gami, double di,
+                                                  double2 *__restrict__ om) {
+#pragma clang fp contract(off)
+  const size_t N = (size_t)nr * ni;
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= N) return;
+  const int ir = (int)(k % nr) + 1, ii = (int)(k / nr) + 1;
+  om[k] = make_double2(omi + dr * (1.0 * (ir - 1)), gami + di * (1.0 * (ii - 1)));
+}
+
 }  // namespace
+
+hipError_t launch_map_grid(hipStream_t st, int nr, int ni, double omi, double dr, double gami,
+                           double di, double2 *om) {
+  const size_t N = (size_t)nr * ni;
+  if (N == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_map_grid, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, nr, ni, omi,
+                     dr, gami, di, om);
+  return hipGetLastError();
+}
 
 hipError_t launch_map_post(hipStream_t st, int nr, int ni, const double2 *cal, double2 *calo,
                            double *val, unsigned char *hit) {

@@ -71,6 +71,22 @@ def test_map_search_matches_oracle():
     assert n_o == n_g and np.array_equal(ir_o, ir_g)
 
 
+@pytest.mark.parametrize("loggrid", [(0, 0), (1, 0), (0, 1)])
+def test_map_grid_on_the_device_is_the_returned_grid(loggrid):
+    """det D of the map equals det D of a batch at the returned grid, bit for
+    bit (the same launch shape): the device-built linear grid rounds like the
+    host's, and log grids are built on the host."""
+    p = cases.small_bimax()
+    E = _engine(p)
+    E.set_k(0.3, 0.3)
+    gam = (0.02, 0.4) if loggrid[1] else (-0.4, 0.05)
+    om, cal, val, _, _ = E.map_search(0.05, 1.5, gam[0], gam[1], 18, 12,
+                                      loggridw=bool(loggrid[0]), loggridg=bool(loggrid[1]))
+    det = E.disp_batch(om.T.ravel())
+    assert np.array_equal(det, cal.T.ravel())
+    E.close()
+
+
 def test_refine_after_map_matches_oracle():
     p = cases.small_bimax()
     p.roots.D_threshold = 1e-14
