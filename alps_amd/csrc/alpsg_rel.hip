@@ -122,11 +122,13 @@ __global__ __launch_bounds__(256) void k_rel_nonres(const ProbDev *__restrict__ 
     {
 #pragma clang fp contract(off)
       const double nq = 1.0 * n * q;
+      // point pt = ip npar + ia, (ip, ia) stepped without a division per point
+      int ip = (int)threadIdx.x / npar, ia = (int)threadIdx.x % npar;
       for (int pt = threadIdx.x; pt < (nperp + 1) * npar; pt += 256) {
-        const int ip = pt / npar, ia = pt % npar;
         const double g = S.gam_pp[ip * (npar + 1) + ia];
         const double pr = (g * m * om.re - nq) / kpar;
         if (S.ppar[ia] <= pr && S.ppar[ia + 1] > pr) hit = 1;
+        for (ia += 256; ia >= npar; ia -= npar) ++ip;
       }
     }
     const int found = __syncthreads_or(hit);
@@ -136,17 +138,22 @@ __global__ __launch_bounds__(256) void k_rel_nonres(const ProbDev *__restrict__ 
     const double *at = S.atab + (size_t)an * (nperp + 1) * ATAB_STRIDE;
     cd acc[6];
     for (int k = 0; k < 6; ++k) acc[k] = mk(0.0, 0.0);
+    // point pt = (ip - 1)(npar - 1) + ia - 1, stepped as above
+    int ip = 1 + (int)threadIdx.x / (npar - 1), ia = 1 + (int)threadIdx.x % (npar - 1);
     for (int pt = threadIdx.x; pt < (nperp - 1) * (npar - 1); pt += 256) {
-      const int ip = 1 + pt / (npar - 1), ia = 1 + pt % (npar - 1);
       const double w = (ip == nperp - 1 ? 1.0 : 2.0) * ((ia == 1 || ia == npar - 1) ? 1.0 : 2.0);
       const size_t o = (size_t)ip * (npar + 1) + ia;
       const double g = S.gam_pp[o], d1 = S.df1[o], d2 = S.df2[o];
       const double p1 = S.pperp[ip], p2 = S.ppar[ia];
-      // resU (ALPS_fns.f90:1587-1592) with the point's Gamma
+      // resU (ALPS_fns.f90:1587-1592) with the point's Gamma; num / den as
+      // num conj(den) / |den|^2 with one reciprocal (|den| is O(1) here: no
+      // scaling needed) instead of three divisions
       const double A = (kpar / (g * m)) * (p1 * d2 - p2 * d1);
       const cd num = mk(q * (om.re * d1 + A), q * (om.im * d1));
       const cd den = mk(((g * m) * om.re - kpar * p2) - (1.0 * n) * q, (g * m) * om.im);
-      const cd r = w * cdivc(num, den);
+      const double inv = w * rcp1_d(den.re * den.re + den.im * den.im);
+      const cd r = mk((num.re * den.re + num.im * den.im) * inv,
+                      (num.im * den.re - num.re * den.im) * inv);
       const double a1 = at[ip * ATAB_STRIDE + 0], a2 = at[ip * ATAB_STRIDE + 1],
                    a3 = at[ip * ATAB_STRIDE + 2];
       const cd u1 = a1 * r, u3 = a3 * r;
@@ -156,6 +163,7 @@ __global__ __launch_bounds__(256) void k_rel_nonres(const ProbDev *__restrict__ 
       acc[3] = acc[3] + a2 * r;
       acc[4] = acc[4] + u3;
       acc[5] = acc[5] + p2 * u3;
+      for (ia += 256; ia > npar - 1; ia -= npar - 1) ++ip;
     }
     cd v[6];
     for (int k = 0; k < 6; ++k) {
