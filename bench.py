@@ -392,9 +392,10 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (torch) and exit: the "
                          "known-size kernel that scales the profiler's FETCH_SIZE/WRITE_SIZE")
-    ap.add_argument("--workload", choices=["map", "serve"], default="map",
+    ap.add_argument("--workload", choices=["map", "serve", "serve_rel"], default="map",
                     help="map: the batched map search (default); serve: the Poisson stream "
-                         "of independent root requests through the root server")
+                         "of independent root requests through the root server; serve_rel: "
+                         "the relativistic kpar-scan requests through the root server")
     ap.add_argument("--requests", type=int, default=4096)
     ap.add_argument("--variants", type=int, default=64, help="plasma variants in the request pool")
     ap.add_argument("--rate", type=float, default=20000.0, help="request arrivals per second")
@@ -427,6 +428,9 @@ def main():
     if args.workload == "serve":
         sv = run_serve(args)
         print(json.dumps({"workload": "serve", **sv}))
+        return
+    if args.workload == "serve_rel":
+        print(json.dumps(run_serve_rel(args)))
         return
 
     if args.pmc_shapes:
