@@ -109,7 +109,10 @@ constexpr int MTAB_S = 4096;
 #endif
 constexpr int SW_SHAPE[4] = {ALPSG_SERVE_SWEEP};
 constexpr int SW_NCH = SW_SHAPE[0], SW_W = SW_SHAPE[1], SW_RC = SW_SHAPE[2], SW_K = SW_SHAPE[3];
-constexpr int RS_RB = 8;
+#ifndef ALPSG_SERVE_RB
+#define ALPSG_SERVE_RB 8
+#endif
+constexpr int RS_RB = ALPSG_SERVE_RB;
 constexpr int REL_NB = 4;  // harmonic blocks of k_rel_nonres in a pass (fixed, as every shape here)
 constexpr size_t ALIGN = 256;
 constexpr int MAX_JOBS = 8192;               // (request, species) jobs per table launch
