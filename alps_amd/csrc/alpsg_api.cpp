@@ -821,7 +821,11 @@ int osc_graph_batch(alpsg_ctx *h, const alpsg_root_params *rp, int n, cplx *om, 
       break;
     }
   }
-  h->evals += (long long)n * OSC_K * (long long)trep.size();
+  // evaluations made: with the fused round (n <= REQ_MAX_N) the rounds after the
+  // last solve finished return at once, and the round counter stops there
+  const bool fused = req && REQ_W == 4 && REQ_NCH == 32 && REQ_K == 4;
+  const long long rounds_run = fused ? (long long)h->oscctl_pin[0] : (long long)OSC_K * trep.size();
+  h->evals += (long long)n * rounds_run;
   if (!finished) return fail(ALPSG_E_DEVICE, "secant rounds on the device did not finish");
   HIPCHK(hipMemcpy(S.data(), h->osc_d.p, sizeof(alpsg_osc::OscMachine) * n, hipMemcpyDeviceToHost));
   h->root_ms.assign(n, 0.0);

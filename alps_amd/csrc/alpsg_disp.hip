@@ -1584,6 +1584,10 @@ __global__ __launch_bounds__(256) void k_assemble_osc(
     const double2 *__restrict__ chiA, const double2 *__restrict__ chiB, double2 *__restrict__ det,
     int nspec, alpsg_osc::OscMachine *__restrict__ S, double2 *__restrict__ om_next,
     int *__restrict__ round_ctr, int *__restrict__ remaining) {
+  // a round after every solve has finished does nothing (its values would be
+  // ignored); the count is read before any thread can change it
+  const int left = *remaining;
+  if (left == 0) return;
   const int opb = 256 / (6 * nspec);
   for (int blk = 0; blk * opb < nom; ++blk) {
     assemble_block(blk, probp, nom, W, RB, om_in, nullptr, chiA, chiB, nullptr, nullptr, nullptr, 0,
@@ -1603,9 +1607,10 @@ __global__ __launch_bounds__(256) void k_sweep_res(const ProbDev *__restrict__ p
                                                    double2 *__restrict__ chiA,
                                                    double2 *__restrict__ chiB,
                                                    const int *__restrict__ zidx, int RC, int RB,
-                                                   int nspec) {
+                                                   int nspec, const int *__restrict__ remaining) {
   constexpr int OMB = 64 / NCHR;
   __shared__ double win[4][OMB * 2 * PV_WIN];
+  if (*remaining == 0) return;  // every solve has finished: nothing to evaluate
   const int A = nom * RC;
   const int b = blockIdx.x;
   if (b < A) {
@@ -1648,10 +1653,10 @@ hipError_t launch_round(hipStream_t st, int nchr, int RB, int RC, const ProbDev 
     const int grid = nom * RC + (gx * nspec * 2 * RB + 3) / 4;                                     \
     if (general)                                                                                   \
       hipLaunchKernelGGL((k_sweep_res<32, 4, N, true>), dim3(grid), dim3(256), 0, st, prob, nom,   \
-                         om, chiA, chiB, zidx, RC, RB, nspec);                                     \
+                         om, chiA, chiB, zidx, RC, RB, nspec, ctl + 1);                            \
     else                                                                                           \
       hipLaunchKernelGGL((k_sweep_res<32, 4, N, false>), dim3(grid), dim3(256), 0, st, prob, nom,  \
-                         om, chiA, chiB, zidx, RC, RB, nspec);                                     \
+                         om, chiA, chiB, zidx, RC, RB, nspec, ctl + 1);                            \
     ok = true;                                                                                     \
   }
   RD(4) RD(8) RD(16) RD(32) RD(64)
