@@ -21,7 +21,9 @@
 #include <vector>
 
 #include "../../include/alpsg.h"
+#include "alpsg_analyt.h"
 #include "alpsg_device.h"
+#include "alpsg_guard.h"
 #include "alpsg_osc.h"
 #include "alpsg_relhost.h"
 
@@ -217,6 +219,9 @@ struct alpsg_ctx {
   int osc_graph_next = 0;
   bool osc_graphs_ok = true;
   std::vector<double> root_ms;  // completion time of each root of the last secant batch
+  std::vector<double> scan_ms;  // end of each k step of the last om_scan (ms from its start)
+  std::vector<int> scan_iters;  // [step][root] iterations of its solves, -1 if not followed
+  bool plain_roots = false;     // root searches step on the host, one disp launch per omega
   bool any_general_fit = false;  // a species needs eval_fit in the Landau term
   // closed-form (use_bM) species and their &bM_spec parameters
   int usebm[ALPSG_MAXSPEC] = {0};
@@ -848,7 +853,7 @@ int osc_graph_batch(alpsg_ctx *h, const alpsg_root_params *rp, int n, cplx *om, 
 // a state machine (alpsg_osc.h) whose disp() requests are served in batches.
 int secant_osc_batch(alpsg_ctx *h, const alpsg_root_params *rp, int nroots, cplx *om, cplx *Dfin,
                      int *iters) {
-  {
+  if (!h->plain_roots) {
     const int rc = osc_graph_batch(h, rp, nroots, om, Dfin, iters);
     if (rc != -1) return rc;
   }
@@ -1383,6 +1388,7 @@ extern "C" {
 const char *alpsg_last_error(void) { return g_err.c_str(); }
 
 int alpsg_create(const alpsg_plasma_desc *d, alpsg_handle *out) {
+  ALPSG_TRY
   if (!d || !out) return fail(ALPSG_E_ARG, "null argument");
   *out = nullptr;
   const int nspec = d->nspec, nperp = d->nperp, npar = d->npar;
@@ -1426,14 +1432,19 @@ int alpsg_create(const alpsg_plasma_desc *d, alpsg_handle *out) {
       if (d->param_fit_nperp < std::max(nperp, d->ngamma)) return fail(ALPSG_E_ARG, "param_fit_nperp < max(nperp, ngamma)");
       continue;
     }
-    if (d->ACmethod[s] != 1 && d->ACmethod[s] != 2) return fail(ALPSG_E_UNSUPPORTED, "ACmethod must be 1 or 2");
-    if (d->ACmethod[s] == 1) {
+    if (d->ACmethod[s] < 0 || d->ACmethod[s] > 2) return fail(ALPSG_E_ARG, "ACmethod must be 0, 1 or 2");
+    if (d->ACmethod[s] == 0) {
+      // the pre-coded f0 of distribution_analyt (ALPS_analyt.f90:77-79)
+      if (!distribution_analyt_defined(s + 1))
+        return fail(ALPSG_E_UNSUPPORTED, "ACmethod 0: distribution_analyt defines no f0 for this species");
+    } else if (d->ACmethod[s] == 1) {
       if (d->n_fits[s] < 1 || d->n_fits[s] > ALPSG_MAXFITS || d->n_fits[s] > d->max_fits)
         return fail(ALPSG_E_ARG, "n_fits out of range");
       if (!d->fit_type || !d->perp_correction || !d->param_fit) return fail(ALPSG_E_ARG, "missing fit arrays");
       for (int f = 0; f < d->n_fits[s]; ++f) {
         int ft = d->fit_type[s + (size_t)nspec * f];
-        if (ft != 1 && ft != 2 && ft != 6) return fail(ALPSG_E_UNSUPPORTED, "fit_type must be 1, 2 or 6");
+        if (ft != 1 && ft != 2 && ft != 3 && ft != 6)
+          return fail(ALPSG_E_UNSUPPORTED, "fit_type must be 1, 2, 3 or 6 (4 and 5 are relativistic)");
       }
       if (d->param_fit_nperp < nperp) return fail(ALPSG_E_ARG, "param_fit_nperp < nperp");
     } else {
@@ -1575,6 +1586,11 @@ int alpsg_create(const alpsg_plasma_desc *d, alpsg_handle *out) {
             c[0] = prm(0);
             c[3] = prm(3);
             c[4] = pc * prm(4) * (pv * pv);
+          } else if (ft == 3) {
+            // Juettner in (pperp, ppar), ALPS_analyt.f90:238-242
+            c[0] = prm(0);
+            c[3] = pv * pv;
+            c[4] = d->vA * d->vA / (d->ms[s] * d->ms[s]);
           } else {
             c[0] = prm(0);
             c[3] = prm(3) * pc * (pv * pv);
@@ -1654,6 +1670,7 @@ int alpsg_create(const alpsg_plasma_desc *d, alpsg_handle *out) {
     S.nperp = nperp;
     S.npar = npar;
     S.acmethod = d->ACmethod[s];
+    S.anum = s + 1;
     S.nfits = d->ACmethod[s] == 1 ? d->n_fits[s] : 0;
     for (int f = 0; f < S.nfits; ++f) S.fit_type[f] = d->fit_type[s + (size_t)nspec * f];
     S.q = d->qs[s];
@@ -1765,14 +1782,18 @@ int alpsg_create(const alpsg_plasma_desc *d, alpsg_handle *out) {
   HIPCHK(h->prob_d.ensure(1));
   *out = hold.release();
   return ALPSG_OK;
+  ALPSG_CATCH
 }
 
 int alpsg_destroy(alpsg_handle h) {
+  ALPSG_TRY
   delete h;
   return ALPSG_OK;
+  ALPSG_CATCH
 }
 
 int alpsg_set_k(alpsg_handle h, double kperp, double kpar, const int *nmax_in, int *nmax_out) {
+  ALPSG_TRY
   if (!h) return fail(ALPSG_E_ARG, "null handle");
   if (!(kpar != 0.0) || !std::isfinite(kpar) || !std::isfinite(kperp))
     return fail(ALPSG_E_ARG, "kpar must be finite and nonzero");
@@ -1899,11 +1920,13 @@ int alpsg_set_k(alpsg_handle h, double kperp, double kpar, const int *nmax_in, i
   if (nmax_out)
     for (int s = 0; s < nspec; ++s) nmax_out[s] = nmax[s];
   return ALPSG_OK;
+  ALPSG_CATCH
 }
 
 int alpsg_calc_eigen(alpsg_handle h, int n, const double *om, int eigen, int heat, double *ef,
                      double *bf, double *Us, double *ds, double *Ps, double *Ps_split,
                      double *W_EM) {
+  ALPSG_TRY
   if (!h || (n > 0 && !om)) return fail(ALPSG_E_ARG, "null argument");
   if (!h->have_k) return fail(ALPSG_E_STATE, "alpsg_set_k has not been called");
   if (n <= 0) return ALPSG_OK;
@@ -1911,14 +1934,18 @@ int alpsg_calc_eigen(alpsg_handle h, int n, const double *om, int eigen, int hea
                           reinterpret_cast<cplx *>(ef), reinterpret_cast<cplx *>(bf),
                           reinterpret_cast<cplx *>(Us), reinterpret_cast<cplx *>(ds), Ps, Ps_split,
                           W_EM);
+  ALPSG_CATCH
 }
 
 int alpsg_autotune(alpsg_handle h, int n, const double *om, int *shape_out) {
+  ALPSG_TRY
   if (!h) return fail(ALPSG_E_ARG, "null handle");
   return autotune(h, n, reinterpret_cast<const double2 *>(om), shape_out);
+  ALPSG_CATCH
 }
 
 int alpsg_set_shape(alpsg_handle h, int n, const int *shape) {
+  ALPSG_TRY
   if (!h) return fail(ALPSG_E_ARG, "null handle");
   if (n < 1) return fail(ALPSG_E_ARG, "n must be >= 1");
   const int b = size_bucket(n);
@@ -1930,68 +1957,103 @@ int alpsg_set_shape(alpsg_handle h, int n, const int *shape) {
   for (int k = 0; k < 5; ++k) h->tuned_shape[b][k] = shape[k];
   h->tuned_valid[b] = 1;
   return ALPSG_OK;
+  ALPSG_CATCH
 }
 
 int alpsg_get_shape(alpsg_handle h, int n, int *shape_out) {
+  ALPSG_TRY
   if (!h || !shape_out) return fail(ALPSG_E_ARG, "null argument");
   if (n < 1) return fail(ALPSG_E_ARG, "n must be >= 1");
   LaunchShape sh = shape_for(h, n);
   const int t[5] = {sh.nch_a, sh.w_a, sh.rc_a, sh.nch_b, sh.rb};
   for (int k = 0; k < 5; ++k) shape_out[k] = t[k];
   return ALPSG_OK;
+  ALPSG_CATCH
 }
 
 int alpsg_get_nmax(alpsg_handle h, int *nmax_out) {
+  ALPSG_TRY
   if (!h || !nmax_out) return fail(ALPSG_E_ARG, "null argument");
   if (!h->have_k) return fail(ALPSG_E_STATE, "alpsg_set_k has not been called");
   for (int s = 0; s < h->nspec; ++s) nmax_out[s] = h->nmax[s];
   return ALPSG_OK;
+  ALPSG_CATCH
 }
 
 long long alpsg_eval_count(alpsg_handle h) { return h ? h->evals : -1; }
 
 int alpsg_last_root_times(alpsg_handle h, int nroots, double *ms_out) {
+  ALPSG_TRY
   if (!h || !ms_out) return fail(ALPSG_E_ARG, "null argument");
   if (nroots != (int)h->root_ms.size()) return fail(ALPSG_E_ARG, "nroots differs from the last secant batch");
   for (int r = 0; r < nroots; ++r) ms_out[r] = h->root_ms[r];
   return ALPSG_OK;
+  ALPSG_CATCH
 }
 
 double alpsg_last_kernel_ms(alpsg_handle h) { return h ? h->last_ms : -1.0; }
 
+int alpsg_last_scan_steps(alpsg_handle h, int nsteps, int nroots, double *ms_out, int *iters_out) {
+  ALPSG_TRY
+  if (!h) return fail(ALPSG_E_ARG, "null handle");
+  if (nsteps != (int)h->scan_ms.size() || (size_t)nsteps * nroots != h->scan_iters.size())
+    return fail(ALPSG_E_ARG, "nsteps or nroots differs from the last om_scan");
+  for (int t = 0; ms_out && t < nsteps; ++t) ms_out[t] = h->scan_ms[t];
+  for (size_t t = 0; iters_out && t < h->scan_iters.size(); ++t) iters_out[t] = h->scan_iters[t];
+  return ALPSG_OK;
+  ALPSG_CATCH
+}
+
+int alpsg_set_plain_roots(alpsg_handle h, int plain) {
+  ALPSG_TRY
+  if (!h) return fail(ALPSG_E_ARG, "null handle");
+  h->plain_roots = plain != 0;
+  return ALPSG_OK;
+  ALPSG_CATCH
+}
+
 int alpsg_last_kernel_split(alpsg_handle h, double *ms3) {
+  ALPSG_TRY
   if (!h || !ms3) return fail(ALPSG_E_ARG, "null argument");
   for (int k = 0; k < 3; ++k) ms3[k] = h->last_split[k];
   return ALPSG_OK;
+  ALPSG_CATCH
 }
 
 int alpsg_disp_batch(alpsg_handle h, int n, const double *om, double *det, double *chi0,
                      double *chi0_low, double *wave) {
+  ALPSG_TRY
   if (!h || (n > 0 && (!om || !det))) return fail(ALPSG_E_ARG, "null argument");
   return run_disp(h, n, om, det, chi0, chi0_low, wave);
+  ALPSG_CATCH
 }
 
 int alpsg_disp_plain(alpsg_handle h, int n, const double *om, double *det) {
+  ALPSG_TRY
   if (!h || (n > 0 && (!om || !det))) return fail(ALPSG_E_ARG, "null argument");
   for (int i = 0; i < n; ++i) {
     int rc = run_disp(h, 1, om + 2 * i, det + 2 * i, nullptr, nullptr, nullptr);
     if (rc) return rc;
   }
   return ALPSG_OK;
+  ALPSG_CATCH
 }
 
 int alpsg_secant_batch(alpsg_handle h, const alpsg_root_params *rp, int nroots, double *om,
                        double *D_final, int *iters) {
+  ALPSG_TRY
   if (!h || !rp || (nroots > 0 && !om)) return fail(ALPSG_E_ARG, "null argument");
   if (!h->have_k) return fail(ALPSG_E_STATE, "alpsg_set_k has not been called");
   if (nroots <= 0) return ALPSG_OK;
   return root_batch(h, rp, nroots, reinterpret_cast<cplx *>(om), reinterpret_cast<cplx *>(D_final),
                     iters);
+  ALPSG_CATCH
 }
 
 int alpsg_map_search(alpsg_handle h, double omi, double omf, double gami, double gamf, int nr,
                      int ni, int loggridw, int loggridg, double *om_out, double *cal_out,
                      double *val_out, int numroots, int *iroots, int *nfound) {
+  ALPSG_TRY
   if (!h) return fail(ALPSG_E_ARG, "null handle");
   if (nr < 1 || ni < 1) return fail(ALPSG_E_ARG, "nr, ni must be >= 1");
   const size_t N = (size_t)nr * ni;
@@ -2101,11 +2163,13 @@ int alpsg_map_search(alpsg_handle h, double omi, double omf, double gami, double
   }
   if (nfound) *nfound = n;
   return ALPSG_OK;
+  ALPSG_CATCH
 }
 
 int alpsg_om_scan(alpsg_handle h, const alpsg_root_params *rp, int scan_type, int log_scan,
                   double diff, double diff2, int n_out, int n_res, int nroots, double *roots,
                   double *kperp_out, double *kpar_out, double *roots_out) {
+  ALPSG_TRY
   if (!h || !rp || !roots || !kperp_out || !kpar_out || !roots_out) return fail(ALPSG_E_ARG, "null argument");
   if (!h->have_k) return fail(ALPSG_E_STATE, "alpsg_set_k has not been called");
   if (scan_type < 0 || scan_type > 4 || n_out < 1 || n_res < 1 || nroots < 1) return fail(ALPSG_E_ARG, "scan parameters");
@@ -2122,6 +2186,10 @@ int alpsg_om_scan(alpsg_handle h, const alpsg_root_params *rp, int scan_type, in
   const int nt = n_out * n_res;
   std::vector<cplx> om;
   std::vector<int> idx;
+  const auto t0 = std::chrono::steady_clock::now();
+  h->scan_ms.assign(nt, 0.0);
+  h->scan_iters.assign((size_t)nt * nroots, -1);
+  std::vector<int> its;
   for (int it = 1; it <= nt; ++it) {
     double kperp = h->kperp, kpar = h->kpar;
     switch (scan_type) {
@@ -2172,9 +2240,11 @@ int alpsg_om_scan(alpsg_handle h, const alpsg_root_params *rp, int scan_type, in
         idx.push_back(r);
       }
     std::vector<cplx> Df(om.size());
-    rc = root_batch(h, rp, (int)om.size(), om.data(), Df.data(), nullptr);
+    its.assign(om.size(), 0);
+    rc = root_batch(h, rp, (int)om.size(), om.data(), Df.data(), its.data());
     if (rc) return rc;
     for (size_t t = 0; t < idx.size(); ++t) R[idx[t]] = om[t];
+    for (size_t t = 0; t < idx.size(); ++t) h->scan_iters[(size_t)(it - 1) * nroots + idx[t]] = its[t];
     for (size_t t = 0; t < idx.size(); ++t) {
       int r = idx[t];
       double re = R[r].real();
@@ -2191,8 +2261,11 @@ int alpsg_om_scan(alpsg_handle h, const alpsg_root_params *rp, int scan_type, in
       kpar_out[row] = h->kpar;
       for (int r = 0; r < nroots; ++r) RO[(size_t)row * nroots + r] = R[r];
     }
+    h->scan_ms[it - 1] =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
   return ALPSG_OK;
+  ALPSG_CATCH
 }
 
 }  // extern "C"
@@ -2254,6 +2327,7 @@ extern "C" int alpsg_om_double_scan(alpsg_handle h, const alpsg_root_params *rp,
                                     double *kperp_out, double *kpar_out, double *roots_out,
                                     int *written, double *ef, double *bf, double *Us, double *ds,
                                     double *Ps, double *Ps_split, double *W_EM) {
+  ALPSG_TRY
   if (!h || !rp || !s1 || !s2 || !roots || !kperp_out || !kpar_out || !roots_out || !written)
     return fail(ALPSG_E_ARG, "null argument");
   if (!h->have_k) return fail(ALPSG_E_STATE, "alpsg_set_k has not been called");
@@ -2388,4 +2462,5 @@ extern "C" int alpsg_om_double_scan(alpsg_handle h, const alpsg_root_params *rp,
   }
   for (int r = 0; r < nroots; ++r) R[r] = omlast[r];
   return ALPSG_OK;
+  ALPSG_CATCH
 }

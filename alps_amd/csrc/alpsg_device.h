@@ -18,6 +18,7 @@
 struct SpecDev {
   int nperp, npar, nmax;
   int acmethod, nfits;
+  int anum;           // 1-based species index: the case of distribution_analyt (ACmethod 0)
   int fit_type[ALPSG_DMAXFITS];
   int poly_kind, poly_order, poly_logfit;
   int fit_uniform;    // every fit Maxwellian with pperp-independent ppar parameters
@@ -133,6 +134,19 @@ __device__ __forceinline__ cd cexpd(cd z) {
   return cd{e * c, e * s};
 }
 __device__ __forceinline__ cd clogd(cd z) { return cd{log(hypot(z.re, z.im)), atan2(z.im, z.re)}; }
+// principal square root (branch cut on the negative real axis): with
+// t = sqrt((|x| + |z|)/2), sqrt(z) = (t, y/2t) for x >= 0 and (|y|/2t, +-t)
+// for x < 0, which avoids the cancellation of the half-angle forms
+__device__ __forceinline__ cd csqrtd(cd z) {
+  const double r = hypot(z.re, z.im);
+  if (r == 0.0) return cd{0.0, 0.0};
+  if (z.re >= 0.0) {
+    const double t = sqrt(0.5 * (r + z.re));
+    return cd{t, z.im / (2.0 * t)};
+  }
+  const double t = sqrt(0.5 * (r - z.re));
+  return cd{fabs(z.im) / (2.0 * t), copysign(t, z.im)};
+}
 // principal z**p for real p
 __device__ __forceinline__ cd cpowr(cd z, double p) {
   if (z.re == 0.0 && z.im == 0.0) return cd{0.0, 0.0};

@@ -3,12 +3,15 @@
 // non-relativistic and relativistic paths.
 #pragma once
 
+#include "alpsg_analyt.h"
 #include "alpsg_device.h"
 
 namespace {
 
 // eval_fit (ALPS_analyt.f90:32-363) at (iperp, complex p).
 __device__ __noinline__ cd eval_fit(const SpecDev &S, int iperp, cd p) {
+  if (S.acmethod == 0)  // the pre-coded f0 (ALPS_analyt.f90:77-79)
+    return distribution_analyt(S.anum, S.pperp[iperp], p);
   if (S.acmethod == 2) {
     // fit_function_poly, Chebyshev kind 1 (ALPS_analyt.f90:307-358)
     cd t = mk((p.re - S.poly_norm1) / S.poly_norm2, p.im / S.poly_norm2);
@@ -42,6 +45,11 @@ __device__ __noinline__ cd eval_fit(const SpecDev &S, int iperp, cd p) {
       case 2: {  // kappa: c3 = u4, c4 = pc*u5*pperp^2
         const cd kp = mk(1.0 + c[1] * sq.re + c[4], c[1] * sq.im);
         f = f + c[0] * cpowr(kp, c[3]);
+      } break;
+      case 3: {  // Juettner in (pperp, ppar): c3 = pperp^2, c4 = vA^2/m^2 (ALPS_analyt.f90:238-242)
+        const cd sq2 = mk(1.0 + (c[3] + sq.re) * c[4], sq.im * c[4]);
+        const cd r = csqrtd(sq2);
+        f = f + c[0] * cexpd(mk(-c[1] * r.re, -c[1] * r.im));
       } break;
       case 6: {  // bi-Moyal: c3 = u4*pc*pperp^2
         const cd arg = mk(c[3] + c[1] * sq.re, c[1] * sq.im);

@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "../../include/alpsg.h"
+#include "alpsg_guard.h"
 
 int alpsg_internal_fail(int code, const std::string &msg);
 
@@ -509,6 +510,7 @@ int nparams_host(int t) {
 extern "C" int alpsg_determine_param_fit(const alpsg_plasma_desc *d, const alpsg_fit_params *fpar,
                                          double *param_fit, int *fit_type, double *perp_correction,
                                          double *poly_fit_coeffs, double *quality) {
+  ALPSG_TRY
   if (!d || !param_fit || !fit_type || !perp_correction)
     return alpsg_internal_fail(ALPSG_E_ARG, "determine_param_fit: null argument");
   const int nspec = d->nspec, nperp = d->nperp, npar = d->npar, mf = d->max_fits;
@@ -809,4 +811,5 @@ extern "C" int alpsg_determine_param_fit(const alpsg_plasma_desc *d, const alpsg
   }
   if (quality) *quality = qt;
   return ALPSG_OK;
+  ALPSG_CATCH
 }

@@ -41,17 +41,6 @@ __device__ __forceinline__ cd cdivc(cd a, cd b) {
   const double r = b.re / b.im, d = b.re * r + b.im;
   return cd{(a.re * r + a.im) / d, (a.im * r - a.re) / d};
 }
-__device__ __forceinline__ cd csqrtd(cd z) {
-  // principal square root
-  const double r = hypot(z.re, z.im);
-  if (r == 0.0) return mk(0.0, 0.0);
-  if (z.re >= 0.0) {
-    const double t = sqrt(0.5 * (r + z.re));
-    return mk(t, z.im / (2.0 * t));
-  }
-  const double t = sqrt(0.5 * (r - z.re));
-  return mk(fabs(z.im) / (2.0 * t), copysign(t, z.im));
-}
 __device__ __forceinline__ double m1p(int n) { return (n % 2 == 0) ? 1.0 : -1.0; }
 
 // mode order k: 0 xx, 1 yy, 2 zz, 3 xy, 4 xz, 5 yz

@@ -44,7 +44,9 @@
 #include <vector>
 
 #include "../../include/alpsg.h"
+#include "alpsg_analyt.h"
 #include "alpsg_device.h"
+#include "alpsg_guard.h"
 #include "alpsg_osc.h"
 #include "alpsg_relhost.h"
 
@@ -297,10 +299,17 @@ struct alpsg_server {
   bool stop = false, failed = false, gc = false;
   std::string fail_msg;
   std::thread th;
-  // statistics
+  // statistics, written by the service thread only
   long long passes = 0, evals = 0, uploads = 0, evictions = 0;
   // host time (ms): whole passes, admission work, disp launch enqueue, waits on the device
   double t_pass = 0, t_admit = 0, t_launch = 0, t_wait = 0;
+  // the copy alpsg_server_stats reads: the service thread publishes it under
+  // mu at the end of every pass, so a stats call never reads state that a
+  // running pass is changing
+  struct Snap {
+    long long passes = 0, evals = 0, uploads = 0, evictions = 0, live = 0, arena_used = 0;
+    double t_pass = 0, t_admit = 0, t_launch = 0, t_wait = 0;
+  } snap;
 };
 
 namespace {
@@ -451,7 +460,8 @@ int prepare_plasma(const alpsg_plasma_desc *d, Plasma &r) {
       bool uni = true;
       for (int f = 0; f < nf; ++f) {
         const int ft = d->fit_type[s + (size_t)nspec * f];
-        if (ft != 1 && ft != 2 && ft != 6) return alpsg_internal_fail(ALPSG_E_UNSUPPORTED, "fit_type must be 1, 2 or 6");
+        if (ft != 1 && ft != 2 && ft != 3 && ft != 6)
+          return alpsg_internal_fail(ALPSG_E_UNSUPPORTED, "fit_type must be 1, 2, 3 or 6 (4 and 5 are relativistic)");
         r.ftype[(size_t)s * ALPSG_MAXFITS + f] = ft;
         // one Maxwellian whose (u2, u3) do not depend on pperp: the uniform Landau part
         if (ft != 1) uni = false;
@@ -474,8 +484,13 @@ int prepare_plasma(const alpsg_plasma_desc *d, Plasma &r) {
       r.plogmax[s] = d->poly_log_max[s];
       r.maxorder = std::max(r.maxorder, r.porder[s]);
       r.general = true;
+    } else if (r.acm[s] == 0) {
+      // the pre-coded f0 of distribution_analyt (ALPS_analyt.f90:77-79)
+      if (!distribution_analyt_defined(s + 1))
+        return alpsg_internal_fail(ALPSG_E_UNSUPPORTED, "ACmethod 0: distribution_analyt defines no f0 for this species");
+      r.general = true;
     } else {
-      return alpsg_internal_fail(ALPSG_E_UNSUPPORTED, "ACmethod must be 1 or 2");
+      return alpsg_internal_fail(ALPSG_E_ARG, "ACmethod must be 0, 1 or 2");
     }
     r.fit_off[s + 1] = r.fit_off[s] + (r.acm[s] == 1 ? (size_t)(nperp + 1) * r.nfits[s] * FIT_STRIDE : 0);
   }
@@ -561,13 +576,18 @@ int prepare_plasma(const alpsg_plasma_desc *d, Plasma &r) {
             c[0] = prm(s, i, 0, f);
             c[3] = prm(s, i, 3, f);
             c[4] = pc * prm(s, i, 4, f) * (pv * pv);
+          } else if (ft == 3) {
+            // Juettner in (pperp, ppar), ALPS_analyt.f90:238-242
+            c[0] = prm(s, i, 0, f);
+            c[3] = pv * pv;
+            c[4] = d->vA * d->vA / (d->ms[s] * d->ms[s]);
           } else {
             c[0] = prm(s, i, 0, f);
             c[3] = prm(s, i, 3, f) * pc * (pv * pv);
           }
         }
       }
-    } else {
+    } else if (r.acm[s] == 2) {
       for (int i = 0; i <= nperp; ++i)
         for (int n = 0; n <= r.porder[s]; ++n)
           polyc[((size_t)s * (nperp + 1) + i) * (r.maxorder + 1) + n] =
@@ -800,6 +820,7 @@ int admit_k(alpsg_server *S, Request &r, int &njobs, int &max_nperp, int &nnew) 
     sd.npar = npar;
     sd.nmax = nm;
     sd.acmethod = P.acm[s];
+    sd.anum = s + 1;
     sd.nfits = P.acm[s] == 1 ? P.nfits[s] : 0;
     for (int f = 0; f < sd.nfits; ++f) sd.fit_type[f] = P.ftype[(size_t)s * ALPSG_MAXFITS + f];
     sd.fit_uniform = P.funi[s];
@@ -869,9 +890,21 @@ void release(alpsg_server *S, Request &r) {
   r.slot = -1;
 }
 
+// the end of a pass: its results and a snapshot of the statistics, under mu
 void post_results(alpsg_server *S, std::vector<Result> &done) {
-  if (done.empty()) return;
   std::lock_guard<std::mutex> lk(S->mu);
+  alpsg_server::Snap &sn = S->snap;
+  sn.passes = S->passes;
+  sn.evals = S->evals;
+  sn.uploads = S->uploads;
+  sn.evictions = S->evictions;
+  sn.live = (long long)S->live.size();
+  sn.arena_used = (long long)S->arena.used;
+  sn.t_pass = S->t_pass;
+  sn.t_admit = S->t_admit;
+  sn.t_launch = S->t_launch;
+  sn.t_wait = S->t_wait;
+  if (done.empty()) return;
   for (auto &d : done) S->results.push_back(d);
   S->finished += (long long)done.size();
   S->cv_done.notify_all();
@@ -1113,7 +1146,10 @@ int server_pass(alpsg_server *S) {
       continue;
     }
     if (r.state == 2 && r.m.done) {
-      done.push_back(Result{r.tag, cplx(r.m.result().re, r.m.result().im), r.m.iter, ALPSG_OK,
+      // a solve that used up numiter iterations returns its best point, as
+      // secant_osc does after "Maximum iteration reached" (ALPS_fns.f90:2093-2099)
+      const int st = r.m.iter >= r.m.numiter ? ALPSG_MAXITER : ALPSG_OK;
+      done.push_back(Result{r.tag, cplx(r.m.result().re, r.m.result().im), r.m.iter, st,
                             now_ms() - r.t_submit});
       release(S, r);
       it = S->live.erase(it);
@@ -1125,8 +1161,8 @@ int server_pass(alpsg_server *S) {
   for (auto &p : S->resident)
     if (p->off != SIZE_MAX && p->users == 0 && p->ee_state != 1 && !p->registered.load()) evict(S, *p);
   drop_nonresident(S);
-  post_results(S, done);
   S->t_pass += now_ms() - tp0;
+  post_results(S, done);
   return ALPSG_OK;
 }
 
@@ -1139,7 +1175,14 @@ void service_loop(alpsg_server *S) {
       if (S->stop && S->queue.empty() && S->live.empty()) return;
       S->gc = false;
     }
-    const int rc = server_pass(S);
+    int rc;
+    try {
+      rc = server_pass(S);
+    } catch (const std::exception &e) {
+      rc = alpsg_internal_fail(ALPSG_E_DEVICE, std::string("internal error in a pass: ") + e.what());
+    } catch (...) {
+      rc = alpsg_internal_fail(ALPSG_E_DEVICE, "internal error in a pass");
+    }
     if (rc) {
       std::lock_guard<std::mutex> lk(S->mu);
       S->failed = true;
@@ -1192,6 +1235,7 @@ int enqueue(alpsg_server *S, std::vector<std::unique_ptr<Request>> &rs) {
 extern "C" {
 
 int alpsg_server_create(double hbm_bytes, int max_slots, alpsg_server **out) {
+  ALPSG_TRY
   if (!out) return alpsg_internal_fail(ALPSG_E_ARG, "null argument");
   *out = nullptr;
   if (max_slots < 1 || max_slots > 65536) return alpsg_internal_fail(ALPSG_E_ARG, "max_slots out of range 1..65536");
@@ -1245,9 +1289,11 @@ int alpsg_server_create(double hbm_bytes, int max_slots, alpsg_server **out) {
   S->th = std::thread(service_loop, S.get());
   *out = S.release();
   return ALPSG_OK;
+  ALPSG_CATCH
 }
 
 int alpsg_server_add_plasma(alpsg_server *S, const alpsg_plasma_desc *d, int *plasma_id) {
+  ALPSG_TRY
   if (!S || !d || !plasma_id) return alpsg_internal_fail(ALPSG_E_ARG, "null argument");
   auto pl = std::make_shared<Plasma>();
   const int rc = prepare_plasma(d, *pl);
@@ -1258,9 +1304,11 @@ int alpsg_server_add_plasma(alpsg_server *S, const alpsg_plasma_desc *d, int *pl
   S->registry[pl->id] = pl;
   *plasma_id = pl->id;
   return ALPSG_OK;
+  ALPSG_CATCH
 }
 
 int alpsg_server_remove_plasma(alpsg_server *S, int plasma_id) {
+  ALPSG_TRY
   if (!S) return alpsg_internal_fail(ALPSG_E_ARG, "null handle");
   std::lock_guard<std::mutex> lk(S->mu);
   auto it = S->registry.find(plasma_id);
@@ -1270,11 +1318,13 @@ int alpsg_server_remove_plasma(alpsg_server *S, int plasma_id) {
   S->gc = true;  // its device block goes once no request reads it
   S->cv.notify_all();
   return ALPSG_OK;
+  ALPSG_CATCH
 }
 
 int alpsg_server_submit_batch(alpsg_server *S, int n, const int *plasma_ids, const double *kperp,
                               const double *kpar, const double *om_guess,
                               const alpsg_root_params *rp, const long long *tags) {
+  ALPSG_TRY
   if (!S) return alpsg_internal_fail(ALPSG_E_ARG, "null handle");
   if (n < 0) return alpsg_internal_fail(ALPSG_E_ARG, "negative request count");
   if (n == 0) return ALPSG_OK;
@@ -1296,15 +1346,19 @@ int alpsg_server_submit_batch(alpsg_server *S, int n, const int *plasma_ids, con
     if (rc) return rc;
   }
   return enqueue(S, rs);
+  ALPSG_CATCH
 }
 
 int alpsg_server_submit_k(alpsg_server *S, int plasma_id, double kperp, double kpar,
                           const double *om_guess, const alpsg_root_params *rp, long long tag) {
+  ALPSG_TRY
   return alpsg_server_submit_batch(S, 1, &plasma_id, &kperp, &kpar, om_guess, rp, &tag);
+  ALPSG_CATCH
 }
 
 int alpsg_server_submit(alpsg_server *S, const alpsg_plasma_desc *d, double kperp, double kpar,
                         const double *om_guess, const alpsg_root_params *rp, long long tag) {
+  ALPSG_TRY
   if (!S || !d) return alpsg_internal_fail(ALPSG_E_ARG, "null argument");
   auto pl = std::make_shared<Plasma>();  // unregistered: freed with its request
   const int rc = prepare_plasma(d, *pl);
@@ -1313,10 +1367,12 @@ int alpsg_server_submit(alpsg_server *S, const alpsg_plasma_desc *d, double kper
   const int rc2 = make_request(std::move(pl), kperp, kpar, om_guess, rp, tag, rs[0]);
   if (rc2) return rc2;
   return enqueue(S, rs);
+  ALPSG_CATCH
 }
 
 int alpsg_server_poll(alpsg_server *S, int max_results, int wait, long long *tags, double *roots,
                       int *iters, int *status, double *latency_ms, int *n_out) {
+  ALPSG_TRY
   if (!S || !n_out) return alpsg_internal_fail(ALPSG_E_ARG, "null argument");
   std::unique_lock<std::mutex> lk(S->mu);
   if (wait)
@@ -1338,33 +1394,37 @@ int alpsg_server_poll(alpsg_server *S, int max_results, int wait, long long *tag
   *n_out = n;
   if (S->failed && n == 0) return alpsg_internal_fail(ALPSG_E_DEVICE, "server stopped: " + S->fail_msg);
   return ALPSG_OK;
+  ALPSG_CATCH
 }
 
 int alpsg_server_stats(alpsg_server *S, long long *counts, double *times_ms) {
+  ALPSG_TRY
   if (!S) return alpsg_internal_fail(ALPSG_E_ARG, "null handle");
   std::lock_guard<std::mutex> lk(S->mu);
   if (counts) {
     counts[0] = S->submitted;
     counts[1] = S->finished;
-    counts[2] = S->passes;
-    counts[3] = S->evals;
-    counts[4] = (long long)S->live.size();
+    counts[2] = S->snap.passes;
+    counts[3] = S->snap.evals;
+    counts[4] = S->snap.live;
     counts[5] = (long long)S->queue.size();
     counts[6] = (long long)S->arena.cap;
-    counts[7] = (long long)S->arena.used;
-    counts[8] = S->uploads;
-    counts[9] = S->evictions;
+    counts[7] = S->snap.arena_used;
+    counts[8] = S->snap.uploads;
+    counts[9] = S->snap.evictions;
   }
   if (times_ms) {
-    times_ms[0] = S->t_pass;
-    times_ms[1] = S->t_admit;
-    times_ms[2] = S->t_launch;
-    times_ms[3] = S->t_wait;
+    times_ms[0] = S->snap.t_pass;
+    times_ms[1] = S->snap.t_admit;
+    times_ms[2] = S->snap.t_launch;
+    times_ms[3] = S->snap.t_wait;
   }
   return ALPSG_OK;
+  ALPSG_CATCH
 }
 
 int alpsg_server_destroy(alpsg_server *S) {
+  ALPSG_TRY
   if (!S) return ALPSG_OK;
   {
     std::lock_guard<std::mutex> lk(S->mu);
@@ -1400,6 +1460,7 @@ int alpsg_server_destroy(alpsg_server *S) {
   if (S->st) (void)hipStreamDestroy(S->st);
   delete S;
   return ALPSG_OK;
+  ALPSG_CATCH
 }
 
 }  // extern "C"

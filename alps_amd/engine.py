@@ -32,8 +32,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libalpsg.so")
 
 ALPSG_OK = 0
+ALPSG_MAXITER = 6  # a served solve ended at numiter without |D| < D_threshold
 _ERRNAMES = {1: "ALPSG_E_ARG", 2: "ALPSG_E_UNSUPPORTED", 3: "ALPSG_E_DEVICE",
-             4: "ALPSG_E_STATE", 5: "ALPSG_E_NOTFOUND"}
+             4: "ALPSG_E_STATE", 5: "ALPSG_E_NOTFOUND", 6: "ALPSG_MAXITER"}
 
 # every symbol declared in include/alpsg.h
 EXPORTS = ["alpsg_last_error", "alpsg_create", "alpsg_destroy", "alpsg_set_k",
@@ -44,7 +45,8 @@ EXPORTS = ["alpsg_last_error", "alpsg_create", "alpsg_destroy", "alpsg_set_k",
            "alpsg_server_add_plasma", "alpsg_server_remove_plasma", "alpsg_server_submit_k",
            "alpsg_server_submit_batch", "alpsg_om_double_scan",
            "alpsg_server_submit", "alpsg_server_poll", "alpsg_server_stats",
-           "alpsg_server_destroy", "alpsg_determine_param_fit"]
+           "alpsg_server_destroy", "alpsg_determine_param_fit", "alpsg_last_scan_steps",
+           "alpsg_set_plain_roots"]
 
 
 class AlpsgError(RuntimeError):
@@ -123,6 +125,10 @@ def load_library(path: str = LIB_PATH):
     L.alpsg_eval_count.restype = C.c_longlong
     L.alpsg_last_root_times.argtypes = [vp, C.c_int, dp]
     L.alpsg_last_root_times.restype = C.c_int
+    L.alpsg_last_scan_steps.argtypes = [vp, C.c_int, C.c_int, dp, ip]
+    L.alpsg_last_scan_steps.restype = C.c_int
+    L.alpsg_set_plain_roots.argtypes = [vp, C.c_int]
+    L.alpsg_set_plain_roots.restype = C.c_int
     L.alpsg_autotune.argtypes = [vp, C.c_int, dp, ip]
     L.alpsg_set_shape.argtypes = [vp, C.c_int, ip]
     L.alpsg_get_shape.argtypes = [vp, C.c_int, ip]
@@ -398,6 +404,19 @@ class Engine:
         out = np.zeros(nroots)
         _check(load_library().alpsg_last_root_times(self._h, nroots, _dp(out)))
         return out
+
+    def last_scan_steps(self, nsteps: int, nroots: int):
+        """The last om_scan per k step: end time (ms from its start) [nsteps]
+        and each root's secant iterations [nsteps, nroots] (-1: not followed)."""
+        ms = np.zeros(nsteps)
+        it = np.zeros(nsteps * nroots, dtype=np.int32)
+        _check(load_library().alpsg_last_scan_steps(self._h, nsteps, nroots, _dp(ms), _ip(it)))
+        return ms, it.reshape(nsteps, nroots)
+
+    def set_plain_roots(self, plain: bool):
+        """Root searches step on the host with one disp launch per omega (the
+        plain form) instead of device-stepped graph rounds."""
+        _check(load_library().alpsg_set_plain_roots(self._h, int(bool(plain))))
 
     def refine_guess(self, guesses, roots: RootParams | None = None):
         """refine_guess (ALPS_fns.f90:3793-3856): secant on each guess, then det D there."""

@@ -84,6 +84,24 @@ def flops_per_eval(nmax, npar):
     return 54.0 * sum((2 * n + 1) * (npar - 1) for n in nmax)
 
 
+def progress(msg):
+    """One line per finished leg of the run, on stderr (stdout carries the JSON line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    """The host CPU's model name (from /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return (line.split(":", 1)[1].strip() +
+                        f" ({os.cpu_count()} logical CPUs visible)")
+    except OSError:
+        pass
+    return "unknown"
+
+
 def run_step(E, nr, ni, numroots=100, out=None):
     t0 = time.perf_counter()
     om, cal, val, iroots, nfound = E.map_search(MAP["omi"], MAP["omf"], MAP["gami"], MAP["gamf"],
@@ -92,11 +110,23 @@ def run_step(E, nr, ni, numroots=100, out=None):
     split = E.last_kernel_split_ms
     t_map = (time.perf_counter() - t0) * 1e3
     guesses = np.array([om[i - 1, j - 1] for i, j in iroots])
-    roots, dfin = E.refine_guess(guesses) if guesses.size else (guesses, guesses)
+    if guesses.size:
+        roots, dfin, iters = E.solve_batch(guesses)
+    else:
+        roots, dfin, iters = guesses, guesses, np.zeros(0, dtype=np.int32)
     lat = t_map + (E.last_root_times_ms(guesses.size) if guesses.size else np.zeros(0))
-    return dict(roots=roots, dfin=dfin, guesses=guesses, latency_ms=lat, map_kernel_ms=map_kernel_ms,
-                split_ms=split,
-                cal=cal, om=om, nfound=nfound)
+    return dict(roots=roots, dfin=dfin, iters=iters, guesses=guesses, latency_ms=lat,
+                map_kernel_ms=map_kernel_ms, split_ms=split, cal=cal, om=om, nfound=nfound)
+
+
+def oracle_roots(plasma, kperp, kpar, guesses, threads):
+    """secant_osc of the CPU restatement from each guess: (roots, iterations)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle
+    O = Oracle(plasma, n_threads=threads)
+    O.set_k(kperp, kpar)
+    out = [O.secant_osc(g) for g in guesses]
+    return np.array([w for w, _ in out]), np.array([it for _, it in out])
 
 
 def cpu_baseline(plasma, om_grid, cal_gpu, n_sample, threads):
@@ -121,7 +151,7 @@ def cpu_baseline(plasma, om_grid, cal_gpu, n_sample, threads):
     return dict(value=n_sample / dt, unit="chi-tensor evals/s", cores=threads, kind="port",
                 sample=f"{n_sample} strided points of the 256x256 map, full 128x256 grid, "
                        f"oracle/liboracle.so with {threads} OpenMP threads",
-                wall_s=dt), err
+                cpu_model=cpu_model(), wall_s=dt), err
 
 
 # -- served root solves ------------------------------------------------------
@@ -185,13 +215,14 @@ def serve_rel_variant(tke, rp, nperp=200, npar=400, ngamma=100, npparbar=200):
         bM_nmax=None, bM_Bessel_zero=None, bM_beta=None, bM_alpha=None, bM_pdrift=None, roots=rp)
 
 
-def run_serve_rel(args, n_req=256, seed=5):
+def run_serve_rel(args, seed=5):
     """Served kpar-scan root requests on relativistic plasmas (kT_e / m_e c^2 in
     {0.1, 0.3, 1.0}): kpar log-uniform in [0.05, 2], kperp = 0.1 kpar, guesses
     near the whistler branch; roots/s and p50/p99 latency, and the plain path on
     a sample."""
-    from alps_amd.engine import Engine, RootServer
+    from alps_amd.engine import ALPSG_OK, Engine, RootServer
     from alps_amd.plasma import RootParams
+    n_req = args.rel_requests
     rp = RootParams(secant_method=2, numiter=50, D_threshold=1e-12, D_prec=1e-5, D_gap=1e-5)
     pool = [serve_rel_variant(t, rp) for t in (0.1, 0.3, 1.0)]
     rng = np.random.default_rng(seed)
@@ -227,33 +258,55 @@ def run_serve_rel(args, n_req=256, seed=5):
         plain_t += time.perf_counter() - tp
         # roots that converged on both paths (|D| < D_threshold fixes them only
         # to |D| / |dD/dw|, so the two paths' rounding shows at that level)
-        if r is not None and status[i] == 0 and it < rp.numiter and iters[i] < rp.numiter:
+        if r is not None and status[i] == ALPSG_OK and it < rp.numiter:
             dev = max(dev, abs(r - served[i]) / max(abs(r), 1e-300))
             n_cmp += 1
-    return {
+    conv = status == ALPSG_OK
+    res = {
         "workload": "relativistic kpar-scan requests: Juettner electrons (kT/mc^2 0.1, 0.3, 1.0) "
                     "and cold protons, 200 x 400 (pperp, ppar), 100 x 200 (Gamma, pparbar), "
                     "kpar in [0.05, 2], all submitted at once",
         "requests": n_req,
         "wall_s": wall,
-        "roots_per_sec": int(np.sum(status == 0)) / wall,
-        "status_ok": int(np.sum(status == 0)),
-        "converged": int(np.sum((status == 0) & (iters < rp.numiter))),
-        "latency_p50_ms": float(np.percentile(lat, 50)),
-        "latency_p99_ms": float(np.percentile(lat, 99)),
+        "roots_per_sec": int(np.sum(conv)) / wall,
+        "converged": int(np.sum(conv)),
+        "max_iterations_reached": int(np.sum(status == 6)),
+        "failed": int(np.sum((status != ALPSG_OK) & (status != 6))),
+        "latency_p50_ms": float(np.percentile(lat[conv], 50)) if conv.any() else None,
+        "latency_p99_ms": float(np.percentile(lat[conv], 99)) if conv.any() else None,
         "mean_iterations": float(iters.mean()),
         "plain_roots_per_sec": len(idx) / plain_t,
         "plain_sample": int(len(idx)),
         "max_rel_dev_served_vs_plain_converged": dev,
         "compared": n_cmp,
     }
+    res["speedup_vs_plain"] = res["roots_per_sec"] / res["plain_roots_per_sec"]
+    if not args.no_cpu:
+        # two converged served roots against the oracle's secant_osc (about
+        # 3 s each on the host at this grid size)
+        ci = [i for i in idx if conv[i]][:2]
+        err = 0.0
+        thr = min(args.cpu_threads, os.cpu_count() or 1)
+        tc = time.perf_counter()
+        for i in ci:
+            w, _ = oracle_roots(pool[int(var[i])], 0.1 * kz[i], kz[i], [guess[i]], thr)
+            err = max(err, abs(served[i] - w[0]) / abs(w[0]))
+        t_cpu = time.perf_counter() - tc
+        res["roots_vs_oracle"] = {"compared": len(ci), "max_rel_err": err, "tolerance": 1e-10}
+        if ci:
+            res["cpu_baseline"] = {"value": len(ci) / t_cpu, "unit": "roots/s", "cores": thr,
+                                   "kind": "port", "cpu_model": cpu_model(),
+                                   "sample": f"{len(ci)} requests of the stream, oracle/"
+                                             f"liboracle.so with {thr} OpenMP threads",
+                                   "wall_s": t_cpu}
+    return res
 
 
 def run_serve(args):
     """Poisson stream of root requests through the continuous-batching server;
     the same requests (a sample) on the plain one-root-per-launch path and on
     the CPU restatement."""
-    from alps_amd.engine import Engine, RootServer
+    from alps_amd.engine import ALPSG_OK, Engine, RootServer
     pool, reqs = serve_requests(args.requests, args.variants)
     rng = np.random.default_rng(11)
     S = RootServer(hbm_bytes=args.hbm_gb * 1e9, max_slots=args.slots)
@@ -320,20 +373,27 @@ def run_serve(args):
         E.set_k(kp, kz)
         r, it = E.secant_osc(g)
         plain_t += time.perf_counter() - tp
-        if status[i] == 0:
+        if status[i] == ALPSG_OK:
             dev = max(dev, abs(r - served[i]) / max(abs(r), 1e-300))
         conv_plain += 1
+    conv = status == ALPSG_OK
     out = {
         "requests": len(reqs),
         "plasma_variants": len(pool),
         "arrival_rate_per_s": args.rate,
         "submit_s": t_sub,
         "wall_s": wall,
-        "roots_per_sec": int(np.sum(status == 0)) / wall,
-        "converged": int(np.sum((status == 0) & (iters < pool[0].roots.numiter))),
-        "latency_p50_ms": float(np.percentile(lat, 50)),
-        "latency_p99_ms": float(np.percentile(lat, 99)),
-        "status_ok": int(np.sum(status == 0)),
+        # converged solves only: a solve that used up numiter iterations
+        # (ALPSG_MAXITER) or failed is not a root
+        "roots_per_sec": int(np.sum(conv)) / wall,
+        "converged": int(np.sum(conv)),
+        "max_iterations_reached": int(np.sum(status == 6)),
+        "failed": int(np.sum((status != ALPSG_OK) & (status != 6))),
+        "finished_per_sec": len(reqs) / wall,
+        "latency_p50_ms": float(np.percentile(lat[conv], 50)),
+        "latency_p99_ms": float(np.percentile(lat[conv], 99)),
+        "latency_all_p50_ms": float(np.percentile(lat, 50)),
+        "latency_all_p99_ms": float(np.percentile(lat, 99)),
         "mean_iterations": float(iters.mean()),
         "passes": passes,
         "disp_evals": evals,
@@ -353,22 +413,122 @@ def run_serve(args):
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         from oracle import Oracle
         thr = min(args.cpu_threads, os.cpu_count() or 1)
-        n_cpu, t_cpu = 0, 0.0
+        n_cpu, t_cpu, err, n_conv_cpu, n_cmp, err_mi, n_mi, st_agree = 0, 0.0, 0.0, 0, 0, 0.0, 0, 0
         for i in idx:
             v, kp, kz, g = reqs[i]
             tc = time.perf_counter()
             O = Oracle(pool[v], n_threads=thr)
             O.set_k(kp, kz)
-            O.secant_osc(g)
+            w, it = O.secant_osc(g)
             t_cpu += time.perf_counter() - tc
             n_cpu += 1
+            n_conv_cpu += int(it < rp.numiter)
+            st_agree += int((status[i] == ALPSG_OK) == (it < rp.numiter))
+            rel = abs(served[i] - w) / max(abs(w), 1e-300)
+            if status[i] == ALPSG_OK and it < rp.numiter:
+                err = max(err, rel)   # converged roots: the 1e-10 bar
+                n_cmp += 1
+            elif status[i] == 6:
+                # both spent numiter: 50 steps of a non-converging secant
+                # amplify rounding, so these best points agree more loosely
+                err_mi = max(err_mi, rel)
+                n_mi += 1
             if t_cpu > args.cpu_seconds:
                 break
-        out["cpu_baseline"] = {"value": n_cpu / t_cpu, "unit": "roots/s", "cores": thr,
-                               "kind": "port",
-                               "sample": f"{n_cpu} requests of the stream (strided), oracle/"
-                                         f"liboracle.so with {thr} OpenMP threads",
+        out["cpu_baseline"] = {"value": n_conv_cpu / t_cpu, "unit": "roots/s", "cores": thr,
+                               "kind": "port", "cpu_model": cpu_model(),
+                               "sample": f"{n_cpu} requests of the stream (strided; "
+                                         f"{n_conv_cpu} converged), oracle/liboracle.so with "
+                                         f"{thr} OpenMP threads",
                                "wall_s": t_cpu}
+        out["roots_vs_oracle"] = {"compared": n_cmp, "max_rel_err": err, "tolerance": 1e-10,
+                                  "same_convergence_status": f"{st_agree}/{n_cpu}",
+                                  "max_iteration_solves_compared": n_mi,
+                                  "max_iteration_max_rel_dev": err_mi}
+        out["speedup_vs_cpu"] = out["roots_per_sec"] / out["cpu_baseline"]["value"]
+    return out
+
+
+# the CPU-runnable scan: 2 species on 50 x 100, the root refined at
+# kperp d_p = 0.01, kpar d_p = 0.1, then followed over 64 log steps to kpar d_p = 1
+SCAN = dict(nperp=50, npar=100, kperp=0.01, kpar=0.1, kpar_end=1.0, guess=0.1 - 0.001j, steps=64)
+
+
+def run_scan(args):
+    """secant_osc from 0.1 - 0.001i to |det D| < 1e-12, then om_scan type 4
+    over 64 logarithmic kpar steps 0.1 -> 1.0 (ALPS_fns.f90:1919-2101,
+    2198-2600): roots/s (converged), per-step latency, the plain path (one
+    disp launch per frequency, stepped on the host), the oracle's own run of
+    the same scan as the CPU baseline, and every root against it."""
+    import math
+    from alps_amd.engine import Engine
+    p = make_plasma(SCAN["nperp"], SCAN["npar"])
+    n = SCAN["steps"]
+    diff = (math.log10(SCAN["kpar_end"]) - math.log10(SCAN["kpar"])) / n
+    E = Engine(p)
+
+    def one():
+        E.set_k(SCAN["kperp"], SCAN["kpar"])
+        t0 = time.perf_counter()
+        om, dfin, it0 = E.solve_batch([SCAN["guess"]])
+        t_ref = (time.perf_counter() - t0) * 1e3
+        kp, kz, roots = E.om_scan(4, True, diff, 0.0, n, 1, [om[0]])
+        wall = time.perf_counter() - t0
+        ms, its = E.last_scan_steps(n, 1)
+        steps_ms = np.diff(np.concatenate([[0.0], ms]))
+        conv = int(it0[0] < p.roots.numiter) + int(np.sum((its >= 0) & (its < p.roots.numiter)))
+        return dict(wall=wall, refine_ms=t_ref, steps_ms=steps_ms, roots=roots[:, 0], kpar=kz,
+                    converged=conv)
+
+    reps = max(1, args.scan_repeats)
+    one()  # warm-up: graphs and launch shapes
+    runs = [one() for _ in range(reps)]
+    wall = sum(r["wall"] for r in runs)
+    n_conv = sum(r["converged"] for r in runs)
+    step_ms = np.concatenate([r["steps_ms"] for r in runs])
+    E.set_plain_roots(True)
+    one()
+    plain = [one() for _ in range(max(1, reps // 2))]
+    E.set_plain_roots(False)
+    plain_rate = sum(r["converged"] for r in plain) / sum(r["wall"] for r in plain)
+    plain_dev = max(float(np.max(np.abs(r["roots"] - runs[-1]["roots"]) / np.abs(runs[-1]["roots"])))
+                    for r in plain)
+    out = {
+        "workload": "refine one root at kperp d_p = 0.01, kpar d_p = 0.1 from 0.1-0.001i, then "
+                    "om_scan type 4 over 64 log kpar steps to 1.0; 2 species on 50 x 100",
+        "scans": reps,
+        "roots_per_scan": n + 1,
+        "roots_per_sec": n_conv / wall,
+        "converged": n_conv,
+        "scan_ms": wall * 1e3 / reps,
+        "refine_ms": float(np.mean([r["refine_ms"] for r in runs])),
+        "step_latency_p50_ms": float(np.percentile(step_ms, 50)),
+        "step_latency_p99_ms": float(np.percentile(step_ms, 99)),
+        "plain_roots_per_sec": plain_rate,
+        "plain_scan_ms": 1e3 * sum(r["wall"] for r in plain) / len(plain),
+        "max_rel_dev_plain_vs_batched": plain_dev,
+        "final_root": [runs[-1]["roots"][-1].real, runs[-1]["roots"][-1].imag],
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from oracle import Oracle
+        thr = min(args.cpu_threads, os.cpu_count() or 1)
+        O = Oracle(p, n_threads=thr)
+        tc = time.perf_counter()
+        O.set_k(SCAN["kperp"], SCAN["kpar"])
+        w0, it0 = O.secant_osc(SCAN["guess"])
+        rc, _, _, oroots = O.om_scan(4, True, diff, 0.0, n, 1, [w0])
+        t_cpu = time.perf_counter() - tc
+        g = runs[-1]["roots"]
+        out["cpu_baseline"] = {"value": (n + 1) / t_cpu, "unit": "roots/s", "cores": thr,
+                               "kind": "port", "cpu_model": cpu_model(),
+                               "sample": f"the whole scan ({n + 1} roots), oracle/liboracle.so "
+                                         f"with {thr} OpenMP threads",
+                               "wall_s": t_cpu}
+        out["roots_vs_oracle"] = {
+            "compared": n + 1,
+            "max_rel_err": float(np.max(np.abs(g - oroots[:, 0]) / np.abs(oroots[:, 0]))),
+            "tolerance": 1e-10}
         out["speedup_vs_cpu"] = out["roots_per_sec"] / out["cpu_baseline"]["value"]
     return out
 
@@ -392,11 +552,15 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (torch) and exit: the "
                          "known-size kernel that scales the profiler's FETCH_SIZE/WRITE_SIZE")
-    ap.add_argument("--workload", choices=["map", "serve", "serve_rel"], default="map",
+    ap.add_argument("--workload", choices=["map", "serve", "serve_rel", "scan"], default="map",
                     help="map: the batched map search (default); serve: the Poisson stream "
                          "of independent root requests through the root server; serve_rel: "
-                         "the relativistic kpar-scan requests through the root server")
+                         "the relativistic kpar-scan requests through the root server; scan: "
+                         "one root refined and followed over 64 log kpar steps (50 x 100)")
     ap.add_argument("--requests", type=int, default=4096)
+    ap.add_argument("--rel-requests", type=int, default=1024,
+                    help="serve_rel: relativistic kpar-scan requests")
+    ap.add_argument("--scan-repeats", type=int, default=10, help="scan: timed repeats")
     ap.add_argument("--variants", type=int, default=64, help="plasma variants in the request pool")
     ap.add_argument("--rate", type=float, default=20000.0, help="request arrivals per second")
     ap.add_argument("--slots", type=int, default=4096)
@@ -431,6 +595,9 @@ def main():
         return
     if args.workload == "serve_rel":
         print(json.dumps(run_serve_rel(args)))
+        return
+    if args.workload == "scan":
+        print(json.dumps({"workload": "scan", **run_scan(args)}))
         return
 
     if args.pmc_shapes:
@@ -488,11 +655,14 @@ def main():
     wall = t1 - t0
     ms_step = wall * 1e3 / args.steps
     n_roots = sum(r["roots"].size for r in res)
-    lat = np.concatenate([r["latency_ms"] for r in res]) if n_roots else np.zeros(1)
+    # a root counts when its solve converged (|D| < D_threshold before numiter)
+    n_conv = sum(int(np.sum(r["iters"] < p.roots.numiter)) for r in res)
+    lat = np.concatenate([r["latency_ms"][r["iters"] < p.roots.numiter] for r in res]) \
+        if n_conv else np.zeros(1)
     kern = np.array([r["map_kernel_ms"] for r in res])
     split = np.array([r["split_ms"] for r in res]).mean(axis=0)
     last = res[-1]
-    conv = int(np.sum(np.abs(last["dfin"]) < p.roots.D_threshold)) if last["roots"].size else 0
+    conv = int(np.sum(last["iters"] < p.roots.numiter))
 
     # roofline of the dominant kernel: k_sweep of the map batch (HIP events
     # around it on the engine's stream, averaged over the timed steps)
@@ -525,8 +695,9 @@ def main():
                                "kperp d_p = kpar d_p = 0.5",
                    "nmax": nmax, "map": [args.nr, args.ni], "omega_re": [MAP["omi"], MAP["omf"]],
                    "omega_im": [MAP["gami"], MAP["gamf"]], "D_threshold": p.roots.D_threshold},
-        "roots_per_sec": n_roots / wall,
+        "roots_per_sec": n_conv / wall,
         "roots_per_step": n_roots / args.steps,
+        "roots_converged_per_step": n_conv / args.steps,
         "roots_converged_last_step": conv,
         "root_latency_p50_ms": float(np.percentile(lat, 50)),
         "root_latency_p99_ms": float(np.percentile(lat, 99)),
@@ -546,16 +717,34 @@ def main():
                      "kernel_ms": float(split[0])},
         "cpu_baseline": None,
     }
+    progress(f"map: {out['value']:.4g} evals/s, {out['roots_per_sec']:.4g} roots/s")
     if not args.no_cpu:
-        cb, err = cpu_baseline(p, last["om"], last["cal"], args.cpu_sample,
-                               min(args.cpu_threads, os.cpu_count() or 1))
+        thr = min(args.cpu_threads, os.cpu_count() or 1)
+        cb, err = cpu_baseline(p, last["om"], last["cal"], args.cpu_sample, thr)
         out["cpu_baseline"] = cb
         out["parity_vs_oracle_on_sample"] = err
+        # refined roots of the last step against the oracle's secant_osc from
+        # the same minima (two of them: the oracle takes ~1 s per root here)
+        pick = np.linspace(0, last["roots"].size - 1, min(2, last["roots"].size)).astype(int)
+        if pick.size:
+            w, it = oracle_roots(p, KPERP, KPAR, last["guesses"][pick], thr)
+            g = last["roots"][pick]
+            out["roots_vs_oracle"] = {
+                "compared": int(pick.size),
+                "max_rel_err": float(np.max(np.abs(g - w) / np.abs(w))),
+                "same_iterations": bool(np.all(it == last["iters"][pick])),
+                "tolerance": 1e-10}
     if not args.no_serve:
         # served root solves beside the map line: 4096 independent requests
+        progress("map: oracle checks done")
         out["serve"] = run_serve(args)
+        progress(f"serve: {out['serve']['roots_per_sec']:.4g} roots/s")
         # and relativistic kpar-scan requests through the same server
         out["serve_relativistic"] = run_serve_rel(args)
+        progress(f"serve_relativistic: {out['serve_relativistic']['roots_per_sec']:.4g} roots/s")
+        # the CPU-runnable scan: one root refined, then 64 log kpar steps
+        out["scan"] = run_scan(args)
+        progress(f"scan: {out['scan']['roots_per_sec']:.4g} roots/s")
     print(json.dumps(out))
 
 

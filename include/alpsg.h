@@ -23,6 +23,10 @@ extern "C" {
 #define ALPSG_E_DEVICE 3      /* HIP runtime error */
 #define ALPSG_E_STATE 4       /* call order (e.g. disp before set_k) */
 #define ALPSG_E_NOTFOUND 5    /* no root survived a scan (reference alps_error(9)) */
+#define ALPSG_MAXITER 6       /* a served solve used up numiter iterations without
+                                 |D| < D_threshold; its root is the best point
+                                 found ("Maximum iteration reached",
+                                 ALPS_fns.f90:2093-2099) */
 
 #define ALPSG_MAXSPEC 8
 #define ALPSG_MAXFITS 4
@@ -233,6 +237,21 @@ long long alpsg_eval_count(alpsg_handle h);
    at which each of its nroots roots finished its secant iterations. */
 int alpsg_last_root_times(alpsg_handle h, int nroots, double *ms_out);
 
+/* The last alpsg_om_scan on this handle, per wavevector step (nsteps =
+   n_out*n_res): ms_out [nsteps] the wall time (ms from the scan's start) at
+   which the step had solved all its roots; iters_out [nsteps*nroots] the
+   iteration count of each root's solve at that step (numiter or more: the
+   solve ended without |D| < D_threshold), -1 for a root no longer followed.
+   Either may be NULL. */
+int alpsg_last_scan_steps(alpsg_handle h, int nsteps, int nroots, double *ms_out, int *iters_out);
+
+/* With plain != 0, the root searches of this handle (alpsg_secant_batch,
+   alpsg_om_scan, alpsg_om_double_scan) step every
+   solve on the host with one disp launch per frequency -- the plain form,
+   timed for comparison -- instead of stepping the secant_osc machines on the
+   device between the launches of a replayed graph.  The roots are the same. */
+int alpsg_set_plain_roots(alpsg_handle h, int plain);
+
 /* Device time (ms) of the last alpsg_disp_batch / alpsg_map_search kernel
    sweep, measured with HIP events on the handle's stream. */
 double alpsg_last_kernel_ms(alpsg_handle h);
@@ -269,7 +288,9 @@ int alpsg_last_kernel_split(alpsg_handle h, double *ms3);
 
    alpsg_server_poll returns finished solves (waits for one when wait != 0 and
    any are outstanding): tags [max_results], roots [2*max_results], iters,
-   status (ALPSG_OK, or ALPSG_E_ARG for a request that cannot fit the arena,
+   status (ALPSG_OK for a converged solve, ALPSG_MAXITER for one that ended at
+   numiter with the best point as its root, or ALPSG_E_ARG for a request that
+   cannot fit the arena,
    whose Bessel cutoff is not reached, or whose relativistic principal-value
    window leaves the subluminal cone, the reference's alps_error(8)) and
    latency_ms (submit to result);

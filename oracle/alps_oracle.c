@@ -340,8 +340,33 @@ static double complex fit_function_poly(const orc_plasma *P, int is, int iperp,
   return res;
 }
 
-/* eval_fit, ALPS_analyt.f90:32-181 (ACmethod 1 and 2). */
+/* distribution_analyt, distribution/distribution_analyt.f90:30-96 as shipped:
+   species 1 and 2 are Maxwellians with beta = 1 and masses 1 and 1/1836.
+   Other species are left undefined by the source (f0 is never assigned);
+   they give NaN here. */
+double complex orc_distribution_analyt(int is, double pperp, double complex ppar) {
+  const double pi = atan(1.0) * 4.0;
+  double beta, ms;
+  switch (is) {
+    case 1:
+      beta = 1.0;
+      ms = 1.0;
+      break;
+    case 2:
+      beta = 1.0;
+      ms = 1.0 / 1836.0;
+      break;
+    default:
+      return NAN;
+  }
+  return (pow(pi, -1.5) / pow(ms * beta, 3.0 / 2.0)) *
+         cexp(-(ppar * ppar / (beta * ms) + (pperp * pperp) / (beta * ms)));
+}
+
+/* eval_fit, ALPS_analyt.f90:32-181. */
 double complex orc_eval_fit(const orc_plasma *P, int is, int iperp, double complex ppar_valC) {
+  if (P->ACmethod[is - 1] == 0) /* ALPS_analyt.f90:77-79 */
+    return orc_distribution_analyt(is, PP(P, is, iperp, 1, 1), ppar_valC);
   if (P->ACmethod[is - 1] == 2) return fit_function_poly(P, is, iperp, ppar_valC);
   /* pperp_val is set per fit in order; the last fit's choice is used for all
      (ALPS_analyt.f90:79-151): pperp for types 1-3 and 6, Gamma for 4 and 5 */
@@ -1024,27 +1049,43 @@ double complex orc_disp(orc_plasma *P, double complex om, double complex *chi0_o
  * Root finders: ALPS_fns.f90:1815-2195
  * ======================================================================== */
 
+/* The solvers take the function whose root they seek as an argument, as the
+   reference's rtsec(func, ...) does (ALPS_fns.f90:2105, 2151): disp for the
+   path, and closed-form test functions (orc_test_fn) for their pins. */
+static double complex disp_fn(void *ctx, double complex om) {
+  return orc_disp((orc_plasma *)ctx, om, NULL, NULL, NULL);
+}
+
+static orc_root_ctl ctl_of(const orc_plasma *P) {
+  orc_root_ctl c;
+  c.numiter = P->numiter;
+  c.D_threshold = P->D_threshold;
+  c.D_prec = P->D_prec;
+  c.D_tol = P->D_tol;
+  return c;
+}
+
 /* secant, ALPS_fns.f90:1815-1917 */
-int orc_secant(orc_plasma *P, double complex *omp) {
+int orc_secant_fn(orc_fn fn, void *ctx, const orc_root_ctl *c, double complex *omp) {
   double complex om = *omp, prevom, D, Dprev, jump, minom, minD;
   int iter, go = 1;
-  prevom = om * (1.0 - P->D_prec);
-  Dprev = orc_disp(P, prevom, NULL, NULL, NULL);
-  minD = orc_disp(P, om, NULL, NULL, NULL);
+  prevom = om * (1.0 - c->D_prec);
+  Dprev = fn(ctx, prevom);
+  minD = fn(ctx, om);
   minom = om;
   if (cabs(Dprev) < cabs(minD)) {
     minom = prevom;
     minD = Dprev;
   }
   iter = 0;
-  while (iter <= P->numiter - 1 && go) {
+  while (iter <= c->numiter - 1 && go) {
     iter = iter + 1;
-    D = orc_disp(P, om, NULL, NULL, NULL);
+    D = fn(ctx, om);
     if (cabs(D - Dprev) < 1.e-80) {
       prevom = prevom + 1.e-8;
-      Dprev = orc_disp(P, prevom, NULL, NULL, NULL);
+      Dprev = fn(ctx, prevom);
     }
-    if (cabs(D) < P->D_threshold) {
+    if (cabs(D) < c->D_threshold) {
       jump = 0.0;
       go = 0;
     } else {
@@ -1058,14 +1099,19 @@ int orc_secant(orc_plasma *P, double complex *omp) {
       minD = D;
     }
   }
-  if (iter >= P->numiter) om = minom;
+  if (iter >= c->numiter) om = minom;
   *omp = om;
   return iter;
 }
 
+int orc_secant(orc_plasma *P, double complex *omp) {
+  const orc_root_ctl c = ctl_of(P);
+  return orc_secant_fn(disp_fn, P, &c, omp);
+}
+
 /* secant_osc, ALPS_fns.f90:1919-2101.  lambda = 0.1 and osc_threshold = 1.E-3
    are default-real literals; 0.1*abs(om) multiplies by the float literal 0.1. */
-int orc_secant_osc(orc_plasma *P, double complex *omp) {
+int orc_secant_osc_fn(orc_fn fn, void *ctx, const orc_root_ctl *c, double complex *omp) {
   const double complex delta = 1.e-6 + 1.e-8 * I;
   const double lambda = (double)0.1f;
   const double osc_threshold = (double)1.e-3f;
@@ -1075,14 +1121,14 @@ int orc_secant_osc(orc_plasma *P, double complex *omp) {
   double damping_factor;
   int iter, go, oscillation_count;
 
-  D = orc_disp(P, om, NULL, NULL, NULL);
+  D = fn(ctx, om);
   minom = om;
   minD = D;
-  prevom = om * (1.0 - P->D_prec);
+  prevom = om * (1.0 - c->D_prec);
   prev2om = om;
   prev3om = om;
   prev4om = om;
-  prevD = orc_disp(P, prevom, NULL, NULL, NULL);
+  prevD = fn(ctx, prevom);
   prev2D = D;
   prev3D = D;
   prev4D = D;
@@ -1097,14 +1143,14 @@ int orc_secant_osc(orc_plasma *P, double complex *omp) {
   go = 1;
   damping_factor = 1.0;
   oscillation_count = 0;
-  while (iter <= P->numiter - 1 && go) {
+  while (iter <= c->numiter - 1 && go) {
     iter = iter + 1;
-    D = orc_disp(P, om, NULL, NULL, NULL);
+    D = fn(ctx, om);
     if (cabs(D - prevD) < 1.e-80) {
       prevom = prevom + 1.e-8;
-      prevD = orc_disp(P, prevom, NULL, NULL, NULL);
+      prevD = fn(ctx, prevom);
     }
-    if (cabs(D) < P->D_threshold) {
+    if (cabs(D) < c->D_threshold) {
       jump = 0.0;
       go = 0;
     } else {
@@ -1118,8 +1164,8 @@ int orc_secant_osc(orc_plasma *P, double complex *omp) {
 #undef CLOSE
       }
       if (oscillation_count > 1) {
-        double complex dp = orc_disp(P, om * (1.0 + delta), NULL, NULL, NULL);
-        double complex dm = orc_disp(P, om * (1.0 - delta), NULL, NULL, NULL);
+        double complex dp = fn(ctx, om * (1.0 + delta));
+        double complex dm = fn(ctx, om * (1.0 - delta));
         Dprime = (dp - dm) / (2.0 * om * delta);
         jump = D / (Dprime + lambda * D);
       } else {
@@ -1142,18 +1188,24 @@ int orc_secant_osc(orc_plasma *P, double complex *omp) {
       om = om - jump;
     }
   }
-  if (iter >= P->numiter) om = minom;
+  if (iter >= c->numiter) om = minom;
   *omp = om;
   return iter;
 }
 
+int orc_secant_osc(orc_plasma *P, double complex *omp) {
+  const orc_root_ctl c = ctl_of(P);
+  return orc_secant_osc_fn(disp_fn, P, &c, omp);
+}
+
 /* rtsec, ALPS_fns.f90:2105-2195 */
-double complex orc_rtsec(orc_plasma *P, double complex xin, int *iflag) {
+double complex orc_rtsec_fn(orc_fn fn, void *ctx, const orc_root_ctl *c, double complex xin,
+                            int *iflag) {
   double complex x1, x2, xl, fl, f, swap, dx, rts;
   x1 = xin * 1.0;
-  x2 = xin * (1.0 + P->D_prec);
-  fl = orc_disp(P, x1, NULL, NULL, NULL);
-  f = orc_disp(P, x2, NULL, NULL, NULL);
+  x2 = xin * (1.0 + c->D_prec);
+  fl = fn(ctx, x1);
+  f = fn(ctx, x2);
   if (cabs(fl) < cabs(f)) {
     rts = x1;
     xl = x2;
@@ -1164,7 +1216,7 @@ double complex orc_rtsec(orc_plasma *P, double complex xin, int *iflag) {
     xl = x1;
     rts = x2;
   }
-  for (int j = 1; j <= P->numiter - 1; ++j) {
+  for (int j = 1; j <= c->numiter - 1; ++j) {
     *iflag = j;
     if (cabs(f - fl) > 1.e-40)
       dx = (xl - rts) * f / (f - fl);
@@ -1173,10 +1225,52 @@ double complex orc_rtsec(orc_plasma *P, double complex xin, int *iflag) {
     xl = rts;
     fl = f;
     rts = rts + dx / 2.0;
-    f = orc_disp(P, rts, NULL, NULL, NULL);
-    if (cabs(dx) < P->D_tol || cabs(f) == 0.0) return rts;
+    f = fn(ctx, rts);
+    if (cabs(dx) < c->D_tol || cabs(f) == 0.0) return rts;
   }
   return rts;
+}
+
+double complex orc_rtsec(orc_plasma *P, double complex xin, int *iflag) {
+  const orc_root_ctl c = ctl_of(P);
+  return orc_rtsec_fn(disp_fn, P, &c, xin, iflag);
+}
+
+/* Closed-form test functions for the solvers' pins (test infrastructure):
+   kind 0: a (om - r1); kind 1: a (om^2 - c); kind 2: a (om - r1)(om - r2) /
+   (om - pole); kind 3: a exp(om) (no root).  Every argument is recorded in
+   the trajectory buffer (up to cap entries; n counts all calls). */
+double complex orc_test_fn(void *ctx, double complex om) {
+  orc_test_ctx *t = (orc_test_ctx *)ctx;
+  const double complex a = t->a[0] + t->a[1] * I, r1 = t->r1[0] + t->r1[1] * I;
+  const double complex r2 = t->r2[0] + t->r2[1] * I, pole = t->pole[0] + t->pole[1] * I;
+  if (t->n < t->cap) {
+    t->traj[2 * t->n] = creal(om);
+    t->traj[2 * t->n + 1] = cimag(om);
+  }
+  t->n += 1;
+  switch (t->kind) {
+    case 0: return a * (om - r1);
+    case 1: return a * (om * om - r2);
+    case 2: return a * (om - r1) * (om - r2) / (om - pole);
+    default: return a * cexp(om);
+  }
+}
+
+/* one solve of a test function: method 0 secant, 1 rtsec, 2 secant_osc;
+   om [2] in/out; returns the iteration count (rtsec: iflag) */
+int orc_solve_test(int method, orc_test_ctx *t, const orc_root_ctl *c, double *om) {
+  double complex w = om[0] + om[1] * I;
+  int it = 0;
+  t->n = 0;
+  switch (method) {
+    case 0: it = orc_secant_fn(orc_test_fn, t, c, &w); break;
+    case 1: w = orc_rtsec_fn(orc_test_fn, t, c, w, &it); break;
+    default: it = orc_secant_osc_fn(orc_test_fn, t, c, &w); break;
+  }
+  om[0] = creal(w);
+  om[1] = cimag(w);
+  return it;
 }
 
 static int orc_solve(orc_plasma *P, double complex *om) {

@@ -133,6 +133,8 @@ double complex orc_funct_g(const orc_plasma *P, int sproc, double ppar_real, int
                            double complex om, int nn, int mode);
 double complex orc_int_ee(const orc_plasma *P, int sproc);
 double complex orc_eval_fit(const orc_plasma *P, int is, int iperp, double complex ppar_valC);
+/* distribution_analyt (distribution/distribution_analyt.f90:30-96), as shipped */
+double complex orc_distribution_analyt(int is, double pperp, double complex ppar);
 
 /* disp(om): det D; optional chi0 [nspec*9], chi0_low [nspec*27] (n = -1,0,1), wave [9],
    all row-major [s][i][j] / [s][i][j][n+1]. */
@@ -172,10 +174,34 @@ void orc_calc_eigen(orc_plasma *P, double complex omega, const double complex *c
                     double complex *bf, double complex *Us, double complex *ds, double *Ps,
                     double *Ps_split, double *ewave);
 
-/* Root finders (ALPS_fns.f90:1815-2195). Return the number of iterations. */
+/* Root finders (ALPS_fns.f90:1815-2195). Return the number of iterations.
+   The *_fn forms take the function whose root they seek, as the reference's
+   rtsec(func, ...) does (ALPS_fns.f90:2105); the plain forms pass disp. */
+typedef double complex (*orc_fn)(void *ctx, double complex om);
+typedef struct {
+  int numiter;
+  double D_threshold, D_prec, D_tol;
+} orc_root_ctl;
+int orc_secant_fn(orc_fn fn, void *ctx, const orc_root_ctl *c, double complex *om);
+int orc_secant_osc_fn(orc_fn fn, void *ctx, const orc_root_ctl *c, double complex *om);
+double complex orc_rtsec_fn(orc_fn fn, void *ctx, const orc_root_ctl *c, double complex xin,
+                            int *iflag);
 int orc_secant(orc_plasma *P, double complex *om);
 int orc_secant_osc(orc_plasma *P, double complex *om);
 double complex orc_rtsec(orc_plasma *P, double complex xin, int *iflag);
+
+/* Closed-form test functions of the solvers' pins (test infrastructure):
+   kind 0 a (om - r1), kind 1 a (om^2 - r2), kind 2 a (om - r1)(om - r2) /
+   (om - pole), kind 3 a exp(om); complex parameters as (re, im).  Each call
+   records its argument in traj [2*cap] and counts itself in n. */
+typedef struct {
+  int kind;
+  double a[2], r1[2], r2[2], pole[2];
+  int n, cap;
+  double *traj;
+} orc_test_ctx;
+double complex orc_test_fn(void *ctx, double complex om);
+int orc_solve_test(int method, orc_test_ctx *t, const orc_root_ctl *c, double *om);
 
 /* Map search (ALPS_fns.f90:3595-3966).  cal/val are [nr][ni] row-major.
    iroots is [numroots][2] (ir, ii, 1-based).  Returns the number of minima recorded. */

@@ -56,6 +56,41 @@ class _C128(C.Structure):
     _fields_ = [("re", C.c_double), ("im", C.c_double)]
 
 
+class _RootCtl(C.Structure):
+    _fields_ = [("numiter", C.c_int), ("D_threshold", C.c_double), ("D_prec", C.c_double),
+                ("D_tol", C.c_double)]
+
+
+class _TestCtx(C.Structure):
+    _fields_ = [("kind", C.c_int), ("a", C.c_double * 2), ("r1", C.c_double * 2),
+                ("r2", C.c_double * 2), ("pole", C.c_double * 2), ("n", C.c_int),
+                ("cap", C.c_int), ("traj", C.POINTER(C.c_double))]
+
+
+TEST_KINDS = {"linear": 0, "quadratic": 1, "rational": 2, "exp": 3}
+SOLVERS = {"secant": 0, "rtsec": 1, "secant_osc": 2}
+
+
+def solve_test(method, kind, om, a=1.0, r1=0.0, r2=0.0, pole=0.0, numiter=50,
+               D_threshold=1e-12, D_prec=1e-5, D_tol=1e-12, cap=4096):
+    """One root solve of a closed-form test function by the restated secant,
+    rtsec or secant_osc (orc_solve_test): kind "linear" a (w - r1),
+    "quadratic" a (w^2 - r2), "rational" a (w - r1)(w - r2)/(w - pole), "exp"
+    a exp(w).  Returns (root, iterations, the arguments of every function
+    call in order)."""
+    t = _TestCtx()
+    t.kind = TEST_KINDS[kind]
+    for name, v in (("a", a), ("r1", r1), ("r2", r2), ("pole", pole)):
+        getattr(t, name)[0], getattr(t, name)[1] = complex(v).real, complex(v).imag
+    buf = np.zeros(2 * cap)
+    t.cap, t.traj = cap, _ptr(buf)
+    c = _RootCtl(numiter, D_threshold, D_prec, D_tol)
+    w = np.array([complex(om).real, complex(om).imag])
+    it = lib().orc_solve_test(SOLVERS[method], C.byref(t), C.byref(c), _ptr(w))
+    n = min(t.n, cap)
+    return complex(w[0], w[1]), it, buf[: 2 * n:2] + 1j * buf[1: 2 * n:2]
+
+
 def build(force: bool = False) -> str:
     """Compile liboracle.so with the Makefile beside this file."""
     if force or not os.path.exists(_LIB):
@@ -129,6 +164,8 @@ def lib():
         L.orc_current_int.argtypes = [P]
         L.orc_calc_eigen.restype = None
         L.orc_calc_eigen.argtypes = [P, _C128, dp, dp, C.c_int, C.c_int, dp, dp, dp, dp, dp, dp, dp]
+        L.orc_solve_test.argtypes = [C.c_int, C.POINTER(_TestCtx), C.POINTER(_RootCtl), dp]
+        L.orc_solve_test.restype = C.c_int
         L.orc_rtsec.restype = _C128
         L.orc_rtsec.argtypes = [P, _C128, C.POINTER(C.c_int)]
         L.orc_map_search.restype = C.c_int

@@ -56,6 +56,20 @@ def poly_case(nperp=24, npar=48, order=8):
                   poly_fit_coeffs=coeffs)
 
 
+def poly_row0_clipped(nperp=24, npar=48, order=8):
+    """poly_case with the Chebyshev row at pperp = 0 replaced by a constant
+    log10 f = -30, below -poly_log_max: the continuation is exactly 0 on that
+    row only.  landau_integrate's zero test looks at fpar_f twice and never at
+    fperp_f = eval_fit(iperp - 1, p_res) (ALPS_fns.f90:1404-1405), so at
+    iperp = 1 the zero on row 0 does not end the sum early."""
+    import dataclasses
+    p = poly_case(nperp, npar, order)
+    c = p.poly_fit_coeffs.copy()
+    c[0, 0, :] = 0.0
+    c[0, 0, 0] = -30.0
+    return dataclasses.replace(p, poly_fit_coeffs=c)
+
+
 def bm_mix(kperp_norm=True, cold=True):
     """Closed-form (use_bM) species beside a tabulated one: an anisotropic
     drifting bi-Maxwellian proton population and, with ``cold``, a cold

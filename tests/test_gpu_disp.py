@@ -68,9 +68,17 @@ def test_kappa_fits():
     _check(p, 0.3, 0.3, cases.BRANCH_OMEGAS)
 
 
-def test_chebyshev_continuation():
-    p = cases.poly_case()
+@pytest.mark.parametrize("case", ["poly_case", "poly_row0_clipped"])
+def test_chebyshev_continuation(case):
+    """Chebyshev continuation, including the Landau early return
+    (ALPS_fns.f90:1404-1407; the oracle's counter shows it is taken) and, with
+    the pperp = 0 row clipped to zero, the zero at row iperp - 1 that the
+    source does not test."""
+    from oracle import branch_counts, reset_branch_counts
+    p = getattr(cases, case)()
+    reset_branch_counts()
     _check(p, 0.3, 0.3, cases.BRANCH_OMEGAS)
+    assert branch_counts()["landau_zero"] > 0
 
 
 @pytest.mark.parametrize("kperp_norm,cold", [(True, True), (False, False)])

@@ -105,22 +105,128 @@ def test_derivatives_match_gaussian():
             exact2 = -2 * pa[j] / a * f[i, j]
             assert abs(df[0, i, j, 0] - exact1) <= bound(pe[i], b, hp, f[i, j])
             assert abs(df[0, i, j, 1] - exact2) <= bound(pa[j], a, hq, f[i, j])
-    # near the centre the stencil is accurate far beyond a factor-2 slip
-    assert abs(df[0, 20, 64, 1] / (-2 * pa[64] / a * f[20, 64]) - 1) < 0.05 or abs(pa[64]) < 1e-12
+    # near the centre the central difference is g' + (h^2/6) g''' + O(h^4):
+    # with the exact h^2 term taken off, what is left is a few per cent of it
+    # (a one-sided stencil, a step of h instead of 2h or a sign slip leaves
+    # an error of the size of g' itself)
+    for j in (60, 63, 65, 66, 70):
+        x = pa[j]
+        exact = -2 * x / a * f[20, j]
+        h2term = hq * hq / 6 * f[20, j] * (12 * x / a ** 2 - 8 * x ** 3 / a ** 3)
+        assert abs(h2term) > 1e-3 * abs(exact)
+        assert abs(df[0, 20, j, 1] - exact - h2term) <= 0.05 * abs(h2term), j
+    # the table is even in ppar, so the centre derivative is exactly zero
+    assert pa[64] == 0.0 and df[0, 20, 64, 1] == 0.0
 
 
 def test_int_ee_closed_form():
-    """int_ee -> (2 pi q/m) norm sqrt(pi a) (a - b)/2 for f = norm exp(-ppar^2/a - pperp^2/b)."""
-    errs = []
-    for nperp, npar in [(64, 128), (128, 256)]:
-        p = _proton(nperp, npar, alph=2.0)
-        O = Oracle(p)
-        got = O.int_ee(1).real
-        a, b = 1.0, 2.0
-        norm = math.pi ** -1.5 / 2.0
-        ref = 2 * math.pi * norm * math.sqrt(math.pi * a) * (a - b) / 2
-        errs.append(abs(got / ref - 1))
-    assert errs[1] < 0.03 and errs[1] < errs[0], errs
+    """int_ee -> (2 pi q/m) norm sqrt(pi a) (a - b)/2 for f = norm exp(-ppar^2/a - pperp^2/b).
+    The sum is a trapezoid rule whose pperp = 0 row is zero (the integrand has
+    a factor pperp there), so the error is second order: it falls 4x per
+    grid doubling, and the Richardson value (4 I(2h) - I(h))/3 meets the
+    closed form to 1e-7 at 256 x 512 -> 512 x 1024.  A dropped factor 2 or 2 pi,
+    a wrong q/m, or an integrand with the wrong sign or power misses both."""
+    a, b = 1.0, 2.0
+    norm = math.pi ** -1.5 / 2.0
+    ref = 2 * math.pi * norm * math.sqrt(math.pi * a) * (a - b) / 2
+    vals = {}
+    for n in (64, 128, 256, 512):
+        vals[n] = Oracle(_proton(n, 2 * n, alph=2.0)).int_ee(1).real
+    errs = [vals[n] / ref - 1 for n in (64, 128, 256, 512)]
+    for e0, e1 in zip(errs, errs[1:]):
+        assert 3.9 < e0 / e1 < 4.1, errs
+    assert abs(errs[-1]) < 5e-5
+    rich = (4 * vals[512] - vals[256]) / 3
+    assert abs(rich / ref - 1) < 1e-7, rich / ref - 1
+
+
+def test_int_ee_weights_by_delta_probes():
+    """int_ee is linear in df0: a table that is zero but for one entry gives
+    that entry's weight.  Every (iperp, ipar, component) of a 6 x 8 grid is
+    probed against the trapezoid weights of ALPS_fns.f90:1481-1549 -- w_perp 2
+    on rows 1..nperp-2 (the row at iperp = 1 also stands for the missing
+    pperp = 0 half cell) and 1 on row nperp-1, w_par 1 on the end columns and
+    2 inside -- times the integrand ppar (pperp d_par f - ppar d_perp f), whose
+    second product at the (1,1) corner reads pperp instead of ppar
+    (ALPS_fns.f90:1483-1486), times (2 pi q/m) dpperp dppar / 4."""
+    nperp, npar = 6, 8
+    p = _proton(nperp, npar, alph=2.0)
+    O = Oracle(p)
+    pe, pa = p.pp[0, :, 0, 0], p.pp[0, 0, :, 1]
+    c = 2 * math.pi * p.qs[0] / p.ms[0] * (pe[1] - pe[0]) * (pa[1] - pa[0]) / 4
+    df = O._df0
+    n_probe = 0
+    for ip in range(1, nperp):
+        for iq in range(1, npar):
+            wp = 1.0 if ip == nperp - 1 else 2.0
+            wq = 1.0 if iq in (1, npar - 1) else 2.0
+            for k in (0, 1):
+                df[:] = 0.0
+                df[0, ip, iq, k] = 1.0
+                got = O.int_ee(1)
+                if k == 1:   # d_par f multiplies ppar * pperp
+                    coef = pa[iq] * pe[ip]
+                elif (ip, iq) == (1, 1):   # the corner's second product
+                    coef = -pa[iq] * pe[ip]
+                else:        # d_perp f multiplies -ppar^2
+                    coef = -pa[iq] * pa[iq]
+                want = c * wp * wq * coef
+                assert got.imag == 0.0
+                assert abs(got.real - want) <= 1e-14 * abs(want), (ip, iq, k, got, want)
+                n_probe += 1
+    assert n_probe == 2 * (nperp - 1) * (npar - 1)
+    # the rows and columns outside 1..nperp-1 x 1..npar-1 carry no weight
+    for ip, iq in ((0, 3), (nperp, 3), (2, 0), (2, npar)):
+        df[:] = 0.0
+        df[0, ip, iq, :] = 1.0
+        assert O.int_ee(1) == 0.0
+
+
+def _landau_zero_expected(O, p, om, n, kpar):
+    """The early-return test of landau_integrate (ALPS_fns.f90:1404-1407),
+    evaluated here from eval_fit (pinned by the table tests above): a zero
+    continuation at p_res +- dppar on a row iperp = 1..nperp-1 or at p_res on
+    row iperp + 1.  Row iperp - 1 is not tested by the source."""
+    pa = p.pp[0, 0, :, 1]
+    dq = abs(pa[1] - pa[0])
+    pr = (p.ms[0] * om - n * p.qs[0]) / kpar
+    hit = any(O.eval_fit(1, i, pr + dq) == 0 or O.eval_fit(1, i, pr - dq) == 0
+              or O.eval_fit(1, i + 1, pr) == 0 for i in range(1, p.nperp))
+    below = any(O.eval_fit(1, i - 1, pr) == 0 for i in range(1, p.nperp))
+    return hit, below
+
+
+@pytest.mark.parametrize("case", ["poly_case", "poly_row0_clipped"])
+def test_landau_early_return(case):
+    """landau_integrate returns exactly 0 for a resonant harmonic iff a zero
+    continuation value appears where the source tests for one; harmonics
+    without one give a nonzero Landau term.  With the pperp = 0 row clipped to
+    zero, every harmonic sees a zero at row iperp - 1 = 0, which the source does
+    not test, and the term stays nonzero.  The oracle's branch counter agrees
+    with the number of zero returns."""
+    from oracle import branch_counts, reset_branch_counts
+    p = getattr(cases, case)()
+    O = Oracle(p)
+    O.set_k(KP, KZ)
+    n_zero = n_nonzero = n_below_only = 0
+    for om in cases.BRANCH_OMEGAS:
+        for n in range(-O.nmax[0], O.nmax[0] + 1):
+            if not O.resonances(1, om, n)[0]:
+                continue
+            hit, below = _landau_zero_expected(O, p, om, n, KZ)
+            reset_branch_counts()
+            L = O.landau(1, om, n, 3)   # zz: T_res ~ p_res^2 J_n^2, nonzero
+            assert branch_counts()["landau_zero"] == int(hit)
+            if hit:
+                assert L == 0, (om, n)
+                n_zero += 1
+            else:
+                assert L != 0 and np.isfinite(L), (om, n)
+                n_nonzero += 1
+                n_below_only += int(below)
+    assert n_zero >= 10 and n_nonzero >= 5
+    if case == "poly_row0_clipped":
+        assert n_below_only == n_nonzero   # every nonzero term had the untested zero
 
 
 def test_branch_omegas_reach_every_branch():
@@ -138,6 +244,14 @@ def test_branch_omegas_reach_every_branch():
     for name in ("nonres", "res_pos", "res_neg", "res_real", "edge_low", "edge_high", "pv_tlim",
                  "pv_analytic", "tiny_rest"):
         assert c[name] > 0, (name, c)
+    # the Landau early return, on the Chebyshev-continued species of the GPU tests
+    reset_branch_counts()
+    for p in (cases.poly_case(), cases.poly_row0_clipped()):
+        O = Oracle(p)
+        O.set_k(KP, KZ)
+        for om in cases.BRANCH_OMEGAS:
+            O.disp(om)
+    assert branch_counts()["landau_zero"] > 0
     reset_branch_counts()
     O = Oracle(cases.rel_pair())
     O.set_k(0.3, 0.1)
