@@ -2,7 +2,11 @@
 !! the ALPS driver would: alps_var-layout arrays in, disp/secant_osc calls out.
 !! The plasma is a proton-electron bi-Maxwellian tabulated with the formulas
 !! of generate_distribution.f90 (distributions=1) on 24 x 48 points, kperp =
-!! kpar = 0.3.  Prints nmax, det D at a few frequencies and one secant root.
+!! kpar = 0.3.  Prints nmax, det D at a few frequencies and one secant root;
+!! then, at kperp = kpar = 0.05, the om_double_scan of a kpar scan to 0.2
+!! (3 log steps) with a kperp scan to 0.2 (2 output steps of 2) from every
+!! outer point, in place of om_double_scan (ALPS.f90:126); and two solves on
+!! the root server with its statistics.
 program alpsg_demo
   use iso_c_binding
   use alpsg
@@ -20,6 +24,14 @@ program alpsg_demo
   real(c_double) :: beta, vA, maxP, pi, norm, pperp_max, ppar_max, dpperp, dppar, pperp, ppar
   real(c_double) :: pcx
   complex(c_double_complex) :: om(4), det, root
+  type(alpsg_scan_params) :: s1, s2
+  integer, parameter :: nrow = (3 + 1) * (2 + 1)
+  real(c_double) :: kpo(nrow), kzo(nrow), lat(2)
+  complex(c_double_complex) :: droots(1), rout(nrow), g(2), sroots(2)
+  integer(c_int) :: written(nrow), pid, iters(2), status(2), nout
+  integer(c_long_long) :: tags(2), counts(10)
+  real(c_double) :: times(4)
+  type(c_ptr) :: srv
 
   pi = atan(1.d0) * 4.d0
   beta = 1.d0; vA = 1.d-4; maxP = 6.d0
@@ -81,5 +93,49 @@ program alpsg_demo
   call alpsg_secant_osc(h, rp, root, rc)
   if (rc /= ALPSG_OK) stop 4
   write(*, '(a,2es25.16e3)') 'ROOT', root
+
+  ! om_double_scan from kperp = kpar = 0.05 (scan_read's log steps)
+  rc = alpsg_set_k(h, 0.05d0, 0.05d0, c_null_ptr, nmax)
+  if (rc /= ALPSG_OK) stop 5
+  rp%D_threshold = 1.d-14
+  droots(1) = (0.048d0, -0.0001d0)
+  call alpsg_secant_osc(h, rp, droots(1), rc)
+  if (rc /= ALPSG_OK) stop 6
+  s1 = alpsg_scan_params(4, 1, (log10(0.2d0) - log10(0.05d0)) / 3.d0, 0.d0, 3, 1)
+  s2 = alpsg_scan_params(3, 1, (log10(0.2d0) - log10(0.05d0)) / 4.d0, 0.d0, 2, 2)
+  rc = alpsg_om_double_scan(h, rp, s1, s2, 1_c_int, droots, 0_c_int, 0_c_int, kpo, kzo, rout, &
+       written, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr)
+  if (rc /= ALPSG_OK) then
+     write(*, '(a,i0,2a)') 'alpsg_om_double_scan failed ', rc, ': ', trim(alpsg_error_text())
+     stop 7
+  endif
+  do k = 1, nrow
+     write(*, '(a,i4,4es25.16e3,i3)') 'DSCAN', k, kpo(k), kzo(k), rout(k), written(k)
+  enddo
   rc = alpsg_destroy(h)
+
+  ! two secant_osc solves on the root server, then its counters
+  rc = alpsg_server_create(1.d9, 16_c_int, srv)
+  if (rc /= ALPSG_OK) stop 8
+  rc = alpsg_server_add_plasma(srv, d, pid)
+  if (rc /= ALPSG_OK) stop 9
+  g = [(0.29d0, -0.001d0), (0.5d0, -0.05d0)]
+  tags = [11_c_long_long, 12_c_long_long]
+  rc = alpsg_server_submit_batch(srv, 2_c_int, [pid, pid], [0.3d0, 0.3d0], [0.3d0, 0.3d0], g, rp, &
+       tags)
+  if (rc /= ALPSG_OK) stop 10
+  nout = 0
+  do while (nout < 2)
+     rc = alpsg_server_poll(srv, 2_c_int - nout, 1_c_int, tags(nout + 1:), sroots(nout + 1:), &
+          iters(nout + 1:), status(nout + 1:), lat(nout + 1:), k)
+     if (rc /= ALPSG_OK) stop 11
+     nout = nout + k
+  enddo
+  do k = 1, 2
+     write(*, '(a,i4,2es25.16e3,2i4)') 'SERVED', tags(k), sroots(k), iters(k), status(k)
+  enddo
+  rc = alpsg_server_stats(srv, counts, times)
+  if (rc /= ALPSG_OK) stop 12
+  write(*, '(a,10i12)') 'STATS', counts
+  rc = alpsg_server_destroy(srv)
 end program alpsg_demo

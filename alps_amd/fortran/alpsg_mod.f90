@@ -10,7 +10,8 @@ module alpsg
   private
 
   integer(c_int), parameter, public :: ALPSG_OK = 0, ALPSG_E_ARG = 1, &
-       ALPSG_E_UNSUPPORTED = 2, ALPSG_E_DEVICE = 3, ALPSG_E_STATE = 4, ALPSG_E_NOTFOUND = 5
+       ALPSG_E_UNSUPPORTED = 2, ALPSG_E_DEVICE = 3, ALPSG_E_STATE = 4, ALPSG_E_NOTFOUND = 5, &
+       ALPSG_MAXITER = 6
 
   type, bind(C), public :: alpsg_plasma_desc
      integer(c_int) :: nspec, nperp, npar
@@ -46,6 +47,14 @@ module alpsg
      real(c_double) :: D_threshold, D_prec, D_gap, D_tol
   end type alpsg_root_params
 
+  !! one &scan_input scan (scan_type, swlog, diff/diff2 as scan_read computes
+  !! them, ALPS_io.f90:482-548, ns, nres)
+  type, bind(C), public :: alpsg_scan_params
+     integer(c_int) :: scan_type, log_scan
+     real(c_double) :: diff, diff2
+     integer(c_int) :: n_out, n_res
+  end type alpsg_scan_params
+
   public :: alpsg_create, alpsg_destroy, alpsg_set_k, alpsg_disp_batch, alpsg_disp_plain
   public :: alpsg_secant_batch, alpsg_map_search, alpsg_om_scan, alpsg_get_nmax
   public :: alpsg_eval_count, alpsg_error_text, alpsg_disp, alpsg_secant_osc
@@ -53,6 +62,7 @@ module alpsg
   public :: alpsg_server_create, alpsg_server_add_plasma, alpsg_server_remove_plasma
   public :: alpsg_server_submit_k, alpsg_server_submit_batch, alpsg_server_poll
   public :: alpsg_server_destroy, alpsg_determine_param_fit
+  public :: alpsg_om_double_scan, alpsg_server_stats, alpsg_last_scan_steps, alpsg_set_plain_roots
 
   interface
      function alpsg_create(desc, h) bind(C, name='alpsg_create') result(rc)
@@ -133,6 +143,40 @@ module alpsg
        complex(c_double_complex), intent(out) :: roots_out(*)
        integer(c_int) :: rc
      end function alpsg_om_scan
+
+     !! om_double_scan (ALPS_fns.f90:2900-3591), called in its place at
+     !! ALPS.f90:126; the eigen/heat arrays may be c_null_ptr when not asked for
+     function alpsg_om_double_scan(h, rp, s1, s2, nroots, roots, eigen, heat, kperp_out, &
+          kpar_out, roots_out, written, ef, bf, Us, ds, Ps, Ps_split, W_EM) &
+          bind(C, name='alpsg_om_double_scan') result(rc)
+       import :: c_int, c_ptr, c_double, c_double_complex, alpsg_root_params, alpsg_scan_params
+       type(c_ptr), value :: h
+       type(alpsg_root_params), intent(in) :: rp
+       type(alpsg_scan_params), intent(in) :: s1, s2
+       integer(c_int), value :: nroots, eigen, heat
+       complex(c_double_complex), intent(inout) :: roots(*)
+       real(c_double), intent(out) :: kperp_out(*), kpar_out(*)
+       complex(c_double_complex), intent(out) :: roots_out(*)
+       integer(c_int), intent(out) :: written(*)
+       type(c_ptr), value :: ef, bf, Us, ds, Ps, Ps_split, W_EM
+       integer(c_int) :: rc
+     end function alpsg_om_double_scan
+
+     function alpsg_last_scan_steps(h, nsteps, nroots, ms_out, iters_out) &
+          bind(C, name='alpsg_last_scan_steps') result(rc)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+       integer(c_int), value :: nsteps, nroots
+       type(c_ptr), value :: ms_out, iters_out
+       integer(c_int) :: rc
+     end function alpsg_last_scan_steps
+
+     function alpsg_set_plain_roots(h, plain) bind(C, name='alpsg_set_plain_roots') result(rc)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+       integer(c_int), value :: plain
+       integer(c_int) :: rc
+     end function alpsg_set_plain_roots
 
      function alpsg_get_nmax(h, nmax_out) bind(C, name='alpsg_get_nmax') result(rc)
        import :: c_int, c_ptr
@@ -258,6 +302,17 @@ module alpsg
        integer(c_int), intent(out) :: n_out
        integer(c_int) :: rc
      end function alpsg_server_poll
+
+     !! counts(10): submitted, finished, passes, disp evaluations, admitted,
+     !! waiting, arena bytes, arena bytes in use, plasma uploads, evictions;
+     !! times_ms(4): passes, admission, launch enqueue, device waits
+     function alpsg_server_stats(s, counts, times_ms) bind(C, name='alpsg_server_stats') result(rc)
+       import :: c_int, c_ptr, c_double, c_long_long
+       type(c_ptr), value :: s
+       integer(c_long_long), intent(out) :: counts(10)
+       real(c_double), intent(out) :: times_ms(4)
+       integer(c_int) :: rc
+     end function alpsg_server_stats
 
      function alpsg_server_destroy(s) bind(C, name='alpsg_server_destroy') result(rc)
        import :: c_int, c_ptr

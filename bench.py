@@ -707,7 +707,7 @@ def main():
                                 "k_assemble": float(split[2])},
         "map_evals_per_sec_device": args.nr * args.ni / (kern.mean() * 1e-3),
         "launch_shapes": {str(k): list(v) for k, v in shapes.items()},
-        "roofline": {"bound": "mfma", "compute": "fp64 (VALU)", "achieved": achieved,
+        "roofline": {"bound": "fp64 valu", "compute": "fp64 (VALU)", "achieved": achieved,
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": (SWEEP_TRAFFIC_BYTES.get(tuple(shapes.get(args.nr * args.ni, (0,) * 5)[:3]))

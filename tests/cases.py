@@ -70,6 +70,40 @@ def poly_row0_clipped(nperp=24, npar=48, order=8):
     return dataclasses.replace(p, poly_fit_coeffs=c)
 
 
+def analytical(nperp=24, npar=48, maxP=6.0, ms2=M_E):
+    """tests/test_analytical.in of the reference: protons and electrons,
+    tabulated Maxwellians (beta = 1, vA = 1e-4), both continued by the
+    pre-coded distribution_analyt (ACmethod 0).  The shipped function uses
+    m = 1/1836 for species 2; ms2 sets the table's electron mass (the
+    reference's input has 5.44662e-4)."""
+    import dataclasses
+    rp = RootParams(secant_method=2, numiter=500, D_threshold=1e-15, D_prec=1e-5, D_gap=1e-5,
+                    D_tol=1e-7)
+    p = bimaxwellian_plasma(ms=[1.0, ms2], qs=[1.0, -1.0], ns=[1.0, 1.0], taus=[1.0, 1.0],
+                            alphs=[1.0, 1.0], ps=[0.0, 0.0], beta=1.0, vA=1e-4, nperp=nperp,
+                            npar=npar, maxP=maxP, positions_principal=3,
+                            n_resonance_interval=100, Tlim=0.01, Bessel_zero=1e-50, roots=rp)
+    return dataclasses.replace(p, ACmethod=np.array([0, 0]))
+
+
+JUETTNER3 = (2.0e-3, 50.0, 0.3)   # (u1, u2, u3) of the fit-type-3 species, vA = 0.2
+
+
+def juettner3(nperp=24, npar=48):
+    """Protons whose table is the fit-type-3 form (Juettner in pperp and ppar,
+    ALPS_analyt.f90:238-242) u1 exp(-u2 sqrt(1 + (pperp^2 + (ppar - u3)^2)
+    vA^2/m^2)) at vA = 0.2, with that fit as its continuation, beside
+    Maxwellian electrons."""
+    import dataclasses
+    p = small_bimax(nperp=nperp, npar=npar)
+    p = dataclasses.replace(p, vA=0.2)
+    u1, u2, u3 = JUETTNER3
+    pe = p.pp[0, :, 0, 0][:, None]
+    pa = p.pp[0, 0, :, 1][None, :]
+    f = u1 * np.exp(-u2 * np.sqrt(1.0 + (pe ** 2 + (pa - u3) ** 2) * 0.2 * 0.2 / 1.0))
+    return _replace_species0(p, f, [3], [1.0], [(u1, u2, u3, 0.0, 0.0)])
+
+
 def bm_mix(kperp_norm=True, cold=True):
     """Closed-form (use_bM) species beside a tabulated one: an anisotropic
     drifting bi-Maxwellian proton population and, with ``cold``, a cold

@@ -175,3 +175,27 @@ def test_fortran_host_matches_python_host():
     p.roots.D_threshold = 1e-15
     w, _ = E.secant_osc(0.5 - 0.05j)
     assert abs(w - complex(*root)) <= 1e-12 * abs(w)
+    # om_double_scan through the Fortran interface equals the Python host's
+    from test_oracle_double_scan import K0, S1, S2
+    rows = [l.split() for l in lines if l.startswith("DSCAN")]
+    assert len(rows) == (S1[4] + 1) * (S2[4] + 1)
+    p.roots.D_threshold = 1e-14
+    E.set_k(*K0)
+    om0, _ = E.secant_osc(0.048 - 0.0001j)
+    r = E.om_double_scan(S1, S2, [om0])
+    for k, row in enumerate(rows):
+        i1, i2 = divmod(k, S2[4] + 1)
+        kp, kz, re, im = (float(x) for x in row[2:6])
+        assert int(row[6]) == int(r["written"][i1, i2, 0])
+        assert abs(kp - r["kperp"][i1, i2]) <= 1e-15 * kp and abs(kz - r["kpar"][i1, i2]) <= 1e-15 * kz
+        assert abs(complex(re, im) - r["roots"][i1, i2, 0]) <= 1e-14 * abs(complex(re, im))
+    # the root server through the Fortran interface: two solves and the counters
+    served = {int(l.split()[1]): l.split()[2:] for l in lines if l.startswith("SERVED")}
+    E.set_k(0.3, 0.3)
+    for tag, g in ((11, 0.29 - 0.001j), (12, 0.5 - 0.05j)):
+        w, it = E.secant_osc(g)
+        re, im, its, st = served[tag]
+        assert abs(complex(float(re), float(im)) - w) <= 1e-12 * abs(w)
+        assert int(its) == it and int(st) == (0 if it < p.roots.numiter else 6)
+    stats = [int(x) for x in next(l for l in lines if l.startswith("STATS")).split()[1:]]
+    assert stats[0] == 2 and stats[1] == 2 and stats[2] >= 1 and stats[3] >= stats[2]
