@@ -51,6 +51,8 @@ hipError_t launch_resonant(hipStream_t st, int nch, int RB, const ProbDev *prob,
 hipError_t launch_nhds(hipStream_t st, const ProbDev *prob, int nspec, int nom, const double2 *om,
                        const int *active, double2 *chiA, double2 *lowA, const int *pidx = nullptr);
 int nhds_tables(double z, int bm_nmax, double bessel_zero, std::vector<double2> &tab);
+hipError_t launch_lsum(hipStream_t st, int nperp, int nmax, double dpp, const double *pperp,
+                       const double *fitc, const double *atab, double *lsum);
 hipError_t launch_split_tables(hipStream_t st, int npar, int nmax, double kpar, double q, double m,
                                const double *ppar, const double *nodes, double *crec,
                                double *rsum);
@@ -181,6 +183,7 @@ struct alpsg_ctx {
   // per-k device data
   DevBuf<double> bes[ALPSG_MAXSPEC], atab[ALPSG_MAXSPEC], nodes[ALPSG_MAXSPEC];
   DevBuf<double> crec[ALPSG_MAXSPEC], rsum[ALPSG_MAXSPEC];  // split-form sweep tables
+  DevBuf<double> lsum[ALPSG_MAXSPEC];  // Landau sums of single-Maxwellian species
   DevBuf<int> zidx;  // zeros: the problem index of every omega of a small batch
   PinBuf om_pin, det_pin;  // staging of om and det for batches of at most PIN_MAX omegas
   // captured disp chains of small batches (index n = 1..REQ_MAX_N), with the
@@ -1364,7 +1367,7 @@ void rel_fit_rows(const alpsg_plasma_desc *d, int s, int r, int nspec_rel, doubl
 void rel_gamma_table(std::vector<double> &gt) {
   // Gamma(m) by the Lanczos series of ALPS_fns_rel.f90:1738-1791 and Fact(k)
   // (:1557-1574), as CBESSJ divides by them
-  gt.assign(GTAB_N + 21, 0.0);
+  gt.assign(GTAB_N + 21 + (size_t)RFG_NU * 21, 0.0);
   const double cof[6] = {76.18009173, -86.50532033, 24.01409822, -1.231739516, 0.120858003e-2,
                          -0.536382e-5};
   for (int mm = 1; mm < GTAB_N; ++mm) {
@@ -1381,6 +1384,12 @@ void rel_gamma_table(std::vector<double> &gt) {
     for (int i = 2; i <= k; ++i) f = f * (1.0 * i);
     gt[GTAB_N + k] = f;
   }
+  // the coefficients of the CBESSJ series, (-z^2/4)^k / (k! Gamma(nu+k+1)),
+  // as one reciprocal per (nu, k): an overflowing Gamma gives 0, as the
+  // source's division by it does
+  for (int nu = 0; nu < RFG_NU; ++nu)
+    for (int k = 0; k <= 20; ++k)
+      gt[GTAB_N + 21 + (size_t)nu * 21 + k] = 1.0 / (gt[GTAB_N + k] * gt[nu + k + 1]);
 }
 
 extern "C" {
@@ -1767,6 +1776,18 @@ int alpsg_create(const alpsg_plasma_desc *d, alpsg_handle *out) {
         }
       }
       S.fit_uniform = uni ? 1 : 0;
+      if (uni && d->n_fits[s] == 1) {
+        // the zero test of landau_integrate (ALPS_fns.f90:1404-1405) over its
+        // rows: c0 = u1 exp(-pc pperp^2) on iperp = 1..nperp-1 and 2..nperp
+        double c1 = INFINITY, c2 = INFINITY;
+        for (int i = 1; i <= nperp; ++i) {
+          const double c0 = std::fabs(fitc[fit_off[s] + (size_t)i * FIT_STRIDE]);
+          if (i <= nperp - 1) c1 = std::min(c1, c0);
+          if (i >= 2) c2 = std::min(c2, c0);
+        }
+        S.lz_c1 = c1;
+        S.lz_c2 = c2;
+      }
     }
     if (!(S.acmethod == 1 && S.fit_uniform && S.nfits == 1)) h->any_general_fit = true;
     if (d->ACmethod[s] == 2) {
@@ -1883,6 +1904,13 @@ int alpsg_set_k(alpsg_handle h, double kperp, double kpar, const int *nmax_in, i
     S.atab = h->atab[s].p;
     S.nodes = h->nodes[s].p;
     S.crec = S.rsum = nullptr;
+    S.lsum = nullptr;
+    if (!h->relsp[s] && S.acmethod == 1 && S.fit_uniform && S.nfits == 1) {
+      HIPCHK(h->lsum[s].ensure((size_t)(nm + 1) * 6));
+      HIPCHK(launch_lsum(h->st, nperp, nm, S.dpperp, h->pperp.p + (size_t)s * (nperp + 1), S.fitc,
+                         h->atab[s].p, h->lsum[s].p));
+      S.lsum = h->lsum[s].p;
+    }
     if (!h->relsp[s]) {
       HIPCHK(h->crec[s].ensure((size_t)(nm + 1) * (npar - 1) * CREC_STRIDE));
       HIPCHK(h->rsum[s].ensure((size_t)(nm + 1) * RSUM_STRIDE));

@@ -14,6 +14,7 @@
 #define CREC_STRIDE 13  // doubles per (n, ipar) record of the split-form sweep table
 #define RSUM_STRIDE 8   // doubles per harmonic of the split-form row sums
 #define GTAB_N 4200     // Gamma(m) table entries (m < GTAB_N), followed by Fact(0..20)
+#define RFG_NU (GTAB_N - 21)  // orders nu of the series coefficients 1/(k! Gamma(nu+k+1)) that follow
 
 struct SpecDev {
   int nperp, npar, nmax;
@@ -40,6 +41,14 @@ struct SpecDev {
   const double *crec;   // [(nmax+1)*(npar-1)*CREC_STRIDE]
   const double *rsum;   // [(nmax+1)*RSUM_STRIDE]
   const double *fitc;   // [(nperp+1)*nfits*FIT_STRIDE]
+  // Landau sums of a species whose continuation is one Maxwellian with
+  // pperp-independent (u2, u3) (NULL otherwise), per k: lsum[n][6] =
+  // {sum h a_k pperp c0, k = 0..2; sum h a_k beta, k = 0..2} over iperp = 0..nperp,
+  // with c0 = u1 exp(-pc pperp^2), beta the pperp difference of c0 that
+  // landau_integrate takes and h its trapezoid weight; lz_c1 / lz_c2 the
+  // smallest |c0| over iperp = 1..nperp-1 / 2..nperp (its zero test)
+  const double *lsum;
+  double lz_c1, lz_c2;
   const double *polyc;  // [(nperp+1)*(poly_order+1)]
   // closed-form (use_bM) species: NHDS parameters (ALPS_NHDS.f90:163-168)
   int usebm, bm_cold, bm_nrun;
@@ -72,7 +81,8 @@ struct TabJob {
 struct ProbDev {
   int nspec, kperp_norm, P, NRI;
   double kperp, kpar, vA, Tlim;
-  const double *gtab;  // Gamma(m) of ALPS_fns_rel.f90:1738 for m = 0..GTAB_N-1, then Fact(k), k = 0..20
+  const double *gtab;  // Gamma(m) of ALPS_fns_rel.f90:1738 for m = 0..GTAB_N-1, then Fact(k), k = 0..20,
+                       // then 1/(Fact(k) Gamma(nu+k+1)) at [nu*21 + k], nu = 0..RFG_NU-1
   int nmaxR;           // largest nmax over the relativistic species
   SpecDev sp[ALPSG_DMAXSPEC];
 };

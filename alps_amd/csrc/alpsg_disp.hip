@@ -974,9 +974,9 @@ __device__ __forceinline__ void wave_sync() {
 // Maxwellian whose ppar parameters do not depend on pperp, exp(-u2 (p-u3)^2) is
 // the same for every iperp and is evaluated once (bit-identical values).
 template <int NCH, bool UNI>
-__device__ void landau_part(const SpecDev &S, const ProbDev &Pd, const double *__restrict__ arow,
-                            const RowClass &rc, double omr, double omi, int c, cd L[3],
-                            int &zero) {
+__device__ void landau_part(const SpecDev &S, const ProbDev &Pd, int an, const RowClass &rc,
+                            double omr, double omi, int c, cd L[3], int &zero) {
+  const double *__restrict__ arow = S.atab + (size_t)an * (S.nperp + 1) * ATAB_STRIDE;
   const int nperp = S.nperp;
   const double d = S.dppar_abs, dpp = S.dpperp;
   const cd p = mk(rc.pr, rc.pim);
@@ -997,6 +997,24 @@ __device__ void landau_part(const SpecDev &S, const ProbDev &Pd, const double *_
     dq = mk(pm_.re - u3, pm_.im);
     sq = dq * dq;
     Em = cexpd(mk(-u2 * sq.re, -u2 * sq.im));
+  }
+  if constexpr (UNI) {
+    if (S.lsum != nullptr) {
+      // eval_fit = c0(iperp) E(p): the sum over iperp is two omega-independent
+      // per-k sums (k_lsum) times A = (E(p+d) - E(p-d))/2d and (om - km p) E(p);
+      // the zero test needs only the smallest |c0| of its rows (a product
+      // c0 x rounds to zero for some row iff it does for the smallest |c0|)
+      if (c == 0) {
+        const cd z1 = S.lz_c1 * Ep, z2 = S.lz_c1 * Em, z3 = S.lz_c2 * E0;
+        if (czero(z1) || czero(z2) || czero(z3)) zero = 1;
+        const double *ls = S.lsum + (size_t)an * 6;
+        const cd A = (Ep - Em) * (1.0 / (2.0 * d));
+        const cd B = (mk(omr, omi) - km * p) * E0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) L[k] = L[k] + qk * ((km * ls[k]) * A + ls[3 + k] * B);
+      }
+      return;
+    }
   }
   auto F = [&](int ip, int which) -> cd {
     if constexpr (!UNI) {
@@ -1127,7 +1145,7 @@ __device__ __forceinline__ void resonant_body(const ProbDev *__restrict__ probp,
     if constexpr (PART != 0) {
       cd L[3] = {mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0)};
       int zero = 0;
-      landau_part<NCH, PART == 1>(S, Pd, S.atab + (size_t)an * (S.nperp + 1) * ATAB_STRIDE, rc, omr, omi, c,
+      landau_part<NCH, PART == 1>(S, Pd, an, rc, omr, omi, c,
                        L, zero);
       zero = gor<NCH>(zero);
       if (!zero) {
@@ -1380,9 +1398,13 @@ hipError_t launch_sweep_req(hipStream_t st, int nch, int W, int RC, int K, const
   if (nom <= 0) return hipSuccess;
   if (pidx == nullptr) return hipErrorInvalidValue;
   if (reduce) {
-    if (!(nch == 32 && W == 4 && K == 4)) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_sweep_req<32, 4, 4, true>), dim3(nom, RC), dim3(256), 0, st, prob, nom,
-                       om, chiA, pidx);
+    if (!(nch == 32 && W == 4 && (K == 4 || K == 2))) return hipErrorInvalidValue;
+    if (K == 4)
+      hipLaunchKernelGGL((k_sweep_req<32, 4, 4, true>), dim3(nom, RC), dim3(256), 0, st, prob, nom,
+                         om, chiA, pidx);
+    else
+      hipLaunchKernelGGL((k_sweep_req<32, 4, 2, true>), dim3(nom, RC), dim3(256), 0, st, prob, nom,
+                         om, chiA, pidx);
     return hipGetLastError();
   }
 #define SQ(N, WV, KK)                                                                             \

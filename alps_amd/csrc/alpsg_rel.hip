@@ -519,18 +519,28 @@ __device__ __forceinline__ cd cpowi_d(cd a, int n) {
   return pw;
 }
 
-// CBESSJ (ALPS_fns_rel.f90:1502-1553): 21 terms of the power series of J_nu(z),
-// with Gamma(nu+k+1) and Fact(k) from the host-built table
-__device__ cd cbessj_d(const double *gtab, cd z, int nu) {
-  cd sum = mk(0.0, 0.0);
+// CBESSJ (ALPS_fns_rel.f90:1502-1553), 21 terms of the power series, for
+// J_nu0, J_nu0+1, J_nu0+2 of the same z: the three
+// orders share the powers (-z^2/4)^k, built by one product per term, and
+// take the coefficients 1/(k! Gamma(nu+k+1)) from the table (requires
+// nu0 + 2 < RFG_NU)
+__device__ void cbessj3_d(const double *gtab, cd z, int nu0, cd J[3]) {
+  const double *rf = gtab + GTAB_N + 21 + (size_t)nu0 * 21;
   const cd w = cdivr(-1.0 * (z * z), 4.0);
+  cd t = mk(1.0, 0.0), s0 = mk(0.0, 0.0), s1 = s0, s2 = s0;
   for (int k = 0; k <= 20; ++k) {
-    cd t = cpowi_d(w, k);
-    t = cdivr(t, gtab[GTAB_N + k]);
-    t = cdivr(t, gtab[nu + k + 1]);
-    sum = sum + t;
+    if (k > 0) t = t * w;
+    s0 = s0 + t * rf[k];
+    s1 = s1 + t * rf[21 + k];
+    s2 = s2 + t * rf[42 + k];
   }
-  return cpowi_d(cdivr(z, 2.0), nu) * sum;
+  const cd h = cdivr(z, 2.0);
+  cd pw = cpowi_d(h, nu0);
+  J[0] = pw * s0;
+  pw = pw * h;
+  J[1] = pw * s1;
+  pw = pw * h;
+  J[2] = pw * s2;
 }
 
 // int_T_res_rel (ALPS_fns_rel.f90:1374-1495), six modes, at complex pparbar p of row ig
@@ -541,20 +551,23 @@ __device__ void int_T_res_rel6(const RelCtx &C, const double *gtab, int ig, cd p
   const cd pperpbar = csqrtd(mk((g * g - 1.0) - pp2.re, -pp2.im));
   const cd z = S.rel_zfac * pperpbar;
   const int n = C.n;
+  const int an = n < 0 ? -n : n;
+  // the orders |n| - 1, |n|, |n| + 1 (0, 1, 2 for |n| <= 1)
+  cd J[3];
+  cbessj3_d(gtab, z, an >= 1 ? an - 1 : 0, J);
   cd b, bP;
-  if (n < 0)
-    b = m1p(n) * cbessj_d(gtab, z, -n);
-  else
-    b = cbessj_d(gtab, z, n);
   if (n >= 1) {
-    bP = 0.5 * (cbessj_d(gtab, z, n - 1) - cbessj_d(gtab, z, n + 1));
+    b = J[1];
+    bP = 0.5 * (J[0] - J[2]);
   } else if (n < -1) {
-    bP = 0.5 * ((m1p(n - 1) * cbessj_d(gtab, z, -(n - 1))) -
-                (m1p(n + 1) * cbessj_d(gtab, z, -(n + 1))));
+    b = m1p(n) * J[1];
+    bP = 0.5 * ((m1p(n - 1) * J[2]) - (m1p(n + 1) * J[0]));
   } else if (n == 0) {
-    bP = -1.0 * cbessj_d(gtab, z, 1);
-  } else {
-    bP = 0.5 * (cbessj_d(gtab, z, 2) - cbessj_d(gtab, z, 0));
+    b = J[0];
+    bP = -1.0 * J[1];
+  } else {  // n == -1
+    b = m1p(n) * J[1];
+    bP = 0.5 * (J[2] - J[0]);
   }
   const double zbar = S.rel_zbar, kp = C.kperp, nn = 1.0 * n;
   const double f = C.kn ? 1.0 : kp, f2 = C.kn ? 1.0 : kp * kp;

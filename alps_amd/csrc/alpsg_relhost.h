@@ -29,5 +29,6 @@ int rel_host_tables(const alpsg_plasma_desc *d, int s, int r, int nspec_rel, con
 // s in the Maxwell form c0 exp(-c1 (p - c2)^2) of the fit table: c[row][fit][FIT_STRIDE].
 void rel_fit_rows(const alpsg_plasma_desc *d, int s, int r, int nspec_rel, double *c);
 
-// Gamma(m), m = 0..GTAB_N-1, then Fact(0..20) (ALPS_fns_rel.f90:1738-1791, :1557-1574).
+// Gamma(m), m = 0..GTAB_N-1, then Fact(0..20) (ALPS_fns_rel.f90:1738-1791, :1557-1574),
+// then the CBESSJ series coefficients 1/(Fact(k) Gamma(nu+k+1)), nu = 0..RFG_NU-1.
 void rel_gamma_table(std::vector<double> &gt);

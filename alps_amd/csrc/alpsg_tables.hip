@@ -253,6 +253,46 @@ __global__ void k_atab(int nperp, int nmax, const double *__restrict__ pperp,
   atab_entry(blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y, nperp, nmax, pperp, bes, atab);
 }
 
+// Landau sums of a single-Maxwellian species (the omega-independent part of
+// landau_integrate, ALPS_fns.f90:1382-1444, when eval_fit = c0(iperp) E(p)):
+// with dfpar = c0 (E(p+d) - E(p-d))/2d and dfperp = beta E(p), the sum over
+// iperp of h a K is qk [km A sum(h a pperp c0) + (om - km p) E(p) sum(h a beta)],
+// A = (E(p+d) - E(p-d))/2d.  One thread per (n, sum), iperp in order.
+__global__ void k_lsum(int nperp, int nmax, double dpp, const double *__restrict__ pperp,
+                       const double *__restrict__ fitc, const double *__restrict__ atab,
+                       double *__restrict__ lsum) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 6 * (nmax + 1)) return;
+  const int n = t / 6, w = t % 6, k = w % 3;
+  const int W = nperp + 1;
+  auto c0 = [&](int ip) { return fitc[(size_t)ip * FIT_STRIDE]; };
+  double acc = 0.0;
+  for (int ip = 0; ip <= nperp; ++ip) {
+    const double h = (ip == 0 || ip >= nperp - 1) ? 0.5 : 1.0;
+    const double a = atab[((size_t)n * W + ip) * ATAB_STRIDE + k];
+    double v;
+    if (w < 3) {
+      v = pperp[ip] * c0(ip);
+    } else if (ip == 0) {
+      v = (c0(1) - c0(0)) * (1.0 / dpp);
+    } else if (ip == nperp) {
+      v = (c0(nperp) - c0(nperp - 1)) * (1.0 / dpp);
+    } else {
+      v = (c0(ip + 1) - c0(ip - 1)) * (1.0 / (2.0 * dpp));
+    }
+    acc += (h * a) * v;
+  }
+  lsum[t] = acc;
+}
+
+hipError_t launch_lsum(hipStream_t st, int nperp, int nmax, double dpp, const double *pperp,
+                       const double *fitc, const double *atab, double *lsum) {
+  const int n = 6 * (nmax + 1);
+  hipLaunchKernelGGL(k_lsum, dim3((n + 63) / 64), dim3(64), 0, st, nperp, nmax, dpp, pperp, fitc,
+                     atab, lsum);
+  return hipGetLastError();
+}
+
 // Node tables: one thread per (n, ipar), summing over iperp.  The a-values of a
 // given n are shared by the whole block (staged in LDS); df1/df2 rows are read
 // coalesced along ipar.
@@ -530,6 +570,101 @@ __global__ __launch_bounds__(64) void k_moments_jobs(const TabJob *__restrict__ 
                    J.df2, J.atab, J.nodes);
 }
 
+// The same node tables on the matrix cores.  For one species the node sums
+// are a real GEMM, nodes[(n, a)][(ipar, g/h)] = sum_iperp Aw[(n, a)][iperp]
+// B[iperp][(ipar, g/h)], with Aw = w_perp a(n, iperp) (a = J^2, pperp^2 J'^2,
+// pperp J J'), B(g) = df1 and B(h) = (kpar/m)(pperp df2 - ppar df1) per point,
+// M = 3 (nmax+1), N = 2 (npar-1), K = nperp-1.  One wavefront computes a
+// 16 x 16 MT tile with v_mfma_f64_16x16x4f64 (lane l holds A[l%16][l/16] and
+// B[l/16][l%16], and D[4 (l/16) + q][l%16] in element q), stepping K by 4;
+// elements outside M, N or K enter as zeros.  The per-point B values are
+// rounded as in moments_block; only the order of the sums over iperp differs.
+constexpr int MT = 4;  // 16-column tiles per wavefront (one A operand feeds MT products)
+
+__device__ __forceinline__ void moments_mfma_tile(int tile_c, int tile_r, int nperp, int npar,
+                                                  int nmax, double kpar, double m,
+                                                  const double *__restrict__ pperp,
+                                                  const double *__restrict__ ppar,
+                                                  const double *__restrict__ df1,
+                                                  const double *__restrict__ df2,
+                                                  const double *__restrict__ atab,
+                                                  double *__restrict__ nodes) {
+  typedef double double4v __attribute__((ext_vector_type(4)));
+  const int l = threadIdx.x & 63;
+  const int i = l & 15, kk = l >> 4;
+  const int M = 3 * (nmax + 1), N = 2 * (npar - 1), W = nperp + 1;
+  const double kom = kpar / m;
+  // this lane's A row and B columns
+  const int r = tile_r * 16 + i;
+  const bool rv = r < M;
+  const int an = rv ? r / 3 : 0, aa = rv ? r % 3 : 0;
+  int ipc[MT], ghc[MT];
+  bool cv[MT];
+  double pac[MT];
+#pragma unroll
+  for (int t = 0; t < MT; ++t) {
+    const int c = (tile_c * MT + t) * 16 + i;
+    cv[t] = c < N;
+    ipc[t] = cv[t] ? 1 + (c >> 1) : 1;
+    ghc[t] = c & 1;
+    pac[t] = ppar[ipc[t]];
+  }
+  double4v acc[MT];
+#pragma unroll
+  for (int t = 0; t < MT; ++t) acc[t] = double4v{0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 1; k0 <= nperp - 1; k0 += 4) {
+    const int iperp = k0 + kk;
+    const bool kv = iperp <= nperp - 1;
+    const int ik = kv ? iperp : 1;
+    const double wperp = (ik == nperp - 1) ? 1.0 : 2.0;
+    const double av = (rv && kv) ? wperp * atab[((size_t)an * W + ik) * ATAB_STRIDE + aa] : 0.0;
+    const double pe = pperp[ik];
+#pragma unroll
+    for (int t = 0; t < MT; ++t) {
+      const size_t o = (size_t)ik * (npar + 1) + ipc[t];
+      const double d1 = df1[o], d2 = df2[o];
+      const double bg = ghc[t] ? kom * (pe * d2 - pac[t] * d1) : d1;
+      const double bv = (cv[t] && kv) ? bg : 0.0;
+      acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[t], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < MT; ++t) {
+    const int c = (tile_c * MT + t) * 16 + i;
+    if (c >= N) continue;
+    const int ipar = 1 + (c >> 1), gh = c & 1;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int rr = tile_r * 16 + 4 * kk + q;
+      if (rr >= M) continue;
+      const int n = rr / 3, a = rr % 3;
+      nodes[((size_t)n * (npar - 1) + (ipar - 1)) * NODE_STRIDE + 2 * a + gh] = acc[t][q];
+    }
+  }
+}
+
+__global__ __launch_bounds__(64) void k_moments_mfma(int nperp, int npar, int nmax, double kpar,
+                                                     double m, const double *__restrict__ pperp,
+                                                     const double *__restrict__ ppar,
+                                                     const double *__restrict__ df1,
+                                                     const double *__restrict__ df2,
+                                                     const double *__restrict__ atab,
+                                                     double *__restrict__ nodes) {
+  moments_mfma_tile(blockIdx.x, blockIdx.y, nperp, npar, nmax, kpar, m, pperp, ppar, df1, df2,
+                    atab, nodes);
+}
+
+__global__ __launch_bounds__(64) void k_moments_mfma_jobs(const TabJob *__restrict__ jobs,
+                                                          const int *__restrict__ start,
+                                                          int njobs) {
+  const int j = job_of(start, njobs, blockIdx.x);
+  const TabJob &J = jobs[j];
+  const int b = blockIdx.x - start[j];
+  const int tc = (2 * (J.npar - 1) + 16 * MT - 1) / (16 * MT);
+  moments_mfma_tile(b % tc, b / tc, J.nperp, J.npar, J.nmax, J.kpar, J.m, J.pperp, J.ppar, J.df1,
+                    J.df2, J.atab, J.nodes);
+}
+
 // ---------------------------------------------------------------------------
 // Host launchers (called from alpsg_api.cpp).
 // ---------------------------------------------------------------------------
@@ -587,9 +722,10 @@ hipError_t launch_species_k(hipStream_t st, int nperp, int npar, int nmax, doubl
                      nmax, kperp, q, pperp, mtab, bes);
   hipLaunchKernelGGL(k_atab, dim3((nperp + 1 + 63) / 64, nmax + 1), dim3(64), 0, st, nperp, nmax,
                      pperp, bes, atab);
-  size_t lds = sizeof(double) * 3 * (size_t)(nperp - 1);
-  hipLaunchKernelGGL(k_moments, dim3((npar - 1 + 63) / 64, nmax + 1), dim3(64), lds, st, nperp,
-                     npar, nmax, kpar, m, pperp, ppar, df1, df2, atab, nodes);
+  const int tc = (2 * (npar - 1) + 16 * MT - 1) / (16 * MT);
+  const int tr = (3 * (nmax + 1) + 15) / 16;
+  hipLaunchKernelGGL(k_moments_mfma, dim3(tc, tr), dim3(64), 0, st, nperp, npar, nmax, kpar, m,
+                     pperp, ppar, df1, df2, atab, nodes);
   return hipGetLastError();
 }
 
@@ -599,7 +735,7 @@ int jobs_blocks_bessel_max(const TabJob &J) { return J.ntry; }
 int jobs_blocks_bessel(const TabJob &J) { return ((J.nperp + 1 + 63) / 64) * (J.nmax + 3); }
 int jobs_blocks_atab(const TabJob &J) { return ((J.nperp + 1 + 63) / 64) * (J.nmax + 1); }
 int jobs_blocks_moments(const TabJob &J) {
-  return ((J.npar - 1 + 63) / 64) * ((J.nmax + 1 + MB - 1) / MB);
+  return ((2 * (J.npar - 1) + 16 * MT - 1) / (16 * MT)) * ((3 * (J.nmax + 1) + 15) / 16);
 }
 
 // start arrays: [njobs + 1] each, device copies; total = start[njobs]
@@ -618,7 +754,7 @@ hipError_t launch_species_k_jobs(hipStream_t st, const TabJob *jobs, int njobs,
   hipLaunchKernelGGL(k_bessel_jobs, dim3(total_bes), dim3(64), 0, st, jobs, start_bes, njobs, mtab);
   hipLaunchKernelGGL(k_atab_jobs, dim3(total_atab), dim3(64), 0, st, jobs, start_atab, njobs);
   (void)max_nperp;
-  hipLaunchKernelGGL(k_moments_jobs, dim3(total_mom), dim3(64), 0, st, jobs, start_mom, njobs);
+  hipLaunchKernelGGL(k_moments_mfma_jobs, dim3(total_mom), dim3(64), 0, st, jobs, start_mom, njobs);
   return hipGetLastError();
 }
 
