@@ -576,7 +576,7 @@ __global__ __launch_bounds__(64) void k_moments_jobs(const TabJob *__restrict__ 
 // pperp J J'), B(g) = df1 and B(h) = (kpar/m)(pperp df2 - ppar df1) per point,
 // M = 3 (nmax+1), N = 2 (npar-1), K = nperp-1.  One wavefront computes a
 // 16 x 16 MT tile with v_mfma_f64_16x16x4f64 (lane l holds A[l%16][l/16] and
-// B[l/16][l%16], and D[4 (l/16) + q][l%16] in element q), stepping K by 4;
+// B[l/16][l%16], and D[l/16 + 4q][l%16] in element q), stepping K by 4;
 // elements outside M, N or K enter as zeros.  The per-point B values are
 // rounded as in moments_block; only the order of the sums over iperp differs.
 constexpr int MT = 4;  // 16-column tiles per wavefront (one A operand feeds MT products)
@@ -635,7 +635,7 @@ __device__ __forceinline__ void moments_mfma_tile(int tile_c, int tile_r, int np
     const int ipar = 1 + (c >> 1), gh = c & 1;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const int rr = tile_r * 16 + 4 * kk + q;
+      const int rr = tile_r * 16 + kk + 4 * q;
       if (rr >= M) continue;
       const int n = rr / 3, a = rr % 3;
       nodes[((size_t)n * (npar - 1) + (ipar - 1)) * NODE_STRIDE + 2 * a + gh] = acc[t][q];

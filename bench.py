@@ -449,6 +449,117 @@ def run_serve(args):
     return out
 
 
+# the non-Maxwellian workload: proton core + 10 % beam drifting at 1.5 v_A,
+# bi-kappa (T_perp/T_par = 2) with kappa in {3, 4, 5, 6}, beside electrons,
+# f0 on 256 x 512; per kappa a 512 x 512 omega map at kperp = kpar = 0.3, and
+# a 32 x 32 (kperp, kpar) double scan of one root
+KAPPAS = (3.0, 4.0, 5.0, 6.0)
+KMAP = dict(kperp=0.3, kpar=0.3, omi=0.01, omf=2.0, gami=-0.5, gamf=0.05)
+
+
+def kappa_plasma_cb(kappa, nperp=256, npar=512):
+    from alps_amd.plasma import RootParams, kappa_plasma
+    rp = RootParams(secant_method=2, numiter=50, D_threshold=1e-12, D_prec=1e-5, D_gap=1e-5)
+    return kappa_plasma(ms=[1.0, 1.0, M_E], qs=[1.0, 1.0, -1.0], ns=[0.9, 0.1, 1.0],
+                        taus=[1.0, 1.0, 1.0], alphs=[2.0, 2.0, 1.0], ps=[0.0, 1.5, 0.0],
+                        kappas=[kappa, kappa, kappa], beta=1.0, vA=1e-4, nperp=nperp, npar=npar,
+                        maxP=6.0, positions_principal=3, n_resonance_interval=100, Tlim=0.01,
+                        Bessel_zero=1e-45, roots=rp)
+
+
+def run_kappa(args):
+    """512 x 512 maps of det D, one per kappa, and a 1024-point double scan
+    (om_double_scan, ALPS_fns.f90:2900-3591: kperp 0.3 -> 1 outer, kpar
+    0.3 -> 1 inner, 31 log steps each) of the refined deepest minimum of the
+    kappa = 4 map; chi-tensor evals/s, roots/s, the sweep's roofline, and the
+    oracle on a sample of map points and scan roots."""
+    import math
+    from alps_amd.engine import Engine
+    nr = ni = args.kmap
+    maps, t_map, evals, split, nmaxs, grids = [], 0.0, 0, [], [], {}
+    engines = {}
+    for kap in KAPPAS:
+        p = kappa_plasma_cb(kap)
+        E = engines[kap] = Engine(p)
+        nmaxs.append(E.set_k(KMAP["kperp"], KMAP["kpar"]))
+        E.map_search(KMAP["omi"], KMAP["omf"], KMAP["gami"], KMAP["gamf"], nr, ni)  # warm-up
+        ev0 = E.eval_count
+        t0 = time.perf_counter()
+        om, cal, val, iroots, nf = E.map_search(KMAP["omi"], KMAP["omf"], KMAP["gami"],
+                                                KMAP["gamf"], nr, ni)
+        t_map += time.perf_counter() - t0
+        evals += E.eval_count - ev0
+        split.append(E.last_kernel_split_ms)
+        grids[kap] = (om, cal, val, iroots, p)
+    flops = sum(flops_per_eval(nm, 512) for nm in nmaxs) * nr * ni
+    sweep_ms = sum(sp[0] for sp in split)
+    # the double scan from the deepest minimum of the kappa = 4 map
+    om4, cal4, val4, ir4, p4 = grids[4.0]
+    E = engines[4.0]
+    E.set_k(KMAP["kperp"], KMAP["kpar"])
+    best = min(ir4, key=lambda ij: val4[ij[0] - 1, ij[1] - 1]) if len(ir4) else (nr // 2, ni // 2)
+    guess = om4[best[0] - 1, best[1] - 1]
+    w0, _, it0 = E.solve_batch([guess])
+    n_s = args.kscan
+    s1 = (3, 1, (math.log10(1.0) - math.log10(KMAP["kperp"])) / (n_s - 1), 0.0, n_s - 1, 1)
+    s2 = (4, 1, (math.log10(1.0) - math.log10(KMAP["kpar"])) / (n_s - 1), 0.0, n_s - 1, 1)
+    t0 = time.perf_counter()
+    r = E.om_double_scan(s1, s2, [w0[0]])
+    t_scan = time.perf_counter() - t0
+    n_pts = int(np.sum(r["written"]))
+    out = {
+        "workload": "kappa core (90 %) + beam (10 %, 1.5 v_A) bi-kappa T_perp/T_par = 2, "
+                    "electrons, 256 x 512; a 512 x 512 omega map per kappa in {3,4,5,6} at "
+                    "kperp = kpar = 0.3; a 32 x 32 (kperp, kpar) om_double_scan 0.3 -> 1",
+        "map": [nr, ni],
+        "nmax": nmaxs,
+        "map_evals_per_sec": evals / t_map,
+        "map_ms": 1e3 * t_map / len(KAPPAS),
+        "map_kernel_split_ms": [list(sp) for sp in split],
+        "roofline_sweep": {"bound": "fp64 valu", "achieved": flops / (sweep_ms * 1e-3) / 1e12,
+                           "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                           "frac": flops / (sweep_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+                           "flops": flops, "kernel_ms": sweep_ms,
+                           "kernel": "k_sweep, the four %dx%d map batches" % (nr, ni)},
+        "scan_points": n_pts,
+        "scan_s": t_scan,
+        "scan_roots_per_sec": n_pts / t_scan,
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from oracle import Oracle
+        thr = min(args.cpu_threads, os.cpu_count() or 1)
+        # 16 strided points of the kappa = 4 map, then the first 2 scan points
+        O = Oracle(p4, n_threads=thr)
+        O.set_k(KMAP["kperp"], KMAP["kpar"])
+        flat_om, flat_cal = om4.T.ravel(), cal4.T.ravel()
+        idx = np.linspace(0, flat_om.size - 1, 16).astype(int)
+        tc = time.perf_counter()
+        err = 0.0
+        for k in idx:
+            d, _, _, wave = O.disp(flat_om[k], want_tensors=True)
+            w = np.abs(wave)
+            sc = (w[0, 0] * (w[1, 1] * w[2, 2] + w[1, 2] ** 2) + 2 * w[0, 1] * w[1, 2] * w[0, 2]
+                  + w[0, 2] ** 2 * w[1, 1] + w[0, 1] ** 2 * w[2, 2])
+            err = max(err, abs(flat_cal[k] - d) / sc)
+        t_eval = time.perf_counter() - tc
+        o0, _ = O.secant_osc(guess)
+        rerr = abs(w0[0] - o0) / abs(o0)
+        for j in (1, 2):   # inner scan steps from the first outer point
+            O.set_k(r["kperp"][0, j], r["kpar"][0, j])
+            oj, _ = O.secant_osc(r["roots"][0, j - 1, 0] if j > 1 else o0)
+            rerr = max(rerr, abs(r["roots"][0, j, 0] - oj) / abs(oj))
+        out["cpu_baseline"] = {"value": len(idx) / t_eval, "unit": "chi-tensor evals/s",
+                               "cores": thr, "kind": "port", "cpu_model": cpu_model(),
+                               "sample": "16 strided points of the kappa = 4 map, oracle/"
+                                         f"liboracle.so with {thr} OpenMP threads",
+                               "wall_s": t_eval}
+        out["map_vs_oracle_on_sample"] = err
+        out["roots_vs_oracle"] = {"compared": 3, "max_rel_err": rerr, "tolerance": 1e-10}
+        out["speedup_vs_cpu"] = out["map_evals_per_sec"] / out["cpu_baseline"]["value"]
+    return out
+
+
 # the CPU-runnable scan: 2 species on 50 x 100, the root refined at
 # kperp d_p = 0.01, kpar d_p = 0.1, then followed over 64 log steps to kpar d_p = 1
 SCAN = dict(nperp=50, npar=100, kperp=0.01, kpar=0.1, kpar_end=1.0, guess=0.1 - 0.001j, steps=64)
@@ -552,15 +663,19 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (torch) and exit: the "
                          "known-size kernel that scales the profiler's FETCH_SIZE/WRITE_SIZE")
-    ap.add_argument("--workload", choices=["map", "serve", "serve_rel", "scan"], default="map",
+    ap.add_argument("--workload", choices=["map", "serve", "serve_rel", "scan", "kappa"],
+                    default="map",
                     help="map: the batched map search (default); serve: the Poisson stream "
                          "of independent root requests through the root server; serve_rel: "
                          "the relativistic kpar-scan requests through the root server; scan: "
-                         "one root refined and followed over 64 log kpar steps (50 x 100)")
+                         "one root refined and followed over 64 log kpar steps (50 x 100); "
+                         "kappa: the non-Maxwellian core-beam maps and double scan")
     ap.add_argument("--requests", type=int, default=4096)
     ap.add_argument("--rel-requests", type=int, default=1024,
                     help="serve_rel: relativistic kpar-scan requests")
     ap.add_argument("--scan-repeats", type=int, default=10, help="scan: timed repeats")
+    ap.add_argument("--kmap", type=int, default=512, help="kappa: omega map side")
+    ap.add_argument("--kscan", type=int, default=32, help="kappa: double-scan side")
     ap.add_argument("--variants", type=int, default=64, help="plasma variants in the request pool")
     ap.add_argument("--rate", type=float, default=20000.0, help="request arrivals per second")
     ap.add_argument("--slots", type=int, default=4096)
@@ -598,6 +713,9 @@ def main():
         return
     if args.workload == "scan":
         print(json.dumps({"workload": "scan", **run_scan(args)}))
+        return
+    if args.workload == "kappa":
+        print(json.dumps(run_kappa(args)))
         return
 
     if args.pmc_shapes:
@@ -745,6 +863,9 @@ def main():
         # the CPU-runnable scan: one root refined, then 64 log kpar steps
         out["scan"] = run_scan(args)
         progress(f"scan: {out['scan']['roots_per_sec']:.4g} roots/s")
+        # the non-Maxwellian kappa core-beam maps and double scan
+        out["kappa"] = run_kappa(args)
+        progress(f"kappa: {out['kappa']['map_evals_per_sec']:.4g} evals/s")
     print(json.dumps(out))
 
 

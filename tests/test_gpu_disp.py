@@ -68,6 +68,12 @@ def test_kappa_fits():
     _check(p, 0.3, 0.3, cases.BRANCH_OMEGAS)
 
 
+def test_bi_moyal_fit():
+    """A bi-Moyal species (fit type 6, ALPS_analyt.f90:256-260): its Landau
+    term takes the general continuation."""
+    _check(cases.fit_moyal(), 0.3, 0.3, cases.BRANCH_OMEGAS)
+
+
 @pytest.mark.parametrize("case", ["poly_case", "poly_row0_clipped"])
 def test_chebyshev_continuation(case):
     """Chebyshev continuation, including the Landau early return

@@ -82,17 +82,56 @@ def test_map_workload_sub_grid_search():
     E.close()
 
 
-def _kappa_core_beam():
+def _kappa_core_beam(kappas=(4.0, 5.0, 4.0)):
     from alps_amd.plasma import kappa_plasma
     return kappa_plasma(ms=[1.0, 1.0, cases.M_E], qs=[1.0, 1.0, -1.0], ns=[0.9, 0.1, 1.0],
                         taus=[1.0, 1.0, 1.0], alphs=[2.0, 2.0, 1.0], ps=[0.0, 1.5, 0.0],
-                        kappas=[4.0, 5.0, 4.0], beta=1.0, vA=1e-4, nperp=256, npar=512, maxP=6.0,
+                        kappas=list(kappas), beta=1.0, vA=1e-4, nperp=256, npar=512, maxP=6.0,
                         positions_principal=3, n_resonance_interval=100, Tlim=0.01,
                         Bessel_zero=1e-45)
 
 
 def test_kappa_core_beam_workload_grid():
     _disp_parity(_kappa_core_beam(), 0.3, 0.3, [0.6 - 0.05j, 0.2 - 0.001j, 1.1 + 0.02j])
+
+
+@pytest.mark.parametrize("kappa", [3.0, 6.0])
+def test_kappa_core_beam_extreme_kappas(kappa):
+    """The ends of the workload's kappa range, every species at that kappa."""
+    _disp_parity(_kappa_core_beam((kappa, kappa, kappa)), 0.3, 0.3,
+                 [0.6 - 0.05j, 0.2 - 0.001j, 1.1 + 0.02j])
+
+
+@pytest.mark.parametrize("tke,oms", [
+    (0.1, [0.1313 - 0.0041j, 0.3071 + 0.0213j]),
+    (0.3, [0.1313 - 0.0041j, 0.3071 + 0.0213j, 0.8133 - 0.0517j]),
+    (1.0, [0.4519 + 0.0311j, 0.5923 - 0.0087j, 0.0973 - 0.0011j])])
+def test_relativistic_workload_at_each_temperature(tke, oms):
+    """Juettner electrons at kT/mc^2 = 0.1, 0.3 and 1.0 beside cold protons,
+    200 x 400 (pperp, ppar) and 100 x 200 (Gamma, pparbar): the served
+    relativistic workload's plasmas at kpar = 0.5, kperp = 0.05.  At kT = m c^2
+    the oracle itself moves by 1e-9 .. 1e-7 under a 1e-13 relative change of
+    omega at most frequencies (a principal-value window of some Gamma row
+    sits on a cell boundary); the frequencies here are the quietest found, and
+    the bar is 1e-11 or ten times the oracle's own jump there, whichever is
+    larger."""
+    from alps_amd.engine import Engine
+    from alps_amd.plasma import RootParams
+    from bench import serve_rel_variant
+    p = serve_rel_variant(tke, RootParams(secant_method=2, numiter=50, D_threshold=1e-12))
+    O, E = Oracle(p), Engine(p)
+    assert O.set_k(0.05, 0.5) == E.set_k(0.05, 0.5)
+    det, chi, low, wave = E.disp_batch(np.array(oms), want_tensors=True)
+    for k, om in enumerate(oms):
+        d_o, chi_o, low_o, wave_o = O.disp(om, want_tensors=True)
+        chi_p = O.disp(om * (1.0 + 1e-13), want_tensors=True)[1]
+        for s in range(p.nspec):
+            sc = np.max(np.abs(chi_o[s]))
+            jump = np.max(np.abs(chi_p[s] - chi_o[s])) / sc
+            e = np.max(np.abs(chi[k, s] - chi_o[s])) / sc
+            print(tke, om, s, "err", e, "oracle jump", jump, flush=True)
+            assert e <= max(TOL, 10 * jump), (tke, om, s, e, jump)
+    E.close()
 
 
 def test_relativistic_workload_grid():

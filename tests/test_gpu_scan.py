@@ -94,3 +94,33 @@ def test_device_path_writes_the_kpar_fast_files(tmp_path):
     gold = np.loadtxt(GOLD_HEAT)
     unit = 10.0 ** (np.floor(np.log10(np.abs(gold) + 1e-300)) - 4)
     assert np.all(np.abs(got - gold) <= unit * (1 + 1e-9))
+
+
+@pytest.mark.parametrize("scan_type,log_scan", [(0, True), (0, False), (1, True), (1, False),
+                                                (2, True), (2, False)])
+def test_om_scan_types_0_to_2_match_oracle(scan_type, log_scan):
+    """om_scan (ALPS_fns.f90:2198-2600) along k1 -> k2 (type 0), in theta
+    (type 1) and in |k| at constant theta (type 2), log and linear steps, n_res
+    sub-steps: every output root against the oracle to 1e-10."""
+    p = _tight(cases.small_bimax())
+    O, E = Oracle(p), _engine(p)
+    O.set_k(*K0)
+    E.set_k(*K0)
+    om, _ = O.secant_osc(0.048 - 0.0001j)
+    theta = math.atan(K0[0] / K0[1])
+    if scan_type == 0:   # kperp 0.05 -> 0.1 and kpar 0.05 -> 0.15
+        d1 = (math.log10(0.1) - math.log10(K0[0])) / 6 if log_scan else (0.1 - K0[0]) / 6
+        d2 = (math.log10(0.15) - math.log10(K0[1])) / 6 if log_scan else (0.15 - K0[1]) / 6
+    elif scan_type == 1:  # theta to 1.2 theta
+        d1 = (math.log10(1.2 * theta) - math.log10(theta)) / 6 if log_scan else 0.2 * theta / 6
+        d2 = 0.0
+    else:                 # |k| to 2 |k|
+        k0 = math.hypot(*K0)
+        d1 = (math.log10(2 * k0) - math.log10(k0)) / 6 if log_scan else k0 / 6
+        d2 = 0.0
+    rc, ok, oz, oroots = O.om_scan(scan_type, log_scan, d1, d2, 3, 2, [om])
+    gk, gz, groots = E.om_scan(scan_type, log_scan, d1, d2, 3, 2, [om])
+    assert rc == 0
+    assert np.allclose(gk, ok, rtol=1e-14, atol=0) and np.allclose(gz, oz, rtol=1e-14, atol=0)
+    rel = np.abs(groots[:, 0] - oroots[:, 0]) / np.abs(oroots[:, 0])
+    assert rel.max() <= ROOT_TOL, rel

@@ -188,3 +188,26 @@ def test_relativistic_and_closed_form_plasmas_are_served():
         assert r[3] == alone[i][3] and r[1] == alone[i][1], (i, r, alone[i])
     for i in range(len(base)):
         assert got[i][3] in (OK, MAXITER)
+
+
+def test_served_roots_at_the_served_workload_grids_match_oracle():
+    """Requests of the served workload (3 species: protons of random anisotropy,
+    electrons, drifting alphas) on its 32 x 64 and 64 x 128 grids, against the
+    oracle's secant_osc to 1e-10."""
+    import bench
+    pool, reqs = bench.serve_requests(200, 64)
+    S = _server()
+    pick = [3, 4, 8, 11]   # converging requests of the generator's stream
+    for i in pick:
+        v, kp, kz, g = reqs[i]
+        S.submit(pool[v], kp, kz, g, pool[v].roots, tag=i)
+    got = S.drain(len(pick))
+    S.close()
+    for i in pick:
+        v, kp, kz, g = reqs[i]
+        O = Oracle(pool[v])
+        O.set_k(kp, kz)
+        w, it = O.secant_osc(g)
+        assert pool[v].nperp <= 64 and it < pool[v].roots.numiter
+        assert got[i][3] == OK and got[i][2] == it, (i, got[i], it)
+        assert abs(got[i][1] - w) <= ROOT_TOL * abs(w), (i, got[i][1], w)
