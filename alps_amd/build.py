@@ -17,7 +17,7 @@ FDIR = os.path.join(HERE, "fortran")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 SOURCES = ["alpsg_api.cpp", "alpsg_server.cpp", "alpsg_tables.hip", "alpsg_disp.hip", "alpsg_nhds.hip",
-           "alpsg_rel.hip", "alpsg_fitsetup.hip"]
+           "alpsg_rel.hip", "alpsg_fitsetup.hip", "alpsg_noalloc.hip"]
 # No packed-FP32 instructions; FP64 code is unaffected by this switch.
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -53,6 +53,7 @@ hipError_t launch_nhds(hipStream_t st, const ProbDev *prob, int nspec, int nom, 
 int nhds_tables(double z, int bm_nmax, double bessel_zero, std::vector<double2> &tab);
 hipError_t launch_lsum(hipStream_t st, int nperp, int nmax, double dpp, const double *pperp,
                        const double *fitc, const double *atab, double *lsum);
+hipError_t launch_fill_i32(hipStream_t st, int *p, int n, int v);
 hipError_t launch_split_tables(hipStream_t st, int npar, int nmax, double kpar, double q, double m,
                                const double *ppar, const double *nodes, double *crec,
                                double *rsum);
@@ -161,6 +162,7 @@ struct PinBytes {
 int alpsg_internal_fail(int code, const std::string &msg) { return fail(code, msg); }
 
 struct alpsg_ctx {
+  int device = 0;  // the device current at alpsg_create, which owns the stream and tables
   hipStream_t st = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t ev_sweep = nullptr, ev_res = nullptr;  // recorded after k_sweep, after k_resonant
@@ -365,7 +367,65 @@ bool shape_is_valid(const int *t) {
   return a && b && t[2] >= 1 && t[2] <= 64 && t[4] >= 1 && t[4] <= 16 && t[1] * t[2] <= 128;
 }
 
-// Runs the disp kernels on n omegas already resident in om_dev.
+// The device buffers of one disp chain (det, the partial-chi slots, and for
+// small batches the zero problem indices of the request sweep).
+struct DispBufs {
+  double2 *det, *chiA, *chiB, *chi, *low, *lowA, *lowB, *wave;
+  const int *zidx;
+};
+
+// The disp kernels on n omegas resident in om_dev, into the buffers of B
+// (sized by the caller; nothing is allocated here).  Relativistic species take
+// the handle's own buffers, so callers with B outside the handle exclude them.
+int disp_chain(alpsg_ctx *h, int n, const double2 *om_dev, const int *active_dev,
+               const DispBufs &B, bool want_chi, bool want_low, bool want_wave, LaunchShape sh) {
+  const int ns = h->nspec;
+  const int TW = sh.w_a * sh.rc_a;
+  // Small batches (a few roots' secant steps, the one-omega plain path): the
+  // sweep that keeps several node records in flight per lane and spreads each
+  // omega's row pairs over REQ_TW wavefronts (problem index 0 for every omega).
+  // The n_s-row partial layout is the same, so the rest of the chain is unchanged.
+  const bool req = !want_low && active_dev == nullptr && n <= REQ_MAX_N;
+  if (req) {
+    HIPCHK(launch_sweep_req(h->st, REQ_NCH, REQ_W, REQ_TW / REQ_W, REQ_K, h->prob_d.p, n, om_dev,
+                            B.chiA, B.zidx, true));
+  } else {
+    HIPCHK(launch_sweep(h->st, sh.nch_a, sh.w_a, sh.rc_a, h->prob_d.p, n, om_dev, active_dev,
+                        B.chiA, want_low ? B.lowA : nullptr));
+  }
+  const int TWa = req ? REQ_TW / REQ_W : TW;  // the request sweep writes one partial per workgroup
+  HIPCHK(hipEventRecord(h->ev_sweep, h->st));
+  if (req)
+    HIPCHK(launch_resonant_all(h->st, sh.nch_b, sh.rb, h->prob_d.p, ns, n, om_dev, B.chiB,
+                               h->any_general_fit));
+  else
+    HIPCHK(launch_resonant(h->st, sh.nch_b, sh.rb, h->prob_d.p, ns, n, om_dev, active_dev, B.chiB,
+                           want_low ? B.lowB : nullptr, h->any_general_fit));
+  if (h->any_bm)
+    HIPCHK(launch_nhds(h->st, h->prob_d.p, ns, n, om_dev, active_dev, B.chiA,
+                       want_low ? B.lowA : nullptr));
+  int RR = 0;
+  if (h->any_rel) {
+    const int nmaxR = h->prob.nmaxR;
+    const int NB = std::max(1, std::min(2 * nmaxR + 1, 1024 / std::max(n, 1)));
+    RR = rel_slots(NB, nmaxR);
+    HIPCHK(h->chiR_d.ensure((size_t)RR * ns * 6 * n));
+    HIPCHK(h->flags_d.ensure((size_t)n * ns * (2 * nmaxR + 1)));
+    if (want_low) HIPCHK(h->lowR_d.ensure((size_t)n * ns * 18));
+    HIPCHK(launch_rel(h->st, h->prob_d.p, ns, nmaxR, NB, n, om_dev, active_dev, h->chiR_d.p,
+                      want_low ? h->lowR_d.p : nullptr, h->flags_d.p, h->err_d.p));
+  }
+  HIPCHK(hipEventRecord(h->ev_res, h->st));
+  HIPCHK(launch_assemble(h->st, h->prob_d.p, ns, n, TWa, sh.rb, om_dev, active_dev, B.chiA, B.chiB,
+                         want_low ? B.lowA : nullptr, want_low ? B.lowB : nullptr,
+                         RR ? h->chiR_d.p : nullptr, RR, (RR && want_low) ? h->lowR_d.p : nullptr,
+                         B.det, want_chi ? B.chi : nullptr, want_low ? B.low : nullptr,
+                         want_wave ? B.wave : nullptr));
+  return ALPSG_OK;
+}
+
+// Runs the disp kernels on n omegas already resident in om_dev, in the
+// handle's buffers (grown on demand).
 int disp_device(alpsg_ctx *h, int n, const double2 *om_dev, const int *active_dev, bool want_chi,
                 bool want_low, bool want_wave, LaunchShape sh) {
   const int ns = h->nspec;
@@ -380,10 +440,6 @@ int disp_device(alpsg_ctx *h, int n, const double2 *om_dev, const int *active_de
     HIPCHK(h->lowB_d.ensure((size_t)n * ns * 2 * sh.rb * 18));
   }
   if (want_wave) HIPCHK(h->wave_d.ensure((size_t)n * 9));
-  // Small batches (a few roots' secant steps, the one-omega plain path): the
-  // sweep that keeps several node records in flight per lane and spreads each
-  // omega's row pairs over REQ_TW wavefronts (problem index 0 for every omega).
-  // The n_s-row partial layout is the same, so the rest of the chain is unchanged.
   const bool req = !want_low && active_dev == nullptr && n <= REQ_MAX_N;
   if (req) {
     HIPCHK(h->chiA_d.ensure((size_t)n * REQ_TW * ns * 6));
@@ -391,43 +447,66 @@ int disp_device(alpsg_ctx *h, int n, const double2 *om_dev, const int *active_de
       HIPCHK(h->zidx.ensure(REQ_MAX_N));
       HIPCHK(hipMemsetAsync(h->zidx.p, 0, sizeof(int) * REQ_MAX_N, h->st));
     }
-    HIPCHK(launch_sweep_req(h->st, REQ_NCH, REQ_W, REQ_TW / REQ_W, REQ_K, h->prob_d.p, n, om_dev,
-                            h->chiA_d.p, h->zidx.p, true));
-  } else {
-    HIPCHK(launch_sweep(h->st, sh.nch_a, sh.w_a, sh.rc_a, h->prob_d.p, n, om_dev, active_dev,
-                        h->chiA_d.p, want_low ? h->lowA_d.p : nullptr));
   }
-  const int TWa = req ? REQ_TW / REQ_W : TW;  // the request sweep writes one partial per workgroup
-  HIPCHK(hipEventRecord(h->ev_sweep, h->st));
-  if (req)
-    HIPCHK(launch_resonant_all(h->st, sh.nch_b, sh.rb, h->prob_d.p, ns, n, om_dev, h->chiB_d.p,
-                               h->any_general_fit));
-  else
-    HIPCHK(launch_resonant(h->st, sh.nch_b, sh.rb, h->prob_d.p, ns, n, om_dev, active_dev,
-                           h->chiB_d.p, want_low ? h->lowB_d.p : nullptr, h->any_general_fit));
-  if (h->any_bm)
-    HIPCHK(launch_nhds(h->st, h->prob_d.p, ns, n, om_dev, active_dev, h->chiA_d.p,
-                       want_low ? h->lowA_d.p : nullptr));
-  int RR = 0;
-  if (h->any_rel) {
-    const int nmaxR = h->prob.nmaxR;
-    const int NB = std::max(1, std::min(2 * nmaxR + 1, 1024 / std::max(n, 1)));
-    RR = rel_slots(NB, nmaxR);
-    HIPCHK(h->chiR_d.ensure((size_t)RR * ns * 6 * n));
-    HIPCHK(h->flags_d.ensure((size_t)n * ns * (2 * nmaxR + 1)));
-    if (want_low) HIPCHK(h->lowR_d.ensure((size_t)n * ns * 18));
-    HIPCHK(launch_rel(h->st, h->prob_d.p, ns, nmaxR, NB, n, om_dev, active_dev, h->chiR_d.p,
-                      want_low ? h->lowR_d.p : nullptr, h->flags_d.p, h->err_d.p));
-  }
-  HIPCHK(hipEventRecord(h->ev_res, h->st));
-  HIPCHK(launch_assemble(h->st, h->prob_d.p, ns, n, TWa, sh.rb, om_dev, active_dev, h->chiA_d.p,
-                         h->chiB_d.p, want_low ? h->lowA_d.p : nullptr,
-                         want_low ? h->lowB_d.p : nullptr, RR ? h->chiR_d.p : nullptr, RR,
-                         (RR && want_low) ? h->lowR_d.p : nullptr, h->det_d.p,
-                         want_chi ? h->chi_d.p : nullptr, want_low ? h->low_d.p : nullptr,
-                         want_wave ? h->wave_d.p : nullptr));
+  DispBufs B{h->det_d.p, h->chiA_d.p, h->chiB_d.p, h->chi_d.p, h->low_d.p, h->lowA_d.p,
+             h->lowB_d.p, h->wave_d.p, h->zidx.p};
+  return disp_chain(h, n, om_dev, active_dev, B, want_chi, want_low, want_wave, sh);
+}
+
+}  // namespace
+
+// det D for n omegas with every device buffer carved from the caller's
+// scratch (scratch_bytes at scratch_dev, on the handle's device): the chain
+// and launch shape of alpsg_disp_batch, nothing allocated.  Layout, each part
+// 256-byte aligned: om [n], det [n], chiA [n TW nspec 6], chiB [n nspec 2 RB 6]
+// (double2), and for n <= REQ_MAX_N the REQ_MAX_N zero problem indices.
+int alpsg_disp_on_scratch(alpsg_ctx *h, int n, const double *om, double *det,
+                          unsigned char *scratch_dev, size_t scratch_bytes) {
+  if (!h->have_k) return fail(ALPSG_E_STATE, "alpsg_set_k has not been called");
+  if (n < 0) return fail(ALPSG_E_ARG, "negative batch size");
+  if (n == 0) return ALPSG_OK;
+  if (h->any_rel || h->any_bm)
+    return fail(ALPSG_E_UNSUPPORTED,
+                "the no-allocation disp takes tabulated non-relativistic species only");
+  int dev = -1;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev != h->device) return fail(ALPSG_E_STATE, "call on the device the handle was created on");
+  const LaunchShape sh = shape_for(h, n);
+  const int ns = h->nspec;
+  const bool req = n <= REQ_MAX_N;
+  const size_t TW = req ? (size_t)REQ_TW : (size_t)sh.w_a * sh.rc_a;
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t b_om = up(sizeof(double2) * n), b_det = b_om;
+  const size_t b_a = up(sizeof(double2) * (size_t)n * TW * ns * 6);
+  const size_t b_b = up(sizeof(double2) * (size_t)n * ns * 2 * sh.rb * 6);
+  const size_t b_z = req ? up(sizeof(int) * REQ_MAX_N) : 0;
+  const size_t need = b_om + b_det + b_a + b_b + b_z;
+  if (need > scratch_bytes)
+    return fail(ALPSG_E_ARG, "batch of " + std::to_string(n) + " omegas needs " +
+                                 std::to_string(need) + " bytes of scratch; the no-allocation path has " +
+                                 std::to_string(scratch_bytes));
+  unsigned char *q = scratch_dev;
+  double2 *om_d = reinterpret_cast<double2 *>(q);
+  q += b_om;
+  double2 *det_d = reinterpret_cast<double2 *>(q);
+  q += b_det;
+  double2 *chiA = reinterpret_cast<double2 *>(q);
+  q += b_a;
+  double2 *chiB = reinterpret_cast<double2 *>(q);
+  q += b_b;
+  int *zidx = req ? reinterpret_cast<int *>(q) : nullptr;
+  HIPCHK(hipMemcpyAsync(om_d, om, sizeof(double2) * n, hipMemcpyHostToDevice, h->st));
+  if (req) HIPCHK(launch_fill_i32(h->st, zidx, REQ_MAX_N, 0));
+  const DispBufs B{det_d, chiA, chiB, nullptr, nullptr, nullptr, nullptr, nullptr, zidx};
+  const int rc = disp_chain(h, n, om_d, nullptr, B, false, false, false, sh);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(det, det_d, sizeof(double2) * n, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  h->evals += n;
   return ALPSG_OK;
 }
+
+namespace {
 
 // Autotuner: times the node-sweep variants (lanes per omega x wavefronts per
 // workgroup x workgroups per omega block) and the resonant-kernel variants
@@ -1498,6 +1577,7 @@ int alpsg_create(const alpsg_plasma_desc *d, alpsg_handle *out) {
       if (!(pa[j + 1] > pa[j]) || std::fabs((pa[j + 1] - pa[j]) - dpa) > 1e-6 * dpa)
         return bail(fail(ALPSG_E_UNSUPPORTED, "ppar grid is not uniform"));
   }
+  HIPCHK(hipGetDevice(&h->device));
   HIPCHK(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
   HIPCHK(hipEventCreate(&h->ev0));
   HIPCHK(hipEventCreate(&h->ev1));

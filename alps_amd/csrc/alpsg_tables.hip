@@ -665,9 +665,22 @@ __global__ __launch_bounds__(64) void k_moments_mfma_jobs(const TabJob *__restri
                     J.df2, J.atab, J.nodes);
 }
 
+// v into p[0..n-1] (a kernel rather than a memset node, so that the chain
+// can also run inside a caller's graph capture)
+__global__ void k_fill_i32(int *__restrict__ p, int n, int v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 // ---------------------------------------------------------------------------
 // Host launchers (called from alpsg_api.cpp).
 // ---------------------------------------------------------------------------
+hipError_t launch_fill_i32(hipStream_t st, int *p, int n, int v) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_i32, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v);
+  return hipGetLastError();
+}
+
 hipError_t launch_derivative(hipStream_t st, int nspec, int nperp, int npar, const double *f0,
                              const double *pperp, const double *ppar, double *df1, double *df2) {
   dim3 grid((npar + 1 + 127) / 128, nperp + 1, nspec);

@@ -46,7 +46,7 @@ EXPORTS = ["alpsg_last_error", "alpsg_create", "alpsg_destroy", "alpsg_set_k",
            "alpsg_server_submit_batch", "alpsg_om_double_scan",
            "alpsg_server_submit", "alpsg_server_poll", "alpsg_server_stats",
            "alpsg_server_destroy", "alpsg_determine_param_fit", "alpsg_last_scan_steps",
-           "alpsg_set_plain_roots"]
+           "alpsg_set_plain_roots", "alpsg_disp_batch_noalloc"]
 
 
 class AlpsgError(RuntimeError):
@@ -111,6 +111,8 @@ def load_library(path: str = LIB_PATH):
     L.alpsg_set_k.argtypes = [vp, C.c_double, C.c_double, ip, ip]
     L.alpsg_disp_batch.argtypes = [vp, C.c_int, dp, dp, dp, dp, dp]
     L.alpsg_disp_plain.argtypes = [vp, C.c_int, dp, dp]
+    L.alpsg_disp_batch_noalloc.argtypes = [vp, C.c_int, dp, dp]
+    L.alpsg_disp_batch_noalloc.restype = C.c_int
     L.alpsg_secant_batch.argtypes = [vp, C.POINTER(_RootParams), C.c_int, dp, dp, ip]
     L.alpsg_map_search.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
                                    C.c_int, C.c_int, C.c_int, dp, dp, dp, C.c_int, ip, ip]
@@ -346,6 +348,14 @@ class Engine:
     def disp(self, om):
         """det D at one complex frequency (ALPS_fns.f90:252)."""
         return complex(self.disp_batch([om])[0])
+
+    def disp_batch_noalloc(self, om):
+        """det D at every omega through alpsg_disp_batch_noalloc (no device
+        allocation: the library's fixed per-GPU scratch array)."""
+        om = np.ascontiguousarray(np.asarray(om, dtype=np.complex128).ravel())
+        det = np.zeros(om.size, dtype=np.complex128)
+        _check(load_library().alpsg_disp_batch_noalloc(self._h, om.size, _dp(om), _dp(det)))
+        return det
 
     def disp_plain(self, om):
         om = np.ascontiguousarray(np.asarray(om, dtype=np.complex128).ravel())

@@ -149,6 +149,19 @@ int alpsg_disp_batch(alpsg_handle h, int n, const double *om, double *det, doubl
 /* The same evaluation with one launch per omega (the plain path). */
 int alpsg_disp_plain(alpsg_handle h, int n, const double *om, double *det);
 
+/* alpsg_disp_batch (det only) without any device allocation: no counterpart
+   in the reference; for callers that cannot allocate (code under HIP graph
+   capture, tight serving loops, harnesses that forbid hipMalloc).  The
+   kernels and launch shape are those of alpsg_disp_batch; the buffers beyond
+   om and det come from a fixed 32 MiB device array per GPU, held until the
+   process ends.  A batch that needs more (n beyond a few thousand omegas,
+   depending on nspec and the launch shape) is refused with ALPSG_E_ARG;
+   handles with relativistic or closed-form species with
+   ALPSG_E_UNSUPPORTED.  Calls on the handle's device only; calls that
+   overlap on one GPU share the array and race, so the caller serialises
+   them.  The handle itself (alpsg_create, alpsg_set_k) allocates as usual. */
+int alpsg_disp_batch_noalloc(alpsg_handle h, int n, const double *om, double *det);
+
 /* Batched root refinement for nroots independent roots at the handle's k with
    the solver rp->secant_method selects: secant (ALPS_fns.f90:1815-1917),
    rtsec (:2105-2195) or secant_osc (:1919-2101).  om [2*nroots] in/out.
