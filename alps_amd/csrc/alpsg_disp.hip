@@ -794,20 +794,24 @@ __device__ __forceinline__ int ipar_close_step(const SpecDev &S, Cursor &cu, dou
     const double half = 0.5 * S.dppar_abs;
     const int npar = S.npar;
     int best = -1;
+    // the candidates' grid values are loaded together (every index is inside
+    // the grid), then the highest match is taken, as the descending search does
     if (dir > 0) {
       // x increases: candidates ic+3 .. ic (descending; two indices can tie)
-      for (int k = min(cu.ic + 3, npar - 1); k >= cu.ic; --k)
-        if (fabs(pa[k] - x) <= half) {
-          best = k;
-          break;
-        }
+      double v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = (cu.ic + u <= npar - 1) ? pa[cu.ic + u] : 0.0;
+#pragma unroll
+      for (int u = 3; u >= 0; --u)
+        if (best < 0 && cu.ic + u <= npar - 1 && fabs(v[u] - x) <= half) best = cu.ic + u;
     } else {
       // x decreases: candidates ic, ic-1, ic-2 (descending)
-      for (int k = cu.ic; k >= max(cu.ic - 2, 1); --k)
-        if (fabs(pa[k] - x) <= half) {
-          best = k;
-          break;
-        }
+      double v[3];
+#pragma unroll
+      for (int u = 0; u < 3; ++u) v[u] = (cu.ic - u >= 1) ? pa[cu.ic - u] : 0.0;
+#pragma unroll
+      for (int u = 0; u < 3; ++u)
+        if (best < 0 && cu.ic - u >= 1 && fabs(v[u] - x) <= half) best = cu.ic - u;
     }
     if (best >= 0) {
       int r = best;
@@ -853,13 +857,13 @@ __device__ __forceinline__ void cache_flush(CellCache &cc, double *win, int lo) 
 
 // add wt * interp(x) to the coefficient window (through the cell cache)
 __device__ __forceinline__ void coef_add(const SpecDev &S, CellCache &cc, Cursor &cu, double *win,
-                                         int lo, double x, int dir, cd wt) {
+                                         int lo, double x, int dir, cd wt, double hinv) {
   const int ic = ipar_close_step(S, cu, x, dir);
   if (ic != cc.ic) {
     cache_flush(cc, win, lo);
     cc.ic = ic;
   }
-  const double f = 0.5 * (x - S.ppar[ic]) / S.dppar_abs;
+  const double f = (x - S.ppar[ic]) * hinv;  // funct_g's 0.5 (x - p_c) / dppar
   cc.c0 = cc.c0 + wt;
   cc.cp = cc.cp + f * wt;
   cc.cm = cc.cm - f * wt;
@@ -878,6 +882,7 @@ __device__ void pv_coefs(const SpecDev &S, const ProbDev &Pd, const RowClass &rc
     capDelta = pr - S.ppar[rc.ipar_res - P];
     smdelta = capDelta / (1.0 * NRI);
   }
+  const double hinv = 0.5 / S.dppar_abs;
   CellCache cd1, cd2;
   Cursor u1, u2;
   u1.ic = u2.ic = -1;
@@ -895,8 +900,8 @@ __device__ void pv_coefs(const SpecDev &S, const ProbDev &Pd, const RowClass &rc
       }
       const double w = (j == 0 || j == NRI) ? 1.0 : 2.0;
       const cd r1 = cscale_inv(w, mk(x - pr, -pim));
-      coef_add(S, cd1, u1, win, lo, x, 1, r1);
-      coef_add(S, cd2, u2, win, lo, xm, -1, mk(-r1.re, r1.im));  // - conj(r1)
+      coef_add(S, cd1, u1, win, lo, x, 1, r1, hinv);
+      coef_add(S, cd2, u2, win, lo, xm, -1, mk(-r1.re, r1.im), hinv);  // - conj(r1)
     }
   } else {
     const int blk = (NRI + NCH - 1) / NCH;
@@ -921,13 +926,13 @@ __device__ void pv_coefs(const SpecDev &S, const ProbDev &Pd, const RowClass &rc
         xq = pr - S.dppar;
       }
       const double f = 2.0 * sphi / (2.0 * S.dppar);
-      coef_add(S, cd1, u1, win, lo, xp, 1, mk(f, 0.0));
-      coef_add(S, cd2, u2, win, lo, xq, -1, mk(-f, 0.0));
+      coef_add(S, cd1, u1, win, lo, xp, 1, mk(f, 0.0), hinv);
+      coef_add(S, cd2, u2, win, lo, xq, -1, mk(-f, 0.0), hinv);
       if (pim != 0.0) {
         const double sg = (pim > 0.0) ? 1.0 : -1.0;
         cache_flush(cd1, win, lo);
         u1.ic = -1;
-        coef_add(S, cd1, u1, win, lo, pr, 1, mk(0.0, sg * 2.0 * PI_D / smdelta));
+        coef_add(S, cd1, u1, win, lo, pr, 1, mk(0.0, sg * 2.0 * PI_D / smdelta), hinv);
       }
     }
   }
@@ -956,7 +961,7 @@ __device__ void pv_coefs(const SpecDev &S, const ProbDev &Pd, const RowClass &rc
         x = (kk == 0) ? x0 : x0 + cs * kk;
       }
       const double w = (kk == 0 || kk == ntiny) ? 1.0 : 2.0;
-      coef_add(S, cd1, u1, win, lo, x, 1, cscale_inv(corr * w, mk(x - pr, -pim)));
+      coef_add(S, cd1, u1, win, lo, x, 1, cscale_inv(corr * w, mk(x - pr, -pim)), hinv);
     }
   }
   cache_flush(cd1, win, lo);
@@ -1413,8 +1418,8 @@ hipError_t launch_sweep_req(hipStream_t st, int nch, int W, int RC, int K, const
                        nom, om, chiA, pidx);                                                      \
     return hipGetLastError();                                                                     \
   }
-  SQ(16, 8, 1) SQ(32, 8, 1) SQ(16, 8, 2) SQ(32, 8, 2) SQ(16, 8, 4) SQ(32, 8, 4) SQ(32, 4, 4)
-  SQ(64, 4, 2) SQ(64, 4, 4)
+  SQ(16, 8, 1) SQ(32, 8, 1) SQ(16, 8, 2) SQ(32, 8, 2) SQ(16, 8, 4) SQ(32, 8, 4) SQ(32, 4, 2)
+  SQ(32, 4, 4) SQ(64, 4, 2) SQ(64, 4, 4)
 #undef SQ
   return hipErrorInvalidValue;
 }

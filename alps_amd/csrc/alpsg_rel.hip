@@ -270,8 +270,9 @@ __device__ __forceinline__ cd rel_num(const RelCtx &C, int pt) {
   return C.S->rel_pref * t;
 }
 
-// funct_g_rel (ALPS_fns_rel.f90:918-999) for the six modes at real pparbar x of row ig
-__device__ void funct_g_rel6(const RelCtx &C, int ig, double x, cd g[6]) {
+// funct_g_rel's interpolation cell (ALPS_fns_rel.f90:918-999): the index
+// close of row ig whose neighbours close-1, close+1 it interpolates at real x
+__device__ int rel_close(const RelCtx &C, int ig, double x) {
   const SpecDev &S = *C.S;
   const int npb = C.npb, row = ig * (npb + 1);
   const double *pb = S.pbrel + row;
@@ -293,6 +294,15 @@ __device__ void funct_g_rel6(const RelCtx &C, int ig, double x, cd g[6]) {
   if (x == pb[npb]) close = npb - 2;
   if (close >= npb - 1) close = npb - 2;
   if (close <= 1) close = 2;
+  return close;
+}
+
+// funct_g_rel (ALPS_fns_rel.f90:918-999) for the six modes at real pparbar x of row ig
+__device__ void funct_g_rel6(const RelCtx &C, int ig, double x, cd g[6]) {
+  const SpecDev &S = *C.S;
+  const int npb = C.npb, row = ig * (npb + 1);
+  const double *pb = S.pbrel + row;
+  const int close = rel_close(C, ig, x);
   cd Tp[6], Tm[6], Tc[6];
   int_T_rel6(C, row + close + 1, Tp);
   int_T_rel6(C, row + close, Tc);
@@ -320,7 +330,57 @@ __device__ __forceinline__ void rel_point(const RelCtx &C, int ig, int ip, doubl
   for (int k = 0; k < 6; ++k) acc[k] = acc[k] + w * (U * T[k]);
 }
 
-// principal_integral_rel (ALPS_fns_rel.f90:724-913), six modes, added to acc
+// funct_g_rel is linear in the values g_k(node) = -(rel_num T_k) at the three
+// nodes of its cell (close - 1, close, close + 1): g(x) = g(close) +
+// f (g(close+1) - g(close-1)), f = 0.5 (x - pparbar(close)) / dpparbar.  A
+// weighted sum of g over a monotone ladder of x is therefore the sum over the
+// cells it passes of three coefficients times the node values: RelCell holds
+// the open cell's coefficients, and the node values are evaluated once per
+// cell, when the ladder leaves it.
+struct RelCell {
+  int close;  // -1: empty
+  cd cm, c0, cp;
+};
+
+__device__ __forceinline__ void rel_node_g(const RelCtx &C, int pt, cd g[6]) {
+  cd T[6];
+  int_T_rel6(C, pt, T);
+  const cd nm = rel_num(C, pt);
+  for (int k = 0; k < 6; ++k) g[k] = -1.0 * (nm * T[k]);
+}
+
+__device__ void rel_cell_flush(const RelCtx &C, int ig, RelCell &cc, cd res[6]) {
+  if (cc.close < 0) return;
+  const int pt = ig * (C.npb + 1) + cc.close;
+  cd g[6];
+  rel_node_g(C, pt, g);
+  for (int k = 0; k < 6; ++k) res[k] = res[k] + cc.c0 * g[k];
+  rel_node_g(C, pt + 1, g);
+  for (int k = 0; k < 6; ++k) res[k] = res[k] + cc.cp * g[k];
+  rel_node_g(C, pt - 1, g);
+  for (int k = 0; k < 6; ++k) res[k] = res[k] + cc.cm * g[k];
+  cc.close = -1;
+  cc.cm = cc.c0 = cc.cp = mk(0.0, 0.0);
+}
+
+// add wt g(x) through the cell cache
+__device__ __forceinline__ void rel_cell_add(const RelCtx &C, int ig, RelCell &cc, double x, cd wt,
+                                             cd res[6]) {
+  const int close = rel_close(C, ig, x);
+  if (close != cc.close) {
+    rel_cell_flush(C, ig, cc, res);
+    cc.close = close;
+  }
+  const double f = 0.5 * ((x - C.S->pbrel[ig * (C.npb + 1) + close]) / C.S->rel_dpb);
+  cc.c0 = cc.c0 + wt;
+  cc.cp = cc.cp + f * wt;
+  cc.cm = cc.cm - f * wt;
+}
+
+// principal_integral_rel (ALPS_fns_rel.f90:724-913), six modes, added to acc.
+// The ladder sums go through cell coefficients (RelCell), with the ladder's
+// points visited in increasing x (the far end of each ladder after its inner
+// points), so that each cell is opened once.
 __device__ void principal_rel6(const RelCtx &C, int ig, int ipres, int upperlimit, double Tlim,
                                int pos, int nri, cd acc[6]) {
   const SpecDev &S = *C.S;
@@ -338,16 +398,16 @@ __device__ void principal_rel6(const RelCtx &C, int ig, int ipres, int upperlimi
   cd res[6];
   for (int k = 0; k < 6; ++k) res[k] = mk(0.0, 0.0);
   cd g[6], g2[6];
+  RelCell up, dn;  // x = denomR + t (increasing) and its mirror 2 denomR - x (decreasing)
+  up.close = dn.close = -1;
+  up.cm = up.c0 = up.cp = dn.cm = dn.c0 = dn.cp = mk(0.0, 0.0);
   auto pv_pair = [&](double x, double wgt) {
-    funct_g_rel6(C, ig, x, g);
-    funct_g_rel6(C, ig, 2.0 * denomR - x, g2);
     const cd dm = mk(x - denomR, -denomI), dp = mk(x - denomR, denomI);
-    for (int k = 0; k < 6; ++k)
-      res[k] = (res[k] + cdivc(wgt * g[k], dm)) - cdivc(wgt * g2[k], dp);
+    rel_cell_add(C, ig, up, x, cdivc(mk(wgt, 0.0), dm), res);
+    rel_cell_add(C, ig, dn, 2.0 * denomR - x, -1.0 * cdivc(mk(wgt, 0.0), dp), res);
   };
   if (fabs(denomI) > Tlim) {
     pv_pair(denomR, 1.0);
-    pv_pair(denomR + capDelta, 1.0);
     for (int ip = 1; ip <= nri - 1; ++ip) {
       double x;
       {
@@ -356,6 +416,9 @@ __device__ void principal_rel6(const RelCtx &C, int ig, int ipres, int upperlimi
       }
       pv_pair(x, 2.0);
     }
+    pv_pair(denomR + capDelta, 1.0);
+    rel_cell_flush(C, ig, up, res);
+    rel_cell_flush(C, ig, dn, res);
   } else {
     funct_g_rel6(C, ig, denomR + S.rel_dpb, g);
     funct_g_rel6(C, ig, denomR - S.rel_dpb, g2);
@@ -399,9 +462,8 @@ __device__ void principal_rel6(const RelCtx &C, int ig, int ipres, int upperlimi
       correction = (base / (1.0 * ntiny)) / smdelta;
     }
     auto tiny = [&](double x, double wgt) {
-      funct_g_rel6(C, ig, x, g);
       const cd dm = mk(x - denomR, -denomI);
-      for (int k = 0; k < 6; ++k) res[k] = res[k] + (wgt * correction) * cdivc(g[k], dm);
+      rel_cell_add(C, ig, up, x, (wgt * correction) * cdivc(mk(1.0, 0.0), dm), res);
     };
     double x0, x1;
     {
@@ -410,7 +472,6 @@ __device__ void principal_rel6(const RelCtx &C, int ig, int ipres, int upperlimi
       x1 = denomR + capDelta + correction * smdelta * ntiny;
     }
     tiny(x0, 1.0);
-    tiny(x1, 1.0);
     for (int ip = 1; ip <= ntiny - 1; ++ip) {
       double x;
       {
@@ -419,6 +480,8 @@ __device__ void principal_rel6(const RelCtx &C, int ig, int ipres, int upperlimi
       }
       tiny(x, 2.0);
     }
+    tiny(x1, 1.0);
+    rel_cell_flush(C, ig, up, res);
   }
   for (int k = 0; k < 6; ++k) acc[k] = acc[k] + res[k] * smdelta;
 }

@@ -301,6 +301,17 @@ struct alpsg_server {
   std::thread th;
   // statistics, written by the service thread only
   long long passes = 0, evals = 0, uploads = 0, evictions = 0;
+  // algorithmic units of the evaluated omegas: node records of the sweep
+  // (species, harmonic, ppar node), relativistic 2-D points x harmonics, and
+  // relativistic Gamma rows x harmonics
+  double work[3] = {0.0, 0.0, 0.0};
+  // device time (ms) by kernel class, from events around each group's launches:
+  // the node sweep, the resonant parts (and closed-form species), the
+  // relativistic kernels, the assembly
+  double kt[4] = {0.0, 0.0, 0.0, 0.0};
+  int pass_groups = 0;
+  static constexpr int NEV = 8;  // groups per pass with events (species counts 1..8)
+  hipEvent_t ev[NEV][5] = {};
   // host time (ms): whole passes, admission work, disp launch enqueue, waits on the device
   double t_pass = 0, t_admit = 0, t_launch = 0, t_wait = 0;
   // the copy alpsg_server_stats reads: the service thread publishes it under
@@ -309,6 +320,8 @@ struct alpsg_server {
   struct Snap {
     long long passes = 0, evals = 0, uploads = 0, evictions = 0, live = 0, arena_used = 0;
     double t_pass = 0, t_admit = 0, t_launch = 0, t_wait = 0;
+    double work[3] = {0.0, 0.0, 0.0};
+    double kt[4] = {0.0, 0.0, 0.0, 0.0};
   } snap;
 };
 
@@ -904,6 +917,8 @@ void post_results(alpsg_server *S, std::vector<Result> &done) {
   sn.t_admit = S->t_admit;
   sn.t_launch = S->t_launch;
   sn.t_wait = S->t_wait;
+  for (int k = 0; k < 3; ++k) sn.work[k] = S->work[k];
+  for (int k = 0; k < 4; ++k) sn.kt[k] = S->kt[k];
   if (done.empty()) return;
   for (auto &d : done) S->results.push_back(d);
   S->finished += (long long)done.size();
@@ -994,6 +1009,17 @@ int server_pass(alpsg_server *S) {
     const cplx w(wz.re, wz.im);
     S->om_h[i] = make_double2(w.real(), w.imag());
     S->pidx_h[i] = act[i]->slot;
+    const Plasma &P = *act[i]->pl;
+    for (int s = 0; s < P.nspec; ++s) {
+      if (P.isbm[s]) continue;
+      const double rows = 2.0 * act[i]->nmax[s] + 1.0;
+      if (P.isrel[s]) {
+        S->work[1] += rows * (P.nperp - 1) * (P.npar - 1);
+        S->work[2] += rows * (P.ngamma - 1);
+      } else {
+        S->work[0] += rows * (P.npar - 1);
+      }
+    }
   }
   bool pass_rel = false;
   if (nb > 0) {
@@ -1029,10 +1055,13 @@ int server_pass(alpsg_server *S) {
     SCHK(hipMemcpyAsync(S->om_d, S->om_h, sizeof(double2) * nb, hipMemcpyHostToDevice, S->st));
     SCHK(hipMemcpyAsync(S->pidx_d, S->pidx_h, sizeof(int) * nb, hipMemcpyHostToDevice, S->st));
     if (pass_rel) SCHK(hipMemsetAsync(S->err_d, 0, sizeof(int) * nb, S->st));
-    int i0 = 0;
+    int i0 = 0, ng = 0;
     while (i0 < nb) {
       int i1 = i0;
       const int ns = act[i0]->pl->nspec;
+      hipEvent_t *ev = ng < alpsg_server::NEV ? S->ev[ng] : nullptr;
+      ++ng;
+      if (ev) SCHK(hipEventRecord(ev[0], S->st));
       bool gen = false, bm = false;
       int relmax = -1;
       for (; i1 < nb && act[i1]->pl->nspec == ns; ++i1) {
@@ -1043,6 +1072,7 @@ int server_pass(alpsg_server *S) {
       const int n = i1 - i0;
       SCHK(launch_sweep_req(S->st, SW_NCH, SW_W, SW_RC, SW_K, S->probs_d, n, S->om_d + i0,
                             S->chiA, S->pidx_d + i0, false));
+      if (ev) SCHK(hipEventRecord(ev[1], S->st));
       SCHK(launch_resonant(S->st, 64, RS_RB, S->probs_d, ns, n, S->om_d + i0, nullptr, S->chiB,
                            nullptr, gen, S->pidx_d + i0));
       // closed-form species: partial slot 0 of their chi (the sweep left it zero)
@@ -1052,16 +1082,20 @@ int server_pass(alpsg_server *S) {
       // relativistic species: REL_NB + 2 relmax + 1 partial slots; an omega whose
       // problem has fewer harmonics gets zeros in the slots past its own
       int RR = 0;
+      if (ev) SCHK(hipEventRecord(ev[2], S->st));
       if (relmax >= 0) {
         RR = rel_slots(REL_NB, relmax);
         SCHK(launch_rel(S->st, S->probs_d, ns, relmax, REL_NB, n, S->om_d + i0, nullptr, S->chiR,
                         nullptr, S->flags_d, S->err_d + i0, S->pidx_d + i0));
       }
+      if (ev) SCHK(hipEventRecord(ev[3], S->st));
       SCHK(launch_assemble(S->st, S->probs_d, ns, n, SW_W * SW_RC, RS_RB, S->om_d + i0, nullptr,
                            S->chiA, S->chiB, nullptr, nullptr, RR ? S->chiR : nullptr, RR, nullptr,
                            S->det_d + i0, nullptr, nullptr, nullptr, S->pidx_d + i0));
+      if (ev) SCHK(hipEventRecord(ev[4], S->st));
       i0 = i1;
     }
+    S->pass_groups = std::min(ng, (int)alpsg_server::NEV);
     SCHK(hipMemcpyAsync(S->det_h, S->det_d, sizeof(double2) * nb, hipMemcpyDeviceToHost, S->st));
     if (pass_rel) SCHK(hipMemcpyAsync(S->err_h, S->err_d, sizeof(int) * nb, hipMemcpyDeviceToHost, S->st));
   }
@@ -1070,6 +1104,12 @@ int server_pass(alpsg_server *S) {
   const double t2 = now_ms();
   S->t_launch += t1 - t0;
   S->t_wait += t2 - t1;
+  for (int g = 0; g < (nb > 0 ? S->pass_groups : 0); ++g)
+    for (int k = 0; k < 4; ++k) {
+      float ms = 0.0f;
+      SCHK(hipEventElapsedTime(&ms, S->ev[g][k], S->ev[g][k + 1]));
+      S->kt[k] += ms;
+    }
   S->passes += 1;
   S->evals += nb;
   // the readback of this pass: int_ee of the uploaded plasmas, max|J_n| of the admissions
@@ -1251,6 +1291,8 @@ int alpsg_server_create(double hbm_bytes, int max_slots, alpsg_server **out) {
     }
   } guard{S};
   SCHK(hipStreamCreateWithFlags(&S->st, hipStreamNonBlocking));
+  for (auto &row : S->ev)
+    for (auto &e : row) SCHK(hipEventCreate(&e));
   const size_t cap = (size_t)hbm_bytes / ALIGN * ALIGN;
   SCHK(hipMalloc((void **)&S->arena_mem, cap));
   S->arena.init(S->arena_mem, cap);
@@ -1423,6 +1465,16 @@ int alpsg_server_stats(alpsg_server *S, long long *counts, double *times_ms) {
   ALPSG_CATCH
 }
 
+int alpsg_server_work(alpsg_server *S, double *work7) {
+  ALPSG_TRY
+  if (!S || !work7) return alpsg_internal_fail(ALPSG_E_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(S->mu);
+  for (int k = 0; k < 3; ++k) work7[k] = S->snap.work[k];
+  for (int k = 0; k < 4; ++k) work7[3 + k] = S->snap.kt[k];
+  return ALPSG_OK;
+  ALPSG_CATCH
+}
+
 int alpsg_server_destroy(alpsg_server *S) {
   ALPSG_TRY
   if (!S) return ALPSG_OK;
@@ -1457,6 +1509,9 @@ int alpsg_server_destroy(alpsg_server *S) {
   if (S->gtab_d) (void)hipFree(S->gtab_d);
   if (S->chiR) (void)hipFree(S->chiR);
   if (S->flags_d) (void)hipFree(S->flags_d);
+  for (auto &row : S->ev)
+    for (auto &e : row)
+      if (e) (void)hipEventDestroy(e);
   if (S->st) (void)hipStreamDestroy(S->st);
   delete S;
   return ALPSG_OK;

@@ -46,7 +46,7 @@ EXPORTS = ["alpsg_last_error", "alpsg_create", "alpsg_destroy", "alpsg_set_k",
            "alpsg_server_submit_batch", "alpsg_om_double_scan",
            "alpsg_server_submit", "alpsg_server_poll", "alpsg_server_stats",
            "alpsg_server_destroy", "alpsg_determine_param_fit", "alpsg_last_scan_steps",
-           "alpsg_set_plain_roots", "alpsg_disp_batch_noalloc"]
+           "alpsg_set_plain_roots", "alpsg_disp_batch_noalloc", "alpsg_server_work"]
 
 
 class AlpsgError(RuntimeError):
@@ -149,6 +149,8 @@ def load_library(path: str = LIB_PATH):
     L.alpsg_server_poll.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_longlong), dp, ip, ip,
                                     dp, ip]
     L.alpsg_server_stats.argtypes = [vp, C.POINTER(C.c_longlong), dp]
+    L.alpsg_server_work.argtypes = [vp, dp]
+    L.alpsg_server_work.restype = C.c_int
     L.alpsg_server_destroy.argtypes = [vp]
     for name in ["alpsg_server_create", "alpsg_server_submit", "alpsg_server_poll",
                  "alpsg_server_stats", "alpsg_server_destroy", "alpsg_server_add_plasma",
@@ -634,6 +636,11 @@ class RootServer:
                 "arena_bytes", "arena_used_bytes", "plasma_uploads", "plasma_evictions")
         out = {k: int(v) for k, v in zip(keys, c)}
         out.update(pass_ms=t[0], admit_ms=t[1], launch_ms=t[2], wait_ms=t[3])
+        w = np.zeros(7)
+        _check(load_library().alpsg_server_work(self._h, _dp(w)))
+        out.update(work_nodes=float(w[0]), work_rel_points=float(w[1]), work_rel_rows=float(w[2]),
+                   sweep_ms=float(w[3]), resonant_ms=float(w[4]), rel_ms=float(w[5]),
+                   assemble_ms=float(w[6]))
         return out
 
     def close(self):

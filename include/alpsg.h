@@ -327,6 +327,16 @@ int alpsg_server_submit(alpsg_server *s, const alpsg_plasma_desc *desc, double k
 int alpsg_server_poll(alpsg_server *s, int max_results, int wait, long long *tags, double *roots,
                       int *iters, int *status, double *latency_ms, int *n_out);
 int alpsg_server_stats(alpsg_server *s, long long *counts, double *times_ms);
+/* The algorithmic units of every disp evaluation the server has run and the
+   device time they took (a diagnostic for rooflines, no counterpart in the
+   reference): work7[0] the node records of the sweep, sum over evaluations and
+   tabulated non-relativistic species of (2 nmax + 1)(npar - 1); work7[1] the
+   relativistic points x harmonics, (2 nmax + 1)(nperp - 1)(npar - 1); work7[2]
+   the relativistic Gamma rows x harmonics, (2 nmax + 1)(ngamma - 1);
+   work7[3..6] the device ms, from events around each pass's launches, of the
+   node sweep, the resonant parts (with closed-form species), the relativistic
+   kernels and the assembly. */
+int alpsg_server_work(alpsg_server *s, double *work7);
 int alpsg_server_destroy(alpsg_server *s);
 
 #ifdef __cplusplus
