@@ -41,7 +41,7 @@ hipError_t launch_species_k(hipStream_t st, int nperp, int npar, int nmax, doubl
                             const int *mtab, double *bes, double *atab, double *nodes);
 hipError_t launch_sweep(hipStream_t st, int nch, int W, int RC, const ProbDev *prob, int nom,
                         const double2 *om, const int *active, double2 *chiA, double2 *lowA,
-                        const int *pidx = nullptr);
+                        const int *pidx = nullptr, unsigned long long *work = nullptr);
 hipError_t launch_sweep_req(hipStream_t st, int nch, int W, int RC, int K, const ProbDev *prob,
                             int nom, const double2 *om, double2 *chiA, const int *pidx,
                             bool reduce);
@@ -56,7 +56,7 @@ hipError_t launch_lsum(hipStream_t st, int nperp, int nmax, double dpp, const do
 hipError_t launch_fill_i32(hipStream_t st, int *p, int n, int v);
 hipError_t launch_split_tables(hipStream_t st, int npar, int nmax, double kpar, double q, double m,
                                const double *ppar, const double *nodes, double *crec,
-                               double *rsum);
+                               double *rsum, double *ser, double *serD);
 hipError_t launch_rel_bessel(hipStream_t st, int npts, int nmax, double zfac, const double *grel,
                              const double *pbrel, const int *mtab, double *out);
 hipError_t launch_map_post(hipStream_t st, int nr, int ni, const double2 *cal, double2 *calo,
@@ -185,8 +185,11 @@ struct alpsg_ctx {
   // per-k device data
   DevBuf<double> bes[ALPSG_MAXSPEC], atab[ALPSG_MAXSPEC], nodes[ALPSG_MAXSPEC];
   DevBuf<double> crec[ALPSG_MAXSPEC], rsum[ALPSG_MAXSPEC];  // split-form sweep tables
+  DevBuf<double> ser[ALPSG_MAXSPEC], serD[ALPSG_MAXSPEC];  // far-row series tables
+  bool series_on = true;  // far rows of the one-lane sweep take their series
   DevBuf<double> lsum[ALPSG_MAXSPEC];  // Landau sums of single-Maxwellian species
   DevBuf<int> zidx;  // zeros: the problem index of every omega of a small batch
+  DevBuf<unsigned long long> work_d;  // the sweep's work counts (alpsg_sweep_work)
   PinBuf om_pin, det_pin;  // staging of om and det for batches of at most PIN_MAX omegas
   // captured disp chains of small batches (index n = 1..REQ_MAX_N), with the
   // shape and buffers they were captured with; graphs_ok turns off after a
@@ -266,6 +269,7 @@ struct alpsg_ctx {
     osc_d.release();
     oscctl_d.release();
     if (oscctl_pin) (void)hipHostFree(oscctl_pin);
+    work_d.release();
     zidx.release();
     mcal_d.release();
     mval_d.release();
@@ -276,6 +280,8 @@ struct alpsg_ctx {
       nodes[s].release();
       crec[s].release();
       rsum[s].release();
+      ser[s].release();
+      serD[s].release();
       bmi[s].release();
       grel[s].release();
       pbrel[s].release();
@@ -1984,6 +1990,7 @@ int alpsg_set_k(alpsg_handle h, double kperp, double kpar, const int *nmax_in, i
     S.atab = h->atab[s].p;
     S.nodes = h->nodes[s].p;
     S.crec = S.rsum = nullptr;
+    S.ser = S.serD = nullptr;
     S.lsum = nullptr;
     if (!h->relsp[s] && S.acmethod == 1 && S.fit_uniform && S.nfits == 1) {
       HIPCHK(h->lsum[s].ensure((size_t)(nm + 1) * 6));
@@ -1994,11 +2001,17 @@ int alpsg_set_k(alpsg_handle h, double kperp, double kpar, const int *nmax_in, i
     if (!h->relsp[s]) {
       HIPCHK(h->crec[s].ensure((size_t)(nm + 1) * (npar - 1) * CREC_STRIDE));
       HIPCHK(h->rsum[s].ensure((size_t)(nm + 1) * RSUM_STRIDE));
+      HIPCHK(h->ser[s].ensure((size_t)(nm + 1) * 2 * SER_Q * SER_STRIDE));
+      HIPCHK(h->serD[s].ensure((size_t)(nm + 1) * 2));
       HIPCHK(launch_split_tables(h->st, npar, nm, kpar, h->qs[s], h->ms[s],
                                  h->ppar.p + (size_t)s * (npar + 1), h->nodes[s].p, h->crec[s].p,
-                                 h->rsum[s].p));
+                                 h->rsum[s].p, h->ser[s].p, h->serD[s].p));
       S.crec = h->crec[s].p;
       S.rsum = h->rsum[s].p;
+      if (h->series_on) {
+        S.ser = h->ser[s].p;
+        S.serD = h->serD[s].p;
+      }
     }
     if (h->relsp[s]) {
       // the harmonics run in k_rel_nonres / k_rel_res, not in the sweep
@@ -2116,6 +2129,48 @@ int alpsg_set_plain_roots(alpsg_handle h, int plain) {
   ALPSG_TRY
   if (!h) return fail(ALPSG_E_ARG, "null handle");
   h->plain_roots = plain != 0;
+  return ALPSG_OK;
+  ALPSG_CATCH
+}
+
+int alpsg_set_sweep_series(alpsg_handle h, int on) {
+  ALPSG_TRY
+  if (!h) return fail(ALPSG_E_ARG, "null handle");
+  h->series_on = on != 0;
+  if (h->have_k) {
+    // the tables of the current k exist whenever the split tables do
+    for (int s = 0; s < h->nspec; ++s) {
+      SpecDev &S = h->prob.sp[s];
+      const bool has = S.crec != nullptr && h->ser[s].p != nullptr;
+      S.ser = (h->series_on && has) ? h->ser[s].p : nullptr;
+      S.serD = (h->series_on && has) ? h->serD[s].p : nullptr;
+    }
+    HIPCHK(hipMemcpyAsync(h->prob_d.p, &h->prob, sizeof(ProbDev), hipMemcpyHostToDevice, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+  }
+  return ALPSG_OK;
+  ALPSG_CATCH
+}
+
+int alpsg_sweep_work(alpsg_handle h, int n, const double *om, double *work3) {
+  ALPSG_TRY
+  if (!h || !work3 || (n > 0 && !om)) return fail(ALPSG_E_ARG, "null argument");
+  if (!h->have_k) return fail(ALPSG_E_STATE, "alpsg_set_k has not been called");
+  if (n < 0) return fail(ALPSG_E_ARG, "negative batch size");
+  for (int k = 0; k < 3; ++k) work3[k] = 0.0;
+  if (n == 0) return ALPSG_OK;
+  const LaunchShape sh = shape_for(h, n);
+  HIPCHK(h->om_d.ensure(n));
+  HIPCHK(h->chiA_d.ensure((size_t)n * sh.w_a * sh.rc_a * h->nspec * 6));
+  HIPCHK(h->work_d.ensure(3));
+  HIPCHK(hipMemcpyAsync(h->om_d.p, om, sizeof(double2) * n, hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipMemsetAsync(h->work_d.p, 0, sizeof(unsigned long long) * 3, h->st));
+  HIPCHK(launch_sweep(h->st, sh.nch_a, sh.w_a, sh.rc_a, h->prob_d.p, n, h->om_d.p, nullptr,
+                      h->chiA_d.p, nullptr, nullptr, h->work_d.p));
+  unsigned long long w[3] = {0, 0, 0};
+  HIPCHK(hipMemcpyAsync(w, h->work_d.p, sizeof(w), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  for (int k = 0; k < 3; ++k) work3[k] = (double)w[k];
   return ALPSG_OK;
   ALPSG_CATCH
 }

@@ -13,6 +13,9 @@
 #define FIT_STRIDE 8    // doubles per (iperp, fit) entry of the analytic-continuation table
 #define CREC_STRIDE 13  // doubles per (n, ipar) record of the split-form sweep table
 #define RSUM_STRIDE 8   // doubles per harmonic of the split-form row sums
+#define SER_Q 64        // coefficients of a far row's series in m w (degrees 0..SER_Q-1)
+#define SER_STRIDE 8    // doubles per series coefficient (the six node products, 2 pad)
+#define SER_RHO 0.55    // largest |m w| / min|kpar p +- n q| a far row's series is used at
 #define GTAB_N 4200     // Gamma(m) table entries (m < GTAB_N), followed by Fact(0..20)
 #define RFG_NU (GTAB_N - 21)  // orders nu of the series coefficients 1/(k! Gamma(nu+k+1)) that follow
 
@@ -40,6 +43,14 @@ struct SpecDev {
   // omega-independent sums of the G/m terms over the interior nodes
   const double *crec;   // [(nmax+1)*(npar-1)*CREC_STRIDE]
   const double *rsum;   // [(nmax+1)*RSUM_STRIDE]
+  // Far rows (may be NULL).  For sign sg (0: +n, 1: -n) with d_i = kpar p_i
+  // +- n q and D = min |d_i| over ipar = 1..npar-1 (serD[2n+sg], 0 if none),
+  // the full-range node sum of a row whose every lane has |m w| <= SER_RHO D
+  // is the polynomial sum_q ser[n][sg][q][k] z^q in z = m w / D (the geometric
+  // series of 1/(m w - d_i) summed over the nodes), truncated where the
+  // remainder falls below double rounding
+  const double *ser;    // [(nmax+1)*2*SER_Q*SER_STRIDE]
+  const double *serD;   // [(nmax+1)*2]
   const double *fitc;   // [(nperp+1)*nfits*FIT_STRIDE]
   // Landau sums of a species whose continuation is one Maxwellian with
   // pperp-independent (u2, u3) (NULL otherwise), per k: lsum[n][6] =

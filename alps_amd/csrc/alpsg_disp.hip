@@ -401,6 +401,55 @@ __device__ __forceinline__ void sweep_split(cdptr crow, cdptr rsum, int npar, do
   }
 }
 
+// A far row's node sum for one sign from its series table c (SER_STRIDE
+// doubles per degree): acc_k = sum_{q<=J} c_q[k] z^q, J >= 1.  The
+// coefficients are real, so the sum runs the second-order recurrence
+// b_q = c_q + 2 Re(z) b_{q+1} - |z|^2 b_{q+2} (q = J..1) and ends with
+// c_0 + z b_1 - |z|^2 b_2: two real FMAs per coefficient and product.
+__device__ __forceinline__ void series_row(cdptr c, int J, double zr, double zi, cd acc[6]) {
+  const double r2 = 2.0 * zr, sq = __builtin_fma(zr, zr, zi * zi);
+  // b1 = b_{q+1}, b2 = b_{q+2}; two steps per pass, so the pair swaps roles
+  // instead of moving
+  double b1[6], b2[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    b1[k] = c[J * SER_STRIDE + k];
+    b2[k] = 0.0;
+  }
+  int q = J - 1;
+  for (; q >= 2; q -= 2) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+      b2[k] = __builtin_fma(r2, b1[k], __builtin_fma(-sq, b2[k], c[q * SER_STRIDE + k]));
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+      b1[k] = __builtin_fma(r2, b2[k], __builtin_fma(-sq, b1[k], c[(q - 1) * SER_STRIDE + k]));
+  }
+  if (q == 1) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const double t = __builtin_fma(r2, b1[k], __builtin_fma(-sq, b2[k], c[SER_STRIDE + k]));
+      b2[k] = b1[k];
+      b1[k] = t;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) acc[k] = mk(__builtin_fma(zr, b1[k], __builtin_fma(-sq, b2[k], c[k])), zi * b1[k]);
+}
+
+// The smallest degree J >= 1 at which the series' remainder bound
+// rho^(J+1) / (1 - rho) is below 2^-54 (rho <= SER_RHO < 1).
+__device__ __forceinline__ int series_degree(double rho) {
+  const double thr = 0x1p-54 * (1.0 - rho);
+  double t = rho * rho;
+  int J = 1;
+  while (t > thr && J < SER_Q - 1) {
+    t *= rho;
+    ++J;
+  }
+  return J;
+}
+
 // endpoint contribution (weight 1) of node e for a row with denominator shift nq
 __device__ __forceinline__ void endpoint_acc(const double *__restrict__ rows, cdptr pa,
                                              int e, int npar,
@@ -433,7 +482,8 @@ __global__ __launch_bounds__(64 * W) void k_sweep(const ProbDev *__restrict__ pr
                                                   const int *__restrict__ active,
                                                   double2 *__restrict__ chiA,
                                                   double2 *__restrict__ lowA,
-                                                  const int *__restrict__ pidx) {
+                                                  const int *__restrict__ pidx,
+                                                  unsigned long long *__restrict__ work) {
   constexpr int OMB = 64 / NCH;
   __shared__ double lchi[W][12][64];
   __shared__ double lpa[LPA];
@@ -451,10 +501,20 @@ __global__ __launch_bounds__(64 * W) void k_sweep(const ProbDev *__restrict__ pr
   const int nspec = Pd.nspec;
   const double kpar = Pd.kpar;
   int rowbase = 0;
+  // work counts (counting launches only): omega-weighted node units (species,
+  // harmonic, ppar node) of node sums, far-row signs and their series degrees
+  unsigned w_nodes = 0, w_ser = 0, w_deg = 0;
   for (int s = 0; s < nspec; ++s) {
     const SpecDev &S = Pd.sp[s];
     const int npar = S.npar;
     const double m = S.m;
+    // the largest |m w| of the wavefront's omegas (far rows compare it with D)
+    double mw2 = (m * m) * __builtin_fma(omr, omr, omi * omi);
+    if (NCH == 1 && S.ser != nullptr) {
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) mw2 = fmax(mw2, __shfl_xor(mw2, o, 64));
+    }
+    const double mwmax = sqrt(mw2);
     // the species' ppar grid in LDS for the resonance classification
     const double *pa = S.ppar;
     if (S.nmax >= 0 && npar + 1 <= LPA) {
@@ -469,84 +529,14 @@ __global__ __launch_bounds__(64 * W) void k_sweep(const ProbDev *__restrict__ pr
     double *lc = &lchi[threadIdx.x >> 6][0][lane];
 #pragma unroll
     for (int t = 0; t < 12; ++t) lc[64 * t] = 0.0;
-    for (int nn = 0; nn <= S.nmax; ++nn) {
-      if (((rowbase + nn) % TW) != wv) continue;
-      const double *rows = S.nodes + (size_t)nn * (npar - 1) * NODE_STRIDE;
-      const cdptr srows = (cdptr)rows;
-      const cdptr spa = (cdptr)S.ppar;  // the nodes' ppar values
-      RowClass rp, rm;
-      classify(S, Pd.P, kpar, omr, omi, nn, rp, pa);
-      if (nn > 0) classify(S, Pd.P, kpar, omr, omi, -nn, rm, pa);
-      const double nq = (1.0 * nn) * S.q;
-      // interior (weight-2) ranges (a, b) of the two integrate() calls per sign
-      const int pl1 = rp.a1 + 1, pc1 = max(rp.b1 - rp.a1 - 1, 0);
-      const int pl2 = rp.a2 + 1, pc2 = max(rp.b2 - rp.a2 - 1, 0);
-      const int ml1 = nn > 0 ? rm.a1 + 1 : 0, mc1 = nn > 0 ? max(rm.b1 - rm.a1 - 1, 0) : 0;
-      const int ml2 = nn > 0 ? rm.a2 + 1 : 0, mc2 = nn > 0 ? max(rm.b2 - rm.a2 - 1, 0) : 0;
-      cd ap[6], am[6];
-#pragma unroll
-      for (int k = 0; k < 6; ++k) ap[k] = am[k] = mk(0.0, 0.0);
-      const double dim = m * omi;
-      // rows whose trapezoid covers the whole grid in every lane take the
-      // uniform-weight loop (no per-node range tests)
-      const bool stdp = (rp.a2 < 0) && rp.a1 == 1 && rp.b1 == npar - 1;
-      const bool stdm = (nn == 0) || ((rm.a2 < 0) && rm.a1 == 1 && rm.b1 == npar - 1);
-      // the split form sums G/m and C/den, which cancel to U/den when |den| is
-      // large against m|w|: it is used where the cancellation factor
-      // (kpar max|p| + |n q|) / (m |w|) stays below SPLIT_MAX for every lane
-      const bool split_ok =
-          NCH == 1 && S.crec != nullptr &&
-          fabs(kpar) * fmax(fabs(pa[0]), fabs(pa[npar])) + fabs(nq) <=
-              SPLIT_MAX * m * sqrt(omr * omr + omi * omi);
-      const bool std_row = __all(stdp && stdm);
-      if (std_row) {
-        if (__all(split_ok)) {
-          const cdptr crow = (cdptr)(S.crec + (size_t)nn * (npar - 1) * CREC_STRIDE);
-          const cdptr rs = (cdptr)(S.rsum + (size_t)nn * RSUM_STRIDE);
-          if (nn > 0)
-            sweep_split<true>(crow, rs, npar, omr, omi, m, nq, dim, ap, am);
-          else
-            sweep_split<false>(crow, rs, npar, omr, omi, m, nq, dim, ap, am);
-        } else if (nn > 0) {
-          sweep_nodes<NCH, true, true>(srows, spa, npar, c, omr, omi, m, kpar, nq, dim, 0, 0, 0, 0, 0, 0,
-                                       0, 0, ap, am);
-        } else {
-          sweep_nodes<NCH, true, false>(srows, spa, npar, c, omr, omi, m, kpar, nq, dim, 0, 0, 0, 0, 0,
-                                        0, 0, 0, ap, am);
-        }
-      } else {
-        sweep_nodes<NCH, false, true>(srows, spa, npar, c, omr, omi, m, kpar, nq, dim, pl1, pc1, pl2, pc2,
-                                      ml1, mc1, ml2, mc2, ap, am);
-      }
-      // (with several lanes per omega a full-range row took its end nodes in the sweep)
-      if (c == 0 && !(NCH > 1 && std_row)) {
-        endpoint_acc(rows, spa, rp.a1, npar, m, kpar, omr, omi, nq, ap);
-        endpoint_acc(rows, spa, rp.b1, npar, m, kpar, omr, omi, nq, ap);
-        if (rp.a2 > 0) {
-          endpoint_acc(rows, spa, rp.a2, npar, m, kpar, omr, omi, nq, ap);
-          endpoint_acc(rows, spa, rp.b2, npar, m, kpar, omr, omi, nq, ap);
-        }
-        if (nn > 0) {
-          endpoint_acc(rows, spa, rm.a1, npar, m, kpar, omr, omi, -nq, am);
-          endpoint_acc(rows, spa, rm.b1, npar, m, kpar, omr, omi, -nq, am);
-          if (rm.a2 > 0) {
-            endpoint_acc(rows, spa, rm.a2, npar, m, kpar, omr, omi, -nq, am);
-            endpoint_acc(rows, spa, rm.b2, npar, m, kpar, omr, omi, -nq, am);
-          }
-        }
-      }
-      if (NCH > 1) {
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-          ap[k] = gsumc<NCH>(ap[k]);
-          if (nn > 0) am[k] = gsumc<NCH>(am[k]);
-        }
-      }
+    // a row pair's node sums (before the row's T constants) into the running
+    // chi in LDS, and the n = -1, 0, 1 rows into lowA
+    auto finish_row = [&](int nn, const cd *xp, const cd *xm) {
       cd vp[6], vm[6];
 #pragma unroll
       for (int k = 0; k < 6; ++k) {
-        vp[k] = (scaleN * ap[ACC_OF_MODE[k]]) * tconst(Pd, S, nn, k);
-        vm[k] = (scaleN * am[ACC_OF_MODE[k]]) * tconst(Pd, S, -nn, k);
+        vp[k] = (scaleN * xp[ACC_OF_MODE[k]]) * tconst(Pd, S, nn, k);
+        vm[k] = (scaleN * xm[ACC_OF_MODE[k]]) * tconst(Pd, S, -nn, k);
       }
       if (nn == 0) {
         lc[64 * 2] += vp[1].re;
@@ -579,6 +569,108 @@ __global__ __launch_bounds__(64 * W) void k_sweep(const ProbDev *__restrict__ pr
           }
         }
       }
+    };
+    for (int nn = 0; nn <= S.nmax; ++nn) {
+      if (((rowbase + nn) % TW) != wv) continue;
+      const double *rows = S.nodes + (size_t)nn * (npar - 1) * NODE_STRIDE;
+      const cdptr srows = (cdptr)rows;
+      const cdptr spa = (cdptr)S.ppar;  // the nodes' ppar values
+      RowClass rp, rm;
+      classify(S, Pd.P, kpar, omr, omi, nn, rp, pa);
+      if (nn > 0) classify(S, Pd.P, kpar, omr, omi, -nn, rm, pa);
+      const double nq = (1.0 * nn) * S.q;
+      // interior (weight-2) ranges (a, b) of the two integrate() calls per sign
+      const int pl1 = rp.a1 + 1, pc1 = max(rp.b1 - rp.a1 - 1, 0);
+      const int pl2 = rp.a2 + 1, pc2 = max(rp.b2 - rp.a2 - 1, 0);
+      const int ml1 = nn > 0 ? rm.a1 + 1 : 0, mc1 = nn > 0 ? max(rm.b1 - rm.a1 - 1, 0) : 0;
+      const int ml2 = nn > 0 ? rm.a2 + 1 : 0, mc2 = nn > 0 ? max(rm.b2 - rm.a2 - 1, 0) : 0;
+      cd ap[6], am[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) ap[k] = am[k] = mk(0.0, 0.0);
+      const double dim = m * omi;
+      // rows whose trapezoid covers the whole grid in every lane take the
+      // uniform-weight loop (no per-node range tests)
+      const bool stdp = (rp.a2 < 0) && rp.a1 == 1 && rp.b1 == npar - 1;
+      const bool stdm = (nn == 0) || ((rm.a2 < 0) && rm.a1 == 1 && rm.b1 == npar - 1);
+      // the split form sums G/m and C/den, which cancel to U/den when |den| is
+      // large against m|w|: it is used where the cancellation factor
+      // (kpar max|p| + |n q|) / (m |w|) stays below SPLIT_MAX for every lane
+      const bool split_ok =
+          NCH == 1 && S.crec != nullptr &&
+          fabs(kpar) * fmax(fabs(pa[0]), fabs(pa[npar])) + fabs(nq) <=
+              SPLIT_MAX * m * sqrt(omr * omr + omi * omi);
+      const bool std_row = __all(stdp && stdm);
+      // a far row (every lane's |m w| within SER_RHO of both signs' D) takes
+      // its series, in arrays of its own, and is done
+      if (NCH == 1 && S.ser != nullptr && std_row) {
+        const double Dp = S.serD[2 * nn], Dm = (nn > 0) ? S.serD[2 * nn + 1] : Dp;
+        if (Dp > 0.0 && Dm > 0.0 && mwmax <= SER_RHO * Dp && mwmax <= SER_RHO * Dm) {
+          const cdptr cp = (cdptr)(S.ser + (size_t)(2 * nn) * SER_Q * SER_STRIDE);
+          cd sp[6], sm[6];
+          // (the degrees are wave-uniform: the loop and its table loads stay scalar)
+          const int Jp = __builtin_amdgcn_readfirstlane(series_degree(mwmax / Dp));
+          series_row(cp, Jp, (m * omr) / Dp, (m * omi) / Dp, sp);
+          w_ser += 1;
+          w_deg += Jp;
+          if (nn > 0) {
+            const int Jm = __builtin_amdgcn_readfirstlane(series_degree(mwmax / Dm));
+            series_row(cp + SER_Q * SER_STRIDE, Jm, (m * omr) / Dm, (m * omi) / Dm, sm);
+            w_ser += 1;
+            w_deg += Jm;
+          } else {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) sm[k] = mk(0.0, 0.0);
+          }
+          finish_row(nn, sp, sm);
+          continue;
+        }
+      }
+      if (std_row) {
+        w_nodes += (nn > 0 ? 2 : 1) * (npar - 1);
+        if (__all(split_ok)) {
+          const cdptr crow = (cdptr)(S.crec + (size_t)nn * (npar - 1) * CREC_STRIDE);
+          const cdptr rs = (cdptr)(S.rsum + (size_t)nn * RSUM_STRIDE);
+          if (nn > 0)
+            sweep_split<true>(crow, rs, npar, omr, omi, m, nq, dim, ap, am);
+          else
+            sweep_split<false>(crow, rs, npar, omr, omi, m, nq, dim, ap, am);
+        } else if (nn > 0) {
+          sweep_nodes<NCH, true, true>(srows, spa, npar, c, omr, omi, m, kpar, nq, dim, 0, 0, 0, 0, 0, 0,
+                                       0, 0, ap, am);
+        } else {
+          sweep_nodes<NCH, true, false>(srows, spa, npar, c, omr, omi, m, kpar, nq, dim, 0, 0, 0, 0, 0,
+                                        0, 0, 0, ap, am);
+        }
+      } else {
+        w_nodes += (nn > 0 ? 2 : 1) * (npar - 1);
+        sweep_nodes<NCH, false, true>(srows, spa, npar, c, omr, omi, m, kpar, nq, dim, pl1, pc1, pl2, pc2,
+                                      ml1, mc1, ml2, mc2, ap, am);
+      }
+      // (with several lanes per omega a full-range row took its end nodes in the sweep)
+      if (c == 0 && !(NCH > 1 && std_row)) {
+        endpoint_acc(rows, spa, rp.a1, npar, m, kpar, omr, omi, nq, ap);
+        endpoint_acc(rows, spa, rp.b1, npar, m, kpar, omr, omi, nq, ap);
+        if (rp.a2 > 0) {
+          endpoint_acc(rows, spa, rp.a2, npar, m, kpar, omr, omi, nq, ap);
+          endpoint_acc(rows, spa, rp.b2, npar, m, kpar, omr, omi, nq, ap);
+        }
+        if (nn > 0) {
+          endpoint_acc(rows, spa, rm.a1, npar, m, kpar, omr, omi, -nq, am);
+          endpoint_acc(rows, spa, rm.b1, npar, m, kpar, omr, omi, -nq, am);
+          if (rm.a2 > 0) {
+            endpoint_acc(rows, spa, rm.a2, npar, m, kpar, omr, omi, -nq, am);
+            endpoint_acc(rows, spa, rm.b2, npar, m, kpar, omr, omi, -nq, am);
+          }
+        }
+      }
+      if (NCH > 1) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+          ap[k] = gsumc<NCH>(ap[k]);
+          if (nn > 0) am[k] = gsumc<NCH>(am[k]);
+        }
+      }
+      finish_row(nn, ap, am);
     }
     rowbase += S.nmax + 1;
     if (valid && c == 0) {
@@ -586,6 +678,14 @@ __global__ __launch_bounds__(64 * W) void k_sweep(const ProbDev *__restrict__ pr
       double2 *o = chiA + ((size_t)(wv * nspec + s) * 6) * nom + iom;
 #pragma unroll
       for (int k = 0; k < 6; ++k) o[(size_t)k * nom] = make_double2(lc[64 * (2 * k)], lc[64 * (2 * k + 1)]);
+    }
+  }
+  if (work != nullptr) {
+    const unsigned long long nvalid = (unsigned long long)__popcll(__ballot(valid && c == 0));
+    if (lane == 0) {
+      atomicAdd(work + 0, w_nodes * nvalid);
+      atomicAdd(work + 1, w_ser * nvalid);
+      atomicAdd(work + 2, w_deg * nvalid);
     }
   }
 }
@@ -1378,14 +1478,14 @@ __global__ __launch_bounds__(256) void k_assemble(
 // ---------------------------------------------------------------------------
 hipError_t launch_sweep(hipStream_t st, int nch, int W, int RC, const ProbDev *prob, int nom,
                         const double2 *om, const int *active, double2 *chiA, double2 *lowA,
-                        const int *pidx) {
+                        const int *pidx, unsigned long long *work) {
   if (nom <= 0) return hipSuccess;
   if (pidx != nullptr && nch != 64) return hipErrorInvalidValue;
 #define SW(N, WV)                                                                                \
   if (nch == N && W == WV) {                                                                     \
     const int grid = (nom + (64 / N) - 1) / (64 / N);                                            \
     hipLaunchKernelGGL((k_sweep<N, WV>), dim3(grid, RC), dim3(64 * WV), 0, st, prob, nom, om, active, \
-                       chiA, lowA, pidx);                                                        \
+                       chiA, lowA, pidx, work);                                                  \
     return hipGetLastError();                                                                    \
   }
   SW(1, 1) SW(1, 2) SW(1, 4) SW(2, 4) SW(4, 4) SW(8, 4) SW(16, 4) SW(16, 8) SW(32, 8) SW(64, 4)

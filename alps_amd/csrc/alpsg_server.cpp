@@ -107,7 +107,7 @@ constexpr int MTAB_S = 4096;
 // SW_W wavefronts per workgroup, SW_RC workgroups per omega and SW_K node records
 // in flight per lane; k_resonant<64, *> over RS_RB harmonic blocks
 #ifndef ALPSG_SERVE_SWEEP
-#define ALPSG_SERVE_SWEEP 32, 4, 4, 4
+#define ALPSG_SERVE_SWEEP 32, 4, 4, 2
 #endif
 constexpr int SW_SHAPE[4] = {ALPSG_SERVE_SWEEP};
 constexpr int SW_NCH = SW_SHAPE[0], SW_W = SW_SHAPE[1], SW_RC = SW_SHAPE[2], SW_K = SW_SHAPE[3];

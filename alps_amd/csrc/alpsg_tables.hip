@@ -378,33 +378,110 @@ __global__ void k_crec(int npar, int nmax, double kpar, double q, double m,
 }
 
 // rsum[n] = sum over the interior nodes ipar = 2..npar-2 of
-// {G1, p G1, p^2 G1, G2, G3, p G3} / m, in ipar order (one thread per (n, term))
-__global__ void k_rsum(int npar, int nmax, double m, const double *__restrict__ ppar,
-                       const double *__restrict__ nodes, double *__restrict__ rsum) {
-  const int n = blockIdx.x;
-  const int k = threadIdx.x;
-  if (n > nmax || k >= 6) {
-    if (n <= nmax && k < RSUM_STRIDE) rsum[(size_t)n * RSUM_STRIDE + k] = 0.0;
-    return;
-  }
-  double s = 0.0;
-  for (int ipar = 2; ipar <= npar - 2; ++ipar) {
+// {G1, p G1, p^2 G1, G2, G3, p G3} / m: one wavefront per harmonic, lanes
+// strided over the nodes, then a fixed-order butterfly over the lanes
+__global__ __launch_bounds__(64) void k_rsum(int npar, int nmax, double m,
+                                             const double *__restrict__ ppar,
+                                             const double *__restrict__ nodes,
+                                             double *__restrict__ rsum) {
+  const int n = blockIdx.x, lane = threadIdx.x;
+  double a[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int ipar = 2 + lane; ipar <= npar - 2; ipar += 64) {
     const double *r = nodes + ((size_t)n * (npar - 1) + (ipar - 1)) * NODE_STRIDE;
-    const double g = (k <= 2) ? r[0] : (k == 3 ? r[2] : r[4]);
     const double p = ppar[ipar];
-    const double pw = (k == 1 || k == 5) ? p : (k == 2 ? p * p : 1.0);
-    s += (pw * g) / m;
+    a[0] += r[0] / m;
+    a[1] += (p * r[0]) / m;
+    a[2] += ((p * p) * r[0]) / m;
+    a[3] += r[2] / m;
+    a[4] += r[4] / m;
+    a[5] += (p * r[4]) / m;
   }
-  rsum[(size_t)n * RSUM_STRIDE + k] = s;
+#pragma unroll
+  for (int k = 0; k < 6; ++k)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) a[k] += __shfl_xor(a[k], o, 64);
+  if (lane < RSUM_STRIDE) rsum[(size_t)n * RSUM_STRIDE + lane] = (lane < 6) ? a[lane] : 0.0;
+}
+
+// Far-row series tables.  For harmonic n, sign sg (d_i = kpar p_i + n q for
+// +n, kpar p_i - n q for -n) and D = min |d_i| over ipar = 1..npar-1, the
+// full-range node sum of node product k (the same products and trapezoid
+// weights as the sweep: 1/2 at ipar = 1 and npar-1, 1 between) is
+//   sum_i w_i pk_i (w G_i + H_i) / (m w - d_i) = sum_q c_q z^q,  z = m w / D,
+// from 1/(m w - d) = -(1/d) sum_j (m w / d)^j and w = z D / m:
+//   c_q = -sum_i w_i pk_i rho_i^q ((G_i / m) [q >= 1] + (H_i / D) rho_i),
+//   rho_i = D / d_i, |rho_i| <= 1.
+// One workgroup per (n, sg): lane q of each of the 4 wavefronts sums c_q over
+// a quarter of the nodes; the quarters are added in wavefront order.
+static_assert(SER_Q == 64, "one lane per series coefficient");
+__global__ __launch_bounds__(256) void k_series(int npar, int nmax, double kpar, double q, double m,
+                                                const double *__restrict__ ppar,
+                                                const double *__restrict__ nodes,
+                                                double *__restrict__ ser, double *__restrict__ serD) {
+  __shared__ double part[4][6][64];
+  __shared__ double dpart[4];
+  const int n = blockIdx.x, sg = blockIdx.y;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const double nq = (sg == 0 ? 1.0 : -1.0) * ((1.0 * n) * q);
+  double dmin = INFINITY;
+  for (int i = 1 + threadIdx.x; i <= npar - 1; i += 256) dmin = fmin(dmin, fabs(kpar * ppar[i] + nq));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) dmin = fmin(dmin, __shfl_xor(dmin, o, 64));
+  if (lane == 0) dpart[wv] = dmin;
+  __syncthreads();
+  dmin = fmin(fmin(dpart[0], dpart[1]), fmin(dpart[2], dpart[3]));
+  // the -0 row does not exist; a node on d = 0 leaves the row to the sweep
+  const bool ok = dmin > 0.0 && dmin < INFINITY && !(n == 0 && sg == 1);
+  double a[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (ok) {
+    const double gq = lane >= 1 ? 1.0 / m : 0.0, hD = 1.0 / dmin;
+    for (int i = 1 + wv; i <= npar - 1; i += 4) {
+      const double *r = nodes + ((size_t)n * (npar - 1) + (i - 1)) * NODE_STRIDE;
+      const double p = ppar[i];
+      const double rho = dmin / (kpar * p + nq);
+      double pw = 1.0, b = rho;
+      int e = lane;
+#pragma unroll
+      for (int t = 0; t < 6; ++t) {
+        pw = (e & 1) ? pw * b : pw;
+        b = b * b;
+        e >>= 1;
+      }
+      const double f = ((i == 1 || i == npar - 1) ? -0.5 : -1.0) * pw;
+      const double hr = hD * rho;
+      const double e1 = f * (r[0] * gq + r[1] * hr);
+      const double e2 = f * (r[2] * gq + r[3] * hr);
+      const double e3 = f * (r[4] * gq + r[5] * hr);
+      a[0] += e1;
+      a[1] += p * e1;
+      a[2] += (p * p) * e1;
+      a[3] += e2;
+      a[4] += e3;
+      a[5] += p * e3;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) part[wv][k][lane] = a[k];
+  __syncthreads();
+  if (wv == 0) {
+    double *o = ser + ((size_t)(2 * n + sg) * SER_Q + lane) * SER_STRIDE;
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+      o[k] = ((part[0][k][lane] + part[1][k][lane]) + part[2][k][lane]) + part[3][k][lane];
+    o[6] = o[7] = 0.0;
+    if (lane == 0) serD[2 * n + sg] = ok ? dmin : 0.0;
+  }
 }
 
 hipError_t launch_split_tables(hipStream_t st, int npar, int nmax, double kpar, double q, double m,
                                const double *ppar, const double *nodes, double *crec,
-                               double *rsum) {
+                               double *rsum, double *ser, double *serD) {
   hipLaunchKernelGGL(k_crec, dim3((npar - 1 + 63) / 64, nmax + 1), dim3(64), 0, st, npar, nmax,
                      kpar, q, m, ppar, nodes, crec);
-  hipLaunchKernelGGL(k_rsum, dim3(nmax + 1), dim3(RSUM_STRIDE), 0, st, npar, nmax, m, ppar, nodes,
-                     rsum);
+  hipLaunchKernelGGL(k_rsum, dim3(nmax + 1), dim3(64), 0, st, npar, nmax, m, ppar, nodes, rsum);
+  if (ser != nullptr)
+    hipLaunchKernelGGL(k_series, dim3(nmax + 1, 2), dim3(256), 0, st, npar, nmax, kpar, q, m, ppar,
+                       nodes, ser, serD);
   return hipGetLastError();
 }
 

@@ -46,7 +46,8 @@ EXPORTS = ["alpsg_last_error", "alpsg_create", "alpsg_destroy", "alpsg_set_k",
            "alpsg_server_submit_batch", "alpsg_om_double_scan",
            "alpsg_server_submit", "alpsg_server_poll", "alpsg_server_stats",
            "alpsg_server_destroy", "alpsg_determine_param_fit", "alpsg_last_scan_steps",
-           "alpsg_set_plain_roots", "alpsg_disp_batch_noalloc", "alpsg_server_work"]
+           "alpsg_set_plain_roots", "alpsg_disp_batch_noalloc", "alpsg_server_work",
+           "alpsg_set_sweep_series", "alpsg_sweep_work"]
 
 
 class AlpsgError(RuntimeError):
@@ -131,6 +132,10 @@ def load_library(path: str = LIB_PATH):
     L.alpsg_last_scan_steps.restype = C.c_int
     L.alpsg_set_plain_roots.argtypes = [vp, C.c_int]
     L.alpsg_set_plain_roots.restype = C.c_int
+    L.alpsg_set_sweep_series.argtypes = [vp, C.c_int]
+    L.alpsg_set_sweep_series.restype = C.c_int
+    L.alpsg_sweep_work.argtypes = [vp, C.c_int, dp, dp]
+    L.alpsg_sweep_work.restype = C.c_int
     L.alpsg_autotune.argtypes = [vp, C.c_int, dp, ip]
     L.alpsg_set_shape.argtypes = [vp, C.c_int, ip]
     L.alpsg_get_shape.argtypes = [vp, C.c_int, ip]
@@ -429,6 +434,22 @@ class Engine:
         """Root searches step on the host with one disp launch per omega (the
         plain form) instead of device-stepped graph rounds."""
         _check(load_library().alpsg_set_plain_roots(self._h, int(bool(plain))))
+
+    def set_sweep_series(self, on: bool):
+        """Far rows of the one-lane node sweep take their per-k series in m w
+        (on, the default) or pass over their nodes (off); both give the
+        trapezoid node sums to double rounding."""
+        _check(load_library().alpsg_set_sweep_series(self._h, int(bool(on))))
+
+    def sweep_work(self, om):
+        """Work of the node sweep over these omegas in the batch's launch shape,
+        counted on the device: (node units summed node by node, far-row signs
+        summed by series, sum of those series' degrees), each summed over the
+        omegas."""
+        om = np.ascontiguousarray(np.asarray(om, dtype=np.complex128).ravel())
+        out = np.zeros(3)
+        _check(load_library().alpsg_sweep_work(self._h, om.size, _dp(om), _dp(out)))
+        return tuple(float(x) for x in out)
 
     def refine_guess(self, guesses, roots: RootParams | None = None):
         """refine_guess (ALPS_fns.f90:3793-3856): secant on each guess, then det D there."""

@@ -265,6 +265,22 @@ int alpsg_last_scan_steps(alpsg_handle h, int nsteps, int nroots, double *ms_out
    device between the launches of a replayed graph.  The roots are the same. */
 int alpsg_set_plain_roots(alpsg_handle h, int plain);
 
+/* on != 0 (the default): rows of the one-lane-per-omega node sweep whose
+   every node is far from resonance for every omega of a wavefront
+   (|m w| <= 0.55 min |kpar p_i -+ n q|) take their node sum from a per-k
+   series table in m w, truncated below double rounding, instead of a pass
+   over the nodes; on == 0: every row passes over its nodes.  Both give the
+   reference's integrate() trapezoid sum (ALPS_fns.f90:799-864) to double
+   rounding; the switch exists to compare them. */
+int alpsg_set_sweep_series(alpsg_handle h, int on);
+
+/* The work of the node sweep (k_sweep, in the launch shape of an n-omega
+   batch) over these n omegas, counted on the device: work3[0] the (species,
+   harmonic, ppar node) units summed node by node, work3[1] the far-row signs
+   summed by their series, work3[2] the sum of those series' degrees, each
+   summed over the omegas.  The sweep's partial results are discarded. */
+int alpsg_sweep_work(alpsg_handle h, int n, const double *om, double *work3);
+
 /* Device time (ms) of the last alpsg_disp_batch / alpsg_map_search kernel
    sweep, measured with HIP events on the handle's stream. */
 double alpsg_last_kernel_ms(alpsg_handle h);
