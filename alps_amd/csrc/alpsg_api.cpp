@@ -56,7 +56,9 @@ hipError_t launch_lsum(hipStream_t st, int nperp, int nmax, double dpp, const do
 hipError_t launch_fill_i32(hipStream_t st, int *p, int n, int v);
 hipError_t launch_split_tables(hipStream_t st, int npar, int nmax, double kpar, double q, double m,
                                const double *ppar, const double *nodes, double *crec,
-                               double *rsum, double *ser, double *serD);
+                               double *rsum);
+hipError_t launch_series_tables(hipStream_t st, int npar, int nmax, double kpar, double q, double m,
+                                const double *ppar, const double *nodes, double *ser, double *serD);
 hipError_t launch_rel_bessel(hipStream_t st, int npts, int nmax, double zfac, const double *grel,
                              const double *pbrel, const int *mtab, double *out);
 hipError_t launch_map_post(hipStream_t st, int nr, int ni, const double2 *cal, double2 *calo,
@@ -186,7 +188,9 @@ struct alpsg_ctx {
   DevBuf<double> bes[ALPSG_MAXSPEC], atab[ALPSG_MAXSPEC], nodes[ALPSG_MAXSPEC];
   DevBuf<double> crec[ALPSG_MAXSPEC], rsum[ALPSG_MAXSPEC];  // split-form sweep tables
   DevBuf<double> ser[ALPSG_MAXSPEC], serD[ALPSG_MAXSPEC];  // far-row series tables
-  bool series_on = true;  // far rows of the one-lane sweep take their series
+  bool series_on = true;    // far rows of the node sweep take their series
+  bool ser_pending = false;  // the current k's series tables are not built yet
+  bool ser_built = false;    // they are (the buffers hold this k's tables)
   DevBuf<double> lsum[ALPSG_MAXSPEC];  // Landau sums of single-Maxwellian species
   DevBuf<int> zidx;  // zeros: the problem index of every omega of a small batch
   DevBuf<unsigned long long> work_d;  // the sweep's work counts (alpsg_sweep_work)
@@ -432,8 +436,32 @@ int disp_chain(alpsg_ctx *h, int n, const double2 *om_dev, const int *active_dev
 
 // Runs the disp kernels on n omegas already resident in om_dev, in the
 // handle's buffers (grown on demand).
+// Batches of at least SER_BUILD_MIN omegas build the current k's far-row
+// series tables (k_series, a few microseconds) if they are not built yet; the
+// descriptor goes to the device again with their pointers.  Smaller batches
+// (secant steps, scans: a few evaluations per k) use them once they exist.
+constexpr int SER_BUILD_MIN = 64;
+
+int ensure_series(alpsg_ctx *h, int n) {
+  if (!h->ser_pending || !h->series_on || n < SER_BUILD_MIN) return ALPSG_OK;
+  const double kpar = h->prob.kpar;
+  for (int s = 0; s < h->nspec; ++s) {
+    SpecDev &S = h->prob.sp[s];
+    if (S.crec == nullptr || h->ser[s].p == nullptr) continue;
+    HIPCHK(launch_series_tables(h->st, S.npar, S.nmax, kpar, h->qs[s], h->ms[s], S.ppar, S.nodes,
+                                h->ser[s].p, h->serD[s].p));
+    S.ser = h->ser[s].p;
+    S.serD = h->serD[s].p;
+  }
+  HIPCHK(hipMemcpyAsync(h->prob_d.p, &h->prob, sizeof(ProbDev), hipMemcpyHostToDevice, h->st));
+  h->ser_pending = false;
+  h->ser_built = true;
+  return ALPSG_OK;
+}
+
 int disp_device(alpsg_ctx *h, int n, const double2 *om_dev, const int *active_dev, bool want_chi,
                 bool want_low, bool want_wave, LaunchShape sh) {
+  if (const int rc = ensure_series(h, n)) return rc;
   const int ns = h->nspec;
   const int TW = sh.w_a * sh.rc_a;
   HIPCHK(h->det_d.ensure(n));
@@ -521,6 +549,7 @@ namespace {
 int autotune(alpsg_ctx *h, int n, const double2 *om_host, int *shape_out) {
   if (!h->have_k) return fail(ALPSG_E_STATE, "alpsg_set_k has not been called");
   if (n < 1) return fail(ALPSG_E_ARG, "n must be >= 1");
+  if (const int rc = ensure_series(h, n)) return rc;  // the variants are timed as they will run
   std::vector<double2> om(n);
   if (om_host) {
     for (int i = 0; i < n; ++i) om[i] = om_host[i];
@@ -2005,13 +2034,9 @@ int alpsg_set_k(alpsg_handle h, double kperp, double kpar, const int *nmax_in, i
       HIPCHK(h->serD[s].ensure((size_t)(nm + 1) * 2));
       HIPCHK(launch_split_tables(h->st, npar, nm, kpar, h->qs[s], h->ms[s],
                                  h->ppar.p + (size_t)s * (npar + 1), h->nodes[s].p, h->crec[s].p,
-                                 h->rsum[s].p, h->ser[s].p, h->serD[s].p));
+                                 h->rsum[s].p));
       S.crec = h->crec[s].p;
       S.rsum = h->rsum[s].p;
-      if (h->series_on) {
-        S.ser = h->ser[s].p;
-        S.serD = h->serD[s].p;
-      }
     }
     if (h->relsp[s]) {
       // the harmonics run in k_rel_nonres / k_rel_res, not in the sweep
@@ -2033,6 +2058,10 @@ int alpsg_set_k(alpsg_handle h, double kperp, double kpar, const int *nmax_in, i
   h->prob.kperp = kperp;
   h->prob.kpar = kpar;
   h->prob.nmaxR = nmaxR;
+  // the far-row series tables of this k are built by the first batch that is
+  // large enough to gain from them (ensure_series)
+  h->ser_pending = true;
+  h->ser_built = false;
   HIPCHK(hipMemcpyAsync(h->prob_d.p, &h->prob, sizeof(ProbDev), hipMemcpyHostToDevice, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
   h->kperp = kperp;
@@ -2137,8 +2166,8 @@ int alpsg_set_sweep_series(alpsg_handle h, int on) {
   ALPSG_TRY
   if (!h) return fail(ALPSG_E_ARG, "null handle");
   h->series_on = on != 0;
-  if (h->have_k) {
-    // the tables of the current k exist whenever the split tables do
+  if (h->have_k && h->ser_built) {
+    // this k's tables exist: point the descriptor at them or away
     for (int s = 0; s < h->nspec; ++s) {
       SpecDev &S = h->prob.sp[s];
       const bool has = S.crec != nullptr && h->ser[s].p != nullptr;
@@ -2160,6 +2189,7 @@ int alpsg_sweep_work(alpsg_handle h, int n, const double *om, double *work3) {
   for (int k = 0; k < 3; ++k) work3[k] = 0.0;
   if (n == 0) return ALPSG_OK;
   const LaunchShape sh = shape_for(h, n);
+  if (const int rc = ensure_series(h, n)) return rc;
   HIPCHK(h->om_d.ensure(n));
   HIPCHK(h->chiA_d.ensure((size_t)n * sh.w_a * sh.rc_a * h->nspec * 6));
   HIPCHK(h->work_d.ensure(3));

@@ -450,6 +450,49 @@ __device__ __forceinline__ int series_degree(double rho) {
   return J;
 }
 
+// A far row with several lanes per omega: lane c < 12 of the omega's lanes
+// (c < 6 for n = 0) sums the series of one (sign c / 6, node product c % 6);
+// lane c = 0 then gathers the twelve sums into ap / am by lane shuffles, and
+// the other lanes keep 0, so summing the lanes gives the row.  Returns whether
+// the row is far for this omega (full-range rows only; all lanes of the omega
+// decide alike, so a shuffle reads only lanes on the same path), and the sum
+// of its signs' degrees in deg.  The omega's lanes are consecutive, c first.
+__device__ __forceinline__ bool far_row_lanes(const SpecDev &S, int nn, int c, double omr,
+                                              double omi, cd ap[6], cd am[6], int &deg) {
+  if (S.ser == nullptr) return false;
+  const double Dp = S.serD[2 * nn], Dm = (nn > 0) ? S.serD[2 * nn + 1] : Dp;
+  const double m = S.m;
+  const double mw = sqrt((m * m) * __builtin_fma(omr, omr, omi * omi));
+  if (!(Dp > 0.0 && Dm > 0.0 && mw <= SER_RHO * Dp && mw <= SER_RHO * Dm)) return false;
+  const int Jp = series_degree(mw / Dp), Jm = (nn > 0) ? series_degree(mw / Dm) : 0;
+  deg = Jp + Jm;
+  cd v = mk(0.0, 0.0);
+  if (c < (nn > 0 ? 12 : 6)) {
+    const int sg = c >= 6 ? 1 : 0, k = c - 6 * sg;
+    const double D = sg ? Dm : Dp;
+    const double *cq = S.ser + (size_t)(2 * nn + sg) * SER_Q * SER_STRIDE + k;
+    const int J = sg ? Jm : Jp;
+    const double zr = (m * omr) / D, zi = (m * omi) / D;
+    const double r2 = 2.0 * zr, sq = __builtin_fma(zr, zr, zi * zi);
+    double b1 = cq[J * SER_STRIDE], b2 = 0.0;
+    for (int q = J - 1; q >= 1; --q) {
+      const double t = __builtin_fma(-sq, b2, cq[q * SER_STRIDE]);
+      b2 = b1;
+      b1 = __builtin_fma(r2, b1, t);
+    }
+    v = mk(__builtin_fma(zr, b1, __builtin_fma(-sq, b2, cq[0])), zi * b1);
+  }
+  const int base = (int)(threadIdx.x & 63) - c;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const cd a = mk(__shfl(v.re, base + k, 64), __shfl(v.im, base + k, 64));
+    const cd b = mk(__shfl(v.re, base + 6 + k, 64), __shfl(v.im, base + 6 + k, 64));
+    ap[k] = (c == 0) ? a : mk(0.0, 0.0);
+    am[k] = (c == 0 && nn > 0) ? b : mk(0.0, 0.0);
+  }
+  return true;
+}
+
 // endpoint contribution (weight 1) of node e for a row with denominator shift nq
 __device__ __forceinline__ void endpoint_acc(const double *__restrict__ rows, cdptr pa,
                                              int e, int npar,
@@ -625,7 +668,19 @@ __global__ __launch_bounds__(64 * W) void k_sweep(const ProbDev *__restrict__ pr
           continue;
         }
       }
-      if (std_row) {
+      // with 16 or more lanes per omega, a far row takes its series per omega
+      // group, one (sign, product) per lane
+      bool farg = false;
+      if constexpr (NCH >= 16) {
+        int deg = 0;
+        farg = stdp && stdm && far_row_lanes(S, nn, c, omr, omi, ap, am, deg);
+        if (farg) {
+          w_ser += (nn > 0 ? 2 : 1);
+          w_deg += deg;
+        }
+      }
+      if (farg) {
+      } else if (std_row) {
         w_nodes += (nn > 0 ? 2 : 1) * (npar - 1);
         if (__all(split_ok)) {
           const cdptr crow = (cdptr)(S.crec + (size_t)nn * (npar - 1) * CREC_STRIDE);
@@ -646,8 +701,9 @@ __global__ __launch_bounds__(64 * W) void k_sweep(const ProbDev *__restrict__ pr
         sweep_nodes<NCH, false, true>(srows, spa, npar, c, omr, omi, m, kpar, nq, dim, pl1, pc1, pl2, pc2,
                                       ml1, mc1, ml2, mc2, ap, am);
       }
-      // (with several lanes per omega a full-range row took its end nodes in the sweep)
-      if (c == 0 && !(NCH > 1 && std_row)) {
+      // (with several lanes per omega a full-range row took its end nodes in
+      // the sweep; a far row's series holds them)
+      if (c == 0 && !(NCH > 1 && std_row) && !farg) {
         endpoint_acc(rows, spa, rp.a1, npar, m, kpar, omr, omi, nq, ap);
         endpoint_acc(rows, spa, rp.b1, npar, m, kpar, omr, omi, nq, ap);
         if (rp.a2 > 0) {
@@ -681,11 +737,19 @@ __global__ __launch_bounds__(64 * W) void k_sweep(const ProbDev *__restrict__ pr
     }
   }
   if (work != nullptr) {
-    const unsigned long long nvalid = (unsigned long long)__popcll(__ballot(valid && c == 0));
+    // each omega's counts (lane c = 0 of its group), summed over the wavefront
+    const bool own = valid && c == 0;
+    unsigned long long a = own ? w_nodes : 0u, b = own ? w_ser : 0u, d = own ? w_deg : 0u;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      a += __shfl_xor(a, o, 64);
+      b += __shfl_xor(b, o, 64);
+      d += __shfl_xor(d, o, 64);
+    }
     if (lane == 0) {
-      atomicAdd(work + 0, w_nodes * nvalid);
-      atomicAdd(work + 1, w_ser * nvalid);
-      atomicAdd(work + 2, w_deg * nvalid);
+      atomicAdd(work + 0, a);
+      atomicAdd(work + 1, b);
+      atomicAdd(work + 2, d);
     }
   }
 }
@@ -714,6 +778,7 @@ __device__ __forceinline__ void sweep_req_body(const ProbDev *__restrict__ probp
                                                double2 *__restrict__ chiA,
                                                const int *__restrict__ pidx, int bx, int by, int gy) {
   constexpr int RPW = 64 / NCH;
+  static_assert(NCH >= 12, "a far row takes 12 lanes of its omega");
   const int iom = bx;
   if (iom >= nom) return;
   const ProbDev &Pd = probp[pidx[iom]];
@@ -765,7 +830,10 @@ __device__ __forceinline__ void sweep_req_body(const ProbDev *__restrict__ probp
       const double dim = m * omi;
       const bool stdp = (rp.a2 < 0) && rp.a1 == 1 && rp.b1 == npar - 1;
       const bool stdm = (nn == 0) || ((rm.a2 < 0) && rm.a1 == 1 && rm.b1 == npar - 1);
-      if (stdp && stdm) {
+      int deg = 0;
+      if (stdp && stdm && far_row_lanes(S, nn, c, omr, omi, ap, am, deg)) {
+        // a far row: its series, one (sign, product) per lane
+      } else if (stdp && stdm) {
         if (nn > 0)
           sweep_nodes_pf<NCH, true, true, K>(srows, spa, npar, c, omr, omi, m, kpar, nq, dim, 0, 0, 0, 0, 0, 0,
                                        0, 0, ap, am);
@@ -969,6 +1037,98 @@ __device__ __forceinline__ void coef_add(const SpecDev &S, CellCache &cc, Cursor
   cc.cm = cc.cm - f * wt;
 }
 
+// A monotone ladder's interpolation cells without per-point grid loads: the
+// last grid index within dppar/2 of x (ipar_close's answer before clamping)
+// can only move to the neighbour in the ladder's direction when a step is
+// shorter than 0.9 dppar (fast; the grid is uniform), so the neighbour's grid
+// value is kept in a register and each point makes the same |pa[k] - x| <=
+// dppar/2 tests as the full search.  A point that matches neither index (a
+// rounding gap between two cells), and every point of a ladder with longer
+// steps, takes the full search.  The current cell's coefficients are kept as c0 = sum wt,
+// cp = sum f wt (f = funct_g's interpolation fraction; the cell below takes
+// -cp) and go to the LDS window when the cell changes.
+struct Walk {
+  int k;           // the last index within dppar/2 of the current x, -1: unknown
+  double pk, pn;   // pa[k] and pa[k + dir]
+  int r;           // the interpolation cell: k clamped to [2, npar-2], -1: none yet
+  double pc;       // pa[r]
+  cd c0, cp;
+};
+
+__device__ __forceinline__ void walk_flush(Walk &w, double *win, int lo) {
+  if (w.r >= 0) {
+    lds_cadd(win, w.r - 1 - lo, mk(-w.cp.re, -w.cp.im));
+    lds_cadd(win, w.r - lo, w.c0);
+    lds_cadd(win, w.r + 1 - lo, w.cp);
+  }
+  w.r = -1;
+  w.c0 = w.cp = mk(0.0, 0.0);
+}
+
+__device__ __forceinline__ void walk_reset(Walk &w) {
+  w.k = -1;
+  w.r = -1;
+  w.pk = w.pn = w.pc = 0.0;
+  w.c0 = w.cp = mk(0.0, 0.0);
+}
+
+template <int DIR>
+__device__ __forceinline__ void walk_add(const SpecDev &S, Walk &w, double *win, int lo, double x,
+                                         cd wt, double half, double hinv, bool fast) {
+#pragma clang fp contract(off)
+  const int npar = S.npar;
+  const double *pa = S.ppar;
+  bool ok = false;
+  if (fast && w.k >= 0) {
+    if (DIR > 0) {
+      if (w.k + 1 <= npar - 1 && fabs(w.pn - x) <= half) {
+        w.k += 1;
+        w.pk = w.pn;
+        w.pn = (w.k + 1 <= npar - 1) ? pa[w.k + 1] : 0.0;
+        ok = true;
+      } else {
+        ok = fabs(w.pk - x) <= half;
+      }
+    } else {
+      if (fabs(w.pk - x) <= half) {
+        ok = true;
+      } else if (w.k - 1 >= 1 && fabs(w.pn - x) <= half) {
+        w.k -= 1;
+        w.pk = w.pn;
+        w.pn = (w.k - 1 >= 1) ? pa[w.k - 1] : 0.0;
+        ok = true;
+      }
+    }
+  }
+  int r;
+  if (ok) {
+    r = min(max(w.k, 2), npar - 2);
+  } else {
+    // full search: ipar_close, and the unclamped index for the next point
+    r = ipar_close(S, x);
+    int best = -1;
+    for (int k = min(r + 3, npar - 1); k >= max(r - 3, 1); --k)
+      if (fabs(pa[k] - x) <= half) {
+        best = k;
+        break;
+      }
+    w.k = best;
+    if (best >= 0) {
+      w.pk = pa[best];
+      const int kn = best + DIR;
+      w.pn = (kn >= 1 && kn <= npar - 1) ? pa[kn] : 0.0;
+    }
+  }
+  if (r != w.r) {
+    walk_flush(w, win, lo);
+    w.r = r;
+    w.pc = pa[r];
+  }
+  const double f = (x - w.pc) * hinv;  // funct_g's 0.5 (x - p_c) / dppar
+  w.c0 = w.c0 + wt;
+  w.cp = w.cp + f * wt;
+}
+
 // Principal-value and tiny-rest coefficients of one resonant row (integrate_res
 // after its two integrate() calls), this lane's share.
 template <int NCH>
@@ -991,6 +1151,11 @@ __device__ void pv_coefs(const SpecDev &S, const ProbDev &Pd, const RowClass &rc
   if (fabs(pim) > Pd.Tlim) {
     const int blk = (NRI + 1 + NCH - 1) / NCH;
     const int j0 = c * blk, j1 = min(j0 + blk, NRI + 1);
+    const double half = 0.5 * S.dppar_abs;
+    const bool fast = fabs(smdelta) < 0.9 * S.dppar_abs;
+    Walk wu, wd;
+    walk_reset(wu);
+    walk_reset(wd);
     for (int j = j0; j < j1; ++j) {
       double x, xm;
       {
@@ -1000,9 +1165,11 @@ __device__ void pv_coefs(const SpecDev &S, const ProbDev &Pd, const RowClass &rc
       }
       const double w = (j == 0 || j == NRI) ? 1.0 : 2.0;
       const cd r1 = cscale_inv(w, mk(x - pr, -pim));
-      coef_add(S, cd1, u1, win, lo, x, 1, r1, hinv);
-      coef_add(S, cd2, u2, win, lo, xm, -1, mk(-r1.re, r1.im), hinv);  // - conj(r1)
+      walk_add<1>(S, wu, win, lo, x, r1, half, hinv, fast);
+      walk_add<-1>(S, wd, win, lo, xm, mk(-r1.re, r1.im), half, hinv, fast);  // - conj(r1)
     }
+    walk_flush(wu, win, lo);
+    walk_flush(wd, win, lo);
   } else {
     const int blk = (NRI + NCH - 1) / NCH;
     const int j0 = 1 + c * blk, j1 = min(j0 + blk, NRI + 1);
@@ -1052,8 +1219,10 @@ __device__ void pv_coefs(const SpecDev &S, const ProbDev &Pd, const RowClass &rc
   if (ntiny > 0) {
     const int blk = (ntiny + 1 + NCH - 1) / NCH;
     const int k0 = c * blk, k1 = min(k0 + blk, ntiny + 1);
-    cache_flush(cd1, win, lo);
-    u1.ic = -1;
+    const double half = 0.5 * S.dppar_abs;
+    const bool fast = fabs(cs) < 0.9 * S.dppar_abs;
+    Walk wt;
+    walk_reset(wt);
     for (int kk = k0; kk < k1; ++kk) {
       double x;
       {
@@ -1061,8 +1230,9 @@ __device__ void pv_coefs(const SpecDev &S, const ProbDev &Pd, const RowClass &rc
         x = (kk == 0) ? x0 : x0 + cs * kk;
       }
       const double w = (kk == 0 || kk == ntiny) ? 1.0 : 2.0;
-      coef_add(S, cd1, u1, win, lo, x, 1, cscale_inv(corr * w, mk(x - pr, -pim)), hinv);
+      walk_add<1>(S, wt, win, lo, x, cscale_inv(corr * w, mk(x - pr, -pim)), half, hinv, fast);
     }
+    walk_flush(wt, win, lo);
   }
   cache_flush(cd1, win, lo);
   cache_flush(cd2, win, lo);

@@ -414,13 +414,12 @@ __global__ __launch_bounds__(64) void k_rsum(int npar, int nmax, double m,
 // One workgroup per (n, sg): lane q of each of the 4 wavefronts sums c_q over
 // a quarter of the nodes; the quarters are added in wavefront order.
 static_assert(SER_Q == 64, "one lane per series coefficient");
-__global__ __launch_bounds__(256) void k_series(int npar, int nmax, double kpar, double q, double m,
-                                                const double *__restrict__ ppar,
-                                                const double *__restrict__ nodes,
-                                                double *__restrict__ ser, double *__restrict__ serD) {
+__device__ __forceinline__ void series_block(int n, int sg, int npar, double kpar, double q,
+                                             double m, const double *__restrict__ ppar,
+                                             const double *__restrict__ nodes,
+                                             double *__restrict__ ser, double *__restrict__ serD) {
   __shared__ double part[4][6][64];
   __shared__ double dpart[4];
-  const int n = blockIdx.x, sg = blockIdx.y;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const double nq = (sg == 0 ? 1.0 : -1.0) * ((1.0 * n) * q);
   double dmin = INFINITY;
@@ -473,15 +472,26 @@ __global__ __launch_bounds__(256) void k_series(int npar, int nmax, double kpar,
   }
 }
 
+__global__ __launch_bounds__(256) void k_series(int npar, int nmax, double kpar, double q, double m,
+                                                const double *__restrict__ ppar,
+                                                const double *__restrict__ nodes,
+                                                double *__restrict__ ser, double *__restrict__ serD) {
+  series_block(blockIdx.x, blockIdx.y, npar, kpar, q, m, ppar, nodes, ser, serD);
+}
+
 hipError_t launch_split_tables(hipStream_t st, int npar, int nmax, double kpar, double q, double m,
                                const double *ppar, const double *nodes, double *crec,
-                               double *rsum, double *ser, double *serD) {
+                               double *rsum) {
   hipLaunchKernelGGL(k_crec, dim3((npar - 1 + 63) / 64, nmax + 1), dim3(64), 0, st, npar, nmax,
                      kpar, q, m, ppar, nodes, crec);
   hipLaunchKernelGGL(k_rsum, dim3(nmax + 1), dim3(64), 0, st, npar, nmax, m, ppar, nodes, rsum);
-  if (ser != nullptr)
-    hipLaunchKernelGGL(k_series, dim3(nmax + 1, 2), dim3(256), 0, st, npar, nmax, kpar, q, m, ppar,
-                       nodes, ser, serD);
+  return hipGetLastError();
+}
+
+hipError_t launch_series_tables(hipStream_t st, int npar, int nmax, double kpar, double q, double m,
+                                const double *ppar, const double *nodes, double *ser, double *serD) {
+  hipLaunchKernelGGL(k_series, dim3(nmax + 1, 2), dim3(256), 0, st, npar, nmax, kpar, q, m, ppar,
+                     nodes, ser, serD);
   return hipGetLastError();
 }
 
@@ -531,6 +541,16 @@ __global__ void k_atab_jobs(const TabJob *__restrict__ jobs, const int *__restri
   const int tiles = (J.nperp + 1 + 63) / 64;
   const int bx = b % tiles, by = b / tiles;
   atab_entry(bx * 64 + threadIdx.x, by, J.nperp, J.nmax, J.pperp, J.bes, J.atab);
+}
+
+// the series tables of many (request, species) jobs: 2 (nmax + 1) workgroups
+// per job with tables, (n, sign) = (b / 2, b % 2)
+__global__ __launch_bounds__(256) void k_series_jobs(const TabJob *__restrict__ jobs,
+                                                     const int *__restrict__ start, int njobs) {
+  const int j = job_of(start, njobs, blockIdx.x);
+  const TabJob &J = jobs[j];
+  const int b = blockIdx.x - start[j];
+  series_block(b >> 1, b & 1, J.npar, J.kpar, J.q, J.m, J.ppar, J.nodes, J.ser, J.serD);
 }
 
 // moments_block for MB consecutive harmonics n0 .. n0+MB-1 (those above nmax
@@ -827,6 +847,7 @@ int jobs_blocks_atab(const TabJob &J) { return ((J.nperp + 1 + 63) / 64) * (J.nm
 int jobs_blocks_moments(const TabJob &J) {
   return ((2 * (J.npar - 1) + 16 * MT - 1) / (16 * MT)) * ((3 * (J.nmax + 1) + 15) / 16);
 }
+int jobs_blocks_series(const TabJob &J) { return J.ser != nullptr ? 2 * (J.nmax + 1) : 0; }
 
 // start arrays: [njobs + 1] each, device copies; total = start[njobs]
 hipError_t launch_bessel_max_jobs(hipStream_t st, const TabJob *jobs, const int *start, int njobs,
@@ -845,6 +866,14 @@ hipError_t launch_species_k_jobs(hipStream_t st, const TabJob *jobs, int njobs,
   hipLaunchKernelGGL(k_atab_jobs, dim3(total_atab), dim3(64), 0, st, jobs, start_atab, njobs);
   (void)max_nperp;
   hipLaunchKernelGGL(k_moments_mfma_jobs, dim3(total_mom), dim3(64), 0, st, jobs, start_mom, njobs);
+  return hipGetLastError();
+}
+
+// after launch_species_k_jobs (the series read the node tables)
+hipError_t launch_series_jobs(hipStream_t st, const TabJob *jobs, int njobs, const int *start,
+                              int total) {
+  if (njobs <= 0 || total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_series_jobs, dim3(total), dim3(256), 0, st, jobs, start, njobs);
   return hipGetLastError();
 }
 

@@ -349,6 +349,8 @@ def run_serve(args):
     pool, reqs = serve_requests(args.requests, args.variants)
     rng = np.random.default_rng(11)
     S = RootServer(hbm_bytes=args.hbm_gb * 1e9, max_slots=args.slots)
+    if args.serve_series:
+        S.set_sweep_series(True)
     # the plasma variants are registered once; requests name them (inline
     # plasmas with --inline: each request carries and uploads its own tables)
     ref = list(pool) if args.inline else [S.add_plasma(p) for p in pool]
@@ -754,7 +756,10 @@ def main():
     ap.add_argument("--no-serve", action="store_true",
                     help="map: skip the served-requests leg reported beside the map line")
     ap.add_argument("--no-series", action="store_true",
-                    help="map: every sweep row summed node by node (no far-row series), for comparison")
+                    help="map: every sweep row summed node by node (no far-row series), "
+                         "for comparison")
+    ap.add_argument("--serve-series", action="store_true",
+                    help="serve: requests build far-row series tables (off by default)")
     ap.add_argument("--inline", action="store_true",
                     help="serve: every request carries its plasma (no registry)")
     args = ap.parse_args()

@@ -91,8 +91,9 @@ def test_served_roots_match_oracle_and_are_mix_independent():
 def test_small_arena_queues_and_rejects_what_cannot_fit():
     reqs = _requests()
     alone = _alone(reqs)
-    # an arena that holds only a few requests at once: the rest wait for space
-    S = _server(hbm=1 << 20, slots=64)
+    # an arena that holds only a few requests at once (their node and series
+    # tables): the rest wait for space
+    S = _server(hbm=5 << 19, slots=64)
     for i, (p, kp, kz, g) in enumerate(reqs):
         S.submit(p, kp, kz, g, p.roots, tag=i)
     big = _tight(cases.config1())
@@ -124,7 +125,7 @@ def test_registered_plasmas_give_the_same_roots_and_survive_eviction():
     import time
     reqs = _requests()
     alone = _alone(reqs)
-    for hbm in (2e9, 1 << 20):
+    for hbm in (2e9, 5 << 19):
         S = _server(hbm=hbm, slots=64)
         ids = {}
         for p, *_ in reqs:
