@@ -56,7 +56,7 @@ hipError_t launch_lsum(hipStream_t st, int nperp, int nmax, double dpp, const do
 hipError_t launch_fill_i32(hipStream_t st, int *p, int n, int v);
 hipError_t launch_split_tables(hipStream_t st, int npar, int nmax, double kpar, double q, double m,
                                const double *ppar, const double *nodes, double *crec,
-                               double *rsum);
+                               double *rsum, double *gcum);
 hipError_t launch_series_tables(hipStream_t st, int npar, int nmax, double kpar, double q, double m,
                                 const double *ppar, const double *nodes, double *ser, double *serD);
 hipError_t launch_rel_bessel(hipStream_t st, int npts, int nmax, double zfac, const double *grel,
@@ -187,6 +187,7 @@ struct alpsg_ctx {
   // per-k device data
   DevBuf<double> bes[ALPSG_MAXSPEC], atab[ALPSG_MAXSPEC], nodes[ALPSG_MAXSPEC];
   DevBuf<double> crec[ALPSG_MAXSPEC], rsum[ALPSG_MAXSPEC];  // split-form sweep tables
+  DevBuf<double> gcum[ALPSG_MAXSPEC];  // cumulative G/m sums (partial-range split rows)
   DevBuf<double> ser[ALPSG_MAXSPEC], serD[ALPSG_MAXSPEC];  // far-row series tables
   bool series_on = true;    // far rows of the node sweep take their series
   bool ser_pending = false;  // the current k's series tables are not built yet
@@ -284,6 +285,7 @@ struct alpsg_ctx {
       nodes[s].release();
       crec[s].release();
       rsum[s].release();
+      gcum[s].release();
       ser[s].release();
       serD[s].release();
       bmi[s].release();
@@ -2018,7 +2020,7 @@ int alpsg_set_k(alpsg_handle h, double kperp, double kpar, const int *nmax_in, i
     S.z = h->kperp_norm ? kperp / h->qs[s] : 1.0 / h->qs[s];
     S.atab = h->atab[s].p;
     S.nodes = h->nodes[s].p;
-    S.crec = S.rsum = nullptr;
+    S.crec = S.rsum = S.gcum = nullptr;
     S.ser = S.serD = nullptr;
     S.lsum = nullptr;
     if (!h->relsp[s] && S.acmethod == 1 && S.fit_uniform && S.nfits == 1) {
@@ -2030,13 +2032,15 @@ int alpsg_set_k(alpsg_handle h, double kperp, double kpar, const int *nmax_in, i
     if (!h->relsp[s]) {
       HIPCHK(h->crec[s].ensure((size_t)(nm + 1) * (npar - 1) * CREC_STRIDE));
       HIPCHK(h->rsum[s].ensure((size_t)(nm + 1) * RSUM_STRIDE));
+      HIPCHK(h->gcum[s].ensure((size_t)(nm + 1) * npar * GCUM_STRIDE));
       HIPCHK(h->ser[s].ensure((size_t)(nm + 1) * 2 * SER_Q * SER_STRIDE));
       HIPCHK(h->serD[s].ensure((size_t)(nm + 1) * 2));
       HIPCHK(launch_split_tables(h->st, npar, nm, kpar, h->qs[s], h->ms[s],
                                  h->ppar.p + (size_t)s * (npar + 1), h->nodes[s].p, h->crec[s].p,
-                                 h->rsum[s].p));
+                                 h->rsum[s].p, h->gcum[s].p));
       S.crec = h->crec[s].p;
       S.rsum = h->rsum[s].p;
+      S.gcum = h->gcum[s].p;
     }
     if (h->relsp[s]) {
       // the harmonics run in k_rel_nonres / k_rel_res, not in the sweep

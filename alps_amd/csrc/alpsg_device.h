@@ -13,6 +13,7 @@
 #define FIT_STRIDE 8    // doubles per (iperp, fit) entry of the analytic-continuation table
 #define CREC_STRIDE 13  // doubles per (n, ipar) record of the split-form sweep table
 #define RSUM_STRIDE 8   // doubles per harmonic of the split-form row sums
+#define GCUM_STRIDE 8   // doubles per (n, ipar) cumulative G/m sum (six used)
 #define SER_Q 64        // coefficients of a far row's series in m w (degrees 0..SER_Q-1)
 #define SER_STRIDE 8    // doubles per series coefficient (the six node products, 2 pad)
 #define SER_RHO 0.55    // largest |m w| / min|kpar p +- n q| a far row's series is used at
@@ -43,6 +44,9 @@ struct SpecDev {
   // omega-independent sums of the G/m terms over the interior nodes
   const double *crec;   // [(nmax+1)*(npar-1)*CREC_STRIDE]
   const double *rsum;   // [(nmax+1)*RSUM_STRIDE]
+  // gcum[n][i] = the G/m terms of rsum summed over ipar = 1..i (gcum[n][0] =
+  // 0): a partial-range row's G/m part over nodes a..b is gcum[b] - gcum[a-1]
+  const double *gcum;   // [(nmax+1)*npar*GCUM_STRIDE]
   // Far rows (may be NULL).  For sign sg (0: +n, 1: -n) with d_i = kpar p_i
   // +- n q and D = min |d_i| over ipar = 1..npar-1 (serD[2n+sg], 0 if none),
   // the full-range node sum of a row whose every lane has |m w| <= SER_RHO D

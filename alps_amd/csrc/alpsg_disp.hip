@@ -493,6 +493,71 @@ __device__ __forceinline__ bool far_row_lanes(const SpecDev &S, int nn, int c, d
   return true;
 }
 
+// The interior nodes of a partial-range row pair in the split form (one lane
+// per omega, records read as wave-uniform scalars): every node i = 2..npar-2
+// is visited and its reciprocal zeroed, by a select, outside the lane's
+// interior ranges (an excluded node may sit on the resonance), so the C/den
+// sums cover exactly the lane's ranges; the G/m sums over the same ranges
+// come from the cumulative table, gcum[b] - gcum[a-1] per range.
+template <bool PAIR>
+__device__ __forceinline__ void sweep_split_masked(cdptr crow, const double *gcum, int npar,
+                                                   double omr, double m, double nq, double dim,
+                                                   int pl1, int pc1, int pl2, int pc2, int ml1,
+                                                   int mc1, int ml2, int mc2, cd ap[6], cd am[6]) {
+  const double dim2 = dim * dim, ndim = -dim;
+  const double mom = m * omr;
+  double cur[CREC_STRIDE];
+#pragma unroll
+  for (int t = 0; t < CREC_STRIDE; ++t) cur[t] = crow[1 * CREC_STRIDE + t];  // node 2
+  for (int i = 2; i <= npar - 2; ++i) {
+    const int in = (i < npar - 2) ? i + 1 : i;
+    double nxt[CREC_STRIDE];
+#pragma unroll
+    for (int t = 0; t < CREC_STRIDE; ++t) nxt[t] = crow[(in - 1) * CREC_STRIDE + t];
+    const double base = mom - cur[0];
+    {
+      const bool inP = ((unsigned)(i - pl1) < (unsigned)pc1) | ((unsigned)(i - pl2) < (unsigned)pc2);
+      const double dre = base - nq;
+      double inv = rcp1_d(__builtin_fma(dre, dre, dim2));
+      inv = inP ? inv : 0.0;
+      const double rr = dre * inv, ri = ndim * inv;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        ap[k].re = __builtin_fma(cur[1 + k], rr, ap[k].re);
+        ap[k].im = __builtin_fma(cur[1 + k], ri, ap[k].im);
+      }
+    }
+    if (PAIR) {
+      const bool inM = ((unsigned)(i - ml1) < (unsigned)mc1) | ((unsigned)(i - ml2) < (unsigned)mc2);
+      const double dre = base + nq;
+      double inv = rcp1_d(__builtin_fma(dre, dre, dim2));
+      inv = inM ? inv : 0.0;
+      const double rr = dre * inv, ri = ndim * inv;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        am[k].re = __builtin_fma(cur[7 + k], rr, am[k].re);
+        am[k].im = __builtin_fma(cur[7 + k], ri, am[k].im);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < CREC_STRIDE; ++t) cur[t] = nxt[t];
+  }
+  auto range = [&](int pl, int pc, cd acc[6]) {
+    if (pc > 0) {
+      const double *hi = gcum + (size_t)(pl + pc - 1) * GCUM_STRIDE;
+      const double *lo = gcum + (size_t)(pl - 1) * GCUM_STRIDE;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) acc[k].re += hi[k] - lo[k];
+    }
+  };
+  range(pl1, pc1, ap);
+  range(pl2, pc2, ap);
+  if (PAIR) {
+    range(ml1, mc1, am);
+    range(ml2, mc2, am);
+  }
+}
+
 // endpoint contribution (weight 1) of node e for a row with denominator shift nq
 __device__ __forceinline__ void endpoint_acc(const double *__restrict__ rows, cdptr pa,
                                              int e, int npar,
@@ -698,8 +763,19 @@ __global__ __launch_bounds__(64 * W) void k_sweep(const ProbDev *__restrict__ pr
         }
       } else {
         w_nodes += (nn > 0 ? 2 : 1) * (npar - 1);
-        sweep_nodes<NCH, false, true>(srows, spa, npar, c, omr, omi, m, kpar, nq, dim, pl1, pc1, pl2, pc2,
-                                      ml1, mc1, ml2, mc2, ap, am);
+        if (NCH == 1 && S.gcum != nullptr && __all(split_ok)) {
+          const cdptr crow = (cdptr)(S.crec + (size_t)nn * (npar - 1) * CREC_STRIDE);
+          const double *gc = S.gcum + (size_t)nn * npar * GCUM_STRIDE;
+          if (nn > 0)
+            sweep_split_masked<true>(crow, gc, npar, omr, m, nq, dim, pl1, pc1, pl2, pc2, ml1, mc1,
+                                     ml2, mc2, ap, am);
+          else
+            sweep_split_masked<false>(crow, gc, npar, omr, m, nq, dim, pl1, pc1, pl2, pc2, ml1, mc1,
+                                      ml2, mc2, ap, am);
+        } else {
+          sweep_nodes<NCH, false, true>(srows, spa, npar, c, omr, omi, m, kpar, nq, dim, pl1, pc1, pl2,
+                                        pc2, ml1, mc1, ml2, mc2, ap, am);
+        }
       }
       // (with several lanes per omega a full-range row took its end nodes in
       // the sweep; a far row's series holds them)

@@ -479,12 +479,55 @@ __global__ __launch_bounds__(256) void k_series(int npar, int nmax, double kpar,
   series_block(blockIdx.x, blockIdx.y, npar, kpar, q, m, ppar, nodes, ser, serD);
 }
 
+// gcum[n][i] = sum over ipar = 1..i of {G1, p G1, p^2 G1, G2, G3, p G3} / m,
+// gcum[n][0] = 0: one wavefront per harmonic, an inclusive lane scan per
+// chunk of 64 nodes plus the running carry
+__global__ __launch_bounds__(64) void k_gcum(int npar, int nmax, double m,
+                                             const double *__restrict__ ppar,
+                                             const double *__restrict__ nodes,
+                                             double *__restrict__ gcum) {
+  const int n = blockIdx.x, lane = threadIdx.x;
+  double *g = gcum + (size_t)n * npar * GCUM_STRIDE;
+  if (lane < GCUM_STRIDE) g[lane] = 0.0;
+  double carry[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int base = 1; base <= npar - 1; base += 64) {
+    const int i = base + lane;
+    double t[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (i <= npar - 1) {
+      const double *r = nodes + ((size_t)n * (npar - 1) + (i - 1)) * NODE_STRIDE;
+      const double p = ppar[i];
+      t[0] = r[0] / m;
+      t[1] = (p * r[0]) / m;
+      t[2] = ((p * p) * r[0]) / m;
+      t[3] = r[2] / m;
+      t[4] = r[4] / m;
+      t[5] = (p * r[4]) / m;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const double v = __shfl_up(t[k], o, 64);
+        if (lane >= o) t[k] += v;
+      }
+      t[k] += carry[k];
+      carry[k] = __shfl(t[k], 63, 64);
+    }
+    if (i <= npar - 1) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) g[(size_t)i * GCUM_STRIDE + k] = t[k];
+      g[(size_t)i * GCUM_STRIDE + 6] = g[(size_t)i * GCUM_STRIDE + 7] = 0.0;
+    }
+  }
+}
+
 hipError_t launch_split_tables(hipStream_t st, int npar, int nmax, double kpar, double q, double m,
                                const double *ppar, const double *nodes, double *crec,
-                               double *rsum) {
+                               double *rsum, double *gcum) {
   hipLaunchKernelGGL(k_crec, dim3((npar - 1 + 63) / 64, nmax + 1), dim3(64), 0, st, npar, nmax,
                      kpar, q, m, ppar, nodes, crec);
   hipLaunchKernelGGL(k_rsum, dim3(nmax + 1), dim3(64), 0, st, npar, nmax, m, ppar, nodes, rsum);
+  hipLaunchKernelGGL(k_gcum, dim3(nmax + 1), dim3(64), 0, st, npar, nmax, m, ppar, nodes, gcum);
   return hipGetLastError();
 }
 

@@ -59,19 +59,20 @@ KPERP = KPAR = 0.5
 # HBM traffic of the map-batch k_sweep launch by its shape.
 # FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of k_sweep on the 256 x 256 map
 # batch for every one-lane launch shape (lanes, wavefronts, workgroups per omega
-# block), one counter pass each over `bench.py --pmc-shapes` (48-byte node
-# records, after one warm-up batch); a 262144 KiB copy
-# reads FETCH_SIZE 131084-131089 and WRITE_SIZE 262144, so FETCH_SIZE counts
-# half the bytes fetched and WRITE_SIZE all bytes written.
+# block), one counter pass each over `bench.py --pmc-shapes` (far rows summed
+# by their series, after one warm-up batch); a 262144 KiB copy reads
+# FETCH_SIZE 131084-131089 and WRITE_SIZE 262144, so FETCH_SIZE counts half
+# the bytes fetched and WRITE_SIZE all bytes written.  The writes are the
+# partial-chi slots, 12288 KiB per slot (W x workgroups slots per omega).
 _SWEEP_PMC_KIB = {
-    (1, 1, 1): (12937.4, 12288.0), (1, 1, 2): (12942.2, 24576.0), (1, 1, 3): (14235.9, 36864.0),
-    (1, 1, 4): (14054.2, 49152.0), (1, 1, 6): (13482.7, 73728.0), (1, 1, 8): (13266.6, 98304.0),
-    (1, 1, 12): (13041.0, 147456.0), (1, 1, 16): (12984.4, 196608.0), (1, 2, 1): (12939.4, 24576.0),
-    (1, 2, 2): (16201.4, 49152.0), (1, 2, 3): (14878.9, 73728.0), (1, 2, 4): (14501.6, 98304.0),
-    (1, 2, 6): (13631.5, 147456.0), (1, 2, 8): (13896.5, 196608.0), (1, 2, 12): (14321.3, 294912.0),
-    (1, 2, 16): (15141.5, 393216.0), (1, 4, 1): (19515.0, 49152.0), (1, 4, 2): (18218.6, 98304.0),
-    (1, 4, 3): (16262.2, 147456.0), (1, 4, 4): (16801.9, 196608.0), (1, 4, 6): (16727.9, 294912.0),
-    (1, 4, 8): (17343.1, 393216.0), (1, 4, 12): (20248.5, 589824.0), (1, 4, 16): (20533.6, 786432.0),
+    (1, 1, 1): (2356.4, 12288.0), (1, 1, 2): (2353.7, 24576.0), (1, 1, 3): (2407.5, 36864.0),
+    (1, 1, 4): (2391.8, 49152.0), (1, 1, 6): (2399.6, 73728.0), (1, 1, 8): (2381.3, 98304.0),
+    (1, 1, 12): (2378.9, 147456.0), (1, 1, 16): (2378.5, 196608.0), (1, 2, 1): (2350.8, 24576.0),
+    (1, 2, 2): (2350.4, 49152.0), (1, 2, 3): (2351.2, 73728.0), (1, 2, 4): (2606.2, 98304.0),
+    (1, 2, 6): (2895.6, 147456.0), (1, 2, 8): (3009.0, 196608.0), (1, 2, 12): (3244.7, 294912.0),
+    (1, 2, 16): (3594.5, 393216.0), (1, 4, 1): (3046.9, 49152.0), (1, 4, 2): (3048.7, 98304.0),
+    (1, 4, 3): (3644.9, 147456.0), (1, 4, 4): (4094.3, 196608.0), (1, 4, 6): (4916.2, 294912.0),
+    (1, 4, 8): (5488.2, 393216.0), (1, 4, 12): (7419.7, 589824.0), (1, 4, 16): (8854.6, 786432.0),
 }
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 SWEEP_TRAFFIC_BYTES = {k: (f * FETCH_SCALE + w * WRITE_SCALE) * 1024.0

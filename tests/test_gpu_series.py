@@ -149,3 +149,27 @@ def test_served_solves_with_and_without_the_series():
             same += 1
             assert abs(a[1] - b[1]) <= 1e-10 * abs(b[1]), (i, a, b)
     assert same >= 1
+
+
+@pytest.mark.parametrize("case,k", [("config1", (0.5, 0.5)), ("three_species", (0.3, 0.3)),
+                                    ("kappa_case", (0.3, 0.3))])
+def test_partial_range_rows_in_the_split_form(case, k):
+    """Rows resonant for some omega of a wavefront: the one-lane sweep sums
+    their lanes' interior ranges in the split form (C/den node by node, G/m
+    from cumulative sums); two lanes per omega take the U/den form.  Both
+    agree to 1e-12 of the determinant's term scale, series off in both."""
+    from alps_amd.engine import Engine
+    p = getattr(cases, case)() if case != "config1" else cases.config1(64, 128)
+    rng = np.random.default_rng(21)
+    om = rng.uniform(0.05, 2.5, 2048) + 1j * rng.choice([-0.2, -0.02, -0.001, 0.0, 0.001, 0.05], 2048)
+    E = Engine(p)
+    E.set_k(*k)
+    E.set_sweep_series(False)
+    E.set_shape(om.size, ONE_LANE)
+    one, _, _, wave = E.disp_batch(om, want_tensors=True)
+    E.set_shape(om.size, (2, 4, 1, 16, 1))
+    two = E.disp_batch(om)
+    E.close()
+    err = max(abs(one[i] - two[i]) / cases.det_scale(wave[i]) for i in range(om.size))
+    print("split vs U/den form:", err, flush=True)
+    assert err <= 1e-12
