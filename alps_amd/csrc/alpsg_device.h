@@ -89,7 +89,6 @@ struct SpecDev {
 struct TabJob {
   const double *pperp, *ppar, *df1, *df2;
   double *bes, *atab, *nodes, *bmax;
-  double *ser, *serD;  // far-row series tables (NULL: none for this job)
   double kperp, kpar, q, m;
   int nperp, npar, nmax, ntry;
 };

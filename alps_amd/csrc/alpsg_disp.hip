@@ -450,49 +450,6 @@ __device__ __forceinline__ int series_degree(double rho) {
   return J;
 }
 
-// A far row with several lanes per omega: lane c < 12 of the omega's lanes
-// (c < 6 for n = 0) sums the series of one (sign c / 6, node product c % 6);
-// lane c = 0 then gathers the twelve sums into ap / am by lane shuffles, and
-// the other lanes keep 0, so summing the lanes gives the row.  Returns whether
-// the row is far for this omega (full-range rows only; all lanes of the omega
-// decide alike, so a shuffle reads only lanes on the same path), and the sum
-// of its signs' degrees in deg.  The omega's lanes are consecutive, c first.
-__device__ __forceinline__ bool far_row_lanes(const SpecDev &S, int nn, int c, double omr,
-                                              double omi, cd ap[6], cd am[6], int &deg) {
-  if (S.ser == nullptr) return false;
-  const double Dp = S.serD[2 * nn], Dm = (nn > 0) ? S.serD[2 * nn + 1] : Dp;
-  const double m = S.m;
-  const double mw = sqrt((m * m) * __builtin_fma(omr, omr, omi * omi));
-  if (!(Dp > 0.0 && Dm > 0.0 && mw <= SER_RHO * Dp && mw <= SER_RHO * Dm)) return false;
-  const int Jp = series_degree(mw / Dp), Jm = (nn > 0) ? series_degree(mw / Dm) : 0;
-  deg = Jp + Jm;
-  cd v = mk(0.0, 0.0);
-  if (c < (nn > 0 ? 12 : 6)) {
-    const int sg = c >= 6 ? 1 : 0, k = c - 6 * sg;
-    const double D = sg ? Dm : Dp;
-    const double *cq = S.ser + (size_t)(2 * nn + sg) * SER_Q * SER_STRIDE + k;
-    const int J = sg ? Jm : Jp;
-    const double zr = (m * omr) / D, zi = (m * omi) / D;
-    const double r2 = 2.0 * zr, sq = __builtin_fma(zr, zr, zi * zi);
-    double b1 = cq[J * SER_STRIDE], b2 = 0.0;
-    for (int q = J - 1; q >= 1; --q) {
-      const double t = __builtin_fma(-sq, b2, cq[q * SER_STRIDE]);
-      b2 = b1;
-      b1 = __builtin_fma(r2, b1, t);
-    }
-    v = mk(__builtin_fma(zr, b1, __builtin_fma(-sq, b2, cq[0])), zi * b1);
-  }
-  const int base = (int)(threadIdx.x & 63) - c;
-#pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    const cd a = mk(__shfl(v.re, base + k, 64), __shfl(v.im, base + k, 64));
-    const cd b = mk(__shfl(v.re, base + 6 + k, 64), __shfl(v.im, base + 6 + k, 64));
-    ap[k] = (c == 0) ? a : mk(0.0, 0.0);
-    am[k] = (c == 0 && nn > 0) ? b : mk(0.0, 0.0);
-  }
-  return true;
-}
-
 // The interior nodes of a partial-range row pair in the split form (one lane
 // per omega, records read as wave-uniform scalars): every node i = 2..npar-2
 // is visited and its reciprocal zeroed, by a select, outside the lane's
@@ -733,19 +690,7 @@ __global__ __launch_bounds__(64 * W) void k_sweep(const ProbDev *__restrict__ pr
           continue;
         }
       }
-      // with 16 or more lanes per omega, a far row takes its series per omega
-      // group, one (sign, product) per lane
-      bool farg = false;
-      if constexpr (NCH >= 16) {
-        int deg = 0;
-        farg = stdp && stdm && far_row_lanes(S, nn, c, omr, omi, ap, am, deg);
-        if (farg) {
-          w_ser += (nn > 0 ? 2 : 1);
-          w_deg += deg;
-        }
-      }
-      if (farg) {
-      } else if (std_row) {
+      if (std_row) {
         w_nodes += (nn > 0 ? 2 : 1) * (npar - 1);
         if (__all(split_ok)) {
           const cdptr crow = (cdptr)(S.crec + (size_t)nn * (npar - 1) * CREC_STRIDE);
@@ -777,9 +722,8 @@ __global__ __launch_bounds__(64 * W) void k_sweep(const ProbDev *__restrict__ pr
                                         pc2, ml1, mc1, ml2, mc2, ap, am);
         }
       }
-      // (with several lanes per omega a full-range row took its end nodes in
-      // the sweep; a far row's series holds them)
-      if (c == 0 && !(NCH > 1 && std_row) && !farg) {
+      // (with several lanes per omega a full-range row took its end nodes in the sweep)
+      if (c == 0 && !(NCH > 1 && std_row)) {
         endpoint_acc(rows, spa, rp.a1, npar, m, kpar, omr, omi, nq, ap);
         endpoint_acc(rows, spa, rp.b1, npar, m, kpar, omr, omi, nq, ap);
         if (rp.a2 > 0) {
@@ -854,7 +798,6 @@ __device__ __forceinline__ void sweep_req_body(const ProbDev *__restrict__ probp
                                                double2 *__restrict__ chiA,
                                                const int *__restrict__ pidx, int bx, int by, int gy) {
   constexpr int RPW = 64 / NCH;
-  static_assert(NCH >= 12, "a far row takes 12 lanes of its omega");
   const int iom = bx;
   if (iom >= nom) return;
   const ProbDev &Pd = probp[pidx[iom]];
@@ -906,10 +849,7 @@ __device__ __forceinline__ void sweep_req_body(const ProbDev *__restrict__ probp
       const double dim = m * omi;
       const bool stdp = (rp.a2 < 0) && rp.a1 == 1 && rp.b1 == npar - 1;
       const bool stdm = (nn == 0) || ((rm.a2 < 0) && rm.a1 == 1 && rm.b1 == npar - 1);
-      int deg = 0;
-      if (stdp && stdm && far_row_lanes(S, nn, c, omr, omi, ap, am, deg)) {
-        // a far row: its series, one (sign, product) per lane
-      } else if (stdp && stdm) {
+      if (stdp && stdm) {
         if (nn > 0)
           sweep_nodes_pf<NCH, true, true, K>(srows, spa, npar, c, omr, omi, m, kpar, nq, dim, 0, 0, 0, 0, 0, 0,
                                        0, 0, ap, am);

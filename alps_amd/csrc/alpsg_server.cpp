@@ -61,9 +61,6 @@ int jobs_blocks_bessel_max(const TabJob &J);
 int jobs_blocks_bessel(const TabJob &J);
 int jobs_blocks_atab(const TabJob &J);
 int jobs_blocks_moments(const TabJob &J);
-int jobs_blocks_series(const TabJob &J);
-hipError_t launch_series_jobs(hipStream_t st, const TabJob *jobs, int njobs, const int *start,
-                              int total);
 hipError_t launch_bessel_max_jobs(hipStream_t st, const TabJob *jobs, const int *start, int njobs,
                                   int total, const int *mtab);
 hipError_t launch_species_k_jobs(hipStream_t st, const TabJob *jobs, int njobs,
@@ -247,13 +244,13 @@ struct Result {
 // one job-list launch's host and device buffers
 struct JobBuf {
   TabJob *jobs_h = nullptr, *jobs_d = nullptr;
-  int *start_h = nullptr, *start_d = nullptr;  // 4 arrays of MAX_JOBS + 1
+  int *start_h = nullptr, *start_d = nullptr;  // 3 arrays of MAX_JOBS + 1
   hipError_t init() {
     hipError_t e = hipHostMalloc((void **)&jobs_h, sizeof(TabJob) * MAX_JOBS, hipHostMallocDefault);
     if (e == hipSuccess) e = hipMalloc((void **)&jobs_d, sizeof(TabJob) * MAX_JOBS);
     if (e == hipSuccess)
-      e = hipHostMalloc((void **)&start_h, sizeof(int) * 4 * (MAX_JOBS + 1), hipHostMallocDefault);
-    if (e == hipSuccess) e = hipMalloc((void **)&start_d, sizeof(int) * 4 * (MAX_JOBS + 1));
+      e = hipHostMalloc((void **)&start_h, sizeof(int) * 3 * (MAX_JOBS + 1), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc((void **)&start_d, sizeof(int) * 3 * (MAX_JOBS + 1));
     return e;
   }
   void free_all() {
@@ -268,9 +265,6 @@ struct JobBuf {
 
 struct alpsg_server {
   hipStream_t st = nullptr;
-  // requests admitted from now on get far-row series tables (off by default:
-  // building a request's tables costs more than its few dozen evaluations save)
-  std::atomic<bool> series_on{false};
   char *arena_mem = nullptr;
   Arena arena;
   int max_slots = 0;
@@ -746,14 +740,9 @@ int admit_k(alpsg_server *S, Request &r, int &njobs, int &max_nperp, int &nnew) 
   const int npts_rel = (P.ngamma + 1) * (P.npparbar + 1);
   Layout L;
   size_t ob[ALPSG_MAXSPEC], oa[ALPSG_MAXSPEC], on[ALPSG_MAXSPEC], orb[ALPSG_MAXSPEC], obm[ALPSG_MAXSPEC];
-  size_t osr[ALPSG_MAXSPEC], osd[ALPSG_MAXSPEC];
-  // far-row series tables for every tabulated non-relativistic species (the
-  // switch is read once per request, so a request's arithmetic does not
-  // depend on what else the server holds)
-  const bool with_ser = S->series_on.load();
   for (int s = 0; s < ns; ++s) {
     const int nm = r.nmax[s];
-    ob[s] = oa[s] = on[s] = orb[s] = obm[s] = osr[s] = osd[s] = 0;
+    ob[s] = oa[s] = on[s] = orb[s] = obm[s] = 0;
     if (P.isbm[s]) {
       if (!P.bm_cold[s]) obm[s] = L.add(sizeof(double2) * (size_t)(2 * bm_nrun[s] + 1));
       continue;
@@ -761,12 +750,7 @@ int admit_k(alpsg_server *S, Request &r, int &njobs, int &max_nperp, int &nnew) 
     ob[s] = L.add(sizeof(double) * (size_t)(nm + 3) * (nperp + 1));
     oa[s] = L.add(sizeof(double) * (size_t)(nm + 1) * (nperp + 1) * ATAB_STRIDE);
     on[s] = L.add(sizeof(double) * (size_t)(nm + 1) * (npar - 1) * NODE_STRIDE);
-    if (P.isrel[s]) {
-      orb[s] = L.add(sizeof(double) * (size_t)(nm + 2) * npts_rel);
-    } else if (with_ser) {
-      osr[s] = L.add(sizeof(double) * (size_t)(nm + 1) * 2 * SER_Q * SER_STRIDE);
-      osd[s] = L.add(sizeof(double) * (size_t)(nm + 1) * 2);
-    }
+    if (P.isrel[s]) orb[s] = L.add(sizeof(double) * (size_t)(nm + 2) * npts_rel);
   }
   const size_t off = alloc_evict(S, L.size);
   if (off == SIZE_MAX) return 1;
@@ -836,8 +820,6 @@ int admit_k(alpsg_server *S, Request &r, int &njobs, int &max_nperp, int &nnew) 
     J.atab = PK(oa[s]);
     J.nodes = PK(on[s]);
     J.bmax = nullptr;
-    J.ser = (P.isrel[s] || !with_ser) ? nullptr : PK(osr[s]);
-    J.serD = (P.isrel[s] || !with_ser) ? nullptr : PK(osd[s]);
     J.kperp = r.kperp;
     J.kpar = r.kpar;
     J.q = P.qs[s];
@@ -867,8 +849,6 @@ int admit_k(alpsg_server *S, Request &r, int &njobs, int &max_nperp, int &nnew) 
     sd.ppar = ppar_d;
     sd.atab = PK(oa[s]);
     sd.nodes = PK(on[s]);
-    sd.ser = J.ser;
-    sd.serD = J.serD;
     sd.fitc = PS(P.o_fit) + P.fit_off[s];
     if (P.acm[s] == 2 && !P.isrel[s]) {
       sd.poly_kind = 1;
@@ -1174,18 +1154,14 @@ int server_pass(alpsg_server *S) {
   }
   if (nj2 > 0) {
     int *sb = S->jb2.start_h, *sa = sb + (MAX_JOBS + 1), *sm = sa + (MAX_JOBS + 1);
-    int *ss = sm + (MAX_JOBS + 1);
     const int tb = fill_start(S->jb2.jobs_h, nj2, sb, jobs_blocks_bessel);
     const int ta = fill_start(S->jb2.jobs_h, nj2, sa, jobs_blocks_atab);
     const int tm = fill_start(S->jb2.jobs_h, nj2, sm, jobs_blocks_moments);
-    const int ts = fill_start(S->jb2.jobs_h, nj2, ss, jobs_blocks_series);
     SCHK(hipMemcpyAsync(S->jb2.jobs_d, S->jb2.jobs_h, sizeof(TabJob) * nj2, hipMemcpyHostToDevice, S->st));
-    SCHK(hipMemcpyAsync(S->jb2.start_d, S->jb2.start_h, sizeof(int) * 4 * (MAX_JOBS + 1),
+    SCHK(hipMemcpyAsync(S->jb2.start_d, S->jb2.start_h, sizeof(int) * 3 * (MAX_JOBS + 1),
                         hipMemcpyHostToDevice, S->st));
     int *db = S->jb2.start_d, *da = db + (MAX_JOBS + 1), *dm = da + (MAX_JOBS + 1);
-    int *ds = dm + (MAX_JOBS + 1);
     SCHK(launch_species_k_jobs(S->st, S->jb2.jobs_d, nj2, db, tb, da, ta, dm, tm, max_nperp, S->mtab_d));
-    SCHK(launch_series_jobs(S->st, S->jb2.jobs_d, nj2, ds, ts));
   }
   if (nnew > 0) {
     SCHK(hipMemcpyAsync(S->newp_d, S->newp_h, sizeof(ProbDev) * nnew, hipMemcpyHostToDevice, S->st));
@@ -1495,14 +1471,6 @@ int alpsg_server_work(alpsg_server *S, double *work7) {
   std::lock_guard<std::mutex> lk(S->mu);
   for (int k = 0; k < 3; ++k) work7[k] = S->snap.work[k];
   for (int k = 0; k < 4; ++k) work7[3 + k] = S->snap.kt[k];
-  return ALPSG_OK;
-  ALPSG_CATCH
-}
-
-int alpsg_server_set_sweep_series(alpsg_server *S, int on) {
-  ALPSG_TRY
-  if (!S) return alpsg_internal_fail(ALPSG_E_ARG, "null server");
-  S->series_on.store(on != 0);
   return ALPSG_OK;
   ALPSG_CATCH
 }

@@ -586,16 +586,6 @@ __global__ void k_atab_jobs(const TabJob *__restrict__ jobs, const int *__restri
   atab_entry(bx * 64 + threadIdx.x, by, J.nperp, J.nmax, J.pperp, J.bes, J.atab);
 }
 
-// the series tables of many (request, species) jobs: 2 (nmax + 1) workgroups
-// per job with tables, (n, sign) = (b / 2, b % 2)
-__global__ __launch_bounds__(256) void k_series_jobs(const TabJob *__restrict__ jobs,
-                                                     const int *__restrict__ start, int njobs) {
-  const int j = job_of(start, njobs, blockIdx.x);
-  const TabJob &J = jobs[j];
-  const int b = blockIdx.x - start[j];
-  series_block(b >> 1, b & 1, J.npar, J.kpar, J.q, J.m, J.ppar, J.nodes, J.ser, J.serD);
-}
-
 // moments_block for MB consecutive harmonics n0 .. n0+MB-1 (those above nmax
 // are computed on n = nmax and not stored): every df0 load serves MB
 // harmonics.  The weighted a-values of the block's harmonics and the pperp
@@ -890,7 +880,6 @@ int jobs_blocks_atab(const TabJob &J) { return ((J.nperp + 1 + 63) / 64) * (J.nm
 int jobs_blocks_moments(const TabJob &J) {
   return ((2 * (J.npar - 1) + 16 * MT - 1) / (16 * MT)) * ((3 * (J.nmax + 1) + 15) / 16);
 }
-int jobs_blocks_series(const TabJob &J) { return J.ser != nullptr ? 2 * (J.nmax + 1) : 0; }
 
 // start arrays: [njobs + 1] each, device copies; total = start[njobs]
 hipError_t launch_bessel_max_jobs(hipStream_t st, const TabJob *jobs, const int *start, int njobs,
@@ -909,14 +898,6 @@ hipError_t launch_species_k_jobs(hipStream_t st, const TabJob *jobs, int njobs,
   hipLaunchKernelGGL(k_atab_jobs, dim3(total_atab), dim3(64), 0, st, jobs, start_atab, njobs);
   (void)max_nperp;
   hipLaunchKernelGGL(k_moments_mfma_jobs, dim3(total_mom), dim3(64), 0, st, jobs, start_mom, njobs);
-  return hipGetLastError();
-}
-
-// after launch_species_k_jobs (the series read the node tables)
-hipError_t launch_series_jobs(hipStream_t st, const TabJob *jobs, int njobs, const int *start,
-                              int total) {
-  if (njobs <= 0 || total <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_series_jobs, dim3(total), dim3(256), 0, st, jobs, start, njobs);
   return hipGetLastError();
 }
 

@@ -47,7 +47,7 @@ EXPORTS = ["alpsg_last_error", "alpsg_create", "alpsg_destroy", "alpsg_set_k",
            "alpsg_server_submit", "alpsg_server_poll", "alpsg_server_stats",
            "alpsg_server_destroy", "alpsg_determine_param_fit", "alpsg_last_scan_steps",
            "alpsg_set_plain_roots", "alpsg_disp_batch_noalloc", "alpsg_server_work",
-           "alpsg_set_sweep_series", "alpsg_sweep_work", "alpsg_server_set_sweep_series"]
+           "alpsg_set_sweep_series", "alpsg_sweep_work"]
 
 
 class AlpsgError(RuntimeError):
@@ -156,8 +156,6 @@ def load_library(path: str = LIB_PATH):
     L.alpsg_server_stats.argtypes = [vp, C.POINTER(C.c_longlong), dp]
     L.alpsg_server_work.argtypes = [vp, dp]
     L.alpsg_server_work.restype = C.c_int
-    L.alpsg_server_set_sweep_series.argtypes = [vp, C.c_int]
-    L.alpsg_server_set_sweep_series.restype = C.c_int
     L.alpsg_server_destroy.argtypes = [vp]
     for name in ["alpsg_server_create", "alpsg_server_submit", "alpsg_server_poll",
                  "alpsg_server_stats", "alpsg_server_destroy", "alpsg_server_add_plasma",
@@ -584,11 +582,6 @@ class RootServer:
             hit = (plasma, d, keep)
             self._descs[key] = hit
         return hit[1]
-
-    def set_sweep_series(self, on: bool):
-        """Requests admitted from now on take (on) or skip (off, the default)
-        the far-row series in their node sweeps."""
-        _check(load_library().alpsg_server_set_sweep_series(self._h, int(bool(on))))
 
     def add_plasma(self, plasma: Plasma) -> int:
         d, keep = _make_desc(plasma)

@@ -93,6 +93,43 @@ def sweep_flops(work):
     return 54.0 * work[0] + 24.0 * work[2] + 12.0 * work[1]
 
 
+def resonant_rows(plasma, kpar, nmax, om):
+    """Rows (species, harmonic) that k_resonant takes at each omega, counted on
+    the host the way the device classifies them (determine_resonances,
+    ALPS_fns.f90:870-950): (principal-value rows, Landau rows) summed over
+    the omegas."""
+    om = np.asarray(om).ravel()
+    P = plasma.positions_principal
+    pv = landau = 0
+    for s, nm in enumerate(nmax):
+        pa = plasma.pp[s, 1, :, 1]
+        npar = pa.size - 1
+        d = pa[2] - pa[1]
+        m, q = plasma.ms[s], plasma.qs[s]
+        for n in range(-nm, nm + 1):
+            pr = (m * om.real - n * q) / kpar
+            found = ((pa[1] <= pr) & (pr < pa[npar])) | ((pr < pa[1]) & (pr >= pa[1] - P * d)) | \
+                    ((pr >= pa[npar - 1]) & (pr < pa[npar - 1] + P * d))
+            inner = (pa[1] <= pr) & (pr < pa[npar - 1])
+            ipr = np.where(inner, np.clip(np.searchsorted(pa, pr, side="right") - 1, 1, npar - 2), 0)
+            ipr = np.where(inner, ipr, np.where(pr < pa[1], 0, npar - 1))
+            is_pv = found & inner & (ipr - P > 2) & (ipr + P < npar - 2)
+            pv += int(np.sum(is_pv))
+            landau += int(np.sum(found & (om.imag <= 0.0)))
+    return pv, landau
+
+
+def resonant_flops(pv_rows, landau_rows, nri, P):
+    """Algorithmic FP64 flops of k_resonant: per principal-value row the
+    ladder's n_resonance_interval + 1 mirror pairs at 24 flops (the point, its
+    complex reciprocal counted with one division, and the two cells' c0 and
+    f-weighted sums) and the window contraction of 2P + 8 nodes at 72 flops
+    (six node products of U = w G + H and six complex multiply-adds); per
+    Landau row 50 flops (the per-k sums' O(1) form; a lower bound for other
+    continuations)."""
+    return pv_rows * (24.0 * (nri + 1) + 72.0 * (2 * P + 8)) + 50.0 * landau_rows
+
+
 def progress(msg):
     """One line per finished leg of the run, on stderr (stdout carries the JSON line)."""
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -350,8 +387,6 @@ def run_serve(args):
     pool, reqs = serve_requests(args.requests, args.variants)
     rng = np.random.default_rng(11)
     S = RootServer(hbm_bytes=args.hbm_gb * 1e9, max_slots=args.slots)
-    if args.serve_series:
-        S.set_sweep_series(True)
     # the plasma variants are registered once; requests name them (inline
     # plasmas with --inline: each request carries and uploads its own tables)
     ref = list(pool) if args.inline else [S.add_plasma(p) for p in pool]
@@ -759,8 +794,6 @@ def main():
     ap.add_argument("--no-series", action="store_true",
                     help="map: every sweep row summed node by node (no far-row series), "
                          "for comparison")
-    ap.add_argument("--serve-series", action="store_true",
-                    help="serve: requests build far-row series tables (off by default)")
     ap.add_argument("--inline", action="store_true",
                     help="serve: every request carries its plasma (no registry)")
     args = ap.parse_args()
@@ -871,6 +904,8 @@ def main():
     work = E.sweep_work(first["om"].T.ravel())
     flops_launch = sweep_flops(work)
     achieved = flops_launch / (split[0] * 1e-3) / 1e12
+    pv_rows, landau_rows = resonant_rows(p, KPAR, nmax, first["om"].T.ravel())
+    res_flops = resonant_flops(pv_rows, landau_rows, p.n_resonance_interval, p.positions_principal)
 
     # plain path: one omega per launch
     plain_om = last["om"].ravel()[:: max(1, last["om"].size // args.plain)][: args.plain]
@@ -922,6 +957,12 @@ def main():
                               "series_degree_sum": work[2],
                               "node_units_if_all_rows_node_summed": f_eval / 54.0 * args.nr * args.ni,
                               "flops_if_all_rows_node_summed": f_eval * args.nr * args.ni}},
+        "roofline_resonant": {"bound": "fp64 valu", "kernel": "k_resonant, both parts (map batch)",
+                              "flops": res_flops, "kernel_ms": float(split[1]),
+                              "achieved": res_flops / (split[1] * 1e-3) / 1e12,
+                              "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                              "frac": res_flops / (split[1] * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+                              "pv_rows": pv_rows, "landau_rows": landau_rows},
         "cpu_baseline": None,
     }
     progress(f"map: {out['value']:.4g} evals/s, {out['roots_per_sec']:.4g} roots/s")

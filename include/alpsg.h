@@ -265,15 +265,15 @@ int alpsg_last_scan_steps(alpsg_handle h, int nsteps, int nroots, double *ms_out
    device between the launches of a replayed graph.  The roots are the same. */
 int alpsg_set_plain_roots(alpsg_handle h, int plain);
 
-/* on != 0 (the default): rows of the node sweep whose every node is far from
-   resonance for the omega (|m w| <= 0.55 min |kpar p_i -+ n q|; with one
-   lane per omega, for every omega of a wavefront) take their node sum from a
+/* on != 0 (the default): in the one-lane-per-omega node sweep of large
+   batches, rows whose every node is far from resonance for every omega of a
+   wavefront (|m w| <= 0.55 min |kpar p_i -+ n q|) take their node sum from a
    per-k series table in m w, truncated below double rounding, instead of a
    pass over the nodes; on == 0: every row passes over its nodes.  Both give
    the reference's integrate() trapezoid sum (ALPS_fns.f90:799-864) to double
    rounding; the switch exists to compare them.  A k's tables are built by its
    first batch of 64 or more omegas (alpsg_disp_batch, alpsg_map_search,
-   alpsg_autotune, alpsg_sweep_work); smaller batches use them once built. */
+   alpsg_autotune, alpsg_sweep_work). */
 int alpsg_set_sweep_series(alpsg_handle h, int on);
 
 /* The work of the node sweep (k_sweep, in the launch shape of an n-omega
@@ -345,13 +345,6 @@ int alpsg_server_submit(alpsg_server *s, const alpsg_plasma_desc *desc, double k
 int alpsg_server_poll(alpsg_server *s, int max_results, int wait, long long *tags, double *roots,
                       int *iters, int *status, double *latency_ms, int *n_out);
 int alpsg_server_stats(alpsg_server *s, long long *counts, double *times_ms);
-/* Requests admitted after this call take (on != 0) or skip (on == 0, the
-   default) the far-row series of alpsg_set_sweep_series in their node
-   sweeps; the results agree to double rounding either way.  Off by default:
-   building each request's series tables costs more than the few dozen
-   evaluations of a solve save. */
-int alpsg_server_set_sweep_series(alpsg_server *s, int on);
-
 /* The algorithmic units of every disp evaluation the server has run and the
    device time they took (a diagnostic for rooflines, no counterpart in the
    reference): work7[0] the node records of the sweep, sum over evaluations and

@@ -87,19 +87,11 @@ def test_series_with_positive_growth_and_far_omegas():
         assert abs(det[i] - O.disp(om[i])) <= 1e-11 * cases.det_scale(wave[i]), (i, om[i])
 
 
-@pytest.mark.parametrize("shape", [(16, 4, 2, 32, 2), (32, 8, 2, 64, 4), (64, 4, 4, 64, 8)])
-def test_series_with_several_lanes_per_omega(shape):
-    """Two lanes of an omega's group sum a far row's two signs; the other
-    lanes' zeros drop out of the group sum."""
-    p = cases.config1(64, 128)
-    om = _map_omegas(600, seed=11)
-    _, _, w = _series_vs_nodes(p, 0.5, 0.5, om, shape=shape)
-    assert w[1] > 0
-
-
 def test_series_in_small_batches_and_secant_rounds():
     """Batches of at most 16 omegas (the request sweep, captured in graphs) and
-    the device-stepped secant rounds take far rows too: same det D and roots."""
+    the device-stepped secant rounds sum every row node by node, also after a
+    large batch has built the k's series tables: the same det D and roots with
+    the series switched on and off."""
     from alps_amd.engine import Engine
     p = cases.config1(64, 128)
     om = _map_omegas(16, seed=12)
@@ -117,38 +109,8 @@ def test_series_in_small_batches_and_secant_rounds():
     _, _, _, wave = E.disp_batch(om, want_tensors=True)
     E.close()
     for a, b in zip(on, off):
-        for i in range(a.size):
-            assert abs(a[i] - b[i]) <= 1e-12 * cases.det_scale(wave[i]), (a.size, i)
-    same = (r_on[2] < p.roots.numiter) & (r_on[2] == r_off[2])
-    print("secant steps with / without the series:", r_on[2], r_off[2], flush=True)
-    assert same.sum() >= 1
-    assert np.all(np.abs(r_on[0][same] - r_off[0][same]) <= 1e-10 * np.abs(r_off[0][same]))
-
-
-def test_served_solves_with_and_without_the_series():
-    """Asked to, the root server builds each request's series tables with its
-    node tables: the same requests solved with and without them agree."""
-    import dataclasses
-    from alps_amd.engine import RootServer
-    p = cases.config1(64, 128)
-    p = dataclasses.replace(p, roots=dataclasses.replace(p.roots, D_threshold=1e-14))
-    reqs = [(0.5, 0.5, g) for g in (0.3 - 0.1j, 0.9 - 0.2j, 1.4 - 0.3j, 0.05 - 0.4j)]
-    out = []
-    for on in (True, False):
-        S = RootServer(hbm_bytes=2e9, max_slots=16)
-        S.set_sweep_series(on)
-        for i, (kp, kz, g) in enumerate(reqs):
-            S.submit(p, kp, kz, g, p.roots, tag=i)
-        out.append(S.drain(len(reqs)))
-        S.close()
-    same = 0
-    for i in range(len(reqs)):
-        a, b = out[0][i], out[1][i]
-        print("served", i, a[1:], b[1:], flush=True)
-        if a[2] == b[2] and a[3] == b[3] == 0:
-            same += 1
-            assert abs(a[1] - b[1]) <= 1e-10 * abs(b[1]), (i, a, b)
-    assert same >= 1
+        assert np.array_equal(a, b)
+    assert np.array_equal(r_on[0], r_off[0]) and np.array_equal(r_on[2], r_off[2])
 
 
 @pytest.mark.parametrize("case,k", [("config1", (0.5, 0.5)), ("three_species", (0.3, 0.3)),
