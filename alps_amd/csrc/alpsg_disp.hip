@@ -1315,6 +1315,45 @@ __device__ void landau_part(const SpecDev &S, const ProbDev &Pd, int an, const R
       return mk(0.0, 0.0) + c0 * (which == 0 ? E0 : (which > 0 ? Ep : Em));
     }
   };
+  if constexpr (!UNI) {
+    // every continuation value is a call of eval_fit: each lane takes a
+    // contiguous block of rows and carries f(iperp-1, p), f(iperp, p) from row
+    // to row, so a row costs f(iperp+1, p) and f(iperp, p +- dppar), three
+    // calls instead of four (the same values and the same row formulas)
+    const int blk = (nperp + 1 + NCH - 1) / NCH;
+    const int r0 = c * blk, r1 = min(r0 + blk, nperp + 1) - 1;
+    if (r0 > r1) return;
+    cd fm = (r0 >= 1) ? F(r0 - 1, 0) : mk(0.0, 0.0);  // f(ip - 1, p)
+    cd f0 = F(r0, 0);                                // f(ip, p)
+    for (int ip = r0; ip <= r1; ++ip) {
+      double h;
+      cd dfperp, dfpar, fn = mk(0.0, 0.0);
+      if (ip < nperp) fn = F(ip + 1, 0);  // f(ip + 1, p)
+      const cd fpi = F(ip, 1), fpf = F(ip, -1);
+      if (ip >= 1 && ip <= nperp - 1) {
+        h = (ip == nperp - 1) ? 0.5 : 1.0;
+        if (czero(fpi) || czero(fpf) || czero(fn)) zero = 1;
+        dfperp = (fn - fm) * (1.0 / (2.0 * dpp));
+        dfpar = (fpi - fpf) * (1.0 / (2.0 * d));
+      } else if (ip == 0) {
+        h = 0.5;
+        dfperp = (fn - f0) * (1.0 / dpp);
+        dfpar = (fpi - fpf) * (1.0 / (2.0 * d));
+      } else {
+        h = 0.5;
+        dfperp = (f0 - fm) * (1.0 / dpp);
+        dfpar = (fpi - fpf) * (1.0 / (2.0 * d));
+      }
+      const cd K = qk * ((S.pperp[ip] * dfpar - p * dfperp) * km + mk(omr, omi) * dfperp);
+      const double *a = arow + (size_t)ip * ATAB_STRIDE;
+      L[0] = L[0] + (h * a[0]) * K;
+      L[1] = L[1] + (h * a[1]) * K;
+      L[2] = L[2] + (h * a[2]) * K;
+      fm = f0;
+      f0 = fn;
+    }
+    return;
+  }
   for (int ip = c; ip <= nperp; ip += NCH) {
     double h;
     cd dfperp, dfpar;

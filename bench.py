@@ -642,11 +642,16 @@ def run_kappa(args):
         for j in (0, 1, 2):
             pts.append((r["kperp"][0, j], r["kpar"][0, j], prev, r["roots"][0, j, 0], None))
             prev = r["roots"][0, j, 0]
-        dmax, agree, rows = 0.0, 0, []
+        dmax, drel, agree, rows = 0.0, 0.0, 0, []
         for kp, kz, start, g, git in pts:
             O.set_k(kp, kz)
-            dg = abs(O.disp(g))
+            dgc, _, _, wave = O.disp(g, want_tensors=True)
+            w = np.abs(wave)
+            sc = (w[0, 0] * (w[1, 1] * w[2, 2] + w[1, 2] ** 2) + 2 * w[0, 1] * w[1, 2] * w[0, 2]
+                  + w[0, 2] ** 2 * w[1, 1] + w[0, 1] ** 2 * w[2, 2])
+            dg = abs(dgc)
             dmax = max(dmax, dg / thr_d)
+            drel = max(drel, dg / sc)
             o, oit = O.secant_osc(start)
             agree += int(abs(g - o) <= 1e-10 * abs(o))
             rows.append([g.real, g.imag, o.real, o.imag, git, oit, dg])
@@ -658,6 +663,7 @@ def run_kappa(args):
         out["map_vs_oracle_on_sample"] = err
         out["roots_vs_oracle"] = {"roots": len(pts),
                                   "max_oracle_abs_D_over_threshold": dmax,
+                                  "max_oracle_abs_D_over_term_scale": drel,
                                   "agree_to_1e-10": agree,
                                   "rows": "GPU root re, im; oracle root from the same start re, im; "
                                           "GPU steps (start solve only); oracle steps; "
