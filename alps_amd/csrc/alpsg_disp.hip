@@ -29,6 +29,7 @@ namespace {
 constexpr double PI_D = 3.141592653589793238462643383279502884;
 
 typedef const __attribute__((address_space(4))) double *cdptr;  // invariant (scalar-loadable)
+typedef const __attribute__((address_space(1))) double *gdptr;  // global (per-lane loads)
 
 struct RowClass {
   int a1, b1, a2, b2;  // ppar index ranges of the trapezoid (integrate calls)
@@ -41,11 +42,10 @@ struct RowClass {
 
 // determine_resonances (ALPS_fns.f90:704-742) and the ipar_res / range logic of
 // integrate_res (ALPS_fns.f90:950-1007) for one (n, omega).
+template <typename PA>
 __device__ __forceinline__ void classify(const SpecDev &S, int P, double kpar, double omr,
-                                         double omi, int n, RowClass &rc,
-                                         const double *pa_copy = nullptr) {
+                                         double omi, int n, RowClass &rc, PA pa) {
 #pragma clang fp contract(off)
-  const double *pa = pa_copy ? pa_copy : S.ppar;
   const int npar = S.npar;
   const double d = S.dppar;
   const double pr = (S.m * omr - (1.0 * n) * S.q) / kpar;
@@ -97,11 +97,13 @@ __device__ __forceinline__ void classify(const SpecDev &S, int P, double kpar, d
   rc.pv = 1;
 }
 
-// funct_g's ipar_close (ALPS_fns.f90:1285-1294): the last ipar in [1,npar-1]
-// with |ppar(ipar) - x| <= dppar/2, clamped to [2, npar-2].
-__device__ __forceinline__ int ipar_close(const SpecDev &S, double x) {
+// The last ipar in [1, npar-1] with |ppar(ipar) - x| <= dppar/2, or -1 when
+// there is none: the candidates around the nearest grid index are loaded
+// together (one memory round trip, not a chain of dependent loads) and tested
+// from the highest index down, as the source's loop does.
+__device__ __forceinline__ int ipar_close_raw(const SpecDev &S, double x) {
 #pragma clang fp contract(off)
-  const double *pa = S.ppar;
+  const gdptr pa = (gdptr)S.ppar;
   const int npar = S.npar;
   const double half = 0.5 * S.dppar_abs;
   double t = (x - pa[1]) / S.dppar_abs;
@@ -112,16 +114,22 @@ __device__ __forceinline__ int ipar_close(const SpecDev &S, double x) {
     k0 = npar + 10;
   else
     k0 = 1 + (int)floor(t + 0.5);
-  int best = 0;
   const int khi = min(k0 + 2, npar - 1), klo = max(k0 - 2, 1);
-  for (int k = khi; k >= klo; --k)
-    if (fabs(pa[k] - x) <= half) {
-      best = k;
-      break;
-    }
-  if (best >= npar - 1) best = npar - 2;
-  if (best <= 1) best = 2;
+  double v[5];
+#pragma unroll
+  for (int u = 0; u < 5; ++u) v[u] = (khi - u >= klo) ? pa[khi - u] : 0.0;
+  int best = -1;
+#pragma unroll
+  for (int u = 0; u < 5; ++u)
+    if (best < 0 && khi - u >= klo && fabs(v[u] - x) <= half) best = khi - u;
   return best;
+}
+
+// funct_g's ipar_close (ALPS_fns.f90:1285-1294): the last ipar in [1,npar-1]
+// with |ppar(ipar) - x| <= dppar/2 (0 when there is none), clamped to [2, npar-2].
+__device__ __forceinline__ int ipar_close(const SpecDev &S, double x) {
+  const int best = ipar_close_raw(S, x);
+  return min(max(best, 2), S.npar - 2);
 }
 
 // node products [U1, p U1, p^2 U1, U2, U3, p U3], U_a = om*G_a + H_a, at the
@@ -1006,22 +1014,10 @@ __device__ __forceinline__ int ipar_close_step(const SpecDev &S, Cursor &cu, dou
       return r;
     }
   }
-  // full search (first point or a miss)
-  const int r = ipar_close(S, x);
-  {
-    const double *pa = S.ppar;
-    const double half = 0.5 * S.dppar_abs;
-    const int npar = S.npar;
-    // recover the unclamped index for the cursor
-    int best = -1;
-    for (int k = min(r + 3, npar - 1); k >= max(r - 3, 1); --k)
-      if (fabs(pa[k] - x) <= half) {
-        best = k;
-        break;
-      }
-    cu.ic = best;
-  }
-  return r;
+  // full search (first point or a miss); the cursor keeps the unclamped index
+  const int best = ipar_close_raw(S, x);
+  cu.ic = best;
+  return min(max(best, 2), S.npar - 2);
 }
 
 __device__ __forceinline__ void lds_cadd(double *w, int t, cd v) {
@@ -1093,7 +1089,7 @@ __device__ __forceinline__ void walk_add(const SpecDev &S, Walk &w, double *win,
                                          cd wt, double half, double hinv, bool fast) {
 #pragma clang fp contract(off)
   const int npar = S.npar;
-  const double *pa = S.ppar;
+  const gdptr pa = (gdptr)S.ppar;
   bool ok = false;
   if (fast && w.k >= 0) {
     if (DIR > 0) {
@@ -1120,14 +1116,9 @@ __device__ __forceinline__ void walk_add(const SpecDev &S, Walk &w, double *win,
   if (ok) {
     r = min(max(w.k, 2), npar - 2);
   } else {
-    // full search: ipar_close, and the unclamped index for the next point
-    r = ipar_close(S, x);
-    int best = -1;
-    for (int k = min(r + 3, npar - 1); k >= max(r - 3, 1); --k)
-      if (fabs(pa[k] - x) <= half) {
-        best = k;
-        break;
-      }
+    // full search: ipar_close before its clamp, kept for the next point
+    const int best = ipar_close_raw(S, x);
+    r = min(max(best, 2), npar - 2);
     w.k = best;
     if (best >= 0) {
       w.pk = pa[best];
@@ -1436,7 +1427,7 @@ __device__ __forceinline__ void resonant_body(const ProbDev *__restrict__ probp,
   const double scaleN = 2.0 * PI_D * S.dpperp * 0.25 * (-S.q / kpar);
   for (int n = nlo + rb; n <= nhi; n += RB) {
     RowClass rc;
-    classify(S, P, kpar, omr, omi, n, rc);
+    classify(S, P, kpar, omr, omi, n, rc, (gdptr)S.ppar);
     if (!rc.found || (PART == 0 && !rc.pv) || (PART != 0 && !rc.landau)) continue;
     const int an = n < 0 ? -n : n;
     const double *rows = S.nodes + (size_t)an * (npar - 1) * NODE_STRIDE;
