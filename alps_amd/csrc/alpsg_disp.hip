@@ -1055,38 +1055,57 @@ __device__ __forceinline__ void coef_add(const SpecDev &S, CellCache &cc, Cursor
 // shorter than 0.9 dppar (fast; the grid is uniform), so the neighbour's grid
 // value is kept in a register and each point makes the same |pa[k] - x| <=
 // dppar/2 tests as the full search.  A point that matches neither index (a
-// rounding gap between two cells), and every point of a ladder with longer
-// steps, takes the full search.  The current cell's coefficients are kept as c0 = sum wt,
-// cp = sum f wt (f = funct_g's interpolation fraction; the cell below takes
-// -cp) and go to the LDS window when the cell changes.
+// rounding gap between two cells), and every point of a coarser ladder, takes
+// the full search.
+//
+// The ladder's weights are r_j = w_j / (t_j - i pim), t_j = x_j - pr, and a
+// point adds r_j to cell c's c0 and f_j r_j to its cp, f_j = (x_j - p_c) /
+// (2 dppar) (funct_g's interpolation fraction; the cell below takes -cp).
+// Since t r = w + i pim r, the cell's sum of f r is (W + (pr - p_c + i pim) S)
+// / (2 dppar) with S = sum r_j and W = sum w_j: a point adds r_j and w_j, and
+// the cell's coefficients are formed when it is flushed.  The mirror ladder
+// x = 2 pr - x_j (weights -conj(r_j)) flushes c0 = -conj(S), cp = (W - (pr -
+// p_c + i pim) conj(S)) / (2 dppar).
 struct Walk {
   int k;           // the last index within dppar/2 of the current x, -1: unknown
   double pk, pn;   // pa[k] and pa[k + dir]
   int r;           // the interpolation cell: k clamped to [2, npar-2], -1: none yet
   double pc;       // pa[r]
-  cd c0, cp;
+  cd s;            // sum of r_j over the cell's points
+  double wsum;     // sum of w_j
 };
 
-__device__ __forceinline__ void walk_flush(Walk &w, double *win, int lo) {
+template <bool MIRROR>
+__device__ __forceinline__ void walk_flush(Walk &w, double *win, int lo, double pr, double pim,
+                                           double hinv) {
   if (w.r >= 0) {
-    lds_cadd(win, w.r - 1 - lo, mk(-w.cp.re, -w.cp.im));
-    lds_cadd(win, w.r - lo, w.c0);
-    lds_cadd(win, w.r + 1 - lo, w.cp);
+    const cd S = MIRROR ? mk(w.s.re, -w.s.im) : w.s;
+    const cd z = mk(pr - w.pc, pim);
+    const cd zS = z * S;
+    const cd c0 = MIRROR ? mk(-S.re, -S.im) : S;
+    const cd cp = MIRROR ? mk(hinv * (w.wsum - zS.re), -hinv * zS.im)
+                         : mk(hinv * (zS.re + w.wsum), hinv * zS.im);
+    lds_cadd(win, w.r - 1 - lo, mk(-cp.re, -cp.im));
+    lds_cadd(win, w.r - lo, c0);
+    lds_cadd(win, w.r + 1 - lo, cp);
   }
   w.r = -1;
-  w.c0 = w.cp = mk(0.0, 0.0);
+  w.s = mk(0.0, 0.0);
+  w.wsum = 0.0;
 }
 
 __device__ __forceinline__ void walk_reset(Walk &w) {
   w.k = -1;
   w.r = -1;
   w.pk = w.pn = w.pc = 0.0;
-  w.c0 = w.cp = mk(0.0, 0.0);
+  w.s = mk(0.0, 0.0);
+  w.wsum = 0.0;
 }
 
 template <int DIR>
 __device__ __forceinline__ void walk_add(const SpecDev &S, Walk &w, double *win, int lo, double x,
-                                         cd wt, double half, double hinv, bool fast) {
+                                         cd r, double wj, double half, double pr, double pim,
+                                         double hinv, bool fast) {
 #pragma clang fp contract(off)
   const int npar = S.npar;
   const gdptr pa = (gdptr)S.ppar;
@@ -1112,13 +1131,13 @@ __device__ __forceinline__ void walk_add(const SpecDev &S, Walk &w, double *win,
       }
     }
   }
-  int r;
+  int rc;
   if (ok) {
-    r = min(max(w.k, 2), npar - 2);
+    rc = min(max(w.k, 2), npar - 2);
   } else {
     // full search: ipar_close before its clamp, kept for the next point
     const int best = ipar_close_raw(S, x);
-    r = min(max(best, 2), npar - 2);
+    rc = min(max(best, 2), npar - 2);
     w.k = best;
     if (best >= 0) {
       w.pk = pa[best];
@@ -1126,14 +1145,13 @@ __device__ __forceinline__ void walk_add(const SpecDev &S, Walk &w, double *win,
       w.pn = (kn >= 1 && kn <= npar - 1) ? pa[kn] : 0.0;
     }
   }
-  if (r != w.r) {
-    walk_flush(w, win, lo);
-    w.r = r;
-    w.pc = pa[r];
+  if (rc != w.r) {
+    walk_flush<(DIR < 0)>(w, win, lo, pr, pim, hinv);
+    w.r = rc;
+    w.pc = pa[rc];
   }
-  const double f = (x - w.pc) * hinv;  // funct_g's 0.5 (x - p_c) / dppar
-  w.c0 = w.c0 + wt;
-  w.cp = w.cp + f * wt;
+  w.s = w.s + r;
+  w.wsum += wj;
 }
 
 // Principal-value and tiny-rest coefficients of one resonant row (integrate_res
@@ -1172,11 +1190,11 @@ __device__ void pv_coefs(const SpecDev &S, const ProbDev &Pd, const RowClass &rc
       }
       const double w = (j == 0 || j == NRI) ? 1.0 : 2.0;
       const cd r1 = cscale_inv(w, mk(x - pr, -pim));
-      walk_add<1>(S, wu, win, lo, x, r1, half, hinv, fast);
-      walk_add<-1>(S, wd, win, lo, xm, mk(-r1.re, r1.im), half, hinv, fast);  // - conj(r1)
+      walk_add<1>(S, wu, win, lo, x, r1, w, half, pr, pim, hinv, fast);
+      walk_add<-1>(S, wd, win, lo, xm, r1, w, half, pr, pim, hinv, fast);  // weight -conj(r1)
     }
-    walk_flush(wu, win, lo);
-    walk_flush(wd, win, lo);
+    walk_flush<false>(wu, win, lo, pr, pim, hinv);
+    walk_flush<true>(wd, win, lo, pr, pim, hinv);
   } else {
     const int blk = (NRI + NCH - 1) / NCH;
     const int j0 = 1 + c * blk, j1 = min(j0 + blk, NRI + 1);
@@ -1237,9 +1255,10 @@ __device__ void pv_coefs(const SpecDev &S, const ProbDev &Pd, const RowClass &rc
         x = (kk == 0) ? x0 : x0 + cs * kk;
       }
       const double w = (kk == 0 || kk == ntiny) ? 1.0 : 2.0;
-      walk_add<1>(S, wt, win, lo, x, cscale_inv(corr * w, mk(x - pr, -pim)), half, hinv, fast);
+      walk_add<1>(S, wt, win, lo, x, cscale_inv(corr * w, mk(x - pr, -pim)), corr * w, half, pr, pim,
+                  hinv, fast);
     }
-    walk_flush(wt, win, lo);
+    walk_flush<false>(wt, win, lo, pr, pim, hinv);
   }
   cache_flush(cd1, win, lo);
   cache_flush(cd2, win, lo);
