@@ -65,14 +65,14 @@ KPERP = KPAR = 0.5
 # the bytes fetched and WRITE_SIZE all bytes written.  The writes are the
 # partial-chi slots, 12288 KiB per slot (W x workgroups slots per omega).
 _SWEEP_PMC_KIB = {
-    (1, 1, 1): (2356.4, 12288.0), (1, 1, 2): (2353.7, 24576.0), (1, 1, 3): (2407.5, 36864.0),
-    (1, 1, 4): (2391.8, 49152.0), (1, 1, 6): (2399.6, 73728.0), (1, 1, 8): (2381.3, 98304.0),
-    (1, 1, 12): (2378.9, 147456.0), (1, 1, 16): (2378.5, 196608.0), (1, 2, 1): (2350.8, 24576.0),
-    (1, 2, 2): (2350.4, 49152.0), (1, 2, 3): (2351.2, 73728.0), (1, 2, 4): (2606.2, 98304.0),
-    (1, 2, 6): (2895.6, 147456.0), (1, 2, 8): (3009.0, 196608.0), (1, 2, 12): (3244.7, 294912.0),
-    (1, 2, 16): (3594.5, 393216.0), (1, 4, 1): (3046.9, 49152.0), (1, 4, 2): (3048.7, 98304.0),
-    (1, 4, 3): (3644.9, 147456.0), (1, 4, 4): (4094.3, 196608.0), (1, 4, 6): (4916.2, 294912.0),
-    (1, 4, 8): (5488.2, 393216.0), (1, 4, 12): (7419.7, 589824.0), (1, 4, 16): (8854.6, 786432.0),
+    (1, 1, 1): (3352.8, 12288.0), (1, 1, 2): (3352.9, 24576.0), (1, 1, 3): (3501.9, 36864.0),
+    (1, 1, 4): (3505.1, 49152.0), (1, 1, 6): (3471.1, 73728.0), (1, 1, 8): (3447.4, 98304.0),
+    (1, 1, 12): (3420.6, 147456.0), (1, 1, 16): (3457.1, 196608.0), (1, 2, 1): (3353.2, 24576.0),
+    (1, 2, 2): (3351.1, 49152.0), (1, 2, 3): (3410.5, 73728.0), (1, 2, 4): (3528.6, 98304.0),
+    (1, 2, 6): (3788.7, 147456.0), (1, 2, 8): (3852.8, 196608.0), (1, 2, 12): (4067.6, 294912.0),
+    (1, 2, 16): (4487.9, 393216.0), (1, 4, 1): (4415.7, 49152.0), (1, 4, 2): (4501.9, 98304.0),
+    (1, 4, 3): (4982.6, 147456.0), (1, 4, 4): (5384.8, 196608.0), (1, 4, 6): (6266.7, 294912.0),
+    (1, 4, 8): (7017.7, 393216.0), (1, 4, 12): (8868.6, 589824.0), (1, 4, 16): (10304.1, 786432.0),
 }
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 SWEEP_TRAFFIC_BYTES = {k: (f * FETCH_SCALE + w * WRITE_SCALE) * 1024.0
@@ -584,12 +584,14 @@ def run_kappa(args):
     q0 = next((q for q in range(len(gs)) if itb[q] < p4.roots.numiter), 0)
     guess, w0 = gs[q0], wb[q0:q0 + 1]
     n_s = args.kscan
-    s1 = (3, 1, (math.log10(1.0) - math.log10(KMAP["kperp"])) / (n_s - 1), 0.0, n_s - 1, 1)
-    s2 = (4, 1, (math.log10(1.0) - math.log10(KMAP["kpar"])) / (n_s - 1), 0.0, n_s - 1, 1)
-    t0 = time.perf_counter()
-    r = E.om_double_scan(s1, s2, [w0[0]])
-    t_scan = time.perf_counter() - t0
-    n_pts = int(np.sum(r["written"]))
+    r, t_scan, n_pts = None, 0.0, 0
+    if n_s >= 2:   # --kscan 0 or 1: the maps only (a profiler run of the maps)
+        s1 = (3, 1, (math.log10(1.0) - math.log10(KMAP["kperp"])) / (n_s - 1), 0.0, n_s - 1, 1)
+        s2 = (4, 1, (math.log10(1.0) - math.log10(KMAP["kpar"])) / (n_s - 1), 0.0, n_s - 1, 1)
+        t0 = time.perf_counter()
+        r = E.om_double_scan(s1, s2, [w0[0]])
+        t_scan = time.perf_counter() - t0
+        n_pts = int(np.sum(r["written"]))
     out = {
         "workload": "kappa core (90 %) + beam (10 %, 1.5 v_A) bi-kappa T_perp/T_par = 2, "
                     "electrons, 256 x 512; a 512 x 512 omega map per kappa in {3,4,5,6} at "
@@ -608,7 +610,7 @@ def run_kappa(args):
                            "kernel": "k_sweep, the four %dx%d map batches" % (nr, ni)},
         "scan_points": n_pts,
         "scan_s": t_scan,
-        "scan_roots_per_sec": n_pts / t_scan,
+        "scan_roots_per_sec": n_pts / t_scan if t_scan > 0 else None,
     }
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -639,7 +641,7 @@ def run_kappa(args):
         numiter, thr_d = p4.roots.numiter, p4.roots.D_threshold
         pts = [(KMAP["kperp"], KMAP["kpar"], guess, w0[0], int(itb[q0]))]
         prev = w0[0]
-        for j in (0, 1, 2):
+        for j in ((0, 1, 2) if r is not None else ()):
             pts.append((r["kperp"][0, j], r["kpar"][0, j], prev, r["roots"][0, j, 0], None))
             prev = r["roots"][0, j, 0]
         dmax, drel, agree, rows = 0.0, 0.0, 0, []
