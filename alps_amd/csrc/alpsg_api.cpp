@@ -47,7 +47,7 @@ hipError_t launch_sweep_req(hipStream_t st, int nch, int W, int RC, int K, const
                             bool reduce);
 hipError_t launch_resonant(hipStream_t st, int nch, int RB, const ProbDev *prob, int nspec, int nom,
                            const double2 *om, const int *active, double2 *chiB, double2 *lowB,
-                           bool general, const int *pidx = nullptr);
+                           int lgen, const int *pidx = nullptr);
 hipError_t launch_nhds(hipStream_t st, const ProbDev *prob, int nspec, int nom, const double2 *om,
                        const int *active, double2 *chiA, double2 *lowA, const int *pidx = nullptr);
 int nhds_tables(double z, int bm_nmax, double bessel_zero, std::vector<double2> &tab);
@@ -68,10 +68,10 @@ hipError_t launch_map_grid(hipStream_t st, int nr, int ni, double omi, double dr
 hipError_t launch_osc_round(hipStream_t st, int n, alpsg_osc::OscMachine *S, const double2 *det,
                             double2 *om, int *round_ctr, int *remaining);
 hipError_t launch_resonant_all(hipStream_t st, int nch, int RB, const ProbDev *prob, int nspec,
-                               int nom, const double2 *om, double2 *chiB, bool general);
+                               int nom, const double2 *om, double2 *chiB, int lgen);
 hipError_t launch_round(hipStream_t st, int nchr, int RB, int RC, const ProbDev *prob, int nspec,
                         int nom, double2 *om, double2 *chiA, double2 *chiB, double2 *det,
-                        const int *zidx, bool general, alpsg_osc::OscMachine *S, int *ctl);
+                        const int *zidx, int lgen, alpsg_osc::OscMachine *S, int *ctl);
 hipError_t launch_assemble_osc(hipStream_t st, const ProbDev *prob, int nspec, int nom, int W, int RB,
                                double2 *om, const double2 *chiA, const double2 *chiB, double2 *det,
                                alpsg_osc::OscMachine *S, int *round_ctr, int *remaining);
@@ -235,7 +235,9 @@ struct alpsg_ctx {
   std::vector<double> scan_ms;  // end of each k step of the last om_scan (ms from its start)
   std::vector<int> scan_iters;  // [step][root] iterations of its solves, -1 if not followed
   bool plain_roots = false;     // root searches step on the host, one disp launch per omega
-  bool any_general_fit = false;  // a species needs eval_fit in the Landau term
+  // Landau forms the species take: bit 0 a call of eval_fit, bit 1 the inlined
+  // kappa evaluation (species whose every fit is a kappa fit)
+  int lgen = 0;
   // closed-form (use_bM) species and their &bM_spec parameters
   int usebm[ALPSG_MAXSPEC] = {0};
   bool any_bm = false;
@@ -409,10 +411,10 @@ int disp_chain(alpsg_ctx *h, int n, const double2 *om_dev, const int *active_dev
   HIPCHK(hipEventRecord(h->ev_sweep, h->st));
   if (req)
     HIPCHK(launch_resonant_all(h->st, sh.nch_b, sh.rb, h->prob_d.p, ns, n, om_dev, B.chiB,
-                               h->any_general_fit));
+                               h->lgen));
   else
     HIPCHK(launch_resonant(h->st, sh.nch_b, sh.rb, h->prob_d.p, ns, n, om_dev, active_dev, B.chiB,
-                           want_low ? B.lowB : nullptr, h->any_general_fit));
+                           want_low ? B.lowB : nullptr, h->lgen));
   if (h->any_bm)
     HIPCHK(launch_nhds(h->st, h->prob_d.p, ns, n, om_dev, active_dev, B.chiA,
                        want_low ? B.lowA : nullptr));
@@ -612,7 +614,7 @@ int autotune(alpsg_ctx *h, int n, const double2 *om_host, int *shape_out) {
       int rc_ = time_launch(
           [&]() {
             return launch_resonant(h->st, nb, rb, h->prob_d.p, ns, n, h->om_d.p, nullptr,
-                                   h->chiB_d.p, nullptr, h->any_general_fit);
+                                   h->chiB_d.p, nullptr, h->lgen);
           },
           t);
       if (rc_) return rc_;
@@ -695,7 +697,7 @@ int graph_disp(alpsg_ctx *h, int n, const double *om, double *det) {
                            h->chiA_d.p, h->zidx.p, true);
     if (e == hipSuccess)
       e = launch_resonant_all(h->st, sh.nch_b, sh.rb, h->prob_d.p, ns, n, h->om_d.p, h->chiB_d.p,
-                              h->any_general_fit);
+                              h->lgen);
     if (e == hipSuccess)
       e = launch_assemble(h->st, h->prob_d.p, ns, n, REQ_TW / REQ_W, sh.rb, h->om_d.p, nullptr, h->chiA_d.p,
                           h->chiB_d.p, nullptr, nullptr, nullptr, 0, nullptr, h->det_d.p, nullptr,
@@ -875,7 +877,7 @@ int osc_graph_batch(alpsg_ctx *h, const alpsg_root_params *rp, int n, cplx *om, 
       if (req && REQ_W == 4 && REQ_NCH == 32 && REQ_K == 4) {
         // the whole round in two launches
         e = launch_round(h->st, sh.nch_b, sh.rb, REQ_TW / REQ_W, h->prob_d.p, ns, n, h->om_d.p,
-                         h->chiA_d.p, h->chiB_d.p, h->det_d.p, h->zidx.p, h->any_general_fit,
+                         h->chiA_d.p, h->chiB_d.p, h->det_d.p, h->zidx.p, h->lgen,
                          h->osc_d.p, h->oscctl_d.p);
         continue;
       }
@@ -890,7 +892,7 @@ int osc_graph_batch(alpsg_ctx *h, const alpsg_root_params *rp, int n, cplx *om, 
         // the secant step in one workgroup
         if (e == hipSuccess)
           e = launch_resonant_all(h->st, sh.nch_b, sh.rb, h->prob_d.p, ns, n, h->om_d.p,
-                                  h->chiB_d.p, h->any_general_fit);
+                                  h->chiB_d.p, h->lgen);
         if (e == hipSuccess)
           e = launch_assemble_osc(h->st, h->prob_d.p, ns, n, TW, sh.rb, h->om_d.p, h->chiA_d.p,
                                   h->chiB_d.p, h->det_d.p, h->osc_d.p, h->oscctl_d.p,
@@ -899,7 +901,7 @@ int osc_graph_batch(alpsg_ctx *h, const alpsg_root_params *rp, int n, cplx *om, 
       }
       if (e == hipSuccess)
         e = launch_resonant(h->st, sh.nch_b, sh.rb, h->prob_d.p, ns, n, h->om_d.p, nullptr,
-                            h->chiB_d.p, nullptr, h->any_general_fit);
+                            h->chiB_d.p, nullptr, h->lgen);
       if (e == hipSuccess)
         e = launch_assemble(h->st, h->prob_d.p, ns, n, TW, sh.rb, h->om_d.p, nullptr, h->chiA_d.p,
                             h->chiB_d.p, nullptr, nullptr, nullptr, 0, nullptr, h->det_d.p, nullptr,
@@ -1782,6 +1784,7 @@ int alpsg_create(const alpsg_plasma_desc *d, alpsg_handle *out) {
   // problem descriptor (per-k pointers filled by set_k)
   ProbDev &pd = h->prob;
   std::memset(&pd, 0, sizeof(pd));
+  h->lgen = 0;
   pd.nspec = nspec;
   pd.kperp_norm = h->kperp_norm;
   pd.P = P;
@@ -1906,7 +1909,15 @@ int alpsg_create(const alpsg_plasma_desc *d, alpsg_handle *out) {
         S.lz_c2 = c2;
       }
     }
-    if (!(S.acmethod == 1 && S.fit_uniform && S.nfits == 1)) h->any_general_fit = true;
+    if (S.acmethod == 1 && !S.fit_uniform && S.nfits >= 1) {
+      S.fit_kappa = 1;
+      for (int f = 0; f < S.nfits; ++f)
+        if (S.fit_type[f] != 2) S.fit_kappa = 0;
+    }
+    if (S.fit_kappa)
+      h->lgen |= 2;
+    else if (!(S.acmethod == 1 && S.fit_uniform && S.nfits == 1))
+      h->lgen |= 1;
     if (d->ACmethod[s] == 2) {
       S.poly_kind = d->poly_kind[s];
       S.poly_order = d->poly_order[s];

@@ -27,6 +27,7 @@ struct SpecDev {
   int fit_type[ALPSG_DMAXFITS];
   int poly_kind, poly_order, poly_logfit;
   int fit_uniform;    // every fit Maxwellian with pperp-independent ppar parameters
+  int fit_kappa;      // ACmethod 1 and every fit a kappa fit (fit_type 2)
   double q, m, dens;
   double dpperp;      // pp(2,2,1) - pp(1,2,1)
   double dppar;       // pp(2,2,2) - pp(2,1,2)  (integrate_res, determine_resonances)

@@ -1270,11 +1270,17 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// How the Landau pass evaluates the continuation of a species: by a call of
+// eval_fit (LM_GEN), from one exponential per p and the per-k sums (LM_UNI: one
+// Maxwellian fit with pperp-independent (u2, u3)), or by eval_fit_kappa inlined
+// (LM_KAPPA: every fit a kappa fit).
+enum { LM_GEN = 0, LM_UNI = 1, LM_KAPPA = 2 };
+
 // Landau sums L_a = sum_iperp h a(n,iperp) K(iperp) of this lane, and the
 // early-return flag of landau_integrate.  When every fit of the species is a
 // Maxwellian whose ppar parameters do not depend on pperp, exp(-u2 (p-u3)^2) is
 // the same for every iperp and is evaluated once (bit-identical values).
-template <int NCH, bool UNI>
+template <int NCH, int LM>
 __device__ void landau_part(const SpecDev &S, const ProbDev &Pd, int an, const RowClass &rc,
                             double omr, double omi, int c, cd L[3], int &zero) {
   const double *__restrict__ arow = S.atab + (size_t)an * (S.nperp + 1) * ATAB_STRIDE;
@@ -1284,7 +1290,7 @@ __device__ void landau_part(const SpecDev &S, const ProbDev &Pd, int an, const R
   const cd pp_ = mk(rc.pr + d, rc.pim), pm_ = mk(rc.pr - d, rc.pim);
   const double qk = S.q / fabs(Pd.kpar), km = Pd.kpar / S.m;
   // single Maxwellian fit with pperp-independent (u2, u3): one exponential per p
-  constexpr bool uni = UNI;
+  constexpr bool UNI = LM == LM_UNI;
   cd E0 = mk(0.0, 0.0), Ep = E0, Em = E0;
   if constexpr (UNI) {
     const double u2 = S.fitc[1], u3 = S.fitc[2];  // iperp = 0 entry
@@ -1318,28 +1324,42 @@ __device__ void landau_part(const SpecDev &S, const ProbDev &Pd, int an, const R
     }
   }
   auto F = [&](int ip, int which) -> cd {
-    if constexpr (!UNI) {
+    if constexpr (LM == LM_GEN) {
       return eval_fit(S, ip, which == 0 ? p : (which > 0 ? pp_ : pm_));
+    } else if constexpr (LM == LM_KAPPA) {
+      return eval_fit_kappa(S, ip, which == 0 ? p : (which > 0 ? pp_ : pm_));
     } else {
       const double c0 = S.fitc[(size_t)ip * FIT_STRIDE];
       return mk(0.0, 0.0) + c0 * (which == 0 ? E0 : (which > 0 ? Ep : Em));
     }
   };
   if constexpr (!UNI) {
-    // every continuation value is a call of eval_fit: each lane takes a
+    // every continuation value is an evaluation of the fit: each lane takes a
     // contiguous block of rows and carries f(iperp-1, p), f(iperp, p) from row
-    // to row, so a row costs f(iperp+1, p) and f(iperp, p +- dppar), three
-    // calls instead of four (the same values and the same row formulas)
+    // to row, so a row costs f(iperp, p +- dppar) and f(iperp+1, p), three
+    // evaluations instead of four (the same values and the same row
+    // formulas), made one at a time (unroll 1: their temporaries are not live
+    // together)
     const int blk = (nperp + 1 + NCH - 1) / NCH;
     const int r0 = c * blk, r1 = min(r0 + blk, nperp + 1) - 1;
     if (r0 > r1) return;
     cd fm = (r0 >= 1) ? F(r0 - 1, 0) : mk(0.0, 0.0);  // f(ip - 1, p)
     cd f0 = F(r0, 0);                                // f(ip, p)
+#pragma unroll 1
     for (int ip = r0; ip <= r1; ++ip) {
       double h;
-      cd dfperp, dfpar, fn = mk(0.0, 0.0);
-      if (ip < nperp) fn = F(ip + 1, 0);  // f(ip + 1, p)
-      const cd fpi = F(ip, 1), fpf = F(ip, -1);
+      cd dfperp, dfpar, fpi, fpf, fn = mk(0.0, 0.0);
+#pragma unroll 1
+      for (int w = 0; w < 3; ++w) {
+        if (w == 2 && ip >= nperp) break;  // f(nperp + 1, p) is not taken
+        const cd v = F(w == 2 ? ip + 1 : ip, w == 0 ? 1 : (w == 1 ? -1 : 0));
+        if (w == 0)
+          fpi = v;  // f(ip, p + dppar)
+        else if (w == 1)
+          fpf = v;  // f(ip, p - dppar)
+        else
+          fn = v;   // f(ip + 1, p)
+      }
       if (ip >= 1 && ip <= nperp - 1) {
         h = (ip == nperp - 1) ? 0.5 : 1.0;
         if (czero(fpi) || czero(fpf) || czero(fn)) zero = 1;
@@ -1394,9 +1414,10 @@ __device__ void landau_part(const SpecDev &S, const ProbDev &Pd, int an, const R
 }  // namespace
 
 // PART 0: principal-value terms; PART 1: Landau terms of species whose analytic
-// continuation is one Maxwellian with pperp-independent (u2, u3); PART 2: Landau
-// terms of the other species.  PV writes partial slots rb, the Landau launches
-// slots RB + rb of chiB / lowB (each species by exactly one of PART 1, 2).
+// continuation is one Maxwellian with pperp-independent (u2, u3); PART 3: Landau
+// terms of species whose every fit is a kappa fit; PART 2: Landau terms of the
+// other species.  PV writes partial slots rb, the Landau launches slots RB + rb
+// of chiB / lowB (each species by exactly one of PART 1, 2, 3).
 // One wavefront's share of k_resonant: omega block bx, species by, harmonic
 // block rb of RB; win_lds holds (64 / NCH) windows of 2 PV_WIN doubles.
 template <int NCH, int PART>
@@ -1421,7 +1442,8 @@ __device__ __forceinline__ void resonant_body(const ProbDev *__restrict__ probp,
   const int npar = S.npar, P = Pd.P;
   const double kpar = Pd.kpar;
   const bool uni_s = (S.fit_uniform != 0) && (S.nfits == 1) && (S.acmethod == 1);
-  if ((PART == 1 && !uni_s) || (PART == 2 && uni_s)) return;
+  const bool kap_s = S.fit_kappa != 0;
+  if ((PART == 1 && !uni_s) || (PART == 2 && (uni_s || kap_s)) || (PART == 3 && !kap_s)) return;
   double *win = win_lds + g * (2 * PV_WIN);
   cd res[6];
 #pragma unroll
@@ -1485,8 +1507,8 @@ __device__ __forceinline__ void resonant_body(const ProbDev *__restrict__ probp,
     if constexpr (PART != 0) {
       cd L[3] = {mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0)};
       int zero = 0;
-      landau_part<NCH, PART == 1>(S, Pd, an, rc, omr, omi, c,
-                       L, zero);
+      landau_part<NCH, PART == 1 ? LM_UNI : (PART == 3 ? LM_KAPPA : LM_GEN)>(S, Pd, an, rc, omr,
+                                                                             omi, c, L, zero);
       zero = gor<NCH>(zero);
       if (!zero) {
         const cd p = mk(rc.pr, rc.pim);
@@ -1530,9 +1552,10 @@ __global__ __launch_bounds__(64) void k_resonant(const ProbDev *__restrict__ pro
 
 // The principal-value and Landau parts in one launch (small batches, where a
 // launch costs more than the work): z < RB runs the principal-value part of
-// harmonic block z, z >= RB the Landau part of block z - RB (the part-1 or
-// part-2 form, whichever the species takes).
-template <int NCH, bool GEN>
+// harmonic block z, z >= RB the Landau part of block z - RB (the part-1, 2 or
+// 3 form, whichever the species takes).  GEN: bit 0 set if a species takes
+// part 2, bit 1 if one takes part 3.
+template <int NCH, int GEN>
 __global__ __launch_bounds__(64) void k_resonant_all(const ProbDev *__restrict__ probp, int nom,
                                                      const double2 *__restrict__ om_in,
                                                      double2 *__restrict__ chiB, int RB) {
@@ -1544,8 +1567,11 @@ __global__ __launch_bounds__(64) void k_resonant_all(const ProbDev *__restrict__
   }
   resonant_body<NCH, 1>(probp, nom, om_in, nullptr, chiB, nullptr, nullptr, RB, z - RB, bx, by, lane,
                         win);
-  if constexpr (GEN)
+  if constexpr ((GEN & 1) != 0)
     resonant_body<NCH, 2>(probp, nom, om_in, nullptr, chiB, nullptr, nullptr, RB, z - RB, bx, by,
+                          lane, win);
+  if constexpr ((GEN & 2) != 0)
+    resonant_body<NCH, 3>(probp, nom, om_in, nullptr, chiB, nullptr, nullptr, RB, z - RB, bx, by,
                           lane, win);
 }
 
@@ -1759,9 +1785,11 @@ hipError_t launch_sweep_req(hipStream_t st, int nch, int W, int RC, int K, const
   return hipErrorInvalidValue;
 }
 
+// lgen: bit 0 set if a species takes the part-2 Landau form, bit 1 if one
+// takes the part-3 (kappa) form
 hipError_t launch_resonant(hipStream_t st, int nch, int RB, const ProbDev *prob, int nspec, int nom,
                            const double2 *om, const int *active, double2 *chiB, double2 *lowB,
-                           bool general, const int *pidx) {
+                           int lgen, const int *pidx) {
   if (nom <= 0) return hipSuccess;
   if (pidx != nullptr && nch != 64) return hipErrorInvalidValue;
 #define RS(N)                                                                                  \
@@ -1771,8 +1799,11 @@ hipError_t launch_resonant(hipStream_t st, int nch, int RB, const ProbDev *prob,
                        om, active, chiB, lowB, pidx);                                          \
     hipLaunchKernelGGL((k_resonant<N, 1>), dim3(grid, nspec, RB), dim3(64), 0, st, prob, nom,  \
                        om, active, chiB, lowB, pidx);                                          \
-    if (general)                                                                               \
+    if (lgen & 1)                                                                              \
       hipLaunchKernelGGL((k_resonant<N, 2>), dim3(grid, nspec, RB), dim3(64), 0, st, prob, nom, \
+                         om, active, chiB, lowB, pidx);                                        \
+    if (lgen & 2)                                                                              \
+      hipLaunchKernelGGL((k_resonant<N, 3>), dim3(grid, nspec, RB), dim3(64), 0, st, prob, nom, \
                          om, active, chiB, lowB, pidx);                                        \
     return hipGetLastError();                                                                  \
   }
@@ -1963,7 +1994,7 @@ __global__ __launch_bounds__(256) void k_assemble_osc(
 // workgroups [0, nom RC) run the sweep (omega b % nom, workgroup b / nom of
 // RC), the others four resonant wavefronts each (every part and harmonic
 // block).
-template <int NCHS, int K, int NCHR, bool GEN>
+template <int NCHS, int K, int NCHR, int GEN>
 __global__ __launch_bounds__(256) void k_sweep_res(const ProbDev *__restrict__ probp, int nom,
                                                    const double2 *__restrict__ om,
                                                    double2 *__restrict__ chiA,
@@ -1991,8 +2022,11 @@ __global__ __launch_bounds__(256) void k_sweep_res(const ProbDev *__restrict__ p
   }
   resonant_body<NCHR, 1>(probp, nom, om, nullptr, chiB, nullptr, nullptr, RB, z - RB, x, y, lane,
                          win[wave]);
-  if constexpr (GEN)
+  if constexpr ((GEN & 1) != 0)
     resonant_body<NCHR, 2>(probp, nom, om, nullptr, chiB, nullptr, nullptr, RB, z - RB, x, y, lane,
+                           win[wave]);
+  if constexpr ((GEN & 2) != 0)
+    resonant_body<NCHR, 3>(probp, nom, om, nullptr, chiB, nullptr, nullptr, RB, z - RB, x, y, lane,
                            win[wave]);
 }
 
@@ -2005,24 +2039,25 @@ __global__ __launch_bounds__(256) void k_sweep_res(const ProbDev *__restrict__ p
 // ctl = {round, unfinished}
 hipError_t launch_round(hipStream_t st, int nchr, int RB, int RC, const ProbDev *prob, int nspec,
                         int nom, double2 *om, double2 *chiA, double2 *chiB, double2 *det,
-                        const int *zidx, bool general, alpsg_osc::OscMachine *S, int *ctl) {
+                        const int *zidx, int lgen, alpsg_osc::OscMachine *S, int *ctl) {
   if (nom <= 0) return hipSuccess;
   if (nom > 256 || 6 * nspec > 256) return hipErrorInvalidValue;
   bool ok = false;
+#define RDG(N, G)                                                                                  \
+  if (lgen == G)                                                                                   \
+    hipLaunchKernelGGL((k_sweep_res<32, 4, N, G>), dim3(grid), dim3(256), 0, st, prob, nom, om,    \
+                       chiA, chiB, zidx, RC, RB, nspec, ctl + 1);
 #define RD(N)                                                                                      \
   if (nchr == N) {                                                                                 \
     const int gx = (nom + (64 / N) - 1) / (64 / N);                                                \
     const int grid = nom * RC + (gx * nspec * 2 * RB + 3) / 4;                                     \
-    if (general)                                                                                   \
-      hipLaunchKernelGGL((k_sweep_res<32, 4, N, true>), dim3(grid), dim3(256), 0, st, prob, nom,   \
-                         om, chiA, chiB, zidx, RC, RB, nspec, ctl + 1);                            \
-    else                                                                                           \
-      hipLaunchKernelGGL((k_sweep_res<32, 4, N, false>), dim3(grid), dim3(256), 0, st, prob, nom,  \
-                         om, chiA, chiB, zidx, RC, RB, nspec, ctl + 1);                            \
+    RDG(N, 0) RDG(N, 1) RDG(N, 2) RDG(N, 3)                                                        \
     ok = true;                                                                                     \
   }
+  if (lgen < 0 || lgen > 3) return hipErrorInvalidValue;
   RD(4) RD(8) RD(16) RD(32) RD(64)
 #undef RD
+#undef RDG
   if (!ok) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_assemble_osc, dim3(1), dim3(256), 0, st, prob, nom, RC, RB, om, chiA, chiB,
                      det, nspec, S, om, ctl, ctl + 1);
@@ -2030,21 +2065,22 @@ hipError_t launch_round(hipStream_t st, int nchr, int RB, int RC, const ProbDev 
 }
 
 hipError_t launch_resonant_all(hipStream_t st, int nch, int RB, const ProbDev *prob, int nspec,
-                               int nom, const double2 *om, double2 *chiB, bool general) {
+                               int nom, const double2 *om, double2 *chiB, int lgen) {
   if (nom <= 0) return hipSuccess;
+  if (lgen < 0 || lgen > 3) return hipErrorInvalidValue;
+#define RAG(N, G)                                                                              \
+  if (lgen == G)                                                                               \
+    hipLaunchKernelGGL((k_resonant_all<N, G>), dim3(grid, nspec, 2 * RB), dim3(64), 0, st,     \
+                       prob, nom, om, chiB, RB);
 #define RA(N)                                                                                  \
   if (nch == N) {                                                                              \
     const int grid = (nom + (64 / N) - 1) / (64 / N);                                          \
-    if (general)                                                                               \
-      hipLaunchKernelGGL((k_resonant_all<N, true>), dim3(grid, nspec, 2 * RB), dim3(64), 0, st,  \
-                         prob, nom, om, chiB, RB);                                             \
-    else                                                                                       \
-      hipLaunchKernelGGL((k_resonant_all<N, false>), dim3(grid, nspec, 2 * RB), dim3(64), 0, st, \
-                         prob, nom, om, chiB, RB);                                             \
+    RAG(N, 0) RAG(N, 1) RAG(N, 2) RAG(N, 3)                                                    \
     return hipGetLastError();                                                                  \
   }
   RA(4) RA(8) RA(16) RA(32) RA(64)
 #undef RA
+#undef RAG
   return hipErrorInvalidValue;
 }
 

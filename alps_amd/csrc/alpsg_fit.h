@@ -63,4 +63,21 @@ __device__ __noinline__ cd eval_fit(const SpecDev &S, int iperp, cd p) {
   return f;
 }
 
+// eval_fit of a species whose every fit is a kappa fit (case 2 above, the same
+// operations), inlined into the Landau pass: the call of eval_fit holds that
+// kernel to one wavefront per SIMD, this form to two.  The fits are summed one
+// at a time (unroll 1), so that their temporaries are not live together.
+__device__ __forceinline__ cd eval_fit_kappa(const SpecDev &S, int iperp, cd p) {
+  cd f = mk(0.0, 0.0);
+#pragma unroll 1
+  for (int fi = 0; fi < S.nfits; ++fi) {
+    const double *c = S.fitc + ((size_t)iperp * S.nfits + fi) * FIT_STRIDE;
+    const cd dq = mk(p.re - c[2], p.im);
+    const cd sq = dq * dq;
+    const cd kp = mk(1.0 + c[1] * sq.re + c[4], c[1] * sq.im);
+    f = f + c[0] * cpowr(kp, c[3]);
+  }
+  return f;
+}
+
 }  // namespace

@@ -74,7 +74,7 @@ hipError_t launch_sweep_req(hipStream_t st, int nch, int W, int RC, int K, const
                             bool reduce);
 hipError_t launch_resonant(hipStream_t st, int nch, int RB, const ProbDev *prob, int nspec, int nom,
                            const double2 *om, const int *active, double2 *chiB, double2 *lowB,
-                           bool general, const int *pidx);
+                           int lgen, const int *pidx);
 hipError_t launch_assemble(hipStream_t st, const ProbDev *prob, int nspec, int nom, int W, int RB,
                            const double2 *om, const int *active, const double2 *chiA,
                            const double2 *chiB, const double2 *lowA, const double2 *lowB,
@@ -186,11 +186,13 @@ struct Plasma {
   int nspec = 0, nperp = 0, npar = 0, P = 3, NRI = 100, kperp_norm = 1;
   double vA = 0, Tlim = 0.01, Bessel_zero = 1e-45;
   std::vector<double> qs, ms, ns, pperp, ppar;
-  std::vector<int> acm, nfits, ftype, funi, porder, plog;
+  std::vector<int> acm, nfits, ftype, funi, fkap, porder, plog;
   std::vector<double> plogmax;
   std::vector<size_t> fit_off;
   int maxorder = 0;
-  bool general = false;  // a species needs the general Landau part
+  // Landau forms the species take: bit 0 a call of eval_fit, bit 1 the inlined
+  // kappa evaluation (species whose every fit is a kappa fit)
+  int lgen = 0;
   // closed-form (use_bM) species: the &bM_spec parameters (ALPS_NHDS.f90:163-168)
   std::vector<int> isbm, bm_cold, bm_nmax;
   std::vector<double> bm_Omega, bm_ell, bm_vth, bm_vd, bm_alpha, bm_bz;
@@ -446,12 +448,13 @@ int prepare_plasma(const alpsg_plasma_desc *d, Plasma &r) {
   r.nfits.assign(nspec, 0);
   r.ftype.assign((size_t)nspec * ALPSG_MAXFITS, 0);
   r.funi.assign(nspec, 0);
+  r.fkap.assign(nspec, 0);
   r.porder.assign(nspec, 0);
   r.plog.assign(nspec, 0);
   r.plogmax.assign(nspec, 0.0);
   r.fit_off.assign(nspec + 1, 0);
   r.maxorder = 0;
-  r.general = false;
+  r.lgen = 0;
   for (int s = 0; s < nspec; ++s) {
     if (r.isbm[s]) {  // no fit table (the closed form has its own continuation)
       r.fit_off[s + 1] = r.fit_off[s];
@@ -486,7 +489,14 @@ int prepare_plasma(const alpsg_plasma_desc *d, Plasma &r) {
             }
       }
       r.funi[s] = uni ? 1 : 0;
-      if (!(uni && nf == 1)) r.general = true;
+      bool kap = !uni && nf >= 1;
+      for (int f = 0; f < nf; ++f)
+        if (r.ftype[(size_t)s * ALPSG_MAXFITS + f] != 2) kap = false;
+      r.fkap[s] = kap ? 1 : 0;
+      if (kap)
+        r.lgen |= 2;
+      else if (!(uni && nf == 1))
+        r.lgen |= 1;
     } else if (r.acm[s] == 2) {
       if (!d->poly_kind || !d->poly_order || !d->logfit || !d->poly_log_max || !d->poly_fit_coeffs)
         return alpsg_internal_fail(ALPSG_E_ARG, "missing Chebyshev arrays");
@@ -496,12 +506,12 @@ int prepare_plasma(const alpsg_plasma_desc *d, Plasma &r) {
       r.plog[s] = d->logfit[s] ? 1 : 0;
       r.plogmax[s] = d->poly_log_max[s];
       r.maxorder = std::max(r.maxorder, r.porder[s]);
-      r.general = true;
+      r.lgen |= 1;
     } else if (r.acm[s] == 0) {
       // the pre-coded f0 of distribution_analyt (ALPS_analyt.f90:77-79)
       if (!distribution_analyt_defined(s + 1))
         return alpsg_internal_fail(ALPSG_E_UNSUPPORTED, "ACmethod 0: distribution_analyt defines no f0 for this species");
-      r.general = true;
+      r.lgen |= 1;
     } else {
       return alpsg_internal_fail(ALPSG_E_ARG, "ACmethod must be 0, 1 or 2");
     }
@@ -837,6 +847,7 @@ int admit_k(alpsg_server *S, Request &r, int &njobs, int &max_nperp, int &nnew) 
     sd.nfits = P.acm[s] == 1 ? P.nfits[s] : 0;
     for (int f = 0; f < sd.nfits; ++f) sd.fit_type[f] = P.ftype[(size_t)s * ALPSG_MAXFITS + f];
     sd.fit_uniform = P.funi[s];
+    sd.fit_kappa = P.fkap[s];
     sd.q = P.qs[s];
     sd.m = P.ms[s];
     sd.dens = P.ns[s];
@@ -1062,10 +1073,11 @@ int server_pass(alpsg_server *S) {
       hipEvent_t *ev = ng < alpsg_server::NEV ? S->ev[ng] : nullptr;
       ++ng;
       if (ev) SCHK(hipEventRecord(ev[0], S->st));
-      bool gen = false, bm = false;
+      int gen = 0;
+      bool bm = false;
       int relmax = -1;
       for (; i1 < nb && act[i1]->pl->nspec == ns; ++i1) {
-        if (act[i1]->pl->general) gen = true;
+        gen |= act[i1]->pl->lgen;
         if (act[i1]->pl->any_bm) bm = true;
         relmax = std::max(relmax, act[i1]->nmaxR);
       }

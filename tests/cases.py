@@ -278,6 +278,39 @@ def fit_core_beam(nperp=24, npar=48):
     return _replace_species0(p, f, [1, 1], pcs, [core, beam])
 
 
+def kappa_poly_mix(nperp=24, npar=48, order=8):
+    """Bi-kappa protons (every fit a kappa fit) beside electrons continued by a
+    Chebyshev fit of log10 f0 (ACmethod 2): the Landau pass takes the kappa
+    form for one species and the general form for the other, in one pass."""
+    import dataclasses
+    p = kappa_case(nperp, npar)
+    s = 1
+    pa = p.pp[s, 0, :, 1]
+    t = (pa - 0.5 * (pa[-1] + pa[0])) / (0.5 * (pa[-1] - pa[0]))
+    coeffs = np.zeros((p.nspec, nperp + 1, order + 1))
+    for i in range(nperp + 1):
+        coeffs[s, i] = np.polynomial.chebyshev.chebfit(t, np.log10(p.f0[s, i]), order)
+    return dataclasses.replace(p, ACmethod=np.array([1, 2], dtype=np.int32),
+                               poly_kind=np.array([0, 1]), poly_order=np.array([0, order]),
+                               logfit=np.array([1, 1]), poly_log_max=np.array([0.0, 18.0]),
+                               poly_fit_coeffs=coeffs)
+
+
+def kappa_two_fits(nperp=24, npar=48):
+    """Protons as a bi-kappa core plus a kappa beam drifting at 1.5, two
+    fit_type-2 fits of one species (f = u1 (1 + u2 (ppar - u3)^2 +
+    pc u5 pperp^2)^u4 each) at the table's own parameters."""
+    p = kappa_case(nperp, npar)
+    pperp = p.pp[0, :, 0, 0][:, None]
+    ppar = p.pp[0, 0, :, 1][None, :]
+    core = (0.17, 0.25, 0.0, -5.0, 1.0)
+    beam = (0.02, 0.5, 1.5, -4.0, 1.0)
+    pcs = (0.25, 0.5)
+    f = sum(u[0] * (1.0 + u[1] * (ppar - u[2]) ** 2 + pc * u[4] * pperp ** 2) ** u[3]
+            for u, pc in zip((core, beam), pcs))
+    return _replace_species0(p, f, [2, 2], pcs, [core, beam])
+
+
 def fit_moyal(nperp=24, npar=48):
     """Protons with a bi-Moyal table (fit_type 6) of known parameters."""
     p = small_bimax(nperp=nperp, npar=npar)
