@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -169,6 +170,10 @@ struct alpsg_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t ev_sweep = nullptr, ev_res = nullptr;  // recorded after k_sweep, after k_resonant
   hipEvent_t ev_done = nullptr;                     // end of a small disp call's work
+  // alpsg_map_search_refine: the map's D and log10|D| go to the host on their
+  // own stream (default priority) while the secant rounds run on st
+  hipStream_t st_copy = nullptr;
+  hipEvent_t ev_map = nullptr, ev_copy = nullptr;
   int nspec = 0, nperp = 0, npar = 0;
   int kperp_norm = 1, P = 5, NRI = 100;
   double vA = 0, Tlim = 0.01, Bessel_zero = 1e-45;
@@ -321,6 +326,9 @@ struct alpsg_ctx {
     if (ev_sweep) (void)hipEventDestroy(ev_sweep);
     if (ev_res) (void)hipEventDestroy(ev_res);
     if (ev_done) (void)hipEventDestroy(ev_done);
+    if (ev_map) (void)hipEventDestroy(ev_map);
+    if (ev_copy) (void)hipEventDestroy(ev_copy);
+    if (st_copy) (void)hipStreamDestroy(st_copy);
     if (st) (void)hipStreamDestroy(st);
   }
 };
@@ -814,8 +822,10 @@ const int OSC_MAX_N = 1024;  // solves of one device-round batch (one workgroup 
 // (its value is then ignored), so the batch and the launch shape stay fixed.
 // Returns -1 when the batch cannot run this way (the caller then steps the
 // solves on the host).
+// work (may be NULL): host work of the caller, run once while the first rounds
+// run on the device (then cleared)
 int osc_graph_batch(alpsg_ctx *h, const alpsg_root_params *rp, int n, cplx *om, cplx *Dfin,
-                    int *iters) {
+                    int *iters, std::function<int()> *work) {
   if (!h->graphs_ok || !h->osc_graphs_ok || h->any_bm || h->any_rel || n > OSC_MAX_N) return -1;
   const int ns = h->nspec;
   LaunchShape sh = shape_for(h, n);
@@ -937,6 +947,15 @@ int osc_graph_batch(alpsg_ctx *h, const alpsg_root_params *rp, int n, cplx *om, 
   for (int rep = 0; rep < maxrep; ++rep) {
     HIPCHK(hipGraphLaunch(g->exec, h->st));
     HIPCHK(hipEventRecord(h->ev_done, h->st));
+    if (work && *work) {  // runs while the rounds just launched run on the device
+      std::function<int()> w;
+      w.swap(*work);
+      const int rc = w();
+      if (rc) {
+        (void)hipStreamSynchronize(h->st);
+        return rc;
+      }
+    }
     hipError_t q;
     while ((q = hipEventQuery(h->ev_done)) == hipErrorNotReady) {
     }
@@ -973,9 +992,9 @@ int osc_graph_batch(alpsg_ctx *h, const alpsg_root_params *rp, int n, cplx *om, 
 // secant_osc (ALPS_fns.f90:1919-2101) for many independent roots: each root is
 // a state machine (alpsg_osc.h) whose disp() requests are served in batches.
 int secant_osc_batch(alpsg_ctx *h, const alpsg_root_params *rp, int nroots, cplx *om, cplx *Dfin,
-                     int *iters) {
+                     int *iters, std::function<int()> *work = nullptr) {
   if (!h->plain_roots) {
-    const int rc = osc_graph_batch(h, rp, nroots, om, Dfin, iters);
+    const int rc = osc_graph_batch(h, rp, nroots, om, Dfin, iters, work);
     if (rc != -1) return rc;
   }
   std::vector<alpsg_osc::OscMachine> S(nroots);
@@ -1216,8 +1235,10 @@ int rtsec_batch(alpsg_ctx *h, const alpsg_root_params *rp, int nroots, cplx *om,
 // The root solver the &system namelist selects (secant_method, as in
 // om_scan, ALPS_fns.f90:2507-2514, and refine_guess, ALPS_fns.f90:3830-3843),
 // then the "final instance of disp" at each root when Dfin is given.
+// work (may be NULL): host work the device-stepped secant_osc runs while its
+// first rounds run; left set for the caller otherwise
 int root_batch(alpsg_ctx *h, const alpsg_root_params *rp, int nroots, cplx *om, cplx *Dfin,
-               int *iters) {
+               int *iters, std::function<int()> *work = nullptr) {
   int rc;
   switch (rp->secant_method) {
     case 0:
@@ -1227,7 +1248,7 @@ int root_batch(alpsg_ctx *h, const alpsg_root_params *rp, int nroots, cplx *om, 
       rc = rtsec_batch(h, rp, nroots, om, iters);
       break;
     case 2:
-      return secant_osc_batch(h, rp, nroots, om, Dfin, iters);
+      return secant_osc_batch(h, rp, nroots, om, Dfin, iters, work);
     default:
       return fail(ALPSG_E_ARG, "secant_method must be 0, 1 or 2");
   }
@@ -1623,6 +1644,9 @@ int alpsg_create(const alpsg_plasma_desc *d, alpsg_handle *out) {
   HIPCHK(hipEventCreate(&h->ev_sweep));
   HIPCHK(hipEventCreate(&h->ev_res));
   HIPCHK(hipEventCreateWithFlags(&h->ev_done, hipEventDisableTiming));
+  HIPCHK(hipStreamCreateWithFlags(&h->st_copy, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&h->ev_map, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&h->ev_copy, hipEventDisableTiming));
   const size_t ngrid = (size_t)nspec * (nperp + 1) * (npar + 1);
   HIPCHK(h->pperp.ensure(h->pperp_h.size()));
   HIPCHK(h->ppar.ensure(h->ppar_h.size()));
@@ -2258,15 +2282,40 @@ int alpsg_secant_batch(alpsg_handle h, const alpsg_root_params *rp, int nroots, 
   ALPSG_CATCH
 }
 
-int alpsg_map_search(alpsg_handle h, double omi, double omf, double gami, double gamf, int nr,
-                     int ni, int loggridw, int loggridg, double *om_out, double *cal_out,
-                     double *val_out, int numroots, int *iroots, int *nfound) {
-  ALPSG_TRY
-  if (!h) return fail(ALPSG_E_ARG, "null handle");
-  if (nr < 1 || ni < 1) return fail(ALPSG_E_ARG, "nr, ni must be >= 1");
+namespace {
+
+// omega of map point (ir, ii), 1-based (ALPS_fns.f90:3650-3700): the products
+// and sums of the grid that alpsg_map_search fills
+cplx map_point(double omi, double omf, double gami, double gamf, int nr, int ni, int loggridw,
+               int loggridg, int ir, int ii) {
+#pragma clang fp contract(off)
+  double dr = omf - omi, di = gamf - gami;
+  if (nr > 1) dr = (omf - omi) / (1.0 * (nr - 1));
+  if (ni > 1) di = (gamf - gami) / (1.0 * (ni - 1));
+  double wr, wi;
+  if (loggridw) {
+    wr = omi;
+    if (nr > 1) wr = omi * std::pow(omf / omi, (1.0 * (ir - 1)) / (1.0 * (nr - 1)));
+  } else {
+    wr = omi + dr * (1.0 * (ir - 1));
+  }
+  if (loggridg) {
+    wi = gami;
+    if (ni > 1) wi = gami * std::pow(gamf / gami, (1.0 * (ii - 1)) / (1.0 * (ni - 1)));
+  } else {
+    wi = gami + di * (1.0 * (ii - 1));
+  }
+  return cplx(wr, wi);
+}
+
+// map_search's grid, det D, sentinels and minima.  With late != NULL the
+// host waits only for the minimum flags: D and log10|D| go to pinned staging
+// on st_copy, and *late is set to the host work that copies them into cal_out
+// / val_out (the caller runs it before it returns).
+int map_search_core(alpsg_ctx *h, double omi, double omf, double gami, double gamf, int nr, int ni,
+                    int loggridw, int loggridg, double *om_out, double *cal_out, double *val_out,
+                    int numroots, int *iroots, int *nfound, std::function<int()> *late) {
   const size_t N = (size_t)nr * ni;
-  if (!h->have_k) return fail(ALPSG_E_STATE, "alpsg_set_k has not been called");
-  if (N > (size_t)INT32_MAX) return fail(ALPSG_E_ARG, "map too large");
   // pinned staging: grid [N] complex, D [N] complex, log10|D| [N], flags [N]
   HIPCHK(h->map_pin.ensure(N * (sizeof(cplx) * 2 + sizeof(double) + 1)));
   cplx *om = reinterpret_cast<cplx *>(h->map_pin.p);
@@ -2322,10 +2371,18 @@ int alpsg_map_search(alpsg_handle h, double omi, double omf, double gami, double
   HIPCHK(h->mhit_d.ensure(N));
   HIPCHK(launch_map_post(h->st, nr, ni, h->det_d.p, h->mcal_d.p, h->mval_d.p, h->mhit_d.p));
   HIPCHK(hipMemcpyAsync(hit, h->mhit_d.p, N, hipMemcpyDeviceToHost, h->st));
+  const bool defer = late != nullptr && (cal_out || val_out);
+  hipStream_t cs = h->st;
+  if (defer) {
+    HIPCHK(hipEventRecord(h->ev_map, h->st));
+    HIPCHK(hipStreamWaitEvent(h->st_copy, h->ev_map, 0));
+    cs = h->st_copy;
+  }
   if (cal_out)
-    HIPCHK(hipMemcpyAsync(cal_h, h->mcal_d.p, sizeof(cplx) * N, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipMemcpyAsync(cal_h, h->mcal_d.p, sizeof(cplx) * N, hipMemcpyDeviceToHost, cs));
   if (val_out)
-    HIPCHK(hipMemcpyAsync(val_h, h->mval_d.p, sizeof(double) * N, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipMemcpyAsync(val_h, h->mval_d.p, sizeof(double) * N, hipMemcpyDeviceToHost, cs));
+  if (defer) HIPCHK(hipEventRecord(h->ev_copy, h->st_copy));
   int relerr = 0;
   if (h->any_rel)
     HIPCHK(hipMemcpyAsync(&relerr, h->err_d.p, sizeof(int), hipMemcpyDeviceToHost, h->st));
@@ -2336,9 +2393,23 @@ int alpsg_map_search(alpsg_handle h, double omi, double omf, double gami, double
       std::memcpy(om_out, om, sizeof(cplx) * N);
   }
   HIPCHK(hipStreamSynchronize(h->st));
-  if (cal_out) std::memcpy(cal_out, cal_h, sizeof(cplx) * N);
-  if (val_out) std::memcpy(val_out, val_h, sizeof(double) * N);
+  if (defer) {
+    *late = [h, cal_out, val_out, cal_h, val_h, N]() -> int {
+      HIPCHK(hipEventSynchronize(h->ev_copy));
+      if (cal_out) std::memcpy(cal_out, cal_h, sizeof(cplx) * N);
+      if (val_out) std::memcpy(val_out, val_h, sizeof(double) * N);
+      return ALPSG_OK;
+    };
+  } else {
+    if (cal_out) std::memcpy(cal_out, cal_h, sizeof(cplx) * N);
+    if (val_out) std::memcpy(val_out, val_h, sizeof(double) * N);
+  }
   if (relerr) {
+    if (defer) {
+      std::function<int()> w;
+      w.swap(*late);
+      (void)w();
+    }
     const int zero = 0;
     HIPCHK(hipMemcpy(h->err_d.p, &zero, sizeof(int), hipMemcpyHostToDevice));
     return fail(ALPSG_E_ARG,
@@ -2370,6 +2441,79 @@ int alpsg_map_search(alpsg_handle h, double omi, double omf, double gami, double
     }
   }
   if (nfound) *nfound = n;
+  return ALPSG_OK;
+}
+
+// runs host work left by a deferred map search, if any
+int finish_host_work(std::function<int()> &work) {
+  if (!work) return ALPSG_OK;
+  std::function<int()> w;
+  w.swap(work);
+  return w();
+}
+
+}  // namespace
+
+int alpsg_map_search(alpsg_handle h, double omi, double omf, double gami, double gamf, int nr,
+                     int ni, int loggridw, int loggridg, double *om_out, double *cal_out,
+                     double *val_out, int numroots, int *iroots, int *nfound) {
+  ALPSG_TRY
+  if (!h) return fail(ALPSG_E_ARG, "null handle");
+  if (nr < 1 || ni < 1) return fail(ALPSG_E_ARG, "nr, ni must be >= 1");
+  if (!h->have_k) return fail(ALPSG_E_STATE, "alpsg_set_k has not been called");
+  if ((size_t)nr * ni > (size_t)INT32_MAX) return fail(ALPSG_E_ARG, "map too large");
+  return map_search_core(h, omi, omf, gami, gamf, nr, ni, loggridw, loggridg, om_out, cal_out,
+                         val_out, numroots, iroots, nfound, nullptr);
+  ALPSG_CATCH
+}
+
+int alpsg_map_search_refine(alpsg_handle h, const alpsg_root_params *rp, double omi, double omf,
+                            double gami, double gamf, int nr, int ni, int loggridw, int loggridg,
+                            double *om_out, double *cal_out, double *val_out, int numroots,
+                            int *iroots, int *nfound, double *roots, double *D_final, int *iters) {
+  ALPSG_TRY
+  if (!h || !rp) return fail(ALPSG_E_ARG, "null argument");
+  if (nr < 1 || ni < 1) return fail(ALPSG_E_ARG, "nr, ni must be >= 1");
+  if (numroots < 0) return fail(ALPSG_E_ARG, "numroots must be >= 0");
+  if (numroots > 0 && !roots) return fail(ALPSG_E_ARG, "null roots");
+  if (!h->have_k) return fail(ALPSG_E_STATE, "alpsg_set_k has not been called");
+  if ((size_t)nr * ni > (size_t)INT32_MAX) return fail(ALPSG_E_ARG, "map too large");
+  if (rp->secant_method < 0 || rp->secant_method > 2)
+    return fail(ALPSG_E_ARG, "secant_method must be 0, 1 or 2");
+  const auto t0 = std::chrono::steady_clock::now();
+  std::vector<int> ir(2 * (size_t)numroots + 2);
+  int nf = 0;
+  std::function<int()> late;
+  int rc = map_search_core(h, omi, omf, gami, gamf, nr, ni, loggridw, loggridg, om_out, cal_out,
+                           val_out, numroots, ir.data(), &nf, &late);
+  if (rc) return rc;
+  // the kernel times this call reports are the map batch's
+  const double map_ms = h->last_ms, map_split[3] = {h->last_split[0], h->last_split[1],
+                                                    h->last_split[2]};
+  if (iroots)
+    for (int i = 0; i < 2 * std::min(nf, numroots); ++i) iroots[i] = ir[i];
+  if (nfound) *nfound = nf;
+  // refine_guess (ALPS_fns.f90:3793-3856) on the recorded minima
+  const int n = std::min(nf, numroots);
+  cplx *w = reinterpret_cast<cplx *>(roots);
+  for (int r = 0; r < n; ++r)
+    w[r] = map_point(omi, omf, gami, gamf, nr, ni, loggridw, loggridg, ir[2 * r], ir[2 * r + 1]);
+  // the solvers that step on the host take the grids first; the device-stepped
+  // secant_osc copies them while its first rounds run
+  if (rp->secant_method != 2 || h->plain_roots || n == 0) {
+    rc = finish_host_work(late);
+    if (rc) return rc;
+  }
+  const double t_ref = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (n > 0) {
+    rc = root_batch(h, rp, n, w, reinterpret_cast<cplx *>(D_final), iters, &late);
+    const int rc2 = finish_host_work(late);
+    if (rc) return rc;
+    if (rc2) return rc2;
+    for (auto &t : h->root_ms) t += t_ref;
+  }
+  h->last_ms = map_ms;
+  for (int k = 0; k < 3; ++k) h->last_split[k] = map_split[k];
   return ALPSG_OK;
   ALPSG_CATCH
 }

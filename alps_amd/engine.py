@@ -47,7 +47,7 @@ EXPORTS = ["alpsg_last_error", "alpsg_create", "alpsg_destroy", "alpsg_set_k",
            "alpsg_server_submit", "alpsg_server_poll", "alpsg_server_stats",
            "alpsg_server_destroy", "alpsg_determine_param_fit", "alpsg_last_scan_steps",
            "alpsg_set_plain_roots", "alpsg_disp_batch_noalloc", "alpsg_server_work",
-           "alpsg_set_sweep_series", "alpsg_sweep_work"]
+           "alpsg_set_sweep_series", "alpsg_sweep_work", "alpsg_map_search_refine"]
 
 
 class AlpsgError(RuntimeError):
@@ -117,6 +117,10 @@ def load_library(path: str = LIB_PATH):
     L.alpsg_secant_batch.argtypes = [vp, C.POINTER(_RootParams), C.c_int, dp, dp, ip]
     L.alpsg_map_search.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
                                    C.c_int, C.c_int, C.c_int, dp, dp, dp, C.c_int, ip, ip]
+    L.alpsg_map_search_refine.argtypes = [vp, C.POINTER(_RootParams), C.c_double, C.c_double,
+                                          C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                                          C.c_int, dp, dp, dp, C.c_int, ip, ip, dp, dp, ip]
+    L.alpsg_map_search_refine.restype = C.c_int
     L.alpsg_om_scan.argtypes = [vp, C.POINTER(_RootParams), C.c_int, C.c_int, C.c_double,
                                 C.c_double, C.c_int, C.c_int, C.c_int, dp, dp, dp, dp]
     L.alpsg_om_double_scan.argtypes = [vp, C.POINTER(_RootParams), C.POINTER(_ScanParams),
@@ -417,7 +421,8 @@ class Engine:
         return tuple(int(x) for x in out)
 
     def last_root_times_ms(self, nroots: int):
-        """Completion time of each root of the last secant batch (ms from its start)."""
+        """Completion time of each root of the last secant batch (ms from its
+        start; after map_search_refine, from the start of that call)."""
         out = np.zeros(nroots)
         _check(load_library().alpsg_last_root_times(self._h, nroots, _dp(out)))
         return out
@@ -499,6 +504,40 @@ class Engine:
         n = int(nf[0])
         f = lambda a: a.reshape(ni, nr).T  # Fortran order (ir fastest)
         return f(om), f(cal), f(val), iroots.reshape(numroots, 2)[: min(n, numroots)], n
+
+    def map_search_refine(self, omi, omf, gami, gamf, nr, ni, loggridw=False, loggridg=False,
+                          numroots=100, out=None, roots: RootParams | None = None):
+        """map_search with refine_guess on its minima (ALPS_fns.f90:3595-3788,
+        3793-3856) in one call: as map_search, plus the refined roots of the
+        recorded minima, D at them and their iteration counts.  Returns om,
+        cal, val, iroots, nfound, roots, D_final, iters."""
+        r = roots or self.plasma.roots
+        N = nr * ni
+        if out is None:
+            om = np.zeros(N, dtype=np.complex128)
+            cal = np.zeros(N, dtype=np.complex128)
+            val = np.zeros(N)
+        else:
+            om, cal, val = out
+            for a, dt in ((om, np.complex128), (cal, np.complex128), (val, np.float64)):
+                if a.dtype != dt or a.size != N or not a.flags.c_contiguous:
+                    raise ValueError("map_search_refine: out arrays must be contiguous with "
+                                     "nr*ni elements of complex128, complex128, float64")
+        iroots = np.zeros(2 * numroots, dtype=np.int32)
+        nf = np.zeros(1, dtype=np.int32)
+        w = np.zeros(numroots, dtype=np.complex128)
+        dfin = np.zeros(numroots, dtype=np.complex128)
+        it = np.zeros(numroots, dtype=np.int32)
+        rp = _rp(r)
+        _check(load_library().alpsg_map_search_refine(
+            self._h, C.byref(rp), omi, omf, gami, gamf, nr, ni, int(loggridw), int(loggridg),
+            _dp(om), _dp(cal), _dp(val), numroots, _ip(iroots), _ip(nf), _dp(w), _dp(dfin),
+            _ip(it)))
+        n = int(nf[0])
+        k = min(n, numroots)
+        f = lambda a: a.reshape(ni, nr).T  # Fortran order (ir fastest)
+        return (f(om), f(cal), f(val), iroots.reshape(numroots, 2)[:k], n, w[:k], dfin[:k],
+                it[:k])
 
     def om_double_scan(self, s1, s2, roots, eigen=False, heat=False,
                        rparams: RootParams | None = None):

@@ -63,6 +63,7 @@ module alpsg
   public :: alpsg_server_submit_k, alpsg_server_submit_batch, alpsg_server_poll
   public :: alpsg_server_destroy, alpsg_determine_param_fit
   public :: alpsg_om_double_scan, alpsg_server_stats, alpsg_last_scan_steps, alpsg_set_plain_roots
+  public :: alpsg_map_search_refine
 
   interface
      function alpsg_create(desc, h) bind(C, name='alpsg_create') result(rc)
@@ -130,6 +131,26 @@ module alpsg
        integer(c_int), intent(out) :: nfound
        integer(c_int) :: rc
      end function alpsg_map_search
+
+     ! map_search with refine_guess on its minima (ALPS_fns.f90:3595-3788,
+     ! 3793-3856): roots, D_final, iters of the first min(nfound, numroots)
+     ! minima
+     function alpsg_map_search_refine(h, rp, omi, omf, gami, gamf, nr, ni, loggridw, loggridg, &
+          om_out, cal_out, val_out, numroots, iroots, nfound, roots, D_final, iters) &
+          bind(C, name='alpsg_map_search_refine') result(rc)
+       import :: c_int, c_ptr, c_double, c_double_complex, alpsg_root_params
+       type(c_ptr), value :: h
+       type(alpsg_root_params), intent(in) :: rp
+       real(c_double), value :: omi, omf, gami, gamf
+       integer(c_int), value :: nr, ni, loggridw, loggridg, numroots
+       type(c_ptr), value :: om_out, cal_out, val_out
+       integer(c_int), intent(out) :: iroots(2, *)
+       integer(c_int), intent(out) :: nfound
+       complex(c_double_complex), intent(out) :: roots(*)
+       complex(c_double_complex), intent(out) :: D_final(*)
+       integer(c_int), intent(out) :: iters(*)
+       integer(c_int) :: rc
+     end function alpsg_map_search_refine
 
      function alpsg_om_scan(h, rp, scan_type, log_scan, diff, diff2, n_out, n_res, nroots, roots, &
           kperp_out, kpar_out, roots_out) bind(C, name='alpsg_om_scan') result(rc)

@@ -149,20 +149,17 @@ def cpu_model():
 
 
 def run_step(E, nr, ni, numroots=100, out=None):
+    """One step of the map workload: map_search with refine_guess on every
+    minimum (ALPS_fns.f90:3595-3856) in one call; root latencies count from
+    the step's start."""
     t0 = time.perf_counter()
-    om, cal, val, iroots, nfound = E.map_search(MAP["omi"], MAP["omf"], MAP["gami"], MAP["gamf"],
-                                                nr, ni, numroots=numroots, out=out)
-    map_kernel_ms = E.last_kernel_ms
-    split = E.last_kernel_split_ms
-    t_map = (time.perf_counter() - t0) * 1e3
+    om, cal, val, iroots, nfound, roots, dfin, iters = E.map_search_refine(
+        MAP["omi"], MAP["omf"], MAP["gami"], MAP["gamf"], nr, ni, numroots=numroots, out=out)
     guesses = np.array([om[i - 1, j - 1] for i, j in iroots])
-    if guesses.size:
-        roots, dfin, iters = E.solve_batch(guesses)
-    else:
-        roots, dfin, iters = guesses, guesses, np.zeros(0, dtype=np.int32)
-    lat = t_map + (E.last_root_times_ms(guesses.size) if guesses.size else np.zeros(0))
+    lat = (E.last_root_times_ms(guesses.size) if guesses.size else np.zeros(0))
     return dict(roots=roots, dfin=dfin, iters=iters, guesses=guesses, latency_ms=lat,
-                map_kernel_ms=map_kernel_ms, split_ms=split, cal=cal, om=om, nfound=nfound)
+                map_kernel_ms=E.last_kernel_ms, split_ms=E.last_kernel_split_ms, cal=cal, om=om,
+                nfound=nfound, wall_ms=(time.perf_counter() - t0) * 1e3)
 
 
 def oracle_roots(plasma, kperp, kpar, guesses, threads):

@@ -180,6 +180,23 @@ int alpsg_map_search(alpsg_handle h, double omi, double omf, double gami, double
                      int ni, int loggridw, int loggridg, double *om_out, double *cal_out,
                      double *val_out, int numroots, int *iroots, int *nfound);
 
+/* map_search with its refinement, the whole of the reference's map_search
+   (ALPS_fns.f90:3595-3788: the map, find_minima, then refine_guess at :3781,
+   :3793-3856) in one call: refine_guess's solver (rp->secant_method) runs on
+   the first min(nfound, numroots) minima in find_minima's order, starting on
+   the device as soon as the minimum flags reach the host, while D and
+   log10|D| travel to the host beside the secant rounds.  Arguments as
+   alpsg_map_search (iroots, nfound may be NULL); roots [2*numroots] gets the
+   refined roots, D_final [2*numroots] and iters [numroots] (both optional)
+   as alpsg_secant_batch.  The results equal alpsg_map_search followed by
+   alpsg_secant_batch on the minima's grid frequencies; alpsg_last_root_times
+   then counts from the start of this call, and alpsg_last_kernel_ms /
+   alpsg_last_kernel_split report the map's batch. */
+int alpsg_map_search_refine(alpsg_handle h, const alpsg_root_params *rp, double omi, double omf,
+                            double gami, double gamf, int nr, int ni, int loggridw, int loggridg,
+                            double *om_out, double *cal_out, double *val_out, int numroots,
+                            int *iroots, int *nfound, double *roots, double *D_final, int *iters);
+
 /* om_scan (ALPS_fns.f90:2198-2600) without eigenfunction/heating output:
    follows nroots roots along the wavevector path; diff/diff2 are the step
    sizes computed by scan_read (ALPS_io.f90:482-548).  roots [2*nroots] in/out.
@@ -246,8 +263,9 @@ int alpsg_get_shape(alpsg_handle h, int n, int *shape_out);
 int alpsg_get_nmax(alpsg_handle h, int *nmax_out);
 long long alpsg_eval_count(alpsg_handle h);
 
-/* Wall time (ms, from the start of the last alpsg_secant_batch on this handle)
-   at which each of its nroots roots finished its secant iterations. */
+/* Wall time (ms, from the start of the last alpsg_secant_batch or
+   alpsg_map_search_refine on this handle) at which each of its nroots roots
+   finished its secant iterations. */
 int alpsg_last_root_times(alpsg_handle h, int nroots, double *ms_out);
 
 /* The last alpsg_om_scan on this handle, per wavevector step (nsteps =

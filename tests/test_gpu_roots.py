@@ -102,6 +102,54 @@ def test_refine_after_map_matches_oracle():
         assert abs(r - ro) <= ROOT_TOL * abs(ro)
 
 
+@pytest.mark.parametrize("method", [2, 0, 1])
+def test_map_search_refine_equals_map_then_batch(method):
+    """alpsg_map_search_refine (the reference's map_search with its
+    refine_guess call) equals alpsg_map_search followed by the batched solver
+    on the minima's grid frequencies, bit for bit: grids, minima, roots, D at
+    the roots and iteration counts; roots to the oracle's at 1e-10.  Also with
+    fewer recorded minima than found, and twice in a row into the same arrays
+    (the device-stepped solver copies the grids while its rounds run)."""
+    import dataclasses
+    p = cases.config1(64, 128)
+    p = dataclasses.replace(p, roots=dataclasses.replace(p.roots, secant_method=method,
+                                                         D_threshold=1e-14))
+    E = _engine(p)
+    E.set_k(0.5, 0.5)
+    args = (0.01, 2.0, -0.5, 0.05, 48, 40)
+    om, cal, val, ir, n = E.map_search(*args, numroots=6)
+    assert n >= 2
+    k = min(n, 6)
+    g = np.array([om[i - 1, j - 1] for i, j in ir])
+    r_b, d_b, it_b = E.solve_batch(g)
+    N = args[4] * args[5]
+    out = (np.zeros(N, dtype=np.complex128), np.zeros(N, dtype=np.complex128), np.zeros(N))
+    for rep in range(2):
+        om2, cal2, val2, ir2, n2, r2, d2, it2 = E.map_search_refine(*args, numroots=6, out=out)
+        assert n2 == n and np.array_equal(ir2, ir)
+        assert np.array_equal(om2, om) and np.array_equal(cal2, cal) and np.array_equal(val2, val)
+        assert np.array_equal(r2, r_b) and np.array_equal(d2, d_b) and np.array_equal(it2, it_b)
+        t = E.last_root_times_ms(k)
+        assert np.all(t > 0.0) and np.all(np.isfinite(t))
+    # fewer recorded minima than found
+    _, _, _, ir3, n3, r3, _, _ = E.map_search_refine(*args, numroots=1)
+    assert n3 == n and ir3.shape == (1, 2) and r3[0] == r_b[0]
+    E.close()
+    if method == 2:
+        # against the oracle on two minima away from the map's Re w = 0.01
+        # edge (a minimum there walks into the flat |D| region near w = 0,
+        # where the secant's end point is ill-conditioned)
+        O = Oracle(p)
+        O.set_k(0.5, 0.5)
+        pick = [q for q in range(k) if g[q].real > 0.05][:2]
+        assert pick
+        for q in pick:
+            # (iteration counts are not compared: at D_threshold 1e-14 |D|
+            # reaches the threshold within the rounding of det D)
+            ro, _ = O.secant_osc(g[q])
+            assert abs(r_b[q] - ro) <= ROOT_TOL * abs(ro), (q, r_b[q], ro)
+
+
 @pytest.mark.slow
 def test_kpar_fast_scan_on_gpu_matches_golden_and_oracle():
     gold = np.loadtxt(GOLD)
