@@ -2282,6 +2282,8 @@ int alpsg_secant_batch(alpsg_handle h, const alpsg_root_params *rp, int nroots, 
   ALPSG_CATCH
 }
 
+}  // extern "C"
+
 namespace {
 
 // omega of map point (ir, ii), 1-based (ALPS_fns.f90:3650-3700): the products
@@ -2453,6 +2455,8 @@ int finish_host_work(std::function<int()> &work) {
 }
 
 }  // namespace
+
+extern "C" {
 
 int alpsg_map_search(alpsg_handle h, double omi, double omf, double gami, double gamf, int nr,
                      int ni, int loggridw, int loggridg, double *om_out, double *cal_out,
