@@ -25,6 +25,7 @@
 #include "alpsg_analyt.h"
 #include "alpsg_device.h"
 #include "alpsg_guard.h"
+#include "alpsg_lean.h"
 #include "alpsg_osc.h"
 #include "alpsg_relhost.h"
 
@@ -1934,9 +1935,15 @@ int alpsg_create(const alpsg_plasma_desc *d, alpsg_handle *out) {
       }
     }
     if (S.acmethod == 1 && !S.fit_uniform && S.nfits >= 1) {
+      // every fit a kappa fit whose exponent u4 stays in the lean power's domain
       S.fit_kappa = 1;
-      for (int f = 0; f < S.nfits; ++f)
+      for (int f = 0; f < S.nfits; ++f) {
         if (S.fit_type[f] != 2) S.fit_kappa = 0;
+        for (int i = 0; i <= nperp && S.fit_kappa; ++i) {
+          const double u4 = d->param_fit[(size_t)s + (size_t)nspec * (i + (size_t)(npf + 1) * (3 + 5 * (size_t)f))];
+          if (!(std::fabs(u4) <= ALPSG_LEAN_MAXC)) S.fit_kappa = 0;
+        }
+      }
     }
     if (S.fit_kappa)
       h->lgen |= 2;

@@ -1550,6 +1550,20 @@ __global__ __launch_bounds__(64) void k_resonant(const ProbDev *__restrict__ pro
                            blockIdx.x, blockIdx.y, threadIdx.x, win);
 }
 
+// The Landau part of the kappa species (part 3): the lean power leaves the
+// pass short of four wavefronts per SIMD; three, with a few spills outside
+// the evaluation loop, run it 12 % faster than the two of its natural
+// register count (15.7 against 17.7 ms on a 512 x 512 map, 256 x 512 grid).
+template <int NCH>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_resonant_kappa(
+    const ProbDev *__restrict__ probp, int nom, const double2 *__restrict__ om_in,
+    const int *__restrict__ active, double2 *__restrict__ chiB, double2 *__restrict__ lowB,
+    const int *__restrict__ pidx) {
+  __shared__ double win[(64 / NCH) * 2 * PV_WIN];
+  resonant_body<NCH, 3>(probp, nom, om_in, active, chiB, lowB, pidx, gridDim.z, blockIdx.z,
+                        blockIdx.x, blockIdx.y, threadIdx.x, win);
+}
+
 // The principal-value and Landau parts in one launch (small batches, where a
 // launch costs more than the work): z < RB runs the principal-value part of
 // harmonic block z, z >= RB the Landau part of block z - RB (the part-1, 2 or
@@ -1803,8 +1817,8 @@ hipError_t launch_resonant(hipStream_t st, int nch, int RB, const ProbDev *prob,
       hipLaunchKernelGGL((k_resonant<N, 2>), dim3(grid, nspec, RB), dim3(64), 0, st, prob, nom, \
                          om, active, chiB, lowB, pidx);                                        \
     if (lgen & 2)                                                                              \
-      hipLaunchKernelGGL((k_resonant<N, 3>), dim3(grid, nspec, RB), dim3(64), 0, st, prob, nom, \
-                         om, active, chiB, lowB, pidx);                                        \
+      hipLaunchKernelGGL((k_resonant_kappa<N>), dim3(grid, nspec, RB), dim3(64), 0, st, prob,   \
+                         nom, om, active, chiB, lowB, pidx);                                   \
     return hipGetLastError();                                                                  \
   }
   RS(4) RS(8) RS(16) RS(32) RS(64)

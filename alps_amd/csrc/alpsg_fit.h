@@ -5,6 +5,7 @@
 
 #include "alpsg_analyt.h"
 #include "alpsg_device.h"
+#include "alpsg_lean.h"
 
 namespace {
 
@@ -63,10 +64,13 @@ __device__ __noinline__ cd eval_fit(const SpecDev &S, int iperp, cd p) {
   return f;
 }
 
-// eval_fit of a species whose every fit is a kappa fit (case 2 above, the same
-// operations), inlined into the Landau pass: the call of eval_fit holds that
-// kernel to one wavefront per SIMD, this form to two.  The fits are summed one
-// at a time (unroll 1), so that their temporaries are not live together.
+// eval_fit of a species whose every fit is a kappa fit (case 2 above), inlined
+// into the Landau pass: the same kappa part, raised to u4 by the register-lean
+// complex power of alpsg_lean.h (the library functions of cpowr, inlined,
+// hold that kernel to two wavefronts per SIMD, a call of eval_fit to one).
+// The host gives a species this form only when every |u4| <= ALPSG_LEAN_MAXC.
+// The fits are summed one at a time (unroll 1), so that their temporaries are
+// not live together.
 __device__ __forceinline__ cd eval_fit_kappa(const SpecDev &S, int iperp, cd p) {
   cd f = mk(0.0, 0.0);
 #pragma unroll 1
@@ -75,7 +79,9 @@ __device__ __forceinline__ cd eval_fit_kappa(const SpecDev &S, int iperp, cd p) 
     const cd dq = mk(p.re - c[2], p.im);
     const cd sq = dq * dq;
     const cd kp = mk(1.0 + c[1] * sq.re + c[4], c[1] * sq.im);
-    f = f + c[0] * cpowr(kp, c[3]);
+    double re, im;
+    alpsg_lean::cpow_(kp.re, kp.im, c[3], re, im);
+    f = f + c[0] * mk(re, im);
   }
   return f;
 }

@@ -47,6 +47,7 @@
 #include "alpsg_analyt.h"
 #include "alpsg_device.h"
 #include "alpsg_guard.h"
+#include "alpsg_lean.h"
 #include "alpsg_osc.h"
 #include "alpsg_relhost.h"
 
@@ -489,9 +490,13 @@ int prepare_plasma(const alpsg_plasma_desc *d, Plasma &r) {
             }
       }
       r.funi[s] = uni ? 1 : 0;
+      // every fit a kappa fit whose exponent u4 stays in the lean power's domain
       bool kap = !uni && nf >= 1;
-      for (int f = 0; f < nf; ++f)
+      for (int f = 0; f < nf; ++f) {
         if (r.ftype[(size_t)s * ALPSG_MAXFITS + f] != 2) kap = false;
+        for (int i = 0; i <= nperp && kap; ++i)
+          if (!(std::fabs(prm(s, i, 3, f)) <= ALPSG_LEAN_MAXC)) kap = false;
+      }
       r.fkap[s] = kap ? 1 : 0;
       if (kap)
         r.lgen |= 2;
