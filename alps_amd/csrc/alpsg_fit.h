@@ -64,6 +64,21 @@ __device__ __noinline__ cd eval_fit(const SpecDev &S, int iperp, cd p) {
   return f;
 }
 
+// eval_fit of a species whose every fit is in the Maxwell form (case 1 above,
+// the same operations), inlined: the relativistic species, whose fits over
+// Gamma the host stores in that form, take it in k_rel_res without a call
+__device__ __forceinline__ cd eval_fit_maxwell(const SpecDev &S, int iperp, cd p) {
+  cd f = mk(0.0, 0.0);
+#pragma unroll 1
+  for (int fi = 0; fi < S.nfits; ++fi) {
+    const double *c = S.fitc + ((size_t)iperp * S.nfits + fi) * FIT_STRIDE;
+    const cd dq = mk(p.re - c[2], p.im);
+    const cd sq = dq * dq;
+    f = f + c[0] * cexpd(mk(-c[1] * sq.re, -c[1] * sq.im));
+  }
+  return f;
+}
+
 // eval_fit of a species whose every fit is a kappa fit (case 2 above), inlined
 // into the Landau pass: the same kappa part, raised to u4 by the register-lean
 // complex power of alpsg_lean.h (the library functions of cpowr, inlined,

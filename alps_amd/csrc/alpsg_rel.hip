@@ -33,14 +33,6 @@ namespace {
 constexpr double PI_R = 3.141592653589793238462643383279502884;
 
 __device__ __forceinline__ cd cdivr(cd a, double d) { return cd{a.re / d, a.im / d}; }
-__device__ __forceinline__ cd cdivc(cd a, cd b) {
-  if (fabs(b.re) >= fabs(b.im)) {
-    const double r = b.im / b.re, d = b.re + b.im * r;
-    return cd{(a.re + a.im * r) / d, (a.im - a.re * r) / d};
-  }
-  const double r = b.re / b.im, d = b.re * r + b.im;
-  return cd{(a.re * r + a.im) / d, (a.im * r - a.re) / d};
-}
 __device__ __forceinline__ double m1p(int n) { return (n % 2 == 0) ? 1.0 : -1.0; }
 
 // mode order k: 0 xx, 1 yy, 2 zz, 3 xy, 4 xz, 5 yz
@@ -145,12 +137,21 @@ __global__ __launch_bounds__(256) void k_rel_nonres(const ProbDev *__restrict__ 
     if (!clear) {
 #pragma clang fp contract(off)
       const double nq = 1.0 * n * q;
-      // point pt = ip npar + ia, (ip, ia) stepped without a division per point
+      const double ik = 1.0 / kpar;
+      // point pt = ip npar + ia, (ip, ia) stepped without a division per point.
+      // The test takes the reference's quotient pr = (g m Re w - n q)/kpar;
+      // the product by 1/kpar is within 4 ulp of it, so where that product is
+      // farther than 1e-14 (relative) from both cell ends the test's answer
+      // is the same, and only the other points take the division
       int ip = (int)threadIdx.x / npar, ia = (int)threadIdx.x % npar;
       for (int pt = threadIdx.x; pt < (nperp + 1) * npar; pt += 256) {
         const double g = S.gam_pp[ip * (npar + 1) + ia];
-        const double pr = (g * m * om.re - nq) / kpar;
-        if (S.ppar[ia] <= pr && S.ppar[ia + 1] > pr) hit = 1;
+        const double num = g * m * om.re - nq;
+        double pr = num * ik;
+        const double lo = S.ppar[ia], hi = S.ppar[ia + 1];
+        const double tol = 1e-14 * fabs(pr);
+        if (fabs(pr - lo) <= tol || fabs(pr - hi) <= tol) pr = num / kpar;
+        if (lo <= pr && hi > pr) hit = 1;
         for (ia += 256; ia >= npar; ia -= npar) ++ip;
       }
     }
@@ -171,7 +172,7 @@ __global__ __launch_bounds__(256) void k_rel_nonres(const ProbDev *__restrict__ 
       // resU (ALPS_fns.f90:1587-1592) with the point's Gamma; num / den as
       // num conj(den) / |den|^2 with one reciprocal (|den| is O(1) here: no
       // scaling needed) instead of three divisions
-      const double A = (kpar / (g * m)) * (p1 * d2 - p2 * d1);
+      const double A = (kpar * rcp_d(g * m)) * (p1 * d2 - p2 * d1);
       const cd num = mk(q * (om.re * d1 + A), q * (om.im * d1));
       const cd den = mk(((g * m) * om.re - kpar * p2) - (1.0 * n) * q, (g * m) * om.im);
       const double inv = w * rcp1_d(den.re * den.re + den.im * den.im);
@@ -215,7 +216,16 @@ struct RelCtx {
   int n, npb;
   double vA, kpar, kperp;
   bool kn;
+  // per-call factors of the integrand (not of any decision), so that a grid
+  // point costs one reciprocal: vA/kpar, n q vA/(kpar m), 1/zbar, 1/dpparbar
+  double vak, nterm, izbar, idpb;
 };
+
+// a / b for complex b of moderate size: a conj(b) / |b|^2 with one reciprocal
+__device__ __forceinline__ cd cdiv_r(cd a, cd b) {
+  const double inv = rcp_d(b.re * b.re + b.im * b.im);
+  return mk((a.re * b.re + a.im * b.im) * inv, (a.im * b.re - a.re * b.im) * inv);
+}
 
 // bessel and besselP of int_T_rel (ALPS_fns_rel.f90:1324-1338) from the J table
 __device__ __forceinline__ void rel_bessel(const RelCtx &C, int pt, double &b, double &bP) {
@@ -245,20 +255,20 @@ __device__ __forceinline__ void int_T_rel6(const RelCtx &C, int pt, cd T[6]) {
   }
   double b, bP;
   rel_bessel(C, pt, b, bP);
-  const double zbar = S.rel_zbar, kp = C.kperp, nn = 1.0 * C.n;
+  const double iz = C.izbar, kp = C.kperp, nn = 1.0 * C.n;
   if (C.kn) {
-    T[0] = mk(1.0 * (C.n * C.n) * b * b / (zbar * zbar), 0.0);
+    T[0] = mk(1.0 * (C.n * C.n) * b * b * (iz * iz), 0.0);
     T[1] = mk(bP * bP * pperpbar * pperpbar, 0.0);
     T[2] = mk(b * b * (pb * pb), 0.0);
-    T[3] = mk(0.0, nn * b * bP * pperpbar / zbar);
-    T[4] = mk(nn * b * b * pb / zbar, 0.0);
+    T[3] = mk(0.0, nn * b * bP * pperpbar * iz);
+    T[4] = mk(nn * b * b * pb * iz, 0.0);
     T[5] = mk(0.0, -(b * bP * pb * pperpbar));
   } else {
-    T[0] = mk(1.0 * (C.n * C.n) * b * b / (zbar * zbar), 0.0);
+    T[0] = mk(1.0 * (C.n * C.n) * b * b * (iz * iz), 0.0);
     T[1] = mk(kp * kp * bP * bP * pperpbar * pperpbar, 0.0);
     T[2] = mk(kp * kp * b * b * (pb * pb), 0.0);
-    T[3] = mk(0.0, nn * kp * b * bP * pperpbar / zbar);
-    T[4] = mk(nn * kp * b * b * pb / zbar, 0.0);
+    T[3] = mk(0.0, nn * kp * b * bP * pperpbar * iz);
+    T[4] = mk(nn * kp * b * b * pb * iz, 0.0);
     T[5] = mk(0.0, -(kp * kp * b * bP * pb * pperpbar));
   }
 }
@@ -312,7 +322,7 @@ __device__ void funct_g_rel6(const RelCtx &C, int ig, double x, cd g[6]) {
   const double dx = x - pb[close];
   for (int k = 0; k < 6; ++k) {
     const cd ip = -1.0 * (np_ * Tp[k]), ic = -1.0 * (nc * Tc[k]), im = -1.0 * (nm * Tm[k]);
-    g[k] = ic + (0.5 * cdivr(ip - im, S.rel_dpb)) * dx;
+    g[k] = ic + (0.5 * ((ip - im) * C.idpb)) * dx;
   }
 }
 
@@ -321,10 +331,9 @@ __device__ __forceinline__ void rel_point(const RelCtx &C, int ig, int ip, doubl
   const SpecDev &S = *C.S;
   const int pt = ig * (C.npb + 1) + ip;
   const double G = S.grel[pt], PB = S.pbrel[pt];
-  const cd gom = mk(G * C.om.re * C.vA / C.kpar, G * C.om.im * C.vA / C.kpar);
-  const double nterm = (1.0 * C.n) * S.q * C.vA / (C.kpar * S.m);
-  const cd den = mk((PB - gom.re) + nterm, -gom.im);
-  const cd U = -1.0 * cdivc(rel_num(C, pt), den);
+  const cd gom = mk(G * C.om.re * C.vak, G * C.om.im * C.vak);
+  const cd den = mk((PB - gom.re) + C.nterm, -gom.im);
+  const cd U = -1.0 * cdiv_r(rel_num(C, pt), den);
   cd T[6];
   int_T_rel6(C, pt, T);
   for (int k = 0; k < 6; ++k) acc[k] = acc[k] + w * (U * T[k]);
@@ -371,7 +380,7 @@ __device__ __forceinline__ void rel_cell_add(const RelCtx &C, int ig, RelCell &c
     rel_cell_flush(C, ig, cc, res);
     cc.close = close;
   }
-  const double f = 0.5 * ((x - C.S->pbrel[ig * (C.npb + 1) + close]) / C.S->rel_dpb);
+  const double f = 0.5 * ((x - C.S->pbrel[ig * (C.npb + 1) + close]) * C.idpb);
   cc.c0 = cc.c0 + wt;
   cc.cp = cc.cp + f * wt;
   cc.cm = cc.cm - f * wt;
@@ -402,9 +411,13 @@ __device__ void principal_rel6(const RelCtx &C, int ig, int ipres, int upperlimi
   up.close = dn.close = -1;
   up.cm = up.c0 = up.cp = dn.cm = dn.c0 = dn.cp = mk(0.0, 0.0);
   auto pv_pair = [&](double x, double wgt) {
-    const cd dm = mk(x - denomR, -denomI), dp = mk(x - denomR, denomI);
-    rel_cell_add(C, ig, up, x, cdivc(mk(wgt, 0.0), dm), res);
-    rel_cell_add(C, ig, dn, 2.0 * denomR - x, -1.0 * cdivc(mk(wgt, 0.0), dp), res);
+    // wgt / (t - i denomI) and wgt / (t + i denomI), t = x - denomR: one
+    // reciprocal, the second the conjugate of the first
+    const double t = x - denomR;
+    const double inv = wgt * rcp_d(t * t + denomI * denomI);
+    const cd r = mk(t * inv, denomI * inv);
+    rel_cell_add(C, ig, up, x, r, res);
+    rel_cell_add(C, ig, dn, 2.0 * denomR - x, mk(-r.re, r.im), res);
   };
   if (fabs(denomI) > Tlim) {
     pv_pair(denomR, 1.0);
@@ -462,8 +475,9 @@ __device__ void principal_rel6(const RelCtx &C, int ig, int ipres, int upperlimi
       correction = (base / (1.0 * ntiny)) / smdelta;
     }
     auto tiny = [&](double x, double wgt) {
-      const cd dm = mk(x - denomR, -denomI);
-      rel_cell_add(C, ig, up, x, (wgt * correction) * cdivc(mk(1.0, 0.0), dm), res);
+      const double t = x - denomR;
+      const double inv = (wgt * correction) * rcp_d(t * t + denomI * denomI);
+      rel_cell_add(C, ig, up, x, mk(t * inv, denomI * inv), res);
     };
     double x0, x1;
     {
@@ -680,6 +694,10 @@ __global__ __launch_bounds__(64) void k_rel_res(const ProbDev *__restrict__ prob
   C.kpar = Pd.kpar;
   C.kperp = Pd.kperp;
   C.kn = Pd.kperp_norm != 0;
+  C.vak = C.vA / C.kpar;
+  C.nterm = (1.0 * C.n) * S.q * C.vA / (C.kpar * S.m);
+  C.izbar = 1.0 / S.rel_zbar;
+  C.idpb = 1.0 / S.rel_dpb;
   const int ng = S.ng;
   cd acc[6], accL[6];
   for (int k = 0; k < 6; ++k) acc[k] = accL[k] = mk(0.0, 0.0);
@@ -706,12 +724,14 @@ __global__ __launch_bounds__(64) void k_rel_res(const ProbDev *__restrict__ prob
       if (in) {
         const double h = (ig == ng - 1) ? 0.5 : 1.0;
         cd dfg;
+        // (a relativistic species' fits are stored in the Maxwell form)
         if (ig == 1)
-          dfg = cdivr(eval_fit(S, ig + 1, pres) - eval_fit(S, ig, pres), S.rel_dg);
+          dfg = cdivr(eval_fit_maxwell(S, ig + 1, pres) - eval_fit_maxwell(S, ig, pres), S.rel_dg);
         else
-          dfg = cdivr(eval_fit(S, ig + 1, pres) - eval_fit(S, ig - 1, pres), 2.0 * S.rel_dg);
-        const cd dfp = cdivr(eval_fit(S, ig, mk(pres.re + S.rel_dpb, pres.im)) -
-                                 eval_fit(S, ig, mk(pres.re - S.rel_dpb, pres.im)),
+          dfg = cdivr(eval_fit_maxwell(S, ig + 1, pres) - eval_fit_maxwell(S, ig - 1, pres),
+                      2.0 * S.rel_dg);
+        const cd dfp = cdivr(eval_fit_maxwell(S, ig, mk(pres.re + S.rel_dpb, pres.im)) -
+                                 eval_fit_maxwell(S, ig, mk(pres.re - S.rel_dpb, pres.im)),
                              2.0 * S.rel_dpb);
         const cd fac = h * (C.om * dfg + (C.kpar / C.vA) * dfp);
         cd T[6];
