@@ -91,7 +91,10 @@ __device__ __forceinline__ cd block_sum(cd v, double *lds) {
 // k_rel_nonres: flags of every harmonic of the block and the integrate()
 // terms of the non-resonant ones.  grid (omega, species, NB), 256 threads.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_rel_nonres(const ProbDev *__restrict__ probp, int nom,
+// three wavefronts per SIMD: with two harmonics per pass the natural
+// register count allows two, and three (a few spills) ran the relativistic
+// served stream 7 % faster
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_rel_nonres(const ProbDev *__restrict__ probp, int nom,
                                                     const double2 *__restrict__ om_in,
                                                     const int *__restrict__ active,
                                                     double2 *__restrict__ chiR,
@@ -123,20 +126,26 @@ __global__ __launch_bounds__(256) void k_rel_nonres(const ProbDev *__restrict__ 
   }
   gmin = -block_max<256>(-gmin, lds);
   gmax = block_max<256>(gmax, lds);
-  for (int n = -nmax + nb; n <= nmax; n += NB) {
-    int hit = 0;
-    bool clear;
-    {
+  // whether harmonic n's p_res stays clear of the ppar grid over [gmin, gmax]
+  auto clear_of = [&](int n) {
 #pragma clang fp contract(off)
-      const double nq = 1.0 * n * q;
-      const double pa = (gmin * m * om.re - nq) / kpar, pb = (gmax * m * om.re - nq) / kpar;
-      const double lo = S.ppar[0], hi = S.ppar[npar];
-      const double tol = 1e-9 * (fabs(pa) + fabs(pb) + fabs(lo) + fabs(hi));
-      clear = fmax(pa, pb) < lo - tol || fmin(pa, pb) > hi + tol;
-    }
-    if (!clear) {
+    const double nq = 1.0 * n * q;
+    const double pa = (gmin * m * om.re - nq) / kpar, pb = (gmax * m * om.re - nq) / kpar;
+    const double lo = S.ppar[0], hi = S.ppar[npar];
+    const double tol = 1e-9 * (fabs(pa) + fabs(pb) + fabs(lo) + fabs(hi));
+    return fmax(pa, pb) < lo - tol || fmin(pa, pb) > hi + tol;
+  };
+  // the block's harmonics n = -nmax + nb + j NB two at a time: both share each
+  // point's loads in the resonance scan and in the trapezoid (the same terms,
+  // summed in the same order, as one harmonic at a time)
+  for (int n0 = -nmax + nb; n0 <= nmax; n0 += 2 * NB) {
+    const int n1 = n0 + NB;
+    const bool has1 = n1 <= nmax;
+    const bool scan0 = !clear_of(n0), scan1 = has1 && !clear_of(n1);
+    int hit0 = 0, hit1 = 0;
+    if (scan0 || scan1) {
 #pragma clang fp contract(off)
-      const double nq = 1.0 * n * q;
+      const double nq0 = 1.0 * n0 * q, nq1 = 1.0 * n1 * q;
       const double ik = 1.0 / kpar;
       // point pt = ip npar + ia, (ip, ia) stepped without a division per point.
       // The test takes the reference's quotient pr = (g m Re w - n q)/kpar;
@@ -146,38 +155,44 @@ __global__ __launch_bounds__(256) void k_rel_nonres(const ProbDev *__restrict__ 
       int ip = (int)threadIdx.x / npar, ia = (int)threadIdx.x % npar;
       for (int pt = threadIdx.x; pt < (nperp + 1) * npar; pt += 256) {
         const double g = S.gam_pp[ip * (npar + 1) + ia];
-        const double num = g * m * om.re - nq;
-        double pr = num * ik;
         const double lo = S.ppar[ia], hi = S.ppar[ia + 1];
-        const double tol = 1e-14 * fabs(pr);
-        if (fabs(pr - lo) <= tol || fabs(pr - hi) <= tol) pr = num / kpar;
-        if (lo <= pr && hi > pr) hit = 1;
+        const double gm = g * m * om.re;
+        if (scan0) {
+          const double num = gm - nq0;
+          double pr = num * ik;
+          const double tol = 1e-14 * fabs(pr);
+          if (fabs(pr - lo) <= tol || fabs(pr - hi) <= tol) pr = num / kpar;
+          if (lo <= pr && hi > pr) hit0 = 1;
+        }
+        if (scan1) {
+          const double num = gm - nq1;
+          double pr = num * ik;
+          const double tol = 1e-14 * fabs(pr);
+          if (fabs(pr - lo) <= tol || fabs(pr - hi) <= tol) pr = num / kpar;
+          if (lo <= pr && hi > pr) hit1 = 1;
+        }
         for (ia += 256; ia >= npar; ia -= npar) ++ip;
       }
     }
-    const int found = __syncthreads_or(hit);
-    if (threadIdx.x == 0) flags[((size_t)iom * nspec + s) * nstride + n + Pd.nmaxR] = found;
-    if (found) continue;
-    const int an = n < 0 ? -n : n;
-    const double *at = S.atab + (size_t)an * (nperp + 1) * ATAB_STRIDE;
-    cd acc[6];
-    for (int k = 0; k < 6; ++k) acc[k] = mk(0.0, 0.0);
-    // point pt = (ip - 1)(npar - 1) + ia - 1, stepped as above
-    int ip = 1 + (int)threadIdx.x / (npar - 1), ia = 1 + (int)threadIdx.x % (npar - 1);
-    for (int pt = threadIdx.x; pt < (nperp - 1) * (npar - 1); pt += 256) {
-      const double w = (ip == nperp - 1 ? 1.0 : 2.0) * ((ia == 1 || ia == npar - 1) ? 1.0 : 2.0);
-      const size_t o = (size_t)ip * (npar + 1) + ia;
-      const double g = S.gam_pp[o], d1 = S.df1[o], d2 = S.df2[o];
-      const double p1 = S.pperp[ip], p2 = S.ppar[ia];
-      // resU (ALPS_fns.f90:1587-1592) with the point's Gamma; num / den as
-      // num conj(den) / |den|^2 with one reciprocal (|den| is O(1) here: no
-      // scaling needed) instead of three divisions
-      const double A = (kpar * rcp_d(g * m)) * (p1 * d2 - p2 * d1);
-      const cd num = mk(q * (om.re * d1 + A), q * (om.im * d1));
-      const cd den = mk(((g * m) * om.re - kpar * p2) - (1.0 * n) * q, (g * m) * om.im);
-      const double inv = w * rcp1_d(den.re * den.re + den.im * den.im);
-      const cd r = mk((num.re * den.re + num.im * den.im) * inv,
-                      (num.im * den.re - num.re * den.im) * inv);
+    const int found0 = __syncthreads_or(hit0);
+    const int found1 = __syncthreads_or(hit1);
+    if (threadIdx.x == 0) {
+      flags[((size_t)iom * nspec + s) * nstride + n0 + Pd.nmaxR] = found0;
+      if (has1) flags[((size_t)iom * nspec + s) * nstride + n1 + Pd.nmaxR] = found1;
+    }
+    const bool do0 = !found0, do1 = has1 && !found1;
+    if (!do0 && !do1) continue;
+    const double *at0 = S.atab + (size_t)(n0 < 0 ? -n0 : n0) * (nperp + 1) * ATAB_STRIDE;
+    const double *at1 = S.atab + (size_t)(n1 < 0 ? -n1 : n1) * (nperp + 1) * ATAB_STRIDE;
+    cd acc0[6], acc1[6];
+    for (int k = 0; k < 6; ++k) acc0[k] = acc1[k] = mk(0.0, 0.0);
+    // one harmonic's terms of the point: resU (ALPS_fns.f90:1587-1592) with the
+    // point's Gamma, num / den as num conj(den) / |den|^2 with one reciprocal
+    // (|den| is O(1) here: no scaling needed), times the int_T moments
+    auto add = [&](const double *at, int ip, double w, cd num, double dre, double dim, double p2,
+                   cd acc[6]) {
+      const double inv = w * rcp1_d(dre * dre + dim * dim);
+      const cd r = mk((num.re * dre + num.im * dim) * inv, (num.im * dre - num.re * dim) * inv);
       const double a1 = at[ip * ATAB_STRIDE + 0], a2 = at[ip * ATAB_STRIDE + 1],
                    a3 = at[ip * ATAB_STRIDE + 2];
       const cd u1 = a1 * r, u3 = a3 * r;
@@ -187,18 +202,37 @@ __global__ __launch_bounds__(256) void k_rel_nonres(const ProbDev *__restrict__ 
       acc[3] = acc[3] + a2 * r;
       acc[4] = acc[4] + u3;
       acc[5] = acc[5] + p2 * u3;
+    };
+    // point pt = (ip - 1)(npar - 1) + ia - 1, stepped as above
+    int ip = 1 + (int)threadIdx.x / (npar - 1), ia = 1 + (int)threadIdx.x % (npar - 1);
+    for (int pt = threadIdx.x; pt < (nperp - 1) * (npar - 1); pt += 256) {
+      const double w = (ip == nperp - 1 ? 1.0 : 2.0) * ((ia == 1 || ia == npar - 1) ? 1.0 : 2.0);
+      const size_t o = (size_t)ip * (npar + 1) + ia;
+      const double g = S.gam_pp[o], d1 = S.df1[o], d2 = S.df2[o];
+      const double p1 = S.pperp[ip], p2 = S.ppar[ia];
+      const double A = (kpar * rcp_d(g * m)) * (p1 * d2 - p2 * d1);
+      const cd num = mk(q * (om.re * d1 + A), q * (om.im * d1));
+      const double base = (g * m) * om.re - kpar * p2, dim = (g * m) * om.im;
+      if (do0) add(at0, ip, w, num, base - (1.0 * n0) * q, dim, p2, acc0);
+      if (do1) add(at1, ip, w, num, base - (1.0 * n1) * q, dim, p2, acc1);
       for (ia += 256; ia > npar - 1; ia -= npar - 1) ++ip;
     }
-    cd v[6];
-    for (int k = 0; k < 6; ++k) {
-      const cd t = block_sum<256>(acc[ACC_OF_MODE_R[k]], lds);
-      v[k] = (scale * t) * tconst_r(Pd, S, n, k);
-    }
-    if (n == 0) v[0] = v[3] = v[4] = mk(0.0, 0.0);
-    for (int k = 0; k < 6; ++k) res[k] = res[k] + v[k];
-    if (threadIdx.x == 0 && lowR != nullptr && an <= 1) {
-      double2 *lo = lowR + ((size_t)iom * nspec + s) * 18 + 6 * (n + 1);
-      for (int k = 0; k < 6; ++k) lo[k] = make_double2(v[k].re, v[k].im);
+    for (int h = 0; h < 2; ++h) {
+      const int n = h == 0 ? n0 : n1;
+      if (h == 0 ? !do0 : !do1) continue;
+      const cd *acc = h == 0 ? acc0 : acc1;
+      const int an = n < 0 ? -n : n;
+      cd v[6];
+      for (int k = 0; k < 6; ++k) {
+        const cd t = block_sum<256>(acc[ACC_OF_MODE_R[k]], lds);
+        v[k] = (scale * t) * tconst_r(Pd, S, n, k);
+      }
+      if (n == 0) v[0] = v[3] = v[4] = mk(0.0, 0.0);
+      for (int k = 0; k < 6; ++k) res[k] = res[k] + v[k];
+      if (threadIdx.x == 0 && lowR != nullptr && an <= 1) {
+        double2 *lo = lowR + ((size_t)iom * nspec + s) * 18 + 6 * (n + 1);
+        for (int k = 0; k < 6; ++k) lo[k] = make_double2(v[k].re, v[k].im);
+      }
     }
   }
   if (threadIdx.x == 0) {
@@ -307,23 +341,25 @@ __device__ int rel_close(const RelCtx &C, int ig, double x) {
   return close;
 }
 
-// funct_g_rel (ALPS_fns_rel.f90:918-999) for the six modes at real pparbar x of row ig
+// funct_g_rel (ALPS_fns_rel.f90:918-999) for the six modes at real pparbar x
+// of row ig: g(close) + (x - pparbar(close)) (g(close+1) - g(close-1)) /
+// (2 dpparbar), g = -(rel_num T), with one node's T values live at a time
 __device__ void funct_g_rel6(const RelCtx &C, int ig, double x, cd g[6]) {
   const SpecDev &S = *C.S;
   const int npb = C.npb, row = ig * (npb + 1);
   const double *pb = S.pbrel + row;
   const int close = rel_close(C, ig, x);
-  cd Tp[6], Tm[6], Tc[6];
-  int_T_rel6(C, row + close + 1, Tp);
-  int_T_rel6(C, row + close, Tc);
-  int_T_rel6(C, row + close - 1, Tm);
-  const cd np_ = rel_num(C, row + close + 1), nc = rel_num(C, row + close),
-           nm = rel_num(C, row + close - 1);
-  const double dx = x - pb[close];
-  for (int k = 0; k < 6; ++k) {
-    const cd ip = -1.0 * (np_ * Tp[k]), ic = -1.0 * (nc * Tc[k]), im = -1.0 * (nm * Tm[k]);
-    g[k] = ic + (0.5 * ((ip - im) * C.idpb)) * dx;
-  }
+  const double f = 0.5 * ((x - pb[close]) * C.idpb);
+  cd T[6];
+  int_T_rel6(C, row + close, T);
+  const cd nc = rel_num(C, row + close);
+  for (int k = 0; k < 6; ++k) g[k] = -1.0 * (nc * T[k]);
+  int_T_rel6(C, row + close + 1, T);
+  const cd np_ = -f * rel_num(C, row + close + 1);
+  for (int k = 0; k < 6; ++k) g[k] = g[k] + np_ * T[k];
+  int_T_rel6(C, row + close - 1, T);
+  const cd nm = f * rel_num(C, row + close - 1);
+  for (int k = 0; k < 6; ++k) g[k] = g[k] + nm * T[k];
 }
 
 // the resU_rel * int_T_rel terms of point ip of row ig, weight w, into acc
