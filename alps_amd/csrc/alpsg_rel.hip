@@ -394,26 +394,29 @@ __device__ __forceinline__ void rel_node_g(const RelCtx &C, int pt, cd g[6]) {
   for (int k = 0; k < 6; ++k) g[k] = -1.0 * (nm * T[k]);
 }
 
-__device__ void rel_cell_flush(const RelCtx &C, int ig, RelCell &cc, cd res[6]) {
+// the open cell's three coefficients, times f, into acc
+__device__ void rel_cell_flush(const RelCtx &C, int ig, RelCell &cc, cd acc[6], double f) {
   if (cc.close < 0) return;
   const int pt = ig * (C.npb + 1) + cc.close;
   cd g[6];
   rel_node_g(C, pt, g);
-  for (int k = 0; k < 6; ++k) res[k] = res[k] + cc.c0 * g[k];
+  const cd c0 = f * cc.c0, cp = f * cc.cp, cm = f * cc.cm;
+  for (int k = 0; k < 6; ++k) acc[k] = acc[k] + c0 * g[k];
   rel_node_g(C, pt + 1, g);
-  for (int k = 0; k < 6; ++k) res[k] = res[k] + cc.cp * g[k];
+  for (int k = 0; k < 6; ++k) acc[k] = acc[k] + cp * g[k];
   rel_node_g(C, pt - 1, g);
-  for (int k = 0; k < 6; ++k) res[k] = res[k] + cc.cm * g[k];
+  for (int k = 0; k < 6; ++k) acc[k] = acc[k] + cm * g[k];
   cc.close = -1;
   cc.cm = cc.c0 = cc.cp = mk(0.0, 0.0);
 }
 
-// add wt g(x) through the cell cache
+// add wt g(x) through the cell cache (a cell leaving it goes into acc, times
+// fac)
 __device__ __forceinline__ void rel_cell_add(const RelCtx &C, int ig, RelCell &cc, double x, cd wt,
-                                             cd res[6]) {
+                                             cd acc[6], double fac) {
   const int close = rel_close(C, ig, x);
   if (close != cc.close) {
-    rel_cell_flush(C, ig, cc, res);
+    rel_cell_flush(C, ig, cc, acc, fac);
     cc.close = close;
   }
   const double f = 0.5 * ((x - C.S->pbrel[ig * (C.npb + 1) + close]) * C.idpb);
@@ -422,12 +425,14 @@ __device__ __forceinline__ void rel_cell_add(const RelCtx &C, int ig, RelCell &c
   cc.cm = cc.cm - f * wt;
 }
 
-// principal_integral_rel (ALPS_fns_rel.f90:724-913), six modes, added to acc.
-// The ladder sums go through cell coefficients (RelCell), with the ladder's
-// points visited in increasing x (the far end of each ladder after its inner
+// principal_integral_rel (ALPS_fns_rel.f90:724-913), six modes, times smdelta
+// and the row's trapezoid weight wr, added to acc (each term carries the
+// factor, so that no partial sum of the row is held beside acc).  The ladder
+// sums go through cell coefficients (RelCell), with the ladder's points
+// visited in increasing x (the far end of each ladder after its inner
 // points), so that each cell is opened once.
 __device__ void principal_rel6(const RelCtx &C, int ig, int ipres, int upperlimit, double Tlim,
-                               int pos, int nri, cd acc[6]) {
+                               int pos, int nri, double wr, cd acc[6]) {
   const SpecDev &S = *C.S;
   const int row = ig * (C.npb + 1);
   double denomR, denomI, capDelta, smdelta;
@@ -440,8 +445,7 @@ __device__ void principal_rel6(const RelCtx &C, int ig, int ipres, int upperlimi
     capDelta = denomR - S.pbrel[1 * (C.npb + 1) + ipres - pos];
     smdelta = capDelta / (1.0 * nri);
   }
-  cd res[6];
-  for (int k = 0; k < 6; ++k) res[k] = mk(0.0, 0.0);
+  const double f = wr * smdelta;
   cd g[6], g2[6];
   RelCell up, dn;  // x = denomR + t (increasing) and its mirror 2 denomR - x (decreasing)
   up.close = dn.close = -1;
@@ -452,8 +456,8 @@ __device__ void principal_rel6(const RelCtx &C, int ig, int ipres, int upperlimi
     const double t = x - denomR;
     const double inv = wgt * rcp_d(t * t + denomI * denomI);
     const cd r = mk(t * inv, denomI * inv);
-    rel_cell_add(C, ig, up, x, r, res);
-    rel_cell_add(C, ig, dn, 2.0 * denomR - x, mk(-r.re, r.im), res);
+    rel_cell_add(C, ig, up, x, r, acc, f);
+    rel_cell_add(C, ig, dn, 2.0 * denomR - x, mk(-r.re, r.im), acc, f);
   };
   if (fabs(denomI) > Tlim) {
     pv_pair(denomR, 1.0);
@@ -466,8 +470,8 @@ __device__ void principal_rel6(const RelCtx &C, int ig, int ipres, int upperlimi
       pv_pair(x, 2.0);
     }
     pv_pair(denomR + capDelta, 1.0);
-    rel_cell_flush(C, ig, up, res);
-    rel_cell_flush(C, ig, dn, res);
+    rel_cell_flush(C, ig, up, acc, f);
+    rel_cell_flush(C, ig, dn, acc, f);
   } else {
     funct_g_rel6(C, ig, denomR + S.rel_dpb, g);
     funct_g_rel6(C, ig, denomR - S.rel_dpb, g2);
@@ -479,7 +483,7 @@ __device__ void principal_rel6(const RelCtx &C, int ig, int ipres, int upperlimi
     };
     {
       const double l = lorentz(denomR + capDelta);
-      for (int k = 0; k < 6; ++k) res[k] = res[k] + (2.0 * gp[k]) * l;
+      for (int k = 0; k < 6; ++k) acc[k] = acc[k] + (f * (2.0 * l)) * gp[k];
     }
     for (int ip = 1; ip <= nri - 1; ++ip) {
       double x;
@@ -488,13 +492,13 @@ __device__ void principal_rel6(const RelCtx &C, int ig, int ipres, int upperlimi
         x = denomR + smdelta * ip;
       }
       const double l = lorentz(x);
-      for (int k = 0; k < 6; ++k) res[k] = res[k] + ((2.0 * 2.0) * gp[k]) * l;
+      for (int k = 0; k < 6; ++k) acc[k] = acc[k] + (f * ((2.0 * 2.0) * l)) * gp[k];
     }
     if (denomI != 0.0) {
       funct_g_rel6(C, ig, denomR, g);
       const double sg = denomI > 0.0 ? 1.0 : -1.0;
       for (int k = 0; k < 6; ++k)
-        res[k] = res[k] + cdivr(timesI((sg * 2.0 * PI_R) * g[k]), smdelta);
+        acc[k] = acc[k] + wr * timesI((sg * 2.0 * PI_R) * g[k]);  // (2 pi i g / smdelta) f
     }
   }
   int ntiny;
@@ -513,7 +517,7 @@ __device__ void principal_rel6(const RelCtx &C, int ig, int ipres, int upperlimi
     auto tiny = [&](double x, double wgt) {
       const double t = x - denomR;
       const double inv = (wgt * correction) * rcp_d(t * t + denomI * denomI);
-      rel_cell_add(C, ig, up, x, mk(t * inv, denomI * inv), res);
+      rel_cell_add(C, ig, up, x, mk(t * inv, denomI * inv), acc, f);
     };
     double x0, x1;
     {
@@ -531,14 +535,15 @@ __device__ void principal_rel6(const RelCtx &C, int ig, int ipres, int upperlimi
       tiny(x, 2.0);
     }
     tiny(x1, 1.0);
-    rel_cell_flush(C, ig, up, res);
+    rel_cell_flush(C, ig, up, acc, f);
   }
-  for (int k = 0; k < 6; ++k) acc[k] = acc[k] + res[k] * smdelta;
 }
 
-// integrate_resU_rel (ALPS_fns_rel.f90:516-719) of row ig, six modes; returns
-// false where the reference calls alps_error(8)
-__device__ bool resU_row6(const RelCtx &C, int ig, int pos, int nri, double Tlim, cd out[6]) {
+// integrate_resU_rel (ALPS_fns_rel.f90:516-719) of row ig, six modes, times
+// the row's trapezoid weight wr, added to acc (term by term); returns false,
+// before adding anything, where the reference calls alps_error(8)
+__device__ bool resU_row6(const RelCtx &C, int ig, int pos, int nri, double Tlim, double wr,
+                          cd acc[6]) {
   const SpecDev &S = *C.S;
   const int npb = C.npb;
   const double dpb = S.rel_dpb;
@@ -601,18 +606,15 @@ __device__ bool resU_row6(const RelCtx &C, int ig, int pos, int nri, double Tlim
     int_end = npb - 1;
     upperlimit = npb;
   }
-  cd acc[6];
-  for (int k = 0; k < 6; ++k) acc[k] = mk(0.0, 0.0);
-  if (int_start <= lowerlimit) rel_point(C, ig, int_start, 1.0, acc);
-  for (int ip = int_start + 1; ip <= lowerlimit - 1; ++ip) rel_point(C, ig, ip, 2.0, acc);
-  if (int_start < lowerlimit) rel_point(C, ig, lowerlimit, 1.0, acc);
-  if (upperlimit <= int_end) rel_point(C, ig, upperlimit, 1.0, acc);
-  for (int ip = upperlimit + 1; ip <= int_end - 1; ++ip) rel_point(C, ig, ip, 2.0, acc);
-  if (upperlimit < int_end) rel_point(C, ig, int_end, 1.0, acc);
-  for (int k = 0; k < 6; ++k) acc[k] = acc[k] * dpb;
+  const double w1 = wr * dpb, w2 = 2.0 * w1;  // trapezoid weights times dpparbar and wr
+  if (int_start <= lowerlimit) rel_point(C, ig, int_start, w1, acc);
+  for (int ip = int_start + 1; ip <= lowerlimit - 1; ++ip) rel_point(C, ig, ip, w2, acc);
+  if (int_start < lowerlimit) rel_point(C, ig, lowerlimit, w1, acc);
+  if (upperlimit <= int_end) rel_point(C, ig, upperlimit, w1, acc);
+  for (int ip = upperlimit + 1; ip <= int_end - 1; ++ip) rel_point(C, ig, ip, w2, acc);
+  if (upperlimit < int_end) rel_point(C, ig, int_end, w1, acc);
   if (found && lowerlimit >= int_start && upperlimit <= int_end)
-    principal_rel6(C, ig, ipres, upperlimit, Tlim, pos, nri, acc);
-  for (int k = 0; k < 6; ++k) out[k] = acc[k];
+    principal_rel6(C, ig, ipres, upperlimit, Tlim, pos, nri, wr, acc);
   return true;
 }
 
@@ -693,9 +695,16 @@ __device__ void int_T_res_rel6(const RelCtx &C, const double *gtab, int ig, cd p
 }
 
 // ---------------------------------------------------------------------------
-// k_rel_res: integrate_res_rel (+ landau_integrate_rel for Im om <= 0) of one
-// resonant harmonic.  grid (omega, species, 2 nmaxR + 1), 64 threads.
+// k_rel_res: one resonant harmonic, grid (omega, species, 2 nmaxR + 1), 64
+// threads over the Gamma rows.  LANDAU = false: integrate_res_rel (the
+// trapezoid and the principal-value part), into slot NB + z of chiR;
+// LANDAU = true: landau_integrate_rel for Im om <= 0, into slot
+// NB + (2 nmaxR + 1) + z, and added to the harmonic's lowR terms (the first
+// launch wrote them).  Two launches rather than one body: the Landau rows'
+// Bessel series and continuation need registers that would hold the whole
+// pass to one wavefront per SIMD; apart, the first runs at two.
 // ---------------------------------------------------------------------------
+template <bool LANDAU>
 __global__ __launch_bounds__(64) void k_rel_res(const ProbDev *__restrict__ probp, int nom,
                                                 const double2 *__restrict__ om_in,
                                                 const int *__restrict__ active,
@@ -712,7 +721,8 @@ __global__ __launch_bounds__(64) void k_rel_res(const ProbDev *__restrict__ prob
   if (!(active == nullptr || active[iom] != 0)) return;
   const int nspec = Pd.nspec;
   const int n = (int)blockIdx.z - Pd.nmaxR;
-  double2 *o = chiR + ((size_t)((NB + blockIdx.z) * nspec + s) * 6) * nom + iom;
+  const int slot = NB + (LANDAU ? (int)gridDim.z : 0) + (int)blockIdx.z;
+  double2 *o = chiR + ((size_t)(slot * nspec + s) * 6) * nom + iom;
   const bool inr = (n >= -S.nmax_rel && n <= S.nmax_rel);
   const bool found = inr && flags[((size_t)iom * nspec + s) * nstride + blockIdx.z] != 0;
   if (!found) {
@@ -735,18 +745,14 @@ __global__ __launch_bounds__(64) void k_rel_res(const ProbDev *__restrict__ prob
   C.izbar = 1.0 / S.rel_zbar;
   C.idpb = 1.0 / S.rel_dpb;
   const int ng = S.ng;
-  cd acc[6], accL[6];
-  for (int k = 0; k < 6; ++k) acc[k] = accL[k] = mk(0.0, 0.0);
+  cd acc[6];
+  for (int k = 0; k < 6; ++k) acc[k] = mk(0.0, 0.0);
   int bad = 0;
   for (int ig = 1 + threadIdx.x; ig <= ng - 1; ig += 64) {
-    cd U[6];
-    if (!resU_row6(C, ig, Pd.P, Pd.NRI, Pd.Tlim, U)) {
-      bad = 1;
-      continue;
-    }
-    const double wr = ig <= ng - 2 ? 2.0 : 1.0;
-    for (int k = 0; k < 6; ++k) acc[k] = acc[k] + wr * U[k];
-    if (C.om.im <= 0.0) {
+    if constexpr (!LANDAU) {
+      const double wr = ig <= ng - 2 ? 2.0 : 1.0;
+      if (!resU_row6(C, ig, Pd.P, Pd.NRI, Pd.Tlim, wr, acc)) bad = 1;
+    } else if (C.om.im <= 0.0) {
       // landau_integrate_rel row (ALPS_fns_rel.f90:1065-1083)
       const double g1 = S.grel[ig * (S.npb + 1) + 1];
       cd pres;
@@ -772,7 +778,7 @@ __global__ __launch_bounds__(64) void k_rel_res(const ProbDev *__restrict__ prob
         const cd fac = h * (C.om * dfg + (C.kpar / C.vA) * dfp);
         cd T[6];
         int_T_res_rel6(C, Pd.gtab, ig, pres, T);
-        for (int k = 0; k < 6; ++k) accL[k] = accL[k] - fac * T[k];
+        for (int k = 0; k < 6; ++k) acc[k] = acc[k] - fac * T[k];
       }
     }
   }
@@ -780,28 +786,36 @@ __global__ __launch_bounds__(64) void k_rel_res(const ProbDev *__restrict__ prob
   cd v[6];
   for (int k = 0; k < 6; ++k) {
     const cd r = block_sum<64>(acc[k], nullptr);
-    v[k] = (r * S.rel_dg) * 0.25;
-    if (C.om.im <= 0.0) {
-      const cd L = block_sum<64>(accL[k], nullptr);
-      const cd Lf = (((timesI(L) * S.rel_dg) * PI_R) * 2.0) * PI_R *
+    if constexpr (!LANDAU) {
+      v[k] = (r * S.rel_dg) * 0.25;
+    } else if (C.om.im <= 0.0) {
+      const cd Lf = (((timesI(r) * S.rel_dg) * PI_R) * 2.0) * PI_R *
                     ((S.q * C.vA / (C.kpar * S.m)) * ((S.m / C.vA) * (S.m / C.vA) * (S.m / C.vA)));
-      v[k] = v[k] + (C.om.im < 0.0 ? 2.0 : 1.0) * Lf;
+      v[k] = (C.om.im < 0.0 ? 2.0 : 1.0) * Lf;
+    } else {
+      v[k] = mk(0.0, 0.0);
     }
   }
   if (n == 0) v[0] = v[3] = v[4] = mk(0.0, 0.0);
   if (threadIdx.x == 0) {
     for (int k = 0; k < 6; ++k) o[(size_t)k * nom] = make_double2(v[k].re, v[k].im);
     if (lowR != nullptr && n >= -1 && n <= 1) {
+      // the harmonic's whole term: the Landau launch adds its part to the
+      // integrate_res_rel part the first launch wrote
       double2 *lo = lowR + ((size_t)iom * nspec + s) * 18 + 6 * (n + 1);
-      for (int k = 0; k < 6; ++k) lo[k] = make_double2(v[k].re, v[k].im);
+      for (int k = 0; k < 6; ++k) {
+        const cd t = LANDAU ? mk(lo[k].x, lo[k].y) + v[k] : v[k];
+        lo[k] = make_double2(t.re, t.im);
+      }
     }
   }
 }
 
 }  // namespace
 
-// partial slots of chiR: NB harmonic blocks of k_rel_nonres, then 2 nmaxR + 1 of k_rel_res
-int rel_slots(int NB, int nmaxR) { return NB + 2 * nmaxR + 1; }
+// partial slots of chiR: NB harmonic blocks of k_rel_nonres, then 2 nmaxR + 1
+// of each k_rel_res launch
+int rel_slots(int NB, int nmaxR) { return NB + 2 * (2 * nmaxR + 1); }
 
 // nmaxR: the largest relativistic nmax of the batch (of any omega's problem
 // with pidx); flags [nom][nspec][2 nmaxR + 1]; err [1], or [nom] with pidx
@@ -812,7 +826,9 @@ hipError_t launch_rel(hipStream_t st, const ProbDev *prob, int nspec, int nmaxR,
   const int nstride = 2 * nmaxR + 1;
   hipLaunchKernelGGL(k_rel_nonres, dim3(nom, nspec, NB), dim3(256), 0, st, prob, nom, om, active,
                      chiR, lowR, flags, pidx, nstride);
-  hipLaunchKernelGGL(k_rel_res, dim3(nom, nspec, 2 * nmaxR + 1), dim3(64), 0, st, prob, nom, om,
-                     active, chiR, lowR, flags, NB, err, pidx, nstride);
+  hipLaunchKernelGGL(k_rel_res<false>, dim3(nom, nspec, 2 * nmaxR + 1), dim3(64), 0, st, prob,
+                     nom, om, active, chiR, lowR, flags, NB, err, pidx, nstride);
+  hipLaunchKernelGGL(k_rel_res<true>, dim3(nom, nspec, 2 * nmaxR + 1), dim3(64), 0, st, prob,
+                     nom, om, active, chiR, lowR, flags, NB, err, pidx, nstride);
   return hipGetLastError();
 }
