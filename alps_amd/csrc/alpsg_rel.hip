@@ -695,24 +695,24 @@ __device__ void int_T_res_rel6(const RelCtx &C, const double *gtab, int ig, cd p
 }
 
 // ---------------------------------------------------------------------------
-// k_rel_res: one resonant harmonic, grid (omega, species, 2 nmaxR + 1), 64
-// threads over the Gamma rows.  LANDAU = false: integrate_res_rel (the
-// trapezoid and the principal-value part), into slot NB + z of chiR;
-// LANDAU = true: landau_integrate_rel for Im om <= 0, into slot
-// NB + (2 nmaxR + 1) + z, and added to the harmonic's lowR terms (the first
-// launch wrote them).  Two launches rather than one body: the Landau rows'
-// Bessel series and continuation need registers that would hold the whole
-// pass to one wavefront per SIMD; apart, the first runs at two.
+// k_rel_res / k_rel_landau: one resonant harmonic, grid (omega, species,
+// 2 nmaxR + 1), 64 threads over the Gamma rows.  k_rel_res (LANDAU = false):
+// integrate_res_rel (the trapezoid and the principal-value part), into slot
+// NB + z of chiR; k_rel_landau (LANDAU = true): landau_integrate_rel for
+// Im om <= 0, into slot NB + (2 nmaxR + 1) + z, and added to the harmonic's
+// lowR terms (k_rel_res wrote them).  Two launches rather than one body: the
+// Landau rows' Bessel series and continuation need registers that would hold
+// the whole pass to one wavefront per SIMD.
 // ---------------------------------------------------------------------------
 template <bool LANDAU>
-__global__ __launch_bounds__(64) void k_rel_res(const ProbDev *__restrict__ probp, int nom,
-                                                const double2 *__restrict__ om_in,
-                                                const int *__restrict__ active,
-                                                double2 *__restrict__ chiR,
-                                                double2 *__restrict__ lowR,
-                                                const int *__restrict__ flags, int NB,
-                                                int *__restrict__ err,
-                                                const int *__restrict__ pidx, int nstride) {
+__device__ __forceinline__ void rel_res_body(const ProbDev *__restrict__ probp, int nom,
+                                             const double2 *__restrict__ om_in,
+                                             const int *__restrict__ active,
+                                             double2 *__restrict__ chiR,
+                                             double2 *__restrict__ lowR,
+                                             const int *__restrict__ flags, int NB,
+                                             int *__restrict__ err,
+                                             const int *__restrict__ pidx, int nstride) {
   const int iom = blockIdx.x, s = blockIdx.y;
   const ProbDev &Pd = pidx != nullptr ? probp[pidx[iom]] : *probp;
   if (s >= Pd.nspec) return;
@@ -811,6 +811,24 @@ __global__ __launch_bounds__(64) void k_rel_res(const ProbDev *__restrict__ prob
   }
 }
 
+// the integrate_res_rel part at three wavefronts per SIMD (a few spilled
+// registers; two by its register count)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_rel_res(
+    const ProbDev *__restrict__ probp, int nom, const double2 *__restrict__ om_in,
+    const int *__restrict__ active, double2 *__restrict__ chiR, double2 *__restrict__ lowR,
+    const int *__restrict__ flags, int NB, int *__restrict__ err, const int *__restrict__ pidx,
+    int nstride) {
+  rel_res_body<false>(probp, nom, om_in, active, chiR, lowR, flags, NB, err, pidx, nstride);
+}
+
+__global__ __launch_bounds__(64) void k_rel_landau(
+    const ProbDev *__restrict__ probp, int nom, const double2 *__restrict__ om_in,
+    const int *__restrict__ active, double2 *__restrict__ chiR, double2 *__restrict__ lowR,
+    const int *__restrict__ flags, int NB, int *__restrict__ err, const int *__restrict__ pidx,
+    int nstride) {
+  rel_res_body<true>(probp, nom, om_in, active, chiR, lowR, flags, NB, err, pidx, nstride);
+}
+
 }  // namespace
 
 // partial slots of chiR: NB harmonic blocks of k_rel_nonres, then 2 nmaxR + 1
@@ -826,9 +844,9 @@ hipError_t launch_rel(hipStream_t st, const ProbDev *prob, int nspec, int nmaxR,
   const int nstride = 2 * nmaxR + 1;
   hipLaunchKernelGGL(k_rel_nonres, dim3(nom, nspec, NB), dim3(256), 0, st, prob, nom, om, active,
                      chiR, lowR, flags, pidx, nstride);
-  hipLaunchKernelGGL(k_rel_res<false>, dim3(nom, nspec, 2 * nmaxR + 1), dim3(64), 0, st, prob,
-                     nom, om, active, chiR, lowR, flags, NB, err, pidx, nstride);
-  hipLaunchKernelGGL(k_rel_res<true>, dim3(nom, nspec, 2 * nmaxR + 1), dim3(64), 0, st, prob,
-                     nom, om, active, chiR, lowR, flags, NB, err, pidx, nstride);
+  hipLaunchKernelGGL(k_rel_res, dim3(nom, nspec, 2 * nmaxR + 1), dim3(64), 0, st, prob, nom, om,
+                     active, chiR, lowR, flags, NB, err, pidx, nstride);
+  hipLaunchKernelGGL(k_rel_landau, dim3(nom, nspec, 2 * nmaxR + 1), dim3(64), 0, st, prob, nom,
+                     om, active, chiR, lowR, flags, NB, err, pidx, nstride);
   return hipGetLastError();
 }
