@@ -316,11 +316,15 @@ __device__ __forceinline__ cd rel_num(const RelCtx &C, int pt) {
 
 // funct_g_rel's interpolation cell (ALPS_fns_rel.f90:918-999): the index
 // close of row ig whose neighbours close-1, close+1 it interpolates at real x
-__device__ int rel_close(const RelCtx &C, int ig, double x) {
+// (*lo, *hi): the bounds pb[k] <= x < pb[k+1] of the cell the search found,
+// or (+inf, -inf) if none: any x' in [lo, hi) gives the same index
+__device__ int rel_close(const RelCtx &C, int ig, double x, double *lo = nullptr,
+                         double *hi = nullptr) {
   const SpecDev &S = *C.S;
   const int npb = C.npb, row = ig * (npb + 1);
   const double *pb = S.pbrel + row;
   int close = -2;
+  double cl = 1.0 / 0.0, ch = -1.0 / 0.0;
   {
 #pragma clang fp contract(off)
     // the last ip in [0, npb-1] with pb[ip] <= x < pb[ip+1] (increasing row)
@@ -329,8 +333,16 @@ __device__ int rel_close(const RelCtx &C, int ig, double x) {
       k = max(0, min(k, npb - 1));
       while (k > 0 && pb[k] > x) --k;
       while (k < npb - 1 && pb[k + 1] <= x) ++k;
-      if (pb[k + 1] > x && pb[k] <= x) close = k;
+      if (pb[k + 1] > x && pb[k] <= x) {
+        close = k;
+        cl = pb[k];
+        ch = pb[k + 1];
+      }
     }
+  }
+  if (lo) {
+    *lo = cl;
+    *hi = ch;
   }
   const double *f0 = S.f0rel + row;
   if (close + 1 >= 0 && close + 1 <= npb && f0[close + 1] <= -1.0) close = close - 1;
@@ -385,6 +397,10 @@ __device__ __forceinline__ void rel_point(const RelCtx &C, int ig, int ip, doubl
 struct RelCell {
   int close;  // -1: empty
   cd cm, c0, cp;
+  // the last cell search of this ladder: any x in [klo, khi) has index kc,
+  // whose pparbar is kp (klo > khi: none yet)
+  double klo, khi, kp;
+  int kc;
 };
 
 __device__ __forceinline__ void rel_node_g(const RelCtx &C, int pt, cd g[6]) {
@@ -411,15 +427,20 @@ __device__ void rel_cell_flush(const RelCtx &C, int ig, RelCell &cc, cd acc[6], 
 }
 
 // add wt g(x) through the cell cache (a cell leaving it goes into acc, times
-// fac)
+// fac).  The ladder's x rise (or fall) monotonically: while x stays in the
+// raw cell of the last search, the index is that search's.
 __device__ __forceinline__ void rel_cell_add(const RelCtx &C, int ig, RelCell &cc, double x, cd wt,
                                              cd acc[6], double fac) {
-  const int close = rel_close(C, ig, x);
+  if (!(cc.klo <= x && x < cc.khi)) {
+    cc.kc = rel_close(C, ig, x, &cc.klo, &cc.khi);
+    cc.kp = C.S->pbrel[ig * (C.npb + 1) + cc.kc];
+  }
+  const int close = cc.kc;
   if (close != cc.close) {
     rel_cell_flush(C, ig, cc, acc, fac);
     cc.close = close;
   }
-  const double f = 0.5 * ((x - C.S->pbrel[ig * (C.npb + 1) + close]) * C.idpb);
+  const double f = 0.5 * ((x - cc.kp) * C.idpb);
   cc.c0 = cc.c0 + wt;
   cc.cp = cc.cp + f * wt;
   cc.cm = cc.cm - f * wt;
@@ -450,6 +471,10 @@ __device__ void principal_rel6(const RelCtx &C, int ig, int ipres, int upperlimi
   RelCell up, dn;  // x = denomR + t (increasing) and its mirror 2 denomR - x (decreasing)
   up.close = dn.close = -1;
   up.cm = up.c0 = up.cp = dn.cm = dn.c0 = dn.cp = mk(0.0, 0.0);
+  up.klo = dn.klo = 1.0 / 0.0;
+  up.khi = dn.khi = -1.0 / 0.0;
+  up.kc = dn.kc = 0;
+  up.kp = dn.kp = 0.0;
   auto pv_pair = [&](double x, double wgt) {
     // wgt / (t - i denomI) and wgt / (t + i denomI), t = x - denomR: one
     // reciprocal, the second the conjugate of the first
