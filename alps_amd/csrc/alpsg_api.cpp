@@ -1479,6 +1479,8 @@ int rel_host_tables(const alpsg_plasma_desc *d, int s, int r, int nspec_rel, con
       const double a = pe[i], b = pa[j];
       o.gpp[(size_t)i * (npar + 1) + j] = std::sqrt((a * a + b * b) * (vA * vA) / (m * m) + 1.0);
     }
+  o.gmin = *std::min_element(o.gpp.begin(), o.gpp.end());
+  o.gmax = *std::max_element(o.gpp.begin(), o.gpp.end());
   return ALPSG_OK;
 }
 
@@ -1901,6 +1903,8 @@ int alpsg_create(const alpsg_plasma_desc *d, alpsg_handle *out) {
       S.df2 = h->df2.p + (size_t)s * (nperp + 1) * (npar + 1);
       S.rel_dg = dg;
       S.rel_dpb = dpb;
+      S.rel_gmin = R.gmin;
+      S.rel_gmax = R.gmax;
       S.int_ee = ee;
       for (int f = 0; f < S.nfits; ++f) S.fit_type[f] = 1;  // the Maxwell form of the table
       S.fit_uniform = 1;

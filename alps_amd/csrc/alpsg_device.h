@@ -82,6 +82,7 @@ struct SpecDev {
   double rel_pref;        // 2 pi (m/vA)^3 (q vA/(kpar m))
   double rel_zfac, rel_zbar;  // z = zfac * pperpbar; zbar of int_T_rel
   double rel_dg, rel_dpb;
+  double rel_gmin, rel_gmax;  // the smallest and largest Gamma of the (pperp, ppar) points
 };
 
 // One (request, species) job of the server's batched table launches: the

@@ -53,20 +53,6 @@ __device__ __forceinline__ cd tconst_r(const ProbDev &Pd, const SpecDev &S, int 
   }
 }
 
-// workgroup maximum of a double (blockDim.x = NT), result to all threads
-template <int NT>
-__device__ __forceinline__ double block_max(double v, double *lds) {
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-  if constexpr (NT == 64) return v;
-  const int w = threadIdx.x >> 6;
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) lds[w] = v;
-  __syncthreads();
-  double t = lds[0];
-  for (int i = 1; i < NT / 64; ++i) t = fmax(t, lds[i]);
-  return t;
-}
-
 // workgroup sum of a complex value (blockDim.x = 256 or 64), result to all
 template <int NT>
 __device__ __forceinline__ cd block_sum(cd v, double *lds) {
@@ -115,17 +101,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   const double scale = 2.0 * PI_R * S.dpperp * S.dppar_abs * 0.25;
   cd res[6];
   for (int k = 0; k < 6; ++k) res[k] = mk(0.0, 0.0);
-  // the Gamma range of the (pperp, ppar) points: p_res(Gamma) = (Gamma m Re w -
-  // n q)/kpar is linear in Gamma, so a harmonic whose p_res over [gmin, gmax]
-  // stays clear of [ppar(0), ppar(npar)] (with a margin far above rounding)
-  // has no point in a resonant cell and skips the scan below
-  double gmin = 1.0 / 0.0, gmax = -1.0 / 0.0;
-  for (int pt = threadIdx.x; pt < (nperp + 1) * (npar + 1); pt += 256) {
-    gmin = fmin(gmin, S.gam_pp[pt]);
-    gmax = fmax(gmax, S.gam_pp[pt]);
-  }
-  gmin = -block_max<256>(-gmin, lds);
-  gmax = block_max<256>(gmax, lds);
+  // the Gamma range of the (pperp, ppar) points (from the host's table):
+  // p_res(Gamma) = (Gamma m Re w - n q)/kpar is linear in Gamma, so a harmonic
+  // whose p_res over [gmin, gmax] stays clear of [ppar(0), ppar(npar)] (with a
+  // margin far above rounding) has no point in a resonant cell and skips the
+  // scan below
+  const double gmin = S.rel_gmin, gmax = S.rel_gmax;
   // whether harmonic n's p_res stays clear of the ppar grid over [gmin, gmax]
   auto clear_of = [&](int n) {
 #pragma clang fp contract(off)
