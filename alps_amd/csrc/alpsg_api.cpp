@@ -1481,6 +1481,10 @@ int rel_host_tables(const alpsg_plasma_desc *d, int s, int r, int nspec_rel, con
     }
   o.gmin = *std::min_element(o.gpp.begin(), o.gpp.end());
   o.gmax = *std::max_element(o.gpp.begin(), o.gpp.end());
+  o.gmono = true;
+  for (int i = 1; i <= nperp && o.gmono; ++i)
+    for (int j = 0; j <= npar; ++j)
+      if (o.gpp[(size_t)i * (npar + 1) + j] < o.gpp[(size_t)(i - 1) * (npar + 1) + j]) o.gmono = false;
   return ALPSG_OK;
 }
 
@@ -1905,6 +1909,7 @@ int alpsg_create(const alpsg_plasma_desc *d, alpsg_handle *out) {
       S.rel_dpb = dpb;
       S.rel_gmin = R.gmin;
       S.rel_gmax = R.gmax;
+      S.rel_gmono = R.gmono ? 1 : 0;
       S.int_ee = ee;
       for (int f = 0; f < S.nfits; ++f) S.fit_type[f] = 1;  // the Maxwell form of the table
       S.fit_uniform = 1;

@@ -83,6 +83,7 @@ struct SpecDev {
   double rel_zfac, rel_zbar;  // z = zfac * pperpbar; zbar of int_T_rel
   double rel_dg, rel_dpb;
   double rel_gmin, rel_gmax;  // the smallest and largest Gamma of the (pperp, ppar) points
+  int rel_gmono;              // that Gamma non-decreasing in iperp along every ppar column
 };
 
 // One (request, species) job of the server's batched table launches: the

@@ -124,7 +124,41 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     const bool has1 = n1 <= nmax;
     const bool scan0 = !clear_of(n0), scan1 = has1 && !clear_of(n1);
     int hit0 = 0, hit1 = 0;
-    if (scan0 || scan1) {
+    if ((scan0 || scan1) && S.rel_gmono) {
+      // Along a ppar column, Gamma does not decrease with iperp, and the
+      // reference's pr = (Gamma m Re w - n q)/kpar (each operation rounded,
+      // hence monotone in Gamma) is monotone along it: the column has a point
+      // with ppar(ia) <= pr < ppar(ia+1) iff the first point past the cell's
+      // near end (found by bisection on that same pr) is short of its far end
+      for (int ia = threadIdx.x; ia < npar; ia += 256) {
+        const double lo = S.ppar[ia], hi = S.ppar[ia + 1];
+        const double *gc = S.gam_pp + ia;
+        for (int h = 0; h < 2; ++h) {
+          if (h == 0 ? !scan0 : !scan1) continue;
+          const double nq = 1.0 * (h == 0 ? n0 : n1) * q;
+          auto pr = [&](int ip) {
+#pragma clang fp contract(off)
+            return (gc[(size_t)ip * (npar + 1)] * m * om.re - nq) / kpar;
+          };
+          const bool up = pr(0) <= pr(nperp);
+          // the first ip whose pr is past the near end: pr >= lo (rising) or
+          // pr < hi (falling); nperp + 1 if none
+          int a0 = 0, a1 = nperp + 1;
+          while (a0 < a1) {
+            const int mid = (a0 + a1) >> 1;
+            const double v = pr(mid);
+            if (up ? (v >= lo) : (v < hi))
+              a1 = mid;
+            else
+              a0 = mid + 1;
+          }
+          if (a0 <= nperp) {
+            const double v = pr(a0);
+            if (lo <= v && hi > v) (h == 0 ? hit0 : hit1) = 1;
+          }
+        }
+      }
+    } else if (scan0 || scan1) {
 #pragma clang fp contract(off)
       const double nq0 = 1.0 * n0 * q, nq1 = 1.0 * n1 * q;
       const double ik = 1.0 / kpar;

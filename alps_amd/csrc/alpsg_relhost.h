@@ -16,6 +16,7 @@ struct RelHost {
   std::vector<int2> lim;    // first and last index of each row inside the subluminal cone
   std::vector<double> gpp;  // Gamma at the (pperp, ppar) points, [iperp*(npar+1)+ipar]
   double gmin = 0.0, gmax = 0.0;  // the smallest and largest of them
+  bool gmono = false;             // non-decreasing in iperp along every ppar column
   double dg = 0.0, dpb = 0.0;
   double ee = 0.0;  // int_ee_rel (ALPS_fns_rel.f90:1097-1215), omega-independent
 };

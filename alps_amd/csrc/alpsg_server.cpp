@@ -201,6 +201,7 @@ struct Plasma {
   std::vector<int> isrel;
   int ngamma = 0, npparbar = 0;
   std::vector<double> rel_dg, rel_dpb, rel_ee, rel_gmin, rel_gmax;
+  std::vector<int> rel_gmono;
   std::vector<size_t> o_grel, o_pbrel, o_f0rel, o_dfrel, o_rowlim, o_gampp;
   bool any_bm = false, any_rel = false;
   // device block: the uploaded prefix (host image img), then df1, df2
@@ -373,6 +374,7 @@ int prepare_plasma(const alpsg_plasma_desc *d, Plasma &r) {
   r.isrel.assign(nspec, 0);
   r.rel_dg.assign(nspec, 0.0);
   r.rel_gmin.assign(nspec, 0.0);
+  r.rel_gmono.assign(nspec, 0);
   r.rel_gmax.assign(nspec, 0.0);
   r.rel_dpb.assign(nspec, 0.0);
   r.rel_ee.assign(nspec, 0.0);
@@ -556,6 +558,7 @@ int prepare_plasma(const alpsg_plasma_desc *d, Plasma &r) {
     r.o_gampp[s] = L.add(sizeof(double) * rh[s].gpp.size());
     r.rel_dg[s] = rh[s].dg;
     r.rel_gmin[s] = rh[s].gmin;
+    r.rel_gmono[s] = rh[s].gmono ? 1 : 0;
     r.rel_gmax[s] = rh[s].gmax;
     r.rel_dpb[s] = rh[s].dpb;
     r.rel_ee[s] = rh[s].ee;
@@ -895,6 +898,7 @@ int admit_k(alpsg_server *S, Request &r, int &njobs, int &max_nperp, int &nnew) 
       sd.df2 = PS(P.o_df2) + (size_t)s * (nperp + 1) * (npar + 1);
       sd.rel_dg = P.rel_dg[s];
       sd.rel_gmin = P.rel_gmin[s];
+      sd.rel_gmono = P.rel_gmono[s];
       sd.rel_gmax = P.rel_gmax[s];
       sd.rel_dpb = P.rel_dpb[s];
       sd.fit_uniform = 1;
