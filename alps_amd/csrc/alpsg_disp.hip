@@ -1601,7 +1601,8 @@ __device__ __forceinline__ void assemble_block(
     const double2 *__restrict__ lowA, const double2 *__restrict__ lowB,
     const double2 *__restrict__ chiR, int RR, const double2 *__restrict__ lowR,
     double2 *__restrict__ det_out, double2 *__restrict__ chi_out, double2 *__restrict__ low_out,
-    double2 *__restrict__ wave_out, int nspec, const int *__restrict__ pidx) {
+    double2 *__restrict__ wave_out, int nspec, const int *__restrict__ pidx,
+    double2 *det_lds = nullptr) {
   __shared__ double2 lchi[256];
   const int per = 6 * nspec;
   const int opb = 256 / per;  // omegas per block
@@ -1726,6 +1727,7 @@ __device__ __forceinline__ void assemble_block(
   const cd det = w11 * (w22 * w33 + w23 * w23) + 2.0 * w12 * w23 * w13 - (w13 * w13) * w22 +
                  (w12 * w12) * w33;
   det_out[iom] = make_double2(det.re, det.im);
+  if (det_lds != nullptr) det_lds[iom] = make_double2(det.re, det.im);
   if (wave_out != nullptr) {
     double2 *o = wave_out + (size_t)iom * 9;
     const cd Wt[3][3] = {{w11, w12, w13},
@@ -1985,23 +1987,47 @@ __global__ __launch_bounds__(1024) void k_osc_round(int n, alpsg_osc::OscMachine
 }
 
 // k_assemble for a small batch in one workgroup (its omegas in chunks of
-// 256 / (6 nspec)), followed by the secant step of every solve.
+// 256 / (6 nspec)), followed by the secant step of every solve (nom <= 256).
+// Each thread's solve state is loaded first, beside the count of unfinished
+// solves, and det D reaches the step through LDS, so that the round waits on
+// global memory for the partial sums only.
 __global__ __launch_bounds__(256) void k_assemble_osc(
     const ProbDev *__restrict__ probp, int nom, int W, int RB, const double2 *__restrict__ om_in,
     const double2 *__restrict__ chiA, const double2 *__restrict__ chiB, double2 *__restrict__ det,
     int nspec, alpsg_osc::OscMachine *__restrict__ S, double2 *__restrict__ om_next,
     int *__restrict__ round_ctr, int *__restrict__ remaining) {
+  __shared__ double2 det_l[256];
+  const int t = threadIdx.x;
+  alpsg_osc::OscMachine m;
+  if (t < nom) m = S[t];
   // a round after every solve has finished does nothing (its values would be
   // ignored); the count is read before any thread can change it
   const int left = *remaining;
+  const int round = *round_ctr;
   if (left == 0) return;
   const int opb = 256 / (6 * nspec);
   for (int blk = 0; blk * opb < nom; ++blk) {
     assemble_block(blk, probp, nom, W, RB, om_in, nullptr, chiA, chiB, nullptr, nullptr, nullptr, 0,
-                   nullptr, det, nullptr, nullptr, nullptr, nspec, nullptr);
+                   nullptr, det, nullptr, nullptr, nullptr, nspec, nullptr, det_l);
     __syncthreads();
   }
-  osc_step(nom, S, det, om_next, round_ctr, remaining);
+  int more = 0;
+  if (t < nom && !m.done) {
+    m.feed(alpsg_osc::zmk(det_l[t].x, det_l[t].y));
+    if (m.done) {
+      m.done_round = round;
+    } else {
+      const alpsg_osc::zc w = m.want();
+      om_next[t] = make_double2(w.re, w.im);
+      more = 1;
+    }
+    S[t] = m;
+  }
+  const int tot = __syncthreads_count(more);
+  if (t == 0) {
+    *remaining = tot;
+    *round_ctr = round + 1;
+  }
 }
 
 // The request sweep and both resonant parts of a small batch in one launch:
