@@ -351,6 +351,7 @@ __device__ __forceinline__ void sweep_nodes_pf(cdptr srows, cdptr pa, int npar, 
 
 constexpr double SPLIT_MAX = 256.0;
 constexpr int LPA = 1025;  // ppar points a sweep workgroup keeps in LDS (npar <= 1024)
+constexpr int LPA_REQ = 2056;  // the request sweep's: every species' grid at once up to this
 
 // The interior nodes of a full-range row pair in the split form (one lane per
 // omega, records read as wave-uniform scalars): with r = 1/den,
@@ -818,16 +819,36 @@ __device__ __forceinline__ void sweep_req_body(const ProbDev *__restrict__ probp
   const double omr = ov.x, omi = ov.y;
   const int nspec = Pd.nspec;
   const double kpar = Pd.kpar;
-  __shared__ double lpa[LPA];
-  __shared__ double red[W][12];
-  int rowbase = 0;
+  __shared__ double lpa[LPA_REQ];
+  __shared__ double red[ALPSG_DMAXSPEC][W][12];
+  // the species' ppar grids (for the resonance classification) go to LDS all
+  // at once when they fit, and the wavefronts' partial sums are combined once
+  // for every species at the end: two barriers per launch instead of four per
+  // species (the same sums in the same order)
+  int tot = 0;
+  for (int s = 0; s < nspec; ++s)
+    if (Pd.sp[s].nmax >= 0) tot += Pd.sp[s].npar + 1;
+  const bool all = tot <= LPA_REQ;
+  if (all) {
+    int o = 0;
+    for (int s = 0; s < nspec; ++s) {
+      const SpecDev &S = Pd.sp[s];
+      if (S.nmax < 0) continue;
+      for (int t = threadIdx.x; t <= S.npar; t += blockDim.x) lpa[o + t] = S.ppar[t];
+      o += S.npar + 1;
+    }
+    __syncthreads();
+  }
+  int rowbase = 0, lo = 0;
   for (int s = 0; s < nspec; ++s) {
     const SpecDev &S = Pd.sp[s];
     const int npar = S.npar;
     const double m = S.m;
-    // the species' ppar grid in LDS for the resonance classification
     const double *pa = S.ppar;
-    if (S.nmax >= 0 && npar + 1 <= LPA) {
+    if (S.nmax >= 0 && all) {
+      pa = lpa + lo;
+      lo += npar + 1;
+    } else if (S.nmax >= 0 && npar + 1 <= LPA_REQ) {
       __syncthreads();
       for (int t = threadIdx.x; t <= npar; t += blockDim.x) lpa[t] = S.ppar[t];
       __syncthreads();
@@ -917,23 +938,43 @@ __device__ __forceinline__ void sweep_req_body(const ProbDev *__restrict__ probp
 #pragma unroll
       for (int t = 0; t < 12; ++t) run[t] += __shfl_xor(run[t], o, 64);
     if constexpr (RED) {
-      __syncthreads();
-      if (lane == 0)
+      if (all) {
+        if (lane == 0)
 #pragma unroll
-        for (int t = 0; t < 12; ++t) red[threadIdx.x >> 6][t] = run[t];
-      __syncthreads();
-      if (threadIdx.x < 12) {
-        double v = red[0][threadIdx.x];
+          for (int t = 0; t < 12; ++t) red[s][threadIdx.x >> 6][t] = run[t];
+      } else {
+        __syncthreads();
+        if (lane == 0)
 #pragma unroll
-        for (int w = 1; w < W; ++w) v += red[w][threadIdx.x];
-        double *o = reinterpret_cast<double *>(
-            chiA + ((size_t)(by * nspec + s) * 6 + (threadIdx.x >> 1)) * nom + iom);
-        o[threadIdx.x & 1] = v;
+          for (int t = 0; t < 12; ++t) red[0][threadIdx.x >> 6][t] = run[t];
+        __syncthreads();
+        if (threadIdx.x < 12) {
+          double v = red[0][0][threadIdx.x];
+#pragma unroll
+          for (int w = 1; w < W; ++w) v += red[0][w][threadIdx.x];
+          double *o = reinterpret_cast<double *>(
+              chiA + ((size_t)(by * nspec + s) * 6 + (threadIdx.x >> 1)) * nom + iom);
+          o[threadIdx.x & 1] = v;
+        }
       }
     } else if (lane == 0) {
       double2 *o = chiA + ((size_t)(wv * nspec + s) * 6) * nom + iom;
 #pragma unroll
       for (int k = 0; k < 6; ++k) o[(size_t)k * nom] = make_double2(run[2 * k], run[2 * k + 1]);
+    }
+  }
+  if constexpr (RED) {
+    if (all) {
+      // every species' partials of the wavefronts, summed in wavefront order
+      __syncthreads();
+      for (int t = threadIdx.x; t < 12 * nspec; t += blockDim.x) {
+        const int s = t / 12, c = t % 12;
+        double v = red[s][0][c];
+#pragma unroll
+        for (int w = 1; w < W; ++w) v += red[s][w][c];
+        double *o = reinterpret_cast<double *>(chiA + ((size_t)(by * nspec + s) * 6 + (c >> 1)) * nom + iom);
+        o[c & 1] = v;
+      }
     }
   }
 }
