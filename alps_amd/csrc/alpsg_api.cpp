@@ -43,7 +43,7 @@ hipError_t launch_species_k(hipStream_t st, int nperp, int npar, int nmax, doubl
                             const int *mtab, double *bes, double *atab, double *nodes);
 hipError_t launch_sweep(hipStream_t st, int nch, int W, int RC, const ProbDev *prob, int nom,
                         const double2 *om, const int *active, double2 *chiA, double2 *lowA,
-                        const int *pidx = nullptr, unsigned long long *work = nullptr);
+                        int npar1, const int *pidx = nullptr, unsigned long long *work = nullptr);
 hipError_t launch_sweep_req(hipStream_t st, int nch, int W, int RC, int K, const ProbDev *prob,
                             int nom, const double2 *om, double2 *chiA, const int *pidx,
                             bool reduce);
@@ -414,7 +414,7 @@ int disp_chain(alpsg_ctx *h, int n, const double2 *om_dev, const int *active_dev
                             B.chiA, B.zidx, true));
   } else {
     HIPCHK(launch_sweep(h->st, sh.nch_a, sh.w_a, sh.rc_a, h->prob_d.p, n, om_dev, active_dev,
-                        B.chiA, want_low ? B.lowA : nullptr));
+                        B.chiA, want_low ? B.lowA : nullptr, h->npar + 1));
   }
   const int TWa = req ? REQ_TW / REQ_W : TW;  // the request sweep writes one partial per workgroup
   HIPCHK(hipEventRecord(h->ev_sweep, h->st));
@@ -607,7 +607,7 @@ int autotune(alpsg_ctx *h, int n, const double2 *om_host, int *shape_out) {
       int rc_ = time_launch(
           [&]() {
             return launch_sweep(h->st, v[0], v[1], rc, h->prob_d.p, n, h->om_d.p, nullptr,
-                                h->chiA_d.p, nullptr);
+                                h->chiA_d.p, nullptr, h->npar + 1);
           },
           t);
       if (rc_) return rc_;
@@ -897,7 +897,7 @@ int osc_graph_batch(alpsg_ctx *h, const alpsg_root_params *rp, int n, cplx *om, 
                              h->chiA_d.p, h->zidx.p, true);
       else
         e = launch_sweep(h->st, sh.nch_a, sh.w_a, sh.rc_a, h->prob_d.p, n, h->om_d.p, nullptr,
-                         h->chiA_d.p, nullptr);
+                         h->chiA_d.p, nullptr, h->npar + 1);
       if (n <= 256) {
         // two launches per round: both resonant parts, then the assembly and
         // the secant step in one workgroup
@@ -2251,7 +2251,7 @@ int alpsg_sweep_work(alpsg_handle h, int n, const double *om, double *work3) {
   HIPCHK(hipMemcpyAsync(h->om_d.p, om, sizeof(double2) * n, hipMemcpyHostToDevice, h->st));
   HIPCHK(hipMemsetAsync(h->work_d.p, 0, sizeof(unsigned long long) * 3, h->st));
   HIPCHK(launch_sweep(h->st, sh.nch_a, sh.w_a, sh.rc_a, h->prob_d.p, n, h->om_d.p, nullptr,
-                      h->chiA_d.p, nullptr, nullptr, h->work_d.p));
+                      h->chiA_d.p, nullptr, h->npar + 1, nullptr, h->work_d.p));
   unsigned long long w[3] = {0, 0, 0};
   HIPCHK(hipMemcpyAsync(w, h->work_d.p, sizeof(w), hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
