@@ -49,7 +49,7 @@ hipError_t launch_sweep_req(hipStream_t st, int nch, int W, int RC, int K, const
                             bool reduce);
 hipError_t launch_resonant(hipStream_t st, int nch, int RB, const ProbDev *prob, int nspec, int nom,
                            const double2 *om, const int *active, double2 *chiB, double2 *lowB,
-                           int lgen, const int *pidx = nullptr);
+                           int lgen, const int *pidx = nullptr, int P = 0);
 hipError_t launch_nhds(hipStream_t st, const ProbDev *prob, int nspec, int nom, const double2 *om,
                        const int *active, double2 *chiA, double2 *lowA, const int *pidx = nullptr);
 int nhds_tables(double z, int bm_nmax, double bessel_zero, std::vector<double2> &tab);
@@ -423,7 +423,7 @@ int disp_chain(alpsg_ctx *h, int n, const double2 *om_dev, const int *active_dev
                                h->lgen));
   else
     HIPCHK(launch_resonant(h->st, sh.nch_b, sh.rb, h->prob_d.p, ns, n, om_dev, active_dev, B.chiB,
-                           want_low ? B.lowB : nullptr, h->lgen));
+                           want_low ? B.lowB : nullptr, h->lgen, nullptr, h->P));
   if (h->any_bm)
     HIPCHK(launch_nhds(h->st, h->prob_d.p, ns, n, om_dev, active_dev, B.chiA,
                        want_low ? B.lowA : nullptr));
@@ -623,7 +623,7 @@ int autotune(alpsg_ctx *h, int n, const double2 *om_host, int *shape_out) {
       int rc_ = time_launch(
           [&]() {
             return launch_resonant(h->st, nb, rb, h->prob_d.p, ns, n, h->om_d.p, nullptr,
-                                   h->chiB_d.p, nullptr, h->lgen);
+                                   h->chiB_d.p, nullptr, h->lgen, nullptr, h->P);
           },
           t);
       if (rc_) return rc_;
@@ -912,7 +912,7 @@ int osc_graph_batch(alpsg_ctx *h, const alpsg_root_params *rp, int n, cplx *om, 
       }
       if (e == hipSuccess)
         e = launch_resonant(h->st, sh.nch_b, sh.rb, h->prob_d.p, ns, n, h->om_d.p, nullptr,
-                            h->chiB_d.p, nullptr, h->lgen);
+                            h->chiB_d.p, nullptr, h->lgen, nullptr, h->P);
       if (e == hipSuccess)
         e = launch_assemble(h->st, h->prob_d.p, ns, n, TW, sh.rb, h->om_d.p, nullptr, h->chiA_d.p,
                             h->chiB_d.p, nullptr, nullptr, nullptr, 0, nullptr, h->det_d.p, nullptr,

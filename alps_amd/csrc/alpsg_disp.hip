@@ -1468,7 +1468,7 @@ __device__ void landau_part(const SpecDev &S, const ProbDev &Pd, int an, const R
 // other species.  PV writes partial slots rb, the Landau launches slots RB + rb
 // of chiB / lowB (each species by exactly one of PART 1, 2, 3).
 // One wavefront's share of k_resonant: omega block bx, species by, harmonic
-// block rb of RB; win_lds holds (64 / NCH) windows of 2 PV_WIN doubles.
+// block rb of RB; win_lds holds (64 / NCH) windows of 2 pvw doubles.
 template <int NCH, int PART>
 __device__ __forceinline__ void resonant_body(const ProbDev *__restrict__ probp, int nom,
                                               const double2 *__restrict__ om_in,
@@ -1476,7 +1476,8 @@ __device__ __forceinline__ void resonant_body(const ProbDev *__restrict__ probp,
                                               double2 *__restrict__ chiB,
                                               double2 *__restrict__ lowB,
                                               const int *__restrict__ pidx, int RB, int rb, int bx,
-                                              int by, int lane, double *win_lds) {
+                                              int by, int lane, double *win_lds,
+                                              int pvw = PV_WIN) {
   constexpr int OMB = 64 / NCH;
   const ProbDev &Pd = (NCH == 64 && pidx != nullptr) ? probp[pidx[bx]] : *probp;
   const int g = lane / NCH, c = lane % NCH;
@@ -1493,7 +1494,7 @@ __device__ __forceinline__ void resonant_body(const ProbDev *__restrict__ probp,
   const bool uni_s = (S.fit_uniform != 0) && (S.nfits == 1) && (S.acmethod == 1);
   const bool kap_s = S.fit_kappa != 0;
   if ((PART == 1 && !uni_s) || (PART == 2 && (uni_s || kap_s)) || (PART == 3 && !kap_s)) return;
-  double *win = win_lds + g * (2 * PV_WIN);
+  double *win = win_lds + g * (2 * pvw);  // pvw >= 2P + 8, the window's extent
   cd res[6];
 #pragma unroll
   for (int k = 0; k < 6; ++k) res[k] = mk(0.0, 0.0);
@@ -1527,7 +1528,7 @@ __device__ __forceinline__ void resonant_body(const ProbDev *__restrict__ probp,
     if constexpr (PART == 0) {
       const int wlo = max(1, rc.ipar_res - P - 3);
       const int whi = min(npar - 1, rc.ipar_res + P + 4);
-      for (int t = c; t < 2 * PV_WIN; t += NCH) win[t] = 0.0;
+      for (int t = c; t < 2 * pvw; t += NCH) win[t] = 0.0;
       wave_sync();
       pv_coefs<NCH>(S, Pd, rc, c, win, wlo);
       wave_sync();
@@ -1587,16 +1588,19 @@ __device__ __forceinline__ void resonant_body(const ProbDev *__restrict__ probp,
   }
 }
 
+// pvw: principal-value window entries per omega (2P + 8, at most PV_WIN); the
+// windows of part 0 are (64 / NCH) 2 pvw doubles of dynamic LDS, so that the
+// usual P = 3 does not cost the occupancy of the largest P
 template <int NCH, int PART>
 __global__ __launch_bounds__(64) void k_resonant(const ProbDev *__restrict__ probp, int nom,
                                                  const double2 *__restrict__ om_in,
                                                  const int *__restrict__ active,
                                                  double2 *__restrict__ chiB,
                                                  double2 *__restrict__ lowB,
-                                                 const int *__restrict__ pidx) {
-  __shared__ double win[(64 / NCH) * 2 * PV_WIN];
+                                                 const int *__restrict__ pidx, int pvw) {
+  extern __shared__ double win_dyn[];
   resonant_body<NCH, PART>(probp, nom, om_in, active, chiB, lowB, pidx, gridDim.z, blockIdx.z,
-                           blockIdx.x, blockIdx.y, threadIdx.x, win);
+                           blockIdx.x, blockIdx.y, threadIdx.x, win_dyn, pvw);
 }
 
 // The Landau part of the kappa species (part 3): the lean power leaves the
@@ -1854,22 +1858,25 @@ hipError_t launch_sweep_req(hipStream_t st, int nch, int W, int RC, int K, const
 }
 
 // lgen: bit 0 set if a species takes the part-2 Landau form, bit 1 if one
-// takes the part-3 (kappa) form
+// takes the part-3 (kappa) form; P: positions_principal of every problem of
+// the launch (<= 0: unknown, the largest window)
 hipError_t launch_resonant(hipStream_t st, int nch, int RB, const ProbDev *prob, int nspec, int nom,
                            const double2 *om, const int *active, double2 *chiB, double2 *lowB,
-                           int lgen, const int *pidx) {
+                           int lgen, const int *pidx, int P) {
   if (nom <= 0) return hipSuccess;
   if (pidx != nullptr && nch != 64) return hipErrorInvalidValue;
+  const int pvw = (P >= 1 && 2 * P + 8 <= PV_WIN) ? 2 * P + 8 : PV_WIN;
 #define RS(N)                                                                                  \
   if (nch == N) {                                                                              \
     const int grid = (nom + (64 / N) - 1) / (64 / N);                                          \
-    hipLaunchKernelGGL((k_resonant<N, 0>), dim3(grid, nspec, RB), dim3(64), 0, st, prob, nom,  \
-                       om, active, chiB, lowB, pidx);                                          \
+    hipLaunchKernelGGL((k_resonant<N, 0>), dim3(grid, nspec, RB), dim3(64),                    \
+                       (64 / N) * 2 * pvw * sizeof(double), st, prob, nom, om, active, chiB,   \
+                       lowB, pidx, pvw);                                                       \
     hipLaunchKernelGGL((k_resonant<N, 1>), dim3(grid, nspec, RB), dim3(64), 0, st, prob, nom,  \
-                       om, active, chiB, lowB, pidx);                                          \
+                       om, active, chiB, lowB, pidx, pvw);                                     \
     if (lgen & 1)                                                                              \
       hipLaunchKernelGGL((k_resonant<N, 2>), dim3(grid, nspec, RB), dim3(64), 0, st, prob, nom, \
-                         om, active, chiB, lowB, pidx);                                        \
+                         om, active, chiB, lowB, pidx, pvw);                                   \
     if (lgen & 2)                                                                              \
       hipLaunchKernelGGL((k_resonant_kappa<N>), dim3(grid, nspec, RB), dim3(64), 0, st, prob,   \
                          nom, om, active, chiB, lowB, pidx);                                   \

@@ -75,7 +75,7 @@ hipError_t launch_sweep_req(hipStream_t st, int nch, int W, int RC, int K, const
                             bool reduce);
 hipError_t launch_resonant(hipStream_t st, int nch, int RB, const ProbDev *prob, int nspec, int nom,
                            const double2 *om, const int *active, double2 *chiB, double2 *lowB,
-                           int lgen, const int *pidx);
+                           int lgen, const int *pidx, int P);
 hipError_t launch_assemble(hipStream_t st, const ProbDev *prob, int nspec, int nom, int W, int RB,
                            const double2 *om, const int *active, const double2 *chiA,
                            const double2 *chiB, const double2 *lowA, const double2 *lowB,
@@ -1101,7 +1101,7 @@ int server_pass(alpsg_server *S) {
                             S->chiA, S->pidx_d + i0, false));
       if (ev) SCHK(hipEventRecord(ev[1], S->st));
       SCHK(launch_resonant(S->st, 64, RS_RB, S->probs_d, ns, n, S->om_d + i0, nullptr, S->chiB,
-                           nullptr, gen, S->pidx_d + i0));
+                           nullptr, gen, S->pidx_d + i0, 0));
       // closed-form species: partial slot 0 of their chi (the sweep left it zero)
       if (bm)
         SCHK(launch_nhds(S->st, S->probs_d, ns, n, S->om_d + i0, nullptr, S->chiA, nullptr,
