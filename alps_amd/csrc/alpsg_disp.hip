@@ -1590,9 +1590,13 @@ __device__ __forceinline__ void resonant_body(const ProbDev *__restrict__ probp,
 
 // pvw: principal-value window entries per omega (2P + 8, at most PV_WIN); the
 // windows of part 0 are (64 / NCH) 2 pvw doubles of dynamic LDS, so that the
-// usual P = 3 does not cost the occupancy of the largest P
+// usual P = 3 does not cost the occupancy of the largest P.  Held to four
+// wavefronts per SIMD (128 registers, a few spilled): the map's PV and Landau
+// passes run about 10 % faster than at three.  Part 2 (the Landau pass on
+// eval_fit calls) is k_resonant_gen, with no such bound: its call needs the
+// full register file, and a bound only spills it
 template <int NCH, int PART>
-__global__ __launch_bounds__(64) void k_resonant(const ProbDev *__restrict__ probp, int nom,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_resonant(const ProbDev *__restrict__ probp, int nom,
                                                  const double2 *__restrict__ om_in,
                                                  const int *__restrict__ active,
                                                  double2 *__restrict__ chiB,
@@ -1601,6 +1605,18 @@ __global__ __launch_bounds__(64) void k_resonant(const ProbDev *__restrict__ pro
   extern __shared__ double win_dyn[];
   resonant_body<NCH, PART>(probp, nom, om_in, active, chiB, lowB, pidx, gridDim.z, blockIdx.z,
                            blockIdx.x, blockIdx.y, threadIdx.x, win_dyn, pvw);
+}
+
+template <int NCH>
+__global__ __launch_bounds__(64) void k_resonant_gen(const ProbDev *__restrict__ probp, int nom,
+                                                     const double2 *__restrict__ om_in,
+                                                     const int *__restrict__ active,
+                                                     double2 *__restrict__ chiB,
+                                                     double2 *__restrict__ lowB,
+                                                     const int *__restrict__ pidx, int pvw) {
+  extern __shared__ double win_dyn[];
+  resonant_body<NCH, 2>(probp, nom, om_in, active, chiB, lowB, pidx, gridDim.z, blockIdx.z,
+                        blockIdx.x, blockIdx.y, threadIdx.x, win_dyn, pvw);
 }
 
 // The Landau part of the kappa species (part 3): the lean power leaves the
@@ -1875,7 +1891,7 @@ hipError_t launch_resonant(hipStream_t st, int nch, int RB, const ProbDev *prob,
     hipLaunchKernelGGL((k_resonant<N, 1>), dim3(grid, nspec, RB), dim3(64), 0, st, prob, nom,  \
                        om, active, chiB, lowB, pidx, pvw);                                     \
     if (lgen & 1)                                                                              \
-      hipLaunchKernelGGL((k_resonant<N, 2>), dim3(grid, nspec, RB), dim3(64), 0, st, prob, nom, \
+      hipLaunchKernelGGL((k_resonant_gen<N>), dim3(grid, nspec, RB), dim3(64), 0, st, prob, nom, \
                          om, active, chiB, lowB, pidx, pvw);                                   \
     if (lgen & 2)                                                                              \
       hipLaunchKernelGGL((k_resonant_kappa<N>), dim3(grid, nspec, RB), dim3(64), 0, st, prob,   \
